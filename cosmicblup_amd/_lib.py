@@ -1,0 +1,168 @@
+"""ctypes binding of libcosmicblup_amd.so (the C-ABI in include/cosmicblup_amd.h).
+
+The library is loaded from this package directory only; a missing library
+is an error, never a silent fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libcosmicblup_amd.so"
+
+CB_OK = 0
+CB_ERR_INVALID = -1
+CB_ERR_OOM = -2
+CB_ERR_DEVICE = -3
+CB_ERR_NOT_CONVERGED = -4
+CB_ERR_INTERNAL = -5
+
+CB_GRM_OUT_F64 = 1
+CB_GRM_OUT_F32 = 2
+CB_GRM_OUT_INT = 4
+
+c_i64 = C.c_int64
+c_vp = C.c_void_p
+c_i64p = C.POINTER(C.c_int64)
+c_dp = C.POINTER(C.c_double)
+
+
+class GrmOptions(C.Structure):
+    _fields_ = [("block_size", C.c_int64), ("min_maf", C.c_float), ("min_hwe", C.c_float),
+                ("algorithm", C.c_int)]
+
+
+class CosmicError(RuntimeError):
+    """A non-zero status from the C-ABI (message from cb_last_error)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class NotConverged(CosmicError):
+    pass
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "cb_context_create": (C.c_int, [C.c_int, C.POINTER(c_vp)]),
+    "cb_context_destroy": (None, [c_vp]),
+    "cb_last_error": (C.c_char_p, []),
+    "cb_context_set_stream": (C.c_int, [c_vp, c_vp]),
+    "cb_context_stream": (c_vp, [c_vp]),
+    "cb_context_synchronize": (C.c_int, [c_vp]),
+    "cb_context_set_profiling": (C.c_int, [c_vp, C.c_int]),
+    "cb_context_reset_profile": (C.c_int, [c_vp]),
+    "cb_context_kernel_time": (C.c_int, [c_vp, C.c_char_p, c_dp, c_i64p]),
+    "cb_genotypes_from_bed": (C.c_int, [c_vp, c_vp, C.c_size_t, c_i64, c_i64, C.POINTER(c_vp)]),
+    "cb_genotypes_from_device_rows": (C.c_int, [c_vp, c_vp, C.c_size_t, c_i64, c_i64, C.POINTER(c_vp)]),
+    "cb_genotypes_destroy": (None, [c_vp]),
+    "cb_genotypes_dims": (C.c_int, [c_vp, c_i64p, c_i64p]),
+    "cb_grm_options_default": (None, [C.POINTER(GrmOptions)]),
+    "cb_grm_create": (C.c_int, [c_vp, C.POINTER(GrmOptions), C.POINTER(c_vp)]),
+    "cb_grm_destroy": (None, [c_vp]),
+    "cb_grm_compute": (C.c_int, [c_vp, c_vp, C.c_int]),
+    "cb_grm_copy_grm": (C.c_int, [c_vp, c_vp, c_i64]),
+    "cb_grm_copy_grm_f32": (C.c_int, [c_vp, c_vp, c_i64]),
+    "cb_grm_copy_int_gram": (C.c_int, [c_vp, c_vp, c_i64]),
+    "cb_grm_device_ptr": (C.c_int, [c_vp, C.c_int, C.POINTER(c_vp), c_i64p]),
+    "cb_grm_sum2pq": (C.c_int, [c_vp, c_dp]),
+    "cb_grm_total_snps": (C.c_int, [c_vp, c_i64p]),
+    "cb_grm_num_samples": (C.c_int, [c_vp, c_i64p]),
+    "cb_grm_copy_allele_freqs": (C.c_int, [c_vp, c_vp, c_i64]),
+    "cb_grm_copy_snp_stats": (C.c_int, [c_vp, c_i64] + [c_vp] * 8),
+}
+
+_lib = None
+
+
+def exported_names() -> list[str]:
+    return list(_SIGS)
+
+
+def lib() -> C.CDLL:
+    """Load the in-tree library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "or `python cosmicblup_amd/build.py` (there is no CPU fallback)")
+    L = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status != CB_OK:
+        msg = lib().cb_last_error().decode(errors="replace")
+        if status == CB_ERR_NOT_CONVERGED:
+            raise NotConverged(status, msg)
+        raise CosmicError(status, msg)
+
+
+def ptr(a) -> int:
+    """Address of a numpy array's data (C-contiguous required)."""
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data
+
+
+class Context:
+    """Owns a cb_context (device + HIP stream + kernel timing)."""
+
+    def __init__(self, device: int = 0):
+        h = c_vp()
+        check(lib().cb_context_create(device, C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().cb_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, hip_stream: int | None) -> None:
+        check(lib().cb_context_set_stream(self.handle, hip_stream))
+
+    @property
+    def stream(self) -> int:
+        return lib().cb_context_stream(self.handle)
+
+    def synchronize(self) -> None:
+        check(lib().cb_context_synchronize(self.handle))
+
+    def set_profiling(self, on: bool) -> None:
+        check(lib().cb_context_set_profiling(self.handle, int(on)))
+
+    def reset_profile(self) -> None:
+        check(lib().cb_context_reset_profile(self.handle))
+
+    def kernel_time(self, name: str) -> tuple[float, int]:
+        ms = C.c_double()
+        cnt = C.c_int64()
+        check(lib().cb_context_kernel_time(self.handle, name.encode(), C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
+
+
+_default_ctx: Context | None = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx

@@ -1,0 +1,813 @@
+// VanRaden-1 GRM on gfx950 — the MI355X counterpart of
+// GrmBuilder::compute(const PlinkReader&) (src/core/grm_builder.cpp:63-405).
+//
+// Algebra. With X the 0/1/2 dosage (missing calls and filtered SNPs set to
+// 0), O the observed mask and mu_j = (double)(float)(2 p_j), the reference
+// forms Z = O .* (X - 1 mu^T) and G = Z Z^T / S. Expanding:
+//   Z Z^T(i,k) = X X^T(i,k) - u_i - u_k + C + E(i,k) + E(k,i)
+//   u_i   = sum_j mu_j X_ij + sum_{j miss i} mu_j^2        (per sample)
+//   C     = sum_j mu_j^2                                   (scalar)
+//   E(k,i)= sum_{j miss k} mu_j X_ij + F(k,i)/2,  F(k,i) = sum_{j miss i and k} mu_j^2
+// X X^T is an exact integer product (INT8 MFMA, INT32 accumulation). Every
+// mu_j is a multiple of 2^-29 (2p >= 2^-6 after the MAF filter), so the
+// mu-weighted dosage sums are computed exactly as integers
+// K_j = mu_j * 2^29 (u_i's first term, and E via dot4 byte chunks of K_j).
+// Without missing calls E = 0 and the correction is rank-1.
+//
+// Kernels:
+//   k_snp_stats   per-SNP counts + filter (grm_builder.cpp:175-263), HBM-bound
+//   k_pack        SNP-major BED -> sample-major 2-bit dosage tiles, HBM-bound
+//   k_row_sums    exact integer sum_j K_j X_ij per sample
+//   k_miss_*      missing-call lists, F (LDS u64 atomics), E (dot4)
+//   k_gram        INT8 MFMA Gram over lower-triangle 256x256 tiles with the
+//                 FP64 centring/scaling fused into the epilogue, MFMA-bound
+#include <cmath>
+#include <numeric>
+
+#include "cb_genotypes.h"
+#include "cb_internal.h"
+
+using namespace cbamd;
+
+namespace {
+
+constexpr int TILE = 256;        // samples per Gram tile side
+constexpr int STAGE_SNPS = 256;  // SNPs per LDS stage (16 packed words)
+constexpr int LDS_PITCH = 20;    // words per LDS row (16 + 4 pad: conflict-free b128 reads)
+constexpr double TWO_M29 = 1.0 / 536870912.0;  // 2^-29
+constexpr int F_CHUNK = 16384;   // samples per F-kernel LDS row chunk (128 KiB of u64)
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// 2-bit-field mask (low bit of each field) for the samples of word w that are < n.
+__device__ __forceinline__ uint32_t field_mask(int64_t n, int64_t w) {
+    const int64_t rem = n - w * 16;
+    if (rem >= 16) return 0x55555555u;
+    if (rem <= 0) return 0u;
+    return 0x55555555u & ((1u << (2 * rem)) - 1u);
+}
+
+// PLINK codes (plink_reader.cpp:17-20) -> dosage in each 2-bit field:
+// 00 -> 2, 01 (missing) -> 0, 10 -> 1, 11 -> 0.
+__device__ __forceinline__ uint32_t dosage_swar(uint32_t x) {
+    const uint32_t lo = x & 0x55555555u;
+    const uint32_t hi = (x >> 1) & 0x55555555u;
+    return (hi & ~lo) | ((~(hi | lo) & 0x55555555u) << 1);
+}
+
+// ---------------------------------------------------------------------------
+// Per-SNP statistics: one wave per SNP, popcounts over 16-sample words.
+__global__ __launch_bounds__(256) void k_snp_stats(
+    const uint8_t* __restrict__ bed, int64_t pitch, int64_t n, int64_t m, double min_maf,
+    int32_t* __restrict__ o_nobs, int32_t* __restrict__ o_sumx, int32_t* __restrict__ o_h1,
+    int32_t* __restrict__ o_het, int32_t* __restrict__ o_h2, uint8_t* __restrict__ o_valid,
+    float* __restrict__ o_freq, float* __restrict__ o_mean, float* __restrict__ o_2pq,
+    uint32_t* __restrict__ o_kfix) {
+#pragma clang fp contract(off)
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t j = (int64_t)blockIdx.x * 4 + wave;
+    if (j >= m) return;
+    const uint32_t* row = reinterpret_cast<const uint32_t*>(bed + j * pitch);
+    const int64_t nwords = (n + 15) / 16;
+    uint32_t c_miss = 0, c_h1 = 0, c_het = 0, c_h2 = 0;
+    for (int64_t w = lane; w < nwords; w += 64) {
+        const uint32_t x = row[w];
+        const uint32_t lo = x & 0x55555555u, hi = (x >> 1) & 0x55555555u;
+        const uint32_t vm = field_mask(n, w);
+        c_h1 += __popc(~hi & ~lo & vm);
+        c_miss += __popc(~hi & lo & vm);
+        c_het += __popc(hi & ~lo & vm);
+        c_h2 += __popc(hi & lo & vm);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        c_miss += __shfl_down(c_miss, off);
+        c_h1 += __shfl_down(c_h1, off);
+        c_het += __shfl_down(c_het, off);
+        c_h2 += __shfl_down(c_h2, off);
+    }
+    if (lane != 0) return;
+    const int64_t obs = n - (int64_t)c_miss;
+    const int64_t sx = 2 * (int64_t)c_h1 + (int64_t)c_het;
+    o_nobs[j] = (int32_t)obs;
+    o_sumx[j] = (int32_t)sx;
+    o_h1[j] = (int32_t)c_h1;
+    o_het[j] = (int32_t)c_het;
+    o_h2[j] = (int32_t)c_h2;
+    uint8_t valid = 0;
+    float freq_f = -1.0f, mean_f = 0.0f, twopq_f = 0.0f;
+    uint32_t kfix = 0;
+    if (obs > 0) {
+        // grm_builder.cpp:219-246, same operations in the same order.
+        const double freq = (double)sx / (2.0 * (double)obs);
+        const double maf = freq < 1.0 - freq ? freq : 1.0 - freq;
+        if (!(maf < min_maf || fabs(maf) < 1e-8)) {
+            twopq_f = (float)((2.0 * freq) * (1.0 - freq));
+            freq_f = (float)freq;
+            mean_f = (float)(freq * 2.0);
+            kfix = (uint32_t)((double)mean_f * 536870912.0);
+            valid = 1;
+        }
+    }
+    o_valid[j] = valid;
+    o_freq[j] = freq_f;
+    o_mean[j] = mean_f;
+    o_2pq[j] = twopq_f;
+    o_kfix[j] = kfix;
+}
+
+// ---------------------------------------------------------------------------
+// SNP-major BED -> sample-major 2-bit dosage, tiled [stage][sample][16 words]:
+// word w of stage s for sample i holds SNPs 256s+16w+t at bits 2t.
+// Filtered SNPs, missing calls, padded SNPs and padded samples are 0.
+__global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ bed, int64_t pitch,
+                                              int64_t n, int64_t m, int64_t n_pad,
+                                              const uint8_t* __restrict__ valid,
+                                              uint32_t* __restrict__ X2) {
+    __shared__ uint32_t lds[256 * 17];
+    const int t = threadIdx.x;
+    const int64_t sb = blockIdx.x;  // 256-sample block
+    const int64_t st = blockIdx.y;  // 256-SNP stage
+    const int64_t j = st * STAGE_SNPS + t;
+    const bool ok = j < m && valid[j] != 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int64_t byte0 = sb * 64 + q * 16;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (ok && byte0 < pitch) v = *reinterpret_cast<const uint4*>(bed + j * pitch + byte0);
+        const int64_t w0 = sb * 16 + q * 4;
+        uint32_t f[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t vm = field_mask(n, w0 + e);
+            lds[t * 17 + q * 4 + e] = dosage_swar(f[e]) & (vm | (vm << 1));
+        }
+    }
+    __syncthreads();
+    const int sw = t >> 4, sf = t & 15;
+    uint32_t outw[16];
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const uint32_t x = lds[(16 * w + s) * 17 + sw];
+            acc |= ((x >> (2 * sf)) & 3u) << (2 * s);
+        }
+        outw[w] = acc;
+    }
+    uint4* dst = reinterpret_cast<uint4*>(X2 + ((st * n_pad) + sb * TILE + t) * 16);
+    dst[0] = make_uint4(outw[0], outw[1], outw[2], outw[3]);
+    dst[1] = make_uint4(outw[4], outw[5], outw[6], outw[7]);
+    dst[2] = make_uint4(outw[8], outw[9], outw[10], outw[11]);
+    dst[3] = make_uint4(outw[12], outw[13], outw[14], outw[15]);
+}
+
+// ---------------------------------------------------------------------------
+// r_i = sum_j mu_j X_ij, exact: integer sum of K_j X_ij times 2^-29.
+__global__ __launch_bounds__(256) void k_row_sums(const uint32_t* __restrict__ X2, int64_t n,
+                                                  int64_t n_pad, int64_t n_stages,
+                                                  const uint32_t* __restrict__ kfix,
+                                                  double* __restrict__ r_out) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + wave;
+    if (i >= n) return;
+    uint64_t acc = 0;
+    const int64_t nq = n_stages * 16;
+    for (int64_t q = lane; q < nq; q += 64) {
+        const int64_t s = q >> 4, w = q & 15;
+        const uint32_t x = X2[(s * n_pad + i) * 16 + w];
+        if (x == 0) continue;
+        const uint32_t* kk = kfix + s * STAGE_SNPS + w * 16;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) acc += (uint64_t)(kk[t] * ((x >> (2 * t)) & 3u));
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+    if (lane == 0) r_out[i] = (double)acc * TWO_M29;
+}
+
+// ---------------------------------------------------------------------------
+// Missing-call lists. Per SNP (valid only): the sorted list S_j of samples
+// missing at j (wave-ordered compaction) and per-sample counts.
+__global__ __launch_bounds__(256) void k_miss_snp_lists(
+    const uint8_t* __restrict__ bed, int64_t pitch, int64_t n, int64_t m,
+    const uint8_t* __restrict__ valid, const int64_t* __restrict__ snp_off,
+    int32_t* __restrict__ snp_list, int32_t* __restrict__ sample_cnt) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t j = (int64_t)blockIdx.x * 4 + wave;
+    if (j >= m || !valid[j]) return;
+    const uint32_t* row = reinterpret_cast<const uint32_t*>(bed + j * pitch);
+    const int64_t nwords = (n + 15) / 16;
+    int64_t base = snp_off[j];
+    for (int64_t w0 = 0; w0 < nwords; w0 += 64) {
+        const int64_t w = w0 + lane;
+        uint32_t mb = 0;
+        if (w < nwords) {
+            const uint32_t x = row[w];
+            mb = ~(x >> 1) & x & field_mask(n, w);  // code 01
+        }
+        const uint32_t c = __popc(mb);
+        uint32_t incl = c;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off);
+            if (lane >= off) incl += y;
+        }
+        const uint32_t total = __shfl(incl, 63);
+        int64_t pos = base + incl - c;
+        while (mb) {
+            const int f = __ffs(mb) - 1;  // even bit position
+            mb &= mb - 1;
+            const int64_t i = w * 16 + (f >> 1);
+            snp_list[pos++] = (int32_t)i;
+            atomicAdd(&sample_cnt[i], 1);
+        }
+        base += total;
+    }
+}
+
+// Per-sample lists (entry = SNP index and K_j), filled from the SNP lists.
+// Entry order inside a sample list is arbitrary: every consumer sums exact
+// integers. Pads (to a multiple of 4) keep j = 0 and K = 0.
+__global__ __launch_bounds__(256) void k_miss_sample_lists(
+    const int64_t* __restrict__ snp_off, const int32_t* __restrict__ snp_list, int64_t m,
+    const uint8_t* __restrict__ valid, const uint32_t* __restrict__ kfix,
+    const int64_t* __restrict__ smp_off, int32_t* __restrict__ cursor,
+    int32_t* __restrict__ lj, uint32_t* __restrict__ lk,
+    unsigned long long* __restrict__ sq_sum) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t j = (int64_t)blockIdx.x * 4 + wave;
+    if (j >= m || !valid[j]) return;
+    const uint32_t K = kfix[j];
+    const unsigned long long k2 = ((unsigned long long)K * K) >> 6;
+    for (int64_t p = snp_off[j] + lane; p < snp_off[j + 1]; p += 64) {
+        const int32_t i = snp_list[p];
+        const int64_t pos = smp_off[i] + atomicAdd(&cursor[i], 1);
+        lj[pos] = (int32_t)j;
+        lk[pos] = K;
+        atomicAdd(&sq_sum[i], k2);
+    }
+}
+
+// F/2 rows: E(k, c0..c0+F_CHUNK) = 1/2 sum_{j miss k and i} mu_j^2, with
+// mu_j^2 carried as (K_j^2 >> 6) * 2^-52 in u64 LDS accumulators.
+// This kernel initialises every element of E it covers.
+__global__ __launch_bounds__(256) void k_miss_f(const int64_t* __restrict__ smp_off,
+                                                const int32_t* __restrict__ smp_cnt,
+                                                const int32_t* __restrict__ lj,
+                                                const uint32_t* __restrict__ lk,
+                                                const int64_t* __restrict__ snp_off,
+                                                const int32_t* __restrict__ snp_list, int64_t n,
+                                                double* __restrict__ E, int64_t lde) {
+    __shared__ unsigned long long acc[F_CHUNK];
+    const int64_t k = blockIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.y * F_CHUNK;
+    const int64_t c1 = c0 + F_CHUNK < n ? c0 + F_CHUNK : n;
+    for (int t = threadIdx.x; t < F_CHUNK; t += 256) acc[t] = 0ull;
+    __syncthreads();
+    const int64_t b = smp_off[k], cnt = smp_cnt[k];
+    for (int64_t p = b + threadIdx.x; p < b + cnt; p += 256) {
+        const int32_t j = lj[p];
+        const uint32_t K = lk[p];
+        const unsigned long long k2 = ((unsigned long long)K * K) >> 6;
+        int64_t lo = snp_off[j], hi = snp_off[j + 1];
+        // first entry >= c0
+        int64_t a = lo, z = hi;
+        while (a < z) {
+            const int64_t mid = (a + z) >> 1;
+            if (snp_list[mid] < c0) a = mid + 1; else z = mid;
+        }
+        for (int64_t q = a; q < hi; ++q) {
+            const int32_t i = snp_list[q];
+            if (i >= c1) break;
+            atomicAdd(&acc[i - c0], k2);
+        }
+    }
+    __syncthreads();
+    const double scale = 0.5 / 4503599627370496.0;  // (1/2) * 2^-52
+    for (int64_t i = c0 + threadIdx.x; i < c1; i += 256)
+        E[k * lde + i] = (double)acc[i - c0] * scale;
+}
+
+// E(k, i) += sum_{j miss k} mu_j X_ij for the 256 samples i of this block:
+// a lane owns 4 samples (one BED byte per SNP row); four list entries are
+// consumed per step as one dword of 4 SNP bytes, turned into per-sample
+// dosage bytes and dotted (v_dot4_u32_u8) with the four byte chunks of K.
+__global__ __launch_bounds__(256) void k_miss_cross(const uint8_t* __restrict__ bed,
+                                                    int64_t pitch, int64_t n,
+                                                    const int64_t* __restrict__ smp_off,
+                                                    const int32_t* __restrict__ lj,
+                                                    const uint32_t* __restrict__ lk, int kw,
+                                                    double* __restrict__ E, int64_t lde) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int64_t I0 = (int64_t)blockIdx.x * TILE;
+    const int64_t byte = I0 / 4 + lane;
+    const bool inrow = byte < pitch;
+    const int per_wave = kw / 4;
+    for (int kq = 0; kq < per_wave; ++kq) {
+        const int64_t k = (int64_t)blockIdx.y * kw + wave * per_wave + kq;
+        if (k >= n) break;
+        uint32_t acc[4][4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[s][c] = 0;
+        const int64_t b = smp_off[k], e = smp_off[k + 1];
+        for (int64_t p = b; p < e; p += 4) {
+            const int32_t j0 = lj[p], j1 = lj[p + 1], j2 = lj[p + 2], j3 = lj[p + 3];
+            const uint32_t K0 = lk[p], K1 = lk[p + 1], K2 = lk[p + 2], K3 = lk[p + 3];
+            uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+            if (inrow) {
+                c0 = bed[(int64_t)j0 * pitch + byte];
+                c1 = bed[(int64_t)j1 * pitch + byte];
+                c2 = bed[(int64_t)j2 * pitch + byte];
+                c3 = bed[(int64_t)j3 * pitch + byte];
+            }
+            const uint32_t D = dosage_swar(c0 | (c1 << 8) | (c2 << 16) | (c3 << 24));
+            uint32_t kc[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                kc[c] = ((K0 >> (8 * c)) & 0xffu) | (((K1 >> (8 * c)) & 0xffu) << 8) |
+                        (((K2 >> (8 * c)) & 0xffu) << 16) | (((K3 >> (8 * c)) & 0xffu) << 24);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const uint32_t A = (D >> (2 * s)) & 0x03030303u;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[s][c] = __builtin_amdgcn_udot4(A, kc[c], acc[s][c], false);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int64_t i = I0 + 4 * lane + s;
+            if (i >= n) continue;
+            const uint64_t v = (uint64_t)acc[s][0] + ((uint64_t)acc[s][1] << 8) +
+                               ((uint64_t)acc[s][2] << 16) + ((uint64_t)acc[s][3] << 24);
+            E[k * lde + i] += (double)v * TWO_M29;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// INT8-MFMA Gram over the lower triangle of 256x256 tiles.
+// 4 waves in 2x2, each 128x128 = 4x4 blocks of v_mfma_i32_32x32x32_i8.
+// A 32x32x32 fragment takes 16 int8 per lane = one packed word decoded with
+// (w >> 2q) & 0x03030303; A and B fragments share that k-layout, so the
+// product is the Gram regardless of how the hardware orders k.
+// MODE 0: FP64/FP32 G with fused correction; MODE 1: raw int32 Gram.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, int64_t n,
+                                              int64_t n_pad, int64_t n_stages,
+                                              const double* __restrict__ u, double Cc,
+                                              double S, const double* __restrict__ E,
+                                              int64_t lde, double* __restrict__ G64,
+                                              float* __restrict__ G32, int32_t* __restrict__ Gi,
+                                              int64_t ld) {
+    __shared__ __attribute__((aligned(16))) uint32_t ldsA[TILE * LDS_PITCH];
+    __shared__ __attribute__((aligned(16))) uint32_t ldsB[TILE * LDS_PITCH];
+
+    // XCD-aware order: hardware deals workgroups round-robin over 8 XCDs;
+    // give each XCD a contiguous run of tiles so neighbours share L2 panels.
+    const int64_t nb = gridDim.x, bid = blockIdx.x;
+    const int64_t xcd = bid % 8, q8 = nb / 8, r8 = nb % 8;
+    const int64_t L = xcd * q8 + (xcd < r8 ? xcd : r8) + bid / 8;
+    int64_t ti = (int64_t)((sqrt(8.0 * (double)L + 1.0) - 1.0) * 0.5);
+    while (ti * (ti + 1) / 2 > L) --ti;
+    while ((ti + 1) * (ti + 2) / 2 <= L) ++ti;
+    const int64_t tj = L - ti * (ti + 1) / 2;
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int h = lane >> 5, r = lane & 31;
+
+    v16i acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = (v16i){0};
+
+    const uint4* gA = reinterpret_cast<const uint4*>(X2 + (ti * TILE + tid) * 16);
+    const uint4* gB = reinterpret_cast<const uint4*>(X2 + (tj * TILE + tid) * 16);
+    const int64_t stage_stride = n_pad * 4;  // in uint4
+    uint4 pa[4], pb[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        pa[e] = gA[e];
+        pb[e] = gB[e];
+    }
+    for (int64_t s = 0; s < n_stages; ++s) {
+        __syncthreads();
+        uint4* la = reinterpret_cast<uint4*>(ldsA + tid * LDS_PITCH);
+        uint4* lb = reinterpret_cast<uint4*>(ldsB + tid * LDS_PITCH);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            la[e] = pa[e];
+            lb[e] = pb[e];
+        }
+        __syncthreads();
+        if (s + 1 < n_stages) {
+            const int64_t o = (s + 1) * stage_stride;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                pa[e] = gA[o + e];
+                pb[e] = gB[o + e];
+            }
+        }
+        uint4 wa[4][2], wb[4][2];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const uint4* p = reinterpret_cast<const uint4*>(ldsA + (wr * 128 + a * 32 + r) * LDS_PITCH + 8 * h);
+            wa[a][0] = p[0];
+            wa[a][1] = p[1];
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const uint4* p = reinterpret_cast<const uint4*>(ldsB + (wc * 128 + b * 32 + r) * LDS_PITCH + 8 * h);
+            wb[b][0] = p[0];
+            wb[b][1] = p[1];
+        }
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) {
+            v4i fa[4], fb[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const uint4 v = wa[a][kk >> 2];
+                const uint32_t w = (kk & 3) == 0 ? v.x : (kk & 3) == 1 ? v.y : (kk & 3) == 2 ? v.z : v.w;
+                fa[a] = (v4i){(int)(w & 0x03030303u), (int)((w >> 2) & 0x03030303u),
+                              (int)((w >> 4) & 0x03030303u), (int)((w >> 6) & 0x03030303u)};
+            }
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint4 v = wb[b][kk >> 2];
+                const uint32_t w = (kk & 3) == 0 ? v.x : (kk & 3) == 1 ? v.y : (kk & 3) == 2 ? v.z : v.w;
+                fb[b] = (v4i){(int)(w & 0x03030303u), (int)((w >> 2) & 0x03030303u),
+                              (int)((w >> 4) & 0x03030303u), (int)((w >> 6) & 0x03030303u)};
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
+        }
+    }
+
+    // Epilogue, one 32x32 block at a time through a wave-private LDS slab
+    // (the main-loop panels are dead by now), so that both the own-position
+    // rows and the mirrored rows are written with coalesced stores.
+    // 32x32 accumulator layout: register q of lane l holds
+    // row 8(q/4) + 4(l/32) + q%4, column l%32.
+    __syncthreads();
+    double* slab = reinterpret_cast<double*>(ldsA) + wave * (32 * 33);  // 4 x 8.4 KB <= 40 KB
+    const bool diag = ti == tj;
+    const double Sdiv = S > 0.0 ? S : 1.0;
+#pragma unroll
+    for (int ab = 0; ab < 16; ++ab) {
+        const int a = ab >> 2, b = ab & 3;
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            slab[(8 * (q >> 2) + 4 * h + (q & 3)) * 33 + r] = (double)acc[a][b][q];
+        __syncthreads();
+        const int64_t I0 = ti * TILE + wr * 128 + a * 32;  // block rows
+        const int64_t J0 = tj * TILE + wc * 128 + b * 32;  // block columns
+        if (MODE == 1) {
+            // own rows (lane -> column), then mirrored rows
+#pragma unroll 1
+            for (int rr = 0; rr < 16; ++rr) {
+                const int row = h * 16 + rr;
+                const int64_t i = I0 + row, j = J0 + r;
+                if (i < n && j < n) Gi[i * ld + j] = (int32_t)slab[row * 33 + r];
+                if (!diag) {
+                    const int64_t mi = J0 + row, mj = I0 + r;  // G(mi, mj) = G(mj, mi)
+                    if (mi < n && mj < n) Gi[mi * ld + mj] = (int32_t)slab[r * 33 + row];
+                }
+            }
+        } else {
+            // pass 1 (row pattern): t = x - u_i - u_j + C (+ E(i,j))
+#pragma unroll 1
+            for (int rr = 0; rr < 16; ++rr) {
+                const int row = h * 16 + rr;
+                const int64_t i = I0 + row, j = J0 + r;
+                if (i < n && j < n) {
+                    double v = ((slab[row * 33 + r] - u[i]) - u[j]) + Cc;
+                    if (E) v += E[i * lde + j];
+                    slab[row * 33 + r] = v;
+                }
+            }
+            __syncthreads();
+            // pass 2 (transposed pattern): + E(j,i), divide, write mirror
+#pragma unroll 1
+            for (int rr = 0; rr < 16; ++rr) {
+                const int col = h * 16 + rr;      // block column of the element
+                const int64_t i = I0 + r, j = J0 + col;
+                if (i < n && j < n) {
+                    double v = slab[r * 33 + col];
+                    if (E) v += E[j * lde + i];
+                    const double g = v / Sdiv;
+                    slab[r * 33 + col] = g;
+                    if (!diag) {
+                        if (G64) G64[j * ld + i] = g;
+                        if (G32) G32[j * ld + i] = (float)g;
+                    }
+                }
+            }
+            __syncthreads();
+            // pass 3 (row pattern): write own positions
+#pragma unroll 1
+            for (int rr = 0; rr < 16; ++rr) {
+                const int row = h * 16 + rr;
+                const int64_t i = I0 + row, j = J0 + r;
+                if (i < n && j < n) {
+                    const double g = slab[row * 33 + r];
+                    if (G64) G64[i * ld + j] = g;
+                    if (G32) G32[i * ld + j] = (float)g;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+struct cb_grm {
+    cb_context* ctx = nullptr;
+    cb_grm_options opts{};
+    int64_t n = 0, m = 0;
+    int outputs = 0;
+    double sum2pq = 0.0;
+    int64_t total_valid = 0;
+    // per-SNP stats (device)
+    DevBuf<int32_t> nobs, sumx, h1, het, h2;
+    DevBuf<uint8_t> valid;
+    DevBuf<float> freq, mean, twopq;
+    DevBuf<uint32_t> kfix;
+    // outputs
+    DevBuf<double> G64;
+    DevBuf<float> G32;
+    DevBuf<int32_t> Gi;
+};
+
+static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
+    cb_context* ctx = grm->ctx;
+    hipStream_t st = ctx->stream;
+    const int64_t n = g->n, m = g->m;
+    const int64_t n_pad = round_up(n, TILE);
+    const int64_t m_pad = round_up(m, STAGE_SNPS);
+    const int64_t n_stages = m_pad / STAGE_SNPS;
+    grm->n = n;
+    grm->m = m;
+    grm->outputs = outputs;
+
+    // 1. per-SNP statistics
+    grm->nobs.alloc(m); grm->sumx.alloc(m); grm->h1.alloc(m); grm->het.alloc(m); grm->h2.alloc(m);
+    grm->valid.alloc(m); grm->freq.alloc(m); grm->mean.alloc(m); grm->twopq.alloc(m);
+    grm->kfix.alloc(m_pad);
+    CB_HIP(hipMemsetAsync(grm->kfix.p, 0, grm->kfix.bytes(), st));
+    const double min_maf = (double)grm->opts.min_maf;
+    timed(ctx, "k_snp_stats", [&] {
+        k_snp_stats<<<dim3((unsigned)ceil_div(m, 4)), dim3(256), 0, st>>>(
+            g->bed.p, g->pitch, n, m, min_maf, grm->nobs.p, grm->sumx.p, grm->h1.p, grm->het.p,
+            grm->h2.p, grm->valid.p, grm->freq.p, grm->mean.p, grm->twopq.p, grm->kfix.p);
+    });
+    std::vector<uint8_t> hvalid(m);
+    std::vector<float> hmean(m), h2pq(m);
+    std::vector<int32_t> hnobs(m);
+    CB_HIP(hipMemcpyAsync(hvalid.data(), grm->valid.p, m, hipMemcpyDeviceToHost, st));
+    CB_HIP(hipMemcpyAsync(hmean.data(), grm->mean.p, m * 4, hipMemcpyDeviceToHost, st));
+    CB_HIP(hipMemcpyAsync(h2pq.data(), grm->twopq.p, m * 4, hipMemcpyDeviceToHost, st));
+    CB_HIP(hipMemcpyAsync(hnobs.data(), grm->nobs.p, m * 4, hipMemcpyDeviceToHost, st));
+    CB_HIP(hipStreamSynchronize(st));
+
+    // sum(2pq) in the reference's order: double sum of float 2pq per SNP
+    // block, then across blocks (grm_builder.cpp:272-278,353); C = sum mu^2.
+    double total = 0.0, Cc = 0.0;
+    int64_t nvalid = 0, nmiss_total = 0;
+    const int64_t bs = grm->opts.block_size;
+    for (int64_t b0 = 0; b0 < m; b0 += bs) {
+        double blk = 0.0;
+        const int64_t b1 = std::min(m, b0 + bs);
+        for (int64_t j = b0; j < b1; ++j) {
+            if (!hvalid[j]) continue;
+            blk += (double)h2pq[j];
+            const double mu = (double)hmean[j];
+            Cc += mu * mu;
+            ++nvalid;
+            nmiss_total += n - hnobs[j];
+        }
+        total += blk;
+    }
+    grm->sum2pq = total;
+    grm->total_valid = nvalid;
+
+    // 2. sample-major packed dosages
+    DevBuf<uint32_t> X2;
+    X2.alloc((size_t)n_stages * n_pad * 16);
+    timed(ctx, "k_pack", [&] {
+        k_pack<<<dim3((unsigned)(n_pad / TILE), (unsigned)n_stages), dim3(256), 0, st>>>(
+            g->bed.p, g->pitch, n, m, n_pad, grm->valid.p, X2.p);
+    });
+
+    // 3. per-sample centring terms
+    DevBuf<double> u;
+    u.alloc(n_pad);
+    CB_HIP(hipMemsetAsync(u.p, 0, u.bytes(), st));
+    timed(ctx, "k_row_sums", [&] {
+        k_row_sums<<<dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st>>>(X2.p, n, n_pad, n_stages,
+                                                                        grm->kfix.p, u.p);
+    });
+
+    // 4. missing calls among the kept SNPs
+    DevBuf<double> E;
+    if (nmiss_total > 0 && (outputs & (CB_GRM_OUT_F64 | CB_GRM_OUT_F32))) {
+        std::vector<int64_t> snp_off(m + 1, 0);
+        for (int64_t j = 0; j < m; ++j) snp_off[j + 1] = snp_off[j] + (hvalid[j] ? n - hnobs[j] : 0);
+        DevBuf<int64_t> d_snp_off;
+        d_snp_off.alloc(m + 1);
+        CB_HIP(hipMemcpyAsync(d_snp_off.p, snp_off.data(), (m + 1) * 8, hipMemcpyHostToDevice, st));
+        DevBuf<int32_t> snp_list, smp_cnt;
+        snp_list.alloc(std::max<int64_t>(1, nmiss_total));
+        smp_cnt.alloc(n);
+        CB_HIP(hipMemsetAsync(smp_cnt.p, 0, smp_cnt.bytes(), st));
+        timed(ctx, "k_miss_snp_lists", [&] {
+            k_miss_snp_lists<<<dim3((unsigned)ceil_div(m, 4)), dim3(256), 0, st>>>(
+                g->bed.p, g->pitch, n, m, grm->valid.p, d_snp_off.p, snp_list.p, smp_cnt.p);
+        });
+        std::vector<int32_t> hcnt(n);
+        CB_HIP(hipMemcpyAsync(hcnt.data(), smp_cnt.p, n * 4, hipMemcpyDeviceToHost, st));
+        CB_HIP(hipStreamSynchronize(st));
+        std::vector<int64_t> smp_off(n + 1, 0);
+        for (int64_t i = 0; i < n; ++i) smp_off[i + 1] = smp_off[i] + round_up(hcnt[i], 4);
+        DevBuf<int64_t> d_smp_off;
+        d_smp_off.alloc(n + 1);
+        CB_HIP(hipMemcpyAsync(d_smp_off.p, smp_off.data(), (n + 1) * 8, hipMemcpyHostToDevice, st));
+        const int64_t nlist = std::max<int64_t>(4, smp_off[n]);
+        DevBuf<int32_t> lj, cursor;
+        DevBuf<uint32_t> lk;
+        DevBuf<unsigned long long> sq;
+        lj.alloc(nlist);
+        lk.alloc(nlist);
+        cursor.alloc(n);
+        sq.alloc(n);
+        CB_HIP(hipMemsetAsync(lj.p, 0, lj.bytes(), st));
+        CB_HIP(hipMemsetAsync(lk.p, 0, lk.bytes(), st));
+        CB_HIP(hipMemsetAsync(cursor.p, 0, cursor.bytes(), st));
+        CB_HIP(hipMemsetAsync(sq.p, 0, sq.bytes(), st));
+        timed(ctx, "k_miss_sample_lists", [&] {
+            k_miss_sample_lists<<<dim3((unsigned)ceil_div(m, 4)), dim3(256), 0, st>>>(
+                d_snp_off.p, snp_list.p, m, grm->valid.p, grm->kfix.p, d_smp_off.p, cursor.p, lj.p,
+                lk.p, sq.p);
+        });
+        // u_i += sum_{j miss i} mu_j^2
+        std::vector<unsigned long long> hsq(n);
+        std::vector<double> hu(n);
+        CB_HIP(hipMemcpyAsync(hsq.data(), sq.p, n * 8, hipMemcpyDeviceToHost, st));
+        CB_HIP(hipMemcpyAsync(hu.data(), u.p, n * 8, hipMemcpyDeviceToHost, st));
+        CB_HIP(hipStreamSynchronize(st));
+        for (int64_t i = 0; i < n; ++i) hu[i] += (double)hsq[i] * (1.0 / 4503599627370496.0);
+        CB_HIP(hipMemcpyAsync(u.p, hu.data(), n * 8, hipMemcpyHostToDevice, st));
+
+        E.alloc((size_t)n * n);
+        timed(ctx, "k_miss_f", [&] {
+            k_miss_f<<<dim3((unsigned)n, (unsigned)ceil_div(n, F_CHUNK)), dim3(256), 0, st>>>(
+                d_smp_off.p, smp_cnt.p, lj.p, lk.p, d_snp_off.p, snp_list.p, n, E.p, n);
+        });
+        const int kw = 64;
+        timed(ctx, "k_miss_cross", [&] {
+            k_miss_cross<<<dim3((unsigned)(n_pad / TILE), (unsigned)ceil_div(n, kw)), dim3(256), 0, st>>>(
+                g->bed.p, g->pitch, n, d_smp_off.p, lj.p, lk.p, kw, E.p, n);
+        });
+        CB_HIP(hipStreamSynchronize(st));  // list buffers die at scope end
+    }
+
+    // 5. Gram + fused correction
+    const int64_t T = n_pad / TILE;
+    const int64_t ntiles = T * (T + 1) / 2;
+    CB_REQUIRE(ntiles < (int64_t(1) << 31), "GRM too large for one launch");
+    if (outputs & CB_GRM_OUT_F64) grm->G64.alloc((size_t)n * n); else grm->G64.release();
+    if (outputs & CB_GRM_OUT_F32) grm->G32.alloc((size_t)n * n); else grm->G32.release();
+    if (outputs & CB_GRM_OUT_INT) grm->Gi.alloc((size_t)n * n); else grm->Gi.release();
+    if (outputs & (CB_GRM_OUT_F64 | CB_GRM_OUT_F32)) {
+        timed(ctx, "k_gram", [&] {
+            k_gram<0><<<dim3((unsigned)ntiles), dim3(256), 0, st>>>(
+                X2.p, n, n_pad, n_stages, u.p, Cc, total, E.p, n, grm->G64.p, grm->G32.p, nullptr, n);
+        });
+    }
+    if (outputs & CB_GRM_OUT_INT) {
+        timed(ctx, "k_gram_int", [&] {
+            k_gram<1><<<dim3((unsigned)ntiles), dim3(256), 0, st>>>(
+                X2.p, n, n_pad, n_stages, u.p, Cc, total, nullptr, n, nullptr, nullptr, grm->Gi.p, n);
+        });
+    }
+    CB_HIP(hipStreamSynchronize(st));  // X2 / u / E are released at scope end
+}
+
+extern "C" {
+
+void cb_grm_options_default(cb_grm_options* o) {
+    if (!o) return;
+    o->block_size = 2048;
+    o->min_maf = 0.01f;
+    o->min_hwe = 0.0f;
+    o->algorithm = 1;
+}
+
+int cb_grm_create(cb_context* ctx, const cb_grm_options* opts, cb_grm** out) {
+    return guarded([&] {
+        CB_REQUIRE(ctx && out, "cb_grm_create: NULL argument");
+        cb_grm_options o;
+        cb_grm_options_default(&o);
+        if (opts) o = *opts;
+        CB_REQUIRE(o.algorithm == 1, "cb_grm_create: only algorithm 1 (VanRaden) is implemented");
+        CB_REQUIRE(o.min_hwe == 0.0f, "cb_grm_create: HWE filtering is not implemented (min_hwe must be 0)");
+        CB_REQUIRE(o.block_size > 0, "cb_grm_create: block_size must be positive");
+        CB_REQUIRE(o.min_maf >= 0.0078125f,
+                   "cb_grm_create: min_maf below 2^-7 is not supported by the integer centring");
+        auto* g = new cb_grm();
+        g->ctx = ctx;
+        g->opts = o;
+        *out = g;
+    });
+}
+
+void cb_grm_destroy(cb_grm* grm) {
+    if (!grm) return;
+    (void)hipSetDevice(grm->ctx->device);
+    delete grm;
+}
+
+int cb_grm_compute(cb_grm* grm, const cb_genotypes* g, int outputs) {
+    return guarded([&] {
+        CB_REQUIRE(grm && g, "cb_grm_compute: NULL argument");
+        CB_REQUIRE(g->ctx->device == grm->ctx->device, "cb_grm_compute: genotypes live on another device");
+        CB_REQUIRE(outputs != 0 && (outputs & ~7) == 0, "cb_grm_compute: bad output mask");
+        set_device(grm->ctx);
+        compute_grm(grm, g, outputs);
+    });
+}
+
+static void copy_out(const cb_grm* grm, const void* src, size_t esz, void* dst, int64_t ld) {
+    CB_REQUIRE(src, "requested GRM output was not computed");
+    CB_REQUIRE(dst && ld >= grm->n, "bad destination / leading dimension");
+    CB_HIP(hipSetDevice(grm->ctx->device));
+    CB_HIP(hipMemcpy2DAsync(dst, ld * esz, src, grm->n * esz, grm->n * esz, grm->n,
+                            hipMemcpyDeviceToHost, grm->ctx->stream));
+    CB_HIP(hipStreamSynchronize(grm->ctx->stream));
+}
+
+int cb_grm_copy_grm(const cb_grm* grm, double* out, int64_t ld) {
+    return guarded([&] { CB_REQUIRE(grm, "NULL grm"); copy_out(grm, grm->G64.p, 8, out, ld); });
+}
+int cb_grm_copy_grm_f32(const cb_grm* grm, float* out, int64_t ld) {
+    return guarded([&] { CB_REQUIRE(grm, "NULL grm"); copy_out(grm, grm->G32.p, 4, out, ld); });
+}
+int cb_grm_copy_int_gram(const cb_grm* grm, int32_t* out, int64_t ld) {
+    return guarded([&] { CB_REQUIRE(grm, "NULL grm"); copy_out(grm, grm->Gi.p, 4, out, ld); });
+}
+
+int cb_grm_device_ptr(const cb_grm* grm, int which, void** ptr, int64_t* ld) {
+    return guarded([&] {
+        CB_REQUIRE(grm && ptr && ld, "cb_grm_device_ptr: NULL argument");
+        void* p = which == CB_GRM_OUT_F64 ? (void*)grm->G64.p
+                  : which == CB_GRM_OUT_F32 ? (void*)grm->G32.p
+                  : which == CB_GRM_OUT_INT ? (void*)grm->Gi.p
+                                            : nullptr;
+        CB_REQUIRE(p, "cb_grm_device_ptr: output not computed");
+        *ptr = p;
+        *ld = grm->n;
+    });
+}
+
+int cb_grm_sum2pq(const cb_grm* grm, double* out) {
+    return guarded([&] { CB_REQUIRE(grm && out, "NULL argument"); *out = grm->sum2pq; });
+}
+int cb_grm_total_snps(const cb_grm* grm, int64_t* out) {
+    return guarded([&] { CB_REQUIRE(grm && out, "NULL argument"); *out = grm->total_valid; });
+}
+int cb_grm_num_samples(const cb_grm* grm, int64_t* out) {
+    return guarded([&] { CB_REQUIRE(grm && out, "NULL argument"); *out = grm->n; });
+}
+
+int cb_grm_copy_allele_freqs(const cb_grm* grm, float* out, int64_t m) {
+    return guarded([&] {
+        CB_REQUIRE(grm && out && m == grm->m && grm->freq.p, "cb_grm_copy_allele_freqs: bad argument");
+        CB_HIP(hipSetDevice(grm->ctx->device));
+        CB_HIP(hipMemcpy(out, grm->freq.p, m * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int cb_grm_copy_snp_stats(const cb_grm* grm, int64_t m, int32_t* n_obs, int32_t* sum_x,
+                          int32_t* n_hom1, int32_t* n_het, int32_t* n_hom2, uint8_t* valid,
+                          float* mean_f, float* twopq_f) {
+    return guarded([&] {
+        CB_REQUIRE(grm && m == grm->m && grm->nobs.p, "cb_grm_copy_snp_stats: bad argument");
+        CB_HIP(hipSetDevice(grm->ctx->device));
+        if (n_obs) CB_HIP(hipMemcpy(n_obs, grm->nobs.p, m * 4, hipMemcpyDeviceToHost));
+        if (sum_x) CB_HIP(hipMemcpy(sum_x, grm->sumx.p, m * 4, hipMemcpyDeviceToHost));
+        if (n_hom1) CB_HIP(hipMemcpy(n_hom1, grm->h1.p, m * 4, hipMemcpyDeviceToHost));
+        if (n_het) CB_HIP(hipMemcpy(n_het, grm->het.p, m * 4, hipMemcpyDeviceToHost));
+        if (n_hom2) CB_HIP(hipMemcpy(n_hom2, grm->h2.p, m * 4, hipMemcpyDeviceToHost));
+        if (valid) CB_HIP(hipMemcpy(valid, grm->valid.p, m, hipMemcpyDeviceToHost));
+        if (mean_f) CB_HIP(hipMemcpy(mean_f, grm->mean.p, m * 4, hipMemcpyDeviceToHost));
+        if (twopq_f) CB_HIP(hipMemcpy(twopq_f, grm->twopq.p, m * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+}  // extern "C"

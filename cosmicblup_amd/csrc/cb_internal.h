@@ -1,0 +1,134 @@
+// Internal runtime shared by the C-ABI translation units: the context
+// (device, stream, per-kernel HIP-event timing), thread-local error text,
+// and device-buffer helpers. Nothing here crosses the C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cosmicblup_amd.h"
+
+namespace cbamd {
+
+void set_error(const std::string& msg);
+
+// Error thrown inside the library; every extern "C" entry point catches it
+// and turns it into a status code plus cb_last_error() text.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define CB_HIP(call)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess)                                                 \
+            throw ::cbamd::Error(CB_ERR_DEVICE,                               \
+                                 std::string(#call " failed: ") +             \
+                                     hipGetErrorString(e_));                  \
+    } while (0)
+
+#define CB_REQUIRE(cond, msg)                                                 \
+    do {                                                                      \
+        if (!(cond)) throw ::cbamd::Error(CB_ERR_INVALID, msg);               \
+    } while (0)
+
+// Wraps an entry point body: returns CB_OK or the error code.
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return CB_OK;
+    } catch (const Error& e) {
+        set_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return CB_ERR_OOM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return CB_ERR_INTERNAL;
+    }
+}
+
+// Device buffer owned by a handle.
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count == n && p) return;
+        release();
+        if (count == 0) return;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error(CB_ERR_OOM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                        " bytes failed: " + hipGetErrorString(e));
+        }
+        n = count;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+struct TimedLaunch {
+    std::string name;
+    hipEvent_t start, stop;
+};
+
+}  // namespace cbamd
+
+struct cb_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool profiling = false;
+    std::vector<cbamd::TimedLaunch> pending;
+    std::map<std::string, std::pair<double, long>> totals;  // name -> (ms, launches)
+
+    // Resolves pending events into totals (blocks on the last event).
+    void resolve_profile();
+};
+
+namespace cbamd {
+
+// Run `launch` on ctx->stream; when profiling is on, bracket it with HIP
+// events recorded on that same stream and book the time under `name`.
+template <class F>
+void timed(cb_context* ctx, const char* name, F&& launch) {
+    if (!ctx->profiling) {
+        launch();
+        CB_HIP(hipGetLastError());
+        return;
+    }
+    TimedLaunch t;
+    t.name = name;
+    CB_HIP(hipEventCreate(&t.start));
+    CB_HIP(hipEventCreate(&t.stop));
+    CB_HIP(hipEventRecord(t.start, ctx->stream));
+    launch();
+    CB_HIP(hipGetLastError());
+    CB_HIP(hipEventRecord(t.stop, ctx->stream));
+    ctx->pending.push_back(t);
+}
+
+inline void set_device(cb_context* ctx) { CB_HIP(hipSetDevice(ctx->device)); }
+
+inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace cbamd
