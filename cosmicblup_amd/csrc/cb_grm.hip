@@ -140,7 +140,8 @@ __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ bed, i
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const uint32_t vm = field_mask(n, w0 + e);
-            lds[t * 17 + q * 4 + e] = dosage_swar(f[e]) & (vm | (vm << 1));
+            // (a zero byte is code 00 = dosage 2, so rows that are not kept are zeroed here)
+            lds[t * 17 + q * 4 + e] = ok ? dosage_swar(f[e]) & (vm | (vm << 1)) : 0u;
         }
     }
     __syncthreads();
@@ -361,8 +362,13 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
                                               int64_t lde, double* __restrict__ G64,
                                               float* __restrict__ G32, int32_t* __restrict__ Gi,
                                               int64_t ld) {
-    __shared__ __attribute__((aligned(16))) uint32_t ldsA[TILE * LDS_PITCH];
-    __shared__ __attribute__((aligned(16))) uint32_t ldsB[TILE * LDS_PITCH];
+    // One LDS block: A and B panels in the main loop (40 KB); four 32x33
+    // FP64 slabs plus four 32x33 int32 slabs (50.7 KB) in the epilogue.
+    constexpr int LDS_WORDS = 4 * 32 * 33 * 3;
+    static_assert(LDS_WORDS >= 2 * TILE * LDS_PITCH, "panels exceed LDS block");
+    __shared__ __attribute__((aligned(16))) uint32_t lds[LDS_WORDS];
+    uint32_t* ldsA = lds;
+    uint32_t* ldsB = lds + TILE * LDS_PITCH;
 
     // XCD-aware order: hardware deals workgroups round-robin over 8 XCDs;
     // give each XCD a contiguous run of tiles so neighbours share L2 panels.
@@ -449,59 +455,57 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
         }
     }
 
-    // Epilogue, one 32x32 block at a time through a wave-private LDS slab
-    // (the main-loop panels are dead by now), so that both the own-position
-    // rows and the mirrored rows are written with coalesced stores.
+    // Epilogue, one 32x32 block at a time through wave-private LDS slabs
+    // (the main-loop panels are dead by now) so that both the own-position
+    // rows and the mirrored rows are written with coalesced stores. The
+    // value is formed symmetrically in (i,j) — x_ij - (u_i + u_j) + C +
+    // (E(i,j) + E(j,i)) — so diagonal tiles come out bitwise symmetric.
     // 32x32 accumulator layout: register q of lane l holds
     // row 8(q/4) + 4(l/32) + q%4, column l%32.
     __syncthreads();
-    double* slab = reinterpret_cast<double*>(ldsA) + wave * (32 * 33);  // 4 x 8.4 KB <= 40 KB
+    double* slabD = reinterpret_cast<double*>(lds) + wave * (32 * 33);
+    int32_t* slabX = reinterpret_cast<int32_t*>(reinterpret_cast<double*>(lds) + 4 * 32 * 33) + wave * (32 * 33);
     const bool diag = ti == tj;
     const double Sdiv = S > 0.0 ? S : 1.0;
 #pragma unroll
     for (int ab = 0; ab < 16; ++ab) {
         const int a = ab >> 2, b = ab & 3;
 #pragma unroll
-        for (int q = 0; q < 16; ++q)
-            slab[(8 * (q >> 2) + 4 * h + (q & 3)) * 33 + r] = (double)acc[a][b][q];
+        for (int q = 0; q < 16; ++q) slabX[(8 * (q >> 2) + 4 * h + (q & 3)) * 33 + r] = acc[a][b][q];
         __syncthreads();
         const int64_t I0 = ti * TILE + wr * 128 + a * 32;  // block rows
         const int64_t J0 = tj * TILE + wc * 128 + b * 32;  // block columns
         if (MODE == 1) {
-            // own rows (lane -> column), then mirrored rows
 #pragma unroll 1
             for (int rr = 0; rr < 16; ++rr) {
                 const int row = h * 16 + rr;
                 const int64_t i = I0 + row, j = J0 + r;
-                if (i < n && j < n) Gi[i * ld + j] = (int32_t)slab[row * 33 + r];
+                if (i < n && j < n) Gi[i * ld + j] = slabX[row * 33 + r];
                 if (!diag) {
                     const int64_t mi = J0 + row, mj = I0 + r;  // G(mi, mj) = G(mj, mi)
-                    if (mi < n && mj < n) Gi[mi * ld + mj] = (int32_t)slab[r * 33 + row];
+                    if (mi < n && mj < n) Gi[mi * ld + mj] = slabX[r * 33 + row];
                 }
             }
         } else {
-            // pass 1 (row pattern): t = x - u_i - u_j + C (+ E(i,j))
+            if (E) {  // pass 1 (row pattern): stage E(i,j)
 #pragma unroll 1
-            for (int rr = 0; rr < 16; ++rr) {
-                const int row = h * 16 + rr;
-                const int64_t i = I0 + row, j = J0 + r;
-                if (i < n && j < n) {
-                    double v = ((slab[row * 33 + r] - u[i]) - u[j]) + Cc;
-                    if (E) v += E[i * lde + j];
-                    slab[row * 33 + r] = v;
+                for (int rr = 0; rr < 16; ++rr) {
+                    const int row = h * 16 + rr;
+                    const int64_t i = I0 + row, j = J0 + r;
+                    if (i < n && j < n) slabD[row * 33 + r] = E[i * lde + j];
                 }
+                __syncthreads();
             }
-            __syncthreads();
-            // pass 2 (transposed pattern): + E(j,i), divide, write mirror
+            // pass 2 (transposed pattern): form g, write the mirror
 #pragma unroll 1
             for (int rr = 0; rr < 16; ++rr) {
-                const int col = h * 16 + rr;      // block column of the element
+                const int col = h * 16 + rr;
                 const int64_t i = I0 + r, j = J0 + col;
                 if (i < n && j < n) {
-                    double v = slab[r * 33 + col];
-                    if (E) v += E[j * lde + i];
+                    const double e = E ? slabD[r * 33 + col] + E[j * lde + i] : 0.0;
+                    const double v = (((double)slabX[r * 33 + col] - (u[i] + u[j])) + Cc) + e;
                     const double g = v / Sdiv;
-                    slab[r * 33 + col] = g;
+                    slabD[r * 33 + col] = g;
                     if (!diag) {
                         if (G64) G64[j * ld + i] = g;
                         if (G32) G32[j * ld + i] = (float)g;
@@ -509,13 +513,13 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
                 }
             }
             __syncthreads();
-            // pass 3 (row pattern): write own positions
+            // pass 3 (row pattern): own positions
 #pragma unroll 1
             for (int rr = 0; rr < 16; ++rr) {
                 const int row = h * 16 + rr;
                 const int64_t i = I0 + row, j = J0 + r;
                 if (i < n && j < n) {
-                    const double g = slab[row * 33 + r];
+                    const double g = slabD[row * 33 + r];
                     if (G64) G64[i * ld + j] = g;
                     if (G32) G32[i * ld + j] = (float)g;
                 }
