@@ -33,6 +33,28 @@ class GrmOptions(C.Structure):
                 ("algorithm", C.c_int)]
 
 
+class SolveOptions(C.Structure):
+    _fields_ = [("precision", C.c_int), ("precond", C.c_int), ("stop_rule", C.c_int),
+                ("batch_rhs", C.c_int), ("tol", C.c_double), ("max_iter", C.c_int64),
+                ("inner_tol", C.c_double)]
+
+
+class RemlOptions(C.Structure):
+    _fields_ = [("algorithm", C.c_int), ("max_iter", C.c_int64), ("tol", C.c_double),
+                ("n_probes", C.c_int64), ("seed", C.c_uint64), ("trace_exact", C.c_int),
+                ("solve", SolveOptions)]
+
+
+CB_PREC_FP64 = 0
+CB_PREC_MIXED = 1
+CB_PRECOND_JACOBI = 0
+CB_PRECOND_BLOCK_JACOBI = 1
+CB_STOP_RELATIVE = 0
+CB_STOP_ABSOLUTE = 1
+CB_VCE_AI = 0
+CB_VCE_EM = 1
+
+
 class CosmicError(RuntimeError):
     """A non-zero status from the C-ABI (message from cb_last_error)."""
 
@@ -73,6 +95,21 @@ _SIGS = {
     "cb_grm_num_samples": (C.c_int, [c_vp, c_i64p]),
     "cb_grm_copy_allele_freqs": (C.c_int, [c_vp, c_vp, c_i64]),
     "cb_grm_copy_snp_stats": (C.c_int, [c_vp, c_i64] + [c_vp] * 8),
+    "cb_mme_create": (C.c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, C.POINTER(c_vp)]),
+    "cb_mme_destroy": (None, [c_vp]),
+    "cb_mme_dims": (C.c_int, [c_vp, c_i64p, c_i64p, c_i64p]),
+    "cb_mme_set_qinv_dense": (C.c_int, [c_vp, c_vp, c_i64]),
+    "cb_mme_set_qinv_dense_device": (C.c_int, [c_vp, c_vp, c_i64]),
+    "cb_mme_set_qinv_csr": (C.c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "cb_mme_rhs": (C.c_int, [c_vp, c_vp]),
+    "cb_mme_apply": (C.c_int, [c_vp, C.c_double, c_i64, c_vp, c_vp]),
+    "cb_solve_options_default": (None, [C.POINTER(SolveOptions)]),
+    "cb_mme_solve": (C.c_int, [c_vp, C.c_double, C.POINTER(SolveOptions), c_i64, c_vp, c_vp, c_i64p, c_dp]),
+    "cb_reml_options_default": (None, [C.POINTER(RemlOptions)]),
+    "cb_reml_run": (C.c_int, [c_vp, C.POINTER(RemlOptions), C.c_double, C.c_double, c_dp, c_dp, c_i64p,
+                              C.POINTER(C.c_int), c_vp, c_vp, c_vp]),
+    "cb_reml_score": (C.c_int, [c_vp, C.POINTER(RemlOptions), C.c_double, C.c_double, c_vp, c_vp, c_dp, c_vp]),
+    "cb_reml_copy_probes": (C.c_int, [c_vp, C.POINTER(RemlOptions), c_vp, c_i64]),
 }
 
 _lib = None

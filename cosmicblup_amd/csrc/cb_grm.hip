@@ -18,7 +18,7 @@
 //   k_snp_stats   per-SNP counts + filter (grm_builder.cpp:175-263), HBM-bound
 //   k_pack        SNP-major BED -> sample-major 2-bit dosage tiles, HBM-bound
 //   k_row_sums    exact integer sum_j K_j X_ij per sample
-//   k_miss_*      missing-call lists, F (LDS u64 atomics), E (dot4)
+//   k_miss_*      missing-call lists; E with F folded in (dot4 + rare-path LDS)
 //   k_gram        INT8 MFMA Gram over lower-triangle 256x256 tiles with the
 //                 FP64 centring/scaling fused into the epilogue, MFMA-bound
 #include <cmath>
@@ -35,10 +35,10 @@ constexpr int TILE = 256;        // samples per Gram tile side
 constexpr int STAGE_SNPS = 256;  // SNPs per LDS stage (16 packed words)
 constexpr int LDS_PITCH = 20;    // words per LDS row (16 + 4 pad: conflict-free b128 reads)
 constexpr double TWO_M29 = 1.0 / 536870912.0;  // 2^-29
-constexpr int F_CHUNK = 16384;   // samples per F-kernel LDS row chunk (128 KiB of u64)
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // 2-bit-field mask (low bit of each field) for the samples of word w that are < n.
 __device__ __forceinline__ uint32_t field_mask(int64_t n, int64_t w) {
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(256) void k_miss_snp_lists(
 
 // Per-sample lists (entry = SNP index and K_j), filled from the SNP lists.
 // Entry order inside a sample list is arbitrary: every consumer sums exact
-// integers. Pads (to a multiple of 4) keep j = 0 and K = 0.
+// integers. Pads (to a multiple of 8) keep j = 0 and K = 0.
 __global__ __launch_bounds__(256) void k_miss_sample_lists(
     const int64_t* __restrict__ snp_off, const int32_t* __restrict__ snp_list, int64_t m,
     const uint8_t* __restrict__ valid, const uint32_t* __restrict__ kfix,
@@ -249,100 +249,112 @@ __global__ __launch_bounds__(256) void k_miss_sample_lists(
     }
 }
 
-// F/2 rows: E(k, c0..c0+F_CHUNK) = 1/2 sum_{j miss k and i} mu_j^2, with
-// mu_j^2 carried as (K_j^2 >> 6) * 2^-52 in u64 LDS accumulators.
-// This kernel initialises every element of E it covers.
-__global__ __launch_bounds__(256) void k_miss_f(const int64_t* __restrict__ smp_off,
-                                                const int32_t* __restrict__ smp_cnt,
-                                                const int32_t* __restrict__ lj,
-                                                const uint32_t* __restrict__ lk,
-                                                const int64_t* __restrict__ snp_off,
-                                                const int32_t* __restrict__ snp_list, int64_t n,
-                                                double* __restrict__ E, int64_t lde) {
-    __shared__ unsigned long long acc[F_CHUNK];
-    const int64_t k = blockIdx.x;
-    const int64_t c0 = (int64_t)blockIdx.y * F_CHUNK;
-    const int64_t c1 = c0 + F_CHUNK < n ? c0 + F_CHUNK : n;
-    for (int t = threadIdx.x; t < F_CHUNK; t += 256) acc[t] = 0ull;
-    __syncthreads();
-    const int64_t b = smp_off[k], cnt = smp_cnt[k];
-    for (int64_t p = b + threadIdx.x; p < b + cnt; p += 256) {
-        const int32_t j = lj[p];
-        const uint32_t K = lk[p];
-        const unsigned long long k2 = ((unsigned long long)K * K) >> 6;
-        int64_t lo = snp_off[j], hi = snp_off[j + 1];
-        // first entry >= c0
-        int64_t a = lo, z = hi;
-        while (a < z) {
-            const int64_t mid = (a + z) >> 1;
-            if (snp_list[mid] < c0) a = mid + 1; else z = mid;
-        }
-        for (int64_t q = a; q < hi; ++q) {
-            const int32_t i = snp_list[q];
-            if (i >= c1) break;
-            atomicAdd(&acc[i - c0], k2);
-        }
-    }
-    __syncthreads();
-    const double scale = 0.5 / 4503599627370496.0;  // (1/2) * 2^-52
-    for (int64_t i = c0 + threadIdx.x; i < c1; i += 256)
-        E[k * lde + i] = (double)acc[i - c0] * scale;
-}
-
-// E(k, i) += sum_{j miss k} mu_j X_ij for the 256 samples i of this block:
-// a lane owns 4 samples (one BED byte per SNP row); four list entries are
-// consumed per step as one dword of 4 SNP bytes, turned into per-sample
-// dosage bytes and dotted (v_dot4_u32_u8) with the four byte chunks of K.
+// E(k, i) = sum_{j miss k} mu_j X_ij + F(k,i)/2 for the 1024 samples of a
+// block (each lane owns 16 consecutive samples = one dword of a BED row).
+// Four list entries (SNPs j0..j3 of miss(k)) are consumed per step: the four
+// row dwords are byte-transposed with v_perm so that, for each byte q,
+// W_q holds samples 4q..4q+3 of all four SNPs; the dosage bytes are then
+// dotted (v_dot4_u32_u8) with the four byte chunks of K_j — exact integers.
+// F(k,i) = sum over SNPs missing in both k and i of mu_j^2: such calls are
+// rare (the missing rate squared), so they take a per-lane slow path that
+// adds (K_j^2 >> 6) into lane-private u64 LDS slots.
+// Every element E(k, i < n) is written (this initialises E).
 __global__ __launch_bounds__(256) void k_miss_cross(const uint8_t* __restrict__ bed,
                                                     int64_t pitch, int64_t n,
                                                     const int64_t* __restrict__ smp_off,
                                                     const int32_t* __restrict__ lj,
                                                     const uint32_t* __restrict__ lk, int kw,
                                                     double* __restrict__ E, int64_t lde) {
+    __shared__ unsigned long long fl[4 * 16 * 64];  // [wave][sample][lane], 32 KB
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int64_t I0 = (int64_t)blockIdx.x * TILE;
-    const int64_t byte = I0 / 4 + lane;
-    const bool inrow = byte < pitch;
+    const int64_t I0 = (int64_t)blockIdx.y * 1024;
+    const int64_t dw = I0 / 16 + lane;  // dword of each SNP row owned by this lane
+    const bool inrow = dw * 4 < pitch;
+    unsigned long long* myf = fl + wave * 16 * 64 + lane;
     const int per_wave = kw / 4;
     for (int kq = 0; kq < per_wave; ++kq) {
-        const int64_t k = (int64_t)blockIdx.y * kw + wave * per_wave + kq;
+        const int64_t k = (int64_t)blockIdx.x * kw + wave * per_wave + kq;
         if (k >= n) break;
-        uint32_t acc[4][4];
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
+        for (int s = 0; s < 16; ++s) myf[s * 64] = 0ull;
+        uint32_t acc[16][4];
+#pragma unroll
+        for (int s = 0; s < 16; ++s)
 #pragma unroll
             for (int c = 0; c < 4; ++c) acc[s][c] = 0;
         const int64_t b = smp_off[k], e = smp_off[k + 1];
-        for (int64_t p = b; p < e; p += 4) {
-            const int32_t j0 = lj[p], j1 = lj[p + 1], j2 = lj[p + 2], j3 = lj[p + 3];
-            const uint32_t K0 = lk[p], K1 = lk[p + 1], K2 = lk[p + 2], K3 = lk[p + 3];
-            uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-            if (inrow) {
-                c0 = bed[(int64_t)j0 * pitch + byte];
-                c1 = bed[(int64_t)j1 * pitch + byte];
-                c2 = bed[(int64_t)j2 * pitch + byte];
-                c3 = bed[(int64_t)j3 * pitch + byte];
-            }
-            const uint32_t D = dosage_swar(c0 | (c1 << 8) | (c2 << 16) | (c3 << 24));
+        // lists are padded to a multiple of 8: two groups of four per step,
+        // all eight row loads issued before either group is consumed
+        for (int64_t p = b; p < e; p += 8) {
+            uint32_t cw[2][4], Kv[2][4];
+#pragma unroll
+            for (int g = 0; g < 2; ++g)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int32_t j = lj[p + 4 * g + t];
+                    Kv[g][t] = lk[p + 4 * g + t];
+                    cw[g][t] = inrow ? reinterpret_cast<const uint32_t*>(bed + (int64_t)j * pitch)[dw] : 0u;
+                }
+#pragma unroll
+            for (int g = 0; g < 2; ++g) {
+            const uint32_t c0 = cw[g][0], c1 = cw[g][1], c2 = cw[g][2], c3 = cw[g][3];
+            const uint32_t K0 = Kv[g][0], K1 = Kv[g][1], K2 = Kv[g][2], K3 = Kv[g][3];
+            // 4x4 byte transpose: W[q] byte t = byte q of c_t
+            const uint32_t ta = __builtin_amdgcn_perm(c1, c0, 0x05010400u);
+            const uint32_t tb = __builtin_amdgcn_perm(c1, c0, 0x07030602u);
+            const uint32_t tc = __builtin_amdgcn_perm(c3, c2, 0x05010400u);
+            const uint32_t td = __builtin_amdgcn_perm(c3, c2, 0x07030602u);
+            uint32_t W[4];
+            W[0] = __builtin_amdgcn_perm(tc, ta, 0x05040100u);
+            W[1] = __builtin_amdgcn_perm(tc, ta, 0x07060302u);
+            W[2] = __builtin_amdgcn_perm(td, tb, 0x05040100u);
+            W[3] = __builtin_amdgcn_perm(td, tb, 0x07060302u);
             uint32_t kc[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c)
                 kc[c] = ((K0 >> (8 * c)) & 0xffu) | (((K1 >> (8 * c)) & 0xffu) << 8) |
                         (((K2 >> (8 * c)) & 0xffu) << 16) | (((K3 >> (8 * c)) & 0xffu) << 24);
+            uint32_t mf[4];
+            uint32_t anym = 0;
 #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const uint32_t A = (D >> (2 * s)) & 0x03030303u;
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t D = dosage_swar(W[q]);
+                mf[q] = W[q] & ~(W[q] >> 1) & 0x55555555u;  // code 01
+                anym |= mf[q];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) acc[s][c] = __builtin_amdgcn_udot4(A, kc[c], acc[s][c], false);
+                for (int s = 0; s < 4; ++s) {
+                    const uint32_t A = (D >> (2 * s)) & 0x03030303u;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        acc[4 * q + s][c] = __builtin_amdgcn_udot4(A, kc[c], acc[4 * q + s][c], false);
+                }
+            }
+            if (anym) {
+                const unsigned long long q0 = ((unsigned long long)K0 * K0) >> 6;
+                const unsigned long long q1 = ((unsigned long long)K1 * K1) >> 6;
+                const unsigned long long q2 = ((unsigned long long)K2 * K2) >> 6;
+                const unsigned long long q3 = ((unsigned long long)K3 * K3) >> 6;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    uint32_t mm = mf[q];
+                    while (mm) {
+                        const int bit = __ffs(mm) - 1;
+                        mm &= mm - 1;
+                        const int t = bit >> 3, s = (bit >> 1) & 3;
+                        const unsigned long long add = t == 0 ? q0 : t == 1 ? q1 : t == 2 ? q2 : q3;
+                        myf[(4 * q + s) * 64] += add;
+                    }
+                }
+            }
             }
         }
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int64_t i = I0 + 4 * lane + s;
+        for (int s = 0; s < 16; ++s) {
+            const int64_t i = I0 + 16 * lane + s;
             if (i >= n) continue;
             const uint64_t v = (uint64_t)acc[s][0] + ((uint64_t)acc[s][1] << 8) +
                                ((uint64_t)acc[s][2] << 16) + ((uint64_t)acc[s][3] << 24);
-            E[k * lde + i] += (double)v * TWO_M29;
+            E[k * lde + i] = (double)v * TWO_M29 + (double)myf[s * 64] * (0.5 / 4503599627370496.0);
         }
     }
 }
@@ -362,12 +374,14 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
                                               int64_t lde, double* __restrict__ G64,
                                               float* __restrict__ G32, int32_t* __restrict__ Gi,
                                               int64_t ld) {
-    // One LDS block: A and B panels in the main loop (40 KB); four 32x33
-    // FP64 slabs plus four 32x33 int32 slabs (50.7 KB) in the epilogue.
-    constexpr int LDS_WORDS = 4 * 32 * 33 * 3;
-    static_assert(LDS_WORDS >= 2 * TILE * LDS_PITCH, "panels exceed LDS block");
+    // One LDS block: two buffers of A and B panels in the main loop (80 KB);
+    // four 32x33 FP64 slabs plus four 32x33 int32 slabs (50.7 KB) in the
+    // epilogue.
+    constexpr int PANELS_WORDS = 2 * TILE * LDS_PITCH;  // A and B of one buffer
+    constexpr int LDS_WORDS = 2 * PANELS_WORDS;
+    static_assert(LDS_WORDS >= 4 * 32 * 33 * 3, "epilogue slabs exceed LDS block");
     __shared__ __attribute__((aligned(16))) uint32_t lds[LDS_WORDS];
-    uint32_t* ldsA = lds;
+    uint32_t* ldsA = lds;                      // buffer b: lds + b * PANELS_WORDS
     uint32_t* ldsB = lds + TILE * LDS_PITCH;
 
     // XCD-aware order: hardware deals workgroups round-robin over 8 XCDs;
@@ -390,43 +404,38 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = (v16i){0};
 
-    const uint4* gA = reinterpret_cast<const uint4*>(X2 + (ti * TILE + tid) * 16);
-    const uint4* gB = reinterpret_cast<const uint4*>(X2 + (tj * TILE + tid) * 16);
-    const int64_t stage_stride = n_pad * 4;  // in uint4
-    uint4 pa[4], pb[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        pa[e] = gA[e];
-        pb[e] = gB[e];
+    const u32x4* gA = reinterpret_cast<const u32x4*>(X2 + (ti * TILE + tid) * 16);
+    const u32x4* gB = reinterpret_cast<const u32x4*>(X2 + (tj * TILE + tid) * 16);
+    const int64_t stage_stride = n_pad * 4;  // in 16-byte units
+    // Double-buffered LDS panels: stage s is computed from buffer s&1 while
+    // stage s+1 (already in registers) is written to the other buffer and
+    // stage s+2 is fetched from global memory — one barrier per stage.
+    u32x4 pa0 = gA[0], pa1 = gA[1], pa2 = gA[2], pa3 = gA[3];
+    u32x4 pb0 = gB[0], pb1 = gB[1], pb2 = gB[2], pb3 = gB[3];
+    {
+        u32x4* la = reinterpret_cast<u32x4*>(ldsA + tid * LDS_PITCH);
+        u32x4* lb = reinterpret_cast<u32x4*>(ldsB + tid * LDS_PITCH);
+        la[0] = pa0; la[1] = pa1; la[2] = pa2; la[3] = pa3;
+        lb[0] = pb0; lb[1] = pb1; lb[2] = pb2; lb[3] = pb3;
     }
+    if (n_stages > 1) {
+        pa0 = gA[stage_stride]; pa1 = gA[stage_stride + 1]; pa2 = gA[stage_stride + 2]; pa3 = gA[stage_stride + 3];
+        pb0 = gB[stage_stride]; pb1 = gB[stage_stride + 1]; pb2 = gB[stage_stride + 2]; pb3 = gB[stage_stride + 3];
+    }
+    __syncthreads();
     for (int64_t s = 0; s < n_stages; ++s) {
-        __syncthreads();
-        uint4* la = reinterpret_cast<uint4*>(ldsA + tid * LDS_PITCH);
-        uint4* lb = reinterpret_cast<uint4*>(ldsB + tid * LDS_PITCH);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            la[e] = pa[e];
-            lb[e] = pb[e];
-        }
-        __syncthreads();
-        if (s + 1 < n_stages) {
-            const int64_t o = (s + 1) * stage_stride;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                pa[e] = gA[o + e];
-                pb[e] = gB[o + e];
-            }
-        }
-        uint4 wa[4][2], wb[4][2];
+        const uint32_t* curA = ldsA + (s & 1) * PANELS_WORDS;
+        const uint32_t* curB = ldsB + (s & 1) * PANELS_WORDS;
+        u32x4 wa[4][2], wb[4][2];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-            const uint4* p = reinterpret_cast<const uint4*>(ldsA + (wr * 128 + a * 32 + r) * LDS_PITCH + 8 * h);
+            const u32x4* p = reinterpret_cast<const u32x4*>(curA + (wr * 128 + a * 32 + r) * LDS_PITCH + 8 * h);
             wa[a][0] = p[0];
             wa[a][1] = p[1];
         }
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const uint4* p = reinterpret_cast<const uint4*>(ldsB + (wc * 128 + b * 32 + r) * LDS_PITCH + 8 * h);
+            const u32x4* p = reinterpret_cast<const u32x4*>(curB + (wc * 128 + b * 32 + r) * LDS_PITCH + 8 * h);
             wb[b][0] = p[0];
             wb[b][1] = p[1];
         }
@@ -435,14 +444,14 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
             v4i fa[4], fb[4];
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
-                const uint4 v = wa[a][kk >> 2];
+                const u32x4 v = wa[a][kk >> 2];
                 const uint32_t w = (kk & 3) == 0 ? v.x : (kk & 3) == 1 ? v.y : (kk & 3) == 2 ? v.z : v.w;
                 fa[a] = (v4i){(int)(w & 0x03030303u), (int)((w >> 2) & 0x03030303u),
                               (int)((w >> 4) & 0x03030303u), (int)((w >> 6) & 0x03030303u)};
             }
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const uint4 v = wb[b][kk >> 2];
+                const u32x4 v = wb[b][kk >> 2];
                 const uint32_t w = (kk & 3) == 0 ? v.x : (kk & 3) == 1 ? v.y : (kk & 3) == 2 ? v.z : v.w;
                 fb[b] = (v4i){(int)(w & 0x03030303u), (int)((w >> 2) & 0x03030303u),
                               (int)((w >> 4) & 0x03030303u), (int)((w >> 6) & 0x03030303u)};
@@ -452,7 +461,21 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
 #pragma unroll
                 for (int b = 0; b < 4; ++b)
                     acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
+            // keep each k-step's decode next to its MFMAs (bounds live VGPRs)
+            __builtin_amdgcn_sched_barrier(0);
         }
+        if (s + 1 < n_stages) {
+            u32x4* la = reinterpret_cast<u32x4*>(ldsA + ((s + 1) & 1) * PANELS_WORDS + tid * LDS_PITCH);
+            u32x4* lb = reinterpret_cast<u32x4*>(ldsB + ((s + 1) & 1) * PANELS_WORDS + tid * LDS_PITCH);
+            la[0] = pa0; la[1] = pa1; la[2] = pa2; la[3] = pa3;
+            lb[0] = pb0; lb[1] = pb1; lb[2] = pb2; lb[3] = pb3;
+            if (s + 2 < n_stages) {
+                const int64_t o = (s + 2) * stage_stride;
+                pa0 = gA[o]; pa1 = gA[o + 1]; pa2 = gA[o + 2]; pa3 = gA[o + 3];
+                pb0 = gB[o]; pb1 = gB[o + 1]; pb2 = gB[o + 2]; pb3 = gB[o + 3];
+            }
+        }
+        __syncthreads();
     }
 
     // Epilogue, one 32x32 block at a time through wave-private LDS slabs
@@ -638,11 +661,11 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
         CB_HIP(hipMemcpyAsync(hcnt.data(), smp_cnt.p, n * 4, hipMemcpyDeviceToHost, st));
         CB_HIP(hipStreamSynchronize(st));
         std::vector<int64_t> smp_off(n + 1, 0);
-        for (int64_t i = 0; i < n; ++i) smp_off[i + 1] = smp_off[i] + round_up(hcnt[i], 4);
+        for (int64_t i = 0; i < n; ++i) smp_off[i + 1] = smp_off[i] + round_up(hcnt[i], 8);
         DevBuf<int64_t> d_smp_off;
         d_smp_off.alloc(n + 1);
         CB_HIP(hipMemcpyAsync(d_smp_off.p, smp_off.data(), (n + 1) * 8, hipMemcpyHostToDevice, st));
-        const int64_t nlist = std::max<int64_t>(4, smp_off[n]);
+        const int64_t nlist = std::max<int64_t>(8, smp_off[n]);
         DevBuf<int32_t> lj, cursor;
         DevBuf<uint32_t> lk;
         DevBuf<unsigned long long> sq;
@@ -669,13 +692,9 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
         CB_HIP(hipMemcpyAsync(u.p, hu.data(), n * 8, hipMemcpyHostToDevice, st));
 
         E.alloc((size_t)n * n);
-        timed(ctx, "k_miss_f", [&] {
-            k_miss_f<<<dim3((unsigned)n, (unsigned)ceil_div(n, F_CHUNK)), dim3(256), 0, st>>>(
-                d_smp_off.p, smp_cnt.p, lj.p, lk.p, d_snp_off.p, snp_list.p, n, E.p, n);
-        });
         const int kw = 64;
         timed(ctx, "k_miss_cross", [&] {
-            k_miss_cross<<<dim3((unsigned)(n_pad / TILE), (unsigned)ceil_div(n, kw)), dim3(256), 0, st>>>(
+            k_miss_cross<<<dim3((unsigned)ceil_div(n, kw), (unsigned)ceil_div(n, 1024)), dim3(256), 0, st>>>(
                 g->bed.p, g->pitch, n, d_smp_off.p, lj.p, lk.p, kw, E.p, n);
         });
         CB_HIP(hipStreamSynchronize(st));  // list buffers die at scope end

@@ -1,0 +1,1343 @@
+// Mixed-model equations on gfx950: the MME operator, block PCG in FP64 and
+// in FP32 with FP64 iterative refinement, and the AI-REML / EM-REML driver
+// with Hutchinson trace probes.
+//
+// Reference counterparts (paths relative to the reference tree):
+//   MMELHSBuilder (LHS = [X'X X'Z; Z'X Z'Z + lambda Q], RHS = W'y)
+//                                   src/core/mme_builder.cpp:72-423
+//   PCGSolver (relative stop, block-Jacobi)  src/core/pcg_solver.h:35-128,436-536
+//   MatrixFreePCG (absolute stop)    src/core/matrix_free_pcg.h:20-66
+//   AI_REML data-driven AI step      src/core/vce.cpp:2414-2857
+//   AI_REML EM + Hutchinson step     src/core/vce.cpp:1490-1574
+//   estimate_trace_Qinv_Cuu          src/core/vce.cpp:1266-1315
+//   project_spd_2x2                  src/core/vce.cpp:267-283
+//
+// Layout. dim = p + q unknowns (fixed effects first). Right-hand-side blocks
+// are column-major [ldv x r] with ldv = dim rounded up to 64. The
+// coefficient matrix C(lambda) = W'W + lambda * blockdiag(0, Q) is applied
+// as a CSR SpMM of W'W (assembled once on the host, like the reference's
+// LHS with its lambda update map) plus lambda * Q X_u, Q dense (G^-1) or
+// CSR (A^-1). Q and W'W are held in FP64 and FP32.
+//
+// Kernels and their bounds (per operator application on an r-column block):
+//   k_dense_symm  Q X_u, split-K register-tiled; q^2*s bytes of Q read once
+//                 (s = 4 or 8), 2 q^2 r flops: HBM-bound for small r, vector
+//                 ALU bound at r >~ 20 (FP32, no packed math)
+//   k_gemv        r <= 2 dense product, HBM-bound
+//   k_csr_spmm    W'W and CSR Q, HBM/L2-bound (8-12 bytes per nonzero)
+//   k_col_dots    per-column FP64 dot products, two-stage deterministic
+//   k_pcg_*       fused CG updates, HBM-bound
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "cb_internal.h"
+
+using namespace cbamd;
+
+namespace {
+
+constexpr int DOT_CHUNKS = 128;   // row chunks of the two-stage column dot
+constexpr int SYMM_ROWS = 128;    // rows per k_dense_symm tile
+constexpr int SYMM_KC = 32;       // k-chunk staged in LDS
+constexpr int SYMM_MAXC = 80;     // max columns per k_dense_symm launch
+
+// ---------------------------------------------------------------------------
+// Generic device kernels
+
+// Y[row, c] = alpha * sum_k val(row,k) X[col(k), c] (+ beta_acc * Y[row, c])
+// thread per (row, c) for rows in [row0, row1)
+template <class TM, class TV>
+__global__ void k_csr_spmm(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                           const TM* __restrict__ val, int64_t row0, int64_t row1,
+                           int64_t col_off, const TV* __restrict__ X, int64_t ldx,
+                           TV* __restrict__ Y, int64_t ldy, int64_t y_off, TV alpha, int accumulate) {
+    const int64_t row = row0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    if (row >= row1) return;
+    const TV* x = X + c * ldx + col_off;
+    TV acc = 0;
+    for (int64_t k = ptr[row]; k < ptr[row + 1]; ++k) acc += (TV)val[k] * x[col[k]];
+    TV* y = Y + c * ldy + y_off + row;
+    *y = accumulate ? *y + alpha * acc : alpha * acc;
+}
+
+// wave per (row, c): for the few long rows (fixed effects)
+template <class TM, class TV>
+__global__ void k_csr_spmm_long(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                const TM* __restrict__ val, int64_t row0, int64_t row1,
+                                const TV* __restrict__ X, int64_t ldx, TV* __restrict__ Y,
+                                int64_t ldy) {
+    const int lane = threadIdx.x & 63;
+    const int64_t row = row0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t c = blockIdx.y;
+    if (row >= row1) return;
+    const TV* x = X + c * ldx;
+    double acc = 0;
+    for (int64_t k = ptr[row] + lane; k < ptr[row + 1]; k += 64) acc += (double)val[k] * (double)x[col[k]];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+    if (lane == 0) Y[c * ldy + row] = (TV)acc;
+}
+
+// Split-K partial of Q X for a row tile: part[split][c][row] = sum_{k in split} Q[row,k] X[k,c]
+// Q row-major q x q (leading dimension ldq); X column-major (ldx), columns [0, r).
+// 256 threads: 16 row groups x 8 rows, 16 column groups x ceil(r/16) columns.
+template <class T>
+__global__ __launch_bounds__(256) void k_dense_symm(const T* __restrict__ Q, int64_t ldq, int64_t q,
+                                                    const T* __restrict__ X, int64_t ldx, int r,
+                                                    int64_t kspan, T* __restrict__ part,
+                                                    int64_t ldp) {
+    __shared__ T Qs[SYMM_ROWS][SYMM_KC + 1];
+    __shared__ T Xs[SYMM_KC][SYMM_MAXC];
+    const int tid = threadIdx.x;
+    const int ty = tid >> 4, tx = tid & 15;
+    const int64_t row0 = (int64_t)blockIdx.x * SYMM_ROWS;
+    const int64_t k0 = (int64_t)blockIdx.y * kspan;
+    const int64_t k1 = min(q, k0 + kspan);
+    constexpr int RPT = SYMM_ROWS / 16;          // rows per thread
+    constexpr int CPT = SYMM_MAXC / 16;          // max columns per thread
+    T acc[RPT][CPT];
+#pragma unroll
+    for (int a = 0; a < RPT; ++a)
+#pragma unroll
+        for (int b = 0; b < CPT; ++b) acc[a][b] = 0;
+    for (int64_t kc = k0; kc < k1; kc += SYMM_KC) {
+        __syncthreads();
+        // Q tile: 128 rows x 32 k (coalesced along k)
+        for (int e = tid; e < SYMM_ROWS * SYMM_KC; e += 256) {
+            const int rr = e / SYMM_KC, kk = e % SYMM_KC;
+            const int64_t gr = row0 + rr, gk = kc + kk;
+            Qs[rr][kk] = (gr < q && gk < k1) ? Q[gr * ldq + gk] : (T)0;
+        }
+        // X chunk: 32 k x r
+        for (int e = tid; e < SYMM_KC * SYMM_MAXC; e += 256) {
+            const int kk = e % SYMM_KC, cc = e / SYMM_KC;
+            const int64_t gk = kc + kk;
+            Xs[kk][cc] = (cc < r && gk < k1) ? X[cc * ldx + gk] : (T)0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < SYMM_KC; ++kk) {
+            T qa[RPT], xb[CPT];
+#pragma unroll
+            for (int a = 0; a < RPT; ++a) qa[a] = Qs[ty * RPT + a][kk];
+#pragma unroll
+            for (int b = 0; b < CPT; ++b) xb[b] = Xs[kk][tx + 16 * b];
+#pragma unroll
+            for (int a = 0; a < RPT; ++a)
+#pragma unroll
+                for (int b = 0; b < CPT; ++b) acc[a][b] += qa[a] * xb[b];
+        }
+    }
+    T* pp = part + (int64_t)blockIdx.y * ldp * r;
+#pragma unroll
+    for (int a = 0; a < RPT; ++a) {
+        const int64_t gr = row0 + ty * RPT + a;
+        if (gr >= q) continue;
+#pragma unroll
+        for (int b = 0; b < CPT; ++b) {
+            const int cc = tx + 16 * b;
+            if (cc < r) pp[(int64_t)cc * ldp + gr] = acc[a][b];
+        }
+    }
+}
+
+// Y[row, c] += alpha * sum_split part[split][c][row]   (fixed split order)
+template <class T>
+__global__ void k_dense_reduce(const T* __restrict__ part, int64_t ldp, int nsplit, int64_t q, int r,
+                               T* __restrict__ Y, int64_t ldy, int64_t y_off, T alpha) {
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (row >= q) return;
+    T s = 0;
+    for (int sp = 0; sp < nsplit; ++sp) s += part[((int64_t)sp * r + c) * ldp + row];
+    Y[(int64_t)c * ldy + y_off + row] += alpha * s;
+}
+
+// y_c[row] += alpha * Q[row,:] x_c for r <= 2 columns; wave per row.
+template <class T>
+__global__ __launch_bounds__(256) void k_gemv(const T* __restrict__ Q, int64_t ldq, int64_t q,
+                                              const T* __restrict__ X, int64_t ldx, int r,
+                                              T* __restrict__ Y, int64_t ldy, int64_t y_off, T alpha) {
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= q) return;
+    const T* qr = Q + row * ldq;
+    double a0 = 0, a1 = 0;
+    for (int64_t k = lane; k < q; k += 64) {
+        const double v = (double)qr[k];
+        a0 += v * (double)X[k];
+        if (r > 1) a1 += v * (double)X[ldx + k];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a0 += __shfl_down(a0, off);
+        a1 += __shfl_down(a1, off);
+    }
+    if (lane == 0) {
+        Y[y_off + row] += alpha * (T)a0;
+        if (r > 1) Y[ldy + y_off + row] += alpha * (T)a1;
+    }
+}
+
+// part[c][chunk] = sum over the chunk's rows of A[row,c] * B[row,c] (FP64)
+template <class T>
+__global__ __launch_bounds__(256) void k_col_dots(const T* __restrict__ A, const T* __restrict__ B,
+                                                  int64_t ld, int64_t rows,
+                                                  double* __restrict__ part) {
+    __shared__ double red[256];
+    const int c = blockIdx.y;
+    const int64_t per = (rows + DOT_CHUNKS - 1) / DOT_CHUNKS;
+    const int64_t r0 = (int64_t)blockIdx.x * per, r1 = min(rows, r0 + per);
+    double s = 0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256)
+        s += (double)A[(int64_t)c * ld + i] * (double)B[(int64_t)c * ld + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)c * DOT_CHUNKS + blockIdx.x] = red[0];
+}
+
+__global__ void k_dots_finish(const double* __restrict__ part, int r, double* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= r) return;
+    double s = 0;
+    for (int k = 0; k < DOT_CHUNKS; ++k) s += part[(int64_t)c * DOT_CHUNKS + k];
+    out[c] = s;
+}
+
+// Jacobi part of the preconditioner: z = r * dinv for rows >= p0
+template <class T>
+__global__ void k_precond_diag(const T* __restrict__ R, T* __restrict__ Z, const double* __restrict__ dinv,
+                               int64_t p0, int64_t rows, int64_t ld) {
+    const int64_t i = p0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= rows) return;
+    Z[(int64_t)c * ld + i] = (T)((double)R[(int64_t)c * ld + i] * dinv[i]);
+}
+
+// block part: z[0:p, c] = Binv r[0:p, c]  (Binv p x p row-major, FP64)
+template <class T>
+__global__ void k_precond_block(const T* __restrict__ R, T* __restrict__ Z, const double* __restrict__ Binv,
+                                int64_t p, int64_t ld) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= p) return;
+    double s = 0;
+    for (int64_t k = 0; k < p; ++k) s += Binv[i * p + k] * (double)R[(int64_t)c * ld + k];
+    Z[(int64_t)c * ld + i] = (T)s;
+}
+
+// CG scalar step per column (runs on the device so the host need not sync
+// every iteration). st layout per column c: [rho, pq, rho_new, rr, bb, active, alpha, beta]
+struct ColState {
+    double rho, pq, rho_new, rr, bb, alpha, beta;
+    int active, restarted;
+};
+
+__global__ void k_cg_alpha(ColState* __restrict__ st, const double* __restrict__ pq, int r) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= r) return;
+    ColState s = st[c];
+    s.pq = pq[c];
+    if (s.active && fabs(s.pq) < 1e-300) s.active = 0;  // breakdown: freeze the column
+    s.alpha = s.active ? s.rho / s.pq : 0.0;
+    st[c] = s;
+}
+
+// x += alpha p ; r -= alpha q   (inactive columns untouched)
+template <class TX, class T>
+__global__ void k_cg_xr(TX* __restrict__ X, T* __restrict__ R, const T* __restrict__ P,
+                        const T* __restrict__ Qp, const ColState* __restrict__ st, int64_t rows,
+                        int64_t ld) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= rows || !st[c].active) return;
+    const double a = st[c].alpha;
+    const int64_t o = (int64_t)c * ld + i;
+    X[o] = (TX)((double)X[o] + a * (double)P[o]);
+    R[o] = (T)((double)R[o] - a * (double)Qp[o]);
+}
+
+// after dots: rho_new = r.z, rr = r.r ; decide convergence ; beta
+__global__ void k_cg_beta(ColState* __restrict__ st, const double* __restrict__ rz,
+                          const double* __restrict__ rr, int r, double tol2, int absolute) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= r) return;
+    ColState s = st[c];
+    if (!s.active) return;
+    s.rho_new = rz[c];
+    s.rr = rr[c];
+    const double thr = absolute ? tol2 : tol2 * s.bb;
+    if (!(s.rr >= thr)) s.active = 0;  // converged (or NaN)
+    s.beta = s.rho != 0.0 ? s.rho_new / s.rho : 0.0;
+    s.rho = s.rho_new;
+    st[c] = s;
+}
+
+template <class T>
+__global__ void k_cg_p(T* __restrict__ P, const T* __restrict__ Z, const ColState* __restrict__ st,
+                       int64_t rows, int64_t ld) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= rows || !st[c].active) return;
+    const int64_t o = (int64_t)c * ld + i;
+    P[o] = (T)((double)Z[o] + st[c].beta * (double)P[o]);
+}
+
+// R = B - A   (column-major, same ld)
+template <class TB, class T>
+__global__ void k_sub(const TB* __restrict__ B, const T* __restrict__ A, T* __restrict__ R, int64_t rows,
+                      int64_t ld) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= rows) return;
+    const int64_t o = (int64_t)c * ld + i;
+    R[o] = (T)((double)B[o] - (double)A[o]);
+}
+
+// out32 = (float)(in64 * s_c)
+__global__ void k_scale_to_f32(const double* __restrict__ in, float* __restrict__ out,
+                               const double* __restrict__ scale, int64_t rows, int64_t ld) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= rows) return;
+    const int64_t o = (int64_t)c * ld + i;
+    out[o] = (float)(in[o] * scale[c]);
+}
+
+// x64 += (double)d32 * s_c
+__global__ void k_axpy_from_f32(double* __restrict__ x, const float* __restrict__ d,
+                                const double* __restrict__ s, int64_t rows, int64_t ld) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= rows) return;
+    const int64_t o = (int64_t)c * ld + i;
+    x[o] += (double)d[o] * s[c];
+}
+
+template <class TD, class TS>
+__global__ void k_convert(const TS* __restrict__ s, TD* __restrict__ d, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (TD)s[i];
+}
+
+// Rademacher probes: z[c][j] = +-1 from splitmix64(seed + golden*(c*q + j + 1));
+// the sign is the top bit of the mixed word. Written at rows [off, off+q).
+__device__ __host__ inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__global__ void k_probes(double* __restrict__ Z, int64_t ld, int64_t off, int64_t q, int k, uint64_t seed,
+                         int exact, int64_t first) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (j >= q) return;
+    double v;
+    if (exact) {
+        v = (first + c == j) ? 1.0 : 0.0;
+    } else {
+        const uint64_t h = splitmix64(seed + 0x9E3779B97F4A7C15ull * ((uint64_t)(first + c) * (uint64_t)q + (uint64_t)j + 1ull));
+        v = (h >> 63) ? -1.0 : 1.0;
+    }
+    Z[(int64_t)c * ld + off + j] = v;
+}
+
+// t = W s (records): t[r] = sum x_val * s[x_col] + (aid>=0 ? s[p+aid] : 0)
+__global__ void k_w_apply(const int64_t* __restrict__ xp, const int32_t* __restrict__ xc,
+                          const double* __restrict__ xv, const int32_t* __restrict__ aid, int64_t n,
+                          int64_t p, const double* __restrict__ S, int64_t lds, double* __restrict__ T,
+                          int64_t ldt) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (r >= n) return;
+    const double* s = S + (int64_t)c * lds;
+    double t = 0;
+    for (int64_t k = xp[r]; k < xp[r + 1]; ++k) t += xv[k] * s[xc[k]];
+    if (aid[r] >= 0) t += s[p + aid[r]];
+    T[(int64_t)c * ldt + r] = t;
+}
+
+// out = W' v via the transposed design (CSR dim x n)
+__global__ void k_wt_apply(const int64_t* __restrict__ tp, const int32_t* __restrict__ tc,
+                           const double* __restrict__ tv, int64_t dim, const double* __restrict__ V,
+                           int64_t ldv, double* __restrict__ O, int64_t ldo) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= dim) return;
+    const double* v = V + (int64_t)c * ldv;
+    double s = 0;
+    for (int64_t k = tp[i]; k < tp[i + 1]; ++k) s += tv[k] * v[tc[k]];
+    O[(int64_t)c * ldo + i] = s;
+}
+
+// AI phase vectors: vu[r] = u_hat[aid[r]] / var_u ; ve[r] = e[r] / var_e
+__global__ void k_ai_vectors(const int32_t* __restrict__ aid, int64_t n, int64_t p,
+                             const double* __restrict__ sol, const double* __restrict__ e,
+                             double inv_vu, double inv_ve, double* __restrict__ V, int64_t ldv) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    V[r] = aid[r] >= 0 ? sol[p + aid[r]] * inv_vu : 0.0;
+    V[ldv + r] = e[r] * inv_ve;
+}
+
+__global__ void k_axpby(const double* __restrict__ a, double alpha, const double* __restrict__ b,
+                        double beta, double* __restrict__ out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = alpha * a[i] + beta * b[i];
+}
+
+inline unsigned blocks(int64_t n, int t = 256) { return (unsigned)std::max<int64_t>(1, ceil_div(n, t)); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct cb_mme {
+    cb_context* ctx = nullptr;
+    int64_t n = 0, p = 0, q = 0, dim = 0, ldv = 0;
+    // design (device): W rows (records), W^T rows (unknowns)
+    DevBuf<int64_t> xp, tp;
+    DevBuf<int32_t> xc, tc, aid;
+    DevBuf<double> xv, tv, y;
+    // W'W CSR
+    DevBuf<int64_t> wp;
+    DevBuf<int32_t> wc;
+    DevBuf<double> wv64;
+    DevBuf<float> wv32;
+    int64_t long_rows = 0;  // rows [0, long_rows) use the wave-per-row SpMM
+    std::vector<double> wtw_diag;  // diagonal of W'W
+    std::vector<double> xtx;       // p x p block (constant in lambda)
+    std::vector<double> rhs;       // W'y
+    double yty = 0.0;
+    // Q
+    int qkind = -1;  // 0 dense, 1 CSR
+    DevBuf<double> Qd64;
+    DevBuf<float> Qd32;
+    DevBuf<int64_t> qp;
+    DevBuf<int32_t> qc;
+    DevBuf<double> qv64;
+    DevBuf<float> qv32;
+    std::vector<double> q_diag;
+    // preconditioner
+    DevBuf<double> Binv;   // p x p
+    DevBuf<double> dinv;   // dim
+    double dinv_lambda = std::nan("");
+    // scratch for dense products
+    DevBuf<double> part64;
+    DevBuf<float> part32;
+};
+
+namespace {
+
+template <class T>
+T* qdense(cb_mme* M);
+template <>
+double* qdense<double>(cb_mme* M) { return M->Qd64.p; }
+template <>
+float* qdense<float>(cb_mme* M) { return M->Qd32.p; }
+template <class T>
+const T* wvals(cb_mme* M);
+template <>
+const double* wvals<double>(cb_mme* M) { return M->wv64.p; }
+template <>
+const float* wvals<float>(cb_mme* M) { return M->wv32.p; }
+template <class T>
+const T* qvals(cb_mme* M);
+template <>
+const double* qvals<double>(cb_mme* M) { return M->qv64.p; }
+template <>
+const float* qvals<float>(cb_mme* M) { return M->qv32.p; }
+template <class T>
+T* partbuf(cb_mme* M, size_t need);
+template <>
+double* partbuf<double>(cb_mme* M, size_t need) {
+    if (M->part64.n < need) M->part64.alloc(need);
+    return M->part64.p;
+}
+template <>
+float* partbuf<float>(cb_mme* M, size_t need) {
+    if (M->part32.n < need) M->part32.alloc(need);
+    return M->part32.p;
+}
+
+// Y_u (+)= alpha * Q X_u  on an r-column block (dense Q)
+template <class T>
+void dense_q_apply(cb_mme* M, const T* X, int64_t ldx, T* Y, int64_t ldy, int r, T alpha) {
+    cb_context* ctx = M->ctx;
+    hipStream_t st = ctx->stream;
+    const int64_t q = M->q;
+    const T* Q = qdense<T>(M);
+    if (r <= 2) {
+        timed(ctx, sizeof(T) == 8 ? "k_gemv_f64" : "k_gemv_f32", [&] {
+            k_gemv<T><<<blocks(q, 4), 256, 0, st>>>(Q, q, q, X + M->p, ldx, r, Y, ldy, M->p, alpha);
+        });
+        return;
+    }
+    for (int c0 = 0; c0 < r; c0 += SYMM_MAXC) {
+        const int rc = std::min(SYMM_MAXC, r - c0);
+        const int64_t row_tiles = ceil_div(q, SYMM_ROWS);
+        int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(1024, row_tiles)));
+        int64_t kspan = round_up(ceil_div(q, nsplit), SYMM_KC);
+        nsplit = (int)ceil_div(q, kspan);
+        const int64_t ldp = round_up(q, 64);
+        T* part = partbuf<T>(M, (size_t)nsplit * rc * ldp);
+        timed(ctx, sizeof(T) == 8 ? "k_dense_symm_f64" : "k_dense_symm_f32", [&] {
+            k_dense_symm<T><<<dim3((unsigned)row_tiles, (unsigned)nsplit), 256, 0, st>>>(
+                Q, q, q, X + (int64_t)c0 * ldx + M->p, ldx, rc, kspan, part, ldp);
+        });
+        k_dense_reduce<T><<<dim3(blocks(q), (unsigned)rc), 256, 0, st>>>(part, ldp, nsplit, q, rc,
+                                                                       Y + (int64_t)c0 * ldy, ldy, M->p, alpha);
+        CB_HIP(hipGetLastError());
+    }
+}
+
+// Y = C(lambda) X for an r-column block, arithmetic in T.
+template <class T>
+void apply_C(cb_mme* M, double lambda, const T* X, T* Y, int r) {
+    cb_context* ctx = M->ctx;
+    hipStream_t st = ctx->stream;
+    const int64_t ld = M->ldv;
+    // W'W part (overwrites Y)
+    if (M->long_rows > 0)
+        k_csr_spmm_long<T, T><<<dim3(blocks(M->long_rows, 4), (unsigned)r), 256, 0, st>>>(
+            M->wp.p, M->wc.p, wvals<T>(M), 0, M->long_rows, X, ld, Y, ld);
+    if (M->dim > M->long_rows)
+        timed(ctx, sizeof(T) == 8 ? "k_csr_spmm_f64" : "k_csr_spmm_f32", [&] {
+            k_csr_spmm<T, T><<<dim3(blocks(M->dim - M->long_rows), (unsigned)r), 256, 0, st>>>(
+                M->wp.p, M->wc.p, wvals<T>(M), M->long_rows, M->dim, 0, X, ld, Y, ld, 0, (T)1, 0);
+        });
+    CB_HIP(hipGetLastError());
+    // lambda Q part
+    if (M->qkind == 0) {
+        dense_q_apply<T>(M, X, ld, Y, ld, r, (T)lambda);
+    } else {
+        k_csr_spmm<T, T><<<dim3(blocks(M->q), (unsigned)r), 256, 0, st>>>(
+            M->qp.p, M->qc.p, qvals<T>(M), 0, M->q, M->p, X, ld, Y, ld, M->p, (T)lambda, 1);
+        CB_HIP(hipGetLastError());
+    }
+}
+
+// Preconditioner inverse for this lambda (block-Jacobi: Cholesky of the p x p
+// block with the reference's ridge, then 1/diag for the random effects).
+void build_precond(cb_mme* M, double lambda, bool block) {
+    if (M->dinv_lambda == lambda && M->Binv.p) return;
+    const int64_t p = M->p, dim = M->dim;
+    std::vector<double> dinv(dim);
+    for (int64_t i = 0; i < dim; ++i) {
+        double d = M->wtw_diag[i] + (i >= p ? lambda * M->q_diag[i - p] : 0.0);
+        dinv[i] = std::fabs(d) > 1e-12 ? 1.0 / d : 1.0;
+    }
+    std::vector<double> binv((size_t)std::max<int64_t>(1, p * p), 0.0);
+    if (p > 0) {
+        bool ok = block;
+        if (block) {
+            // pcg_solver.h:77-91: ridge = max(1e-8 * max|diag|, 1e-6), then factorise
+            std::vector<double> A = M->xtx;
+            double maxd = 0;
+            for (int64_t i = 0; i < p; ++i) maxd = std::max(maxd, std::fabs(A[i * p + i]));
+            const double ridge = std::max(1e-8 * maxd, 1e-6);
+            for (int64_t i = 0; i < p; ++i) A[i * p + i] += ridge;
+            // Cholesky A = L L^T (row-major, lower)
+            for (int64_t j = 0; j < p && ok; ++j) {
+                double s = A[j * p + j];
+                for (int64_t k = 0; k < j; ++k) s -= A[j * p + k] * A[j * p + k];
+                if (!(s > 0)) { ok = false; break; }
+                const double ljj = std::sqrt(s);
+                A[j * p + j] = ljj;
+                for (int64_t i = j + 1; i < p; ++i) {
+                    double t = A[i * p + j];
+                    for (int64_t k = 0; k < j; ++k) t -= A[i * p + k] * A[j * p + k];
+                    A[i * p + j] = t / ljj;
+                }
+            }
+            if (ok) {
+                // inverse via solves with unit vectors
+                std::vector<double> col(p);
+                for (int64_t c = 0; c < p; ++c) {
+                    std::fill(col.begin(), col.end(), 0.0);
+                    col[c] = 1.0;
+                    for (int64_t i = 0; i < p; ++i) {
+                        double t = col[i];
+                        for (int64_t k = 0; k < i; ++k) t -= A[i * p + k] * col[k];
+                        col[i] = t / A[i * p + i];
+                    }
+                    for (int64_t i = p - 1; i >= 0; --i) {
+                        double t = col[i];
+                        for (int64_t k = i + 1; k < p; ++k) t -= A[k * p + i] * col[k];
+                        col[i] = t / A[i * p + i];
+                    }
+                    for (int64_t i = 0; i < p; ++i) binv[i * p + c] = col[i];
+                }
+            }
+        }
+        if (!ok) {  // diagonal fallback for the fixed effects
+            for (int64_t i = 0; i < p; ++i) {
+                const double d = M->wtw_diag[i];
+                binv[i * p + i] = std::fabs(d) > 1e-14 ? 1.0 / d : 1.0;
+            }
+        }
+    }
+    M->Binv.alloc(binv.size());
+    M->dinv.alloc(dim);
+    CB_HIP(hipMemcpyAsync(M->Binv.p, binv.data(), binv.size() * 8, hipMemcpyHostToDevice, M->ctx->stream));
+    CB_HIP(hipMemcpyAsync(M->dinv.p, dinv.data(), dim * 8, hipMemcpyHostToDevice, M->ctx->stream));
+    CB_HIP(hipStreamSynchronize(M->ctx->stream));
+    M->dinv_lambda = lambda;
+}
+
+template <class T>
+void apply_precond(cb_mme* M, const T* R, T* Z, int r) {
+    hipStream_t st = M->ctx->stream;
+    k_precond_diag<T><<<dim3(blocks(M->dim - M->p), (unsigned)r), 256, 0, st>>>(R, Z, M->dinv.p, M->p, M->dim, M->ldv);
+    if (M->p > 0)
+        k_precond_block<T><<<dim3(blocks(M->p), (unsigned)r), 256, 0, st>>>(R, Z, M->Binv.p, M->p, M->ldv);
+    CB_HIP(hipGetLastError());
+}
+
+template <class T>
+void col_dots(cb_mme* M, const T* A, const T* B, int r, double* part, double* out) {
+    hipStream_t st = M->ctx->stream;
+    k_col_dots<T><<<dim3(DOT_CHUNKS, (unsigned)r), 256, 0, st>>>(A, B, M->ldv, M->dim, part);
+    k_dots_finish<<<blocks(r, 64), 64, 0, st>>>(part, r, out);
+    CB_HIP(hipGetLastError());
+}
+
+struct SolveStats {
+    int64_t iterations = 0;
+    double max_rel = 0.0;
+};
+
+// Block PCG (one CG recurrence per column) on C(lambda) X = B, arithmetic T,
+// solution accumulated in TX. X holds the start (zero or warm). The stop
+// test is per column: ||r|| < tol ||b|| (relative) or ||r|| < tol (absolute);
+// converged columns are frozen. Returns iterations of the slowest column.
+template <class T, class TX>
+int64_t pcg_block(cb_mme* M, double lambda, const T* B, TX* X, int r, double tol, bool absolute,
+                  int64_t max_iter, bool block_precond, std::vector<double>* rel_out, bool x_zero) {
+    cb_context* ctx = M->ctx;
+    hipStream_t st = ctx->stream;
+    const int64_t ld = M->ldv, dim = M->dim;
+    const size_t nv = (size_t)ld * r;
+    build_precond(M, lambda, block_precond);
+    DevBuf<T> R, Z, P, Qp;
+    DevBuf<TX> tmpX;
+    R.alloc(nv); Z.alloc(nv); P.alloc(nv); Qp.alloc(nv);
+    DevBuf<double> part, d1, d2;
+    part.alloc((size_t)DOT_CHUNKS * r);
+    d1.alloc(r); d2.alloc(r);
+    DevBuf<ColState> cs;
+    cs.alloc(r);
+    CB_HIP(hipMemsetAsync(R.p, 0, R.bytes(), st));
+    CB_HIP(hipMemsetAsync(Z.p, 0, Z.bytes(), st));
+    CB_HIP(hipMemsetAsync(P.p, 0, P.bytes(), st));
+    CB_HIP(hipMemsetAsync(Qp.p, 0, Qp.bytes(), st));
+    const dim3 g2(blocks(dim), (unsigned)r);
+    // r = b - C x
+    if (x_zero) {
+        CB_HIP(hipMemcpyAsync(R.p, B, nv * sizeof(T), hipMemcpyDeviceToDevice, st));
+    } else {
+        tmpX.alloc(nv);
+        CB_HIP(hipMemcpyAsync(tmpX.p, X, nv * sizeof(TX), hipMemcpyDeviceToDevice, st));
+        DevBuf<T> xt;
+        xt.alloc(nv);
+        k_convert<T, TX><<<blocks((int64_t)nv), 256, 0, st>>>(tmpX.p, xt.p, (int64_t)nv);
+        apply_C<T>(M, lambda, xt.p, Qp.p, r);
+        k_sub<T, T><<<g2, 256, 0, st>>>(B, Qp.p, R.p, dim, ld);
+        CB_HIP(hipStreamSynchronize(st));
+    }
+    apply_precond<T>(M, R.p, Z.p, r);
+    CB_HIP(hipMemcpyAsync(P.p, Z.p, nv * sizeof(T), hipMemcpyDeviceToDevice, st));
+    col_dots<T>(M, R.p, Z.p, r, part.p, d1.p);
+    col_dots<T>(M, B, B, r, part.p, d2.p);
+    std::vector<double> rz(r), bb(r);
+    CB_HIP(hipMemcpyAsync(rz.data(), d1.p, r * 8, hipMemcpyDeviceToHost, st));
+    CB_HIP(hipMemcpyAsync(bb.data(), d2.p, r * 8, hipMemcpyDeviceToHost, st));
+    CB_HIP(hipStreamSynchronize(st));
+    std::vector<ColState> hs(r);
+    for (int c = 0; c < r; ++c) {
+        hs[c] = ColState{};
+        hs[c].rho = rz[c];
+        hs[c].bb = bb[c] == 0.0 ? 1.0 : bb[c];  // pcg_solver.h:463: bnorm == 0 -> 1
+        hs[c].active = 1;
+    }
+    // initial residual check (a zero RHS column is solved by x = 0)
+    col_dots<T>(M, R.p, R.p, r, part.p, d1.p);
+    std::vector<double> rr0(r);
+    CB_HIP(hipMemcpyAsync(rr0.data(), d1.p, r * 8, hipMemcpyDeviceToHost, st));
+    CB_HIP(hipStreamSynchronize(st));
+    const double tol2 = tol * tol;
+    for (int c = 0; c < r; ++c) {
+        const double thr = absolute ? tol2 : tol2 * hs[c].bb;
+        hs[c].rr = rr0[c];
+        if (!(rr0[c] >= thr)) hs[c].active = 0;
+    }
+    CB_HIP(hipMemcpyAsync(cs.p, hs.data(), r * sizeof(ColState), hipMemcpyHostToDevice, st));
+    int64_t it = 0;
+    const int check_every = 8;
+    bool any_active = std::any_of(hs.begin(), hs.end(), [](const ColState& s) { return s.active; });
+    while (any_active && it < max_iter) {
+        apply_C<T>(M, lambda, P.p, Qp.p, r);
+        col_dots<T>(M, P.p, Qp.p, r, part.p, d1.p);
+        k_cg_alpha<<<blocks(r, 64), 64, 0, st>>>(cs.p, d1.p, r);
+        k_cg_xr<TX, T><<<g2, 256, 0, st>>>(X, R.p, P.p, Qp.p, cs.p, dim, ld);
+        apply_precond<T>(M, R.p, Z.p, r);
+        col_dots<T>(M, R.p, Z.p, r, part.p, d1.p);
+        col_dots<T>(M, R.p, R.p, r, part.p, d2.p);
+        k_cg_beta<<<blocks(r, 64), 64, 0, st>>>(cs.p, d1.p, d2.p, r, tol2, absolute ? 1 : 0);
+        k_cg_p<T><<<g2, 256, 0, st>>>(P.p, Z.p, cs.p, dim, ld);
+        CB_HIP(hipGetLastError());
+        ++it;
+        if (it % check_every == 0 || it == max_iter) {
+            CB_HIP(hipMemcpyAsync(hs.data(), cs.p, r * sizeof(ColState), hipMemcpyDeviceToHost, st));
+            CB_HIP(hipStreamSynchronize(st));
+            any_active = std::any_of(hs.begin(), hs.end(), [](const ColState& s) { return s.active; });
+        }
+    }
+    CB_HIP(hipMemcpyAsync(hs.data(), cs.p, r * sizeof(ColState), hipMemcpyDeviceToHost, st));
+    CB_HIP(hipStreamSynchronize(st));
+    if (rel_out) {
+        rel_out->resize(r);
+        for (int c = 0; c < r; ++c)
+            (*rel_out)[c] = absolute ? std::sqrt(hs[c].rr) : std::sqrt(hs[c].rr / hs[c].bb);
+    }
+    return it;
+}
+
+}  // namespace
+
+// Solve driver shared by cb_mme_solve and the REML loop. B, X device FP64
+// blocks [ldv x r]; X is the output (zero start).
+static SolveStats solve_block(cb_mme* M, double lambda, const cb_solve_options& o, const double* B,
+                              double* X, int r) {
+    cb_context* ctx = M->ctx;
+    hipStream_t st = ctx->stream;
+    const int64_t ld = M->ldv, dim = M->dim;
+    const size_t nv = (size_t)ld * r;
+    const bool absolute = o.stop_rule == CB_STOP_ABSOLUTE;
+    const bool block = o.precond == CB_PRECOND_BLOCK_JACOBI;
+    SolveStats S;
+    CB_HIP(hipMemsetAsync(X, 0, nv * 8, st));
+    std::vector<double> rel;
+    if (o.precision == CB_PREC_FP64) {
+        if (o.batch_rhs || r == 1) {
+            S.iterations = pcg_block<double, double>(M, lambda, B, X, r, o.tol, absolute, o.max_iter, block, &rel, true);
+            for (double v : rel) S.max_rel = std::max(S.max_rel, v);
+        } else {
+            // the plain path: one right-hand side at a time
+            for (int c = 0; c < r; ++c) {
+                const int64_t itc = pcg_block<double, double>(M, lambda, B + (size_t)c * ld, X + (size_t)c * ld, 1,
+                                                              o.tol, absolute, o.max_iter, block, &rel, true);
+                S.iterations += itc;
+                S.max_rel = std::max(S.max_rel, rel[0]);
+            }
+        }
+        if (S.max_rel > o.tol) throw Error(CB_ERR_NOT_CONVERGED, "PCG did not reach the tolerance");
+        return S;
+    }
+    // Mixed: FP32 inner block PCG, FP64 residual and refinement.
+    CB_REQUIRE(M->qkind == 1 || M->Qd32.p, "FP32 copy of Q missing");
+    DevBuf<double> R64, CX, part, nrm, bnorm, scale, inv;
+    DevBuf<float> R32, D32;
+    R64.alloc(nv); CX.alloc(nv); R32.alloc(nv); D32.alloc(nv);
+    part.alloc((size_t)DOT_CHUNKS * r);
+    nrm.alloc(r); bnorm.alloc(r); scale.alloc(r); inv.alloc(r);
+    col_dots<double>(M, B, B, r, part.p, bnorm.p);
+    std::vector<double> hb(r), hr(r), hs(r), hi(r);
+    CB_HIP(hipMemcpyAsync(hb.data(), bnorm.p, r * 8, hipMemcpyDeviceToHost, st));
+    CB_HIP(hipStreamSynchronize(st));
+    const dim3 g2(blocks(dim), (unsigned)r);
+    const int max_outer = 60;
+    int64_t inner_total = 0;
+    for (int outer = 0; outer < max_outer; ++outer) {
+        // r64 = B - C64 x64
+        CB_HIP(hipMemsetAsync(CX.p, 0, nv * 8, st));
+        if (outer > 0) apply_C<double>(M, lambda, X, CX.p, r);
+        k_sub<double, double><<<g2, 256, 0, st>>>(B, CX.p, R64.p, dim, ld);
+        col_dots<double>(M, R64.p, R64.p, r, part.p, nrm.p);
+        CB_HIP(hipMemcpyAsync(hr.data(), nrm.p, r * 8, hipMemcpyDeviceToHost, st));
+        CB_HIP(hipStreamSynchronize(st));
+        double worst = 0;
+        for (int c = 0; c < r; ++c) {
+            const double bn = hb[c] == 0.0 ? 1.0 : hb[c];
+            const double rel = absolute ? std::sqrt(hr[c]) : std::sqrt(hr[c] / bn);
+            worst = std::max(worst, rel);
+            const double rn = std::sqrt(hr[c]);
+            hs[c] = rn > 0 ? rn : 1.0;        // unscale factor
+            hi[c] = rn > 0 ? 1.0 / rn : 0.0;  // scale to unit norm
+        }
+        S.max_rel = worst;
+        S.iterations = inner_total;
+        if (worst < o.tol) return S;
+        CB_HIP(hipMemcpyAsync(scale.p, hs.data(), r * 8, hipMemcpyHostToDevice, st));
+        CB_HIP(hipMemcpyAsync(inv.p, hi.data(), r * 8, hipMemcpyHostToDevice, st));
+        k_scale_to_f32<<<g2, 256, 0, st>>>(R64.p, R32.p, inv.p, dim, ld);
+        CB_HIP(hipMemsetAsync(D32.p, 0, nv * 4, st));
+        inner_total += pcg_block<float, float>(M, lambda, R32.p, D32.p, r, o.inner_tol, false,
+                                               o.max_iter, block, nullptr, true);
+        k_axpy_from_f32<<<g2, 256, 0, st>>>(X, D32.p, scale.p, dim, ld);
+        CB_HIP(hipGetLastError());
+        if (inner_total > o.max_iter * 4) break;
+    }
+    throw Error(CB_ERR_NOT_CONVERGED, "mixed-precision PCG refinement did not reach the tolerance");
+}
+
+// ---------------------------------------------------------------------------
+// MME assembly (host) — MMELHSBuilder::initialize (mme_builder.cpp:72-257)
+static void assemble(cb_mme* M, const int64_t* x_ptr, const int32_t* x_col, const double* x_val,
+                     const int32_t* aid, const double* y) {
+    const int64_t n = M->n, p = M->p, q = M->q, dim = M->dim;
+    struct Trip { int64_t r, c; double v; };
+    std::vector<Trip> wt;  // W^T entries (unknown, record, value)
+    std::vector<Trip> tr;  // W'W entries
+    wt.reserve((size_t)(x_ptr[n] + n));
+    for (int64_t r = 0; r < n; ++r) {
+        CB_REQUIRE(aid[r] < q && aid[r] >= -1, "record random-effect index out of range");
+        std::vector<std::pair<int64_t, double>> row;
+        for (int64_t k = x_ptr[r]; k < x_ptr[r + 1]; ++k) {
+            CB_REQUIRE(x_col[k] >= 0 && x_col[k] < p, "fixed-effect column out of range");
+            row.emplace_back(x_col[k], x_val[k]);
+        }
+        if (aid[r] >= 0) row.emplace_back(p + aid[r], 1.0);
+        for (auto& a : row) {
+            wt.push_back({a.first, r, a.second});
+            for (auto& b : row) tr.push_back({a.first, b.first, a.second * b.second});
+        }
+    }
+    auto to_csr = [](std::vector<Trip>& t, int64_t rows, std::vector<int64_t>& ptr, std::vector<int32_t>& col,
+                     std::vector<double>& val) {
+        std::sort(t.begin(), t.end(), [](const Trip& a, const Trip& b) { return a.r != b.r ? a.r < b.r : a.c < b.c; });
+        ptr.assign(rows + 1, 0);
+        col.clear();
+        val.clear();
+        for (size_t i = 0; i < t.size();) {
+            size_t j = i;
+            double s = 0;
+            while (j < t.size() && t[j].r == t[i].r && t[j].c == t[i].c) s += t[j++].v;
+            col.push_back((int32_t)t[i].c);
+            val.push_back(s);
+            ptr[t[i].r + 1]++;
+            i = j;
+        }
+        for (int64_t r = 0; r < rows; ++r) ptr[r + 1] += ptr[r];
+    };
+    std::vector<int64_t> tp, wp;
+    std::vector<int32_t> tc, wc;
+    std::vector<double> tv, wv;
+    to_csr(wt, dim, tp, tc, tv);
+    to_csr(tr, dim, wp, wc, wv);
+    M->wtw_diag.assign(dim, 0.0);
+    M->xtx.assign((size_t)p * p, 0.0);
+    int64_t long_rows = 0;
+    for (int64_t r = 0; r < dim; ++r)
+        for (int64_t k = wp[r]; k < wp[r + 1]; ++k) {
+            if (wc[k] == r) M->wtw_diag[r] = wv[k];
+            if (r < p && wc[k] < p) M->xtx[r * p + wc[k]] = wv[k];
+        }
+    for (int64_t r = 0; r < p; ++r)
+        if (wp[r + 1] - wp[r] > 32) long_rows = r + 1;
+    M->long_rows = long_rows;
+    // RHS = W'y (mme_builder.cpp:321-328), y'y
+    M->rhs.assign(dim, 0.0);
+    for (int64_t i = 0; i < dim; ++i)
+        for (int64_t k = tp[i]; k < tp[i + 1]; ++k) M->rhs[i] += tv[k] * y[tc[k]];
+    M->yty = 0;
+    for (int64_t r = 0; r < n; ++r) M->yty += y[r] * y[r];
+    // upload
+    hipStream_t st = M->ctx->stream;
+    auto up = [&](auto& buf, const auto& vec) {
+        using E = std::remove_reference_t<decltype(*buf.p)>;
+        buf.alloc(std::max<size_t>(1, vec.size()));
+        if (!vec.empty()) CB_HIP(hipMemcpyAsync(buf.p, vec.data(), vec.size() * sizeof(E), hipMemcpyHostToDevice, st));
+    };
+    std::vector<int64_t> xp(x_ptr, x_ptr + n + 1);
+    std::vector<int32_t> xc(x_col, x_col + x_ptr[n]), av(aid, aid + n);
+    std::vector<double> xv(x_val, x_val + x_ptr[n]), yv(y, y + n);
+    up(M->xp, xp); up(M->xc, xc); up(M->xv, xv); up(M->aid, av); up(M->y, yv);
+    up(M->tp, tp); up(M->tc, tc); up(M->tv, tv);
+    up(M->wp, wp); up(M->wc, wc); up(M->wv64, wv);
+    std::vector<float> wv32(wv.begin(), wv.end());
+    up(M->wv32, wv32);
+    CB_HIP(hipStreamSynchronize(st));
+}
+
+// ---------------------------------------------------------------------------
+// REML
+namespace {
+
+struct RemlState {
+    double vu, ve;
+    double grad[2], ai[4];
+    double tr, uQu, esse;
+};
+
+// 2x2 SPD projection of vce.cpp:267-283 (closed-form symmetric eigen)
+void project_spd_2x2(double A[4]) {
+    const double a = A[0], b = 0.5 * (A[1] + A[2]), d = A[3];
+    const double tr = a + d, det = a * d - b * b;
+    const double disc = std::sqrt(std::max(0.0, 0.25 * (a - d) * (a - d) + b * b));
+    double l1 = 0.5 * tr + disc, l2 = 0.5 * tr - disc;  // l1 >= l2
+    double v1x, v1y;
+    if (std::fabs(b) > 0) { v1x = l1 - d; v1y = b; }
+    else if (a >= d) { v1x = 1; v1y = 0; }
+    else { v1x = 0; v1y = 1; }
+    const double nv = std::sqrt(v1x * v1x + v1y * v1y);
+    v1x /= nv; v1y /= nv;
+    const double v2x = -v1y, v2y = v1x;
+    const double maxe = std::max(std::max(l1, l2), 0.0);
+    const double floor = std::max(1e-12, 1e-8 * maxe);
+    l1 = std::max(l1, floor);
+    l2 = std::max(l2, floor);
+    (void)det;
+    A[0] = l1 * v1x * v1x + l2 * v2x * v2x;
+    A[1] = A[2] = l1 * v1x * v1y + l2 * v2x * v2y;
+    A[3] = l1 * v1y * v1y + l2 * v2y * v2y;
+}
+
+}  // namespace
+
+struct cb_reml_work {
+    DevBuf<double> B, S, Qz, B2, S2, V, T, part, dots, e;
+};
+
+// One evaluation at (vu, ve): solves, score terms, optionally the AI matrix.
+static void reml_eval(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, double vu, double ve,
+                      bool want_ai, RemlState& out, double* sol_out) {
+    cb_context* ctx = M->ctx;
+    hipStream_t st = ctx->stream;
+    const int64_t ld = M->ldv, p = M->p, q = M->q, n = M->n, dim = M->dim;
+    const int k = (int)(o.trace_exact ? q : o.n_probes);
+    const int r = 1 + k;
+    const double lambda = ve / std::max(vu, 1e-9);
+    // block: column 0 = W'y (already in B), columns 1..k = [0; z]
+    SolveStats S1 = solve_block(M, lambda, o.solve, w.B.p, w.S.p, r);
+    (void)S1;
+    // score terms ("adjoint" pass over the solution block)
+    DevBuf<double> Qu;
+    Qu.alloc(ld);
+    CB_HIP(hipMemsetAsync(Qu.p, 0, ld * 8, st));
+    if (M->qkind == 0) dense_q_apply<double>(M, w.S.p, ld, Qu.p, ld, 1, 1.0);
+    else {
+        k_csr_spmm<double, double><<<dim3(blocks(q), 1), 256, 0, st>>>(M->qp.p, M->qc.p, M->qv64.p, 0, q, p,
+                                                                      w.S.p, ld, Qu.p, ld, p, 1.0, 1);
+    }
+    // e = y - W s ; e_sse ; u'Qu ; traces t_i = (Q z_i)' S_u,i
+    k_w_apply<<<dim3(blocks(n), 1), 256, 0, st>>>(M->xp.p, M->xc.p, M->xv.p, M->aid.p, n, p, w.S.p, ld, w.T.p, round_up(n, 64));
+    k_axpby<<<blocks(n), 256, 0, st>>>(M->y.p, 1.0, w.T.p, -1.0, w.e.p, n);
+    CB_HIP(hipGetLastError());
+    std::vector<double> hd(r + 2);
+    {
+        // e'e over n rows: reuse col_dots with ld = n-padded by a temporary view
+        const int64_t ldn = round_up(n, 64);
+        DevBuf<double> tmp;
+        tmp.alloc(ldn);
+        CB_HIP(hipMemsetAsync(tmp.p, 0, ldn * 8, st));
+        CB_HIP(hipMemcpyAsync(tmp.p, w.e.p, n * 8, hipMemcpyDeviceToDevice, st));
+        k_col_dots<double><<<dim3(DOT_CHUNKS, 1), 256, 0, st>>>(tmp.p, tmp.p, ldn, n, w.part.p);
+        k_dots_finish<<<1, 64, 0, st>>>(w.part.p, 1, w.dots.p);
+        // u'Qu (rows p..dim of S col 0 against Qu)
+        k_col_dots<double><<<dim3(DOT_CHUNKS, 1), 256, 0, st>>>(w.S.p + p, Qu.p + p, ld, q, w.part.p);
+        k_dots_finish<<<1, 64, 0, st>>>(w.part.p, 1, w.dots.p + 1);
+        // traces: columns 1..k of S_u against Qz
+        if (k > 0) {
+            k_col_dots<double><<<dim3(DOT_CHUNKS, (unsigned)k), 256, 0, st>>>(w.S.p + ld + p, w.Qz.p + p, ld, q, w.part.p);
+            k_dots_finish<<<blocks(k, 64), 64, 0, st>>>(w.part.p, k, w.dots.p + 2);
+        }
+        CB_HIP(hipGetLastError());
+        CB_HIP(hipMemcpyAsync(hd.data(), w.dots.p, (2 + k) * 8, hipMemcpyDeviceToHost, st));
+        CB_HIP(hipStreamSynchronize(st));
+    }
+    double tr = 0;
+    for (int i = 0; i < k; ++i) tr += hd[2 + i];
+    if (!o.trace_exact && k > 0) tr /= k;
+    out.vu = vu;
+    out.ve = ve;
+    out.esse = hd[0];
+    out.uQu = hd[1];
+    out.tr = tr;
+    // score, vce.cpp:2655 and :2661-2675 (lambda*tr capped below q)
+    double lambda_tr = lambda * tr;
+    if (lambda_tr > q - 1e-6) lambda_tr = q - 1e-6;
+    out.grad[0] = 0.5 * (out.uQu + tr * ve - q * vu) / (vu * vu);
+    out.grad[1] = 0.5 * (out.esse - ve * ((double)n - p - q + lambda_tr)) / (ve * ve);
+    if (sol_out) CB_HIP(hipMemcpy(sol_out, w.S.p, dim * 8, hipMemcpyDeviceToHost));
+    if (!want_ai) return;
+    // AI matrix, vce.cpp:2596-2645: v_u = Z u / vu, v_e = e / ve; P v = (v - W C^-1 W'v) / ve
+    const int64_t ldn = round_up(n, 64);
+    k_ai_vectors<<<blocks(n), 256, 0, st>>>(M->aid.p, n, p, w.S.p, w.e.p, 1.0 / vu, 1.0 / ve, w.V.p, ldn);
+    k_wt_apply<<<dim3(blocks(dim), 2), 256, 0, st>>>(M->tp.p, M->tc.p, M->tv.p, dim, w.V.p, ldn, w.B2.p, ld);
+    CB_HIP(hipGetLastError());
+    solve_block(M, lambda, o.solve, w.B2.p, w.S2.p, 2);
+    k_w_apply<<<dim3(blocks(n), 2), 256, 0, st>>>(M->xp.p, M->xc.p, M->xv.p, M->aid.p, n, p, w.S2.p, ld, w.T.p, ldn);
+    std::vector<double> hV((size_t)ldn * 2), hT((size_t)ldn * 2);
+    CB_HIP(hipMemcpyAsync(hV.data(), w.V.p, ldn * 2 * 8, hipMemcpyDeviceToHost, st));
+    CB_HIP(hipMemcpyAsync(hT.data(), w.T.p, ldn * 2 * 8, hipMemcpyDeviceToHost, st));
+    CB_HIP(hipStreamSynchronize(st));
+    double a_uu = 0, a_ue = 0, a_ee = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double vu_i = hV[i], ve_i = hV[ldn + i];
+        const double pvu = (vu_i - hT[i]) / ve, pve = (ve_i - hT[ldn + i]) / ve;
+        a_uu += vu_i * pvu;
+        a_ue += vu_i * pve;
+        a_ee += ve_i * pve;
+    }
+    out.ai[0] = 0.5 * a_uu;
+    out.ai[1] = out.ai[2] = 0.5 * a_ue;
+    out.ai[3] = 0.5 * a_ee;
+}
+
+static void reml_setup(cb_mme* M, const cb_reml_options& o, cb_reml_work& w) {
+    cb_context* ctx = M->ctx;
+    hipStream_t st = ctx->stream;
+    const int64_t ld = M->ldv, p = M->p, q = M->q, n = M->n;
+    const int k = (int)(o.trace_exact ? q : o.n_probes);
+    const int r = 1 + k;
+    w.B.alloc((size_t)ld * r);
+    w.S.alloc((size_t)ld * r);
+    w.Qz.alloc((size_t)ld * std::max(1, k));
+    w.B2.alloc((size_t)ld * 2);
+    w.S2.alloc((size_t)ld * 2);
+    const int64_t ldn = round_up(n, 64);
+    w.V.alloc((size_t)ldn * 2);
+    w.T.alloc((size_t)ldn * 2);
+    w.e.alloc(ldn);
+    w.part.alloc((size_t)DOT_CHUNKS * std::max(2, r));
+    w.dots.alloc(2 + r);
+    CB_HIP(hipMemsetAsync(w.B.p, 0, w.B.bytes(), st));
+    CB_HIP(hipMemsetAsync(w.Qz.p, 0, w.Qz.bytes(), st));
+    CB_HIP(hipMemsetAsync(w.B2.p, 0, w.B2.bytes(), st));
+    CB_HIP(hipMemcpyAsync(w.B.p, M->rhs.data(), M->dim * 8, hipMemcpyHostToDevice, st));
+    if (k > 0) {
+        k_probes<<<dim3(blocks(q), (unsigned)k), 256, 0, st>>>(w.B.p + ld, ld, p, q, k, o.seed, o.trace_exact, 0);
+        CB_HIP(hipGetLastError());
+        // Qz (fixed for the run): Q applied to the probe columns
+        if (M->qkind == 0) dense_q_apply<double>(M, w.B.p + ld, ld, w.Qz.p, ld, k, 1.0);
+        else
+            k_csr_spmm<double, double><<<dim3(blocks(q), (unsigned)k), 256, 0, st>>>(
+                M->qp.p, M->qc.p, M->qv64.p, 0, q, p, w.B.p + ld, ld, w.Qz.p, ld, p, 1.0, 1);
+        CB_HIP(hipGetLastError());
+    }
+    CB_HIP(hipStreamSynchronize(st));
+}
+
+extern "C" {
+
+void cb_solve_options_default(cb_solve_options* o) {
+    if (!o) return;
+    o->precision = CB_PREC_MIXED;
+    o->precond = CB_PRECOND_BLOCK_JACOBI;
+    o->stop_rule = CB_STOP_RELATIVE;
+    o->batch_rhs = 1;
+    o->tol = 1e-10;
+    o->max_iter = 10000;
+    o->inner_tol = 1e-5;
+}
+
+void cb_reml_options_default(cb_reml_options* o) {
+    if (!o) return;
+    o->algorithm = CB_VCE_AI;
+    o->max_iter = 20;
+    o->tol = 1e-4;
+    o->n_probes = 64;
+    o->seed = 42;
+    o->trace_exact = 0;
+    cb_solve_options_default(&o->solve);
+}
+
+int cb_mme_create(cb_context* ctx, int64_t n, int64_t p, int64_t q, const int64_t* x_ptr,
+                  const int32_t* x_col, const double* x_val, const int32_t* aid, const double* y,
+                  cb_mme** out) {
+    return guarded([&] {
+        CB_REQUIRE(ctx && out && x_ptr && aid && y, "cb_mme_create: NULL argument");
+        CB_REQUIRE(n > 0 && p >= 0 && q > 0, "cb_mme_create: bad dimensions");
+        CB_REQUIRE(x_ptr[0] == 0 && x_ptr[n] >= 0 && (x_ptr[n] == 0 || (x_col && x_val)),
+                   "cb_mme_create: bad fixed-effect CSR");
+        set_device(ctx);
+        auto* M = new cb_mme();
+        M->ctx = ctx;
+        M->n = n; M->p = p; M->q = q; M->dim = p + q;
+        M->ldv = round_up(M->dim, 64);
+        try {
+            assemble(M, x_ptr, x_col, x_val, aid, y);
+        } catch (...) {
+            delete M;
+            throw;
+        }
+        *out = M;
+    });
+}
+
+void cb_mme_destroy(cb_mme* M) {
+    if (!M) return;
+    (void)hipSetDevice(M->ctx->device);
+    delete M;
+}
+
+int cb_mme_dims(const cb_mme* M, int64_t* n, int64_t* p, int64_t* q) {
+    return guarded([&] {
+        CB_REQUIRE(M, "cb_mme_dims: NULL");
+        if (n) *n = M->n;
+        if (p) *p = M->p;
+        if (q) *q = M->q;
+    });
+}
+
+static void set_q_dense_common(cb_mme* M, const double* Q, int64_t ld, hipMemcpyKind kind) {
+    const int64_t q = M->q;
+    CB_REQUIRE(Q && ld >= q, "cb_mme_set_qinv_dense: bad matrix");
+    set_device(M->ctx);
+    hipStream_t st = M->ctx->stream;
+    M->Qd64.alloc((size_t)q * q);
+    M->Qd32.alloc((size_t)q * q);
+    CB_HIP(hipMemcpy2DAsync(M->Qd64.p, q * 8, Q, ld * 8, q * 8, q, kind, st));
+    k_convert<float, double><<<blocks(q * q), 256, 0, st>>>(M->Qd64.p, M->Qd32.p, q * q);
+    CB_HIP(hipGetLastError());
+    M->q_diag.resize(q);
+    DevBuf<double> dg;
+    dg.alloc(q);
+    CB_HIP(hipMemcpy2DAsync(dg.p, 8, M->Qd64.p, (q + 1) * 8, 8, q, hipMemcpyDeviceToDevice, st));
+    CB_HIP(hipMemcpyAsync(M->q_diag.data(), dg.p, q * 8, hipMemcpyDeviceToHost, st));
+    CB_HIP(hipStreamSynchronize(st));
+    M->qkind = 0;
+    M->qp.release(); M->qc.release(); M->qv64.release(); M->qv32.release();
+    M->dinv_lambda = std::nan("");
+}
+
+int cb_mme_set_qinv_dense(cb_mme* M, const double* Q, int64_t ld) {
+    return guarded([&] {
+        CB_REQUIRE(M, "NULL mme");
+        set_q_dense_common(M, Q, ld, hipMemcpyHostToDevice);
+    });
+}
+
+int cb_mme_set_qinv_dense_device(cb_mme* M, const double* dQ, int64_t ld) {
+    return guarded([&] {
+        CB_REQUIRE(M, "NULL mme");
+        set_q_dense_common(M, dQ, ld, hipMemcpyDeviceToDevice);
+    });
+}
+
+int cb_mme_set_qinv_csr(cb_mme* M, const int64_t* ptr, const int32_t* col, const double* val) {
+    return guarded([&] {
+        CB_REQUIRE(M && ptr && col && val, "cb_mme_set_qinv_csr: NULL argument");
+        set_device(M->ctx);
+        const int64_t q = M->q, nnz = ptr[q];
+        CB_REQUIRE(ptr[0] == 0 && nnz >= 0, "cb_mme_set_qinv_csr: bad CSR");
+        M->q_diag.assign(q, 0.0);
+        for (int64_t i = 0; i < q; ++i)
+            for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+                CB_REQUIRE(col[k] >= 0 && col[k] < q, "cb_mme_set_qinv_csr: column out of range");
+                if (col[k] == i) M->q_diag[i] += val[k];
+            }
+        hipStream_t st = M->ctx->stream;
+        M->qp.alloc(q + 1);
+        M->qc.alloc(std::max<int64_t>(1, nnz));
+        M->qv64.alloc(std::max<int64_t>(1, nnz));
+        M->qv32.alloc(std::max<int64_t>(1, nnz));
+        CB_HIP(hipMemcpyAsync(M->qp.p, ptr, (q + 1) * 8, hipMemcpyHostToDevice, st));
+        CB_HIP(hipMemcpyAsync(M->qc.p, col, nnz * 4, hipMemcpyHostToDevice, st));
+        CB_HIP(hipMemcpyAsync(M->qv64.p, val, nnz * 8, hipMemcpyHostToDevice, st));
+        k_convert<float, double><<<blocks(nnz), 256, 0, st>>>(M->qv64.p, M->qv32.p, nnz);
+        CB_HIP(hipGetLastError());
+        CB_HIP(hipStreamSynchronize(st));
+        M->qkind = 1;
+        M->Qd64.release(); M->Qd32.release();
+        M->dinv_lambda = std::nan("");
+    });
+}
+
+int cb_mme_rhs(const cb_mme* M, double* rhs) {
+    return guarded([&] {
+        CB_REQUIRE(M && rhs, "cb_mme_rhs: NULL argument");
+        std::copy(M->rhs.begin(), M->rhs.end(), rhs);
+    });
+}
+
+int cb_mme_apply(cb_mme* M, double lambda, int64_t r, const double* X, double* Y) {
+    return guarded([&] {
+        CB_REQUIRE(M && X && Y && r > 0, "cb_mme_apply: bad argument");
+        CB_REQUIRE(M->qkind >= 0, "cb_mme_apply: Qinv not set");
+        set_device(M->ctx);
+        hipStream_t st = M->ctx->stream;
+        const int64_t ld = M->ldv;
+        DevBuf<double> dX, dY;
+        dX.alloc((size_t)ld * r);
+        dY.alloc((size_t)ld * r);
+        CB_HIP(hipMemsetAsync(dX.p, 0, dX.bytes(), st));
+        CB_HIP(hipMemcpy2DAsync(dX.p, ld * 8, X, M->dim * 8, M->dim * 8, r, hipMemcpyHostToDevice, st));
+        apply_C<double>(M, lambda, dX.p, dY.p, (int)r);
+        CB_HIP(hipMemcpy2DAsync(Y, M->dim * 8, dY.p, ld * 8, M->dim * 8, r, hipMemcpyDeviceToHost, st));
+        CB_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int cb_mme_solve(cb_mme* M, double lambda, const cb_solve_options* opts, int64_t r, const double* B,
+                 double* X, int64_t* iterations, double* rel_residual) {
+    cb_solve_options o;
+    cb_solve_options_default(&o);
+    if (opts) o = *opts;
+    int64_t its = 0;
+    double rel = 0;
+    const int st = guarded([&] {
+        CB_REQUIRE(M && B && X && r > 0, "cb_mme_solve: bad argument");
+        CB_REQUIRE(M->qkind >= 0, "cb_mme_solve: Qinv not set");
+        CB_REQUIRE(lambda >= 0 && std::isfinite(lambda), "cb_mme_solve: lambda must be finite and >= 0");
+        set_device(M->ctx);
+        hipStream_t s = M->ctx->stream;
+        const int64_t ld = M->ldv;
+        DevBuf<double> dB, dX;
+        dB.alloc((size_t)ld * r);
+        dX.alloc((size_t)ld * r);
+        CB_HIP(hipMemsetAsync(dB.p, 0, dB.bytes(), s));
+        CB_HIP(hipMemcpy2DAsync(dB.p, ld * 8, B, M->dim * 8, M->dim * 8, r, hipMemcpyHostToDevice, s));
+        try {
+            SolveStats S = solve_block(M, lambda, o, dB.p, dX.p, (int)r);
+            its = S.iterations;
+            rel = S.max_rel;
+        } catch (const Error& e) {
+            // hand back the unconverged iterate (matrix_free_pcg.h:64-65 returns it silently)
+            CB_HIP(hipMemcpy2DAsync(X, M->dim * 8, dX.p, ld * 8, M->dim * 8, r, hipMemcpyDeviceToHost, s));
+            CB_HIP(hipStreamSynchronize(s));
+            throw;
+        }
+        CB_HIP(hipMemcpy2DAsync(X, M->dim * 8, dX.p, ld * 8, M->dim * 8, r, hipMemcpyDeviceToHost, s));
+        CB_HIP(hipStreamSynchronize(s));
+    });
+    if (iterations) *iterations = its;
+    if (rel_residual) *rel_residual = rel;
+    return st;
+}
+
+int cb_reml_score(cb_mme* M, const cb_reml_options* opts, double var_u, double var_e, double* grad,
+                  double* ai, double* trace, double* solution) {
+    return guarded([&] {
+        CB_REQUIRE(M && opts, "cb_reml_score: NULL argument");
+        CB_REQUIRE(M->qkind >= 0, "cb_reml_score: Qinv not set");
+        CB_REQUIRE(var_u > 0 && var_e > 0, "cb_reml_score: variances must be positive");
+        set_device(M->ctx);
+        cb_reml_work w;
+        reml_setup(M, *opts, w);
+        RemlState s;
+        reml_eval(M, *opts, w, var_u, var_e, ai != nullptr, s, solution);
+        if (grad) { grad[0] = s.grad[0]; grad[1] = s.grad[1]; }
+        if (ai) for (int i = 0; i < 4; ++i) ai[i] = s.ai[i];
+        if (trace) *trace = s.tr;
+    });
+}
+
+// AI_REML::solve (vce.cpp:741-837) with the data-driven AI step
+// (vce.cpp:2414-2857) or the EM + Hutchinson step (vce.cpp:1490-1574).
+int cb_reml_run(cb_mme* M, const cb_reml_options* opts, double var_u0, double var_e0, double* var_u,
+                double* var_e, int64_t* iterations, int* converged, double* hist_vu, double* hist_ve,
+                double* solution) {
+    return guarded([&] {
+        CB_REQUIRE(M && opts && var_u && var_e, "cb_reml_run: NULL argument");
+        CB_REQUIRE(M->qkind >= 0, "cb_reml_run: Qinv not set");
+        CB_REQUIRE(var_u0 > 0 && var_e0 > 0, "cb_reml_run: initial variances must be positive");
+        CB_REQUIRE(opts->algorithm == CB_VCE_AI || opts->algorithm == CB_VCE_EM, "cb_reml_run: unknown algorithm");
+        set_device(M->ctx);
+        const cb_reml_options& o = *opts;
+        cb_reml_work w;
+        reml_setup(M, o, w);
+        // variance stabiliser of vce.cpp:1370-1389
+        const int64_t n = M->n;
+        std::vector<double> hy(n);
+        CB_HIP(hipMemcpy(hy.data(), M->y.p, n * 8, hipMemcpyDeviceToHost));
+        double ymean = 0;
+        for (double v : hy) ymean += v;
+        ymean = n > 0 ? ymean / n : 0.0;
+        double yvar = 0;
+        for (double v : hy) yvar += (v - ymean) * (v - ymean);
+        yvar = n > 1 ? std::max(1e-12, yvar / (double)(n - 1)) : std::max(1e-12, M->yty);
+        const double vfloor = 1e-12, vceil = std::max(1.0, yvar * 1e6);
+        auto stabilize = [&](double proposed, double previous) {
+            if (!std::isfinite(proposed) || proposed <= 0.0) return std::max(previous, vfloor);
+            double lo = vfloor, hi = vceil;
+            if (std::isfinite(previous) && previous > 0.0) {
+                lo = std::max(lo, previous * 0.1);
+                hi = std::min(hi, previous * 10.0);
+            }
+            if (lo > hi) lo = vfloor;
+            return std::min(std::max(proposed, lo), hi);
+        };
+        double vu = var_u0, ve = var_e0;
+        int64_t it = 0;
+        int conv = 0;
+        for (; it < o.max_iter;) {
+            const double old_u = vu, old_e = ve;
+            RemlState s;
+            reml_eval(M, o, w, vu, ve, o.algorithm == CB_VCE_AI, s, nullptr);
+            if (o.algorithm == CB_VCE_EM) {
+                const double lambda = ve / std::max(vu, 1e-9);
+                const double vu_new = (s.uQu + s.tr * ve) / (double)M->q;
+                const double ve_new = (s.esse + ve * ((double)(M->p + M->q) - lambda * s.tr)) / (double)n;
+                vu = stabilize(vu_new, old_u);
+                ve = stabilize(ve_new, old_e);
+            } else {
+                double A[4] = {s.ai[0], s.ai[1], s.ai[2], s.ai[3]};
+                project_spd_2x2(A);
+                const double det = A[0] * A[3] - A[1] * A[2];
+                double d0 = (A[3] * s.grad[0] - A[1] * s.grad[1]) / det;
+                double d1 = (A[0] * s.grad[1] - A[2] * s.grad[0]) / det;
+                bool ok = std::isfinite(d0) && std::isfinite(d1) &&
+                          std::fabs(d0) <= 100.0 * vu && std::fabs(d1) <= 100.0 * ve;
+                if (ok) {
+                    // halve until positive and within 5x (vce.cpp:2797-2813)
+                    int h = 0;
+                    for (; h < 15; ++h) {
+                        const double nu = vu + d0, ne = ve + d1;
+                        if (nu > 0 && ne > 0 && std::fabs(d0) <= 5.0 * vu && std::fabs(d1) <= 5.0 * ve) break;
+                        d0 *= 0.5;
+                        d1 *= 0.5;
+                    }
+                    ok = h < 15;
+                }
+                if (ok) {
+                    vu = stabilize(vu + d0, old_u);
+                    ve = stabilize(ve + d1, old_e);
+                } else {
+                    // EM fallback for this iteration (vce.cpp:1441-1454, PCG path)
+                    const double lambda = ve / std::max(vu, 1e-9);
+                    vu = stabilize((s.uQu + s.tr * ve) / (double)M->q, old_u);
+                    ve = stabilize((s.esse + ve * ((double)(M->p + M->q) - lambda * s.tr)) / (double)n, old_e);
+                }
+            }
+            if (hist_vu) hist_vu[it] = vu;
+            if (hist_ve) hist_ve[it] = ve;
+            ++it;
+            const double diff = std::fabs(ve - old_e) + std::fabs(vu - old_u);
+            if (diff < o.tol) {
+                conv = 1;
+                break;
+            }
+        }
+        *var_u = vu;
+        *var_e = ve;
+        if (iterations) *iterations = it;
+        if (converged) *converged = conv;
+        if (solution) {
+            // final solve at the estimates (GSSolve uses lambda = ve/vu)
+            const double lambda = ve / std::max(vu, 1e-9);
+            solve_block(M, lambda, o.solve, w.B.p, w.S.p, 1);
+            CB_HIP(hipMemcpy(solution, w.S.p, M->dim * 8, hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+int cb_reml_copy_probes(cb_mme* M, const cb_reml_options* opts, double* out, int64_t ld) {
+    return guarded([&] {
+        CB_REQUIRE(M && opts && out && ld >= M->q, "cb_reml_copy_probes: bad argument");
+        set_device(M->ctx);
+        cb_reml_work w;
+        reml_setup(M, *opts, w);
+        const int k = (int)(opts->trace_exact ? M->q : opts->n_probes);
+        CB_HIP(hipMemcpy2D(out, ld * 8, w.B.p + M->ldv + M->p, M->ldv * 8, M->q * 8, k, hipMemcpyDeviceToHost));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,174 @@
+"""Mixed-model equations and REML — host mirrors of the reference's solver
+interfaces over the C-ABI (cb_mme_*, cb_reml_*).
+
+  MixedModel       MMELHSBuilder + the relationship inverse of RandomComponent
+                   (src/core/mme_builder.{h,cpp}, src/core/matrix_adapter.h)
+  MixedModel.solve PCGSolver / MatrixFreePCG (src/core/pcg_solver.h,
+                   src/core/matrix_free_pcg.h)
+  AIREML           VCESolver / AI_REML (src/core/vce.h:223-547): solve(),
+                   getVarE(), getVarsU(), getSolution(), history
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .design import FixedDesignG
+
+
+def solve_options(precision=_lib.CB_PREC_MIXED, precond=_lib.CB_PRECOND_BLOCK_JACOBI,
+                  stop_rule=_lib.CB_STOP_RELATIVE, tol=1e-10, max_iter=10000, inner_tol=1e-5,
+                  batch_rhs=True) -> _lib.SolveOptions:
+    o = _lib.SolveOptions()
+    _lib.lib().cb_solve_options_default(C.byref(o))
+    o.precision, o.precond, o.stop_rule = precision, precond, stop_rule
+    o.tol, o.max_iter, o.inner_tol, o.batch_rhs = tol, max_iter, inner_tol, int(batch_rhs)
+    return o
+
+
+def reml_options(algorithm=_lib.CB_VCE_AI, max_iter=20, tol=1e-4, n_probes=64, seed=42,
+                 trace_exact=False, solve: _lib.SolveOptions | None = None) -> _lib.RemlOptions:
+    o = _lib.RemlOptions()
+    _lib.lib().cb_reml_options_default(C.byref(o))
+    o.algorithm, o.max_iter, o.tol, o.n_probes, o.seed = algorithm, max_iter, tol, n_probes, seed
+    o.trace_exact = int(trace_exact)
+    if solve is not None:
+        o.solve = solve
+    return o
+
+
+class MixedModel:
+    """One random component: y = X b + Z u + e, Var(u) = sigma_u^2 Q^-1."""
+
+    def __init__(self, fd: FixedDesignG, aid: np.ndarray, y: np.ndarray, q: int,
+                 ctx: _lib.Context | None = None):
+        self.ctx = ctx or _lib.default_context()
+        self.fd = fd
+        self.n = int(y.size)
+        self.p = int(fd.p)
+        self.q = int(q)
+        self._keep = [np.ascontiguousarray(a) for a in (fd.ptr, fd.col, fd.val)]
+        aid = np.ascontiguousarray(aid, dtype=np.int32)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        h = C.c_void_p()
+        _lib.check(_lib.lib().cb_mme_create(self.ctx.handle, self.n, self.p, self.q,
+                                            _lib.ptr(self._keep[0]), _lib.ptr(self._keep[1]),
+                                            _lib.ptr(self._keep[2]), _lib.ptr(aid), _lib.ptr(y),
+                                            C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.lib().cb_mme_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def dim(self) -> int:
+        return self.p + self.q
+
+    def set_qinv_dense(self, Q: np.ndarray) -> None:
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        _lib.check(_lib.lib().cb_mme_set_qinv_dense(self.handle, _lib.ptr(Q), Q.shape[1]))
+
+    def set_qinv_dense_device(self, dptr: int, ld: int) -> None:
+        _lib.check(_lib.lib().cb_mme_set_qinv_dense_device(self.handle, dptr, ld))
+
+    def set_qinv_csr(self, ptr, col, val) -> None:
+        ptr = np.ascontiguousarray(ptr, dtype=np.int64)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        _lib.check(_lib.lib().cb_mme_set_qinv_csr(self.handle, _lib.ptr(ptr), _lib.ptr(col), _lib.ptr(val)))
+
+    def rhs(self) -> np.ndarray:
+        out = np.empty(self.dim)
+        _lib.check(_lib.lib().cb_mme_rhs(self.handle, _lib.ptr(out)))
+        return out
+
+    def apply(self, lam: float, X: np.ndarray) -> np.ndarray:
+        X2 = np.ascontiguousarray(np.asarray(X, dtype=np.float64).reshape(self.dim, -1).T)
+        r = X2.shape[0]
+        Y = np.empty_like(X2)
+        _lib.check(_lib.lib().cb_mme_apply(self.handle, lam, r, _lib.ptr(X2), _lib.ptr(Y)))
+        return Y.T.reshape(np.asarray(X).shape)
+
+    def solve(self, lam: float, B: np.ndarray, opts: _lib.SolveOptions | None = None):
+        """Solve C(lam) X = B; returns (X, iterations, max relative residual)."""
+        opts = opts or solve_options()
+        B2 = np.ascontiguousarray(np.asarray(B, dtype=np.float64).reshape(self.dim, -1).T)
+        r = B2.shape[0]
+        X = np.empty_like(B2)
+        it = C.c_int64()
+        rel = C.c_double()
+        _lib.check(_lib.lib().cb_mme_solve(self.handle, lam, C.byref(opts), r, _lib.ptr(B2), _lib.ptr(X),
+                                           C.byref(it), C.byref(rel)))
+        return X.T.reshape(np.asarray(B).shape), it.value, rel.value
+
+
+class AIREML:
+    """VCESolver for the single-component model (vce.h:223-390)."""
+
+    def __init__(self, model: MixedModel, opts: _lib.RemlOptions | None = None,
+                 var_u: float = 1.0, var_e: float = 1.0):
+        self.model = model
+        self.opts = opts or reml_options()
+        self.var_u0, self.var_e0 = var_u, var_e  # phenotype_loader.cpp:174-176 defaults
+        self.var_u = self.var_e = None
+        self.solution = None
+        self.history: list[tuple[float, float]] = []
+        self.iterations = 0
+        self.converged = False
+
+    def setInitialVars(self, var_u: float, var_e: float) -> None:
+        self.var_u0, self.var_e0 = var_u, var_e
+
+    def solve(self) -> None:
+        m = self.model
+        kmax = int(self.opts.max_iter)
+        hu = np.zeros(max(1, kmax))
+        he = np.zeros(max(1, kmax))
+        sol = np.empty(m.dim)
+        vu, ve = C.c_double(), C.c_double()
+        it = C.c_int64()
+        conv = C.c_int()
+        _lib.check(_lib.lib().cb_reml_run(m.handle, C.byref(self.opts), self.var_u0, self.var_e0,
+                                          C.byref(vu), C.byref(ve), C.byref(it), C.byref(conv),
+                                          _lib.ptr(hu), _lib.ptr(he), _lib.ptr(sol)))
+        self.var_u, self.var_e = vu.value, ve.value
+        self.iterations, self.converged = it.value, bool(conv.value)
+        self.history = list(zip(hu[: it.value], he[: it.value]))
+        self.solution = sol
+
+    def score(self, var_u: float, var_e: float, want_ai: bool = True):
+        m = self.model
+        g = np.empty(2)
+        ai = np.empty(4)
+        tr = C.c_double()
+        sol = np.empty(m.dim)
+        _lib.check(_lib.lib().cb_reml_score(m.handle, C.byref(self.opts), var_u, var_e, _lib.ptr(g),
+                                            _lib.ptr(ai) if want_ai else None, C.byref(tr), _lib.ptr(sol)))
+        return g, ai.reshape(2, 2) if want_ai else None, tr.value, sol
+
+    def probes(self) -> np.ndarray:
+        m = self.model
+        k = m.q if self.opts.trace_exact else int(self.opts.n_probes)
+        out = np.empty((k, m.q))
+        _lib.check(_lib.lib().cb_reml_copy_probes(m.handle, C.byref(self.opts), _lib.ptr(out), m.q))
+        return out.T
+
+    # VCESolver getters
+    def getVarE(self) -> float:
+        return self.var_e
+
+    def getVarsU(self) -> list[float]:
+        return [self.var_u]
+
+    def getSolution(self) -> np.ndarray:
+        return self.solution

@@ -98,3 +98,24 @@ def bed_on_device(n: int, m: int, maf=(0.01, 0.5), missing_rate: float = 0.0, se
         rows[j0:j1, :bps] = packed
         del u, code, c, packed
     return rows, pitch
+
+
+def simulate_phenotype(dosage: np.ndarray, n_qtl: int = 500, h2: float = 0.3, n_groups: int = 20,
+                       seed: int = 2) -> tuple[np.ndarray, list[str]]:
+    """y = mu + CG effect + g + e: g from n_qtl QTL with N(0,1) effects scaled
+    to variance h2, e ~ N(0, 1 - h2), contemporary groups ~ N(0, 1) assigned
+    uniformly. Group labels are zero-padded so string order is numeric order."""
+    rng = np.random.default_rng(seed)
+    n, m = dosage.shape
+    qtl = rng.choice(m, size=min(n_qtl, m), replace=False)
+    eff = rng.standard_normal(qtl.size)
+    xq = np.where(dosage[:, qtl] < 0, 0, dosage[:, qtl]).astype(np.float64)
+    g = (xq - xq.mean(axis=0)) @ eff
+    g *= np.sqrt(h2 / max(g.var(), 1e-12))
+    e = rng.standard_normal(n) * np.sqrt(1.0 - h2)
+    grp = rng.integers(0, n_groups, size=n)
+    ge = rng.standard_normal(n_groups)
+    y = 10.0 + ge[grp] + g + e
+    width = len(str(n_groups))
+    labels = [f"cg{int(c):0{width}d}" for c in grp]
+    return y, labels

@@ -13,6 +13,7 @@
  *   PlinkReader::load / openBed           src/core/plink_reader.cpp:27-91
  *   GrmBuilder::compute(PlinkReader)      src/core/grm_builder.cpp:63-405
  *   GrmBuilder getters                    src/core/grm_builder.h:55-64
+ *   MMELHSBuilder / PCGSolver / MatrixFreePCG / AI_REML (see each entry)
  */
 #ifndef COSMICBLUP_AMD_H
 #define COSMICBLUP_AMD_H
@@ -123,6 +124,93 @@ int cb_grm_copy_allele_freqs(const cb_grm* grm, float* out, int64_t m);
 int cb_grm_copy_snp_stats(const cb_grm* grm, int64_t m, int32_t* n_obs, int32_t* sum_x,
                           int32_t* n_hom1, int32_t* n_het, int32_t* n_hom2, uint8_t* valid,
                           float* mean_f, float* twopq_f);
+
+
+/* ---- mixed-model equations -------------------------------------------- */
+typedef struct cb_mme cb_mme;
+
+/* MMELHSBuilder(recs, fd, comps) (src/core/mme_builder.cpp:72-257) for one
+ * random component: n records; record r has the fixed-effect row
+ * X[r,:] as CSR (x_ptr[n+1], x_col, x_val over p columns — FixedDesignG.rows,
+ * src/core/design.cpp:93-108), random level aid[r] in [0,q) or -1 for none
+ * (GenRecord.aid - 1, mme_builder.cpp:137) and phenotype y[r]. The LHS
+ * W'W (W = [X Z]) and RHS W'y are assembled once on the host and uploaded. */
+int cb_mme_create(cb_context* ctx, int64_t n, int64_t p, int64_t q, const int64_t* x_ptr,
+                  const int32_t* x_col, const double* x_val, const int32_t* aid, const double* y,
+                  cb_mme** out);
+void cb_mme_destroy(cb_mme* mme);
+int cb_mme_dims(const cb_mme* mme, int64_t* n, int64_t* p, int64_t* q);
+/* The relationship inverse Q of RandomComponent (mme_builder.h:10-21):
+ * dense q x q row-major (DenseMatrixAdapter, src/core/matrix_adapter.h:116-153),
+ * from host or device memory, or CSR with full symmetric storage
+ * (SparseMatrixAdapter, matrix_adapter.h:56-113). Kept in FP64 and FP32. */
+int cb_mme_set_qinv_dense(cb_mme* mme, const double* Q, int64_t ld);
+int cb_mme_set_qinv_dense_device(cb_mme* mme, const double* dQ, int64_t ld);
+int cb_mme_set_qinv_csr(cb_mme* mme, const int64_t* ptr, const int32_t* col, const double* val);
+/* build_rhs (mme_builder.cpp:321-328): [X'y; Z'y], length p+q. */
+int cb_mme_rhs(const cb_mme* mme, double* rhs);
+/* Y = C(lambda) X, C = W'W + lambda blockdiag(0, Q), FP64; X, Y column-major (p+q) x r. */
+int cb_mme_apply(cb_mme* mme, double lambda, int64_t r, const double* X, double* Y);
+
+#define CB_PREC_FP64 0   /* FP64 operator and vectors ("plain") */
+#define CB_PREC_MIXED 1  /* FP32 block PCG inside FP64 residual refinement */
+#define CB_PRECOND_JACOBI 0        /* JacobiPreconditioner, pcg_solver.h:35-57 */
+#define CB_PRECOND_BLOCK_JACOBI 1  /* BlockJacobiPreconditioner, pcg_solver.h:60-128 */
+#define CB_STOP_RELATIVE 0  /* ||r|| / ||b|| < tol, PCGSolver (pcg_solver.h:501) */
+#define CB_STOP_ABSOLUTE 1  /* ||r|| < tol, MatrixFreePCG (matrix_free_pcg.h:51) */
+
+typedef struct {
+    int precision;      /* CB_PREC_*; default MIXED */
+    int precond;        /* CB_PRECOND_*; default BLOCK_JACOBI */
+    int stop_rule;      /* CB_STOP_*; default RELATIVE */
+    int batch_rhs;      /* FP64 only: 1 = all columns as one block, 0 = one at a time */
+    double tol;         /* default 1e-10 (config.h:41) */
+    int64_t max_iter;   /* default 10000 (config.h:42) */
+    double inner_tol;   /* MIXED: residual reduction per FP32 inner solve, default 1e-5 */
+} cb_solve_options;
+
+void cb_solve_options_default(cb_solve_options* opts);
+/* Solve C(lambda) X = B for r right-hand sides (column-major (p+q) x r host
+ * arrays), each column its own CG recurrence and stop test. Replaces
+ * PCGSolver::solve / MatrixFreePCG::solve as called by AI_REML::solve_lhs
+ * (vce.cpp:1124-1236) and GSSolve (mixed_model_solver.cpp:243-255).
+ * Returns CB_ERR_NOT_CONVERGED (X then holds the last iterate) when the
+ * iteration cap is hit. */
+int cb_mme_solve(cb_mme* mme, double lambda, const cb_solve_options* opts, int64_t r,
+                 const double* B, double* X, int64_t* iterations, double* rel_residual);
+
+/* ---- REML -------------------------------------------------------------- */
+#define CB_VCE_AI 0  /* data-driven AI step, vce.cpp:2414-2857 */
+#define CB_VCE_EM 1  /* EM + Hutchinson step, vce.cpp:1490-1574 */
+
+typedef struct {
+    int algorithm;      /* CB_VCE_*; default AI */
+    int64_t max_iter;   /* ai_maxit, default 20 (config.h:52) */
+    double tol;         /* vce_tol on sum |d sigma^2|, default 1e-4 (config.h:57) */
+    int64_t n_probes;   /* Rademacher probes for tr(Q C^uu), fixed for the run; default 64 */
+    uint64_t seed;      /* probe stream: sign of splitmix64(seed + g*(c*q + j + 1)), g = 0x9E3779B97F4A7C15 */
+    int trace_exact;    /* 1: probes are the q unit vectors (exact trace) */
+    cb_solve_options solve;
+} cb_reml_options;
+
+void cb_reml_options_default(cb_reml_options* opts);
+/* VCESolver::solve for the single-component GBLUP/PBLUP model
+ * (VCESolver::create + AI_REML::solve, vce.cpp:741-837,3592-3611):
+ * iterate from (var_u0, var_e0) until sum |d sigma^2| < tol or max_iter.
+ * hist_vu / hist_ve (optional, length max_iter) receive each iterate;
+ * solution (optional, p+q) receives the MME solution at the estimates
+ * (getSolution()). */
+int cb_reml_run(cb_mme* mme, const cb_reml_options* opts, double var_u0, double var_e0,
+                double* var_u, double* var_e, int64_t* iterations, int* converged,
+                double* hist_vu, double* hist_ve, double* solution);
+/* REML score dlogL/d(var_u, var_e) at the given variances from one block
+ * solve [W'y | probes] (the data-driven gradient of vce.cpp:2647-2675),
+ * optionally the AI matrix (2x2 row-major, vce.cpp:2596-2645), the trace
+ * estimate of tr(Q C^uu) and the MME solution. */
+int cb_reml_score(cb_mme* mme, const cb_reml_options* opts, double var_u, double var_e,
+                  double* grad, double* ai, double* trace, double* solution);
+/* The probe matrix the run uses (q x k, column-major, leading dimension ld). */
+int cb_reml_copy_probes(cb_mme* mme, const cb_reml_options* opts, double* out, int64_t ld);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,212 @@
+"""Oracle (test infrastructure) for the MME, PCG and AI-REML part of the hot
+path: dense FP64 numpy restatements of the reference algorithms.
+
+  mme_dense            MMELHSBuilder LHS/RHS (src/core/mme_builder.cpp:72-328)
+  probes               the probe stream the GPU uses (splitmix64 signs)
+  reml_eval            data-driven score + AI (src/core/vce.cpp:2414-2675)
+  reml_run             AI_REML::solve loop (vce.cpp:741-837) with the AI step
+                       (vce.cpp:2596-2813 semantics for the PCG path) or the EM
+                       + Hutchinson step (vce.cpp:1490-1574); variance
+                       stabiliser vce.cpp:1370-1389
+  logL                 calc_logL_internal (vce.cpp:3533-3588)
+  henderson_ainv       invertA_henderson without inbreeding
+                       (src/core/a_matrix.cpp:392-439, textbook rules)
+
+Pinning (tests/test_oracle_reml.py): MME solution vs the V-form BLUP/GLS
+formulas; the score vs central finite differences of logL; EM and AI fixed
+points coincide; Henderson A^-1 vs the inverse of the tabular A on small
+pedigrees and the 4-animal fixture by hand; probes vs a scalar splitmix64.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+GOLD = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    x = np.asarray(x, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        x = x + GOLD
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return x ^ (x >> np.uint64(31))
+
+
+def probes(q: int, k: int, seed: int, exact: bool = False) -> np.ndarray:
+    """q x k probe matrix: sign of splitmix64(seed + g*(c*q + j + 1))."""
+    if exact:
+        return np.eye(q)
+    c = np.arange(k, dtype=np.uint64)[None, :]
+    j = np.arange(q, dtype=np.uint64)[:, None]
+    with np.errstate(over="ignore"):
+        arg = np.uint64(seed) + GOLD * (c * np.uint64(q) + j + np.uint64(1))
+    h = splitmix64(arg)
+    return np.where((h >> np.uint64(63)) != 0, -1.0, 1.0)
+
+
+def design_w(X: np.ndarray, aid: np.ndarray, q: int) -> np.ndarray:
+    n = X.shape[0]
+    Z = np.zeros((n, q))
+    ok = aid >= 0
+    Z[np.nonzero(ok)[0], aid[ok]] = 1.0
+    return np.hstack([X, Z])
+
+
+def mme_dense(X, aid, y, Q, lam):
+    """C = W'W + lam blockdiag(0, Q), rhs = W'y (data scale, as build_lhs)."""
+    p, q = X.shape[1], Q.shape[0]
+    W = design_w(X, aid, q)
+    C = W.T @ W
+    C[p:, p:] += lam * Q
+    return C, W.T @ y, W
+
+
+def reml_eval(X, aid, y, Q, vu, ve, Zp, want_ai=True, exact=False):
+    """Score, AI and trace at (vu, ve) with probe matrix Zp (q x k)."""
+    n, p = X.shape
+    q = Q.shape[0]
+    lam = ve / max(vu, 1e-9)
+    C, rhs, W = mme_dense(X, aid, y, Q, lam)
+    k = Zp.shape[1]
+    B = np.zeros((p + q, 1 + k))
+    B[:, 0] = rhs
+    B[p:, 1:] = Zp
+    S = np.linalg.solve(C, B)
+    u = S[p:, 0]
+    e = y - W @ S[:, 0]
+    esse = float(e @ e)
+    uQu = float(u @ Q @ u)
+    QZ = Q @ Zp
+    t = np.einsum("ij,ij->j", QZ, S[p:, 1:])
+    tr = float(t.sum()) if exact else float(t.mean())
+    lam_tr = min(lam * tr, q - 1e-6)
+    grad = np.array([0.5 * (uQu + tr * ve - q * vu) / (vu * vu),
+                     0.5 * (esse - ve * (n - p - q + lam_tr)) / (ve * ve)])
+    out = {"grad": grad, "tr": tr, "uQu": uQu, "esse": esse, "sol": S[:, 0]}
+    if want_ai:
+        v_u = np.zeros(n)
+        ok = aid >= 0
+        v_u[ok] = u[aid[ok]] / vu
+        v_e = e / ve
+        S2 = np.linalg.solve(C, W.T @ np.stack([v_u, v_e], axis=1))
+        T = W @ S2
+        Pv_u = (v_u - T[:, 0]) / ve
+        Pv_e = (v_e - T[:, 1]) / ve
+        ai = 0.5 * np.array([[v_u @ Pv_u, v_u @ Pv_e], [v_u @ Pv_e, v_e @ Pv_e]])
+        out["ai"] = ai
+    return out
+
+
+def project_spd_2x2(A, eps_rel=1e-8, eps_abs=1e-12):
+    """vce.cpp:267-283."""
+    S = 0.5 * (A + A.T)
+    w, V = np.linalg.eigh(S)
+    floor = max(eps_abs, eps_rel * max(w.max(), 0.0))
+    w = np.maximum(w, floor)
+    return (V * w) @ V.T
+
+
+def reml_run(X, aid, y, Q, vu0=1.0, ve0=1.0, algorithm="ai", max_iter=20, tol=1e-4, Zp=None,
+             exact=False):
+    n, p = X.shape
+    q = Q.shape[0]
+    ymean = y.mean()
+    yvar = max(1e-12, float(((y - ymean) ** 2).sum() / (n - 1))) if n > 1 else max(1e-12, float(y @ y))
+    vfloor, vceil = 1e-12, max(1.0, yvar * 1e6)
+
+    def stabilize(prop, prev):
+        if not np.isfinite(prop) or prop <= 0:
+            return max(prev, vfloor)
+        lo, hi = vfloor, vceil
+        if np.isfinite(prev) and prev > 0:
+            lo, hi = max(lo, prev * 0.1), min(hi, prev * 10.0)
+        if lo > hi:
+            lo = vfloor
+        return min(max(prop, lo), hi)
+
+    vu, ve = vu0, ve0
+    hist = []
+    conv = False
+    for _ in range(max_iter):
+        ou, oe = vu, ve
+        s = reml_eval(X, aid, y, Q, vu, ve, Zp, want_ai=(algorithm == "ai"), exact=exact)
+        lam = ve / max(vu, 1e-9)
+        em_u = (s["uQu"] + s["tr"] * ve) / q
+        em_e = (s["esse"] + ve * ((p + q) - lam * s["tr"])) / n
+        if algorithm == "em":
+            vu, ve = stabilize(em_u, ou), stabilize(em_e, oe)
+        else:
+            A = project_spd_2x2(s["ai"])
+            d = np.linalg.solve(A, s["grad"])
+            ok = np.all(np.isfinite(d)) and abs(d[0]) <= 100 * vu and abs(d[1]) <= 100 * ve
+            if ok:
+                for h in range(15):
+                    if vu + d[0] > 0 and ve + d[1] > 0 and abs(d[0]) <= 5 * vu and abs(d[1]) <= 5 * ve:
+                        break
+                    d = d * 0.5
+                else:
+                    ok = False
+            if ok:
+                vu, ve = stabilize(vu + d[0], ou), stabilize(ve + d[1], oe)
+            else:
+                vu, ve = stabilize(em_u, ou), stabilize(em_e, oe)
+        hist.append((vu, ve))
+        if abs(ve - oe) + abs(vu - ou) < tol:
+            conv = True
+            break
+    lam = ve / max(vu, 1e-9)
+    C, rhs, _ = mme_dense(X, aid, y, Q, lam)
+    sol = np.linalg.solve(C, rhs)
+    return {"vu": vu, "ve": ve, "hist": hist, "converged": conv, "sol": sol}
+
+
+def logL(X, aid, y, Q, vu, ve):
+    """-1/2 (log|C| + (n-p-q) log ve + q log vu + y'Py), y'Py = (y'y - sol'rhs)/ve."""
+    n, p = X.shape
+    q = Q.shape[0]
+    lam = ve / vu
+    C, rhs, _ = mme_dense(X, aid, y, Q, lam)
+    sign, logdet = np.linalg.slogdet(C)
+    sol = np.linalg.solve(C, rhs)
+    yPy = (y @ y - sol @ rhs) / ve
+    return -0.5 * (logdet + (n - p - q) * np.log(ve) + q * np.log(vu) + yPy)
+
+
+def henderson_ainv(sire: np.ndarray, dam: np.ndarray) -> np.ndarray:
+    """Dense A^-1 by Henderson's rules without inbreeding (0-based parents,
+    -1 unknown); parents must precede offspring."""
+    nanim = sire.size
+    A = np.zeros((nanim, nanim))
+    for i in range(nanim):
+        s, d = sire[i], dam[i]
+        par = [x for x in (s, d) if x >= 0]
+        if len(par) == 2:
+            alpha = 2.0
+        elif len(par) == 1:
+            alpha = 4.0 / 3.0
+        else:
+            alpha = 1.0
+        idx = [i] + par
+        coef = [1.0] + [-0.5] * len(par)
+        for a, ca in zip(idx, coef):
+            for b, cb in zip(idx, coef):
+                A[a, b] += alpha * ca * cb
+    return A
+
+
+def tabular_a(sire: np.ndarray, dam: np.ndarray) -> np.ndarray:
+    """Numerator relationship matrix by the tabular method."""
+    nanim = sire.size
+    A = np.zeros((nanim, nanim))
+    for i in range(nanim):
+        s, d = sire[i], dam[i]
+        for j in range(i):
+            v = 0.0
+            if s >= 0:
+                v += 0.5 * A[j, s]
+            if d >= 0:
+                v += 0.5 * A[j, d]
+            A[i, j] = A[j, i] = v
+        A[i, i] = 1.0 + (0.5 * A[s, d] if s >= 0 and d >= 0 else 0.0)
+    return A

@@ -1,0 +1,191 @@
+"""GPU parity of the MME operator, block PCG (FP64 plain and mixed FP32 +
+FP64 refinement) and AI/EM-REML against the dense FP64 oracle on the same
+inputs and the same probe vectors.
+
+Tolerances (BASELINE metric): EBV error max|du| / max|u| <= 1e-6; variance
+components relative error <= 1e-4; the operator itself to 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+from cosmicblup_amd import _lib, design, mme, synth
+from oracle import grm as og
+from oracle import reml as orl
+
+pytestmark = pytest.mark.gpu
+
+EBV_TOL = 1e-6
+VC_TOL = 1e-4
+
+
+def gblup_problem(n=400, m=2000, ngroups=8, seed=11, h2=0.3, missing_records=0):
+    x, _ = synth.simulate_dosages(n, m, maf=(0.05, 0.5), seed=seed)
+    bed = synth.encode_bed(x)
+    G, _, _ = og.grm_vanraden(bed, n, m)
+    Q = og.invert_psd_matrix(G)
+    y, cg = synth.simulate_phenotype(x, n_qtl=50, h2=h2, n_groups=ngroups, seed=seed + 1)
+    aid = np.arange(n, dtype=np.int32)
+    if missing_records:
+        keep = np.ones(n, bool)
+        keep[np.random.default_rng(seed).choice(n, missing_records, replace=False)] = False
+        y, aid = y[keep], aid[keep]
+        cg = [c for c, k in zip(cg, keep) if k]
+    fd = design.build_fixed_design([[c] for c in cg])
+    return fd, aid, y, Q
+
+
+def ebv_err(u, u_ref):
+    return np.abs(u - u_ref).max() / np.abs(u_ref).max()
+
+
+def test_operator_matches_dense(cb, ctx):
+    fd, aid, y, Q = gblup_problem()
+    M = mme.MixedModel(fd, aid, y, Q.shape[0])
+    M.set_qinv_dense(Q)
+    C, rhs, _ = orl.mme_dense(design.dense_x(fd), aid, y, Q, 2.3)
+    assert np.allclose(M.rhs(), rhs, rtol=1e-14, atol=1e-12)
+    Xb = np.random.default_rng(0).standard_normal((M.dim, 70))
+    for r in (1, 2, 3, 70):
+        Y = M.apply(2.3, Xb[:, :r])
+        ref = C @ Xb[:, :r]
+        assert np.abs(Y - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("precision", [_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED])
+@pytest.mark.parametrize("precond", [_lib.CB_PRECOND_JACOBI, _lib.CB_PRECOND_BLOCK_JACOBI])
+def test_pcg_solves_match_dense(cb, ctx, precision, precond):
+    fd, aid, y, Q = gblup_problem(missing_records=17)
+    M = mme.MixedModel(fd, aid, y, Q.shape[0])
+    M.set_qinv_dense(Q)
+    lam = 1.7
+    C, rhs, _ = orl.mme_dense(design.dense_x(fd), aid, y, Q, lam)
+    B = np.zeros((M.dim, 5))
+    B[:, 0] = rhs
+    B[M.p:, 1:] = orl.probes(M.q, 4, seed=3)
+    X, it, rel = M.solve(lam, B, mme.solve_options(precision=precision, precond=precond, tol=1e-12))
+    ref = np.linalg.solve(C, B)
+    assert rel < 1e-12
+    for c in range(5):
+        assert ebv_err(X[M.p:, c], ref[M.p:, c]) < EBV_TOL * 1e-2
+    # fixed effects are not identifiable beyond X b: compare fitted values
+    Xd = design.dense_x(fd)
+    np.testing.assert_allclose(Xd @ X[:M.p, 0], Xd @ ref[:M.p, 0], atol=1e-8)
+
+
+def test_plain_unbatched_equals_batched(cb, ctx):
+    fd, aid, y, Q = gblup_problem(n=300, m=1500)
+    M = mme.MixedModel(fd, aid, y, Q.shape[0])
+    M.set_qinv_dense(Q)
+    B = np.zeros((M.dim, 3))
+    B[:, 0] = M.rhs()
+    B[M.p:, 1:] = orl.probes(M.q, 2, seed=8)
+    X1, _, _ = M.solve(1.1, B, mme.solve_options(precision=_lib.CB_PREC_FP64, batch_rhs=True, tol=1e-12))
+    X2, _, _ = M.solve(1.1, B, mme.solve_options(precision=_lib.CB_PREC_FP64, batch_rhs=False, tol=1e-12))
+    assert np.abs(X1 - X2).max() < 1e-9 * np.abs(X1).max()
+
+
+def test_absolute_stop_rule(cb, ctx):
+    fd, aid, y, Q = gblup_problem(n=200, m=800)
+    M = mme.MixedModel(fd, aid, y, Q.shape[0])
+    M.set_qinv_dense(Q)
+    b = M.rhs()
+    X, it, res = M.solve(2.0, b, mme.solve_options(precision=_lib.CB_PREC_FP64,
+                                                  stop_rule=_lib.CB_STOP_ABSOLUTE, tol=1e-6))
+    C, _, _ = orl.mme_dense(design.dense_x(fd), aid, y, Q, 2.0)
+    assert np.linalg.norm(C @ X - b) < 1e-6
+
+
+def test_iteration_cap_reports_not_converged(cb, ctx):
+    fd, aid, y, Q = gblup_problem(n=200, m=800)
+    M = mme.MixedModel(fd, aid, y, Q.shape[0])
+    M.set_qinv_dense(Q)
+    with pytest.raises(cb.NotConverged):
+        M.solve(2.0, M.rhs(), mme.solve_options(precision=_lib.CB_PREC_FP64, tol=1e-14, max_iter=2))
+
+
+def test_probes_match_oracle(cb, ctx):
+    fd, aid, y, Q = gblup_problem(n=150, m=600)
+    M = mme.MixedModel(fd, aid, y, Q.shape[0])
+    M.set_qinv_dense(Q)
+    R = mme.AIREML(M, mme.reml_options(n_probes=9, seed=1234))
+    assert np.array_equal(R.probes(), orl.probes(M.q, 9, 1234))
+
+
+@pytest.mark.parametrize("precision", [_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED])
+def test_score_and_ai_match_oracle(cb, ctx, precision):
+    fd, aid, y, Q = gblup_problem()
+    M = mme.MixedModel(fd, aid, y, Q.shape[0])
+    M.set_qinv_dense(Q)
+    opts = mme.reml_options(n_probes=16, seed=77,
+                            solve=mme.solve_options(precision=precision, tol=1e-12))
+    R = mme.AIREML(M, opts)
+    g, ai, tr, sol = R.score(0.5, 0.8)
+    ref = orl.reml_eval(design.dense_x(fd), aid, y, Q, 0.5, 0.8, orl.probes(M.q, 16, 77))
+    np.testing.assert_allclose(tr, ref["tr"], rtol=1e-8)
+    np.testing.assert_allclose(g, ref["grad"], rtol=1e-7, atol=1e-9 * np.abs(ref["grad"]).max())
+    np.testing.assert_allclose(ai, ref["ai"], rtol=1e-7)
+    assert ebv_err(sol[M.p:], ref["sol"][M.p:]) < EBV_TOL
+
+
+def test_exact_trace_score_matches_fd_of_logL(cb, ctx):
+    fd, aid, y, Q = gblup_problem(n=120, m=600, ngroups=3)
+    M = mme.MixedModel(fd, aid, y, Q.shape[0])
+    M.set_qinv_dense(Q)
+    R = mme.AIREML(M, mme.reml_options(trace_exact=True, solve=mme.solve_options(tol=1e-12)))
+    vu, ve = 0.45, 0.7
+    g, _, _, _ = R.score(vu, ve, want_ai=False)
+    X = design.dense_x(fd)
+    h = 1e-4
+    fd_u = (orl.logL(X, aid, y, Q, vu * (1 + h), ve) - orl.logL(X, aid, y, Q, vu * (1 - h), ve)) / (2 * h * vu)
+    fd_e = (orl.logL(X, aid, y, Q, vu, ve * (1 + h)) - orl.logL(X, aid, y, Q, vu, ve * (1 - h))) / (2 * h * ve)
+    np.testing.assert_allclose(g, [fd_u, fd_e], rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("algorithm", [_lib.CB_VCE_AI, _lib.CB_VCE_EM])
+def test_reml_run_matches_oracle(cb, ctx, algorithm):
+    fd, aid, y, Q = gblup_problem()
+    M = mme.MixedModel(fd, aid, y, Q.shape[0])
+    M.set_qinv_dense(Q)
+    k, seed = 32, 5
+    opts = mme.reml_options(algorithm=algorithm, n_probes=k, seed=seed, max_iter=20,
+                            solve=mme.solve_options(tol=1e-12))
+    R = mme.AIREML(M, opts)
+    R.solve()
+    ref = orl.reml_run(design.dense_x(fd), aid, y, Q, algorithm="ai" if algorithm == _lib.CB_VCE_AI else "em",
+                       max_iter=20, Zp=orl.probes(M.q, k, seed))
+    assert R.iterations == len(ref["hist"])
+    np.testing.assert_allclose([R.var_u, R.var_e], [ref["vu"], ref["ve"]], rtol=VC_TOL)
+    assert ebv_err(R.solution[M.p:], ref["sol"][M.p:]) < EBV_TOL
+
+
+def test_pblup_csr_ainv(cb, ctx):
+    """PBLUP with Henderson A^-1 as CSR (SparseMatrixAdapter path)."""
+    rng = np.random.default_rng(2)
+    nan = 600
+    sire = np.full(nan, -1)
+    dam = np.full(nan, -1)
+    for i in range(100, nan):
+        sire[i] = rng.integers(0, 50)
+        dam[i] = rng.integers(50, 100)
+    Ai = orl.henderson_ainv(sire, dam)
+    nz = np.nonzero(Ai)
+    ptr = np.zeros(nan + 1, dtype=np.int64)
+    np.add.at(ptr, nz[0] + 1, 1)
+    ptr = np.cumsum(ptr)
+    col = nz[1].astype(np.int32)
+    val = Ai[nz]
+    rec = rng.choice(nan, 450, replace=False).astype(np.int32)
+    y = rng.standard_normal(rec.size) + 3.0
+    cats = [[f"g{int(v)}"] for v in rng.integers(0, 5, rec.size)]
+    fd = design.build_fixed_design(cats)
+    M = mme.MixedModel(fd, rec, y, nan)
+    M.set_qinv_csr(ptr, col, val)
+    C, rhs, _ = orl.mme_dense(design.dense_x(fd), rec, y, Ai, 1.5)
+    X, _, _ = M.solve(1.5, rhs, mme.solve_options(tol=1e-12))
+    ref = np.linalg.solve(C, rhs)
+    assert ebv_err(X[M.p:], ref[M.p:]) < EBV_TOL
+    opts = mme.reml_options(n_probes=16, seed=3, solve=mme.solve_options(tol=1e-12))
+    R = mme.AIREML(M, opts)
+    R.solve()
+    o = orl.reml_run(design.dense_x(fd), rec, y, Ai, Zp=orl.probes(nan, 16, 3))
+    np.testing.assert_allclose([R.var_u, R.var_e], [o["vu"], o["ve"]], rtol=VC_TOL)

@@ -1,0 +1,126 @@
+"""Pins the MME / REML oracle (oracle/reml.py) against facts that do not
+restate it: V-form BLUP/GLS, central finite differences of the REML
+log-likelihood, the EM = AI fixed point, Henderson's A^-1 against the inverse
+of the tabular A, the reference's 4-animal pedigree fixture, and the probe
+stream against a scalar splitmix64."""
+import numpy as np
+import pytest
+
+from cosmicblup_amd import design, synth
+from oracle import grm as og
+from oracle import reml as orl
+
+
+def small_problem(n=60, m=300, ngroups=4, seed=3):
+    x, _ = synth.simulate_dosages(n, m, maf=(0.05, 0.5), seed=seed)
+    bed = synth.encode_bed(x)
+    G, _, _ = og.grm_vanraden(bed, n, m)
+    Q = og.invert_psd_matrix(G)
+    y, cg = synth.simulate_phenotype(x, n_qtl=30, h2=0.4, n_groups=ngroups, seed=seed + 1)
+    fd = design.build_fixed_design([[c] for c in cg])
+    X = design.dense_x(fd)
+    aid = np.arange(n, dtype=np.int32)
+    return X, aid, y, Q, G
+
+
+def test_design_levels_string_sorted_first_dropped():
+    fd = design.build_fixed_design([["b"], ["a"], ["c"], ["a"], ["10"], ["9"]])
+    # std::set<string> order: "10" < "9" < "a" < "b" < "c"; "10" is the base
+    assert fd.names == ["mu", "factor1_9", "factor1_a", "factor1_b", "factor1_c"]
+    X = design.dense_x(fd)
+    assert X[4].tolist() == [1, 0, 0, 0, 0]
+    assert X[0].tolist() == [1, 0, 0, 1, 0]
+
+
+def test_mme_solution_equals_vform_blup():
+    X, aid, y, Q, G = small_problem()
+    vu, ve = 0.7, 1.3
+    C, rhs, W = orl.mme_dense(X, aid, y, Q, ve / vu)
+    sol = np.linalg.lstsq(C, rhs, rcond=None)[0]
+    Gb = og.bent_matrix(G)  # Q^-1
+    V = vu * Gb + ve * np.eye(len(y))
+    Vi = np.linalg.inv(V)
+    XtViX = X.T @ Vi @ X
+    b = np.linalg.lstsq(XtViX, X.T @ Vi @ y, rcond=None)[0]
+    u = vu * Gb @ Vi @ (y - X @ b)
+    np.testing.assert_allclose(sol[X.shape[1]:], u, atol=1e-7 * np.abs(u).max())
+    np.testing.assert_allclose(X @ sol[:X.shape[1]], X @ b, atol=1e-7)
+
+
+def test_score_matches_central_fd_of_logL():
+    X, aid, y, Q, _ = small_problem()
+    vu, ve = 0.6, 0.9
+    s = orl.reml_eval(X, aid, y, Q, vu, ve, np.eye(Q.shape[0]), exact=True)
+    h = 1e-4
+    fd_u = (orl.logL(X, aid, y, Q, vu * (1 + h), ve) - orl.logL(X, aid, y, Q, vu * (1 - h), ve)) / (2 * h * vu)
+    fd_e = (orl.logL(X, aid, y, Q, vu, ve * (1 + h)) - orl.logL(X, aid, y, Q, vu, ve * (1 - h))) / (2 * h * ve)
+    np.testing.assert_allclose(s["grad"], [fd_u, fd_e], rtol=1e-5, atol=1e-6)
+
+
+def test_ai_matrix_is_spd_average_information():
+    # AI_ij = 1/2 v_i' P v_j with P positive semidefinite -> SPD 2x2
+    X, aid, y, Q, _ = small_problem()
+    s = orl.reml_eval(X, aid, y, Q, 0.6, 0.9, np.eye(Q.shape[0]), exact=True)
+    assert np.all(np.linalg.eigvalsh(s["ai"]) > 0)
+    assert np.allclose(s["ai"], s["ai"].T)
+
+
+def test_em_and_ai_share_the_fixed_point():
+    X, aid, y, Q, _ = small_problem()
+    Zp = orl.probes(Q.shape[0], 16, seed=5)
+    ai = orl.reml_run(X, aid, y, Q, algorithm="ai", max_iter=50, tol=1e-10, Zp=Zp)
+    em = orl.reml_run(X, aid, y, Q, algorithm="em", max_iter=3000, tol=1e-12, Zp=Zp)
+    assert ai["converged"]
+    np.testing.assert_allclose([ai["vu"], ai["ve"]], [em["vu"], em["ve"]], rtol=2e-5)
+    s = orl.reml_eval(X, aid, y, Q, ai["vu"], ai["ve"], Zp, want_ai=False)
+    assert np.abs(s["grad"]).max() < 1e-5
+
+
+def test_probe_stream_scalar():
+    def sm(x):
+        M = (1 << 64) - 1
+        x = (x + 0x9E3779B97F4A7C15) & M
+        x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & M
+        x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & M
+        return x ^ (x >> 31)
+
+    q, k, seed = 7, 3, 42
+    Z = orl.probes(q, k, seed)
+    for c in range(k):
+        for j in range(q):
+            h = sm((seed + 0x9E3779B97F4A7C15 * (c * q + j + 1)) & ((1 << 64) - 1))
+            assert Z[j, c] == (-1.0 if h >> 63 else 1.0)
+    assert set(np.unique(orl.probes(50, 8, 1))) == {-1.0, 1.0}
+
+
+def test_hutchinson_unbiased():
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((40, 40))
+    A = A @ A.T
+    Zp = orl.probes(40, 4000, seed=9)
+    est = np.einsum("ij,ij->j", Zp, A @ Zp).mean()
+    assert abs(est - np.trace(A)) < 4 * np.sqrt(2 * (A * A).sum() / 4000)
+
+
+def test_henderson_ainv_inverts_tabular_a():
+    rng = np.random.default_rng(4)
+    nan = 40
+    sire = np.full(nan, -1)
+    dam = np.full(nan, -1)
+    for i in range(10, nan):
+        sire[i] = rng.integers(0, 10) if rng.random() > 0.1 else -1
+        dam[i] = rng.integers(0, 10) if rng.random() > 0.1 else -1
+        if sire[i] == dam[i] and sire[i] >= 0:
+            dam[i] = -1
+    A = orl.tabular_a(sire, dam)
+    # no inbreeding here (parents are unrelated founders)
+    np.testing.assert_allclose(orl.henderson_ainv(sire, dam) @ A, np.eye(nan), atol=1e-12)
+
+
+def test_pblup_minimal_fixture_ainv_by_hand():
+    # tests/fixtures/pblup_minimal/pedigree.txt: id3, id4 full sibs of id1 x id2
+    sire = np.array([-1, -1, 1, 1])   # column "sire" = id2
+    dam = np.array([-1, -1, 0, 0])    # column "dam" = id1
+    Ai = orl.henderson_ainv(sire, dam)
+    expect = np.array([[2, 1, -1, -1], [1, 2, -1, -1], [-1, -1, 2, 0], [-1, -1, 0, 2]], dtype=float)
+    np.testing.assert_allclose(Ai, expect)
