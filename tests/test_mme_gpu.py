@@ -62,9 +62,12 @@ def test_pcg_solves_match_dense(cb, ctx, precision, precond):
     B = np.zeros((M.dim, 5))
     B[:, 0] = rhs
     B[M.p:, 1:] = orl.probes(M.q, 4, seed=3)
-    X, it, rel = M.solve(lam, B, mme.solve_options(precision=precision, precond=precond, tol=1e-12))
+    # the mixed path tests the true FP64 residual, whose rounding floor sits
+    # near 1e-12 here; the FP64 path tests the recursive residual
+    tol = 1e-12 if precision == _lib.CB_PREC_FP64 else 1e-10
+    X, it, rel = M.solve(lam, B, mme.solve_options(precision=precision, precond=precond, tol=tol))
     ref = np.linalg.solve(C, B)
-    assert rel < 1e-12
+    assert rel < tol
     for c in range(5):
         assert ebv_err(X[M.p:, c], ref[M.p:, c]) < EBV_TOL * 1e-2
     # fixed effects are not identifiable beyond X b: compare fitted values
@@ -117,7 +120,8 @@ def test_score_and_ai_match_oracle(cb, ctx, precision):
     M = mme.MixedModel(fd, aid, y, Q.shape[0])
     M.set_qinv_dense(Q)
     opts = mme.reml_options(n_probes=16, seed=77,
-                            solve=mme.solve_options(precision=precision, tol=1e-12))
+                            solve=mme.solve_options(precision=precision,
+                                                    tol=1e-12 if precision == _lib.CB_PREC_FP64 else 1e-10))
     R = mme.AIREML(M, opts)
     g, ai, tr, sol = R.score(0.5, 0.8)
     ref = orl.reml_eval(design.dense_x(fd), aid, y, Q, 0.5, 0.8, orl.probes(M.q, 16, 77))
@@ -131,7 +135,7 @@ def test_exact_trace_score_matches_fd_of_logL(cb, ctx):
     fd, aid, y, Q = gblup_problem(n=120, m=600, ngroups=3)
     M = mme.MixedModel(fd, aid, y, Q.shape[0])
     M.set_qinv_dense(Q)
-    R = mme.AIREML(M, mme.reml_options(trace_exact=True, solve=mme.solve_options(tol=1e-12)))
+    R = mme.AIREML(M, mme.reml_options(trace_exact=True, solve=mme.solve_options(tol=1e-10)))
     vu, ve = 0.45, 0.7
     g, _, _, _ = R.score(vu, ve, want_ai=False)
     X = design.dense_x(fd)
@@ -148,7 +152,7 @@ def test_reml_run_matches_oracle(cb, ctx, algorithm):
     M.set_qinv_dense(Q)
     k, seed = 32, 5
     opts = mme.reml_options(algorithm=algorithm, n_probes=k, seed=seed, max_iter=20,
-                            solve=mme.solve_options(tol=1e-12))
+                            solve=mme.solve_options(tol=1e-10))
     R = mme.AIREML(M, opts)
     R.solve()
     ref = orl.reml_run(design.dense_x(fd), aid, y, Q, algorithm="ai" if algorithm == _lib.CB_VCE_AI else "em",
@@ -181,10 +185,10 @@ def test_pblup_csr_ainv(cb, ctx):
     M = mme.MixedModel(fd, rec, y, nan)
     M.set_qinv_csr(ptr, col, val)
     C, rhs, _ = orl.mme_dense(design.dense_x(fd), rec, y, Ai, 1.5)
-    X, _, _ = M.solve(1.5, rhs, mme.solve_options(tol=1e-12))
+    X, _, _ = M.solve(1.5, rhs, mme.solve_options(tol=1e-10))
     ref = np.linalg.solve(C, rhs)
     assert ebv_err(X[M.p:], ref[M.p:]) < EBV_TOL
-    opts = mme.reml_options(n_probes=16, seed=3, solve=mme.solve_options(tol=1e-12))
+    opts = mme.reml_options(n_probes=16, seed=3, solve=mme.solve_options(tol=1e-10))
     R = mme.AIREML(M, opts)
     R.solve()
     o = orl.reml_run(design.dense_x(fd), rec, y, Ai, Zp=orl.probes(nan, 16, 3))
