@@ -249,32 +249,63 @@ __global__ __launch_bounds__(256) void k_miss_sample_lists(
     }
 }
 
-// E(k, i) = sum_{j miss k} mu_j X_ij + F(k,i)/2 for the 1024 samples of a
-// block (each lane owns 16 consecutive samples = one dword of a BED row).
-// Four list entries (SNPs j0..j3 of miss(k)) are consumed per step: the four
-// row dwords are byte-transposed with v_perm so that, for each byte q,
-// W_q holds samples 4q..4q+3 of all four SNPs; the dosage bytes are then
-// dotted (v_dot4_u32_u8) with the four byte chunks of K_j — exact integers.
-// F(k,i) = sum over SNPs missing in both k and i of mu_j^2: such calls are
-// rare (the missing rate squared), so they take a per-lane slow path that
-// adds (K_j^2 >> 6) into lane-private u64 LDS slots.
+// Rows of the BED re-encoded for k_miss_cross: each 2-bit field holds the
+// dosage (0, 1, 2), or 3 for a missing call.
+__global__ void k_encode_d3(const uint8_t* __restrict__ bed, uint8_t* __restrict__ out, int64_t pitch,
+                            int64_t n, int64_t m) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // dword within the row
+    const int64_t j = blockIdx.y;
+    if (w * 4 >= pitch || j >= m) return;
+    const uint32_t x = reinterpret_cast<const uint32_t*>(bed + j * pitch)[w];
+    const uint32_t vm = field_mask(n, w);
+    const uint32_t miss = ~(x >> 1) & x & vm;
+    const uint32_t d = dosage_swar(x) & (vm | (vm << 1));
+    reinterpret_cast<uint32_t*>(out + j * pitch)[w] = d | miss | (miss << 1);
+}
+
+// E for the 1024 samples i of a block and the rows k of this wave, with
+// E(k,i) + E(i,k) = e(k,i) + e(i,k) + F(k,i) as the Gram epilogue needs,
+// e(k,i) = sum_{j miss k} mu_j X_ij and F(k,i) = sum_{j miss k and i} mu_j^2.
+// Each lane owns 16 consecutive samples = one dword of a row of the
+// re-encoded BED (field = dosage, or 3 for missing). Four list entries
+// (SNPs of miss(k)) are consumed per step: the four row dwords are
+// byte-transposed with v_perm so that W_q holds samples 4q..4q+3 of all
+// four SNPs; sample s of W_q is selected in place with 0x03030303 << 2s and
+// dotted (v_dot4_u32_u8) with the four byte chunks of K_j — exact integers
+// that carry a factor 4^s, shifted out at the end.
+// A call missing in both k and i entered the dot as 3 K_j. Such events
+// are rare (the missing rate squared) and symmetric in (k, i), so each is
+// taken once, in row k for i > k, adding mu_j^2 - 6 mu_j (signed 2^-48
+// units, lane-private LDS). The diagonal is set directly: E(k,k) = s_k/2.
 // Every element E(k, i < n) is written (this initialises E).
-__global__ __launch_bounds__(256) void k_miss_cross(const uint8_t* __restrict__ bed,
+__global__ __launch_bounds__(256) void k_miss_cross(const uint8_t* __restrict__ bedd,
                                                     int64_t pitch, int64_t n,
                                                     const int64_t* __restrict__ smp_off,
                                                     const int32_t* __restrict__ lj,
-                                                    const uint32_t* __restrict__ lk, int kw,
+                                                    const uint32_t* __restrict__ lk,
+                                                    const unsigned long long* __restrict__ sq, int kw,
                                                     double* __restrict__ E, int64_t lde) {
     __shared__ unsigned long long fl[4 * 16 * 64];  // [wave][sample][lane], 32 KB
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int64_t I0 = (int64_t)blockIdx.y * 1024;
     const int64_t dw = I0 / 16 + lane;  // dword of each SNP row owned by this lane
-    const bool inrow = dw * 4 < pitch;
+    const int64_t i_lane = I0 + 16 * lane;
+    const int64_t dwc = dw * 4 < pitch ? dw : 0;  // clamped in-row dword
     unsigned long long* myf = fl + wave * 16 * 64 + lane;
     const int per_wave = kw / 4;
     for (int kq = 0; kq < per_wave; ++kq) {
         const int64_t k = (int64_t)blockIdx.x * kw + wave * per_wave + kq;
         if (k >= n) break;
+        // event masks: field positions (2s + 8t) of W_q whose sample index exceeds k
+        uint32_t mk[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint32_t mq = 0;
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                if (i_lane + 4 * q + s > k) mq |= 0x01010101u << (2 * s);
+            mk[q] = mq;
+        }
 #pragma unroll
         for (int s = 0; s < 16; ++s) myf[s * 64] = 0ull;
         uint32_t acc[16][4];
@@ -283,83 +314,104 @@ __global__ __launch_bounds__(256) void k_miss_cross(const uint8_t* __restrict__ 
 #pragma unroll
             for (int c = 0; c < 4; ++c) acc[s][c] = 0;
         const int64_t b = smp_off[k], e = smp_off[k + 1];
-        // lists are padded to a multiple of 8: two groups of four per step,
-        // all eight row loads issued before either group is consumed
-        for (int64_t p = b; p < e; p += 8) {
-            uint32_t cw[2][4], Kv[2][4];
+        // One step = two groups of four list entries (lists are padded to a
+        // multiple of 8). Lanes past the row read an in-row dword instead;
+        // their samples are >= n and never written.
+        auto fetch = [&](int64_t pp, uint32_t (&c)[8]) {
 #pragma unroll
-            for (int g = 0; g < 2; ++g)
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const int32_t j = lj[p + 4 * g + t];
-                    Kv[g][t] = lk[p + 4 * g + t];
-                    cw[g][t] = inrow ? reinterpret_cast<const uint32_t*>(bed + (int64_t)j * pitch)[dw] : 0u;
-                }
+            for (int t = 0; t < 8; ++t) {
+                const int32_t jj = lj[pp + t];
+                c[t] = reinterpret_cast<const uint32_t*>(bedd + (int64_t)jj * pitch)[dwc];
+            }
+        };
+        auto step = [&](int64_t pp, const uint32_t (&cw)[8]) {
 #pragma unroll
             for (int g = 0; g < 2; ++g) {
-            const uint32_t c0 = cw[g][0], c1 = cw[g][1], c2 = cw[g][2], c3 = cw[g][3];
-            const uint32_t K0 = Kv[g][0], K1 = Kv[g][1], K2 = Kv[g][2], K3 = Kv[g][3];
-            // 4x4 byte transpose: W[q] byte t = byte q of c_t
-            const uint32_t ta = __builtin_amdgcn_perm(c1, c0, 0x05010400u);
-            const uint32_t tb = __builtin_amdgcn_perm(c1, c0, 0x07030602u);
-            const uint32_t tc = __builtin_amdgcn_perm(c3, c2, 0x05010400u);
-            const uint32_t td = __builtin_amdgcn_perm(c3, c2, 0x07030602u);
-            uint32_t W[4];
-            W[0] = __builtin_amdgcn_perm(tc, ta, 0x05040100u);
-            W[1] = __builtin_amdgcn_perm(tc, ta, 0x07060302u);
-            W[2] = __builtin_amdgcn_perm(td, tb, 0x05040100u);
-            W[3] = __builtin_amdgcn_perm(td, tb, 0x07060302u);
-            uint32_t kc[4];
+                const uint32_t c0 = cw[4 * g], c1 = cw[4 * g + 1], c2 = cw[4 * g + 2], c3 = cw[4 * g + 3];
+                const uint32_t K0 = lk[pp + 4 * g], K1 = lk[pp + 4 * g + 1], K2 = lk[pp + 4 * g + 2],
+                               K3 = lk[pp + 4 * g + 3];
+                // 4x4 byte transpose: W[q] byte t = byte q of c_t
+                const uint32_t ta = __builtin_amdgcn_perm(c1, c0, 0x05010400u);
+                const uint32_t tb = __builtin_amdgcn_perm(c1, c0, 0x07030602u);
+                const uint32_t tc = __builtin_amdgcn_perm(c3, c2, 0x05010400u);
+                const uint32_t td = __builtin_amdgcn_perm(c3, c2, 0x07030602u);
+                uint32_t W[4];
+                W[0] = __builtin_amdgcn_perm(tc, ta, 0x05040100u);
+                W[1] = __builtin_amdgcn_perm(tc, ta, 0x07060302u);
+                W[2] = __builtin_amdgcn_perm(td, tb, 0x05040100u);
+                W[3] = __builtin_amdgcn_perm(td, tb, 0x07060302u);
+                uint32_t kc[4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
-                kc[c] = ((K0 >> (8 * c)) & 0xffu) | (((K1 >> (8 * c)) & 0xffu) << 8) |
-                        (((K2 >> (8 * c)) & 0xffu) << 16) | (((K3 >> (8 * c)) & 0xffu) << 24);
-            uint32_t mf[4];
-            uint32_t anym = 0;
+                for (int c = 0; c < 4; ++c)
+                    kc[c] = ((K0 >> (8 * c)) & 0xffu) | (((K1 >> (8 * c)) & 0xffu) << 8) |
+                            (((K2 >> (8 * c)) & 0xffu) << 16) | (((K3 >> (8 * c)) & 0xffu) << 24);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t D = dosage_swar(W[q]);
-                mf[q] = W[q] & ~(W[q] >> 1) & 0x55555555u;  // code 01
-                anym |= mf[q];
+                for (int q = 0; q < 4; ++q)
 #pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const uint32_t A = (D >> (2 * s)) & 0x03030303u;
+                    for (int s = 0; s < 4; ++s) {
+                        const uint32_t A = W[q] & (0x03030303u << (2 * s));
 #pragma unroll
-                    for (int c = 0; c < 4; ++c)
-                        acc[4 * q + s][c] = __builtin_amdgcn_udot4(A, kc[c], acc[4 * q + s][c], false);
-                }
-            }
-            if (anym) {
-                const unsigned long long q0 = ((unsigned long long)K0 * K0) >> 6;
-                const unsigned long long q1 = ((unsigned long long)K1 * K1) >> 6;
-                const unsigned long long q2 = ((unsigned long long)K2 * K2) >> 6;
-                const unsigned long long q3 = ((unsigned long long)K3 * K3) >> 6;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    uint32_t mm = mf[q];
-                    while (mm) {
-                        const int bit = __ffs(mm) - 1;
-                        mm &= mm - 1;
-                        const int t = bit >> 3, s = (bit >> 1) & 3;
-                        const unsigned long long add = t == 0 ? q0 : t == 1 ? q1 : t == 2 ? q2 : q3;
-                        myf[(4 * q + s) * 64] += add;
+                        for (int c = 0; c < 4; ++c)
+                            acc[4 * q + s][c] = __builtin_amdgcn_udot4(A, kc[c], acc[4 * q + s][c], false);
                     }
+                // events: field 3 (missing in both) at a sample > k; q = 0,1 in
+                // the even/odd bits of ev0, q = 2,3 in those of ev1
+                uint32_t ev0 = (W[0] & (W[0] >> 1) & mk[0]) | ((W[1] & (W[1] >> 1) & mk[1]) << 1);
+                uint32_t ev1 = (W[2] & (W[2] >> 1) & mk[2]) | ((W[3] & (W[3] >> 1) & mk[3]) << 1);
+                while (ev0 | ev1) {
+                    const bool lo = ev0 != 0;
+                    const uint32_t ev = lo ? ev0 : ev1;
+                    const int bit = __ffs(ev) - 1;
+                    if (lo) ev0 &= ev0 - 1; else ev1 &= ev1 - 1;
+                    const int t = bit >> 3, smp = (lo ? 0 : 8) + 4 * (bit & 1) + ((bit >> 1) & 3);
+                    const uint32_t Kt = t == 0 ? K0 : t == 1 ? K1 : t == 2 ? K2 : K3;
+                    // mu^2 - 6 mu in 2^-48 units (two's complement in the u64 slot)
+                    const unsigned long long add =
+                        (((unsigned long long)Kt * Kt) >> 10) - ((unsigned long long)Kt * 6 << 19);
+                    atomicAdd(&myf[smp * 64], add);
                 }
             }
-            }
+        };
+        uint32_t ra[8], rb[8];
+        if (b < e) fetch(b, ra);
+        for (int64_t p = b; p < e; p += 16) {
+            const bool m1 = p + 8 < e;
+            if (m1) fetch(p + 8, rb);
+            step(p, ra);
+            if (!m1) break;
+            if (p + 16 < e) fetch(p + 16, ra);
+            step(p + 8, rb);
         }
 #pragma unroll
         for (int s = 0; s < 16; ++s) {
-            const int64_t i = I0 + 16 * lane + s;
+            const int64_t i = i_lane + s;
             if (i >= n) continue;
-            const uint64_t v = (uint64_t)acc[s][0] + ((uint64_t)acc[s][1] << 8) +
-                               ((uint64_t)acc[s][2] << 16) + ((uint64_t)acc[s][3] << 24);
-            E[k * lde + i] = (double)v * TWO_M29 + (double)myf[s * 64] * (0.5 / 4503599627370496.0);
+            double v;
+            if (i == k) {
+                v = (double)sq[k] * (0.5 / 4503599627370496.0);  // s_k / 2
+            } else {
+                const uint64_t T = ((uint64_t)acc[s][0] + ((uint64_t)acc[s][1] << 8) +
+                                    ((uint64_t)acc[s][2] << 16) + ((uint64_t)acc[s][3] << 24)) >> (2 * (s & 3));
+                v = (double)T * TWO_M29 + (double)(long long)myf[s * 64] * (1.0 / 281474976710656.0);
+            }
+            E[k * lde + i] = v;
         }
     }
 }
 
 // ---------------------------------------------------------------------------
+
+// One 32x32x32 int8 fragment (16 bytes per lane) from one packed word:
+// byte t of dword q is the 2-bit field at bits 8t+2q.
+__device__ __forceinline__ v4i decode16(uint32_t w) {
+    return (v4i){(int)(w & 0x03030303u), (int)((w >> 2) & 0x03030303u),
+                 (int)((w >> 4) & 0x03030303u), (int)((w >> 6) & 0x03030303u)};
+}
+__device__ __forceinline__ uint32_t word_of(const u32x4 (&v)[2], int kk) {
+    const u32x4 x = v[kk >> 2];
+    return (kk & 3) == 0 ? x.x : (kk & 3) == 1 ? x.y : (kk & 3) == 2 ? x.z : x.w;
+}
+
 // INT8-MFMA Gram over the lower triangle of 256x256 tiles.
 // 4 waves in 2x2, each 128x128 = 4x4 blocks of v_mfma_i32_32x32x32_i8.
 // A 32x32x32 fragment takes 16 int8 per lane = one packed word decoded with
@@ -439,30 +491,39 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
             wb[b][0] = p[0];
             wb[b][1] = p[1];
         }
+        // Software-pipelined k-steps: the decode of step kk+1 (VALU) is
+        // interleaved with the 16 MFMAs of step kk, four VALU per MFMA.
+        v4i fa[4], fb[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) fa[a] = decode16(word_of(wa[a], 0));
+#pragma unroll
+        for (int b = 0; b < 4; ++b) fb[b] = decode16(word_of(wb[b], 0));
 #pragma unroll
         for (int kk = 0; kk < 8; ++kk) {
-            v4i fa[4], fb[4];
+            v4i na[4], nb[4];
+            if (kk < 7) {
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const u32x4 v = wa[a][kk >> 2];
-                const uint32_t w = (kk & 3) == 0 ? v.x : (kk & 3) == 1 ? v.y : (kk & 3) == 2 ? v.z : v.w;
-                fa[a] = (v4i){(int)(w & 0x03030303u), (int)((w >> 2) & 0x03030303u),
-                              (int)((w >> 4) & 0x03030303u), (int)((w >> 6) & 0x03030303u)};
-            }
+                for (int a = 0; a < 4; ++a) na[a] = decode16(word_of(wa[a], kk + 1));
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const u32x4 v = wb[b][kk >> 2];
-                const uint32_t w = (kk & 3) == 0 ? v.x : (kk & 3) == 1 ? v.y : (kk & 3) == 2 ? v.z : v.w;
-                fb[b] = (v4i){(int)(w & 0x03030303u), (int)((w >> 2) & 0x03030303u),
-                              (int)((w >> 4) & 0x03030303u), (int)((w >> 6) & 0x03030303u)};
+                for (int b = 0; b < 4; ++b) nb[b] = decode16(word_of(wb[b], kk + 1));
             }
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int b = 0; b < 4; ++b)
                     acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
-            // keep each k-step's decode next to its MFMAs (bounds live VGPRs)
+            if (kk < 7) {
+#pragma unroll
+                for (int g = 0; g < 16; ++g) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+                    __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // four VALU
+                }
+            }
             __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) fa[a] = na[a];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) fb[b] = nb[b];
         }
         if (s + 1 < n_stages) {
             u32x4* la = reinterpret_cast<u32x4*>(ldsA + ((s + 1) & 1) * PANELS_WORDS + tid * LDS_PITCH);
@@ -692,10 +753,16 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
         CB_HIP(hipMemcpyAsync(u.p, hu.data(), n * 8, hipMemcpyHostToDevice, st));
 
         E.alloc((size_t)n * n);
+        DevBuf<uint8_t> bedd;
+        bedd.alloc((size_t)g->pitch * m);
+        timed(ctx, "k_encode_d3", [&] {
+            k_encode_d3<<<dim3((unsigned)ceil_div(g->pitch / 4, 256), (unsigned)m), 256, 0, st>>>(
+                g->bed.p, bedd.p, g->pitch, n, m);
+        });
         const int kw = 64;
         timed(ctx, "k_miss_cross", [&] {
             k_miss_cross<<<dim3((unsigned)ceil_div(n, kw), (unsigned)ceil_div(n, 1024)), dim3(256), 0, st>>>(
-                g->bed.p, g->pitch, n, d_smp_off.p, lj.p, lk.p, kw, E.p, n);
+                bedd.p, g->pitch, n, d_smp_off.p, lj.p, lk.p, sq.p, kw, E.p, n);
         });
         CB_HIP(hipStreamSynchronize(st));  // list buffers die at scope end
     }
