@@ -48,6 +48,116 @@ def cpu_baseline(n_cpu: int = 1536, m_cpu: int = 50000, seed: int = 4) -> dict:
                       f"({dt:.1f} s; unique-pair op count m*n(n+1))"}
 
 
+def decode_rows_dosage(rows, pitch: int, n: int, snps):
+    """Dosage matrix (n x len(snps), float64, missing -> 0) of selected SNP rows
+    of a device BED (torch ops; setup only)."""
+    import torch
+
+    sel = rows[snps.to(rows.device)]  # k x pitch bytes
+    codes = torch.stack([(sel >> (2 * t)) & 3 for t in range(4)], dim=2).reshape(sel.shape[0], -1)[:, :n]
+    table = torch.tensor([2.0, 0.0, 1.0, 0.0], dtype=torch.float64, device=rows.device)
+    return table[codes.long()].T.contiguous()
+
+
+def reml_leg(ctx, n: int = 20000, m: int = 50000, k: int = 64, h2: float = 0.4, ngroups: int = 20,
+             max_iter: int = 20, plain_iters: int = 2, seed: int = 20264203) -> dict:
+    """REML iterations/s, GPU mixed (FP32 block PCG of 64 probes + y with FP64
+    refinement) against GPU FP64 plain (FP64 PCG, one right-hand side at a
+    time), same AI-REML algorithm and probes."""
+    import time
+
+    import numpy as np
+    import torch
+
+    import cosmicblup_amd as cb
+    from cosmicblup_amd import _lib, design, mme, synth
+
+    print(f"[bench] REML leg: n={n} m={m} probes={k}", flush=True)
+    rows, pitch = synth.bed_on_device(n, m, maf=(0.05, 0.5), missing_rate=0.0, seed=seed)
+    reader = cb.PlinkReader.from_device_rows(ctx, rows.data_ptr(), pitch, n, m)
+    gb = cb.GrmBuilder(cb.Options(), ctx=ctx)
+    gb.compute(reader, outputs=cb.CB_GRM_OUT_F64)
+    # G as a torch view (setup): Q = (G + 1e-6 I)^-1 stands in for the
+    # eigen-bent inverse (same 1e-6 floor on the centring null direction)
+    G = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    gb.copyGrmToDevice(G.data_ptr(), n)
+    del gb, reader
+    # Q = (G + 1e-6 I)^-1 (setup, not timed): torch's Cholesky inverse on the
+    # device through MAGMA when available, else host LAPACK (dpotrf + dpotri)
+    G.diagonal().add_(1e-6)
+    Q = None
+    try:
+        torch.backends.cuda.preferred_linalg_library("magma")
+        Q = torch.cholesky_inverse(torch.linalg.cholesky(G))
+        torch.cuda.synchronize()
+    except RuntimeError as err:
+        print(f"[bench] device inverse failed ({str(err)[:120]}); host LAPACK", flush=True)
+    if Q is None:
+        import scipy.linalg.lapack as lapack
+
+        Gh = G.cpu().numpy()
+        del G
+        c, info = lapack.dpotrf(Gh, lower=1, overwrite_a=1)
+        assert info == 0, f"dpotrf info={info}"
+        Qh, info = lapack.dpotri(c, lower=1, overwrite_c=1)
+        assert info == 0, f"dpotri info={info}"
+        Q = torch.from_numpy(np.tril(Qh) + np.tril(Qh, -1).T).cuda()
+        del Qh, c, Gh
+    else:
+        del G
+    torch.cuda.synchronize()
+    # phenotype: 500 QTL, h2, 20 contemporary groups
+    rng = np.random.default_rng(seed)
+    qtl = torch.from_numpy(rng.choice(m, 500, replace=False))
+    Xq = decode_rows_dosage(rows, pitch, n, qtl)
+    del rows
+    g = (Xq - Xq.mean(0)) @ torch.from_numpy(rng.standard_normal(500)).cuda()
+    g = (g * np.sqrt(h2) / g.std()).cpu().numpy()
+    grp = rng.integers(0, ngroups, n)
+    y = 10.0 + rng.standard_normal(ngroups)[grp] + g + rng.standard_normal(n) * np.sqrt(1 - h2)
+    fd = design.build_fixed_design([[f"cg{int(c):02d}"] for c in grp])
+    model = mme.MixedModel(fd, np.arange(n, dtype=np.int32), y, n, ctx=ctx)
+    model.set_qinv_dense_device(Q.data_ptr(), n)
+    del Q
+    torch.cuda.empty_cache()
+
+    def run(precision, batch, iters):
+        opts = mme.reml_options(n_probes=k, seed=7, max_iter=iters,
+                                solve=mme.solve_options(precision=precision, batch_rhs=batch, tol=1e-10))
+        R = mme.AIREML(model, opts)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        R.solve()
+        ctx.synchronize()
+        return R, time.perf_counter() - t0
+
+    ctx.reset_profile()
+    ctx.set_profiling(True)
+    Rm, tm = run(_lib.CB_PREC_MIXED, True, max_iter)
+    ctx.set_profiling(False)
+    print(f"[bench] mixed: {Rm.iterations} iterations in {tm:.2f} s, vu={Rm.var_u:.5f} ve={Rm.var_e:.5f}", flush=True)
+    symm_ms, symm_n = ctx.kernel_time("k_dense_symm_f32")
+    Rp, tp = run(_lib.CB_PREC_FP64, False, plain_iters)
+    print(f"[bench] fp64 plain: {Rp.iterations} iterations in {tp:.2f} s", flush=True)
+    # accuracy of the mixed path against FP64 at identical variances
+    opts64 = mme.reml_options(n_probes=k, seed=7, solve=mme.solve_options(precision=_lib.CB_PREC_FP64, tol=1e-12))
+    opts32 = mme.reml_options(n_probes=k, seed=7, solve=mme.solve_options(precision=_lib.CB_PREC_MIXED, tol=1e-10))
+    g64, _, tr64, s64 = mme.AIREML(model, opts64).score(Rm.var_u, Rm.var_e, want_ai=False)
+    g32, _, tr32, s32 = mme.AIREML(model, opts32).score(Rm.var_u, Rm.var_e, want_ai=False)
+    p = model.p
+    ebv_err = float(np.abs(s32[p:] - s64[p:]).max() / np.abs(s64[p:]).max())
+    rate_m = Rm.iterations / tm
+    rate_p = Rp.iterations / tp
+    # per launch of the dominant REML kernel: q^2 FP32 of Q read, 2 q^2 r flops
+    return {"n": n, "m": m, "probes": k, "iters_per_s_mixed": rate_m, "iters_per_s_fp64_plain": rate_p,
+            "speedup": rate_m / rate_p, "iterations_mixed": Rm.iterations, "converged": Rm.converged,
+            "var_u": Rm.var_u, "var_e": Rm.var_e, "ebv_rel_err_mixed_vs_fp64": ebv_err,
+            "trace_rel_err_mixed_vs_fp64": abs(tr32 - tr64) / abs(tr64),
+            "k_dense_symm_f32_avg_ms": symm_ms / max(symm_n, 1),
+            "qinv": "(G + 1e-6 I)^-1 (stands in for the eigen-bent G^-1)"}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,6 +167,10 @@ def main() -> None:
     ap.add_argument("--m", type=int, default=50000)
     ap.add_argument("--missing", type=float, default=0.01)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-reml", action="store_true")
+    ap.add_argument("--reml-n", type=int, default=20000)
+    ap.add_argument("--reml-m", type=int, default=50000)
+    ap.add_argument("--plain-iters", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -122,8 +236,11 @@ def main() -> None:
                      "frac": achieved / PEAK_INT8_TOPS, "traffic": None},
         "kernel_ms": kern,
     }
+    del gb, reader
+    torch.cuda.empty_cache()
+    if not args.no_reml:
+        out["reml"] = reml_leg(ctx, n=args.reml_n, m=args.reml_m, plain_iters=args.plain_iters)
     if not args.no_cpu:
-        del gb, reader
         torch.cuda.empty_cache()
         print("[bench] timing the CPU oracle sample", flush=True)
         out["cpu_baseline"] = cpu_baseline()

@@ -36,7 +36,7 @@ class GrmOptions(C.Structure):
 class SolveOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("precond", C.c_int), ("stop_rule", C.c_int),
                 ("batch_rhs", C.c_int), ("tol", C.c_double), ("max_iter", C.c_int64),
-                ("inner_tol", C.c_double)]
+                ("inner_tol", C.c_double), ("stall_tol", C.c_double)]
 
 
 class RemlOptions(C.Structure):
@@ -90,6 +90,7 @@ _SIGS = {
     "cb_grm_copy_grm_f32": (C.c_int, [c_vp, c_vp, c_i64]),
     "cb_grm_copy_int_gram": (C.c_int, [c_vp, c_vp, c_i64]),
     "cb_grm_device_ptr": (C.c_int, [c_vp, C.c_int, C.POINTER(c_vp), c_i64p]),
+    "cb_grm_copy_grm_device": (C.c_int, [c_vp, c_vp, c_i64]),
     "cb_grm_sum2pq": (C.c_int, [c_vp, c_dp]),
     "cb_grm_total_snps": (C.c_int, [c_vp, c_i64p]),
     "cb_grm_num_samples": (C.c_int, [c_vp, c_i64p]),

@@ -852,6 +852,17 @@ int cb_grm_copy_int_gram(const cb_grm* grm, int32_t* out, int64_t ld) {
     return guarded([&] { CB_REQUIRE(grm, "NULL grm"); copy_out(grm, grm->Gi.p, 4, out, ld); });
 }
 
+int cb_grm_copy_grm_device(const cb_grm* grm, void* dst, int64_t ld) {
+    return guarded([&] {
+        CB_REQUIRE(grm && grm->G64.p, "cb_grm_copy_grm_device: FP64 G not computed");
+        CB_REQUIRE(dst && ld >= grm->n, "cb_grm_copy_grm_device: bad destination");
+        CB_HIP(hipSetDevice(grm->ctx->device));
+        CB_HIP(hipMemcpy2DAsync(dst, ld * 8, grm->G64.p, grm->n * 8, grm->n * 8, grm->n,
+                                hipMemcpyDeviceToDevice, grm->ctx->stream));
+        CB_HIP(hipStreamSynchronize(grm->ctx->stream));
+    });
+}
+
 int cb_grm_device_ptr(const cb_grm* grm, int which, void** ptr, int64_t* ld) {
     return guarded([&] {
         CB_REQUIRE(grm && ptr && ld, "cb_grm_device_ptr: NULL argument");

@@ -231,6 +231,51 @@ __global__ void k_precond_block(const T* __restrict__ R, T* __restrict__ Z, cons
     Z[(int64_t)c * ld + i] = (T)s;
 }
 
+// partial dots of each column's u-block with the stiff vector
+template <class T>
+__global__ __launch_bounds__(256) void k_stiff_dot(const T* __restrict__ Zu, const double* __restrict__ v,
+                                                   int64_t ld, int64_t q, double* __restrict__ part) {
+    __shared__ double red[256];
+    const int c = blockIdx.y;
+    const int64_t per = (q + DOT_CHUNKS - 1) / DOT_CHUNKS;
+    const int64_t r0 = (int64_t)blockIdx.x * per, r1 = min(q, r0 + per);
+    double s = 0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) s += (double)Zu[(int64_t)c * ld + i] * v[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)c * DOT_CHUNKS + blockIdx.x] = red[0];
+}
+
+// Zu[:, c] += coef * (sum of the column's partial dots) * y
+template <class T>
+__global__ void k_stiff_axpy(T* __restrict__ Zu, const double* __restrict__ y, const double* __restrict__ part,
+                             double coef, int64_t ld, int64_t q) {
+    __shared__ double t;
+    const int c = blockIdx.y;
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int k = 0; k < DOT_CHUNKS; ++k) s += part[(int64_t)c * DOT_CHUNKS + k];
+        t = s;
+    }
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= q) return;
+    Zu[(int64_t)c * ld + i] = (T)((double)Zu[(int64_t)c * ld + i] + coef * t * y[i]);
+}
+
+// Q32[i][j] = (float)(Q64[i][j] - mu v_i v_j)
+__global__ void k_q32_bulk(const double* __restrict__ Q, float* __restrict__ Q32, const double* __restrict__ v,
+                           double mu, int64_t q) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= q * q) return;
+    const int64_t i = e / q, j = e - i * q;
+    Q32[e] = (float)(Q[e] - mu * v[i] * v[j]);
+}
+
 // CG scalar step per column (runs on the device so the host need not sync
 // every iteration). st layout per column c: [rho, pq, rho_new, rr, bb, active, alpha, beta]
 struct ColState {
@@ -424,6 +469,15 @@ struct cb_mme {
     DevBuf<double> qv64;
     DevBuf<float> qv32;
     std::vector<double> q_diag;
+    // Stiff direction of a dense Q (the eigenvector of the floored centring
+    // null direction of an eigen-bent G^-1): unit v and its eigenvalue mu.
+    // Deflated in the preconditioner when mu >> the mean diagonal of Q.
+    bool stiff = false;
+    double stiff_mu = 0.0;
+    DevBuf<double> stiff_v;   // q
+    DevBuf<double> stiff_y;   // q: D'^-1 v for the current lambda
+    DevBuf<double> stiff_part;  // DOT_CHUNKS x r partial dots
+    double stiff_gamma = 0.0; // 1 / (1/(lambda mu) + v' D'^-1 v)
     // preconditioner
     DevBuf<double> Binv;   // p x p
     DevBuf<double> dinv;   // dim
@@ -517,6 +571,15 @@ void apply_C(cb_mme* M, double lambda, const T* X, T* Y, int r) {
     // lambda Q part
     if (M->qkind == 0) {
         dense_q_apply<T>(M, X, ld, Y, ld, r, (T)lambda);
+        if (M->stiff && sizeof(T) == 4) {
+            // the FP32 copy holds Q - mu v v': add lambda mu v (v' x_u) here
+            if (M->stiff_part.n < (size_t)DOT_CHUNKS * r) M->stiff_part.alloc((size_t)DOT_CHUNKS * r);
+            k_stiff_dot<T><<<dim3(DOT_CHUNKS, (unsigned)r), 256, 0, st>>>(X + M->p, M->stiff_v.p, ld, M->q,
+                                                                        M->stiff_part.p);
+            k_stiff_axpy<T><<<dim3(blocks(M->q), (unsigned)r), 256, 0, st>>>(Y + M->p, M->stiff_v.p, M->stiff_part.p,
+                                                                           lambda * M->stiff_mu, ld, M->q);
+            CB_HIP(hipGetLastError());
+        }
     } else {
         k_csr_spmm<T, T><<<dim3(blocks(M->q), (unsigned)r), 256, 0, st>>>(
             M->qp.p, M->qc.p, qvals<T>(M), 0, M->q, M->p, X, ld, Y, ld, M->p, (T)lambda, 1);
@@ -530,9 +593,28 @@ void build_precond(cb_mme* M, double lambda, bool block) {
     if (M->dinv_lambda == lambda && M->dinv_block == (int)block && M->Binv.p) return;
     const int64_t p = M->p, dim = M->dim;
     std::vector<double> dinv(dim);
+    std::vector<double> hv;
+    if (M->stiff) {
+        hv.resize(M->q);
+        CB_HIP(hipMemcpy(hv.data(), M->stiff_v.p, M->q * 8, hipMemcpyDeviceToHost));
+    }
     for (int64_t i = 0; i < dim; ++i) {
         double d = M->wtw_diag[i] + (i >= p ? lambda * M->q_diag[i - p] : 0.0);
+        // with deflation the stiff rank-1 part lambda mu v v' leaves the diagonal
+        if (M->stiff && i >= p) d -= lambda * M->stiff_mu * hv[i - p] * hv[i - p];
         dinv[i] = std::fabs(d) > 1e-12 ? 1.0 / d : 1.0;
+    }
+    if (M->stiff) {
+        // Sherman-Morrison for (D' + s v v')^-1, s = lambda mu
+        std::vector<double> y(M->q);
+        double vy = 0;
+        for (int64_t i = 0; i < M->q; ++i) {
+            y[i] = dinv[p + i] * hv[i];
+            vy += hv[i] * y[i];
+        }
+        M->stiff_gamma = 1.0 / (1.0 / (lambda * M->stiff_mu) + vy);
+        M->stiff_y.alloc(M->q);
+        CB_HIP(hipMemcpy(M->stiff_y.p, y.data(), M->q * 8, hipMemcpyHostToDevice));
     }
     std::vector<double> binv((size_t)std::max<int64_t>(1, p * p), 0.0);
     if (p > 0) {
@@ -599,6 +681,13 @@ void apply_precond(cb_mme* M, const T* R, T* Z, int r) {
     k_precond_diag<T><<<dim3(blocks(M->dim - M->p), (unsigned)r), 256, 0, st>>>(R, Z, M->dinv.p, M->p, M->dim, M->ldv);
     if (M->p > 0)
         k_precond_block<T><<<dim3(blocks(M->p), (unsigned)r), 256, 0, st>>>(R, Z, M->Binv.p, M->p, M->ldv);
+    if (M->stiff) {
+        // z_u -= gamma (v' z_u) D'^-1 v
+        if (M->stiff_part.n < (size_t)DOT_CHUNKS * r) M->stiff_part.alloc((size_t)DOT_CHUNKS * r);
+        k_stiff_dot<T><<<dim3(DOT_CHUNKS, (unsigned)r), 256, 0, st>>>(Z + M->p, M->stiff_v.p, M->ldv, M->q, M->stiff_part.p);
+        k_stiff_axpy<T><<<dim3(blocks(M->q), (unsigned)r), 256, 0, st>>>(Z + M->p, M->stiff_y.p, M->stiff_part.p,
+                                                                        -M->stiff_gamma, M->ldv, M->q);
+    }
     CB_HIP(hipGetLastError());
 }
 
@@ -713,6 +802,148 @@ int64_t pcg_block(cb_mme* M, double lambda, const T* B, TX* X, int r, double tol
 
 }  // namespace
 
+namespace {
+
+__global__ void k_to_f32(const double* __restrict__ in, float* __restrict__ out, int64_t rows, int64_t ld) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= rows) return;
+    out[(int64_t)c * ld + i] = (float)in[(int64_t)c * ld + i];
+}
+
+// Block PCG in FP32 with FP64 iterate and FP64 residual replacement.
+// Returns iterations of the slowest column; *max_rel = worst true relative
+// (or absolute) residual at exit.
+int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int r,
+                     const cb_solve_options& o, double* max_rel) {
+    cb_context* ctx = M->ctx;
+    hipStream_t st = ctx->stream;
+    const int64_t ld = M->ldv, dim = M->dim;
+    const size_t nv = (size_t)ld * r;
+    const bool absolute = o.stop_rule == CB_STOP_ABSOLUTE;
+    build_precond(M, lambda, o.precond == CB_PRECOND_BLOCK_JACOBI);
+    DevBuf<float> R, Z, P, Qp;
+    DevBuf<double> R64, CX, part, d1, d2;
+    R.alloc(nv); Z.alloc(nv); P.alloc(nv); Qp.alloc(nv);
+    R64.alloc(nv); CX.alloc(nv);
+    part.alloc((size_t)DOT_CHUNKS * r);
+    d1.alloc(r); d2.alloc(r);
+    DevBuf<ColState> cs;
+    cs.alloc(r);
+    CB_HIP(hipMemsetAsync(R.p, 0, R.bytes(), st));
+    CB_HIP(hipMemsetAsync(Z.p, 0, Z.bytes(), st));
+    CB_HIP(hipMemsetAsync(P.p, 0, P.bytes(), st));
+    CB_HIP(hipMemsetAsync(Qp.p, 0, Qp.bytes(), st));
+    CB_HIP(hipMemsetAsync(CX.p, 0, CX.bytes(), st));
+    const dim3 g2(blocks(dim), (unsigned)r);
+    const double tol2 = o.tol * o.tol;
+    std::vector<double> bb(r), rr(r), rz(r), rr_ref(r);
+    std::vector<std::vector<double>> hist(r);  // true residual at each replacement
+    std::vector<ColState> hs(r);
+    col_dots<double>(M, B64, B64, r, part.p, d1.p);
+    CB_HIP(hipMemcpyAsync(bb.data(), d1.p, r * 8, hipMemcpyDeviceToHost, st));
+    CB_HIP(hipStreamSynchronize(st));
+    for (int c = 0; c < r; ++c) bb[c] = bb[c] == 0.0 ? 1.0 : bb[c];
+    std::vector<int> done(r, 0);  // true-residual convergence per column
+    // (re)start from the true residual: R64 = B - C x, R = (float)R64,
+    // Z = M^-1 R, rho = R.Z; returns true when every column has converged
+    auto replace = [&](bool first) -> bool {
+        if (first) {
+            CB_HIP(hipMemcpyAsync(R64.p, B64, nv * 8, hipMemcpyDeviceToDevice, st));
+        } else {
+            apply_C<double>(M, lambda, X, CX.p, r);
+            k_sub<double, double><<<g2, 256, 0, st>>>(B64, CX.p, R64.p, dim, ld);
+        }
+        k_to_f32<<<g2, 256, 0, st>>>(R64.p, R.p, dim, ld);
+        apply_precond<float>(M, R.p, Z.p, r);
+        col_dots<double>(M, R64.p, R64.p, r, part.p, d1.p);
+        col_dots<float>(M, R.p, Z.p, r, part.p, d2.p);
+        CB_HIP(hipMemcpyAsync(rr.data(), d1.p, r * 8, hipMemcpyDeviceToHost, st));
+        CB_HIP(hipMemcpyAsync(rz.data(), d2.p, r * 8, hipMemcpyDeviceToHost, st));
+        if (!first) CB_HIP(hipMemcpyAsync(hs.data(), cs.p, r * sizeof(ColState), hipMemcpyDeviceToHost, st));
+        CB_HIP(hipStreamSynchronize(st));
+        bool all = true;
+        for (int c = 0; c < r; ++c) {
+            if (!std::isfinite(rr[c]))
+                throw Error(CB_ERR_NOT_CONVERGED, "mixed-precision PCG: non-finite residual");
+            const double thr = absolute ? tol2 : tol2 * bb[c];
+            if (first) { hs[c] = ColState{}; hs[c].bb = bb[c]; }
+            if (!done[c]) {
+                if (!(rr[c] >= thr)) {
+                    done[c] = 1;
+                } else if (hist[c].size() >= 8) {
+                    // the best true residual of the last eight replacements is
+                    // not half the best before them: the FP64 rounding floor,
+                    // around which the true residual only fluctuates
+                    const size_t h = hist[c].size();
+                    double before = hist[c][0], recent = rr[c];
+                    for (size_t k = 1; k + 7 < h; ++k) before = std::min(before, hist[c][k]);
+                    for (size_t k = h - 7; k < h; ++k) recent = std::min(recent, hist[c][k]);
+                    const double rel = absolute ? std::sqrt(rr[c]) : std::sqrt(rr[c] / bb[c]);
+                    if (recent > 0.25 * before && rel < o.stall_tol) done[c] = 1;
+                }
+            }
+            // a column paused by the recursive test (or one whose recursion
+            // blew up) holds a direction that predates its last residual:
+            // restart it from the new preconditioned residual
+            const bool restart = first || !hs[c].active || !(hs[c].rr <= 100.0 * rr_ref[c]);
+            hist[c].push_back(rr[c]);
+            rr_ref[c] = rr[c];
+            hs[c].rho = rz[c];
+            hs[c].rr = rr[c];
+            hs[c].active = done[c] ? 0 : 1;
+            if (restart && !done[c])
+                CB_HIP(hipMemcpyAsync(P.p + (size_t)c * ld, Z.p + (size_t)c * ld, ld * 4,
+                                      hipMemcpyDeviceToDevice, st));
+            all = all && done[c];
+        }
+        CB_HIP(hipMemcpyAsync(cs.p, hs.data(), r * sizeof(ColState), hipMemcpyHostToDevice, st));
+        return all;
+    };
+    bool all_done = replace(true);
+    int64_t it = 0;
+    const int check_every = 4;
+    int64_t replacements = 0;
+    while (!all_done && it < o.max_iter) {
+        apply_C<float>(M, lambda, P.p, Qp.p, r);
+        col_dots<float>(M, P.p, Qp.p, r, part.p, d1.p);
+        k_cg_alpha<<<blocks(r, 64), 64, 0, st>>>(cs.p, d1.p, r);
+        k_cg_xr<double, float><<<g2, 256, 0, st>>>(X, R.p, P.p, Qp.p, cs.p, dim, ld);
+        apply_precond<float>(M, R.p, Z.p, r);
+        col_dots<float>(M, R.p, Z.p, r, part.p, d1.p);
+        col_dots<float>(M, R.p, R.p, r, part.p, d2.p);
+        // the recursive test only pauses a column until the next replacement
+        k_cg_beta<<<blocks(r, 64), 64, 0, st>>>(cs.p, d1.p, d2.p, r, tol2, absolute ? 1 : 0);
+        k_cg_p<float><<<g2, 256, 0, st>>>(P.p, Z.p, cs.p, dim, ld);
+        CB_HIP(hipGetLastError());
+        ++it;
+        if (it % check_every == 0 || it == o.max_iter) {
+            CB_HIP(hipMemcpyAsync(hs.data(), cs.p, r * sizeof(ColState), hipMemcpyDeviceToHost, st));
+            CB_HIP(hipStreamSynchronize(st));
+            bool need = false;
+            for (int c = 0; c < r; ++c) {
+                if (done[c]) continue;
+                // paused by the recursive test, fallen tenfold since the last
+                // replacement, or grown tenfold (lost conjugacy)
+                if (!hs[c].active || hs[c].rr < 0.01 * rr_ref[c] || !(hs[c].rr <= 100.0 * rr_ref[c])) need = true;
+            }
+            if (need) {
+                all_done = replace(false);
+                ++replacements;
+            }
+        }
+    }
+    if (!all_done) all_done = replace(false);
+    double worst = 0;
+    for (int c = 0; c < r; ++c)
+        worst = std::max(worst, absolute ? std::sqrt(rr[c]) : std::sqrt(rr[c] / bb[c]));
+    *max_rel = worst;
+    if (!all_done) throw Error(CB_ERR_NOT_CONVERGED, "mixed-precision PCG did not reach the tolerance");
+    return it;
+}
+
+}  // namespace
+
 // Solve driver shared by cb_mme_solve and the REML loop. B, X device FP64
 // blocks [ldv x r]; X is the output (zero start).
 static SolveStats solve_block(cb_mme* M, double lambda, const cb_solve_options& o, const double* B,
@@ -742,56 +973,15 @@ static SolveStats solve_block(cb_mme* M, double lambda, const cb_solve_options& 
         if (S.max_rel > o.tol) throw Error(CB_ERR_NOT_CONVERGED, "PCG did not reach the tolerance");
         return S;
     }
-    // Mixed: FP32 inner block PCG, FP64 residual and refinement.
+    // Mixed: one CG recursion per column with the operator, preconditioner
+    // and direction vectors in FP32, the iterate accumulated in FP64, and the
+    // residual replaced by the true FP64 residual b - C x whenever the
+    // recursive one has dropped tenfold since the last replacement (and
+    // before any column is declared converged). The search directions are
+    // kept across replacements.
     CB_REQUIRE(M->qkind == 1 || M->Qd32.p, "FP32 copy of Q missing");
-    DevBuf<double> R64, CX, part, nrm, bnorm, scale, inv;
-    DevBuf<float> R32, D32;
-    R64.alloc(nv); CX.alloc(nv); R32.alloc(nv); D32.alloc(nv);
-    part.alloc((size_t)DOT_CHUNKS * r);
-    nrm.alloc(r); bnorm.alloc(r); scale.alloc(r); inv.alloc(r);
-    col_dots<double>(M, B, B, r, part.p, bnorm.p);
-    std::vector<double> hb(r), hr(r), hs(r), hi(r);
-    CB_HIP(hipMemcpyAsync(hb.data(), bnorm.p, r * 8, hipMemcpyDeviceToHost, st));
-    CB_HIP(hipStreamSynchronize(st));
-    const dim3 g2(blocks(dim), (unsigned)r);
-    const int max_outer = 60;
-    int64_t inner_total = 0;
-    std::vector<double> hist;  // worst relative residual per refinement
-    for (int outer = 0; outer < max_outer; ++outer) {
-        // r64 = B - C64 x64
-        CB_HIP(hipMemsetAsync(CX.p, 0, nv * 8, st));
-        if (outer > 0) apply_C<double>(M, lambda, X, CX.p, r);
-        k_sub<double, double><<<g2, 256, 0, st>>>(B, CX.p, R64.p, dim, ld);
-        col_dots<double>(M, R64.p, R64.p, r, part.p, nrm.p);
-        CB_HIP(hipMemcpyAsync(hr.data(), nrm.p, r * 8, hipMemcpyDeviceToHost, st));
-        CB_HIP(hipStreamSynchronize(st));
-        double worst = 0;
-        for (int c = 0; c < r; ++c) {
-            const double bn = hb[c] == 0.0 ? 1.0 : hb[c];
-            const double rel = absolute ? std::sqrt(hr[c]) : std::sqrt(hr[c] / bn);
-            worst = std::max(worst, rel);
-            const double rn = std::sqrt(hr[c]);
-            hs[c] = rn > 0 ? rn : 1.0;        // unscale factor
-            hi[c] = rn > 0 ? 1.0 / rn : 0.0;  // scale to unit norm
-        }
-        S.max_rel = worst;
-        S.iterations = inner_total;
-        if (worst < o.tol) return S;
-        // the true FP64 residual has a rounding floor; stop once it no longer
-        // halves over three refinements
-        hist.push_back(worst);
-        if (outer >= 3 && worst > 0.5 * hist[outer - 3]) break;
-        CB_HIP(hipMemcpyAsync(scale.p, hs.data(), r * 8, hipMemcpyHostToDevice, st));
-        CB_HIP(hipMemcpyAsync(inv.p, hi.data(), r * 8, hipMemcpyHostToDevice, st));
-        k_scale_to_f32<<<g2, 256, 0, st>>>(R64.p, R32.p, inv.p, dim, ld);
-        CB_HIP(hipMemsetAsync(D32.p, 0, nv * 4, st));
-        inner_total += pcg_block<float, float>(M, lambda, R32.p, D32.p, r, o.inner_tol, false,
-                                               o.max_iter, block, nullptr, true);
-        k_axpy_from_f32<<<g2, 256, 0, st>>>(X, D32.p, scale.p, dim, ld);
-        CB_HIP(hipGetLastError());
-        if (inner_total > o.max_iter * 4) break;
-    }
-    throw Error(CB_ERR_NOT_CONVERGED, "mixed-precision PCG refinement did not reach the tolerance");
+    S.iterations = pcg_reliable(M, lambda, B, X, r, o, &S.max_rel);
+    return S;
 }
 
 // ---------------------------------------------------------------------------
@@ -1043,6 +1233,7 @@ void cb_solve_options_default(cb_solve_options* o) {
     o->tol = 1e-10;
     o->max_iter = 10000;
     o->inner_tol = 1e-5;
+    o->stall_tol = 1e-8;
 }
 
 void cb_reml_options_default(cb_reml_options* o) {
@@ -1094,6 +1285,51 @@ int cb_mme_dims(const cb_mme* M, int64_t* n, int64_t* p, int64_t* q) {
     });
 }
 
+// Dominant eigenpair of a dense Q by power iteration from the constant
+// vector (the centring null direction of a VanRaden G). It is deflated in
+// the preconditioner when it stands far above the rest of the spectrum:
+// mu > 1e3 * (trace(Q) - mu) / q.
+static void find_stiff_direction(cb_mme* M) {
+    const int64_t q = M->q;
+    hipStream_t st = M->ctx->stream;
+    M->stiff = false;
+    if (q < 2) return;
+    std::vector<double> v(q, 1.0 / std::sqrt((double)q)), y(q);
+    DevBuf<double> dv, dy;
+    dv.alloc(q);
+    dy.alloc(q);
+    double mu = 0, mu_prev = 0;
+    for (int it = 0; it < 100; ++it) {
+        CB_HIP(hipMemcpyAsync(dv.p, v.data(), q * 8, hipMemcpyHostToDevice, st));
+        CB_HIP(hipMemsetAsync(dy.p, 0, q * 8, st));
+        k_gemv<double><<<blocks(q, 4), 256, 0, st>>>(M->Qd64.p, q, q, dv.p, q, 1, dy.p, q, 0, 1.0);
+        CB_HIP(hipGetLastError());
+        CB_HIP(hipMemcpyAsync(y.data(), dy.p, q * 8, hipMemcpyDeviceToHost, st));
+        CB_HIP(hipStreamSynchronize(st));
+        double vy = 0, yy = 0;
+        for (int64_t i = 0; i < q; ++i) { vy += v[i] * y[i]; yy += y[i] * y[i]; }
+        mu = vy;
+        const double ny = std::sqrt(yy);
+        if (!(ny > 0)) return;
+        for (int64_t i = 0; i < q; ++i) v[i] = y[i] / ny;
+        if (it > 2 && std::fabs(mu - mu_prev) <= 1e-13 * std::fabs(mu)) break;
+        mu_prev = mu;
+    }
+    double tr = 0;
+    for (double d : M->q_diag) tr += d;
+    const double bulk = (tr - mu) / (double)q;
+    if (!(mu > 1e3 * std::fabs(bulk))) return;
+    M->stiff = true;
+    M->stiff_mu = mu;
+    M->stiff_v.alloc(q);
+    CB_HIP(hipMemcpy(M->stiff_v.p, v.data(), q * 8, hipMemcpyHostToDevice));
+    // FP32 copy of the bulk only: Q32 = fp32(Q - mu v v'); the rank-1 part is
+    // applied separately with FP64 accumulation (apply_C<float>)
+    k_q32_bulk<<<blocks(q * q), 256, 0, st>>>(M->Qd64.p, M->Qd32.p, M->stiff_v.p, mu, q);
+    CB_HIP(hipGetLastError());
+    CB_HIP(hipStreamSynchronize(st));
+}
+
 static void set_q_dense_common(cb_mme* M, const double* Q, int64_t ld, hipMemcpyKind kind) {
     const int64_t q = M->q;
     CB_REQUIRE(Q && ld >= q, "cb_mme_set_qinv_dense: bad matrix");
@@ -1113,6 +1349,7 @@ static void set_q_dense_common(cb_mme* M, const double* Q, int64_t ld, hipMemcpy
     M->qkind = 0;
     M->qp.release(); M->qc.release(); M->qv64.release(); M->qv32.release();
     M->dinv_lambda = std::nan("");
+    find_stiff_direction(M);
 }
 
 int cb_mme_set_qinv_dense(cb_mme* M, const double* Q, int64_t ld) {
@@ -1153,6 +1390,7 @@ int cb_mme_set_qinv_csr(cb_mme* M, const int64_t* ptr, const int32_t* col, const
         CB_HIP(hipGetLastError());
         CB_HIP(hipStreamSynchronize(st));
         M->qkind = 1;
+        M->stiff = false;
         M->Qd64.release(); M->Qd32.release();
         M->dinv_lambda = std::nan("");
     });

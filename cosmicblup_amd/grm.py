@@ -77,6 +77,10 @@ class GrmBuilder:
         _lib.check(_lib.lib().cb_grm_copy_int_gram(self.handle, _lib.ptr(G), self._n))
         return G
 
+    def copyGrmToDevice(self, dptr: int, ld: int) -> None:
+        """FP64 G into device memory owned by the caller (same device)."""
+        _lib.check(_lib.lib().cb_grm_copy_grm_device(self.handle, dptr, ld))
+
     def devicePtr(self, which: int = _lib.CB_GRM_OUT_F64) -> tuple[int, int]:
         p = C.c_void_p()
         ld = C.c_int64()

@@ -108,6 +108,9 @@ int cb_grm_compute(cb_grm* grm, const cb_genotypes* g, int outputs);
 int cb_grm_copy_grm(const cb_grm* grm, double* out, int64_t ld);
 int cb_grm_copy_grm_f32(const cb_grm* grm, float* out, int64_t ld);
 int cb_grm_copy_int_gram(const cb_grm* grm, int32_t* out, int64_t ld);
+/* FP64 G into caller-owned device memory on the same device (e.g. to form
+ * G^-1 there); leading dimension ld. */
+int cb_grm_copy_grm_device(const cb_grm* grm, void* dst, int64_t ld);
 /* Device pointer to an output (CB_GRM_OUT_*) and its leading dimension;
  * valid until the next compute or destroy. */
 int cb_grm_device_ptr(const cb_grm* grm, int which, void** ptr, int64_t* ld);
@@ -167,6 +170,11 @@ typedef struct {
     double tol;         /* default 1e-10 (config.h:41) */
     int64_t max_iter;   /* default 10000 (config.h:42) */
     double inner_tol;   /* MIXED: residual reduction per FP32 inner solve, default 1e-5 */
+    double stall_tol;   /* MIXED: the stop test is on the true FP64 residual, which has a
+                           rounding floor near cond(C) * eps; a refinement that stalls
+                           (no halving over three steps) below stall_tol is accepted.
+                           Default 1e-8. (The reference's 1e-10 tests the recursive
+                           residual, pcg_solver.h:501.) */
 } cb_solve_options;
 
 void cb_solve_options_default(cb_solve_options* opts);
