@@ -20,9 +20,10 @@
 // CSR (A^-1). Q and W'W are held in FP64 and FP32.
 //
 // Kernels and their bounds (per operator application on an r-column block):
-//   k_dense_symm  Q X_u, split-K register-tiled; q^2*s bytes of Q read once
-//                 (s = 4 or 8), 2 q^2 r flops: HBM-bound for small r, vector
-//                 ALU bound at r >~ 20 (FP32, no packed math)
+//   k_dense_mfma  Q X_u on MFMA 16x16x4 (FP32 or FP64), split-K; q^2*s bytes
+//                 of Q read once (s = 4 or 8), 2 q^2 r flops: HBM-bound for
+//                 small r, matrix-core bound above ~20 (FP32) / ~10 (FP64)
+//                 columns
 //   k_gemv        r <= 2 dense product, HBM-bound
 //   k_csr_spmm    W'W and CSR Q, HBM/L2-bound (8-12 bytes per nonzero)
 //   k_col_dots    per-column FP64 dot products, two-stage deterministic
@@ -39,9 +40,10 @@ using namespace cbamd;
 namespace {
 
 constexpr int DOT_CHUNKS = 128;   // row chunks of the two-stage column dot
-constexpr int SYMM_ROWS = 128;    // rows per k_dense_symm tile
-constexpr int SYMM_KC = 32;       // k-chunk staged in LDS
-constexpr int SYMM_MAXC = 80;     // max columns per k_dense_symm launch
+constexpr int MF_ROWS = 128;      // rows per k_dense_mfma tile (4 waves x 32)
+constexpr int MF_KC = 32;         // k per chunk
+constexpr int MF_KCP = MF_KC + 4; // LDS pitch of a staged X column
+constexpr int MF_MAXNT = 5;       // 16-column subtiles per launch (80 columns)
 
 // ---------------------------------------------------------------------------
 // Generic device kernels
@@ -80,67 +82,136 @@ __global__ void k_csr_spmm_long(const int64_t* __restrict__ ptr, const int32_t* 
     if (lane == 0) Y[c * ldy + row] = (TV)acc;
 }
 
-// Split-K partial of Q X for a row tile: part[split][c][row] = sum_{k in split} Q[row,k] X[k,c]
-// Q row-major q x q (leading dimension ldq); X column-major (ldx), columns [0, r).
-// 256 threads: 16 row groups x 8 rows, 16 column groups x ceil(r/16) columns.
+// MFMA 16x16x4 (FP32 or FP64). Both take lane l's operands as row (l & 15)
+// of A and column (l & 15) of B at k-slot l >> 4. The accumulators differ:
+// register j of lane l is element (4 * (l >> 4) + j, l & 15) of the 16 x 16
+// tile for FP32 and ((l >> 4) + 4 j, l & 15) for FP64 (checked on gfx950).
 template <class T>
-__global__ __launch_bounds__(256) void k_dense_symm(const T* __restrict__ Q, int64_t ldq, int64_t q,
+struct Mfma16;
+template <>
+struct Mfma16<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 step(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int g, int j) { return 4 * g + j; }
+};
+template <>
+struct Mfma16<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 step(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int g, int j) { return g + 4 * j; }
+};
+
+// Split-K partial of Q X on the matrix cores:
+//   part[split][c][row] = sum_{k in split} Q[row,k] X[k,c]
+// Q row-major q x q with leading dimension ldq (a multiple of 32, columns
+// [q, ldq) zero); X column-major (ldx), columns [0, r), r <= 16 NT.
+// 1-D grid: block b takes split b % nsplit of row tile b / nsplit, so
+// consecutive blocks (which the dispatcher deals to different XCDs) share a
+// row tile and each XCD keeps to a few splits of X in its L2.
+// Wave w owns rows [32w, 32w + 32) of the 128-row tile as two 16-row
+// subtiles. In each 32-k chunk, lane l holds row (l & 15) and the 8
+// consecutive k of group g = l >> 4: its A operand is one contiguous run of a
+// Q row loaded straight from global memory (Q is read exactly once), and MFMA
+// step t multiplies k = kc + 8g + t. The X chunk is staged in LDS as
+// [column][k] (double-buffered) and read in the same k order, which is all a
+// product needs.
+template <class T, int NT>
+__global__ __launch_bounds__(256) void k_dense_mfma(const T* __restrict__ Q, int64_t ldq, int64_t q,
                                                     const T* __restrict__ X, int64_t ldx, int r,
-                                                    int64_t kspan, T* __restrict__ part,
+                                                    int64_t kspan, int nsplit, T* __restrict__ part,
                                                     int64_t ldp) {
-    __shared__ T Qs[SYMM_ROWS][SYMM_KC + 1];
-    __shared__ T Xs[SYMM_KC][SYMM_MAXC];
-    const int tid = threadIdx.x;
-    const int ty = tid >> 4, tx = tid & 15;
-    const int64_t row0 = (int64_t)blockIdx.x * SYMM_ROWS;
-    const int64_t k0 = (int64_t)blockIdx.y * kspan;
+    using V4 = typename Mfma16<T>::v4;
+    constexpr int XE = NT * 16 * MF_KC / 256;  // staged X elements per thread
+    __shared__ T xs[2][NT * 16][MF_KCP];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int li = lane & 15, g = lane >> 4;
+    const int split = (int)(blockIdx.x % (unsigned)nsplit);
+    const int64_t row0 = (int64_t)(blockIdx.x / (unsigned)nsplit) * MF_ROWS;
+    const int64_t k0 = (int64_t)split * kspan;
     const int64_t k1 = min(q, k0 + kspan);
-    constexpr int RPT = SYMM_ROWS / 16;          // rows per thread
-    constexpr int CPT = SYMM_MAXC / 16;          // max columns per thread
-    T acc[RPT][CPT];
+    const int nchunks = (int)((k1 - k0 + MF_KC - 1) / MF_KC);
+    const T* qa[2];
 #pragma unroll
-    for (int a = 0; a < RPT; ++a)
-#pragma unroll
-        for (int b = 0; b < CPT; ++b) acc[a][b] = 0;
-    for (int64_t kc = k0; kc < k1; kc += SYMM_KC) {
-        __syncthreads();
-        // Q tile: 128 rows x 32 k (coalesced along k)
-        for (int e = tid; e < SYMM_ROWS * SYMM_KC; e += 256) {
-            const int rr = e / SYMM_KC, kk = e % SYMM_KC;
-            const int64_t gr = row0 + rr, gk = kc + kk;
-            Qs[rr][kk] = (gr < q && gk < k1) ? Q[gr * ldq + gk] : (T)0;
-        }
-        // X chunk: 32 k x r
-        for (int e = tid; e < SYMM_KC * SYMM_MAXC; e += 256) {
-            const int kk = e % SYMM_KC, cc = e / SYMM_KC;
-            const int64_t gk = kc + kk;
-            Xs[kk][cc] = (cc < r && gk < k1) ? X[cc * ldx + gk] : (T)0;
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < SYMM_KC; ++kk) {
-            T qa[RPT], xb[CPT];
-#pragma unroll
-            for (int a = 0; a < RPT; ++a) qa[a] = Qs[ty * RPT + a][kk];
-#pragma unroll
-            for (int b = 0; b < CPT; ++b) xb[b] = Xs[kk][tx + 16 * b];
-#pragma unroll
-            for (int a = 0; a < RPT; ++a)
-#pragma unroll
-                for (int b = 0; b < CPT; ++b) acc[a][b] += qa[a] * xb[b];
-        }
+    for (int s = 0; s < 2; ++s) {
+        const int64_t rr = min(row0 + w * 32 + s * 16 + li, q - 1);  // rows >= q: computed, not stored
+        qa[s] = Q + rr * ldq + k0 + 8 * g;
     }
-    T* pp = part + (int64_t)blockIdx.y * ldp * r;
+    T xr[XE], ar[2][8], an[2][8];
+    V4 acc[2][NT];
 #pragma unroll
-    for (int a = 0; a < RPT; ++a) {
-        const int64_t gr = row0 + ty * RPT + a;
-        if (gr >= q) continue;
+    for (int s = 0; s < 2; ++s)
 #pragma unroll
-        for (int b = 0; b < CPT; ++b) {
-            const int cc = tx + 16 * b;
-            if (cc < r) pp[(int64_t)cc * ldp + gr] = acc[a][b];
+        for (int n = 0; n < NT; ++n) acc[s][n] = V4{0, 0, 0, 0};
+    auto load_x = [&](int64_t kc) {
+#pragma unroll
+        for (int i = 0; i < XE; ++i) {
+            const int e = tid + 256 * i, c = e >> 5;
+            const int64_t k = kc + (e & 31);
+            xr[i] = (c < r && k < k1) ? X[(int64_t)c * ldx + k] : (T)0;
         }
+    };
+    auto store_x = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < XE; ++i) {
+            const int e = tid + 256 * i;
+            xs[buf][e >> 5][e & 31] = xr[i];
+        }
+    };
+    // k beyond k1 reads the zero padding of Q (last split) and meets zero X
+    auto load_a = [&](int off, T (&dst)[2][8]) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int t = 0; t < 8; ++t) dst[s][t] = __builtin_nontemporal_load(qa[s] + off + t);
+    };
+    load_x(k0);
+    load_a(0, ar);
+    store_x(0);
+    __syncthreads();
+    for (int ch = 0; ch < nchunks; ++ch) {
+        const int buf = ch & 1;
+        const bool more = ch + 1 < nchunks;
+        if (more) {
+            load_x(k0 + (int64_t)(ch + 1) * MF_KC);
+            load_a((ch + 1) * MF_KC, an);
+        }
+        T bx[NT][8];
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+#pragma unroll
+            for (int t = 0; t < 8; ++t) bx[n][t] = xs[buf][n * 16 + li][8 * g + t];
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int n = 0; n < NT; ++n) acc[s][n] = Mfma16<T>::step(ar[s][t], bx[n][t], acc[s][n]);
+        if (more) {
+            store_x(buf ^ 1);
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int t = 0; t < 8; ++t) ar[s][t] = an[s][t];
+        }
+        __syncthreads();
     }
+    T* pp = part + (int64_t)split * r * ldp;
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            const int c = n * 16 + li;
+            if (c >= r) continue;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int64_t row = row0 + w * 32 + s * 16 + Mfma16<T>::row(g, j);
+                if (row < q) pp[(int64_t)c * ldp + row] = acc[s][n][j];
+            }
+        }
 }
 
 // Y[row, c] += alpha * sum_split part[split][c][row]   (fixed split order)
@@ -150,9 +221,10 @@ __global__ void k_dense_reduce(const T* __restrict__ part, int64_t ldp, int nspl
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
     if (row >= q) return;
-    T s = 0;
-    for (int sp = 0; sp < nsplit; ++sp) s += part[((int64_t)sp * r + c) * ldp + row];
-    Y[(int64_t)c * ldy + y_off + row] += alpha * s;
+    double s = 0;
+    for (int sp = 0; sp < nsplit; ++sp) s += (double)part[((int64_t)sp * r + c) * ldp + row];
+    T* y = Y + (int64_t)c * ldy + y_off + row;
+    *y = (T)((double)*y + (double)alpha * s);
 }
 
 // y_c[row] += alpha * Q[row,:] x_c for r <= 2 columns; wave per row.
@@ -269,11 +341,11 @@ __global__ void k_stiff_axpy(T* __restrict__ Zu, const double* __restrict__ y, c
 
 // Q32[i][j] = (float)(Q64[i][j] - mu v_i v_j)
 __global__ void k_q32_bulk(const double* __restrict__ Q, float* __restrict__ Q32, const double* __restrict__ v,
-                           double mu, int64_t q) {
+                           double mu, int64_t q, int64_t ldq) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= q * q) return;
-    const int64_t i = e / q, j = e - i * q;
-    Q32[e] = (float)(Q[e] - mu * v[i] * v[j]);
+    if (e >= q * ldq) return;
+    const int64_t i = e / ldq, j = e - i * ldq;
+    Q32[e] = j < q ? (float)(Q[e] - mu * v[i] * v[j]) : 0.0f;  // padding stays zero
 }
 
 // CG scalar step per column (runs on the device so the host need not sync
@@ -462,8 +534,9 @@ struct cb_mme {
     double yty = 0.0;
     // Q
     int qkind = -1;  // 0 dense, 1 CSR
-    DevBuf<double> Qd64;
+    DevBuf<double> Qd64;  // q rows, leading dimension ldq (zero-padded)
     DevBuf<float> Qd32;
+    int64_t ldq = 0;
     DevBuf<int64_t> qp;
     DevBuf<int32_t> qc;
     DevBuf<double> qv64;
@@ -530,21 +603,31 @@ void dense_q_apply(cb_mme* M, const T* X, int64_t ldx, T* Y, int64_t ldy, int r,
     const T* Q = qdense<T>(M);
     if (r <= 2) {
         timed(ctx, sizeof(T) == 8 ? "k_gemv_f64" : "k_gemv_f32", [&] {
-            k_gemv<T><<<blocks(q, 4), 256, 0, st>>>(Q, q, q, X + M->p, ldx, r, Y, ldy, M->p, alpha);
+            k_gemv<T><<<blocks(q, 4), 256, 0, st>>>(Q, M->ldq, q, X + M->p, ldx, r, Y, ldy, M->p, alpha);
         });
         return;
     }
-    for (int c0 = 0; c0 < r; c0 += SYMM_MAXC) {
-        const int rc = std::min(SYMM_MAXC, r - c0);
-        const int64_t row_tiles = ceil_div(q, SYMM_ROWS);
-        int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(1024, row_tiles)));
-        int64_t kspan = round_up(ceil_div(q, nsplit), SYMM_KC);
-        nsplit = (int)ceil_div(q, kspan);
-        const int64_t ldp = round_up(q, 64);
+    const int64_t row_tiles = ceil_div(q, MF_ROWS);
+    // about 2048 blocks, splits of at least 256 k
+    int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(2048, row_tiles), q / 256));
+    const int64_t kspan = round_up(ceil_div(q, nsplit), MF_KC);
+    nsplit = (int)ceil_div(q, kspan);
+    const int64_t ldp = round_up(q, 64);
+    const int64_t ldq = M->ldq;
+    for (int c0 = 0; c0 < r; c0 += MF_MAXNT * 16) {
+        const int rc = std::min(MF_MAXNT * 16, r - c0);
+        const int nt = (rc + 15) / 16;
         T* part = partbuf<T>(M, (size_t)nsplit * rc * ldp);
-        timed(ctx, sizeof(T) == 8 ? "k_dense_symm_f64" : "k_dense_symm_f32", [&] {
-            k_dense_symm<T><<<dim3((unsigned)row_tiles, (unsigned)nsplit), 256, 0, st>>>(
-                Q, q, q, X + (int64_t)c0 * ldx + M->p, ldx, rc, kspan, part, ldp);
+        const T* Xc = X + (int64_t)c0 * ldx + M->p;
+        const unsigned grid = (unsigned)(row_tiles * nsplit);
+        timed(ctx, sizeof(T) == 8 ? "k_dense_mfma_f64" : "k_dense_mfma_f32", [&] {
+            switch (nt) {
+                case 1: k_dense_mfma<T, 1><<<grid, 256, 0, st>>>(Q, ldq, q, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
+                case 2: k_dense_mfma<T, 2><<<grid, 256, 0, st>>>(Q, ldq, q, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
+                case 3: k_dense_mfma<T, 3><<<grid, 256, 0, st>>>(Q, ldq, q, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
+                case 4: k_dense_mfma<T, 4><<<grid, 256, 0, st>>>(Q, ldq, q, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
+                default: k_dense_mfma<T, 5><<<grid, 256, 0, st>>>(Q, ldq, q, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
+            }
         });
         k_dense_reduce<T><<<dim3(blocks(q), (unsigned)rc), 256, 0, st>>>(part, ldp, nsplit, q, rc,
                                                                        Y + (int64_t)c0 * ldy, ldy, M->p, alpha);
@@ -1302,7 +1385,7 @@ static void find_stiff_direction(cb_mme* M) {
     for (int it = 0; it < 100; ++it) {
         CB_HIP(hipMemcpyAsync(dv.p, v.data(), q * 8, hipMemcpyHostToDevice, st));
         CB_HIP(hipMemsetAsync(dy.p, 0, q * 8, st));
-        k_gemv<double><<<blocks(q, 4), 256, 0, st>>>(M->Qd64.p, q, q, dv.p, q, 1, dy.p, q, 0, 1.0);
+        k_gemv<double><<<blocks(q, 4), 256, 0, st>>>(M->Qd64.p, M->ldq, q, dv.p, q, 1, dy.p, q, 0, 1.0);
         CB_HIP(hipGetLastError());
         CB_HIP(hipMemcpyAsync(y.data(), dy.p, q * 8, hipMemcpyDeviceToHost, st));
         CB_HIP(hipStreamSynchronize(st));
@@ -1325,7 +1408,7 @@ static void find_stiff_direction(cb_mme* M) {
     CB_HIP(hipMemcpy(M->stiff_v.p, v.data(), q * 8, hipMemcpyHostToDevice));
     // FP32 copy of the bulk only: Q32 = fp32(Q - mu v v'); the rank-1 part is
     // applied separately with FP64 accumulation (apply_C<float>)
-    k_q32_bulk<<<blocks(q * q), 256, 0, st>>>(M->Qd64.p, M->Qd32.p, M->stiff_v.p, mu, q);
+    k_q32_bulk<<<blocks(q * M->ldq), 256, 0, st>>>(M->Qd64.p, M->Qd32.p, M->stiff_v.p, mu, q, M->ldq);
     CB_HIP(hipGetLastError());
     CB_HIP(hipStreamSynchronize(st));
 }
@@ -1335,15 +1418,20 @@ static void set_q_dense_common(cb_mme* M, const double* Q, int64_t ld, hipMemcpy
     CB_REQUIRE(Q && ld >= q, "cb_mme_set_qinv_dense: bad matrix");
     set_device(M->ctx);
     hipStream_t st = M->ctx->stream;
-    M->Qd64.alloc((size_t)q * q);
-    M->Qd32.alloc((size_t)q * q);
-    CB_HIP(hipMemcpy2DAsync(M->Qd64.p, q * 8, Q, ld * 8, q * 8, q, kind, st));
-    k_convert<float, double><<<blocks(q * q), 256, 0, st>>>(M->Qd64.p, M->Qd32.p, q * q);
+    // rows padded to a multiple of 32 elements with zeros (k_dense_mfma reads
+    // whole 32-k chunks)
+    const int64_t ldq = round_up(q, MF_KC);
+    M->ldq = ldq;
+    M->Qd64.alloc((size_t)q * ldq);
+    M->Qd32.alloc((size_t)q * ldq);
+    if (ldq > q) CB_HIP(hipMemsetAsync(M->Qd64.p, 0, M->Qd64.bytes(), st));
+    CB_HIP(hipMemcpy2DAsync(M->Qd64.p, ldq * 8, Q, ld * 8, q * 8, q, kind, st));
+    k_convert<float, double><<<blocks(q * ldq), 256, 0, st>>>(M->Qd64.p, M->Qd32.p, q * ldq);
     CB_HIP(hipGetLastError());
     M->q_diag.resize(q);
     DevBuf<double> dg;
     dg.alloc(q);
-    CB_HIP(hipMemcpy2DAsync(dg.p, 8, M->Qd64.p, (q + 1) * 8, 8, q, hipMemcpyDeviceToDevice, st));
+    CB_HIP(hipMemcpy2DAsync(dg.p, 8, M->Qd64.p, (ldq + 1) * 8, 8, q, hipMemcpyDeviceToDevice, st));
     CB_HIP(hipMemcpyAsync(M->q_diag.data(), dg.p, q * 8, hipMemcpyDeviceToHost, st));
     CB_HIP(hipStreamSynchronize(st));
     M->qkind = 0;

@@ -70,8 +70,10 @@ def bed_on_device(n: int, m: int, maf=(0.01, 0.5), missing_rate: float = 0.0, se
                   device: str = "cuda"):
     """Large synthetic BED generated directly in device memory with torch.
 
-    Returns (rows uint8 tensor m x row_pitch, row_pitch). Same distribution as
-    simulate_dosages; the random stream is torch's, so it is not the numpy one.
+    Returns (rows uint8 tensor m x row_pitch, row_pitch) once the device has
+    finished writing them (the library reads them on its own stream). Same
+    distribution as simulate_dosages; the random stream is torch's, so it is
+    not the numpy one.
     """
     import torch
 
@@ -97,6 +99,7 @@ def bed_on_device(n: int, m: int, maf=(0.01, 0.5), missing_rate: float = 0.0, se
         packed = c[:, :, 0] | (c[:, :, 1] << 2) | (c[:, :, 2] << 4) | (c[:, :, 3] << 6)
         rows[j0:j1, :bps] = packed
         del u, code, c, packed
+    torch.cuda.synchronize(device)
     return rows, pitch
 
 

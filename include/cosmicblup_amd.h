@@ -69,7 +69,9 @@ int cb_context_kernel_time(cb_context* ctx, const char* kernel, double* total_ms
 int cb_genotypes_from_bed(cb_context* ctx, const uint8_t* bed, size_t bed_size,
                           int64_t n_samples, int64_t n_snps, cb_genotypes** out);
 /* Same, from SNP rows already resident on `ctx`'s device (no magic):
- * row j starts at d_rows + j*row_pitch; row_pitch >= ceil(n/4). Copies. */
+ * row j starts at d_rows + j*row_pitch; row_pitch >= ceil(n/4). Copies on
+ * the context's stream and returns when the copy is done; work on other
+ * streams that writes d_rows must be finished before the call. */
 int cb_genotypes_from_device_rows(cb_context* ctx, const void* d_rows, size_t row_pitch,
                                   int64_t n_samples, int64_t n_snps, cb_genotypes** out);
 void cb_genotypes_destroy(cb_genotypes* g);
