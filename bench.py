@@ -143,6 +143,11 @@ def reml_leg(ctx, n: int = 20000, m: int = 50000, k: int = 64, h2: float = 0.4, 
     symm_ms, symm_n = ctx.kernel_time("k_dense_mfma_f32")
     Rp, tp = run(_lib.CB_PREC_FP64, False, plain_iters)
     print(f"[bench] fp64 plain: {Rp.iterations} iterations in {tp:.2f} s", flush=True)
+    # FP64 with the same batching: the variance-component reference for the
+    # mixed path (same probes, same AI-REML) and a second speed point
+    Rb, tb = run(_lib.CB_PREC_FP64, True, max_iter)
+    print(f"[bench] fp64 batched: {Rb.iterations} iterations in {tb:.2f} s", flush=True)
+    vc_err = max(abs(Rm.var_u - Rb.var_u) / Rb.var_u, abs(Rm.var_e - Rb.var_e) / Rb.var_e)
     # accuracy of the mixed path against FP64 at identical variances
     opts64 = mme.reml_options(n_probes=k, seed=7, solve=mme.solve_options(precision=_lib.CB_PREC_FP64, tol=1e-12))
     opts32 = mme.reml_options(n_probes=k, seed=7, solve=mme.solve_options(precision=_lib.CB_PREC_MIXED, tol=1e-10))
@@ -154,8 +159,10 @@ def reml_leg(ctx, n: int = 20000, m: int = 50000, k: int = 64, h2: float = 0.4, 
     rate_p = Rp.iterations / tp
     # per launch of the dominant REML kernel: q^2 FP32 of Q read, 2 q^2 r flops
     return {"n": n, "m": m, "probes": k, "iters_per_s_mixed": rate_m, "iters_per_s_fp64_plain": rate_p,
-            "speedup": rate_m / rate_p, "iterations_mixed": Rm.iterations, "converged": Rm.converged,
-            "var_u": Rm.var_u, "var_e": Rm.var_e, "ebv_rel_err_mixed_vs_fp64": ebv_err,
+            "speedup": rate_m / rate_p, "iters_per_s_fp64_batched": Rb.iterations / tb,
+            "iterations_mixed": Rm.iterations, "converged": Rm.converged,
+            "var_u": Rm.var_u, "var_e": Rm.var_e, "vc_rel_err_mixed_vs_fp64": vc_err,
+            "ebv_rel_err_mixed_vs_fp64": ebv_err,
             "trace_rel_err_mixed_vs_fp64": abs(tr32 - tr64) / abs(tr64),
             "k_dense_mfma_f32_avg_ms": symm_ms / max(symm_n, 1),
             "qinv": "(G + 1e-6 I)^-1 (stands in for the eigen-bent G^-1)"}
@@ -174,6 +181,8 @@ def main() -> None:
     ap.add_argument("--reml-n", type=int, default=20000)
     ap.add_argument("--reml-m", type=int, default=50000)
     ap.add_argument("--plain-iters", type=int, default=2)
+    ap.add_argument("--reml-only", action="store_true",
+                    help="only run the REML leg and print its JSON object")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -197,6 +206,11 @@ def main() -> None:
 
     import cosmicblup_amd as cb
     from cosmicblup_amd import _lib, synth
+
+    if args.reml_only:
+        ctx = _lib.Context(0)
+        print(json.dumps(reml_leg(ctx, n=args.reml_n, m=args.reml_m, plain_iters=args.plain_iters)), flush=True)
+        return
 
     n, m = args.n, args.m
     print(f"[bench] generating BED n={n} m={m} missing={args.missing} on device", flush=True)

@@ -40,6 +40,7 @@ using namespace cbamd;
 namespace {
 
 constexpr int DOT_CHUNKS = 128;   // row chunks of the two-stage column dot
+constexpr double REPLACE_DROP2 = 1e-6;  // squared residual drop that triggers a replacement
 constexpr int MF_ROWS = 128;      // rows per k_dense_mfma tile (4 waves x 32)
 constexpr int MF_KC = 32;         // k per chunk
 constexpr int MF_KCP = MF_KC + 4; // LDS pitch of a staged X column
@@ -65,21 +66,23 @@ __global__ void k_csr_spmm(const int64_t* __restrict__ ptr, const int32_t* __res
     *y = accumulate ? *y + alpha * acc : alpha * acc;
 }
 
-// wave per (row, c): for the few long rows (fixed effects)
+// block per (row, c): for the few long rows (fixed effects; the mean row
+// touches every unknown). FP64 accumulation, fixed reduction order.
 template <class TM, class TV>
-__global__ void k_csr_spmm_long(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
-                                const TM* __restrict__ val, int64_t row0, int64_t row1,
-                                const TV* __restrict__ X, int64_t ldx, TV* __restrict__ Y,
-                                int64_t ldy) {
-    const int lane = threadIdx.x & 63;
-    const int64_t row = row0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+__global__ __launch_bounds__(256) void k_csr_spmm_long(const int64_t* __restrict__ ptr,
+                                                       const int32_t* __restrict__ col,
+                                                       const TM* __restrict__ val, const TV* __restrict__ X,
+                                                       int64_t ldx, TV* __restrict__ Y, int64_t ldy) {
+    __shared__ double red[4];
+    const int64_t row = blockIdx.x;
     const int64_t c = blockIdx.y;
-    if (row >= row1) return;
     const TV* x = X + c * ldx;
     double acc = 0;
-    for (int64_t k = ptr[row] + lane; k < ptr[row + 1]; k += 64) acc += (double)val[k] * (double)x[col[k]];
+    for (int64_t k = ptr[row] + threadIdx.x; k < ptr[row + 1]; k += 256) acc += (double)val[k] * (double)x[col[k]];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
-    if (lane == 0) Y[c * ldy + row] = (TV)acc;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) Y[c * ldy + row] = (TV)((red[0] + red[1]) + (red[2] + red[3]));
 }
 
 // MFMA 16x16x4 (FP32 or FP64). Both take lane l's operands as row (l & 15)
@@ -227,28 +230,83 @@ __global__ void k_dense_reduce(const T* __restrict__ part, int64_t ldp, int nspl
     *y = (T)((double)*y + (double)alpha * s);
 }
 
-// y_c[row] += alpha * Q[row,:] x_c for r <= 2 columns; wave per row.
+// 16-byte vector of T
+template <class T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+    static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+    static constexpr int n = 2;
+};
+
+// y_c[row] += alpha * Q[row,:] x_c for r <= 2 columns. Wave per GEMV_ROWS
+// rows: each lane streams 16-byte runs of those rows (Q rows start 128-byte
+// aligned, leading dimension a multiple of 32) and loads x once for all of
+// them; FP64 accumulation. HBM-bound: q^2 * sizeof(T) bytes per launch.
+constexpr int GEMV_ROWS = 4;
 template <class T>
 __global__ __launch_bounds__(256) void k_gemv(const T* __restrict__ Q, int64_t ldq, int64_t q,
                                               const T* __restrict__ X, int64_t ldx, int r,
                                               T* __restrict__ Y, int64_t ldy, int64_t y_off, T alpha) {
+    using VT = typename Vec16<T>::type;
+    constexpr int VW = Vec16<T>::n;
     const int lane = threadIdx.x & 63;
-    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= q) return;
-    const T* qr = Q + row * ldq;
-    double a0 = 0, a1 = 0;
-    for (int64_t k = lane; k < q; k += 64) {
-        const double v = (double)qr[k];
-        a0 += v * (double)X[k];
-        if (r > 1) a1 += v * (double)X[ldx + k];
+    const int64_t row0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * GEMV_ROWS;
+    if (row0 >= q) return;
+    const VT* qr[GEMV_ROWS];
+#pragma unroll
+    for (int a = 0; a < GEMV_ROWS; ++a) qr[a] = (const VT*)(Q + min(row0 + a, q - 1) * ldq);
+    double acc0[GEMV_ROWS], acc1[GEMV_ROWS];
+#pragma unroll
+    for (int a = 0; a < GEMV_ROWS; ++a) acc0[a] = acc1[a] = 0;
+    const int64_t nv = q / VW;
+    const bool two = r > 1;
+#pragma unroll 2
+    for (int64_t v = lane; v < nv; v += 64) {
+        double x0[VW], x1[VW];
+#pragma unroll
+        for (int e = 0; e < VW; ++e) {
+            x0[e] = (double)X[v * VW + e];
+            x1[e] = two ? (double)X[ldx + v * VW + e] : 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < GEMV_ROWS; ++a) {
+            const VT w = __builtin_nontemporal_load(qr[a] + v);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) {
+                acc0[a] += (double)w[e] * x0[e];
+                acc1[a] += (double)w[e] * x1[e];
+            }
+        }
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        a0 += __shfl_down(a0, off);
-        a1 += __shfl_down(a1, off);
+    for (int64_t k = nv * VW + lane; k < q; k += 64) {  // tail of rows whose length is not a multiple of VW
+        const double x0 = (double)X[k], x1 = two ? (double)X[ldx + k] : 0.0;
+#pragma unroll
+        for (int a = 0; a < GEMV_ROWS; ++a) {
+            const double w = (double)((const T*)qr[a])[k];
+            acc0[a] += w * x0;
+            acc1[a] += w * x1;
+        }
     }
+#pragma unroll
+    for (int a = 0; a < GEMV_ROWS; ++a)
+        for (int off = 32; off > 0; off >>= 1) {
+            acc0[a] += __shfl_down(acc0[a], off);
+            acc1[a] += __shfl_down(acc1[a], off);
+        }
     if (lane == 0) {
-        Y[y_off + row] += alpha * (T)a0;
-        if (r > 1) Y[ldy + y_off + row] += alpha * (T)a1;
+#pragma unroll
+        for (int a = 0; a < GEMV_ROWS; ++a) {
+            const int64_t row = row0 + a;
+            if (row >= q) break;
+            Y[y_off + row] += alpha * (T)acc0[a];
+            if (two) Y[ldy + y_off + row] += alpha * (T)acc1[a];
+        }
     }
 }
 
@@ -603,7 +661,8 @@ void dense_q_apply(cb_mme* M, const T* X, int64_t ldx, T* Y, int64_t ldy, int r,
     const T* Q = qdense<T>(M);
     if (r <= 2) {
         timed(ctx, sizeof(T) == 8 ? "k_gemv_f64" : "k_gemv_f32", [&] {
-            k_gemv<T><<<blocks(q, 4), 256, 0, st>>>(Q, M->ldq, q, X + M->p, ldx, r, Y, ldy, M->p, alpha);
+            k_gemv<T><<<(unsigned)ceil_div(q, 4 * GEMV_ROWS), 256, 0, st>>>(Q, M->ldq, q, X + M->p, ldx, r, Y, ldy,
+                                                                        M->p, alpha);
         });
         return;
     }
@@ -643,8 +702,8 @@ void apply_C(cb_mme* M, double lambda, const T* X, T* Y, int r) {
     const int64_t ld = M->ldv;
     // W'W part (overwrites Y)
     if (M->long_rows > 0)
-        k_csr_spmm_long<T, T><<<dim3(blocks(M->long_rows, 4), (unsigned)r), 256, 0, st>>>(
-            M->wp.p, M->wc.p, wvals<T>(M), 0, M->long_rows, X, ld, Y, ld);
+        k_csr_spmm_long<T, T><<<dim3((unsigned)M->long_rows, (unsigned)r), 256, 0, st>>>(
+            M->wp.p, M->wc.p, wvals<T>(M), X, ld, Y, ld);
     if (M->dim > M->long_rows)
         timed(ctx, sizeof(T) == 8 ? "k_csr_spmm_f64" : "k_csr_spmm_f32", [&] {
             k_csr_spmm<T, T><<<dim3(blocks(M->dim - M->long_rows), (unsigned)r), 256, 0, st>>>(
@@ -1006,9 +1065,12 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
             bool need = false;
             for (int c = 0; c < r; ++c) {
                 if (done[c]) continue;
-                // paused by the recursive test, fallen tenfold since the last
-                // replacement, or grown tenfold (lost conjugacy)
-                if (!hs[c].active || hs[c].rr < 0.01 * rr_ref[c] || !(hs[c].rr <= 100.0 * rr_ref[c])) need = true;
+                // paused by the recursive test, fallen a thousandfold since the
+                // last replacement (the FP32 recursion drifts from the true
+                // residual by about eps32 times the largest residual since
+                // then), or grown tenfold (lost conjugacy)
+                if (!hs[c].active || hs[c].rr < REPLACE_DROP2 * rr_ref[c] || !(hs[c].rr <= 100.0 * rr_ref[c]))
+                    need = true;
             }
             if (need) {
                 all_done = replace(false);
@@ -1059,8 +1121,8 @@ static SolveStats solve_block(cb_mme* M, double lambda, const cb_solve_options& 
     // Mixed: one CG recursion per column with the operator, preconditioner
     // and direction vectors in FP32, the iterate accumulated in FP64, and the
     // residual replaced by the true FP64 residual b - C x whenever the
-    // recursive one has dropped tenfold since the last replacement (and
-    // before any column is declared converged). The search directions are
+    // recursive one has dropped a thousandfold since the last replacement
+    // (and before any column is declared converged). The search directions are
     // kept across replacements.
     CB_REQUIRE(M->qkind == 1 || M->Qd32.p, "FP32 copy of Q missing");
     S.iterations = pcg_reliable(M, lambda, B, X, r, o, &S.max_rel);
@@ -1385,7 +1447,8 @@ static void find_stiff_direction(cb_mme* M) {
     for (int it = 0; it < 100; ++it) {
         CB_HIP(hipMemcpyAsync(dv.p, v.data(), q * 8, hipMemcpyHostToDevice, st));
         CB_HIP(hipMemsetAsync(dy.p, 0, q * 8, st));
-        k_gemv<double><<<blocks(q, 4), 256, 0, st>>>(M->Qd64.p, M->ldq, q, dv.p, q, 1, dy.p, q, 0, 1.0);
+        k_gemv<double><<<(unsigned)ceil_div(q, 4 * GEMV_ROWS), 256, 0, st>>>(M->Qd64.p, M->ldq, q, dv.p, q, 1, dy.p, q,
+                                                                             0, 1.0);
         CB_HIP(hipGetLastError());
         CB_HIP(hipMemcpyAsync(y.data(), dy.p, q * 8, hipMemcpyDeviceToHost, st));
         CB_HIP(hipStreamSynchronize(st));
