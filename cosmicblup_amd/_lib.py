@@ -36,7 +36,7 @@ class GrmOptions(C.Structure):
 class SolveOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("precond", C.c_int), ("stop_rule", C.c_int),
                 ("batch_rhs", C.c_int), ("tol", C.c_double), ("max_iter", C.c_int64),
-                ("inner_tol", C.c_double), ("stall_tol", C.c_double)]
+                ("stall_tol", C.c_double)]
 
 
 class RemlOptions(C.Structure):

@@ -1377,7 +1377,6 @@ void cb_solve_options_default(cb_solve_options* o) {
     o->batch_rhs = 1;
     o->tol = 1e-10;
     o->max_iter = 10000;
-    o->inner_tol = 1e-5;
     o->stall_tol = 1e-8;
 }
 

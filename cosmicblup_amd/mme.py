@@ -19,12 +19,12 @@ from .design import FixedDesignG
 
 
 def solve_options(precision=_lib.CB_PREC_MIXED, precond=_lib.CB_PRECOND_BLOCK_JACOBI,
-                  stop_rule=_lib.CB_STOP_RELATIVE, tol=1e-10, max_iter=10000, inner_tol=1e-5,
+                  stop_rule=_lib.CB_STOP_RELATIVE, tol=1e-10, max_iter=10000,
                   batch_rhs=True, stall_tol=1e-8) -> _lib.SolveOptions:
     o = _lib.SolveOptions()
     _lib.lib().cb_solve_options_default(C.byref(o))
     o.precision, o.precond, o.stop_rule = precision, precond, stop_rule
-    o.tol, o.max_iter, o.inner_tol, o.batch_rhs = tol, max_iter, inner_tol, int(batch_rhs)
+    o.tol, o.max_iter, o.batch_rhs = tol, max_iter, int(batch_rhs)
     o.stall_tol = stall_tol
     return o
 

@@ -171,12 +171,12 @@ typedef struct {
     int batch_rhs;      /* FP64 only: 1 = all columns as one block, 0 = one at a time */
     double tol;         /* default 1e-10 (config.h:41) */
     int64_t max_iter;   /* default 10000 (config.h:42) */
-    double inner_tol;   /* MIXED: residual reduction per FP32 inner solve, default 1e-5 */
-    double stall_tol;   /* MIXED: the stop test is on the true FP64 residual, which has a
-                           rounding floor near cond(C) * eps; a refinement that stalls
-                           (no halving over three steps) below stall_tol is accepted.
-                           Default 1e-8. (The reference's 1e-10 tests the recursive
-                           residual, pcg_solver.h:501.) */
+    double stall_tol;   /* MIXED: the stop test is on the true FP64 residual b - C x,
+                           which has a rounding floor near cond(C) * eps64; a column
+                           whose best true residual has not halved over eight
+                           residual replacements is accepted when it is below
+                           stall_tol. Default 1e-8. (The reference's 1e-10 tests the
+                           recursive residual, pcg_solver.h:501.) */
 } cb_solve_options;
 
 void cb_solve_options_default(cb_solve_options* opts);
