@@ -111,6 +111,11 @@ _SIGS = {
                               C.POINTER(C.c_int), c_vp, c_vp, c_vp]),
     "cb_reml_score": (C.c_int, [c_vp, C.POINTER(RemlOptions), C.c_double, C.c_double, c_vp, c_vp, c_dp, c_vp]),
     "cb_reml_copy_probes": (C.c_int, [c_vp, C.POINTER(RemlOptions), c_vp, c_i64]),
+    "cb_ainv_henderson": (C.c_int, [c_i64, c_vp, c_vp, C.POINTER(c_vp)]),
+    "cb_ainv_nnz": (C.c_int, [c_vp, c_i64p]),
+    "cb_ainv_copy_csr": (C.c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "cb_ainv_copy_inbreeding": (C.c_int, [c_vp, c_vp, c_vp]),
+    "cb_ainv_destroy": (None, [c_vp]),
 }
 
 _lib = None

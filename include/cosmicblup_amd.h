@@ -222,6 +222,23 @@ int cb_reml_score(cb_mme* mme, const cb_reml_options* opts, double var_u, double
 /* The probe matrix the run uses (q x k, column-major, leading dimension ld). */
 int cb_reml_copy_probes(cb_mme* mme, const cb_reml_options* opts, double* out, int64_t ld);
 
+/* ---- pedigree relationship inverse ----------------------------------- */
+typedef struct cb_ainv cb_ainv;
+/* genmatrix::invertA_henderson(dam, sire) (src/core/a_matrix.cpp:392-439)
+ * with the inbreeding of ml() (a_matrix.cpp:127-186): Henderson's rules with
+ * d_ii from the parents' F, the reference's duplicate-summing order,
+ * symmetrisation, diagonal floor and prune (a_matrix.cpp:58-80). Parents are
+ * 0-based row indices; an index outside [0, n) is an unknown parent
+ * (complete_missing_parents_add_phantoms, a_matrix.cpp:43-52). Parents must
+ * precede their offspring (CB_ERR_INVALID otherwise). Host code. */
+int cb_ainv_henderson(int64_t n, const int32_t* dam, const int32_t* sire, cb_ainv** out);
+int cb_ainv_nnz(const cb_ainv* a, int64_t* nnz);
+/* CSR rows (ptr: n+1, col/val: nnz), columns ascending within a row. */
+int cb_ainv_copy_csr(const cb_ainv* a, int64_t* ptr, int32_t* col, double* val);
+/* Inbreeding coefficients F and Mendelian-sampling variances d_ii (either may be NULL). */
+int cb_ainv_copy_inbreeding(const cb_ainv* a, double* F, double* dii);
+void cb_ainv_destroy(cb_ainv* a);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,11 +11,18 @@ path: dense FP64 numpy restatements of the reference algorithms.
   logL                 calc_logL_internal (vce.cpp:3533-3588)
   henderson_ainv       invertA_henderson without inbreeding
                        (src/core/a_matrix.cpp:392-439, textbook rules)
+  ml_inbreeding        ml() (a_matrix.cpp:127-186): F and d_ii by the dense
+                       descending scan, pure Python (small pedigrees)
+  henderson_ainv_f     invertA_henderson with ml() inbreeding, the
+                       reference's triplet order, symmetrisation, diagonal
+                       floor and prune (a_matrix.cpp:43-80,392-439)
 
 Pinning (tests/test_oracle_reml.py): MME solution vs the V-form BLUP/GLS
 formulas; the score vs central finite differences of logL; EM and AI fixed
 points coincide; Henderson A^-1 vs the inverse of the tabular A on small
-pedigrees and the 4-animal fixture by hand; probes vs a scalar splitmix64.
+pedigrees and the 4-animal fixture by hand; ml_inbreeding's F vs
+diag(tabular A) - 1 on inbred pedigrees and henderson_ainv_f vs the inverse
+of the tabular A there; probes vs a scalar splitmix64.
 """
 from __future__ import annotations
 
@@ -210,3 +217,74 @@ def tabular_a(sire: np.ndarray, dam: np.ndarray) -> np.ndarray:
             A[i, j] = A[j, i] = v
         A[i, i] = 1.0 + (0.5 * A[s, d] if s >= 0 and d >= 0 else 0.0)
     return A
+
+
+def ml_inbreeding(dam, sire):
+    """F and d_ii as ml() computes them (a_matrix.cpp:127-186): for an
+    animal with both parents known, L starts at 1/2 on each parent and the
+    dense scan j = max(s, d) .. 0 accumulates L_j^2 d_j while passing half of
+    L_j to each parent of j; F_i = sum + d_i - 1, clipped at 0."""
+    n = len(dam)
+    F = [0.0] * n
+    dii = [1.0] * n
+    L = [0.0] * n
+    for i in range(n):
+        s, d = int(sire[i]), int(dam[i])
+        s_ok, d_ok = 0 <= s < n, 0 <= d < n
+        Fs = F[s] if s_ok else 0.0
+        Fd = F[d] if d_ok else 0.0
+        if s_ok and d_ok:
+            dii[i] = 1.0 - 0.25 * (1.0 + Fs) - 0.25 * (1.0 + Fd)
+        elif s_ok:
+            dii[i] = 1.0 - 0.25 * (1.0 + Fs)
+        elif d_ok:
+            dii[i] = 1.0 - 0.25 * (1.0 + Fd)
+        else:
+            dii[i] = 1.0
+        if s_ok and d_ok:
+            L[s] += 0.5
+            L[d] += 0.5
+            acc = 0.0
+            for j in range(max(s, d), -1, -1):
+                if L[j] > 0.0:
+                    acc += L[j] * L[j] * dii[j]
+                    sj, dj = int(sire[j]), int(dam[j])
+                    if 0 <= sj < n:
+                        L[sj] += 0.5 * L[j]
+                    if 0 <= dj < n:
+                        L[dj] += 0.5 * L[j]
+                    L[j] = 0.0
+            f = acc + dii[i] - 1.0
+            F[i] = f if f >= 0.0 else 0.0
+        else:
+            F[i] = 0.0
+    return np.array(F), np.array(dii)
+
+
+def henderson_ainv_f(dam, sire):
+    """Dense A^-1 as invertA_henderson builds it (a_matrix.cpp:392-439):
+    parents outside [0, n) become unknown, F and d_ii from ml_inbreeding,
+    triplets summed in insertion order, 0.5 (M + M^T), diagonal < 1e-15 ->
+    1e-12, entries with |v| <= 1e-24 dropped (zero)."""
+    n = len(dam)
+    d = [int(x) if 0 <= int(x) < n else -1 for x in dam]
+    s = [int(x) if 0 <= int(x) < n else -1 for x in sire]
+    F, dii = ml_inbreeding(d, s)
+    M = np.zeros((n, n))
+    for i in range(n):
+        di = 1.0 / dii[i]
+        trip = [(i, i, di)]
+        if s[i] >= 0:
+            trip += [(i, s[i], -0.5 * di), (s[i], i, -0.5 * di), (s[i], s[i], 0.25 * di)]
+        if d[i] >= 0:
+            trip += [(i, d[i], -0.5 * di), (d[i], i, -0.5 * di), (d[i], d[i], 0.25 * di)]
+        if s[i] >= 0 and d[i] >= 0:
+            trip += [(s[i], d[i], 0.25 * di), (d[i], s[i], 0.25 * di)]
+        for r, c, v in trip:  # row-major accumulation in insertion order per entry
+            M[r, c] += v
+    M = 0.5 * (M + M.T)
+    dg = np.diag(M).copy()
+    dg[dg < 1e-15] = 1e-12
+    np.fill_diagonal(M, dg)
+    M[np.abs(M) <= 1e-24] = 0.0
+    return M, F, dii
