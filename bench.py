@@ -172,6 +172,62 @@ def reml_leg(ctx, n: int = 20000, m: int = 50000, k: int = 64, h2: float = 0.4, 
             "qinv": "(G + 1e-6 I)^-1 (stands in for the eigen-bent G^-1)"}
 
 
+def pcg_leg(ctx, n_gen: int = 10, per_gen: int = 100000, n_rec: int = 600000, n_groups: int = 2000,
+            h2: float = 0.25, tol: float = 1e-10, seed: int = 20263203) -> dict:
+    """PBLUP sparse-MME PCG at the 1M-animal pedigree shape: A^-1 by
+    Henderson's rules with inbreeding (host builder), 600k records, 2000
+    contemporary groups; one right-hand side solved to relative residual
+    `tol` by GPU FP64 PCG and by the mixed FP32/FP64-replacement PCG."""
+    import time
+
+    import numpy as np
+
+    from cosmicblup_amd import _lib, design, mme, synth
+    from cosmicblup_amd.pedigree import invertA_henderson
+
+    print(f"[bench] PCG leg: pedigree {n_gen}x{per_gen}, {n_rec} records, {n_groups} groups", flush=True)
+    dam, sire = synth.simulate_pedigree(n_gen, per_gen, unknown=0.05, seed=seed)
+    t0 = time.perf_counter()
+    A = invertA_henderson(dam, sire)
+    t_ainv = time.perf_counter() - t0
+    rec, y, grp = synth.simulate_pedigree_phenotype(dam, sire, A.dii, n_rec, h2=h2, n_groups=n_groups,
+                                                    seed=seed + 1)
+    width = len(str(n_groups))
+    fd = design.build_fixed_design([[f"cg{int(c):0{width}d}"] for c in grp])
+    model = mme.MixedModel(fd, rec, y, dam.size, ctx=ctx)
+    model.set_qinv_csr(A.ptr, A.col, A.val)
+    nnz = int(A.ptr[-1])
+    del A
+    lam = (1.0 - h2) / h2
+    b = model.rhs()
+    out = {"animals": int(dam.size), "records": n_rec, "p": model.p, "ainv_nnz": nnz,
+           "ainv_build_s": t_ainv, "lambda": lam, "tol": tol}
+    sols = {}
+    for name, prec in (("fp64", _lib.CB_PREC_FP64), ("mixed", _lib.CB_PREC_MIXED)):
+        opts = mme.solve_options(precision=prec, tol=tol, max_iter=20000)
+        model.solve(lam, b, opts)  # warm-up: preconditioner set-up, allocations
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        x, it, rel = model.solve(lam, b, opts)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        sols[name] = x
+        out[name] = {"iterations": it, "seconds": dt, "iters_per_s": it / dt, "rel_residual": rel}
+        print(f"[bench] PCG {name}: {it} iterations in {dt:.3f} s (rel {rel:.2e})", flush=True)
+    # both stop at relative residual `tol`; their EBVs differ by up to
+    # cond(C) * tol. Against a tight FP64 solve (1e-13) each shows its own error.
+    x_ref, it_ref, _ = model.solve(lam, b, mme.solve_options(precision=_lib.CB_PREC_FP64, tol=1e-13,
+                                                             max_iter=20000))
+    p = model.p
+    u_ref = x_ref[p:]
+    scale = np.abs(u_ref).max()
+    out["ebv_rel_err_mixed_vs_fp64"] = float(np.abs(sols["mixed"][p:] - sols["fp64"][p:]).max() / scale)
+    out["ebv_rel_err_vs_fp64_1e-13"] = {k: float(np.abs(v[p:] - u_ref).max() / scale) for k, v in sols.items()}
+    out["fp64_1e-13_iterations"] = it_ref
+    out["time_speedup_mixed_vs_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +243,9 @@ def main() -> None:
     ap.add_argument("--plain-iters", type=int, default=2)
     ap.add_argument("--reml-only", action="store_true",
                     help="only run the REML leg and print its JSON object")
+    ap.add_argument("--no-pcg", action="store_true", help="skip the 1M-animal PBLUP PCG leg")
+    ap.add_argument("--pcg-only", action="store_true",
+                    help="only run the PBLUP PCG leg and print its JSON object")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -214,6 +273,10 @@ def main() -> None:
     if args.reml_only:
         ctx = _lib.Context(0)
         print(json.dumps(reml_leg(ctx, n=args.reml_n, m=args.reml_m, plain_iters=args.plain_iters)), flush=True)
+        return
+    if args.pcg_only:
+        ctx = _lib.Context(0)
+        print(json.dumps(pcg_leg(ctx)), flush=True)
         return
 
     n, m = args.n, args.m
@@ -275,6 +338,9 @@ def main() -> None:
     torch.cuda.empty_cache()
     if not args.no_reml:
         out["reml"] = reml_leg(ctx, n=args.reml_n, m=args.reml_m, plain_iters=args.plain_iters)
+        torch.cuda.empty_cache()
+    if not args.no_pcg:
+        out["pblup_pcg"] = pcg_leg(ctx)
     if not args.no_cpu:
         torch.cuda.empty_cache()
         print("[bench] timing the CPU oracle sample", flush=True)

@@ -41,6 +41,7 @@ namespace {
 
 constexpr int DOT_CHUNKS = 128;   // row chunks of the two-stage column dot
 constexpr double REPLACE_DROP2 = 1e-6;  // squared residual drop that triggers a replacement
+constexpr int64_t LONG_CHUNK = 4096;    // nonzeros per block of a long W'W row
 constexpr int MF_ROWS = 128;      // rows per k_dense_mfma tile (4 waves x 32)
 constexpr int MF_KC = 32;         // k per chunk
 constexpr int MF_KCP = MF_KC + 4; // LDS pitch of a staged X column
@@ -66,23 +67,37 @@ __global__ void k_csr_spmm(const int64_t* __restrict__ ptr, const int32_t* __res
     *y = accumulate ? *y + alpha * acc : alpha * acc;
 }
 
-// block per (row, c): for the few long rows (fixed effects; the mean row
-// touches every unknown). FP64 accumulation, fixed reduction order.
+// The few long rows of W'W (fixed effects; the mean row touches every
+// unknown) in two deterministic stages: block per (chunk of <= LONG_CHUNK
+// nonzeros, column) -> partial, then thread per (row, column) summing its
+// chunks in order. FP64 accumulation.
 template <class TM, class TV>
-__global__ __launch_bounds__(256) void k_csr_spmm_long(const int64_t* __restrict__ ptr,
-                                                       const int32_t* __restrict__ col,
-                                                       const TM* __restrict__ val, const TV* __restrict__ X,
-                                                       int64_t ldx, TV* __restrict__ Y, int64_t ldy) {
+__global__ __launch_bounds__(256) void k_csr_spmm_chunks(const int64_t* __restrict__ cbeg,
+                                                         const int64_t* __restrict__ cend,
+                                                         int64_t nchunks, const int32_t* __restrict__ col,
+                                                         const TM* __restrict__ val, const TV* __restrict__ X,
+                                                         int64_t ldx, double* __restrict__ part) {
     __shared__ double red[4];
-    const int64_t row = blockIdx.x;
+    const int64_t ch = blockIdx.x;
     const int64_t c = blockIdx.y;
     const TV* x = X + c * ldx;
     double acc = 0;
-    for (int64_t k = ptr[row] + threadIdx.x; k < ptr[row + 1]; k += 256) acc += (double)val[k] * (double)x[col[k]];
+    for (int64_t k = cbeg[ch] + threadIdx.x; k < cend[ch]; k += 256) acc += (double)val[k] * (double)x[col[k]];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) Y[c * ldy + row] = (TV)((red[0] + red[1]) + (red[2] + red[3]));
+    if (threadIdx.x == 0) part[c * nchunks + ch] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+template <class TV>
+__global__ void k_csr_chunks_finish(const int64_t* __restrict__ rstart, int64_t rows, int64_t nchunks,
+                                    const double* __restrict__ part, TV* __restrict__ Y, int64_t ldy) {
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    if (row >= rows) return;
+    double s = 0;
+    for (int64_t k = rstart[row]; k < rstart[row + 1]; ++k) s += part[c * nchunks + k];
+    Y[c * ldy + row] = (TV)s;
 }
 
 // MFMA 16x16x4 (FP32 or FP64). Both take lane l's operands as row (l & 15)
@@ -349,16 +364,20 @@ __global__ void k_precond_diag(const T* __restrict__ R, T* __restrict__ Z, const
     Z[(int64_t)c * ld + i] = (T)((double)R[(int64_t)c * ld + i] * dinv[i]);
 }
 
-// block part: z[0:p, c] = Binv r[0:p, c]  (Binv p x p row-major, FP64)
+// block part: z[0:p, c] = Binv r[0:p, c]  (Binv p x p row-major, FP64);
+// wave per row, coalesced along the row
 template <class T>
-__global__ void k_precond_block(const T* __restrict__ R, T* __restrict__ Z, const double* __restrict__ Binv,
-                                int64_t p, int64_t ld) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_precond_block(const T* __restrict__ R, T* __restrict__ Z,
+                                                       const double* __restrict__ Binv, int64_t p, int64_t ld) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int c = blockIdx.y;
     if (i >= p) return;
+    const T* rc = R + (int64_t)c * ld;
     double s = 0;
-    for (int64_t k = 0; k < p; ++k) s += Binv[i * p + k] * (double)R[(int64_t)c * ld + k];
-    Z[(int64_t)c * ld + i] = (T)s;
+    for (int64_t k = lane; k < p; k += 64) s += Binv[i * p + k] * (double)rc[k];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+    if (lane == 0) Z[(int64_t)c * ld + i] = (T)s;
 }
 
 // partial dots of each column's u-block with the stiff vector
@@ -585,7 +604,11 @@ struct cb_mme {
     DevBuf<int32_t> wc;
     DevBuf<double> wv64;
     DevBuf<float> wv32;
-    int64_t long_rows = 0;  // rows [0, long_rows) use the wave-per-row SpMM
+    int64_t long_rows = 0;  // rows [0, long_rows) use the chunked two-stage SpMM
+    // chunks of at most LONG_CHUNK nonzeros of the long rows, in row order
+    int64_t n_lchunks = 0;
+    DevBuf<int64_t> lc_beg, lc_end, lr_start;
+    DevBuf<double> lc_part;
     std::vector<double> wtw_diag;  // diagonal of W'W
     std::vector<double> xtx;       // p x p block (constant in lambda)
     std::vector<double> rhs;       // W'y
@@ -701,9 +724,14 @@ void apply_C(cb_mme* M, double lambda, const T* X, T* Y, int r) {
     hipStream_t st = ctx->stream;
     const int64_t ld = M->ldv;
     // W'W part (overwrites Y)
-    if (M->long_rows > 0)
-        k_csr_spmm_long<T, T><<<dim3((unsigned)M->long_rows, (unsigned)r), 256, 0, st>>>(
-            M->wp.p, M->wc.p, wvals<T>(M), X, ld, Y, ld);
+    if (M->long_rows > 0) {
+        if (M->lc_part.n < (size_t)M->n_lchunks * r) M->lc_part.alloc((size_t)M->n_lchunks * r);
+        if (M->n_lchunks > 0)
+            k_csr_spmm_chunks<T, T><<<dim3((unsigned)M->n_lchunks, (unsigned)r), 256, 0, st>>>(
+            M->lc_beg.p, M->lc_end.p, M->n_lchunks, M->wc.p, wvals<T>(M), X, ld, M->lc_part.p);
+        k_csr_chunks_finish<T><<<dim3(blocks(M->long_rows), (unsigned)r), 256, 0, st>>>(
+            M->lr_start.p, M->long_rows, M->n_lchunks, M->lc_part.p, Y, ld);
+    }
     if (M->dim > M->long_rows)
         timed(ctx, sizeof(T) == 8 ? "k_csr_spmm_f64" : "k_csr_spmm_f32", [&] {
             k_csr_spmm<T, T><<<dim3(blocks(M->dim - M->long_rows), (unsigned)r), 256, 0, st>>>(
@@ -822,7 +850,7 @@ void apply_precond(cb_mme* M, const T* R, T* Z, int r) {
     hipStream_t st = M->ctx->stream;
     k_precond_diag<T><<<dim3(blocks(M->dim - M->p), (unsigned)r), 256, 0, st>>>(R, Z, M->dinv.p, M->p, M->dim, M->ldv);
     if (M->p > 0)
-        k_precond_block<T><<<dim3(blocks(M->p), (unsigned)r), 256, 0, st>>>(R, Z, M->Binv.p, M->p, M->ldv);
+        k_precond_block<T><<<dim3(blocks(M->p, 4), (unsigned)r), 256, 0, st>>>(R, Z, M->Binv.p, M->p, M->ldv);
     if (M->stiff) {
         // z_u -= gamma (v' z_u) D'^-1 v
         if (M->stiff_part.n < (size_t)DOT_CHUNKS * r) M->stiff_part.alloc((size_t)DOT_CHUNKS * r);
@@ -1184,6 +1212,15 @@ static void assemble(cb_mme* M, const int64_t* x_ptr, const int32_t* x_col, cons
     for (int64_t r = 0; r < p; ++r)
         if (wp[r + 1] - wp[r] > 32) long_rows = r + 1;
     M->long_rows = long_rows;
+    std::vector<int64_t> lcb, lce, lrs(long_rows + 1, 0);
+    for (int64_t r = 0; r < long_rows; ++r) {
+        for (int64_t k = wp[r]; k < wp[r + 1]; k += LONG_CHUNK) {
+            lcb.push_back(k);
+            lce.push_back(std::min<int64_t>(wp[r + 1], k + LONG_CHUNK));
+        }
+        lrs[r + 1] = (int64_t)lcb.size();
+    }
+    M->n_lchunks = (int64_t)lcb.size();
     // RHS = W'y (mme_builder.cpp:321-328), y'y
     M->rhs.assign(dim, 0.0);
     for (int64_t i = 0; i < dim; ++i)
@@ -1203,6 +1240,7 @@ static void assemble(cb_mme* M, const int64_t* x_ptr, const int32_t* x_col, cons
     up(M->xp, xp); up(M->xc, xc); up(M->xv, xv); up(M->aid, av); up(M->y, yv);
     up(M->tp, tp); up(M->tc, tc); up(M->tv, tv);
     up(M->wp, wp); up(M->wc, wc); up(M->wv64, wv);
+    up(M->lc_beg, lcb); up(M->lc_end, lce); up(M->lr_start, lrs);
     std::vector<float> wv32(wv.begin(), wv.end());
     up(M->wv32, wv32);
     CB_HIP(hipStreamSynchronize(st));

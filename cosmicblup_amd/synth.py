@@ -122,3 +122,51 @@ def simulate_phenotype(dosage: np.ndarray, n_qtl: int = 500, h2: float = 0.3, n_
     width = len(str(n_groups))
     labels = [f"cg{int(c):0{width}d}" for c in grp]
     return y, labels
+
+
+def simulate_pedigree(n_gen: int, per_gen: int, unknown: float = 0.05, seed: int = 3):
+    """Discrete generations of per_gen animals; sire and dam of every
+    non-founder drawn uniformly from the previous generation, each unknown
+    with probability `unknown`. Returns (dam, sire) as int32 row indices
+    (-1 unknown); parents precede offspring."""
+    rng = np.random.default_rng(seed)
+    n = n_gen * per_gen
+    dam = np.full(n, -1, dtype=np.int32)
+    sire = np.full(n, -1, dtype=np.int32)
+    for g in range(1, n_gen):
+        lo = (g - 1) * per_gen
+        s = lo + rng.integers(0, per_gen, per_gen)
+        d = lo + rng.integers(0, per_gen, per_gen)
+        s[rng.random(per_gen) < unknown] = -1
+        d[rng.random(per_gen) < unknown] = -1
+        sire[g * per_gen:(g + 1) * per_gen] = s
+        dam[g * per_gen:(g + 1) * per_gen] = d
+    return dam, sire
+
+
+def simulate_pedigree_phenotype(dam, sire, dii, n_rec: int, h2: float = 0.25, n_groups: int = 2000,
+                                seed: int = 4):
+    """Breeding values down the pedigree, u_i = (u_s + u_d)/2 + sqrt(d_ii)
+    N(0, h2) (unknown parent = 0), then n_rec distinct animals phenotyped:
+    y = 10 + CG + u + e, e ~ N(0, 1 - h2), CG ~ N(0, 1) over n_groups groups.
+    Returns (record animal rows int32, y, group index per record)."""
+    rng = np.random.default_rng(seed)
+    n = dam.size
+    u = np.zeros(n)
+    ms = rng.standard_normal(n) * np.sqrt(np.asarray(dii) * h2)
+    # parents precede offspring: one pass in index order, vectorised per block
+    # of animals whose parents all lie before the block
+    i = 0
+    while i < n:
+        j = i + 1
+        lim = min(n, i + 65536)
+        while j < lim and max(dam[j], sire[j]) < i:
+            j += 1
+        pu = np.where(sire[i:j] >= 0, u[np.maximum(sire[i:j], 0)], 0.0)
+        pd = np.where(dam[i:j] >= 0, u[np.maximum(dam[i:j], 0)], 0.0)
+        u[i:j] = 0.5 * (pu + pd) + ms[i:j]
+        i = j
+    rec = np.sort(rng.choice(n, n_rec, replace=False)).astype(np.int32)
+    grp = rng.integers(0, n_groups, n_rec)
+    y = 10.0 + rng.standard_normal(n_groups)[grp] + u[rec] + rng.standard_normal(n_rec) * np.sqrt(1.0 - h2)
+    return rec, y, grp
