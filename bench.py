@@ -77,42 +77,26 @@ def reml_leg(ctx, n: int = 20000, m: int = 50000, k: int = 64, h2: float = 0.4, 
     import torch
 
     import cosmicblup_amd as cb
-    from cosmicblup_amd import _lib, design, mme, synth
+    from cosmicblup_amd import _lib, design, linalg, mme, synth
 
     print(f"[bench] REML leg: n={n} m={m} probes={k}", flush=True)
     rows, pitch = synth.bed_on_device(n, m, maf=(0.05, 0.5), missing_rate=0.0, seed=seed)
     reader = cb.PlinkReader.from_device_rows(ctx, rows.data_ptr(), pitch, n, m)
     gb = cb.GrmBuilder(cb.Options(), ctx=ctx)
     gb.compute(reader, outputs=cb.CB_GRM_OUT_F64)
-    # G as a torch view (setup): Q = (G + 1e-6 I)^-1 stands in for the
-    # eigen-bent inverse (same 1e-6 floor on the centring null direction)
+    # G on the device (torch buffer), then G^-1 by the library's eigen-bent
+    # inverse (invert_psd_matrix, as the reference's GBLUP path builds it) — set-up
     G = torch.empty((n, n), dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
     gb.copyGrmToDevice(G.data_ptr(), n)
     del gb, reader
-    # Q = (G + 1e-6 I)^-1 (setup, not timed): torch's Cholesky inverse on the
-    # device through MAGMA when available, else host LAPACK (dpotrf + dpotri)
-    G.diagonal().add_(1e-6)
-    Q = None
-    try:
-        torch.backends.cuda.preferred_linalg_library("magma")
-        Q = torch.cholesky_inverse(torch.linalg.cholesky(G))
-        torch.cuda.synchronize()
-    except RuntimeError as err:
-        print(f"[bench] device inverse failed ({str(err)[:120]}); host LAPACK", flush=True)
-    if Q is None:
-        import scipy.linalg.lapack as lapack
-
-        Gh = G.cpu().numpy()
-        del G
-        c, info = lapack.dpotrf(Gh, lower=1, overwrite_a=1)
-        assert info == 0, f"dpotrf info={info}"
-        Qh, info = lapack.dpotri(c, lower=1, overwrite_c=1)
-        assert info == 0, f"dpotri info={info}"
-        Q = torch.from_numpy(np.tril(Qh) + np.tril(Qh, -1).T).cuda()
-        del Qh, c, Gh
-    else:
-        del G
+    Q = torch.empty_like(G)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lo, hi = linalg.invert_psd_device(G.data_ptr(), n, n, Q.data_ptr(), n, ctx=ctx)
+    t_inv = time.perf_counter() - t0
+    print(f"[bench] G^-1 (eigen-bent) in {t_inv:.1f} s; eigenvalues [{lo:.3g}, {hi:.3g}]", flush=True)
+    del G
     torch.cuda.synchronize()
     # phenotype: 500 QTL, h2, 20 contemporary groups
     rng = np.random.default_rng(seed)
@@ -169,7 +153,7 @@ def reml_leg(ctx, n: int = 20000, m: int = 50000, k: int = 64, h2: float = 0.4, 
             "ebv_rel_err_mixed_vs_fp64": ebv_err,
             "trace_rel_err_mixed_vs_fp64": abs(tr32 - tr64) / abs(tr64),
             "k_dense_mfma_f32_avg_ms": symm_ms / max(symm_n, 1),
-            "qinv": "(G + 1e-6 I)^-1 (stands in for the eigen-bent G^-1)"}
+            "qinv": "eigen-bent G^-1 (invert_psd_matrix semantics)", "qinv_setup_s": t_inv}
 
 
 def pcg_leg(ctx, n_gen: int = 10, per_gen: int = 100000, n_rec: int = 600000, n_groups: int = 2000,

@@ -108,9 +108,12 @@ _SIGS = {
     "cb_mme_solve": (C.c_int, [c_vp, C.c_double, C.POINTER(SolveOptions), c_i64, c_vp, c_vp, c_i64p, c_dp]),
     "cb_reml_options_default": (None, [C.POINTER(RemlOptions)]),
     "cb_reml_run": (C.c_int, [c_vp, C.POINTER(RemlOptions), C.c_double, C.c_double, c_dp, c_dp, c_i64p,
-                              C.POINTER(C.c_int), c_vp, c_vp, c_vp]),
+                              C.POINTER(C.c_int), c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "cb_reml_logl": (C.c_int, [c_vp, C.c_double, C.c_double, c_dp, c_dp]),
     "cb_reml_score": (C.c_int, [c_vp, C.POINTER(RemlOptions), C.c_double, C.c_double, c_vp, c_vp, c_dp, c_vp]),
     "cb_reml_copy_probes": (C.c_int, [c_vp, C.POINTER(RemlOptions), c_vp, c_i64]),
+    "cb_invert_psd": (C.c_int, [c_vp, c_vp, c_i64, c_i64, C.c_double, c_vp, c_i64, c_dp, c_dp]),
+    "cb_invert_psd_device": (C.c_int, [c_vp, c_vp, c_i64, c_i64, C.c_double, c_vp, c_i64, c_dp, c_dp]),
     "cb_ainv_henderson": (C.c_int, [c_i64, c_vp, c_vp, C.POINTER(c_vp)]),
     "cb_ainv_nnz": (C.c_int, [c_vp, c_i64p]),
     "cb_ainv_copy_csr": (C.c_int, [c_vp, c_vp, c_vp, c_vp]),

@@ -15,6 +15,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libcosmicblup_amd.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM_LIB = os.environ.get("ROCM_LIB", "/opt/rocm/lib")
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",
@@ -59,7 +60,8 @@ def build(verbose: bool = False, jobs: int = 8) -> Path:
         msg = "\n".join(f"--- {s.name}\n{t}" for s, t in failed)
         raise RuntimeError(f"hipcc failed:\n{msg}")
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
+           f"-L{ROCM_LIB}", "-lrocsolver", "-lrocblas", f"-Wl,-rpath,{ROCM_LIB}"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

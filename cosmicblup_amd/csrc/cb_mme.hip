@@ -28,6 +28,9 @@
 //   k_csr_spmm    W'W and CSR Q, HBM/L2-bound (8-12 bytes per nonzero)
 //   k_col_dots    per-column FP64 dot products, two-stage deterministic
 //   k_pcg_*       fused CG updates, HBM-bound
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -869,6 +872,41 @@ void col_dots(cb_mme* M, const T* A, const T* B, int r, double* part, double* ou
     CB_HIP(hipGetLastError());
 }
 
+// C[row, col - col_off] (column-major, ld) += alpha * CSR(row, col) for rows [0, rows)
+__global__ void k_dense_from_csr(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                 const double* __restrict__ val, int64_t rows, int64_t col_off, double alpha,
+                                 double* __restrict__ C, int64_t ld) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    for (int64_t k = ptr[r]; k < ptr[r + 1]; ++k) C[((int64_t)col[k] - col_off) * ld + r] += alpha * val[k];
+}
+
+// C[i, j] += alpha * Q[i, j] (Q row-major with leading dimension ldq, symmetric)
+__global__ void k_dense_add_q(const double* __restrict__ Q, int64_t ldq, int64_t q, double alpha,
+                              double* __restrict__ C, int64_t ld) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (i >= q) return;
+    C[j * ld + i] += alpha * Q[j * ldq + i];
+}
+
+// partial sums of log(L_ii) over DOT_CHUNKS row chunks (fixed order)
+__global__ __launch_bounds__(256) void k_logdiag_part(const double* __restrict__ L, int64_t n,
+                                                      double* __restrict__ part) {
+    __shared__ double red[256];
+    const int64_t per = (n + DOT_CHUNKS - 1) / DOT_CHUNKS;
+    const int64_t r0 = (int64_t)blockIdx.x * per, r1 = min(n, r0 + per);
+    double s = 0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) s += log(L[i * n + i]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
 struct SolveStats {
     int64_t iterations = 0;
     double max_rel = 0.0;
@@ -1667,7 +1705,7 @@ int cb_reml_score(cb_mme* M, const cb_reml_options* opts, double var_u, double v
 // (vce.cpp:2414-2857) or the EM + Hutchinson step (vce.cpp:1490-1574).
 int cb_reml_run(cb_mme* M, const cb_reml_options* opts, double var_u0, double var_e0, double* var_u,
                 double* var_e, int64_t* iterations, int* converged, double* hist_vu, double* hist_ve,
-                double* solution) {
+                double* solution, double* ai_last, double* se) {
     return guarded([&] {
         CB_REQUIRE(M && opts && var_u && var_e, "cb_reml_run: NULL argument");
         CB_REQUIRE(M->qkind >= 0, "cb_reml_run: Qinv not set");
@@ -1701,6 +1739,8 @@ int cb_reml_run(cb_mme* M, const cb_reml_options* opts, double var_u0, double va
         double vu = var_u0, ve = var_e0;
         int64_t it = 0;
         int conv = 0;
+        double last_ai[4] = {0, 0, 0, 0};
+        bool have_ai = false;
         for (; it < o.max_iter;) {
             const double old_u = vu, old_e = ve;
             RemlState s;
@@ -1714,6 +1754,8 @@ int cb_reml_run(cb_mme* M, const cb_reml_options* opts, double var_u0, double va
             } else {
                 double A[4] = {s.ai[0], s.ai[1], s.ai[2], s.ai[3]};
                 project_spd_2x2(A);
+                std::copy(A, A + 4, last_ai);  // last_AI_mat (vce.cpp:3245-3252)
+                have_ai = true;
                 const double det = A[0] * A[3] - A[1] * A[2];
                 double d0 = (A[3] * s.grad[0] - A[1] * s.grad[1]) / det;
                 double d1 = (A[0] * s.grad[1] - A[2] * s.grad[0]) / det;
@@ -1753,12 +1795,86 @@ int cb_reml_run(cb_mme* M, const cb_reml_options* opts, double var_u0, double va
         *var_e = ve;
         if (iterations) *iterations = it;
         if (converged) *converged = conv;
+        if (ai_last || se) {
+            if (!have_ai) {  // EM run: the AI matrix at the estimates
+                RemlState s;
+                reml_eval(M, o, w, vu, ve, true, s, nullptr);
+                std::copy(s.ai, s.ai + 4, last_ai);
+                project_spd_2x2(last_ai);
+            }
+            if (ai_last) std::copy(last_ai, last_ai + 4, ai_last);
+            if (se) {
+                // calculate_SE (vce.cpp:3342-3352): Cov = AI^-1, SE = sqrt(max(0, diag))
+                const double det = last_ai[0] * last_ai[3] - last_ai[1] * last_ai[2];
+                se[0] = std::sqrt(std::max(0.0, last_ai[3] / det));
+                se[1] = std::sqrt(std::max(0.0, last_ai[0] / det));
+            }
+        }
         if (solution) {
             // final solve at the estimates (GSSolve uses lambda = ve/vu)
             const double lambda = ve / std::max(vu, 1e-9);
             solve_block(M, lambda, o.solve, w.B.p, w.S.p, 1);
             CB_HIP(hipMemcpy(solution, w.S.p, M->dim * 8, hipMemcpyDeviceToHost));
         }
+    });
+}
+
+int cb_reml_logl(cb_mme* M, double var_u, double var_e, double* logl, double* logdet_c) {
+    return guarded([&] {
+        CB_REQUIRE(M && logl, "cb_reml_logl: NULL argument");
+        CB_REQUIRE(M->qkind >= 0, "cb_reml_logl: Qinv not set");
+        CB_REQUIRE(var_u > 0 && var_e > 0, "cb_reml_logl: variances must be positive");
+        CB_REQUIRE(M->dim <= 65536, "cb_reml_logl: dense factorisation limited to p + q <= 65536");
+        set_device(M->ctx);
+        hipStream_t st = M->ctx->stream;
+        const int64_t dim = M->dim, p = M->p, q = M->q;
+        const double lambda = var_e / std::max(var_u, 1e-9);
+        DevBuf<double> C, b;
+        C.alloc((size_t)dim * dim);
+        b.alloc(dim);
+        CB_HIP(hipMemsetAsync(C.p, 0, C.bytes(), st));
+        k_dense_from_csr<<<blocks(dim), 256, 0, st>>>(M->wp.p, M->wc.p, M->wv64.p, dim, 0, 1.0, C.p, dim);
+        if (M->qkind == 0)
+            k_dense_add_q<<<dim3(blocks(q), (unsigned)q), 256, 0, st>>>(M->Qd64.p, M->ldq, q, lambda, C.p + p * dim + p, dim);
+        else
+            k_dense_from_csr<<<blocks(q), 256, 0, st>>>(M->qp.p, M->qc.p, M->qv64.p, q, 0, lambda, C.p + p * dim + p, dim);
+        CB_HIP(hipGetLastError());
+        CB_HIP(hipMemcpyAsync(b.p, M->rhs.data(), dim * 8, hipMemcpyHostToDevice, st));
+        rocblas_handle h = nullptr;
+        CB_REQUIRE(rocblas_create_handle(&h) == rocblas_status_success, "cb_reml_logl: rocblas handle");
+        struct HG { rocblas_handle h; ~HG() { rocblas_destroy_handle(h); } } hg{h};
+        DevBuf<int> info;
+        info.alloc(1);
+        CB_REQUIRE(rocblas_set_stream(h, st) == rocblas_status_success, "cb_reml_logl: rocblas stream");
+        CB_REQUIRE(rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)dim, C.p, (rocblas_int)dim, info.p) ==
+                       rocblas_status_success, "cb_reml_logl: dpotrf failed");
+        int hinfo = 0;
+        CB_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        CB_HIP(hipStreamSynchronize(st));
+        if (hinfo != 0) {  // not positive definite: the reference's failed-determinant value
+            *logl = -1e20;
+            if (logdet_c) *logdet_c = NAN;
+            return;
+        }
+        CB_REQUIRE(rocsolver_dpotrs(h, rocblas_fill_lower, (rocblas_int)dim, 1, C.p, (rocblas_int)dim, b.p,
+                                    (rocblas_int)dim) == rocblas_status_success, "cb_reml_logl: dpotrs failed");
+        DevBuf<double> part;
+        part.alloc(DOT_CHUNKS);
+        k_logdiag_part<<<DOT_CHUNKS, 256, 0, st>>>(C.p, dim, part.p);
+        std::vector<double> hp(DOT_CHUNKS), sol(dim);
+        CB_HIP(hipMemcpyAsync(hp.data(), part.p, DOT_CHUNKS * 8, hipMemcpyDeviceToHost, st));
+        CB_HIP(hipMemcpyAsync(sol.data(), b.p, dim * 8, hipMemcpyDeviceToHost, st));
+        CB_HIP(hipStreamSynchronize(st));
+        double ld = 0;
+        for (double v : hp) ld += v;
+        ld *= 2.0;  // log|C| = 2 sum log L_ii
+        double sr = 0;
+        for (int64_t i = 0; i < dim; ++i) sr += sol[i] * M->rhs[i];
+        // calc_logL_internal (vce.cpp:3533-3588)
+        const double yPy = (M->yty - sr) / var_e;
+        const double n = (double)M->n;
+        *logl = -0.5 * (ld + (n - (double)p - (double)q) * std::log(var_e) + (double)q * std::log(std::max(var_u, 1e-9)) + yPy);
+        if (logdet_c) *logdet_c = ld;
     });
 }
 

@@ -126,6 +126,8 @@ class AIREML:
         self.history: list[tuple[float, float]] = []
         self.iterations = 0
         self.converged = False
+        self.ai_last = None
+        self.var_u_se = self.var_e_se = None
 
     def setInitialVars(self, var_u: float, var_e: float) -> None:
         self.var_u0, self.var_e0 = var_u, var_e
@@ -139,10 +141,14 @@ class AIREML:
         vu, ve = C.c_double(), C.c_double()
         it = C.c_int64()
         conv = C.c_int()
+        ai = np.empty(4)
+        se = np.empty(2)
         _lib.check(_lib.lib().cb_reml_run(m.handle, C.byref(self.opts), self.var_u0, self.var_e0,
                                           C.byref(vu), C.byref(ve), C.byref(it), C.byref(conv),
-                                          _lib.ptr(hu), _lib.ptr(he), _lib.ptr(sol)))
+                                          _lib.ptr(hu), _lib.ptr(he), _lib.ptr(sol), _lib.ptr(ai), _lib.ptr(se)))
         self.var_u, self.var_e = vu.value, ve.value
+        self.ai_last = ai.reshape(2, 2)
+        self.var_u_se, self.var_e_se = float(se[0]), float(se[1])
         self.iterations, self.converged = it.value, bool(conv.value)
         self.history = list(zip(hu[: it.value], he[: it.value]))
         self.solution = sol
@@ -164,7 +170,25 @@ class AIREML:
         _lib.check(_lib.lib().cb_reml_copy_probes(m.handle, C.byref(self.opts), _lib.ptr(out), m.q))
         return out.T
 
+    def logL(self, var_u: float, var_e: float) -> float:
+        """calc_logL_internal (vce.cpp:3533-3588) at the given variances."""
+        v = C.c_double()
+        _lib.check(_lib.lib().cb_reml_logl(self.model.handle, var_u, var_e, C.byref(v), None))
+        return v.value
+
+    def calculate_SE(self) -> None:
+        """SEs from the last AI matrix; solve() already computed them
+        (vce.cpp:3197-3368)."""
+        if self.var_u_se is None:
+            raise RuntimeError("calculate_SE: run solve() first")
+
     # VCESolver getters
+    def getSEVarsU(self) -> list[float]:
+        return [self.var_u_se]
+
+    def getSEVarE(self) -> float:
+        return self.var_e_se
+
     def getVarE(self) -> float:
         return self.var_e
 

@@ -209,18 +209,40 @@ void cb_reml_options_default(cb_reml_options* opts);
  * iterate from (var_u0, var_e0) until sum |d sigma^2| < tol or max_iter.
  * hist_vu / hist_ve (optional, length max_iter) receive each iterate;
  * solution (optional, p+q) receives the MME solution at the estimates
- * (getSolution()). */
+ * (getSolution()). ai_last (optional, 2x2 row-major) receives the projected
+ * AI matrix of the last AI step (last_AI_mat; for an EM run, the AI matrix
+ * at the estimates) and se (optional, 2) the standard errors of
+ * (var_u, var_e) from it: sqrt(diag(AI^-1)) (calculate_SE, vce.cpp:3197-3368). */
 int cb_reml_run(cb_mme* mme, const cb_reml_options* opts, double var_u0, double var_e0,
                 double* var_u, double* var_e, int64_t* iterations, int* converged,
-                double* hist_vu, double* hist_ve, double* solution);
+                double* hist_vu, double* hist_ve, double* solution, double* ai_last, double* se);
 /* REML score dlogL/d(var_u, var_e) at the given variances from one block
  * solve [W'y | probes] (the data-driven gradient of vce.cpp:2647-2675),
  * optionally the AI matrix (2x2 row-major, vce.cpp:2596-2645), the trace
  * estimate of tr(Q C^uu) and the MME solution. */
 int cb_reml_score(cb_mme* mme, const cb_reml_options* opts, double var_u, double var_e,
                   double* grad, double* ai, double* trace, double* solution);
+/* REML log-likelihood calc_logL_internal (vce.cpp:3533-3588):
+ * -1/2 (log|C| + (n-p-q) log var_e + q log var_u + (y'y - sol'rhs)/var_e)
+ * with C = C(var_e/var_u) assembled densely on the device and factorised by
+ * Cholesky (p + q <= 65536). A C that is not positive definite gives
+ * -1e20, the reference's value for a failed determinant. logdet_c may be NULL. */
+int cb_reml_logl(cb_mme* mme, double var_u, double var_e, double* logl, double* logdet_c);
 /* The probe matrix the run uses (q x k, column-major, leading dimension ld). */
 int cb_reml_copy_probes(cb_mme* mme, const cb_reml_options* opts, double* out, int64_t ld);
+
+/* ---- relationship-matrix inverse --------------------------------------- */
+/* invert_psd_matrix(M, ridge) (src/core/matrix_io.cpp:488-501): eigen-bent
+ * inverse of a symmetric PSD n x n matrix, M + ridge I = U diag(l) U^T,
+ * eps = max(1e-12 max(l), 1e-6), l <- max(l, eps), out = U diag(1/l) U^T
+ * (G^-1 at relationship_builder.cpp:471). Host arrays (M, out: leading
+ * dimensions ldm, ldo). min_eig / max_eig (may be NULL) receive the extreme
+ * eigenvalues of M + ridge I before the floor. */
+int cb_invert_psd(cb_context* ctx, const double* M, int64_t ldm, int64_t n, double ridge, double* out,
+                  int64_t ldo, double* min_eig, double* max_eig);
+/* Same with M and out in device memory of ctx's device (they may alias). */
+int cb_invert_psd_device(cb_context* ctx, const double* dM, int64_t ldm, int64_t n, double ridge, double* dout,
+                         int64_t ldo, double* min_eig, double* max_eig);
 
 /* ---- pedigree relationship inverse ----------------------------------- */
 typedef struct cb_ainv cb_ainv;

@@ -135,6 +135,7 @@ def reml_run(X, aid, y, Q, vu0=1.0, ve0=1.0, algorithm="ai", max_iter=20, tol=1e
     vu, ve = vu0, ve0
     hist = []
     conv = False
+    ai_last = None
     for _ in range(max_iter):
         ou, oe = vu, ve
         s = reml_eval(X, aid, y, Q, vu, ve, Zp, want_ai=(algorithm == "ai"), exact=exact)
@@ -145,6 +146,7 @@ def reml_run(X, aid, y, Q, vu0=1.0, ve0=1.0, algorithm="ai", max_iter=20, tol=1e
             vu, ve = stabilize(em_u, ou), stabilize(em_e, oe)
         else:
             A = project_spd_2x2(s["ai"])
+            ai_last = A
             d = np.linalg.solve(A, s["grad"])
             ok = np.all(np.isfinite(d)) and abs(d[0]) <= 100 * vu and abs(d[1]) <= 100 * ve
             if ok:
@@ -162,10 +164,14 @@ def reml_run(X, aid, y, Q, vu0=1.0, ve0=1.0, algorithm="ai", max_iter=20, tol=1e
         if abs(ve - oe) + abs(vu - ou) < tol:
             conv = True
             break
+    if ai_last is None:  # EM run: the AI matrix at the estimates
+        ai_last = project_spd_2x2(reml_eval(X, aid, y, Q, vu, ve, Zp, want_ai=True, exact=exact)["ai"])
+    # calculate_SE (vce.cpp:3342-3352): SE = sqrt(max(0, diag(AI^-1)))
+    se = np.sqrt(np.maximum(0.0, np.diag(np.linalg.inv(ai_last))))
     lam = ve / max(vu, 1e-9)
     C, rhs, _ = mme_dense(X, aid, y, Q, lam)
     sol = np.linalg.solve(C, rhs)
-    return {"vu": vu, "ve": ve, "hist": hist, "converged": conv, "sol": sol}
+    return {"vu": vu, "ve": ve, "hist": hist, "converged": conv, "sol": sol, "ai_last": ai_last, "se": se}
 
 
 def logL(X, aid, y, Q, vu, ve):

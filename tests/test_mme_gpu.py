@@ -160,6 +160,60 @@ def test_reml_run_matches_oracle(cb, ctx, algorithm):
     assert R.iterations == len(ref["hist"])
     np.testing.assert_allclose([R.var_u, R.var_e], [ref["vu"], ref["ve"]], rtol=VC_TOL)
     assert ebv_err(R.solution[M.p:], ref["sol"][M.p:]) < EBV_TOL
+    # calculate_SE from the last AI matrix
+    np.testing.assert_allclose(R.ai_last, ref["ai_last"], rtol=1e-4)
+    np.testing.assert_allclose([R.getSEVarsU()[0], R.getSEVarE()], ref["se"], rtol=1e-4)
+
+
+@pytest.mark.parametrize("which", ["dense", "csr"])
+def test_logl_matches_oracle(cb, ctx, which):
+    """calc_logL_internal (vce.cpp:3533-3588) through a device Cholesky."""
+    if which == "dense":
+        fd, aid, y, Q = gblup_problem(n=300, m=1500, missing_records=11)
+        M = mme.MixedModel(fd, aid, y, Q.shape[0])
+        M.set_qinv_dense(Q)
+    else:
+        from cosmicblup_amd.pedigree import invertA_henderson
+
+        dam, sire = synth.simulate_pedigree(5, 80, seed=9)
+        A = invertA_henderson(dam, sire)
+        rng = np.random.default_rng(1)
+        aid = np.sort(rng.choice(dam.size, 300, replace=False)).astype(np.int32)
+        y = rng.standard_normal(aid.size) + 2.0
+        fd = design.build_fixed_design([[f"g{int(v)}"] for v in rng.integers(0, 6, aid.size)])
+        M = mme.MixedModel(fd, aid, y, dam.size)
+        M.set_qinv_csr(A.ptr, A.col, A.val)
+        Q = A.toDense()
+    R = mme.AIREML(M)
+    X = design.dense_x(fd)
+    for vu, ve in ((0.5, 0.8), (1.3, 0.4)):
+        np.testing.assert_allclose(R.logL(vu, ve), orl.logL(X, aid, y, Q, vu, ve), rtol=1e-10)
+
+
+def test_pblup_native_ainv_with_inbreeding_mixed_vs_dense(cb, ctx):
+    """PBLUP on the library's own A^-1 (inbred pedigree): mixed and FP64 PCG
+    against the dense solve, and REML against the oracle on the same A^-1."""
+    from cosmicblup_amd.pedigree import invertA_henderson
+
+    dam, sire = synth.simulate_pedigree(6, 150, seed=12)
+    A = invertA_henderson(dam, sire)
+    assert A.F.max() > 0
+    rng = np.random.default_rng(5)
+    rec = np.sort(rng.choice(dam.size, 600, replace=False)).astype(np.int32)
+    y = rng.standard_normal(rec.size) + 5.0
+    fd = design.build_fixed_design([[f"g{int(v):02d}"] for v in rng.integers(0, 12, rec.size)])
+    M = mme.MixedModel(fd, rec, y, dam.size)
+    M.set_qinv_csr(A.ptr, A.col, A.val)
+    Ad = A.toDense()
+    C, rhs, _ = orl.mme_dense(design.dense_x(fd), rec, y, Ad, 2.0)
+    ref = np.linalg.solve(C, rhs)
+    for prec in (_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED):
+        X, _, _ = M.solve(2.0, rhs, mme.solve_options(precision=prec, tol=1e-12 if prec == 0 else 1e-10))
+        assert ebv_err(X[M.p:], ref[M.p:]) < EBV_TOL
+    R = mme.AIREML(M, mme.reml_options(n_probes=16, seed=4, solve=mme.solve_options(tol=1e-10)))
+    R.solve()
+    o = orl.reml_run(design.dense_x(fd), rec, y, Ad, Zp=orl.probes(dam.size, 16, 4))
+    np.testing.assert_allclose([R.var_u, R.var_e], [o["vu"], o["ve"]], rtol=VC_TOL)
 
 
 def test_pblup_csr_ainv(cb, ctx):
