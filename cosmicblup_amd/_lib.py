@@ -39,6 +39,11 @@ class SolveOptions(C.Structure):
                 ("stall_tol", C.c_double)]
 
 
+class HinvOptions(C.Structure):
+    _fields_ = [("tune_g", C.c_int), ("alpha", C.c_double), ("tau", C.c_double), ("omega", C.c_double),
+                ("ridge", C.c_double)]
+
+
 class RemlOptions(C.Structure):
     _fields_ = [("algorithm", C.c_int), ("max_iter", C.c_int64), ("tol", C.c_double),
                 ("n_probes", C.c_int64), ("seed", C.c_uint64), ("trace_exact", C.c_int),
@@ -114,6 +119,10 @@ _SIGS = {
     "cb_reml_copy_probes": (C.c_int, [c_vp, C.POINTER(RemlOptions), c_vp, c_i64]),
     "cb_invert_psd": (C.c_int, [c_vp, c_vp, c_i64, c_i64, C.c_double, c_vp, c_i64, c_dp, c_dp]),
     "cb_invert_psd_device": (C.c_int, [c_vp, c_vp, c_i64, c_i64, C.c_double, c_vp, c_i64, c_dp, c_dp]),
+    "cb_mme_set_qinv_split": (C.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_i64]),
+    "cb_hinv_options_default": (None, [C.POINTER(HinvOptions)]),
+    "cb_hinv_blocks": (C.c_int, [c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, C.POINTER(HinvOptions), c_vp,
+                                 c_i64, c_vp, c_i64, c_vp]),
     "cb_ainv_henderson": (C.c_int, [c_i64, c_vp, c_vp, C.POINTER(c_vp)]),
     "cb_ainv_nnz": (C.c_int, [c_vp, c_i64p]),
     "cb_ainv_copy_csr": (C.c_int, [c_vp, c_vp, c_vp, c_vp]),

@@ -131,4 +131,10 @@ inline void set_device(cb_context* ctx) { CB_HIP(hipSetDevice(ctx->device)); }
 inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Shared between translation units (defined in cb_linalg.hip / cb_pedigree.hip).
+void invert_psd(cb_context* ctx, const double* dM, int64_t ldm, int64_t n, double ridge, double* dOut,
+                int64_t ldo, hipMemcpyKind in_kind, hipMemcpyKind out_kind, double* min_eig, double* max_eig);
+void pedigree_inbreeding(const std::vector<int32_t>& dam, const std::vector<int32_t>& sire, std::vector<double>& F,
+                         std::vector<double>& dii);
+
 }  // namespace cbamd

@@ -47,6 +47,10 @@ struct Handle {
     }
 };
 
+}  // namespace
+
+namespace cbamd {
+
 void invert_psd(cb_context* ctx, const double* dM, int64_t ldm, int64_t n, double ridge, double* dOut,
                 int64_t ldo, hipMemcpyKind in_kind, hipMemcpyKind out_kind, double* min_eig, double* max_eig) {
     CB_REQUIRE(n > 0 && n < (int64_t)1 << 31 && ldm >= n && ldo >= n, "cb_invert_psd: bad dimensions");
@@ -98,7 +102,7 @@ void invert_psd(cb_context* ctx, const double* dM, int64_t ldm, int64_t n, doubl
     CB_HIP(hipStreamSynchronize(st));
 }
 
-}  // namespace
+}  // namespace cbamd
 
 extern "C" {
 

@@ -617,7 +617,11 @@ struct cb_mme {
     std::vector<double> rhs;       // W'y
     double yty = 0.0;
     // Q
-    int qkind = -1;  // 0 dense, 1 CSR
+    int qkind = -1;  // 0 dense, 1 CSR, 2 split (CSR A^-1 + dense block on the genotyped rows)
+    int64_t ng = 0;                // split: genotyped rows
+    DevBuf<int32_t> gmap;          // split: row of Q for each genotyped row (-1: none)
+    DevBuf<double> xg64, yg64;     // split: gathered blocks
+    DevBuf<float> xg32, yg32;
     DevBuf<double> Qd64;  // q rows, leading dimension ldq (zero-padded)
     DevBuf<float> Qd32;
     int64_t ldq = 0;
@@ -678,44 +682,107 @@ float* partbuf<float>(cb_mme* M, size_t need) {
     return M->part32.p;
 }
 
-// Y_u (+)= alpha * Q X_u  on an r-column block (dense Q)
+// xg[i, c] = X[gmap[i], c] (0 for gmap < 0); yg[i, c] = 0
 template <class T>
-void dense_q_apply(cb_mme* M, const T* X, int64_t ldx, T* Y, int64_t ldy, int r, T alpha) {
+__global__ void k_gather_rows(const T* __restrict__ X, int64_t ldx, const int32_t* __restrict__ gmap, int64_t ng,
+                              T* __restrict__ xg, int64_t ldg, T* __restrict__ yg) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    if (i >= ng) return;
+    const int32_t g = gmap[i];
+    xg[c * ldg + i] = g >= 0 ? X[c * ldx + g] : (T)0;
+    yg[c * ldg + i] = (T)0;
+}
+
+// Y[gmap[i], c] += alpha * yg[i, c] (gmap entries distinct)
+template <class T>
+__global__ void k_scatter_add_rows(const T* __restrict__ yg, int64_t ldg, const int32_t* __restrict__ gmap,
+                                   int64_t ng, T alpha, T* __restrict__ Y, int64_t ldy) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    if (i >= ng) return;
+    const int32_t g = gmap[i];
+    if (g >= 0) Y[c * ldy + g] += alpha * yg[c * ldg + i];
+}
+
+// Y[0:qn, c] += alpha * Qd X[0:qn, c] for r columns: the dense block Qd
+// (qn rows, leading dimension ldq, zero-padded) of this MME. X and Y point
+// at row 0 of the block's index space.
+template <class T>
+void dense_apply(cb_mme* M, int64_t qn, const T* X, int64_t ldx, T* Y, int64_t ldy, int r, T alpha) {
     cb_context* ctx = M->ctx;
     hipStream_t st = ctx->stream;
-    const int64_t q = M->q;
     const T* Q = qdense<T>(M);
+    const int64_t ldq = M->ldq;
     if (r <= 2) {
         timed(ctx, sizeof(T) == 8 ? "k_gemv_f64" : "k_gemv_f32", [&] {
-            k_gemv<T><<<(unsigned)ceil_div(q, 4 * GEMV_ROWS), 256, 0, st>>>(Q, M->ldq, q, X + M->p, ldx, r, Y, ldy,
-                                                                        M->p, alpha);
+            k_gemv<T><<<(unsigned)ceil_div(qn, 4 * GEMV_ROWS), 256, 0, st>>>(Q, ldq, qn, X, ldx, r, Y, ldy, 0,
+                                                                         alpha);
         });
         return;
     }
-    const int64_t row_tiles = ceil_div(q, MF_ROWS);
+    const int64_t row_tiles = ceil_div(qn, MF_ROWS);
     // about 2048 blocks, splits of at least 256 k
-    int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(2048, row_tiles), q / 256));
-    const int64_t kspan = round_up(ceil_div(q, nsplit), MF_KC);
-    nsplit = (int)ceil_div(q, kspan);
-    const int64_t ldp = round_up(q, 64);
-    const int64_t ldq = M->ldq;
+    int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(2048, row_tiles), qn / 256));
+    const int64_t kspan = round_up(ceil_div(qn, nsplit), MF_KC);
+    nsplit = (int)ceil_div(qn, kspan);
+    const int64_t ldp = round_up(qn, 64);
     for (int c0 = 0; c0 < r; c0 += MF_MAXNT * 16) {
         const int rc = std::min(MF_MAXNT * 16, r - c0);
         const int nt = (rc + 15) / 16;
         T* part = partbuf<T>(M, (size_t)nsplit * rc * ldp);
-        const T* Xc = X + (int64_t)c0 * ldx + M->p;
+        const T* Xc = X + (int64_t)c0 * ldx;
         const unsigned grid = (unsigned)(row_tiles * nsplit);
         timed(ctx, sizeof(T) == 8 ? "k_dense_mfma_f64" : "k_dense_mfma_f32", [&] {
             switch (nt) {
-                case 1: k_dense_mfma<T, 1><<<grid, 256, 0, st>>>(Q, ldq, q, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
-                case 2: k_dense_mfma<T, 2><<<grid, 256, 0, st>>>(Q, ldq, q, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
-                case 3: k_dense_mfma<T, 3><<<grid, 256, 0, st>>>(Q, ldq, q, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
-                case 4: k_dense_mfma<T, 4><<<grid, 256, 0, st>>>(Q, ldq, q, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
-                default: k_dense_mfma<T, 5><<<grid, 256, 0, st>>>(Q, ldq, q, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
+                case 1: k_dense_mfma<T, 1><<<grid, 256, 0, st>>>(Q, ldq, qn, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
+                case 2: k_dense_mfma<T, 2><<<grid, 256, 0, st>>>(Q, ldq, qn, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
+                case 3: k_dense_mfma<T, 3><<<grid, 256, 0, st>>>(Q, ldq, qn, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
+                case 4: k_dense_mfma<T, 4><<<grid, 256, 0, st>>>(Q, ldq, qn, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
+                default: k_dense_mfma<T, 5><<<grid, 256, 0, st>>>(Q, ldq, qn, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
             }
         });
-        k_dense_reduce<T><<<dim3(blocks(q), (unsigned)rc), 256, 0, st>>>(part, ldp, nsplit, q, rc,
-                                                                       Y + (int64_t)c0 * ldy, ldy, M->p, alpha);
+        k_dense_reduce<T><<<dim3(blocks(qn), (unsigned)rc), 256, 0, st>>>(part, ldp, nsplit, qn, rc,
+                                                                        Y + (int64_t)c0 * ldy, ldy, 0, alpha);
+        CB_HIP(hipGetLastError());
+    }
+}
+
+template <class T>
+DevBuf<T>& gbuf(cb_mme* M, int which);
+template <>
+DevBuf<double>& gbuf<double>(cb_mme* M, int which) { return which ? M->yg64 : M->xg64; }
+template <>
+DevBuf<float>& gbuf<float>(cb_mme* M, int which) { return which ? M->yg32 : M->xg32; }
+
+// Y_u += alpha * Q X_u on an r-column block, every kind of Q:
+//   dense   Qd X_u
+//   CSR     Q X_u
+//   split   A^-1 X_u + scatter(D gather(X_u))  (SplitMatrixAdapter, matrix_adapter.h:253-278,
+//           with D = tau G^-1 - omega A22^-1 held as one dense block)
+// The rank-1 stiff term of a dense Q's FP32 copy is added by apply_C.
+template <class T>
+void q_apply(cb_mme* M, const T* X, int64_t ldx, T* Y, int64_t ldy, int r, T alpha) {
+    hipStream_t st = M->ctx->stream;
+    const int64_t p = M->p, q = M->q;
+    if (M->qkind == 0) {
+        dense_apply<T>(M, q, X + p, ldx, Y + p, ldy, r, alpha);
+        return;
+    }
+    k_csr_spmm<T, T><<<dim3(blocks(q), (unsigned)r), 256, 0, st>>>(M->qp.p, M->qc.p, qvals<T>(M), 0, q, p, X, ldx,
+                                                                   Y, ldy, p, alpha, 1);
+    CB_HIP(hipGetLastError());
+    if (M->qkind == 2) {
+        const int64_t ng = M->ng, ldg = round_up(ng, 64);
+        DevBuf<T>& xg = gbuf<T>(M, 0);
+        DevBuf<T>& yg = gbuf<T>(M, 1);
+        if (xg.n < (size_t)ldg * r) xg.alloc((size_t)ldg * r);
+        if (yg.n < (size_t)ldg * r) yg.alloc((size_t)ldg * r);
+        k_gather_rows<T><<<dim3(blocks(ng), (unsigned)r), 256, 0, st>>>(X + p, ldx, M->gmap.p, ng, xg.p, ldg, yg.p);
+        CB_HIP(hipGetLastError());
+        dense_apply<T>(M, ng, xg.p, ldg, yg.p, ldg, r, (T)1);
+        k_scatter_add_rows<T><<<dim3(blocks(ng), (unsigned)r), 256, 0, st>>>(yg.p, ldg, M->gmap.p, ng, alpha, Y + p,
+                                                                            ldy);
         CB_HIP(hipGetLastError());
     }
 }
@@ -742,20 +809,14 @@ void apply_C(cb_mme* M, double lambda, const T* X, T* Y, int r) {
         });
     CB_HIP(hipGetLastError());
     // lambda Q part
-    if (M->qkind == 0) {
-        dense_q_apply<T>(M, X, ld, Y, ld, r, (T)lambda);
-        if (M->stiff && sizeof(T) == 4) {
-            // the FP32 copy holds Q - mu v v': add lambda mu v (v' x_u) here
-            if (M->stiff_part.n < (size_t)DOT_CHUNKS * r) M->stiff_part.alloc((size_t)DOT_CHUNKS * r);
-            k_stiff_dot<T><<<dim3(DOT_CHUNKS, (unsigned)r), 256, 0, st>>>(X + M->p, M->stiff_v.p, ld, M->q,
-                                                                        M->stiff_part.p);
-            k_stiff_axpy<T><<<dim3(blocks(M->q), (unsigned)r), 256, 0, st>>>(Y + M->p, M->stiff_v.p, M->stiff_part.p,
-                                                                           lambda * M->stiff_mu, ld, M->q);
-            CB_HIP(hipGetLastError());
-        }
-    } else {
-        k_csr_spmm<T, T><<<dim3(blocks(M->q), (unsigned)r), 256, 0, st>>>(
-            M->qp.p, M->qc.p, qvals<T>(M), 0, M->q, M->p, X, ld, Y, ld, M->p, (T)lambda, 1);
+    q_apply<T>(M, X, ld, Y, ld, r, (T)lambda);
+    if (M->qkind == 0 && M->stiff && sizeof(T) == 4) {
+        // the FP32 copy holds Q - mu v v': add lambda mu v (v' x_u) here
+        if (M->stiff_part.n < (size_t)DOT_CHUNKS * r) M->stiff_part.alloc((size_t)DOT_CHUNKS * r);
+        k_stiff_dot<T><<<dim3(DOT_CHUNKS, (unsigned)r), 256, 0, st>>>(X + M->p, M->stiff_v.p, ld, M->q,
+                                                                    M->stiff_part.p);
+        k_stiff_axpy<T><<<dim3(blocks(M->q), (unsigned)r), 256, 0, st>>>(Y + M->p, M->stiff_v.p, M->stiff_part.p,
+                                                                       lambda * M->stiff_mu, ld, M->q);
         CB_HIP(hipGetLastError());
     }
 }
@@ -870,6 +931,16 @@ void col_dots(cb_mme* M, const T* A, const T* B, int r, double* part, double* ou
     k_col_dots<T><<<dim3(DOT_CHUNKS, (unsigned)r), 256, 0, st>>>(A, B, M->ldv, M->dim, part);
     k_dots_finish<<<blocks(r, 64), 64, 0, st>>>(part, r, out);
     CB_HIP(hipGetLastError());
+}
+
+// C[gmap[i], gmap[j]] += alpha * D[i, j] (D row-major, ldd)
+__global__ void k_dense_add_block(const double* __restrict__ D, int64_t ldd, int64_t ng,
+                                  const int32_t* __restrict__ gmap, double alpha, double* __restrict__ C, int64_t ld) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (i >= ng) return;
+    const int32_t gi = gmap[i], gj = gmap[j];
+    if (gi >= 0 && gj >= 0) C[(int64_t)gj * ld + gi] += alpha * D[j * ldd + i];
 }
 
 // C[row, col - col_off] (column-major, ld) += alpha * CSR(row, col) for rows [0, rows)
@@ -1339,11 +1410,7 @@ static void reml_eval(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, doub
     DevBuf<double> Qu;
     Qu.alloc(ld);
     CB_HIP(hipMemsetAsync(Qu.p, 0, ld * 8, st));
-    if (M->qkind == 0) dense_q_apply<double>(M, w.S.p, ld, Qu.p, ld, 1, 1.0);
-    else {
-        k_csr_spmm<double, double><<<dim3(blocks(q), 1), 256, 0, st>>>(M->qp.p, M->qc.p, M->qv64.p, 0, q, p,
-                                                                      w.S.p, ld, Qu.p, ld, p, 1.0, 1);
-    }
+    q_apply<double>(M, w.S.p, ld, Qu.p, ld, 1, 1.0);
     // e = y - W s ; e_sse ; u'Qu ; traces t_i = (Q z_i)' S_u,i
     k_w_apply<<<dim3(blocks(n), 1), 256, 0, st>>>(M->xp.p, M->xc.p, M->xv.p, M->aid.p, n, p, w.S.p, ld, w.T.p, round_up(n, 64));
     k_axpby<<<blocks(n), 256, 0, st>>>(M->y.p, 1.0, w.T.p, -1.0, w.e.p, n);
@@ -1434,10 +1501,7 @@ static void reml_setup(cb_mme* M, const cb_reml_options& o, cb_reml_work& w) {
         k_probes<<<dim3(blocks(q), (unsigned)k), 256, 0, st>>>(w.B.p + ld, ld, p, q, k, o.seed, o.trace_exact, 0);
         CB_HIP(hipGetLastError());
         // Qz (fixed for the run): Q applied to the probe columns
-        if (M->qkind == 0) dense_q_apply<double>(M, w.B.p + ld, ld, w.Qz.p, ld, k, 1.0);
-        else
-            k_csr_spmm<double, double><<<dim3(blocks(q), (unsigned)k), 256, 0, st>>>(
-                M->qp.p, M->qc.p, M->qv64.p, 0, q, p, w.B.p + ld, ld, w.Qz.p, ld, p, 1.0, 1);
+        q_apply<double>(M, w.B.p + ld, ld, w.Qz.p, ld, k, 1.0);
         CB_HIP(hipGetLastError());
     }
     CB_HIP(hipStreamSynchronize(st));
@@ -1551,6 +1615,34 @@ static void find_stiff_direction(cb_mme* M) {
     CB_HIP(hipStreamSynchronize(st));
 }
 
+static void set_q_csr_common(cb_mme* M, const int64_t* ptr, const int32_t* col, const double* val) {
+    set_device(M->ctx);
+    const int64_t q = M->q, nnz = ptr[q];
+    CB_REQUIRE(ptr[0] == 0 && nnz >= 0, "Qinv CSR: bad row pointers");
+    M->q_diag.assign(q, 0.0);
+    for (int64_t i = 0; i < q; ++i)
+        for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+            CB_REQUIRE(col[k] >= 0 && col[k] < q, "Qinv CSR: column out of range");
+            if (col[k] == i) M->q_diag[i] += val[k];
+        }
+    hipStream_t st = M->ctx->stream;
+    M->qp.alloc(q + 1);
+    M->qc.alloc(std::max<int64_t>(1, nnz));
+    M->qv64.alloc(std::max<int64_t>(1, nnz));
+    M->qv32.alloc(std::max<int64_t>(1, nnz));
+    CB_HIP(hipMemcpyAsync(M->qp.p, ptr, (q + 1) * 8, hipMemcpyHostToDevice, st));
+    CB_HIP(hipMemcpyAsync(M->qc.p, col, nnz * 4, hipMemcpyHostToDevice, st));
+    CB_HIP(hipMemcpyAsync(M->qv64.p, val, nnz * 8, hipMemcpyHostToDevice, st));
+    k_convert<float, double><<<blocks(nnz), 256, 0, st>>>(M->qv64.p, M->qv32.p, nnz);
+    CB_HIP(hipGetLastError());
+    CB_HIP(hipStreamSynchronize(st));
+    M->stiff = false;
+    M->Qd64.release();
+    M->Qd32.release();
+    M->ng = 0;
+    M->dinv_lambda = std::nan("");
+}
+
 static void set_q_dense_common(cb_mme* M, const double* Q, int64_t ld, hipMemcpyKind kind) {
     const int64_t q = M->q;
     CB_REQUIRE(Q && ld >= q, "cb_mme_set_qinv_dense: bad matrix");
@@ -1595,30 +1687,43 @@ int cb_mme_set_qinv_dense_device(cb_mme* M, const double* dQ, int64_t ld) {
 int cb_mme_set_qinv_csr(cb_mme* M, const int64_t* ptr, const int32_t* col, const double* val) {
     return guarded([&] {
         CB_REQUIRE(M && ptr && col && val, "cb_mme_set_qinv_csr: NULL argument");
-        set_device(M->ctx);
-        const int64_t q = M->q, nnz = ptr[q];
-        CB_REQUIRE(ptr[0] == 0 && nnz >= 0, "cb_mme_set_qinv_csr: bad CSR");
-        M->q_diag.assign(q, 0.0);
-        for (int64_t i = 0; i < q; ++i)
-            for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
-                CB_REQUIRE(col[k] >= 0 && col[k] < q, "cb_mme_set_qinv_csr: column out of range");
-                if (col[k] == i) M->q_diag[i] += val[k];
-            }
-        hipStream_t st = M->ctx->stream;
-        M->qp.alloc(q + 1);
-        M->qc.alloc(std::max<int64_t>(1, nnz));
-        M->qv64.alloc(std::max<int64_t>(1, nnz));
-        M->qv32.alloc(std::max<int64_t>(1, nnz));
-        CB_HIP(hipMemcpyAsync(M->qp.p, ptr, (q + 1) * 8, hipMemcpyHostToDevice, st));
-        CB_HIP(hipMemcpyAsync(M->qc.p, col, nnz * 4, hipMemcpyHostToDevice, st));
-        CB_HIP(hipMemcpyAsync(M->qv64.p, val, nnz * 8, hipMemcpyHostToDevice, st));
-        k_convert<float, double><<<blocks(nnz), 256, 0, st>>>(M->qv64.p, M->qv32.p, nnz);
-        CB_HIP(hipGetLastError());
-        CB_HIP(hipStreamSynchronize(st));
+        set_q_csr_common(M, ptr, col, val);
         M->qkind = 1;
-        M->stiff = false;
-        M->Qd64.release(); M->Qd32.release();
-        M->dinv_lambda = std::nan("");
+    });
+}
+
+int cb_mme_set_qinv_split(cb_mme* M, const int64_t* ptr, const int32_t* col, const double* val, int64_t ng,
+                          const int32_t* gmap, const double* dD, int64_t ldd) {
+    return guarded([&] {
+        CB_REQUIRE(M && ptr && col && val && gmap && dD, "cb_mme_set_qinv_split: NULL argument");
+        CB_REQUIRE(ng > 0 && ldd >= ng, "cb_mme_set_qinv_split: bad block dimensions");
+        std::vector<char> seen(M->q, 0);
+        for (int64_t i = 0; i < ng; ++i) {
+            CB_REQUIRE(gmap[i] >= -1 && gmap[i] < M->q, "cb_mme_set_qinv_split: map entry out of range");
+            if (gmap[i] >= 0) {
+                CB_REQUIRE(!seen[gmap[i]], "cb_mme_set_qinv_split: map entries must be distinct");
+                seen[gmap[i]] = 1;
+            }
+        }
+        set_q_csr_common(M, ptr, col, val);
+        hipStream_t st = M->ctx->stream;
+        const int64_t ldq = round_up(ng, MF_KC);
+        M->ng = ng;
+        M->ldq = ldq;
+        M->Qd64.alloc((size_t)ng * ldq);
+        M->Qd32.alloc((size_t)ng * ldq);
+        if (ldq > ng) CB_HIP(hipMemsetAsync(M->Qd64.p, 0, M->Qd64.bytes(), st));
+        CB_HIP(hipMemcpy2DAsync(M->Qd64.p, ldq * 8, dD, ldd * 8, ng * 8, ng, hipMemcpyDeviceToDevice, st));
+        k_convert<float, double><<<blocks(ng * ldq), 256, 0, st>>>(M->Qd64.p, M->Qd32.p, ng * ldq);
+        CB_HIP(hipGetLastError());
+        M->gmap.alloc(ng);
+        CB_HIP(hipMemcpyAsync(M->gmap.p, gmap, ng * 4, hipMemcpyHostToDevice, st));
+        std::vector<double> dg(ng);
+        CB_HIP(hipMemcpy2DAsync(dg.data(), 8, M->Qd64.p, (ldq + 1) * 8, 8, ng, hipMemcpyDeviceToHost, st));
+        CB_HIP(hipStreamSynchronize(st));
+        for (int64_t i = 0; i < ng; ++i)
+            if (gmap[i] >= 0) M->q_diag[gmap[i]] += dg[i];  // add_diagonal_to (matrix_adapter.h:280-300)
+        M->qkind = 2;
     });
 }
 
@@ -1834,10 +1939,14 @@ int cb_reml_logl(cb_mme* M, double var_u, double var_e, double* logl, double* lo
         b.alloc(dim);
         CB_HIP(hipMemsetAsync(C.p, 0, C.bytes(), st));
         k_dense_from_csr<<<blocks(dim), 256, 0, st>>>(M->wp.p, M->wc.p, M->wv64.p, dim, 0, 1.0, C.p, dim);
-        if (M->qkind == 0)
+        if (M->qkind == 0) {
             k_dense_add_q<<<dim3(blocks(q), (unsigned)q), 256, 0, st>>>(M->Qd64.p, M->ldq, q, lambda, C.p + p * dim + p, dim);
-        else
+        } else {
             k_dense_from_csr<<<blocks(q), 256, 0, st>>>(M->qp.p, M->qc.p, M->qv64.p, q, 0, lambda, C.p + p * dim + p, dim);
+            if (M->qkind == 2)
+                k_dense_add_block<<<dim3(blocks(M->ng), (unsigned)M->ng), 256, 0, st>>>(M->Qd64.p, M->ldq, M->ng, M->gmap.p,
+                                                                                      lambda, C.p + p * dim + p, dim);
+        }
         CB_HIP(hipGetLastError());
         CB_HIP(hipMemcpyAsync(b.p, M->rhs.data(), dim * 8, hipMemcpyHostToDevice, st));
         rocblas_handle h = nullptr;

@@ -86,45 +86,6 @@ void ml_range(const std::vector<int32_t>& dam, const std::vector<int32_t>& sire,
     }
 }
 
-void inbreeding(const std::vector<int32_t>& dam, const std::vector<int32_t>& sire, std::vector<double>& F,
-                std::vector<double>& dii) {
-    const int64_t n = (int64_t)dam.size();
-    std::vector<int32_t> level(n, 0);
-    int32_t maxl = 0;
-    for (int64_t i = 0; i < n; ++i) {
-        int32_t l = 0;
-        if (sire[i] >= 0) l = std::max(l, level[sire[i]] + 1);
-        if (dam[i] >= 0) l = std::max(l, level[dam[i]] + 1);
-        level[i] = l;
-        maxl = std::max(maxl, l);
-    }
-    // animals grouped by level, index order inside a level
-    std::vector<int64_t> start(maxl + 2, 0), order(n);
-    for (int64_t i = 0; i < n; ++i) ++start[level[i] + 1];
-    for (int32_t l = 0; l <= maxl; ++l) start[l + 1] += start[l];
-    {
-        std::vector<int64_t> fill(start.begin(), start.end() - 1);
-        for (int64_t i = 0; i < n; ++i) order[fill[level[i]]++] = i;
-    }
-    F.assign(n, 0.0);
-    dii.assign(n, 1.0);
-    const int hw = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    for (int32_t l = 0; l <= maxl; ++l) {
-        const int64_t lo = start[l], hi = start[l + 1], cnt = hi - lo;
-        const int nt = (int)std::min<int64_t>(hw, std::max<int64_t>(1, cnt / 2048));
-        if (nt <= 1) {
-            ml_range(dam, sire, order, lo, hi, F, dii);
-            continue;
-        }
-        std::vector<std::thread> th;
-        for (int t = 0; t < nt; ++t) {
-            const int64_t a = lo + cnt * t / nt, b = lo + cnt * (t + 1) / nt;
-            th.emplace_back([&, a, b] { ml_range(dam, sire, order, a, b, F, dii); });
-        }
-        for (auto& x : th) x.join();
-    }
-}
-
 struct Trip {
     int32_t r, c;
     double v;
@@ -209,6 +170,49 @@ void assemble(cb_ainv* A, const std::vector<int32_t>& dam, const std::vector<int
 
 }  // namespace
 
+namespace cbamd {
+
+void pedigree_inbreeding(const std::vector<int32_t>& dam, const std::vector<int32_t>& sire, std::vector<double>& F,
+                std::vector<double>& dii) {
+    const int64_t n = (int64_t)dam.size();
+    std::vector<int32_t> level(n, 0);
+    int32_t maxl = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        int32_t l = 0;
+        if (sire[i] >= 0) l = std::max(l, level[sire[i]] + 1);
+        if (dam[i] >= 0) l = std::max(l, level[dam[i]] + 1);
+        level[i] = l;
+        maxl = std::max(maxl, l);
+    }
+    // animals grouped by level, index order inside a level
+    std::vector<int64_t> start(maxl + 2, 0), order(n);
+    for (int64_t i = 0; i < n; ++i) ++start[level[i] + 1];
+    for (int32_t l = 0; l <= maxl; ++l) start[l + 1] += start[l];
+    {
+        std::vector<int64_t> fill(start.begin(), start.end() - 1);
+        for (int64_t i = 0; i < n; ++i) order[fill[level[i]]++] = i;
+    }
+    F.assign(n, 0.0);
+    dii.assign(n, 1.0);
+    const int hw = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    for (int32_t l = 0; l <= maxl; ++l) {
+        const int64_t lo = start[l], hi = start[l + 1], cnt = hi - lo;
+        const int nt = (int)std::min<int64_t>(hw, std::max<int64_t>(1, cnt / 2048));
+        if (nt <= 1) {
+            ml_range(dam, sire, order, lo, hi, F, dii);
+            continue;
+        }
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t) {
+            const int64_t a = lo + cnt * t / nt, b = lo + cnt * (t + 1) / nt;
+            th.emplace_back([&, a, b] { ml_range(dam, sire, order, a, b, F, dii); });
+        }
+        for (auto& x : th) x.join();
+    }
+}
+
+}  // namespace cbamd
+
 extern "C" {
 
 int cb_ainv_henderson(int64_t n, const int32_t* dam, const int32_t* sire, cb_ainv** out) {
@@ -226,7 +230,7 @@ int cb_ainv_henderson(int64_t n, const int32_t* dam, const int32_t* sire, cb_ain
         auto* A = new cb_ainv;
         A->n = n;
         try {
-            inbreeding(d, s, A->F, A->dii);
+            pedigree_inbreeding(d, s, A->F, A->dii);
             assemble(A, d, s);
         } catch (...) {
             delete A;

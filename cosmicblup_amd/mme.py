@@ -88,6 +88,16 @@ class MixedModel:
         val = np.ascontiguousarray(val, dtype=np.float64)
         _lib.check(_lib.lib().cb_mme_set_qinv_csr(self.handle, _lib.ptr(ptr), _lib.ptr(col), _lib.ptr(val)))
 
+    def set_qinv_split(self, ptr, col, val, gmap, dD: int, ldd: int) -> None:
+        """H^-1 = A^-1 (CSR) + scatter(D) on the genotyped rows gmap; D a
+        device pointer (ng x ng, leading dimension ldd)."""
+        ptr = np.ascontiguousarray(ptr, dtype=np.int64)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        gmap = np.ascontiguousarray(gmap, dtype=np.int32)
+        _lib.check(_lib.lib().cb_mme_set_qinv_split(self.handle, _lib.ptr(ptr), _lib.ptr(col), _lib.ptr(val),
+                                                    gmap.size, _lib.ptr(gmap), dD, ldd))
+
     def rhs(self) -> np.ndarray:
         out = np.empty(self.dim)
         _lib.check(_lib.lib().cb_mme_rhs(self.handle, _lib.ptr(out)))

@@ -152,6 +152,13 @@ int cb_mme_dims(const cb_mme* mme, int64_t* n, int64_t* p, int64_t* q);
 int cb_mme_set_qinv_dense(cb_mme* mme, const double* Q, int64_t ld);
 int cb_mme_set_qinv_dense_device(cb_mme* mme, const double* dQ, int64_t ld);
 int cb_mme_set_qinv_csr(cb_mme* mme, const int64_t* ptr, const int32_t* col, const double* val);
+/* ssGBLUP H^-1 as SplitMatrixAdapter (matrix_adapter.h:232-373) holds it:
+ * Q x = A^-1 x + scatter(D gather(x)) with A^-1 CSR (as above) and
+ * D = tau G^-1 - omega A22^-1 one dense ng x ng block in device memory
+ * (leading dimension ldd, e.g. from cb_hinv_blocks); gmap[i] is the row of
+ * Q of genotyped row i (-1: not in the pedigree), entries distinct. */
+int cb_mme_set_qinv_split(cb_mme* mme, const int64_t* ptr, const int32_t* col, const double* val, int64_t ng,
+                          const int32_t* gmap, const double* dD, int64_t ldd);
 /* build_rhs (mme_builder.cpp:321-328): [X'y; Z'y], length p+q. */
 int cb_mme_rhs(const cb_mme* mme, double* rhs);
 /* Y = C(lambda) X, C = W'W + lambda blockdiag(0, Q), FP64; X, Y column-major (p+q) x r. */
@@ -243,6 +250,28 @@ int cb_invert_psd(cb_context* ctx, const double* M, int64_t ldm, int64_t n, doub
 /* Same with M and out in device memory of ctx's device (they may alias). */
 int cb_invert_psd_device(cb_context* ctx, const double* dM, int64_t ldm, int64_t n, double ridge, double* dout,
                          int64_t ldo, double* min_eig, double* max_eig);
+
+/* ---- ssGBLUP H^-1 ------------------------------------------------------ */
+typedef struct {
+    int tune_g;     /* align G to A22 (config.h:120), default 1 */
+    double alpha;   /* G <- (1 - alpha) G + alpha A22 (config.h:121), default 0.05 */
+    double tau;     /* G^-1 scale (config.h:88), default 1 */
+    double omega;   /* A22^-1 scale (config.h:89), default 1 */
+    double ridge;   /* added to G and A22 before inversion (config.h:118), default 0 */
+} cb_hinv_options;
+void cb_hinv_options_default(cb_hinv_options* opts);
+/* The split-mode H^-1 set-up of src/io/relationship_builder.cpp:481-633:
+ * A22 of the genotyped animals (row gmap[i] of the pedigree for genotyped
+ * row i, -1 if absent: identity row/column), tuneG a + b G matching A22's
+ * diagonal and off-diagonal means, the blend, G^-1 by invert_psd_matrix,
+ * A22^-1 by Cholesky, and D = tau G^-1 - omega A22^-1 (ng x ng, device,
+ * leading dimension ldd). dam/sire as for cb_ainv_henderson (host arrays);
+ * dG: the ng x ng GRM of the genotyped rows on the device (unchanged).
+ * dA22 (optional, device) receives A22. stats (optional, 6) receives
+ * a, b, mean diag A22, mean off-diag A22, mean diag G, mean off-diag G. */
+int cb_hinv_blocks(cb_context* ctx, int64_t n, const int32_t* dam, const int32_t* sire, int64_t ng,
+                   const int32_t* gmap, const double* dG, int64_t ldg, const cb_hinv_options* opts, double* dD,
+                   int64_t ldd, double* dA22, int64_t lda22, double* stats);
 
 /* ---- pedigree relationship inverse ----------------------------------- */
 typedef struct cb_ainv cb_ainv;

@@ -13,6 +13,9 @@ path: dense FP64 numpy restatements of the reference algorithms.
                        (src/core/a_matrix.cpp:392-439, textbook rules)
   ml_inbreeding        ml() (a_matrix.cpp:127-186): F and d_ii by the dense
                        descending scan, pure Python (small pedigrees)
+  hinv_blocks          ssGBLUP split-mode set-up (src/io/relationship_builder.cpp:
+                       481-633): A22 from the tabular A, tuneG, blend, G^-1
+                       by the eigen-bent inverse, A22^-1, D = tau G^-1 - omega A22^-1
   henderson_ainv_f     invertA_henderson with ml() inbreeding, the
                        reference's triplet order, symmetrisation, diagonal
                        floor and prune (a_matrix.cpp:43-80,392-439)
@@ -294,3 +297,36 @@ def henderson_ainv_f(dam, sire):
     np.fill_diagonal(M, dg)
     M[np.abs(M) <= 1e-24] = 0.0
     return M, F, dii
+
+
+def hinv_blocks(dam, sire, gmap, G, tune_g=True, alpha=0.05, tau=1.0, omega=1.0, ridge=0.0):
+    """relationship_builder.cpp:481-633 on a dense A (tabular method; small
+    pedigrees). Returns (D, A22, (a, b))."""
+    from . import grm as og
+
+    A = tabular_a(np.asarray(sire), np.asarray(dam))
+    ng = len(gmap)
+    A22 = np.eye(ng)
+    for i in range(ng):
+        for j in range(ng):
+            if gmap[i] >= 0 and gmap[j] >= 0:
+                A22[i, j] = A[gmap[i], gmap[j]]
+            else:
+                A22[i, j] = 1.0 if i == j else 0.0
+    G = np.array(G, dtype=np.float64, copy=True)
+    a, b = 0.0, 1.0
+    if tune_g:
+        off = ~np.eye(ng, dtype=bool)
+        mgd, mad = np.trace(G) / ng, np.trace(A22) / ng
+        noff = ng * (ng - 1.0)
+        mgo = G[off].sum() / noff if noff > 0 else 0.0
+        mao = A22[off].sum() / noff if noff > 0 else 0.0
+        if abs(mgd - mgo) > 1e-12:
+            b = (mad - mao) / (mgd - mgo)
+        a = mad - b * mgd
+        G = b * G + a
+    G = (1.0 - alpha) * G + alpha * A22
+    Ginv = og.invert_psd_matrix(G, ridge)
+    A22r = A22 + ridge * np.eye(ng) if ridge > 0 else A22
+    A22inv = np.linalg.inv(A22r)
+    return tau * Ginv - omega * A22inv, A22, (a, b)

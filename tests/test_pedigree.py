@@ -98,3 +98,23 @@ def test_native_large_pedigree_properties(cb):
     diag = A.val[A.col == rows]
     leaves = ~has_off
     np.testing.assert_allclose(diag[leaves], 1.0 / A.dii[leaves], rtol=1e-15)
+
+
+def test_oracle_tune_g_matches_a22_means():
+    """The tuneG closed form: b G + a has A22's mean diagonal and mean
+    off-diagonal (relationship_builder.cpp:539-565)."""
+    from oracle import grm as og
+    from cosmicblup_amd import synth
+
+    dam, sire = inbred_pedigree(n_gen=5, per_gen=30, seed=3)
+    gmap = np.arange(60, 150, dtype=np.int32)
+    x, _ = synth.simulate_dosages(gmap.size, 400, seed=2)
+    G, _, _ = og.grm_vanraden(synth.encode_bed(x), gmap.size, 400)
+    D, A22, (a, b) = orl.hinv_blocks(dam, sire, gmap, G, alpha=0.0)
+    Ga = b * G + a
+    off = ~np.eye(gmap.size, dtype=bool)
+    np.testing.assert_allclose(np.diag(Ga).mean(), np.diag(A22).mean(), rtol=1e-12)
+    np.testing.assert_allclose(Ga[off].mean(), A22[off].mean(), rtol=1e-10)
+    # A22 is the genotyped block of the inverse of Henderson's A^-1
+    Ai, _, _ = orl.henderson_ainv_f(dam, sire)
+    np.testing.assert_allclose(A22, np.linalg.inv(Ai)[np.ix_(gmap, gmap)], atol=1e-10)
