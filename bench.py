@@ -157,20 +157,27 @@ def reml_leg(ctx, n: int = 20000, m: int = 50000, k: int = 64, h2: float = 0.4, 
 
 
 def pcg_leg(ctx, n_gen: int = 10, per_gen: int = 100000, n_rec: int = 600000, n_groups: int = 2000,
-            h2: float = 0.25, tol: float = 1e-10, seed: int = 20263203) -> dict:
-    """PBLUP sparse-MME PCG at the 1M-animal pedigree shape: A^-1 by
-    Henderson's rules with inbreeding (host builder), 600k records, 2000
-    contemporary groups; one right-hand side solved to relative residual
-    `tol` by GPU FP64 PCG and by the mixed FP32/FP64-replacement PCG."""
+            h2: float = 0.25, n_geno: int = 20000, m_geno: int = 40000, tol: float = 1e-10,
+            seed: int = 20263203) -> dict:
+    """Sparse-MME PCG at the 1M-animal pedigree shape: A^-1 by Henderson's
+    rules with inbreeding (host builder), 600k records, 2000 contemporary
+    groups; PBLUP (A^-1) and ssGBLUP (H^-1 with the last n_geno animals
+    genotyped at m_geno SNPs, set up on the device). One right-hand side per
+    model solved to relative residual `tol` by GPU FP64 PCG and by the mixed
+    FP32/FP64-replacement PCG."""
     import time
 
     import numpy as np
+    import torch
 
-    from cosmicblup_amd import _lib, design, mme, synth
+    import cosmicblup_amd as cb
+    from cosmicblup_amd import _lib, design, mme, ssgblup, synth
     from cosmicblup_amd.pedigree import invertA_henderson
 
-    print(f"[bench] PCG leg: pedigree {n_gen}x{per_gen}, {n_rec} records, {n_groups} groups", flush=True)
+    print(f"[bench] PCG leg: pedigree {n_gen}x{per_gen}, {n_rec} records, {n_groups} groups, "
+          f"{n_geno} genotyped x {m_geno} SNPs", flush=True)
     dam, sire = synth.simulate_pedigree(n_gen, per_gen, unknown=0.05, seed=seed)
+    n = dam.size
     t0 = time.perf_counter()
     A = invertA_henderson(dam, sire)
     t_ainv = time.perf_counter() - t0
@@ -178,37 +185,62 @@ def pcg_leg(ctx, n_gen: int = 10, per_gen: int = 100000, n_rec: int = 600000, n_
                                                     seed=seed + 1)
     width = len(str(n_groups))
     fd = design.build_fixed_design([[f"cg{int(c):0{width}d}"] for c in grp])
-    model = mme.MixedModel(fd, rec, y, dam.size, ctx=ctx)
-    model.set_qinv_csr(A.ptr, A.col, A.val)
-    nnz = int(A.ptr[-1])
-    del A
     lam = (1.0 - h2) / h2
-    b = model.rhs()
-    out = {"animals": int(dam.size), "records": n_rec, "p": model.p, "ainv_nnz": nnz,
+    out = {"animals": int(n), "records": n_rec, "groups": n_groups, "ainv_nnz": int(A.ptr[-1]),
            "ainv_build_s": t_ainv, "lambda": lam, "tol": tol}
-    sols = {}
-    for name, prec in (("fp64", _lib.CB_PREC_FP64), ("mixed", _lib.CB_PREC_MIXED)):
-        opts = mme.solve_options(precision=prec, tol=tol, max_iter=20000)
-        model.solve(lam, b, opts)  # warm-up: preconditioner set-up, allocations
-        ctx.synchronize()
+
+    def run_solves(model, tag):
+        b = model.rhs()
+        res = {}
+        sols = {}
+        for name, prec in (("fp64", _lib.CB_PREC_FP64), ("mixed", _lib.CB_PREC_MIXED)):
+            opts = mme.solve_options(precision=prec, tol=tol, max_iter=20000)
+            model.solve(lam, b, opts)  # warm-up: preconditioner set-up, allocations
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            x, it, rel = model.solve(lam, b, opts)
+            ctx.synchronize()
+            dt = time.perf_counter() - t0
+            sols[name] = x
+            res[name] = {"iterations": it, "seconds": dt, "iters_per_s": it / dt, "rel_residual": rel}
+            print(f"[bench] {tag} PCG {name}: {it} iterations in {dt:.3f} s (rel {rel:.2e})", flush=True)
+        # both stop at relative residual `tol`, so their EBVs differ by up to
+        # cond(C) * tol; against a tight FP64 solve (1e-13) each shows its own error
+        x_ref, it_ref, _ = model.solve(lam, b, mme.solve_options(precision=_lib.CB_PREC_FP64, tol=1e-13,
+                                                                 max_iter=20000))
+        p = model.p
+        scale = np.abs(x_ref[p:]).max()
+        res["ebv_rel_err_mixed_vs_fp64"] = float(np.abs(sols["mixed"][p:] - sols["fp64"][p:]).max() / scale)
+        res["ebv_rel_err_vs_fp64_1e-13"] = {k: float(np.abs(v[p:] - x_ref[p:]).max() / scale)
+                                           for k, v in sols.items()}
+        res["fp64_1e-13_iterations"] = it_ref
+        res["time_speedup_mixed_vs_fp64"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
+        return res
+
+    model = mme.MixedModel(fd, rec, y, n, ctx=ctx)
+    model.set_qinv_csr(A.ptr, A.col, A.val)
+    out["pblup"] = run_solves(model, "PBLUP")
+    if n_geno > 0:
+        # genotyped: the last n_geno animals (youngest generations)
+        gmap = np.arange(n - n_geno, n, dtype=np.int32)
+        rows, pitch = synth.bed_on_device(n_geno, m_geno, maf=(0.01, 0.5), missing_rate=0.0, seed=seed + 2)
+        reader = cb.PlinkReader.from_device_rows(ctx, rows.data_ptr(), pitch, n_geno, m_geno)
+        gb = cb.GrmBuilder(cb.Options(), ctx=ctx)
+        gb.compute(reader, outputs=cb.CB_GRM_OUT_F64)
+        G = torch.empty((n_geno, n_geno), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        gb.copyGrmToDevice(G.data_ptr(), n_geno)
+        del gb, reader, rows
         t0 = time.perf_counter()
-        x, it, rel = model.solve(lam, b, opts)
-        ctx.synchronize()
-        dt = time.perf_counter() - t0
-        sols[name] = x
-        out[name] = {"iterations": it, "seconds": dt, "iters_per_s": it / dt, "rel_residual": rel}
-        print(f"[bench] PCG {name}: {it} iterations in {dt:.3f} s (rel {rel:.2e})", flush=True)
-    # both stop at relative residual `tol`; their EBVs differ by up to
-    # cond(C) * tol. Against a tight FP64 solve (1e-13) each shows its own error.
-    x_ref, it_ref, _ = model.solve(lam, b, mme.solve_options(precision=_lib.CB_PREC_FP64, tol=1e-13,
-                                                             max_iter=20000))
-    p = model.p
-    u_ref = x_ref[p:]
-    scale = np.abs(u_ref).max()
-    out["ebv_rel_err_mixed_vs_fp64"] = float(np.abs(sols["mixed"][p:] - sols["fp64"][p:]).max() / scale)
-    out["ebv_rel_err_vs_fp64_1e-13"] = {k: float(np.abs(v[p:] - u_ref).max() / scale) for k, v in sols.items()}
-    out["fp64_1e-13_iterations"] = it_ref
-    out["time_speedup_mixed_vs_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+        D, _, st = ssgblup.build_hinv_blocks(dam, sire, gmap, G, ctx=ctx)
+        t_h = time.perf_counter() - t0
+        del G
+        print(f"[bench] H^-1 blocks in {t_h:.1f} s (tuneG a={st['a']:.4f} b={st['b']:.4f})", flush=True)
+        model.set_qinv_split(A.ptr, A.col, A.val, gmap, D.data_ptr(), n_geno)
+        del D
+        torch.cuda.empty_cache()
+        out["ssgblup"] = run_solves(model, "ssGBLUP")
+        out["ssgblup"].update({"genotyped": n_geno, "snps": m_geno, "hinv_setup_s": t_h})
     return out
 
 
