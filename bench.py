@@ -156,6 +156,54 @@ def reml_leg(ctx, n: int = 20000, m: int = 50000, k: int = 64, h2: float = 0.4, 
             "qinv": "eigen-bent G^-1 (invert_psd_matrix semantics)", "qinv_setup_s": t_inv}
 
 
+def c1_leg(ctx, n: int = 2000, m: int = 10000, h2: float = 0.3, ngroups: int = 20, seed: int = 20261203) -> dict:
+    """The reference's CPU-runnable GBLUP case: n=2000, m=10000, no missing,
+    G^-1 eigen-bent, AI-REML by the dense-exact step (the branch the reference
+    takes below p+q = 5000), from (1, 1) to sum |d sigma^2| < 1e-4."""
+    import time
+
+    import numpy as np
+    import torch
+
+    import cosmicblup_amd as cb
+    from cosmicblup_amd import _lib, design, linalg, mme, synth
+
+    print(f"[bench] C1 leg: n={n} m={m} dense-exact AI-REML", flush=True)
+    rows, pitch = synth.bed_on_device(n, m, maf=(0.05, 0.5), missing_rate=0.0, seed=seed)
+    reader = cb.PlinkReader.from_device_rows(ctx, rows.data_ptr(), pitch, n, m)
+    gb = cb.GrmBuilder(cb.Options(), ctx=ctx)
+    gb.compute(reader, outputs=cb.CB_GRM_OUT_F64)
+    G = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    gb.copyGrmToDevice(G.data_ptr(), n)
+    Q = torch.empty_like(G)
+    torch.cuda.synchronize()
+    linalg.invert_psd_device(G.data_ptr(), n, n, Q.data_ptr(), n, ctx=ctx)
+    rng = np.random.default_rng(seed)
+    qtl = torch.from_numpy(rng.choice(m, 500, replace=False))
+    Xq = decode_rows_dosage(rows, pitch, n, qtl)
+    g = (Xq - Xq.mean(0)) @ torch.from_numpy(rng.standard_normal(500)).cuda()
+    g = (g * np.sqrt(h2) / g.std()).cpu().numpy()
+    grp = rng.integers(0, ngroups, n)
+    y = 10.0 + rng.standard_normal(ngroups)[grp] + g + rng.standard_normal(n) * np.sqrt(1 - h2)
+    fd = design.build_fixed_design([[f"cg{int(c):02d}"] for c in grp])
+    model = mme.MixedModel(fd, np.arange(n, dtype=np.int32), y, n, ctx=ctx)
+    model.set_qinv_dense_device(Q.data_ptr(), n)
+    del G, Q, rows, gb, reader
+    R = mme.AIREML(model, mme.reml_options(algorithm=_lib.CB_VCE_AI_EXACT, max_iter=20, tol=1e-4))
+    R.solve()  # warm-up (allocations, library initialisation)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    R.solve()
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    print(f"[bench] C1 dense-exact: {R.iterations} iterations in {dt:.3f} s, vu={R.var_u:.5f} ve={R.var_e:.5f}",
+          flush=True)
+    return {"n": n, "m": m, "algorithm": "dense-exact AI (vce.cpp:2859-3195)", "iterations": R.iterations,
+            "converged": R.converged, "seconds": dt, "iters_per_s": R.iterations / dt, "var_u": R.var_u,
+            "var_e": R.var_e, "se_var_u": R.getSEVarsU()[0], "se_var_e": R.getSEVarE()}
+
+
 def pcg_leg(ctx, n_gen: int = 10, per_gen: int = 100000, n_rec: int = 600000, n_groups: int = 2000,
             h2: float = 0.25, n_geno: int = 20000, m_geno: int = 40000, tol: float = 1e-10,
             seed: int = 20263203) -> dict:
@@ -260,6 +308,7 @@ def main() -> None:
     ap.add_argument("--reml-only", action="store_true",
                     help="only run the REML leg and print its JSON object")
     ap.add_argument("--no-pcg", action="store_true", help="skip the 1M-animal PBLUP PCG leg")
+    ap.add_argument("--no-c1", action="store_true", help="skip the n=2000 dense-exact REML leg")
     ap.add_argument("--pcg-only", action="store_true",
                     help="only run the PBLUP PCG leg and print its JSON object")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -357,6 +406,9 @@ def main() -> None:
         torch.cuda.empty_cache()
     if not args.no_pcg:
         out["pblup_pcg"] = pcg_leg(ctx)
+        torch.cuda.empty_cache()
+    if not args.no_c1:
+        out["c1_dense_exact"] = c1_leg(ctx)
     if not args.no_cpu:
         torch.cuda.empty_cache()
         print("[bench] timing the CPU oracle sample", flush=True)

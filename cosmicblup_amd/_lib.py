@@ -58,6 +58,7 @@ CB_STOP_RELATIVE = 0
 CB_STOP_ABSOLUTE = 1
 CB_VCE_AI = 0
 CB_VCE_EM = 1
+CB_VCE_AI_EXACT = 2
 
 
 class CosmicError(RuntimeError):

@@ -1789,6 +1789,269 @@ int cb_mme_solve(cb_mme* M, double lambda, const cb_solve_options* opts, int64_t
     return st;
 }
 
+}  // extern "C"
+
+namespace {
+
+// C(lambda) = W'W + lambda blockdiag(0, Q) as a dense column-major dim x dim
+// matrix (C must hold dim * dim doubles).
+void assemble_dense_C(cb_mme* M, double lambda, double* C) {
+    hipStream_t st = M->ctx->stream;
+    const int64_t dim = M->dim, p = M->p, q = M->q;
+    CB_HIP(hipMemsetAsync(C, 0, (size_t)dim * dim * 8, st));
+    k_dense_from_csr<<<blocks(dim), 256, 0, st>>>(M->wp.p, M->wc.p, M->wv64.p, dim, 0, 1.0, C, dim);
+    if (M->qkind == 0) {
+        k_dense_add_q<<<dim3(blocks(q), (unsigned)q), 256, 0, st>>>(M->Qd64.p, M->ldq, q, lambda, C + p * dim + p, dim);
+    } else {
+        k_dense_from_csr<<<blocks(q), 256, 0, st>>>(M->qp.p, M->qc.p, M->qv64.p, q, 0, lambda, C + p * dim + p, dim);
+        if (M->qkind == 2)
+            k_dense_add_block<<<dim3(blocks(M->ng), (unsigned)M->ng), 256, 0, st>>>(M->Qd64.p, M->ldq, M->ng,
+                                                                                  M->gmap.p, lambda, C + p * dim + p,
+                                                                                  dim);
+    }
+    CB_HIP(hipGetLastError());
+}
+
+// W = [X Z] dense (n x dim, column-major) from the design rows
+__global__ void k_build_w(const int64_t* __restrict__ xp, const int32_t* __restrict__ xc,
+                          const double* __restrict__ xv, const int32_t* __restrict__ aid, int64_t n, int64_t p,
+                          double* __restrict__ W) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    for (int64_t k = xp[r]; k < xp[r + 1]; ++k) W[(int64_t)xc[k] * n + r] += xv[k];
+    if (aid[r] >= 0) W[(p + aid[r]) * n + r] = 1.0;
+}
+
+// out (cols x rows) = in^T, in rows x cols, both column-major
+__global__ void k_transpose(const double* __restrict__ in, int64_t rows, int64_t cols, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (i >= rows) return;
+    out[i * cols + j] = in[j * rows + i];
+}
+
+// Z A Z' for incidence Z: ZAZ[i, j] = A[aid_i, aid_j] (0 for records without a level)
+__global__ void k_zazt(const double* __restrict__ A, int64_t lda, const int32_t* __restrict__ aid, int64_t n,
+                       double* __restrict__ ZAZ) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (i >= n) return;
+    const int32_t ai = aid[i], aj = aid[j];
+    ZAZ[j * n + i] = (ai >= 0 && aj >= 0) ? A[(int64_t)aj * lda + ai] : 0.0;
+}
+
+// copy the lower triangle onto the upper (column-major)
+__global__ void k_sym_from_lower(double* __restrict__ A, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (i >= n || i >= j) return;
+    A[j * n + i] = A[i * n + j];
+}
+
+// P = (I - H) / ve in place
+__global__ void k_p_from_h(double* __restrict__ H, int64_t n, double ve) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (i >= n) return;
+    const double h = H[j * n + i];
+    H[j * n + i] = ((i == j ? 1.0 : 0.0) - h) / ve;
+}
+
+// partial sums over column chunks: sum Mu_ij Mu_ji, sum Mu_ij P_ij, sum P_ij^2, tr Mu, tr P
+__global__ __launch_bounds__(256) void k_exact_traces(const double* __restrict__ Mu, const double* __restrict__ P,
+                                                      int64_t n, double* __restrict__ part) {
+    __shared__ double red[5][256];
+    const int64_t per = (n + DOT_CHUNKS - 1) / DOT_CHUNKS;
+    const int64_t j0 = (int64_t)blockIdx.x * per, j1 = min(n, j0 + per);
+    double uu = 0, ue = 0, ee = 0, tu = 0, te = 0;
+    for (int64_t j = j0; j < j1; ++j)
+        for (int64_t i = threadIdx.x; i < n; i += 256) {
+            const double mij = Mu[j * n + i], mji = Mu[i * n + j], pij = P[j * n + i];
+            uu += mij * mji;
+            ue += mij * pij;
+            ee += pij * pij;
+            if (i == j) {
+                tu += mij;
+                te += pij;
+            }
+        }
+    red[0][threadIdx.x] = uu;
+    red[1][threadIdx.x] = ue;
+    red[2][threadIdx.x] = ee;
+    red[3][threadIdx.x] = tu;
+    red[4][threadIdx.x] = te;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) part[blockIdx.x * 5 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+#define CB_RB(call, what)                                                                              \
+    do {                                                                                               \
+        if ((call) != rocblas_status_success) throw Error(CB_ERR_DEVICE, std::string(what " failed")); \
+    } while (0)
+
+// run_dense_exact_step (vce.cpp:2859-3195) on dense device matrices.
+struct DenseExact {
+    cb_mme* M;
+    int64_t n, p, q, dim;
+    rocblas_handle h = nullptr;
+    DevBuf<double> W, Wt, ZAZ, C, B, H, Mu, b, e, t, part;
+    DevBuf<int> info;
+    std::vector<double> y;
+    double logdetA = 0;
+    explicit DenseExact(cb_mme* m) : M(m), n(m->n), p(m->p), q(m->q), dim(m->dim) {}
+    ~DenseExact() {
+        if (h) rocblas_destroy_handle(h);
+    }
+    hipStream_t st() const { return M->ctx->stream; }
+
+    void init() {
+        CB_REQUIRE(n <= 40000 && dim <= 40000, "dense-exact AI-REML: n and p + q are limited to 40000");
+        CB_RB(rocblas_create_handle(&h), "rocblas_create_handle");
+        CB_RB(rocblas_set_stream(h, st()), "rocblas_set_stream");
+        info.alloc(1);
+        // W and W'
+        W.alloc((size_t)n * dim);
+        CB_HIP(hipMemsetAsync(W.p, 0, W.bytes(), st()));
+        k_build_w<<<blocks(n), 256, 0, st()>>>(M->xp.p, M->xc.p, M->xv.p, M->aid.p, n, p, W.p);
+        Wt.alloc((size_t)dim * n);
+        k_transpose<<<dim3(blocks(n), (unsigned)dim), 256, 0, st()>>>(W.p, n, dim, Wt.p);
+        CB_HIP(hipGetLastError());
+        // A = Q^-1 (LLT) and log|A| = -log|Q| (initialize_dense, vce.cpp:2888-2895,2963-2970)
+        DevBuf<double> A;
+        A.alloc((size_t)q * q);
+        CB_HIP(hipMemsetAsync(A.p, 0, A.bytes(), st()));
+        if (M->qkind == 0) {
+            k_dense_add_q<<<dim3(blocks(q), (unsigned)q), 256, 0, st()>>>(M->Qd64.p, M->ldq, q, 1.0, A.p, q);
+        } else {
+            k_dense_from_csr<<<blocks(q), 256, 0, st()>>>(M->qp.p, M->qc.p, M->qv64.p, q, 0, 1.0, A.p, q);
+            if (M->qkind == 2)
+                k_dense_add_block<<<dim3(blocks(M->ng), (unsigned)M->ng), 256, 0, st()>>>(M->Qd64.p, M->ldq, M->ng,
+                                                                                        M->gmap.p, 1.0, A.p, q);
+        }
+        CB_HIP(hipGetLastError());
+        CB_RB(rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)q, A.p, (rocblas_int)q, info.p), "dpotrf(Q)");
+        check_info("Qinv is not positive definite");
+        logdetA = -logdiag2(A.p, q);
+        CB_RB(rocsolver_dpotri(h, rocblas_fill_lower, (rocblas_int)q, A.p, (rocblas_int)q, info.p), "dpotri(Q)");
+        k_sym_from_lower<<<dim3(blocks(q), (unsigned)q), 256, 0, st()>>>(A.p, q);
+        ZAZ.alloc((size_t)n * n);
+        k_zazt<<<dim3(blocks(n), (unsigned)n), 256, 0, st()>>>(A.p, q, M->aid.p, n, ZAZ.p);
+        CB_HIP(hipGetLastError());
+        C.alloc((size_t)dim * dim);
+        B.alloc((size_t)dim * n);
+        H.alloc((size_t)n * n);
+        Mu.alloc((size_t)n * n);
+        b.alloc(dim);
+        e.alloc(n);
+        t.alloc(n);
+        part.alloc((size_t)DOT_CHUNKS * 5);
+        y.resize(n);
+        CB_HIP(hipMemcpyAsync(y.data(), M->y.p, n * 8, hipMemcpyDeviceToHost, st()));
+        CB_HIP(hipStreamSynchronize(st()));
+    }
+    void check_info(const char* msg) {
+        int hi = 0;
+        CB_HIP(hipMemcpyAsync(&hi, info.p, sizeof(int), hipMemcpyDeviceToHost, st()));
+        CB_HIP(hipStreamSynchronize(st()));
+        CB_REQUIRE(hi == 0, msg);
+    }
+    // 2 sum log L_ii of a Cholesky factor
+    double logdiag2(const double* L, int64_t m) {
+        DevBuf<double> pp;
+        pp.alloc(DOT_CHUNKS);
+        k_logdiag_part<<<DOT_CHUNKS, 256, 0, st()>>>(L, m, pp.p);
+        std::vector<double> hp(DOT_CHUNKS);
+        CB_HIP(hipMemcpyAsync(hp.data(), pp.p, DOT_CHUNKS * 8, hipMemcpyDeviceToHost, st()));
+        CB_HIP(hipStreamSynchronize(st()));
+        double sum = 0;
+        for (double v : hp) sum += v;
+        return 2.0 * sum;
+    }
+    // factor C(ve/vu), solve the MME, residuals e; returns false when C is
+    // not positive definite. logL of the dense-exact step (vce.cpp:2972-3003).
+    bool solve_logl(double vu, double ve, double* logl, std::vector<double>* sol) {
+        assemble_dense_C(M, ve / vu, C.p);
+        CB_RB(rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)dim, C.p, (rocblas_int)dim, info.p), "dpotrf(C)");
+        int hi = 0;
+        CB_HIP(hipMemcpyAsync(&hi, info.p, sizeof(int), hipMemcpyDeviceToHost, st()));
+        CB_HIP(hipStreamSynchronize(st()));
+        if (hi != 0) return false;
+        const double ldC = logdiag2(C.p, dim);
+        CB_HIP(hipMemcpyAsync(b.p, M->rhs.data(), dim * 8, hipMemcpyHostToDevice, st()));
+        CB_RB(rocsolver_dpotrs(h, rocblas_fill_lower, (rocblas_int)dim, 1, C.p, (rocblas_int)dim, b.p,
+                               (rocblas_int)dim), "dpotrs(C, rhs)");
+        // e = y - W sol
+        CB_HIP(hipMemcpyAsync(e.p, M->y.p, n * 8, hipMemcpyDeviceToDevice, st()));
+        const double one = 1.0, mone = -1.0;
+        CB_RB(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host), "pointer mode");
+        CB_RB(rocblas_dgemv(h, rocblas_operation_none, (rocblas_int)n, (rocblas_int)dim, &mone, W.p, (rocblas_int)n,
+                            b.p, 1, &one, e.p, 1), "dgemv(W sol)");
+        std::vector<double> he(n);
+        CB_HIP(hipMemcpyAsync(he.data(), e.p, n * 8, hipMemcpyDeviceToHost, st()));
+        if (sol) {
+            sol->resize(dim);
+            CB_HIP(hipMemcpyAsync(sol->data(), b.p, dim * 8, hipMemcpyDeviceToHost, st()));
+        }
+        CB_HIP(hipStreamSynchronize(st()));
+        double ye = 0;
+        for (int64_t i = 0; i < n; ++i) ye += y[i] * he[i];
+        *logl = -0.5 * (ldC + ((double)n - (double)p - (double)q) * std::log(ve) + (double)q * std::log(vu) + ye / ve +
+                        logdetA);
+        return true;
+    }
+    // gradient and AI terms at (vu, ve) (vce.cpp:3048-3120); solve_logl at the
+    // same point must have run just before (C holds its factor, e its residuals)
+    void terms(double ve, double grad[2], double ai[4]) {
+        CB_HIP(hipMemcpyAsync(B.p, Wt.p, B.bytes(), hipMemcpyDeviceToDevice, st()));
+        CB_RB(rocsolver_dpotrs(h, rocblas_fill_lower, (rocblas_int)dim, (rocblas_int)n, C.p, (rocblas_int)dim, B.p,
+                               (rocblas_int)dim), "dpotrs(C, W')");
+        const double one = 1.0, zero = 0.0;
+        CB_RB(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
+                            (rocblas_int)dim, &one, W.p, (rocblas_int)n, B.p, (rocblas_int)dim, &zero, H.p,
+                            (rocblas_int)n), "dgemm(W C^-1 W')");
+        k_p_from_h<<<dim3(blocks(n), (unsigned)n), 256, 0, st()>>>(H.p, n, ve);  // H now holds P
+        CB_HIP(hipGetLastError());
+        CB_RB(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
+                            (rocblas_int)n, &one, H.p, (rocblas_int)n, ZAZ.p, (rocblas_int)n, &zero, Mu.p,
+                            (rocblas_int)n), "dgemm(P ZAZ')");
+        k_exact_traces<<<DOT_CHUNKS, 256, 0, st()>>>(Mu.p, H.p, n, part.p);
+        CB_HIP(hipGetLastError());
+        // Py = e / ve ; t = ZAZ' Py
+        std::vector<double> he(n), py(n), ht(n), hp((size_t)DOT_CHUNKS * 5);
+        CB_HIP(hipMemcpyAsync(he.data(), e.p, n * 8, hipMemcpyDeviceToHost, st()));
+        CB_HIP(hipStreamSynchronize(st()));
+        for (int64_t i = 0; i < n; ++i) py[i] = he[i] / ve;
+        CB_HIP(hipMemcpyAsync(e.p, py.data(), n * 8, hipMemcpyHostToDevice, st()));
+        CB_RB(rocblas_dgemv(h, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n, &one, ZAZ.p, (rocblas_int)n,
+                            e.p, 1, &zero, t.p, 1), "dgemv(ZAZ' Py)");
+        CB_HIP(hipMemcpyAsync(ht.data(), t.p, n * 8, hipMemcpyDeviceToHost, st()));
+        CB_HIP(hipMemcpyAsync(hp.data(), part.p, hp.size() * 8, hipMemcpyDeviceToHost, st()));
+        CB_HIP(hipStreamSynchronize(st()));
+        double s[5] = {0, 0, 0, 0, 0};
+        for (int k = 0; k < DOT_CHUNKS; ++k)
+            for (int j = 0; j < 5; ++j) s[j] += hp[(size_t)k * 5 + j];
+        double qu = 0, qe = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            qu += py[i] * ht[i];
+            qe += py[i] * py[i];
+        }
+        ai[0] = 0.5 * s[0];
+        ai[1] = ai[2] = 0.5 * s[1];
+        ai[3] = 0.5 * s[2];
+        grad[0] = -0.5 * s[3] + 0.5 * qu;
+        grad[1] = -0.5 * s[4] + 0.5 * qe;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
 int cb_reml_score(cb_mme* M, const cb_reml_options* opts, double var_u, double var_e, double* grad,
                   double* ai, double* trace, double* solution) {
     return guarded([&] {
@@ -1796,6 +2059,20 @@ int cb_reml_score(cb_mme* M, const cb_reml_options* opts, double var_u, double v
         CB_REQUIRE(M->qkind >= 0, "cb_reml_score: Qinv not set");
         CB_REQUIRE(var_u > 0 && var_e > 0, "cb_reml_score: variances must be positive");
         set_device(M->ctx);
+        if (opts->algorithm == CB_VCE_AI_EXACT) {
+            DenseExact de(M);
+            de.init();
+            double lg = 0, g[2], a[4];
+            std::vector<double> sol;
+            CB_REQUIRE(de.solve_logl(var_u, var_e, &lg, solution ? &sol : nullptr),
+                       "cb_reml_score: C is not positive definite");
+            de.terms(var_e, g, a);
+            if (grad) { grad[0] = g[0]; grad[1] = g[1]; }
+            if (ai) std::copy(a, a + 4, ai);
+            if (trace) *trace = std::nan("");
+            if (solution) std::copy(sol.begin(), sol.end(), solution);
+            return;
+        }
         cb_reml_work w;
         reml_setup(M, *opts, w);
         RemlState s;
@@ -1815,9 +2092,78 @@ int cb_reml_run(cb_mme* M, const cb_reml_options* opts, double var_u0, double va
         CB_REQUIRE(M && opts && var_u && var_e, "cb_reml_run: NULL argument");
         CB_REQUIRE(M->qkind >= 0, "cb_reml_run: Qinv not set");
         CB_REQUIRE(var_u0 > 0 && var_e0 > 0, "cb_reml_run: initial variances must be positive");
-        CB_REQUIRE(opts->algorithm == CB_VCE_AI || opts->algorithm == CB_VCE_EM, "cb_reml_run: unknown algorithm");
+        CB_REQUIRE(opts->algorithm == CB_VCE_AI || opts->algorithm == CB_VCE_EM || opts->algorithm == CB_VCE_AI_EXACT,
+                   "cb_reml_run: unknown algorithm");
         set_device(M->ctx);
         const cb_reml_options& o = *opts;
+        if (o.algorithm == CB_VCE_AI_EXACT) {
+            // AI_REML::solve with run_dense_exact_step (vce.cpp:741-837,2949-3195)
+            DenseExact de(M);
+            de.init();
+            double vu = var_u0, ve = var_e0, A[4] = {0, 0, 0, 0};
+            int64_t it = 0;
+            int conv = 0;
+            bool have_ai = false;
+            for (; it < o.max_iter;) {
+                double cur = 0, g[2], a[4];
+                if (!de.solve_logl(vu, ve, &cur, nullptr)) break;  // C not PD: step failed
+                de.terms(ve, g, a);
+                std::copy(a, a + 4, A);
+                project_spd_2x2(A);
+                have_ai = true;
+                const double det = A[0] * A[3] - A[1] * A[2];
+                double d0 = (A[3] * g[0] - A[1] * g[1]) / det;
+                double d1 = (A[0] * g[1] - A[2] * g[0]) / det;
+                bool ok = false;
+                double nu = vu, ne = ve;
+                for (int k = 0; k < 6; ++k) {  // vce.cpp:3141-3171
+                    nu = vu + d0;
+                    ne = ve + d1;
+                    if (nu > 0 && ne > 0) {
+                        double nl = 0;
+                        if (de.solve_logl(nu, ne, &nl, nullptr) && (nl > cur || k == 5)) {
+                            ok = true;
+                            break;
+                        }
+                    }
+                    d0 *= 0.5;
+                    d1 *= 0.5;
+                }
+                if (!ok) break;  // failed step: stop, unconverged
+                const double diff = std::fabs(ne - ve) + std::fabs(nu - vu);
+                vu = nu;
+                ve = ne;
+                if (hist_vu) hist_vu[it] = vu;
+                if (hist_ve) hist_ve[it] = ve;
+                ++it;
+                if (diff < o.tol) {
+                    conv = 1;
+                    break;
+                }
+            }
+            *var_u = vu;
+            *var_e = ve;
+            if (iterations) *iterations = it;
+            if (converged) *converged = conv;
+            if (solution || ai_last || se) {
+                double lg = 0, g[2], a[4];
+                std::vector<double> sol;
+                CB_REQUIRE(de.solve_logl(vu, ve, &lg, &sol), "cb_reml_run: C is not positive definite");
+                if (solution) std::copy(sol.begin(), sol.end(), solution);
+                if (!have_ai) {
+                    de.terms(ve, g, a);
+                    std::copy(a, a + 4, A);
+                    project_spd_2x2(A);
+                }
+                if (ai_last) std::copy(A, A + 4, ai_last);
+                if (se) {
+                    const double det = A[0] * A[3] - A[1] * A[2];
+                    se[0] = std::sqrt(std::max(0.0, A[3] / det));
+                    se[1] = std::sqrt(std::max(0.0, A[0] / det));
+                }
+            }
+            return;
+        }
         cb_reml_work w;
         reml_setup(M, o, w);
         // variance stabiliser of vce.cpp:1370-1389
@@ -1937,17 +2283,7 @@ int cb_reml_logl(cb_mme* M, double var_u, double var_e, double* logl, double* lo
         DevBuf<double> C, b;
         C.alloc((size_t)dim * dim);
         b.alloc(dim);
-        CB_HIP(hipMemsetAsync(C.p, 0, C.bytes(), st));
-        k_dense_from_csr<<<blocks(dim), 256, 0, st>>>(M->wp.p, M->wc.p, M->wv64.p, dim, 0, 1.0, C.p, dim);
-        if (M->qkind == 0) {
-            k_dense_add_q<<<dim3(blocks(q), (unsigned)q), 256, 0, st>>>(M->Qd64.p, M->ldq, q, lambda, C.p + p * dim + p, dim);
-        } else {
-            k_dense_from_csr<<<blocks(q), 256, 0, st>>>(M->qp.p, M->qc.p, M->qv64.p, q, 0, lambda, C.p + p * dim + p, dim);
-            if (M->qkind == 2)
-                k_dense_add_block<<<dim3(blocks(M->ng), (unsigned)M->ng), 256, 0, st>>>(M->Qd64.p, M->ldq, M->ng, M->gmap.p,
-                                                                                      lambda, C.p + p * dim + p, dim);
-        }
-        CB_HIP(hipGetLastError());
+        assemble_dense_C(M, lambda, C.p);
         CB_HIP(hipMemcpyAsync(b.p, M->rhs.data(), dim * 8, hipMemcpyHostToDevice, st));
         rocblas_handle h = nullptr;
         CB_REQUIRE(rocblas_create_handle(&h) == rocblas_status_success, "cb_reml_logl: rocblas handle");

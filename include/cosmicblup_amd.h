@@ -199,6 +199,9 @@ int cb_mme_solve(cb_mme* mme, double lambda, const cb_solve_options* opts, int64
 /* ---- REML -------------------------------------------------------------- */
 #define CB_VCE_AI 0  /* data-driven AI step, vce.cpp:2414-2857 */
 #define CB_VCE_EM 1  /* EM + Hutchinson step, vce.cpp:1490-1574 */
+#define CB_VCE_AI_EXACT 2  /* dense-exact AI step (run_dense_exact_step, vce.cpp:2859-3195):
+                              exact traces from dense P = (I - W C^-1 W')/ve on the device,
+                              <= 6 logL halvings; n, p + q <= 40000 */
 
 typedef struct {
     int algorithm;      /* CB_VCE_*; default AI */

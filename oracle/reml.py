@@ -9,6 +9,12 @@ path: dense FP64 numpy restatements of the reference algorithms.
                        + Hutchinson step (vce.cpp:1490-1574); variance
                        stabiliser vce.cpp:1370-1389
   logL                 calc_logL_internal (vce.cpp:3533-3588)
+  dense_exact_eval     run_dense_exact_step terms (vce.cpp:2859-3120): P from
+                       the MME, M_u = P Z A Z', AI = 1/2 [tr(M_u M_u),
+                       tr(M_u P); ., ||P||_F^2], exact gradient, logL with
+                       log|A|
+  dense_exact_run      AI_REML loop with that step and its <= 6 logL halvings
+                       (vce.cpp:3121-3195)
   henderson_ainv       invertA_henderson without inbreeding
                        (src/core/a_matrix.cpp:392-439, textbook rules)
   ml_inbreeding        ml() (a_matrix.cpp:127-186): F and d_ii by the dense
@@ -25,7 +31,9 @@ formulas; the score vs central finite differences of logL; EM and AI fixed
 points coincide; Henderson A^-1 vs the inverse of the tabular A on small
 pedigrees and the 4-animal fixture by hand; ml_inbreeding's F vs
 diag(tabular A) - 1 on inbred pedigrees and henderson_ainv_f vs the inverse
-of the tabular A there; probes vs a scalar splitmix64.
+of the tabular A there; dense_exact_eval's gradient vs central finite
+differences of its logL and its P vs the V-form projection
+V^-1 - V^-1 X (X'V^-1 X)^-1 X'V^-1; probes vs a scalar splitmix64.
 """
 from __future__ import annotations
 
@@ -330,3 +338,65 @@ def hinv_blocks(dam, sire, gmap, G, tune_g=True, alpha=0.05, tau=1.0, omega=1.0,
     A22r = A22 + ridge * np.eye(ng) if ridge > 0 else A22
     A22inv = np.linalg.inv(A22r)
     return tau * Ginv - omega * A22inv, A22, (a, b)
+
+
+def dense_exact_eval(X, aid, y, Q, vu, ve):
+    """The terms of run_dense_exact_step (vce.cpp:2859-3120) at (vu, ve)."""
+    n, p = X.shape
+    q = Q.shape[0]
+    A = np.linalg.inv(Q)
+    sign, ldQ = np.linalg.slogdet(Q)
+    log_det_A = -ldQ
+    lam = ve / vu
+    C, rhs, W = mme_dense(X, aid, y, Q, lam)
+    sign, ldC = np.linalg.slogdet(C)
+    sol = np.linalg.solve(C, rhs)
+    e = y - W @ sol
+    logL = -0.5 * (ldC + (n - p - q) * np.log(ve) + q * np.log(vu) + y @ e / ve + log_det_A)
+    H = W @ np.linalg.solve(C, W.T)
+    P = (np.eye(n) - H) / ve
+    Z = W[:, p:]
+    ZAZ = Z @ A @ Z.T
+    Mu = P @ ZAZ
+    ai = 0.5 * np.array([[np.sum(Mu * Mu.T), np.sum(Mu * P)], [np.sum(Mu * P), np.sum(P * P)]])
+    Py = e / ve
+    grad = np.array([-0.5 * np.trace(Mu) + 0.5 * Py @ ZAZ @ Py, -0.5 * np.trace(P) + 0.5 * Py @ Py])
+    return {"logL": logL, "grad": grad, "ai": ai, "P": P, "sol": sol}
+
+
+def dense_exact_run(X, aid, y, Q, vu0=1.0, ve0=1.0, max_iter=20, tol=1e-4):
+    """AI_REML::solve with run_dense_exact_step (vce.cpp:741-837,2949-3195):
+    Newton on the projected AI, up to 6 halvings, accepting a positive step
+    that raises logL or, at the last halving, any positive step. A failed
+    step stops the run (unconverged)."""
+    vu, ve = vu0, ve0
+    hist = []
+    conv = False
+    ai_last = None
+    for _ in range(max_iter):
+        t = dense_exact_eval(X, aid, y, Q, vu, ve)
+        A = project_spd_2x2(t["ai"])
+        ai_last = A
+        delta = np.linalg.solve(A, t["grad"])
+        ok = False
+        for k in range(6):
+            nu, ne = vu + delta[0], ve + delta[1]
+            if nu > 0 and ne > 0:
+                nl = dense_exact_eval(X, aid, y, Q, nu, ne)["logL"]
+                if nl > t["logL"] or k == 5:
+                    ok = True
+                    break
+            delta = delta * 0.5
+        if not ok:
+            break
+        diff = abs(ne - ve) + abs(nu - vu)
+        vu, ve = nu, ne
+        hist.append((vu, ve))
+        if diff < tol:
+            conv = True
+            break
+    se = np.sqrt(np.maximum(0.0, np.diag(np.linalg.inv(ai_last))))
+    lam = ve / vu
+    C, rhs, _ = mme_dense(X, aid, y, Q, lam)
+    return {"vu": vu, "ve": ve, "hist": hist, "converged": conv, "sol": np.linalg.solve(C, rhs),
+            "ai_last": ai_last, "se": se}

@@ -247,3 +247,26 @@ def test_pblup_csr_ainv(cb, ctx):
     R.solve()
     o = orl.reml_run(design.dense_x(fd), rec, y, Ai, Zp=orl.probes(nan, 16, 3))
     np.testing.assert_allclose([R.var_u, R.var_e], [o["vu"], o["ve"]], rtol=VC_TOL)
+
+
+def test_dense_exact_score_and_run_match_oracle(cb, ctx):
+    """run_dense_exact_step (vce.cpp:2859-3195) on the device vs the oracle."""
+    fd, aid, y, Q = gblup_problem(n=300, m=1500, missing_records=9)
+    M = mme.MixedModel(fd, aid, y, Q.shape[0])
+    M.set_qinv_dense(Q)
+    X = design.dense_x(fd)
+    opts = mme.reml_options(algorithm=_lib.CB_VCE_AI_EXACT, max_iter=30, tol=1e-6)
+    R = mme.AIREML(M, opts)
+    g, ai, _, sol = R.score(0.6, 0.9)
+    t = orl.dense_exact_eval(X, aid, y, Q, 0.6, 0.9)
+    np.testing.assert_allclose(g, t["grad"], rtol=1e-8, atol=1e-10 * np.abs(t["grad"]).max())
+    np.testing.assert_allclose(ai, t["ai"], rtol=1e-9)
+    assert ebv_err(sol[M.p:], t["sol"][M.p:]) < 1e-10
+    R.solve()
+    o = orl.dense_exact_run(X, aid, y, Q, max_iter=30, tol=1e-6)
+    assert R.iterations == len(o["hist"]) and R.converged == o["converged"]
+    # the same Newton path; its last step (< tol = 1e-6) carries the rounding
+    # differences of the two factorisations
+    np.testing.assert_allclose([R.var_u, R.var_e], [o["vu"], o["ve"]], rtol=1e-6)
+    np.testing.assert_allclose([R.getSEVarsU()[0], R.getSEVarE()], o["se"], rtol=1e-5)
+    assert ebv_err(R.solution[M.p:], o["sol"][M.p:]) < 1e-6

@@ -124,3 +124,45 @@ def test_pblup_minimal_fixture_ainv_by_hand():
     Ai = orl.henderson_ainv(sire, dam)
     expect = np.array([[2, 1, -1, -1], [1, 2, -1, -1], [-1, -1, 2, 0], [-1, -1, 0, 2]], dtype=float)
     np.testing.assert_allclose(Ai, expect)
+
+
+def _small_gblup(n=120, m=500, seed=3):
+    from cosmicblup_amd import design, synth
+    from oracle import grm as og
+
+    x, _ = synth.simulate_dosages(n, m, maf=(0.05, 0.5), seed=seed)
+    G, _, _ = og.grm_vanraden(synth.encode_bed(x), n, m)
+    Q = og.invert_psd_matrix(G + 0.05 * np.eye(n))
+    y, cg = synth.simulate_phenotype(x, n_qtl=30, h2=0.4, n_groups=4, seed=seed + 1)
+    fd = design.build_fixed_design([[c] for c in cg])
+    return design.dense_x(fd), np.arange(n, dtype=np.int32), y, Q
+
+
+def test_dense_exact_gradient_is_fd_of_logL():
+    X, aid, y, Q = _small_gblup()
+    vu, ve = 0.6, 0.9
+    t = orl.dense_exact_eval(X, aid, y, Q, vu, ve)
+    h = 1e-5
+    fu = (orl.dense_exact_eval(X, aid, y, Q, vu * (1 + h), ve)["logL"]
+          - orl.dense_exact_eval(X, aid, y, Q, vu * (1 - h), ve)["logL"]) / (2 * h * vu)
+    fe = (orl.dense_exact_eval(X, aid, y, Q, vu, ve * (1 + h))["logL"]
+          - orl.dense_exact_eval(X, aid, y, Q, vu, ve * (1 - h))["logL"]) / (2 * h * ve)
+    np.testing.assert_allclose(t["grad"], [fu, fe], rtol=1e-6)
+
+
+def test_dense_exact_P_is_the_V_form_projection():
+    X, aid, y, Q = _small_gblup(n=80, m=300)
+    vu, ve = 0.7, 1.1
+    P = orl.dense_exact_eval(X, aid, y, Q, vu, ve)["P"]
+    V = vu * np.linalg.inv(Q) + ve * np.eye(y.size)
+    Vi = np.linalg.inv(V)
+    PV = Vi - Vi @ X @ np.linalg.solve(X.T @ Vi @ X, X.T @ Vi)
+    np.testing.assert_allclose(P, PV, atol=1e-10)
+
+
+def test_dense_exact_run_reaches_the_score_zero():
+    X, aid, y, Q = _small_gblup()
+    r = orl.dense_exact_run(X, aid, y, Q, max_iter=50, tol=1e-8)
+    assert r["converged"]
+    g = orl.dense_exact_eval(X, aid, y, Q, r["vu"], r["ve"])["grad"]
+    assert np.abs(g).max() < 1e-4 * np.abs(orl.dense_exact_eval(X, aid, y, Q, 1.0, 1.0)["grad"]).max()
