@@ -44,6 +44,10 @@ class HinvOptions(C.Structure):
                 ("ridge", C.c_double)]
 
 
+class MvOptions(C.Structure):
+    _fields_ = [("max_iter", C.c_int64), ("tol", C.c_double)]
+
+
 class RemlOptions(C.Structure):
     _fields_ = [("algorithm", C.c_int), ("max_iter", C.c_int64), ("tol", C.c_double),
                 ("n_probes", C.c_int64), ("seed", C.c_uint64), ("trace_exact", C.c_int),
@@ -124,6 +128,10 @@ _SIGS = {
     "cb_hinv_options_default": (None, [C.POINTER(HinvOptions)]),
     "cb_hinv_blocks": (C.c_int, [c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, C.POINTER(HinvOptions), c_vp,
                                  c_i64, c_vp, c_i64, c_vp]),
+    "cb_mv_options_default": (None, [C.POINTER(MvOptions)]),
+    "cb_mv_vce": (C.c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, C.POINTER(MvOptions), c_vp, c_vp,
+                            c_i64p, C.POINTER(C.c_int)]),
+    "cb_mv_solve": (C.c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
     "cb_ainv_henderson": (C.c_int, [c_i64, c_vp, c_vp, C.POINTER(c_vp)]),
     "cb_ainv_nnz": (C.c_int, [c_vp, c_i64p]),
     "cb_ainv_copy_csr": (C.c_int, [c_vp, c_vp, c_vp, c_vp]),

@@ -276,6 +276,25 @@ int cb_hinv_blocks(cb_context* ctx, int64_t n, const int32_t* dam, const int32_t
                    const int32_t* gmap, const double* dG, int64_t ldg, const cb_hinv_options* opts, double* dD,
                    int64_t ldd, double* dA22, int64_t lda22, double* stats);
 
+/* ---- multi-trait ------------------------------------------------------- */
+typedef struct {
+    int64_t max_iter;  /* default 500 = max(500, 25 * ai_maxit) (variance_estimator.cpp:291) */
+    double tol;        /* relative Frobenius change of Vg and Ve, default 1e-4 (vce_tol) */
+} cb_mv_options;
+void cb_mv_options_default(cb_mv_options* opts);
+/* MvEigenVce::prepare + runNullModel with covariate projection
+ * (src/core/eigen_vce.cpp:106-262, as variance_estimator.cpp:285-294 runs
+ * it): complete cases only. Y: n x t (t <= 4) column-major host array, X:
+ * n x c dense column-major host array (c < n - 1), dG: the n x n G on the
+ * device (leading dimension ldg). Vg, Ve: t x t outputs (row-major). */
+int cb_mv_vce(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* Y, const double* X, const double* dG,
+              int64_t ldg, const cb_mv_options* opts, double* Vg, double* Ve, int64_t* iterations, int* converged);
+/* GSSolve::solveMultiTrait (src/solve/mixed_model_solver.cpp:425-524): the
+ * dense Kronecker MME solved by Cholesky on the device ((n + c) t <= 60000).
+ * dGinv: n x n G^-1 on the device; beta (c x t) and u (n x t) column-major. */
+int cb_mv_solve(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* Y, const double* X,
+                const double* dGinv, int64_t ldg, const double* Vg, const double* Ve, double* beta, double* u);
+
 /* ---- pedigree relationship inverse ----------------------------------- */
 typedef struct cb_ainv cb_ainv;
 /* genmatrix::invertA_henderson(dam, sire) (src/core/a_matrix.cpp:392-439)

@@ -400,3 +400,77 @@ def dense_exact_run(X, aid, y, Q, vu0=1.0, ve0=1.0, max_iter=20, tol=1e-4):
     C, rhs, _ = mme_dense(X, aid, y, Q, lam)
     return {"vu": vu, "ve": ve, "hist": hist, "converged": conv, "sol": np.linalg.solve(C, rhs),
             "ai_last": ai_last, "se": se}
+
+
+def mv_eigen_vce(Y, X, G, max_iter=500, tol=1e-4):
+    """MvEigenVce with covariate projection (src/core/eigen_vce.cpp:106-262,
+    options of variance_estimator.cpp:289-294): A = null space of X' from a
+    QR of X, eigen-rotation of A'GA, EM on the rotated rows; start
+    Vg = Ve = Cov(Y)/2 with the divisor n - c - 1 the reference uses; stop
+    when both relative Frobenius changes are below tol. Returns (Vg, Ve,
+    iterations, converged)."""
+    n, t = Y.shape
+    c = X.shape[1]
+    Qf, _ = np.linalg.qr(X, mode="complete")
+    A = Qf[:, c:]
+    Yr = A.T @ Y
+    Gr = A.T @ G @ A
+    D, U = np.linalg.eigh(Gr)
+    D = np.where(D < 0, 0.0, D)
+    Uty = U.T @ Yr
+    ns = n - c
+    Yc = Y - Y.mean(axis=0)
+    Cov = Yc.T @ Yc / (ns - 1)
+    Vg, Ve = 0.5 * Cov, 0.5 * Cov
+    it = 0
+    conv = False
+    for it in range(1, max_iter + 1):
+        # the per-row updates of updateEM, batched over the rotated rows
+        Vi = D[:, None, None] * Vg[None] + Ve[None]
+        Pi = np.linalg.inv(Vi)
+        Wg = D[:, None, None] * np.einsum("ab,ibc->iac", Vg, Pi)
+        uh = np.einsum("iab,ib->ia", Wg, Uty)
+        ok = D > 1e-12
+        Sg = (np.einsum("ia,ib->ab", uh[ok], uh[ok] / D[ok, None]) + ok.sum() * Vg
+              - np.einsum("iab,bc->ac", Wg[ok], Vg) + (~ok).sum() * Vg)
+        We = np.einsum("ab,ibc->iac", Ve, Pi)
+        eh = np.einsum("iab,ib->ia", We, Uty)
+        Se = np.einsum("ia,ib->ab", eh, eh) + ns * Ve - np.einsum("iab,bc->ac", We, Ve)
+        Vg_new, Ve_new = Sg / ns, Se / ns
+        dg = np.linalg.norm(Vg_new - Vg) / max(np.linalg.norm(Vg), 1e-12)
+        de = np.linalg.norm(Ve_new - Ve) / max(np.linalg.norm(Ve), 1e-12)
+        Vg, Ve = Vg_new, Ve_new
+        if dg < tol and de < tol:
+            conv = True
+            break
+    return Vg, Ve, it, conv
+
+
+def mv_solve(Y, X, Ginv, Vg, Ve):
+    """GSSolve::solveMultiTrait (src/solve/mixed_model_solver.cpp:425-524):
+    the dense Kronecker MME with unknowns trait-major; returns (beta p x t,
+    u n x t)."""
+    n, t = Y.shape
+    p = X.shape[1]
+    Vei, Vgi = np.linalg.inv(Ve), np.linalg.inv(Vg)
+    C11 = np.kron(Vei, X.T @ X)
+    C12 = np.kron(Vei, X.T)
+    C22 = np.kron(Vei, np.eye(n)) + np.kron(Vgi, Ginv)
+    L = np.block([[C11, C12], [C12.T, C22]])
+    rhs = np.concatenate([(X.T @ Y @ Vei.T).T.ravel(), (Y @ Vei.T).T.ravel()])
+    s = np.linalg.solve(L, rhs)
+    return s[:p * t].reshape(t, p).T, s[p * t:].reshape(t, n).T
+
+
+def mv_reml_logl(Y, X, G, Vg, Ve):
+    """Multivariate REML log-likelihood (up to a constant) of vec(Y) with
+    V = Vg (x) G + Ve (x) I: -1/2 (log|V| + log|X~' V^-1 X~| + y'Py),
+    X~ = I_t (x) X, unknowns trait-major."""
+    n, t = Y.shape
+    V = np.kron(Vg, G) + np.kron(Ve, np.eye(n))
+    Xt = np.kron(np.eye(t), X)
+    y = Y.T.ravel()
+    Vi = np.linalg.inv(V)
+    XVX = Xt.T @ Vi @ Xt
+    P = Vi - Vi @ Xt @ np.linalg.solve(XVX, Xt.T @ Vi)
+    return -0.5 * (np.linalg.slogdet(V)[1] + np.linalg.slogdet(XVX)[1] + y @ P @ y)

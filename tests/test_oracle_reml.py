@@ -166,3 +166,53 @@ def test_dense_exact_run_reaches_the_score_zero():
     assert r["converged"]
     g = orl.dense_exact_eval(X, aid, y, Q, r["vu"], r["ve"])["grad"]
     assert np.abs(g).max() < 1e-4 * np.abs(orl.dense_exact_eval(X, aid, y, Q, 1.0, 1.0)["grad"]).max()
+
+
+def _mv_problem(n=60, t=2, seed=4):
+    from cosmicblup_amd import synth
+    from oracle import grm as og
+
+    rng = np.random.default_rng(seed)
+    x, _ = synth.simulate_dosages(n, 400, maf=(0.1, 0.5), seed=seed)
+    G, _, _ = og.grm_vanraden(synth.encode_bed(x), n, 400)
+    G = G + 0.05 * np.eye(n)
+    Lg = np.linalg.cholesky(G)
+    Vg = np.array([[1.0, 0.4], [0.4, 0.8]])[:t, :t]
+    Ve = np.array([[1.0, 0.2], [0.2, 1.2]])[:t, :t]
+    U = Lg @ rng.standard_normal((n, t)) @ np.linalg.cholesky(Vg).T
+    E = rng.standard_normal((n, t)) @ np.linalg.cholesky(Ve).T
+    X = np.column_stack([np.ones(n), rng.integers(0, 2, n)])
+    Y = X @ rng.standard_normal((2, t)) + U + E
+    return Y, X, G
+
+
+def test_mv_em_fixed_point_is_a_reml_stationary_point():
+    """At the EM fixed point every partial derivative of the multivariate REML
+    logL vanishes (central differences on each (co)variance parameter)."""
+    Y, X, G = _mv_problem()
+    Vg, Ve, it, conv = orl.mv_eigen_vce(Y, X, G, max_iter=200000, tol=1e-12)
+    assert conv
+    h = 1e-6
+    for M in ("g", "e"):
+        for (a, b) in ((0, 0), (0, 1), (1, 1)):
+            E = np.zeros((2, 2))
+            E[a, b] = E[b, a] = h
+            up = orl.mv_reml_logl(Y, X, G, Vg + (E if M == "g" else 0), Ve + (E if M == "e" else 0))
+            dn = orl.mv_reml_logl(Y, X, G, Vg - (E if M == "g" else 0), Ve - (E if M == "e" else 0))
+            assert abs((up - dn) / (2 * h)) < 1e-5
+
+
+def test_mv_solve_is_the_v_form_blup():
+    Y, X, G = _mv_problem(n=40)
+    Vg = np.array([[0.9, 0.3], [0.3, 0.7]])
+    Ve = np.array([[1.1, 0.1], [0.1, 0.9]])
+    beta, u = orl.mv_solve(Y, X, np.linalg.inv(G), Vg, Ve)
+    n, t = Y.shape
+    V = np.kron(Vg, G) + np.kron(Ve, np.eye(n))
+    Xt = np.kron(np.eye(t), X)
+    y = Y.T.ravel()
+    Vi = np.linalg.inv(V)
+    b = np.linalg.solve(Xt.T @ Vi @ Xt, Xt.T @ Vi @ y)
+    uv = np.kron(Vg, G) @ Vi @ (y - Xt @ b)
+    np.testing.assert_allclose(beta.T.ravel(), b, atol=1e-9)
+    np.testing.assert_allclose(u.T.ravel(), uv, atol=1e-9)
