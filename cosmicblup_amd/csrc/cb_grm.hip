@@ -249,6 +249,26 @@ __global__ __launch_bounds__(256) void k_miss_sample_lists(
     }
 }
 
+// Sample lists regrouped for k_miss_cross_h: every 4 consecutive entries
+// (lists are padded to a multiple of 8) become 8 words, the 4 SNP indices
+// and the 4 byte-chunk words of their K (word c, byte t = byte c of K_t),
+// the operand layout of v_dot4 against 4 SNPs' dosages.
+__global__ void k_miss_groups(const int32_t* __restrict__ lj, const uint32_t* __restrict__ lk, int64_t ngroups,
+                              uint32_t* __restrict__ lg) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    uint32_t K[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        lg[g * 8 + t] = (uint32_t)lj[4 * g + t];
+        K[t] = lk[4 * g + t];
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        lg[g * 8 + 4 + c] = ((K[0] >> (8 * c)) & 0xffu) | (((K[1] >> (8 * c)) & 0xffu) << 8) |
+                            (((K[2] >> (8 * c)) & 0xffu) << 16) | (((K[3] >> (8 * c)) & 0xffu) << 24);
+}
+
 // Rows of the BED re-encoded for k_miss_cross: each 2-bit field holds the
 // dosage (0, 1, 2), or 3 for a missing call.
 __global__ void k_encode_d3(const uint8_t* __restrict__ bed, uint8_t* __restrict__ out, int64_t pitch,
@@ -393,6 +413,127 @@ __global__ __launch_bounds__(256) void k_miss_cross(const uint8_t* __restrict__ 
                 const uint64_t T = ((uint64_t)acc[s][0] + ((uint64_t)acc[s][1] << 8) +
                                     ((uint64_t)acc[s][2] << 16) + ((uint64_t)acc[s][3] << 24)) >> (2 * (s & 3));
                 v = (double)T * TWO_M29 + (double)(long long)myf[s * 64] * (1.0 / 281474976710656.0);
+            }
+            E[k * lde + i] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The same E with two lanes per row dword: lane 2w + h takes samples
+// 8h .. 8h + 7 of dword w (W_q for q = 2h, 2h + 1), so a wave covers 512
+// samples and each lane keeps 32 accumulators instead of 64 — twice the
+// resident waves to hide the row-gather latency, the same VALU work per
+// (k, i, j) term. The list comes regrouped (k_miss_groups), so the K byte
+// chunks are loaded, not assembled per step. Arithmetic as in k_miss_cross.
+constexpr int MC_I = 512;  // samples per block column
+__global__ __launch_bounds__(256) void k_miss_cross_h(const uint8_t* __restrict__ bedd, int64_t pitch, int64_t n,
+                                                      int64_t bed_bytes,
+                                                      const int64_t* __restrict__ smp_off,
+                                                      const uint32_t* __restrict__ lg,
+                                                      const unsigned long long* __restrict__ sq, int kw,
+                                                      double* __restrict__ E, int64_t lde) {
+    __shared__ unsigned long long fl[4 * 8 * 64];  // [wave][sample][lane], 16 KB
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int h = lane & 1;
+    const int64_t I0 = (int64_t)blockIdx.y * MC_I;
+    const int64_t dw = I0 / 16 + (lane >> 1);  // dword of each SNP row shared by this lane pair
+    const int64_t i_lane = I0 + 16 * (lane >> 1) + 8 * h;
+    const int64_t dwc = dw * 4 < pitch ? dw : 0;  // clamped in-row dword
+    const uint32_t voff = (uint32_t)(dwc * 4);
+    // raw buffer over the re-encoded rows (pitch * m < 2^31 bytes, checked by the host);
+    // 0x00020000: the CDNA buffer-descriptor word 3 for untyped dword access
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)bedd, (short)0, (int)bed_bytes,
+                                                                       0x00020000);
+    unsigned long long* myf = fl + wave * 8 * 64 + lane;
+    // byte selectors of the first transpose stage for this half
+    const uint32_t sel = h ? 0x07030602u : 0x05010400u;
+    const int per_wave = kw / 4;
+    for (int kq = 0; kq < per_wave; ++kq) {
+        const int64_t k = (int64_t)blockIdx.x * kw + wave * per_wave + kq;
+        if (k >= n) break;
+        uint32_t mk[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            uint32_t mq = 0;
+#pragma unroll
+            for (int s2 = 0; s2 < 4; ++s2)
+                if (i_lane + 4 * q + s2 > k) mq |= 0x01010101u << (2 * s2);
+            mk[q] = mq;
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < 8; ++s2) myf[s2 * 64] = 0ull;
+        uint32_t acc[8][4];
+#pragma unroll
+        for (int s2 = 0; s2 < 8; ++s2)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[s2][c] = 0;
+        const int64_t b = smp_off[k], e = smp_off[k + 1];
+        // entries pp .. pp + 7 = groups pp/4, pp/4 + 1 of the regrouped list;
+        // row dword loads through a buffer resource: scalar row offset
+        // j * pitch, loop-invariant lane offset, no per-load address VALU
+        auto fetch = [&](int64_t pp, uint32_t (&c)[8]) {
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const uint32_t jj = lg[(pp / 4 + t / 4) * 8 + (t & 3)];
+                c[t] = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, jj * (uint32_t)pitch, 0);
+            }
+        };
+        auto step = [&](int64_t pp, const uint32_t (&cw)[8]) {
+#pragma unroll
+            for (int g = 0; g < 2; ++g) {
+                const uint32_t c0 = cw[4 * g], c1 = cw[4 * g + 1], c2 = cw[4 * g + 2], c3 = cw[4 * g + 3];
+                const uint32_t* grp = lg + (pp / 4 + g) * 8 + 4;
+                const uint32_t kc[4] = {grp[0], grp[1], grp[2], grp[3]};
+                const uint32_t t0 = __builtin_amdgcn_perm(c1, c0, sel);
+                const uint32_t t1 = __builtin_amdgcn_perm(c3, c2, sel);
+                uint32_t W[2];
+                W[0] = __builtin_amdgcn_perm(t1, t0, 0x05040100u);
+                W[1] = __builtin_amdgcn_perm(t1, t0, 0x07060302u);
+#pragma unroll
+                for (int q = 0; q < 2; ++q)
+#pragma unroll
+                    for (int s2 = 0; s2 < 4; ++s2) {
+                        const uint32_t A = W[q] & (0x03030303u << (2 * s2));
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+                            acc[4 * q + s2][c] = __builtin_amdgcn_udot4(A, kc[c], acc[4 * q + s2][c], false);
+                    }
+                uint32_t ev = (W[0] & (W[0] >> 1) & mk[0]) | ((W[1] & (W[1] >> 1) & mk[1]) << 1);
+                while (ev) {
+                    const int bit = __ffs(ev) - 1;
+                    ev &= ev - 1;
+                    const int t = bit >> 3, smp = 4 * (bit & 1) + ((bit >> 1) & 3);
+                    // K_t from byte t of the four chunk words
+                    const uint32_t Kt = ((kc[0] >> (8 * t)) & 0xffu) | (((kc[1] >> (8 * t)) & 0xffu) << 8) |
+                                        (((kc[2] >> (8 * t)) & 0xffu) << 16) | (((kc[3] >> (8 * t)) & 0xffu) << 24);
+                    const unsigned long long add =
+                        (((unsigned long long)Kt * Kt) >> 10) - ((unsigned long long)Kt * 6 << 19);
+                    atomicAdd(&myf[smp * 64], add);
+                }
+            }
+        };
+        uint32_t ra[8], rb[8];
+        if (b < e) fetch(b, ra);
+        for (int64_t p = b; p < e; p += 16) {
+            const bool m1 = p + 8 < e;
+            if (m1) fetch(p + 8, rb);
+            step(p, ra);
+            if (!m1) break;
+            if (p + 16 < e) fetch(p + 16, ra);
+            step(p + 8, rb);
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < 8; ++s2) {
+            const int64_t i = i_lane + s2;
+            if (i >= n) continue;
+            double v;
+            if (i == k) {
+                v = (double)sq[k] * (0.5 / 4503599627370496.0);  // s_k / 2
+            } else {
+                const uint64_t T = ((uint64_t)acc[s2][0] + ((uint64_t)acc[s2][1] << 8) +
+                                    ((uint64_t)acc[s2][2] << 16) + ((uint64_t)acc[s2][3] << 24)) >> (2 * (s2 & 3));
+                v = (double)T * TWO_M29 + (double)(long long)myf[s2 * 64] * (1.0 / 281474976710656.0);
             }
             E[k * lde + i] = v;
         }
@@ -759,11 +900,23 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
             k_encode_d3<<<dim3((unsigned)ceil_div(g->pitch / 4, 256), (unsigned)m), 256, 0, st>>>(
                 g->bed.p, bedd.p, g->pitch, n, m);
         });
-        const int kw = 64;
-        timed(ctx, "k_miss_cross", [&] {
-            k_miss_cross<<<dim3((unsigned)ceil_div(n, kw), (unsigned)ceil_div(n, 1024)), dim3(256), 0, st>>>(
-                bedd.p, g->pitch, n, d_smp_off.p, lj.p, lk.p, sq.p, kw, E.p, n);
-        });
+        if (getenv("CB_MISS_CROSS_V5") || g->pitch * m >= ((int64_t)1 << 31)) {
+            const int kw = 64;
+            timed(ctx, "k_miss_cross", [&] {
+                k_miss_cross<<<dim3((unsigned)ceil_div(n, kw), (unsigned)ceil_div(n, 1024)), dim3(256), 0, st>>>(
+                    bedd.p, g->pitch, n, d_smp_off.p, lj.p, lk.p, sq.p, kw, E.p, n);
+            });
+        } else {
+            const int kw = 64;
+            DevBuf<uint32_t> lg;
+            lg.alloc((size_t)nlist * 2);  // nlist / 4 groups of 8 words
+            k_miss_groups<<<dim3((unsigned)ceil_div(nlist / 4, 256)), dim3(256), 0, st>>>(lj.p, lk.p, nlist / 4, lg.p);
+            timed(ctx, "k_miss_cross", [&] {
+                k_miss_cross_h<<<dim3((unsigned)ceil_div(n, kw), (unsigned)ceil_div(n, MC_I)), dim3(256), 0, st>>>(
+                    bedd.p, g->pitch, n, g->pitch * m, d_smp_off.p, lg.p, sq.p, kw, E.p, n);
+            });
+            CB_HIP(hipStreamSynchronize(st));
+        }
         CB_HIP(hipStreamSynchronize(st));  // list buffers die at scope end
     }
 
