@@ -204,6 +204,108 @@ def c1_leg(ctx, n: int = 2000, m: int = 10000, h2: float = 0.3, ngroups: int = 2
             "var_e": R.var_e, "se_var_u": R.getSEVarsU()[0], "se_var_e": R.getSEVarE()}
 
 
+def c5_leg(ctx, n_inst: int = 96, m: int = 20000, n_lo: int = 500, n_hi: int = 8000, k: int = 32,
+           seed: int = 20265203) -> dict:
+    """Batched uneven instances: n log-uniform in [n_lo, n_hi], m SNPs,
+    MAF U(0.05, 0.5); even instances single-trait (AI-REML, mixed batched
+    PCG with k probes + y), odd instances 3-trait (r_g ~ U(-0.5, 0.8), h2 ~
+    U(0.1, 0.6), r_e = 0.2; projected eigen-rotation EM, then the Kronecker
+    MME solve). Per instance: GRM and G^-1 on the device (set-up, untimed),
+    then the timed estimation. Reports REML (or EM) iterations/s per size
+    bucket."""
+    import math
+    import time
+
+    import numpy as np
+    import torch
+
+    import cosmicblup_amd as cb
+    from cosmicblup_amd import _lib, design, linalg, mme, multitrait, synth
+
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(math.log(n_lo), math.log(n_hi), n_inst)).astype(int)
+    print(f"[bench] C5 leg: {n_inst} instances, n in [{sizes.min()}, {sizes.max()}], m={m}", flush=True)
+    recs = []
+    for idx, n in enumerate(sizes):
+        n = int(n)
+        rows, pitch = synth.bed_on_device(n, m, maf=(0.05, 0.5), missing_rate=0.0, seed=seed + 1 + idx)
+        reader = cb.PlinkReader.from_device_rows(ctx, rows.data_ptr(), pitch, n, m)
+        gb = cb.GrmBuilder(cb.Options(), ctx=ctx)
+        gb.compute(reader, outputs=cb.CB_GRM_OUT_F64)
+        G = torch.empty((n, n), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        gb.copyGrmToDevice(G.data_ptr(), n)
+        Q = torch.empty_like(G)
+        torch.cuda.synchronize()
+        linalg.invert_psd_device(G.data_ptr(), n, n, Q.data_ptr(), n, ctx=ctx)
+        grp = rng.integers(0, 10, n)
+        fd = design.build_fixed_design([[f"cg{int(c)}"] for c in grp])
+        qtl = torch.from_numpy(rng.choice(m, 200, replace=False))
+        Xq = decode_rows_dosage(rows, pitch, n, qtl)
+        Xc = (Xq - Xq.mean(0)).cpu().numpy()
+        del rows, gb, reader
+        if idx % 2 == 0:
+            h2 = float(rng.uniform(0.1, 0.6))
+            g = Xc @ rng.standard_normal(200)
+            g *= math.sqrt(h2) / g.std()
+            y = 10.0 + rng.standard_normal(10)[grp] + g + rng.standard_normal(n) * math.sqrt(1 - h2)
+            model = mme.MixedModel(fd, np.arange(n, dtype=np.int32), y, n, ctx=ctx)
+            model.set_qinv_dense_device(Q.data_ptr(), n)
+            R = mme.AIREML(model, mme.reml_options(n_probes=k, seed=idx, max_iter=20, tol=1e-4,
+                                                   solve=mme.solve_options(tol=1e-8)))
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            R.solve()
+            ctx.synchronize()
+            dt = time.perf_counter() - t0
+            recs.append({"n": n, "traits": 1, "iterations": R.iterations, "seconds": dt, "converged": R.converged})
+            del model
+        else:
+            t = 3
+            h2 = rng.uniform(0.1, 0.6, t)
+            rg = rng.uniform(-0.5, 0.8, 3)
+            Cg = np.array([[1, rg[0], rg[1]], [rg[0], 1, rg[2]], [rg[1], rg[2], 1]])
+            w, V = np.linalg.eigh(Cg)
+            Cg = (V * np.maximum(w, 0.05)) @ V.T  # nearest valid correlation-like matrix
+            d = np.sqrt(np.diag(Cg))
+            Cg = Cg / np.outer(d, d)
+            Vg = Cg * np.sqrt(np.outer(h2, h2))
+            Ve = (0.2 + 0.8 * np.eye(t)) * np.sqrt(np.outer(1 - h2, 1 - h2))
+            B = Xc @ rng.standard_normal((200, t))
+            B = (B - B.mean(0)) / B.std(0)
+            U = B @ np.linalg.cholesky(Vg + 1e-9 * np.eye(t)).T
+            E = rng.standard_normal((n, t)) @ np.linalg.cholesky(Ve).T
+            X = design.dense_x(fd)
+            Y = X @ rng.standard_normal((X.shape[1], t)) + U + E
+            v = multitrait.MvEigenVce(multitrait.MvEigenVce.Options(max_iter=500, tol=1e-4), ctx=ctx)
+            v.prepare(Y, X, G)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            v.runNullModel()
+            beta, u = multitrait.solveMultiTrait(Y, X, Q, v.getVg(), v.getVe(), ctx=ctx)
+            ctx.synchronize()
+            dt = time.perf_counter() - t0
+            recs.append({"n": n, "traits": 3, "iterations": v.iterations, "seconds": dt, "converged": v.converged})
+        del G, Q
+        torch.cuda.empty_cache()
+        print(f"[bench] C5 instance {idx + 1}/{n_inst}: n={n} traits={recs[-1]['traits']} "
+              f"{recs[-1]['iterations']} iterations in {recs[-1]['seconds']:.3f} s", flush=True)
+    buckets = [(500, 1000), (1000, 2000), (2000, 4000), (4000, 8001)]
+    out = {"instances": n_inst, "m": m, "probes": k, "per_bucket": []}
+    for lo, hi in buckets:
+        for tr in (1, 3):
+            sel = [r for r in recs if lo <= r["n"] < hi and r["traits"] == tr]
+            if not sel:
+                continue
+            it = sum(r["iterations"] for r in sel)
+            sec = sum(r["seconds"] for r in sel)
+            out["per_bucket"].append({"n_range": [lo, hi], "traits": tr, "instances": len(sel),
+                                      "iterations": it, "seconds": sec, "iters_per_s": it / sec,
+                                      "converged": sum(r["converged"] for r in sel)})
+    out["total_seconds"] = sum(r["seconds"] for r in recs)
+    return out
+
+
 def pcg_leg(ctx, n_gen: int = 10, per_gen: int = 100000, n_rec: int = 600000, n_groups: int = 2000,
             h2: float = 0.25, n_geno: int = 20000, m_geno: int = 40000, tol: float = 1e-10,
             seed: int = 20263203) -> dict:
@@ -309,6 +411,8 @@ def main() -> None:
                     help="only run the REML leg and print its JSON object")
     ap.add_argument("--no-pcg", action="store_true", help="skip the 1M-animal PBLUP PCG leg")
     ap.add_argument("--no-c1", action="store_true", help="skip the n=2000 dense-exact REML leg")
+    ap.add_argument("--no-c5", action="store_true", help="skip the 96-instance batched workload")
+    ap.add_argument("--c5-only", action="store_true", help="only run the 96-instance workload")
     ap.add_argument("--pcg-only", action="store_true",
                     help="only run the PBLUP PCG leg and print its JSON object")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -342,6 +446,10 @@ def main() -> None:
     if args.pcg_only:
         ctx = _lib.Context(0)
         print(json.dumps(pcg_leg(ctx)), flush=True)
+        return
+    if args.c5_only:
+        ctx = _lib.Context(0)
+        print(json.dumps(c5_leg(ctx)), flush=True)
         return
 
     n, m = args.n, args.m
@@ -409,6 +517,9 @@ def main() -> None:
         torch.cuda.empty_cache()
     if not args.no_c1:
         out["c1_dense_exact"] = c1_leg(ctx)
+    if not args.no_c5:
+        torch.cuda.empty_cache()
+        out["c5_batched"] = c5_leg(ctx)
     if not args.no_cpu:
         torch.cuda.empty_cache()
         print("[bench] timing the CPU oracle sample", flush=True)
