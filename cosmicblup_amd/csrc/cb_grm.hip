@@ -425,15 +425,30 @@ __global__ __launch_bounds__(256) void k_miss_cross(const uint8_t* __restrict__ 
 // samples and each lane keeps 32 accumulators instead of 64 — twice the
 // resident waves to hide the row-gather latency, the same VALU work per
 // (k, i, j) term. The list comes regrouped (k_miss_groups), so the K byte
-// chunks are loaded, not assembled per step. Arithmetic as in k_miss_cross.
-constexpr int MC_I = 512;  // samples per block column
+// chunks are loaded, not assembled per step. Arithmetic as in k_miss_cross,
+// except that both-missing events are not handled inside the dot loop: a
+// lane with events (about one in four per group) queues one record, and the
+// wave applies 64 events at a time, one per lane, instead of looping as long
+// as its busiest lane has events. Row gathers run two pairs of groups ahead
+// of the dot loop to cover their latency (the rows are random SNPs).
+constexpr int MC_I = 512;    // samples per block column
+constexpr int MC_RING = 256; // event records per wave (power of two)
+// Orders one wave's LDS accesses across its lanes (hardware keeps a wave's
+// LDS operations in order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 __global__ __launch_bounds__(256) void k_miss_cross_h(const uint8_t* __restrict__ bedd, int64_t pitch, int64_t n,
                                                       int64_t bed_bytes,
                                                       const int64_t* __restrict__ smp_off,
                                                       const uint32_t* __restrict__ lg,
+                                                      const uint32_t* __restrict__ lk,
                                                       const unsigned long long* __restrict__ sq, int kw,
                                                       double* __restrict__ E, int64_t lde) {
     __shared__ unsigned long long fl[4 * 8 * 64];  // [wave][sample][lane], 16 KB
+    __shared__ uint2 rings[4 * MC_RING];           // [wave][record], 8 KB
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int h = lane & 1;
     const int64_t I0 = (int64_t)blockIdx.y * MC_I;
@@ -445,7 +460,9 @@ __global__ __launch_bounds__(256) void k_miss_cross_h(const uint8_t* __restrict_
     // 0x00020000: the CDNA buffer-descriptor word 3 for untyped dword access
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)bedd, (short)0, (int)bed_bytes,
                                                                        0x00020000);
-    unsigned long long* myf = fl + wave * 8 * 64 + lane;
+    unsigned long long* wf = fl + wave * 8 * 64;
+    unsigned long long* myf = wf + lane;
+    uint2* ring = rings + wave * MC_RING;
     // byte selectors of the first transpose stage for this half
     const uint32_t sel = h ? 0x07030602u : 0x05010400u;
     const int per_wave = kw / 4;
@@ -463,21 +480,59 @@ __global__ __launch_bounds__(256) void k_miss_cross_h(const uint8_t* __restrict_
         }
 #pragma unroll
         for (int s2 = 0; s2 < 8; ++s2) myf[s2 * 64] = 0ull;
+        wave_sync();
         uint32_t acc[8][4];
 #pragma unroll
         for (int s2 = 0; s2 < 8; ++s2)
 #pragma unroll
             for (int c = 0; c < 4; ++c) acc[s2][c] = 0;
         const int64_t b = smp_off[k], e = smp_off[k + 1];
+        // Both-missing events leave the dot loop as (bits, group, lane)
+        // records in a wave-private ring; full batches of 64 records are
+        // drained one event per lane, the rest of a record's bits re-queued.
+        // head and tail are wave-uniform; the ring holds at most 191 records
+        // before a drain and 255 during one, so a drain's reads and re-queue
+        // writes never share a slot.
+        uint32_t head = 0, tail = 0;
+        auto push = [&](uint32_t bits, uint32_t meta) {
+            const unsigned long long bal = __ballot(bits != 0);
+            if (bal == 0) return;
+            if (bits != 0) {
+                const uint32_t pos =
+                    tail + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                ring[pos & (MC_RING - 1)] = make_uint2(bits, meta);
+            }
+            tail += (uint32_t)__popcll(bal);
+        };
+        auto drain = [&](uint32_t cnt) {
+            wave_sync();
+            uint32_t bits = 0, meta = 0;
+            if ((uint32_t)lane < cnt) {
+                const uint2 r = ring[(head + lane) & (MC_RING - 1)];
+                bits = r.x;
+                meta = r.y;
+            }
+            head += cnt;
+            if (bits != 0) {
+                const int bit = __ffs(bits) - 1;
+                bits &= bits - 1;
+                const int t = bit >> 3, smp = 4 * (bit & 1) + ((bit >> 1) & 3);
+                const unsigned long long Kt = lk[b + 4 * (int64_t)(meta >> 6) + t];
+                atomicAdd(&wf[smp * 64 + (meta & 63)], ((Kt * Kt) >> 10) - (Kt * 6 << 19));
+            }
+            push(bits, meta);
+        };
         // entries pp .. pp + 7 = groups pp/4, pp/4 + 1 of the regrouped list;
         // row dword loads through a buffer resource: scalar row offset
         // j * pitch, loop-invariant lane offset, no per-load address VALU
-        auto fetch = [&](int64_t pp, uint32_t (&c)[8]) {
+        auto jwords = [&](int64_t pp, uint32_t (&jw)[8]) {
 #pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const uint32_t jj = lg[(pp / 4 + t / 4) * 8 + (t & 3)];
-                c[t] = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, jj * (uint32_t)pitch, 0);
-            }
+            for (int t = 0; t < 8; ++t) jw[t] = lg[(pp / 4 + t / 4) * 8 + (t & 3)];
+        };
+        auto fetch = [&](const uint32_t (&jw)[8], uint32_t (&c)[8]) {
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                c[t] = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, jw[t] * (uint32_t)pitch, 0);
         };
         auto step = [&](int64_t pp, const uint32_t (&cw)[8]) {
 #pragma unroll
@@ -499,30 +554,40 @@ __global__ __launch_bounds__(256) void k_miss_cross_h(const uint8_t* __restrict_
                         for (int c = 0; c < 4; ++c)
                             acc[4 * q + s2][c] = __builtin_amdgcn_udot4(A, kc[c], acc[4 * q + s2][c], false);
                     }
-                uint32_t ev = (W[0] & (W[0] >> 1) & mk[0]) | ((W[1] & (W[1] >> 1) & mk[1]) << 1);
-                while (ev) {
-                    const int bit = __ffs(ev) - 1;
-                    ev &= ev - 1;
-                    const int t = bit >> 3, smp = 4 * (bit & 1) + ((bit >> 1) & 3);
-                    // K_t from byte t of the four chunk words
-                    const uint32_t Kt = ((kc[0] >> (8 * t)) & 0xffu) | (((kc[1] >> (8 * t)) & 0xffu) << 8) |
-                                        (((kc[2] >> (8 * t)) & 0xffu) << 16) | (((kc[3] >> (8 * t)) & 0xffu) << 24);
-                    const unsigned long long add =
-                        (((unsigned long long)Kt * Kt) >> 10) - ((unsigned long long)Kt * 6 << 19);
-                    atomicAdd(&myf[smp * 64], add);
-                }
+                const uint32_t ev = (W[0] & (W[0] >> 1) & mk[0]) | ((W[1] & (W[1] >> 1) & mk[1]) << 1);
+                push(ev, ((uint32_t)((pp - b) / 4 + g) << 6) | (uint32_t)lane);
             }
         };
-        uint32_t ra[8], rb[8];
-        if (b < e) fetch(b, ra);
-        for (int64_t p = b; p < e; p += 16) {
-            const bool m1 = p + 8 < e;
-            if (m1) fetch(p + 8, rb);
-            step(p, ra);
-            if (!m1) break;
-            if (p + 16 < e) fetch(p + 16, ra);
-            step(p + 8, rb);
+        // three row buffers: pair n is dotted while the rows of pair n + 2
+        // are in flight and the SNP indices of pair n + 3 load
+        uint32_t r0[8], r1[8], r2[8], jw[8];
+        if (b < e) {
+            jwords(b, jw);
+            fetch(jw, r0);
         }
+        if (b + 8 < e) {
+            jwords(b + 8, jw);
+            fetch(jw, r1);
+        }
+        if (b + 16 < e) jwords(b + 16, jw);
+        for (int64_t p = b; p < e; p += 24) {
+            if (p + 16 < e) fetch(jw, r2);
+            if (p + 24 < e) jwords(p + 24, jw);
+            step(p, r0);
+            while (tail - head >= 64) drain(64);
+            if (p + 8 >= e) break;
+            if (p + 24 < e) fetch(jw, r0);
+            if (p + 32 < e) jwords(p + 32, jw);
+            step(p + 8, r1);
+            while (tail - head >= 64) drain(64);
+            if (p + 16 >= e) break;
+            if (p + 32 < e) fetch(jw, r1);
+            if (p + 40 < e) jwords(p + 40, jw);
+            step(p + 16, r2);
+            while (tail - head >= 64) drain(64);
+        }
+        while (tail != head) drain(tail - head < 64 ? tail - head : 64);
+        wave_sync();
 #pragma unroll
         for (int s2 = 0; s2 < 8; ++s2) {
             const int64_t i = i_lane + s2;
@@ -900,7 +965,8 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
             k_encode_d3<<<dim3((unsigned)ceil_div(g->pitch / 4, 256), (unsigned)m), 256, 0, st>>>(
                 g->bed.p, bedd.p, g->pitch, n, m);
         });
-        if (getenv("CB_MISS_CROSS_V5") || g->pitch * m >= ((int64_t)1 << 31)) {
+        // the regrouped kernel: buffer offsets below 2^31, ring records carry 26-bit group indices
+        if (getenv("CB_MISS_CROSS_V5") || g->pitch * m >= ((int64_t)1 << 31) || m >= ((int64_t)1 << 26)) {
             const int kw = 64;
             timed(ctx, "k_miss_cross", [&] {
                 k_miss_cross<<<dim3((unsigned)ceil_div(n, kw), (unsigned)ceil_div(n, 1024)), dim3(256), 0, st>>>(
@@ -913,7 +979,7 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
             k_miss_groups<<<dim3((unsigned)ceil_div(nlist / 4, 256)), dim3(256), 0, st>>>(lj.p, lk.p, nlist / 4, lg.p);
             timed(ctx, "k_miss_cross", [&] {
                 k_miss_cross_h<<<dim3((unsigned)ceil_div(n, kw), (unsigned)ceil_div(n, MC_I)), dim3(256), 0, st>>>(
-                    bedd.p, g->pitch, n, g->pitch * m, d_smp_off.p, lg.p, sq.p, kw, E.p, n);
+                    bedd.p, g->pitch, n, g->pitch * m, d_smp_off.p, lg.p, lk.p, sq.p, kw, E.p, n);
             });
             CB_HIP(hipStreamSynchronize(st));
         }
