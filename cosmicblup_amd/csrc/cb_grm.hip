@@ -752,21 +752,32 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
     // (E(i,j) + E(j,i)) — so diagonal tiles come out bitwise symmetric.
     // 32x32 accumulator layout: register q of lane l holds
     // row 8(q/4) + 4(l/32) + q%4, column l%32.
+    // Each pass issues all 16 of a lane's loads before using any, so a
+    // block costs a few memory round trips, not one per row; the slabs are
+    // wave-private, so a wave orders its own LDS traffic and never waits
+    // for the other waves.
     __syncthreads();
     double* slabD = reinterpret_cast<double*>(lds) + wave * (32 * 33);
     int32_t* slabX = reinterpret_cast<int32_t*>(reinterpret_cast<double*>(lds) + 4 * 32 * 33) + wave * (32 * 33);
     const bool diag = ti == tj;
     const double Sdiv = S > 0.0 ? S : 1.0;
-#pragma unroll
+#pragma unroll 1
     for (int ab = 0; ab < 16; ++ab) {
         const int a = ab >> 2, b = ab & 3;
+        // registers cannot be indexed at run time: block ab's accumulator is
+        // picked by a (wave-uniform) branch, the rest of the body stays rolled
 #pragma unroll
-        for (int q = 0; q < 16; ++q) slabX[(8 * (q >> 2) + 4 * h + (q & 3)) * 33 + r] = acc[a][b][q];
-        __syncthreads();
+        for (int c = 0; c < 16; ++c)
+            if (c == ab) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q)
+                    slabX[(8 * (q >> 2) + 4 * h + (q & 3)) * 33 + r] = acc[c >> 2][c & 3][q];
+            }
+        wave_sync();
         const int64_t I0 = ti * TILE + wr * 128 + a * 32;  // block rows
         const int64_t J0 = tj * TILE + wc * 128 + b * 32;  // block columns
         if (MODE == 1) {
-#pragma unroll 1
+#pragma unroll
             for (int rr = 0; rr < 16; ++rr) {
                 const int row = h * 16 + rr;
                 const int64_t i = I0 + row, j = J0 + r;
@@ -778,22 +789,33 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
             }
         } else {
             if (E) {  // pass 1 (row pattern): stage E(i,j)
-#pragma unroll 1
+                double ev[16];
+#pragma unroll
                 for (int rr = 0; rr < 16; ++rr) {
-                    const int row = h * 16 + rr;
-                    const int64_t i = I0 + row, j = J0 + r;
-                    if (i < n && j < n) slabD[row * 33 + r] = E[i * lde + j];
+                    const int64_t i = I0 + h * 16 + rr, j = J0 + r;
+                    ev[rr] = (i < n && j < n) ? E[i * lde + j] : 0.0;
                 }
-                __syncthreads();
+#pragma unroll
+                for (int rr = 0; rr < 16; ++rr) slabD[(h * 16 + rr) * 33 + r] = ev[rr];
+                wave_sync();
             }
             // pass 2 (transposed pattern): form g, write the mirror
-#pragma unroll 1
+            const int64_t i = I0 + r;
+            const double ui = i < n ? u[i] : 0.0;
+            double et[16], uj[16];
+#pragma unroll
+            for (int rr = 0; rr < 16; ++rr) {
+                const int64_t j = J0 + h * 16 + rr;
+                et[rr] = (E && i < n && j < n) ? E[j * lde + i] : 0.0;
+                uj[rr] = j < n ? u[j] : 0.0;
+            }
+#pragma unroll
             for (int rr = 0; rr < 16; ++rr) {
                 const int col = h * 16 + rr;
-                const int64_t i = I0 + r, j = J0 + col;
+                const int64_t j = J0 + col;
                 if (i < n && j < n) {
-                    const double e = E ? slabD[r * 33 + col] + E[j * lde + i] : 0.0;
-                    const double v = (((double)slabX[r * 33 + col] - (u[i] + u[j])) + Cc) + e;
+                    const double e = E ? slabD[r * 33 + col] + et[rr] : 0.0;
+                    const double v = (((double)slabX[r * 33 + col] - (ui + uj[rr])) + Cc) + e;
                     const double g = v / Sdiv;
                     slabD[r * 33 + col] = g;
                     if (!diag) {
@@ -802,25 +824,24 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
                     }
                 }
             }
-            __syncthreads();
+            wave_sync();
             // pass 3 (row pattern): own positions
-#pragma unroll 1
+#pragma unroll
             for (int rr = 0; rr < 16; ++rr) {
                 const int row = h * 16 + rr;
-                const int64_t i = I0 + row, j = J0 + r;
-                if (i < n && j < n) {
+                const int64_t ii = I0 + row, j = J0 + r;
+                if (ii < n && j < n) {
                     const double g = slabD[row * 33 + r];
-                    if (G64) G64[i * ld + j] = g;
-                    if (G32) G32[i * ld + j] = (float)g;
+                    if (G64) G64[ii * ld + j] = g;
+                    if (G32) G32[ii * ld + j] = (float)g;
                 }
             }
         }
-        __syncthreads();
+        wave_sync();
     }
 }
 
 }  // namespace
-
 struct cb_grm {
     cb_context* ctx = nullptr;
     cb_grm_options opts{};
