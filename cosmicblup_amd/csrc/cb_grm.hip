@@ -636,10 +636,10 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
     // four 32x33 FP64 slabs plus four 32x33 int32 slabs (50.7 KB) in the
     // epilogue.
     constexpr int PANELS_WORDS = 2 * TILE * LDS_PITCH;  // A and B of one buffer
-    constexpr int LDS_WORDS = 2 * PANELS_WORDS;
+    constexpr int LDS_WORDS = 3 * PANELS_WORDS;
     static_assert(LDS_WORDS >= 4 * 32 * 33 * 3, "epilogue slabs exceed LDS block");
     __shared__ __attribute__((aligned(16))) uint32_t lds[LDS_WORDS];
-    uint32_t* ldsA = lds;                      // buffer b: lds + b * PANELS_WORDS
+    uint32_t* ldsA = lds;  // buffer b: lds + b * PANELS_WORDS
     uint32_t* ldsB = lds + TILE * LDS_PITCH;
 
     // XCD-aware order: hardware deals workgroups round-robin over 8 XCDs;
@@ -665,65 +665,85 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
     const u32x4* gA = reinterpret_cast<const u32x4*>(X2 + (ti * TILE + tid) * 16);
     const u32x4* gB = reinterpret_cast<const u32x4*>(X2 + (tj * TILE + tid) * 16);
     const int64_t stage_stride = n_pad * 4;  // in 16-byte units
-    // Double-buffered LDS panels: stage s is computed from buffer s&1 while
-    // stage s+1 (already in registers) is written to the other buffer and
-    // stage s+2 is fetched from global memory — one barrier per stage.
-    u32x4 pa0 = gA[0], pa1 = gA[1], pa2 = gA[2], pa3 = gA[3];
-    u32x4 pb0 = gB[0], pb1 = gB[1], pb2 = gB[2], pb3 = gB[3];
-    {
-        u32x4* la = reinterpret_cast<u32x4*>(ldsA + tid * LDS_PITCH);
-        u32x4* lb = reinterpret_cast<u32x4*>(ldsB + tid * LDS_PITCH);
+    // Three LDS buffers: stage s is computed from buffer s%3 while the rows
+    // of stage s+2 (in registers) go to buffer (s+2)%3 and stage s+3 is
+    // fetched. A lane's words of a stage are read in two halves (k-steps
+    // 0-3, 4-7), each half one stage-half ahead of its use, and the decode
+    // of the next k-step runs under the MFMAs of the current one, across
+    // stage boundaries, so the MFMA pipe never waits for LDS. One barrier
+    // per stage, at its start.
+    u32x4 pa0, pa1, pa2, pa3, pb0, pb1, pb2, pb3;
+    auto gload = [&](int64_t st_) {
+        const int64_t o = st_ * stage_stride;
+        pa0 = gA[o]; pa1 = gA[o + 1]; pa2 = gA[o + 2]; pa3 = gA[o + 3];
+        pb0 = gB[o]; pb1 = gB[o + 1]; pb2 = gB[o + 2]; pb3 = gB[o + 3];
+    };
+    auto lstore = [&](int buf) {
+        u32x4* la = reinterpret_cast<u32x4*>(ldsA + buf * PANELS_WORDS + tid * LDS_PITCH);
+        u32x4* lb = reinterpret_cast<u32x4*>(ldsB + buf * PANELS_WORDS + tid * LDS_PITCH);
         la[0] = pa0; la[1] = pa1; la[2] = pa2; la[3] = pa3;
         lb[0] = pb0; lb[1] = pb1; lb[2] = pb2; lb[3] = pb3;
-    }
+    };
+    // half hf (0: k-steps 0-3, 1: 4-7) of buffer buf: word kk of row block a is .[kk&3]
+    u32x4 H0a[4], H0b[4], H1a[4], H1b[4];
+    auto lread = [&](int buf, int hf, u32x4 (&xa)[4], u32x4 (&xb)[4]) {
+        const uint32_t* cA = ldsA + buf * PANELS_WORDS;
+        const uint32_t* cB = ldsB + buf * PANELS_WORDS;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+            xa[a] = *reinterpret_cast<const u32x4*>(cA + (wr * 128 + a * 32 + r) * LDS_PITCH + 8 * h + 4 * hf);
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            xb[b] = *reinterpret_cast<const u32x4*>(cB + (wc * 128 + b * 32 + r) * LDS_PITCH + 8 * h + 4 * hf);
+    };
+    auto wsel = [](const u32x4& x, int k) { return k == 0 ? x.x : k == 1 ? x.y : k == 2 ? x.z : x.w; };
+
+    gload(0);
+    lstore(0);
     if (n_stages > 1) {
-        pa0 = gA[stage_stride]; pa1 = gA[stage_stride + 1]; pa2 = gA[stage_stride + 2]; pa3 = gA[stage_stride + 3];
-        pb0 = gB[stage_stride]; pb1 = gB[stage_stride + 1]; pb2 = gB[stage_stride + 2]; pb3 = gB[stage_stride + 3];
+        gload(1);
+        lstore(1);
     }
+    gload(n_stages > 2 ? 2 : n_stages - 1);
     __syncthreads();
+    lread(0, 0, H0a, H0b);
+    v4i fa[4], fb[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) fa[a] = decode16(H0a[a].x);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) fb[b] = decode16(H0b[b].x);
+    int cur = 0;
     for (int64_t s = 0; s < n_stages; ++s) {
-        const uint32_t* curA = ldsA + (s & 1) * PANELS_WORDS;
-        const uint32_t* curB = ldsB + (s & 1) * PANELS_WORDS;
-        u32x4 wa[4][2], wb[4][2];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const u32x4* p = reinterpret_cast<const u32x4*>(curA + (wr * 128 + a * 32 + r) * LDS_PITCH + 8 * h);
-            wa[a][0] = p[0];
-            wa[a][1] = p[1];
-        }
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const u32x4* p = reinterpret_cast<const u32x4*>(curB + (wc * 128 + b * 32 + r) * LDS_PITCH + 8 * h);
-            wb[b][0] = p[0];
-            wb[b][1] = p[1];
-        }
-        // Software-pipelined k-steps: the decode of step kk+1 (VALU) is
-        // interleaved with the 16 MFMAs of step kk, four VALU per MFMA.
-        v4i fa[4], fb[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) fa[a] = decode16(word_of(wa[a], 0));
-#pragma unroll
-        for (int b = 0; b < 4; ++b) fb[b] = decode16(word_of(wb[b], 0));
+        if (s > 0) __syncthreads();
+        const int nxt = cur == 2 ? 0 : cur + 1, far = nxt == 2 ? 0 : nxt + 1;
 #pragma unroll
         for (int kk = 0; kk < 8; ++kk) {
+            // branch-free stage body: past the last stage the copies are
+            // redundant (clamped stage index, buffers no longer read)
+            if (kk == 0) lread(cur, 1, H1a, H1b);
+            if (kk == 1) lstore(far);
+            if (kk == 2) gload(s + 3 < n_stages ? s + 3 : n_stages - 1);
+            if (kk == 4) lread(nxt, 0, H0a, H0b);
             v4i na[4], nb[4];
-            if (kk < 7) {
+            {
 #pragma unroll
-                for (int a = 0; a < 4; ++a) na[a] = decode16(word_of(wa[a], kk + 1));
+                for (int a = 0; a < 4; ++a)
+                    na[a] = decode16(kk < 3 ? wsel(H0a[a], kk + 1) : kk < 7 ? wsel(H1a[a], kk - 3) : H0a[a].x);
 #pragma unroll
-                for (int b = 0; b < 4; ++b) nb[b] = decode16(word_of(wb[b], kk + 1));
+                for (int b = 0; b < 4; ++b)
+                    nb[b] = decode16(kk < 3 ? wsel(H0b[b], kk + 1) : kk < 7 ? wsel(H1b[b], kk - 3) : H0b[b].x);
             }
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int b = 0; b < 4; ++b)
                     acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
-            if (kk < 7) {
 #pragma unroll
-                for (int g = 0; g < 16; ++g) {
-                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
-                    __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // four VALU
-                }
+            for (int g = 0; g < 16; ++g) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+                __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // four VALU
+                if (g < 8 && (kk == 0 || kk == 1 || kk == 2 || kk == 4))
+                    __builtin_amdgcn_sched_group_barrier(0x320, 1, 0);  // one LDS or global access
             }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -731,18 +751,7 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
 #pragma unroll
             for (int b = 0; b < 4; ++b) fb[b] = nb[b];
         }
-        if (s + 1 < n_stages) {
-            u32x4* la = reinterpret_cast<u32x4*>(ldsA + ((s + 1) & 1) * PANELS_WORDS + tid * LDS_PITCH);
-            u32x4* lb = reinterpret_cast<u32x4*>(ldsB + ((s + 1) & 1) * PANELS_WORDS + tid * LDS_PITCH);
-            la[0] = pa0; la[1] = pa1; la[2] = pa2; la[3] = pa3;
-            lb[0] = pb0; lb[1] = pb1; lb[2] = pb2; lb[3] = pb3;
-            if (s + 2 < n_stages) {
-                const int64_t o = (s + 2) * stage_stride;
-                pa0 = gA[o]; pa1 = gA[o + 1]; pa2 = gA[o + 2]; pa3 = gA[o + 3];
-                pb0 = gB[o]; pb1 = gB[o + 1]; pb2 = gB[o + 2]; pb3 = gB[o + 3];
-            }
-        }
-        __syncthreads();
+        cur = nxt;
     }
 
     // Epilogue, one 32x32 block at a time through wave-private LDS slabs
