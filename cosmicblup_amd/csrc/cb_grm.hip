@@ -623,7 +623,8 @@ __device__ __forceinline__ uint32_t word_of(const u32x4 (&v)[2], int kk) {
 // A 32x32x32 fragment takes 16 int8 per lane = one packed word decoded with
 // (w >> 2q) & 0x03030303; A and B fragments share that k-layout, so the
 // product is the Gram regardless of how the hardware orders k.
-// MODE 0: FP64/FP32 G with fused correction; MODE 1: raw int32 Gram.
+// MODE 0: FP64/FP32 G with fused correction including the missing-call
+// term E; MODE 2: the same without missing calls; MODE 1: raw int32 Gram.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, int64_t n,
                                               int64_t n_pad, int64_t n_stages,
@@ -632,7 +633,8 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
                                               int64_t lde, double* __restrict__ G64,
                                               float* __restrict__ G32, int32_t* __restrict__ Gi,
                                               int64_t ld) {
-    // One LDS block: two buffers of A and B panels in the main loop (80 KB);
+    constexpr bool HAS_E = MODE == 0;
+    // One LDS block: three buffers of A and B panels in the main loop (120 KB);
     // four 32x33 FP64 slabs plus four 32x33 int32 slabs (50.7 KB) in the
     // epilogue.
     constexpr int PANELS_WORDS = 2 * TILE * LDS_PITCH;  // A and B of one buffer
@@ -761,18 +763,26 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
     // (E(i,j) + E(j,i)) — so diagonal tiles come out bitwise symmetric.
     // 32x32 accumulator layout: register q of lane l holds
     // row 8(q/4) + 4(l/32) + q%4, column l%32.
-    // Each pass issues all 16 of a lane's loads before using any, so a
-    // block costs a few memory round trips, not one per row; the slabs are
-    // wave-private, so a wave orders its own LDS traffic and never waits
-    // for the other waves.
+    // The E values a block needs (16 per lane in each pattern) are loaded
+    // one block ahead, so their latency hides under the previous block's
+    // stores; u of the tile's rows and columns is staged in LDS once. The
+    // slabs are wave-private, so a wave orders its own LDS traffic and
+    // never waits for the other waves.
     __syncthreads();
     double* slabD = reinterpret_cast<double*>(lds) + wave * (32 * 33);
     int32_t* slabX = reinterpret_cast<int32_t*>(reinterpret_cast<double*>(lds) + 4 * 32 * 33) + wave * (32 * 33);
+    double* uI = reinterpret_cast<double*>(lds) + 4 * 32 * 33 + 2 * 32 * 33;  // after the int32 slabs
+    double* uJ = uI + TILE;
+    static_assert((4 * 32 * 33 + 2 * 32 * 33 + 2 * TILE) * 8 <= LDS_WORDS * 4, "epilogue LDS");
     const bool diag = ti == tj;
     const double Sdiv = S > 0.0 ? S : 1.0;
-#pragma unroll 1
-    for (int ab = 0; ab < 16; ++ab) {
-        const int a = ab >> 2, b = ab & 3;
+    if (MODE != 1) {
+        const int64_t ri = ti * TILE + tid, rj = tj * TILE + tid;
+        uI[tid] = ri < n ? u[ri] : 0.0;
+        uJ[tid] = rj < n ? u[rj] : 0.0;
+        __syncthreads();
+    }
+    auto block_to_slab = [&](int ab) {
         // registers cannot be indexed at run time: block ab's accumulator is
         // picked by a (wave-uniform) branch, the rest of the body stays rolled
 #pragma unroll
@@ -783,9 +793,13 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
                     slabX[(8 * (q >> 2) + 4 * h + (q & 3)) * 33 + r] = acc[c >> 2][c & 3][q];
             }
         wave_sync();
-        const int64_t I0 = ti * TILE + wr * 128 + a * 32;  // block rows
-        const int64_t J0 = tj * TILE + wc * 128 + b * 32;  // block columns
-        if (MODE == 1) {
+    };
+    if (MODE == 1) {
+#pragma unroll 1
+        for (int ab = 0; ab < 16; ++ab) {
+            block_to_slab(ab);
+            const int64_t I0 = ti * TILE + wr * 128 + (ab >> 2) * 32;  // block rows
+            const int64_t J0 = tj * TILE + wc * 128 + (ab & 3) * 32;   // block columns
 #pragma unroll
             for (int rr = 0; rr < 16; ++rr) {
                 const int row = h * 16 + rr;
@@ -796,57 +810,80 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
                     if (mi < n && mj < n) Gi[mi * ld + mj] = slabX[r * 33 + row];
                 }
             }
-        } else {
-            if (E) {  // pass 1 (row pattern): stage E(i,j)
-                double ev[16];
-#pragma unroll
-                for (int rr = 0; rr < 16; ++rr) {
-                    const int64_t i = I0 + h * 16 + rr, j = J0 + r;
-                    ev[rr] = (i < n && j < n) ? E[i * lde + j] : 0.0;
-                }
-#pragma unroll
-                for (int rr = 0; rr < 16; ++rr) slabD[(h * 16 + rr) * 33 + r] = ev[rr];
-                wave_sync();
-            }
-            // pass 2 (transposed pattern): form g, write the mirror
-            const int64_t i = I0 + r;
-            const double ui = i < n ? u[i] : 0.0;
-            double et[16], uj[16];
-#pragma unroll
-            for (int rr = 0; rr < 16; ++rr) {
-                const int64_t j = J0 + h * 16 + rr;
-                et[rr] = (E && i < n && j < n) ? E[j * lde + i] : 0.0;
-                uj[rr] = j < n ? u[j] : 0.0;
-            }
-#pragma unroll
-            for (int rr = 0; rr < 16; ++rr) {
-                const int col = h * 16 + rr;
-                const int64_t j = J0 + col;
-                if (i < n && j < n) {
-                    const double e = E ? slabD[r * 33 + col] + et[rr] : 0.0;
-                    const double v = (((double)slabX[r * 33 + col] - (ui + uj[rr])) + Cc) + e;
-                    const double g = v / Sdiv;
-                    slabD[r * 33 + col] = g;
-                    if (!diag) {
-                        if (G64) G64[j * ld + i] = g;
-                        if (G32) G32[j * ld + i] = (float)g;
-                    }
-                }
-            }
             wave_sync();
-            // pass 3 (row pattern): own positions
+        }
+        return;
+    }
+    // E(i,j) for rows I0 + 16h + rr, column J0 + r (row pattern) and
+    // E(j,i) for rows J0 + 16h + rr, column I0 + r (transposed pattern)
+    // Indices are clamped into the matrix instead of predicating the loads
+    // (values outside it are never used), which keeps the loads unbranched
+    // so that waits cover only the block they belong to.
+    auto load_e = [&](int ab, double (&ev)[16], double (&et)[16]) {
+        const int64_t I0 = ti * TILE + wr * 128 + (ab >> 2) * 32;
+        const int64_t J0 = tj * TILE + wc * 128 + (ab & 3) * 32;
+        const int64_t jr = J0 + r < n ? J0 + r : n - 1, ir = I0 + r < n ? I0 + r : n - 1;
 #pragma unroll
-            for (int rr = 0; rr < 16; ++rr) {
-                const int row = h * 16 + rr;
-                const int64_t ii = I0 + row, j = J0 + r;
-                if (ii < n && j < n) {
-                    const double g = slabD[row * 33 + r];
-                    if (G64) G64[ii * ld + j] = g;
-                    if (G32) G32[ii * ld + j] = (float)g;
+        for (int rr = 0; rr < 16; ++rr) {
+            const int64_t i = I0 + h * 16 + rr < n ? I0 + h * 16 + rr : n - 1;
+            ev[rr] = E[i * lde + jr];
+        }
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) {
+            const int64_t j = J0 + h * 16 + rr < n ? J0 + h * 16 + rr : n - 1;
+            et[rr] = E[j * lde + ir];
+        }
+    };
+    auto process = [&](int ab, const double (&ev)[16], const double (&et)[16]) {
+        const int a = ab >> 2, b = ab & 3;
+        block_to_slab(ab);
+        const int64_t I0 = ti * TILE + wr * 128 + a * 32;  // block rows
+        const int64_t J0 = tj * TILE + wc * 128 + b * 32;  // block columns
+        if (HAS_E) {  // pass 1 (row pattern): stage E(i,j)
+#pragma unroll
+            for (int rr = 0; rr < 16; ++rr) slabD[(h * 16 + rr) * 33 + r] = ev[rr];
+            wave_sync();
+        }
+        // pass 2 (transposed pattern): form g, write the mirror
+        const int64_t i = I0 + r;
+        const double ui = uI[wr * 128 + a * 32 + r];
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) {
+            const int col = h * 16 + rr;
+            const int64_t j = J0 + col;
+            if (i < n && j < n) {
+                const double e = HAS_E ? slabD[r * 33 + col] + et[rr] : 0.0;
+                const double v = (((double)slabX[r * 33 + col] - (ui + uJ[wc * 128 + b * 32 + col])) + Cc) + e;
+                const double g = v / Sdiv;
+                slabD[r * 33 + col] = g;
+                if (!diag) {
+                    if (G64) G64[j * ld + i] = g;
+                    if (G32) G32[j * ld + i] = (float)g;
                 }
             }
         }
         wave_sync();
+        // pass 3 (row pattern): own positions
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) {
+            const int row = h * 16 + rr;
+            const int64_t ii = I0 + row, j = J0 + r;
+            if (ii < n && j < n) {
+                const double g = slabD[row * 33 + r];
+                if (G64) G64[ii * ld + j] = g;
+                if (G32) G32[ii * ld + j] = (float)g;
+            }
+        }
+        wave_sync();
+    };
+    double evA[16], etA[16], evB[16], etB[16];
+    if (HAS_E) load_e(0, evA, etA);
+#pragma unroll 1
+    for (int ab = 0; ab < 16; ab += 2) {
+        if (HAS_E) load_e(ab + 1, evB, etB);
+        process(ab, evA, etA);
+        if (HAS_E && ab + 2 < 16) load_e(ab + 2, evA, etA);
+        process(ab + 1, evB, etB);
     }
 }
 
@@ -1025,8 +1062,12 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
     if (outputs & CB_GRM_OUT_INT) grm->Gi.alloc((size_t)n * n); else grm->Gi.release();
     if (outputs & (CB_GRM_OUT_F64 | CB_GRM_OUT_F32)) {
         timed(ctx, "k_gram", [&] {
-            k_gram<0><<<dim3((unsigned)ntiles), dim3(256), 0, st>>>(
-                X2.p, n, n_pad, n_stages, u.p, Cc, total, E.p, n, grm->G64.p, grm->G32.p, nullptr, n);
+            if (E.p)
+                k_gram<0><<<dim3((unsigned)ntiles), dim3(256), 0, st>>>(
+                    X2.p, n, n_pad, n_stages, u.p, Cc, total, E.p, n, grm->G64.p, grm->G32.p, nullptr, n);
+            else
+                k_gram<2><<<dim3((unsigned)ntiles), dim3(256), 0, st>>>(
+                    X2.p, n, n_pad, n_stages, u.p, Cc, total, nullptr, n, grm->G64.p, grm->G32.p, nullptr, n);
         });
     }
     if (outputs & CB_GRM_OUT_INT) {
