@@ -233,19 +233,43 @@ __global__ __launch_bounds__(256) void k_miss_sample_lists(
     const int64_t* __restrict__ snp_off, const int32_t* __restrict__ snp_list, int64_t m,
     const uint8_t* __restrict__ valid, const uint32_t* __restrict__ kfix,
     const int64_t* __restrict__ smp_off, int32_t* __restrict__ cursor,
-    int32_t* __restrict__ lj, uint32_t* __restrict__ lk,
-    unsigned long long* __restrict__ sq_sum) {
+    int32_t* __restrict__ lj, uint32_t* __restrict__ lk) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t j = (int64_t)blockIdx.x * 4 + wave;
     if (j >= m || !valid[j]) return;
     const uint32_t K = kfix[j];
-    const unsigned long long k2 = ((unsigned long long)K * K) >> 6;
     for (int64_t p = snp_off[j] + lane; p < snp_off[j + 1]; p += 64) {
         const int32_t i = snp_list[p];
         const int64_t pos = smp_off[i] + atomicAdd(&cursor[i], 1);
         lj[pos] = (int32_t)j;
         lk[pos] = K;
-        atomicAdd(&sq_sum[i], k2);
+    }
+}
+
+// s_i = sum over sample i's list of K^2 >> 6 (mu^2 in 2^-52 units), one
+// wave per sample. A long list overflows 64 bits, so the low and high words
+// of the terms are summed apart (exact for lists below 2^31 entries).
+__global__ __launch_bounds__(256) void k_sample_sq(const int64_t* __restrict__ smp_off,
+                                                   const uint32_t* __restrict__ lk, int64_t n,
+                                                   unsigned long long* __restrict__ sq_lo,
+                                                   unsigned long long* __restrict__ sq_hi) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + wave;
+    if (i >= n) return;
+    unsigned long long lo = 0, hi = 0;
+    for (int64_t p = smp_off[i] + lane; p < smp_off[i + 1]; p += 64) {
+        const unsigned long long K = lk[p];
+        const unsigned long long k2 = (K * K) >> 6;
+        lo += k2 & 0xffffffffull;
+        hi += k2 >> 32;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo += __shfl_down(lo, off);
+        hi += __shfl_down(hi, off);
+    }
+    if (lane == 0) {
+        sq_lo[i] = lo;
+        sq_hi[i] = hi;
     }
 }
 
@@ -283,6 +307,22 @@ __global__ void k_encode_d3(const uint8_t* __restrict__ bed, uint8_t* __restrict
     reinterpret_cast<uint32_t*>(out + j * pitch)[w] = d | miss | (miss << 1);
 }
 
+// Both-missing event sums are exact integers in 2^-48 units. Each term is
+// split into its low 32 bits and its signed rest, summed in two u64 slots,
+// so that no number of events overflows; ev_value() scales the total. While
+// the total fits in 63 bits (all but extreme cases) it is converted once,
+// exactly as a single 64-bit sum would be.
+__device__ __forceinline__ void ev_add(unsigned long long* lo, unsigned long long* hi, unsigned long long add) {
+    atomicAdd(lo, add & 0xffffffffull);
+    atomicAdd(hi, (unsigned long long)((long long)add >> 32));
+}
+__device__ __forceinline__ double ev_value(unsigned long long lo, unsigned long long hi) {
+    const long long h = (long long)hi;
+    if (h > -(1ll << 30) && h < (1ll << 30))
+        return (double)(long long)(((unsigned long long)h << 32) + lo) * (1.0 / 281474976710656.0);
+    return ((double)h * 4294967296.0 + (double)lo) * (1.0 / 281474976710656.0);
+}
+
 // E for the 1024 samples i of a block and the rows k of this wave, with
 // E(k,i) + E(i,k) = e(k,i) + e(i,k) + F(k,i) as the Gram epilogue needs,
 // e(k,i) = sum_{j miss k} mu_j X_ij and F(k,i) = sum_{j miss k and i} mu_j^2.
@@ -303,15 +343,15 @@ __global__ __launch_bounds__(256) void k_miss_cross(const uint8_t* __restrict__ 
                                                     const int64_t* __restrict__ smp_off,
                                                     const int32_t* __restrict__ lj,
                                                     const uint32_t* __restrict__ lk,
-                                                    const unsigned long long* __restrict__ sq, int kw,
+                                                    const double* __restrict__ sqd, int kw,
                                                     double* __restrict__ E, int64_t lde) {
-    __shared__ unsigned long long fl[4 * 16 * 64];  // [wave][sample][lane], 32 KB
+    __shared__ unsigned long long fl[4 * 2 * 16 * 64];  // [wave][lo, hi][sample][lane], 64 KB
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int64_t I0 = (int64_t)blockIdx.y * 1024;
     const int64_t dw = I0 / 16 + lane;  // dword of each SNP row owned by this lane
     const int64_t i_lane = I0 + 16 * lane;
     const int64_t dwc = dw * 4 < pitch ? dw : 0;  // clamped in-row dword
-    unsigned long long* myf = fl + wave * 16 * 64 + lane;
+    unsigned long long* myf = fl + wave * 2 * 16 * 64 + lane;  // lo slots; hi slots at + 16 * 64
     const int per_wave = kw / 4;
     for (int kq = 0; kq < per_wave; ++kq) {
         const int64_t k = (int64_t)blockIdx.x * kw + wave * per_wave + kq;
@@ -327,7 +367,7 @@ __global__ __launch_bounds__(256) void k_miss_cross(const uint8_t* __restrict__ 
             mk[q] = mq;
         }
 #pragma unroll
-        for (int s = 0; s < 16; ++s) myf[s * 64] = 0ull;
+        for (int s = 0; s < 32; ++s) myf[s * 64] = 0ull;
         uint32_t acc[16][4];
 #pragma unroll
         for (int s = 0; s < 16; ++s)
@@ -385,16 +425,37 @@ __global__ __launch_bounds__(256) void k_miss_cross(const uint8_t* __restrict__ 
                     if (lo) ev0 &= ev0 - 1; else ev1 &= ev1 - 1;
                     const int t = bit >> 3, smp = (lo ? 0 : 8) + 4 * (bit & 1) + ((bit >> 1) & 3);
                     const uint32_t Kt = t == 0 ? K0 : t == 1 ? K1 : t == 2 ? K2 : K3;
-                    // mu^2 - 6 mu in 2^-48 units (two's complement in the u64 slot)
+                    // mu^2 - 6 mu in 2^-48 units (two's complement, split by ev_add)
                     const unsigned long long add =
                         (((unsigned long long)Kt * Kt) >> 10) - ((unsigned long long)Kt * 6 << 19);
-                    atomicAdd(&myf[smp * 64], add);
+                    ev_add(&myf[smp * 64], &myf[(16 + smp) * 64], add);
                 }
             }
         };
+        // accumulator folds as in k_miss_cross_h, every 4096 16-entry iterations
+        bool folded = false;
+        auto fold = [&]() {
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const uint64_t t = (uint64_t)acc[s][0] + ((uint64_t)acc[s][1] << 8) +
+                                   ((uint64_t)acc[s][2] << 16) + ((uint64_t)acc[s][3] << 24);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[s][c] = 0;
+                if (i_lane + s < n) {
+                    uint64_t* slot = reinterpret_cast<uint64_t*>(E + k * lde + i_lane + s);
+                    *slot = (folded ? *slot : 0) + t;
+                }
+            }
+            folded = true;
+        };
+        int it = 0;
         uint32_t ra[8], rb[8];
         if (b < e) fetch(b, ra);
         for (int64_t p = b; p < e; p += 16) {
+            if (++it == 4096) {
+                fold();
+                it = 0;
+            }
             const bool m1 = p + 8 < e;
             if (m1) fetch(p + 8, rb);
             step(p, ra);
@@ -408,11 +469,12 @@ __global__ __launch_bounds__(256) void k_miss_cross(const uint8_t* __restrict__ 
             if (i >= n) continue;
             double v;
             if (i == k) {
-                v = (double)sq[k] * (0.5 / 4503599627370496.0);  // s_k / 2
+                v = sqd[k] * 0.5;  // s_k / 2
             } else {
-                const uint64_t T = ((uint64_t)acc[s][0] + ((uint64_t)acc[s][1] << 8) +
+                const uint64_t part = folded ? *reinterpret_cast<const uint64_t*>(E + k * lde + i) : 0;
+                const uint64_t T = (part + (uint64_t)acc[s][0] + ((uint64_t)acc[s][1] << 8) +
                                     ((uint64_t)acc[s][2] << 16) + ((uint64_t)acc[s][3] << 24)) >> (2 * (s & 3));
-                v = (double)T * TWO_M29 + (double)(long long)myf[s * 64] * (1.0 / 281474976710656.0);
+                v = (double)T * TWO_M29 + ev_value(myf[s * 64], myf[(16 + s) * 64]);
             }
             E[k * lde + i] = v;
         }
@@ -433,6 +495,7 @@ __global__ __launch_bounds__(256) void k_miss_cross(const uint8_t* __restrict__ 
 // of the dot loop to cover their latency (the rows are random SNPs).
 constexpr int MC_I = 512;    // samples per block column
 constexpr int MC_RING = 256; // event records per wave (power of two)
+constexpr int MC_FOLD_ITERS = 2048;  // 24-entry iterations per accumulator fold (< 2^32 / (24 * 48960))
 // Orders one wave's LDS accesses across its lanes (hardware keeps a wave's
 // LDS operations in order; this keeps the compiler from moving them).
 __device__ __forceinline__ void wave_sync() {
@@ -440,14 +503,14 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-__global__ __launch_bounds__(256) void k_miss_cross_h(const uint8_t* __restrict__ bedd, int64_t pitch, int64_t n,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_miss_cross_h(const uint8_t* __restrict__ bedd, int64_t pitch, int64_t n,
                                                       int64_t bed_bytes,
                                                       const int64_t* __restrict__ smp_off,
                                                       const uint32_t* __restrict__ lg,
                                                       const uint32_t* __restrict__ lk,
-                                                      const unsigned long long* __restrict__ sq, int kw,
+                                                      const double* __restrict__ sqd, int kw,
                                                       double* __restrict__ E, int64_t lde) {
-    __shared__ unsigned long long fl[4 * 8 * 64];  // [wave][sample][lane], 16 KB
+    __shared__ unsigned long long fl[4 * 2 * 8 * 64];  // [wave][lo, hi][sample][lane], 32 KB
     __shared__ uint2 rings[4 * MC_RING];           // [wave][record], 8 KB
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int h = lane & 1;
@@ -460,7 +523,7 @@ __global__ __launch_bounds__(256) void k_miss_cross_h(const uint8_t* __restrict_
     // 0x00020000: the CDNA buffer-descriptor word 3 for untyped dword access
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)bedd, (short)0, (int)bed_bytes,
                                                                        0x00020000);
-    unsigned long long* wf = fl + wave * 8 * 64;
+    unsigned long long* wf = fl + wave * 2 * 8 * 64;  // lo slots; hi slots at + 8 * 64
     unsigned long long* myf = wf + lane;
     uint2* ring = rings + wave * MC_RING;
     // byte selectors of the first transpose stage for this half
@@ -479,7 +542,7 @@ __global__ __launch_bounds__(256) void k_miss_cross_h(const uint8_t* __restrict_
             mk[q] = mq;
         }
 #pragma unroll
-        for (int s2 = 0; s2 < 8; ++s2) myf[s2 * 64] = 0ull;
+        for (int s2 = 0; s2 < 16; ++s2) myf[s2 * 64] = 0ull;
         wave_sync();
         uint32_t acc[8][4];
 #pragma unroll
@@ -518,7 +581,8 @@ __global__ __launch_bounds__(256) void k_miss_cross_h(const uint8_t* __restrict_
                 bits &= bits - 1;
                 const int t = bit >> 3, smp = 4 * (bit & 1) + ((bit >> 1) & 3);
                 const unsigned long long Kt = lk[b + 4 * (int64_t)(meta >> 6) + t];
-                atomicAdd(&wf[smp * 64 + (meta & 63)], ((Kt * Kt) >> 10) - (Kt * 6 << 19));
+                ev_add(&wf[smp * 64 + (meta & 63)], &wf[(8 + smp) * 64 + (meta & 63)],
+                       ((Kt * Kt) >> 10) - (Kt * 6 << 19));
             }
             push(bits, meta);
         };
@@ -570,7 +634,30 @@ __global__ __launch_bounds__(256) void k_miss_cross_h(const uint8_t* __restrict_
             fetch(jw, r1);
         }
         if (b + 16 < e) jwords(b + 16, jw);
+        // A long list would overflow the 32-bit accumulators (one entry adds
+        // at most 3 * 64 * 255 to each): every MC_FOLD_ITERS iterations the
+        // partial sums move, exactly, into this lane's own E slots as u64.
+        bool folded = false;
+        auto fold = [&]() {
+#pragma unroll
+            for (int s2 = 0; s2 < 8; ++s2) {
+                const uint64_t t = (uint64_t)acc[s2][0] + ((uint64_t)acc[s2][1] << 8) +
+                                   ((uint64_t)acc[s2][2] << 16) + ((uint64_t)acc[s2][3] << 24);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[s2][c] = 0;
+                if (i_lane + s2 < n) {
+                    uint64_t* slot = reinterpret_cast<uint64_t*>(E + k * lde + i_lane + s2);
+                    *slot = (folded ? *slot : 0) + t;
+                }
+            }
+            folded = true;
+        };
+        int it = 0;
         for (int64_t p = b; p < e; p += 24) {
+            if (++it == MC_FOLD_ITERS) {
+                fold();
+                it = 0;
+            }
             if (p + 16 < e) fetch(jw, r2);
             if (p + 24 < e) jwords(p + 24, jw);
             step(p, r0);
@@ -594,11 +681,12 @@ __global__ __launch_bounds__(256) void k_miss_cross_h(const uint8_t* __restrict_
             if (i >= n) continue;
             double v;
             if (i == k) {
-                v = (double)sq[k] * (0.5 / 4503599627370496.0);  // s_k / 2
+                v = sqd[k] * 0.5;  // s_k / 2
             } else {
-                const uint64_t T = ((uint64_t)acc[s2][0] + ((uint64_t)acc[s2][1] << 8) +
+                const uint64_t part = folded ? *reinterpret_cast<const uint64_t*>(E + k * lde + i) : 0;
+                const uint64_t T = (part + (uint64_t)acc[s2][0] + ((uint64_t)acc[s2][1] << 8) +
                                     ((uint64_t)acc[s2][2] << 16) + ((uint64_t)acc[s2][3] << 24)) >> (2 * (s2 & 3));
-                v = (double)T * TWO_M29 + (double)(long long)myf[s2 * 64] * (1.0 / 281474976710656.0);
+                v = (double)T * TWO_M29 + ev_value(myf[s2 * 64], myf[(8 + s2) * 64]);
             }
             E[k * lde + i] = v;
         }
@@ -1002,27 +1090,38 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
         const int64_t nlist = std::max<int64_t>(8, smp_off[n]);
         DevBuf<int32_t> lj, cursor;
         DevBuf<uint32_t> lk;
-        DevBuf<unsigned long long> sq;
+        DevBuf<unsigned long long> sq;  // per sample: low words, then high words
         lj.alloc(nlist);
         lk.alloc(nlist);
         cursor.alloc(n);
-        sq.alloc(n);
+        sq.alloc(2 * n);
         CB_HIP(hipMemsetAsync(lj.p, 0, lj.bytes(), st));
         CB_HIP(hipMemsetAsync(lk.p, 0, lk.bytes(), st));
         CB_HIP(hipMemsetAsync(cursor.p, 0, cursor.bytes(), st));
-        CB_HIP(hipMemsetAsync(sq.p, 0, sq.bytes(), st));
         timed(ctx, "k_miss_sample_lists", [&] {
             k_miss_sample_lists<<<dim3((unsigned)ceil_div(m, 4)), dim3(256), 0, st>>>(
                 d_snp_off.p, snp_list.p, m, grm->valid.p, grm->kfix.p, d_smp_off.p, cursor.p, lj.p,
-                lk.p, sq.p);
+                lk.p);
+        });
+        timed(ctx, "k_sample_sq", [&] {
+            k_sample_sq<<<dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st>>>(d_smp_off.p, lk.p, n, sq.p, sq.p + n);
         });
         // u_i += sum_{j miss i} mu_j^2
-        std::vector<unsigned long long> hsq(n);
+        std::vector<unsigned long long> hsq(2 * n);
         std::vector<double> hu(n);
-        CB_HIP(hipMemcpyAsync(hsq.data(), sq.p, n * 8, hipMemcpyDeviceToHost, st));
+        CB_HIP(hipMemcpyAsync(hsq.data(), sq.p, 2 * n * 8, hipMemcpyDeviceToHost, st));
         CB_HIP(hipMemcpyAsync(hu.data(), u.p, n * 8, hipMemcpyDeviceToHost, st));
         CB_HIP(hipStreamSynchronize(st));
-        for (int64_t i = 0; i < n; ++i) hu[i] += (double)hsq[i] * (1.0 / 4503599627370496.0);
+        // s_i = sum_{j miss i} mu_j^2, an exact integer in 2^-52 units, converted once
+        std::vector<double> hsqd(n);
+        for (int64_t i = 0; i < n; ++i) {
+            const unsigned __int128 s_i = ((unsigned __int128)hsq[n + i] << 32) + hsq[i];
+            hsqd[i] = (double)s_i * (1.0 / 4503599627370496.0);
+            hu[i] += hsqd[i];
+        }
+        DevBuf<double> sqd;
+        sqd.alloc(n);
+        CB_HIP(hipMemcpyAsync(sqd.p, hsqd.data(), n * 8, hipMemcpyHostToDevice, st));
         CB_HIP(hipMemcpyAsync(u.p, hu.data(), n * 8, hipMemcpyHostToDevice, st));
 
         E.alloc((size_t)n * n);
@@ -1037,7 +1136,7 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
             const int kw = 64;
             timed(ctx, "k_miss_cross", [&] {
                 k_miss_cross<<<dim3((unsigned)ceil_div(n, kw), (unsigned)ceil_div(n, 1024)), dim3(256), 0, st>>>(
-                    bedd.p, g->pitch, n, d_smp_off.p, lj.p, lk.p, sq.p, kw, E.p, n);
+                    bedd.p, g->pitch, n, d_smp_off.p, lj.p, lk.p, sqd.p, kw, E.p, n);
             });
         } else {
             const int kw = 64;
@@ -1046,7 +1145,7 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
             k_miss_groups<<<dim3((unsigned)ceil_div(nlist / 4, 256)), dim3(256), 0, st>>>(lj.p, lk.p, nlist / 4, lg.p);
             timed(ctx, "k_miss_cross", [&] {
                 k_miss_cross_h<<<dim3((unsigned)ceil_div(n, kw), (unsigned)ceil_div(n, MC_I)), dim3(256), 0, st>>>(
-                    bedd.p, g->pitch, n, g->pitch * m, d_smp_off.p, lg.p, lk.p, sq.p, kw, E.p, n);
+                    bedd.p, g->pitch, n, g->pitch * m, d_smp_off.p, lg.p, lk.p, sqd.p, kw, E.p, n);
             });
             CB_HIP(hipStreamSynchronize(st));
         }
