@@ -294,17 +294,23 @@ __global__ void k_miss_groups(const int32_t* __restrict__ lj, const uint32_t* __
 }
 
 // Rows of the BED re-encoded for k_miss_cross: each 2-bit field holds the
-// dosage (0, 1, 2), or 3 for a missing call.
+// dosage (0, 1, 2), or 3 for a missing call. Output rows are padded to
+// out_pitch (a multiple of 128 bytes, zero-filled) so that a wave's 128-byte
+// slice of a row is one cache line.
 __global__ void k_encode_d3(const uint8_t* __restrict__ bed, uint8_t* __restrict__ out, int64_t pitch,
-                            int64_t n, int64_t m) {
+                            int64_t out_pitch, int64_t n, int64_t m) {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // dword within the row
     const int64_t j = blockIdx.y;
-    if (w * 4 >= pitch || j >= m) return;
-    const uint32_t x = reinterpret_cast<const uint32_t*>(bed + j * pitch)[w];
-    const uint32_t vm = field_mask(n, w);
-    const uint32_t miss = ~(x >> 1) & x & vm;
-    const uint32_t d = dosage_swar(x) & (vm | (vm << 1));
-    reinterpret_cast<uint32_t*>(out + j * pitch)[w] = d | miss | (miss << 1);
+    if (w * 4 >= out_pitch || j >= m) return;
+    uint32_t v = 0;
+    if (w * 4 < pitch) {
+        const uint32_t x = reinterpret_cast<const uint32_t*>(bed + j * pitch)[w];
+        const uint32_t vm = field_mask(n, w);
+        const uint32_t miss = ~(x >> 1) & x & vm;
+        const uint32_t d = dosage_swar(x) & (vm | (vm << 1));
+        v = d | miss | (miss << 1);
+    }
+    reinterpret_cast<uint32_t*>(out + j * out_pitch)[w] = v;
 }
 
 // Both-missing event sums are exact integers in 2^-48 units. Each term is
@@ -1126,17 +1132,18 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
 
         E.alloc((size_t)n * n);
         DevBuf<uint8_t> bedd;
-        bedd.alloc((size_t)g->pitch * m);
+        const int64_t dpitch = round_up(g->pitch, 128);
+        bedd.alloc((size_t)dpitch * m);
         timed(ctx, "k_encode_d3", [&] {
-            k_encode_d3<<<dim3((unsigned)ceil_div(g->pitch / 4, 256), (unsigned)m), 256, 0, st>>>(
-                g->bed.p, bedd.p, g->pitch, n, m);
+            k_encode_d3<<<dim3((unsigned)ceil_div(dpitch / 4, 256), (unsigned)m), 256, 0, st>>>(
+                g->bed.p, bedd.p, g->pitch, dpitch, n, m);
         });
         // the regrouped kernel: buffer offsets below 2^31, ring records carry 26-bit group indices
-        if (getenv("CB_MISS_CROSS_V5") || g->pitch * m >= ((int64_t)1 << 31) || m >= ((int64_t)1 << 26)) {
+        if (getenv("CB_MISS_CROSS_V5") || dpitch * m >= ((int64_t)1 << 31) || m >= ((int64_t)1 << 26)) {
             const int kw = 64;
             timed(ctx, "k_miss_cross", [&] {
                 k_miss_cross<<<dim3((unsigned)ceil_div(n, kw), (unsigned)ceil_div(n, 1024)), dim3(256), 0, st>>>(
-                    bedd.p, g->pitch, n, d_smp_off.p, lj.p, lk.p, sqd.p, kw, E.p, n);
+                    bedd.p, dpitch, n, d_smp_off.p, lj.p, lk.p, sqd.p, kw, E.p, n);
             });
         } else {
             const int kw = 64;
@@ -1145,7 +1152,7 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
             k_miss_groups<<<dim3((unsigned)ceil_div(nlist / 4, 256)), dim3(256), 0, st>>>(lj.p, lk.p, nlist / 4, lg.p);
             timed(ctx, "k_miss_cross", [&] {
                 k_miss_cross_h<<<dim3((unsigned)ceil_div(n, kw), (unsigned)ceil_div(n, MC_I)), dim3(256), 0, st>>>(
-                    bedd.p, g->pitch, n, g->pitch * m, d_smp_off.p, lg.p, lk.p, sqd.p, kw, E.p, n);
+                    bedd.p, dpitch, n, dpitch * m, d_smp_off.p, lg.p, lk.p, sqd.p, kw, E.p, n);
             });
             CB_HIP(hipStreamSynchronize(st));
         }
