@@ -26,8 +26,8 @@ sys.path.insert(0, str(ROOT))
 PEAK_INT8_TOPS = 5000.0  # MI355X dense INT8 MFMA, TOP/s
 # HBM bytes per k_gram launch at n = m = 50,000 with 1% missing, from two
 # rocprofv3 --pmc runs of this script's GRM leg (FETCH_SIZE x 2.0 + WRITE_SIZE
-# x 1.0, the factors a 1 GiB device copy gives): 86.0 GB read + 30.3 GB written
-K_GRAM_TRAFFIC_BYTES = 1.1634e11
+# x 1.0, the factors a 1 GiB device copy gives): 86.5 GB read + 30.3 GB written
+K_GRAM_TRAFFIC_BYTES = 1.1684e11
 # headline metric: the GRM part is `value` (TOPS on the n = m = 50,000 GRM);
 # the REML part (GPU mixed vs GPU FP64 plain iterations/s) is under "reml"
 METRIC = "REML iters/sec + GRM TOPS, GPU mixed vs GPU FP64 at \u22641e-6 rel EBV error"
