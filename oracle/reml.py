@@ -33,7 +33,12 @@ pedigrees and the 4-animal fixture by hand; ml_inbreeding's F vs
 diag(tabular A) - 1 on inbred pedigrees and henderson_ainv_f vs the inverse
 of the tabular A there; dense_exact_eval's gradient vs central finite
 differences of its logL and its P vs the V-form projection
-V^-1 - V^-1 X (X'V^-1 X)^-1 X'V^-1; probes vs a scalar splitmix64.
+V^-1 - V^-1 X (X'V^-1 X)^-1 X'V^-1; mv_eigen_vce's fixed point as a
+stationary point of mv_reml_logl (central finite differences) and mv_solve
+vs the multi-trait V-form BLUP; hinv_blocks' tuneG (b G + a matches A22's
+mean diagonal and off-diagonal) and its A22 vs the genotyped block of the
+inverse of Henderson's A^-1 (tests/test_pedigree.py); probes vs a scalar
+splitmix64.
 """
 from __future__ import annotations
 
