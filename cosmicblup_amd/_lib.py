@@ -86,6 +86,7 @@ _SIGS = {
     "cb_context_stream": (c_vp, [c_vp]),
     "cb_context_synchronize": (C.c_int, [c_vp]),
     "cb_context_set_profiling": (C.c_int, [c_vp, C.c_int]),
+    "cb_context_set_graphs": (C.c_int, [c_vp, C.c_int]),
     "cb_context_reset_profile": (C.c_int, [c_vp]),
     "cb_context_kernel_time": (C.c_int, [c_vp, C.c_char_p, c_dp, c_i64p]),
     "cb_genotypes_from_bed": (C.c_int, [c_vp, c_vp, C.c_size_t, c_i64, c_i64, C.POINTER(c_vp)]),
@@ -211,6 +212,10 @@ class Context:
 
     def set_profiling(self, on: bool) -> None:
         check(lib().cb_context_set_profiling(self.handle, int(on)))
+
+    def set_graphs(self, on: bool) -> None:
+        """Replay PCG iterations as HIP graphs, or launch them one by one (default)."""
+        check(lib().cb_context_set_graphs(self.handle, int(on)))
 
     def reset_profile(self) -> None:
         check(lib().cb_context_reset_profile(self.handle))

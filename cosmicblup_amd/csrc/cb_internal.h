@@ -97,6 +97,7 @@ struct cb_context {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     bool profiling = false;
+    bool graphs = false;  // replay fixed launch sequences (PCG iterations) as HIP graphs
     std::vector<cbamd::TimedLaunch> pending;
     std::map<std::string, std::pair<double, long>> totals;  // name -> (ms, launches)
 
@@ -127,6 +128,51 @@ void timed(cb_context* ctx, const char* name, F&& launch) {
 }
 
 inline void set_device(cb_context* ctx) { CB_HIP(hipSetDevice(ctx->device)); }
+
+// A fixed sequence of launches on ctx->stream, captured once and replayed
+// with hipGraphLaunch. The sequence must not allocate, copy to or from the
+// host, or clear memory with hipMemsetAsync (kernels only); its buffers must
+// stay in place while the graph lives. Kernel timing is suspended during
+// the capture (a replay is not booked per kernel).
+struct LaunchGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    LaunchGraph() = default;
+    LaunchGraph(const LaunchGraph&) = delete;
+    LaunchGraph& operator=(const LaunchGraph&) = delete;
+    ~LaunchGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+    // graphs are used on the context's own or a caller's non-null stream
+    // that is not already being captured by the caller
+    static bool usable(cb_context* ctx) {
+        if (!ctx->graphs || ctx->stream == nullptr) return false;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(ctx->stream, &cs) != hipSuccess) return false;
+        return cs == hipStreamCaptureStatusNone;
+    }
+    template <class F>
+    void capture(cb_context* ctx, F&& body) {
+        hipStream_t st = ctx->stream;
+        const bool prof = ctx->profiling;
+        ctx->profiling = false;
+        CB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            ctx->profiling = prof;
+            throw;
+        }
+        ctx->profiling = prof;
+        CB_HIP(hipStreamEndCapture(st, &graph));
+        CB_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    void launch(cb_context* ctx) { CB_HIP(hipGraphLaunch(exec, ctx->stream)); }
+};
 
 inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }

@@ -1051,7 +1051,7 @@ int64_t pcg_block(cb_mme* M, double lambda, const T* B, TX* X, int r, double tol
     int64_t it = 0;
     const int check_every = 8;
     bool any_active = std::any_of(hs.begin(), hs.end(), [](const ColState& s) { return s.active; });
-    while (any_active && it < max_iter) {
+    auto iteration = [&] {
         apply_C<T>(M, lambda, P.p, Qp.p, r);
         col_dots<T>(M, P.p, Qp.p, r, part.p, d1.p);
         k_cg_alpha<<<blocks(r, 64), 64, 0, st>>>(cs.p, d1.p, r);
@@ -1062,7 +1062,25 @@ int64_t pcg_block(cb_mme* M, double lambda, const T* B, TX* X, int r, double tol
         k_cg_beta<<<blocks(r, 64), 64, 0, st>>>(cs.p, d1.p, d2.p, r, tol2, absolute ? 1 : 0);
         k_cg_p<T><<<g2, 256, 0, st>>>(P.p, Z.p, cs.p, dim, ld);
         CB_HIP(hipGetLastError());
-        ++it;
+    };
+    // The first check_every iterations run as plain launches (they size the
+    // operator's scratch); later blocks of check_every iterations replay one
+    // captured graph. Converged columns are frozen by their ColState, so a
+    // block runs the same launches whatever converged inside it.
+    LaunchGraph graph;
+    const bool use_graph = LaunchGraph::usable(ctx);
+    while (any_active && it < max_iter) {
+        if (use_graph && it >= check_every && it % check_every == 0 && it + check_every <= max_iter) {
+            if (!graph.exec)
+                graph.capture(ctx, [&] {
+                    for (int k = 0; k < check_every; ++k) iteration();
+                });
+            graph.launch(ctx);
+            it += check_every;
+        } else {
+            iteration();
+            ++it;
+        }
         if (it % check_every == 0 || it == max_iter) {
             CB_HIP(hipMemcpyAsync(hs.data(), cs.p, r * sizeof(ColState), hipMemcpyDeviceToHost, st));
             CB_HIP(hipStreamSynchronize(st));
@@ -1183,7 +1201,7 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
     int64_t it = 0;
     const int check_every = 4;
     int64_t replacements = 0;
-    while (!all_done && it < o.max_iter) {
+    auto iteration = [&] {
         apply_C<float>(M, lambda, P.p, Qp.p, r);
         col_dots<float>(M, P.p, Qp.p, r, part.p, d1.p);
         k_cg_alpha<<<blocks(r, 64), 64, 0, st>>>(cs.p, d1.p, r);
@@ -1195,7 +1213,24 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
         k_cg_beta<<<blocks(r, 64), 64, 0, st>>>(cs.p, d1.p, d2.p, r, tol2, absolute ? 1 : 0);
         k_cg_p<float><<<g2, 256, 0, st>>>(P.p, Z.p, cs.p, dim, ld);
         CB_HIP(hipGetLastError());
-        ++it;
+    };
+    // as in pcg_block: plain launches for the first block, then a captured
+    // block of check_every iterations replayed between the host checks (the
+    // replacements in between are ordinary launches and copies)
+    LaunchGraph graph;
+    const bool use_graph = LaunchGraph::usable(ctx);
+    while (!all_done && it < o.max_iter) {
+        if (use_graph && it >= check_every && it % check_every == 0 && it + check_every <= o.max_iter) {
+            if (!graph.exec)
+                graph.capture(ctx, [&] {
+                    for (int k = 0; k < check_every; ++k) iteration();
+                });
+            graph.launch(ctx);
+            it += check_every;
+        } else {
+            iteration();
+            ++it;
+        }
         if (it % check_every == 0 || it == o.max_iter) {
             CB_HIP(hipMemcpyAsync(hs.data(), cs.p, r * sizeof(ColState), hipMemcpyDeviceToHost, st));
             CB_HIP(hipStreamSynchronize(st));

@@ -94,6 +94,13 @@ int cb_context_set_profiling(cb_context* ctx, int enabled) {
     });
 }
 
+int cb_context_set_graphs(cb_context* ctx, int enabled) {
+    return guarded([&] {
+        CB_REQUIRE(ctx, "cb_context_set_graphs: ctx is NULL");
+        ctx->graphs = enabled != 0;
+    });
+}
+
 int cb_context_reset_profile(cb_context* ctx) {
     return guarded([&] {
         CB_REQUIRE(ctx, "cb_context_reset_profile: ctx is NULL");

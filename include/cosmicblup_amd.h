@@ -52,6 +52,13 @@ void* cb_context_stream(cb_context* ctx);
 int cb_context_synchronize(cb_context* ctx);
 /* Per-kernel timing with HIP events recorded on the launch stream. */
 int cb_context_set_profiling(cb_context* ctx, int enabled);
+/* Replay the fixed launch sequence of PCG iterations as HIP graphs (used
+ * only on a non-null stream that the caller is not capturing). Results are
+ * identical to individual launches. Default off: on MI355X the per-kernel
+ * cost of these iterations is device-side, so a replay saves nothing and
+ * the per-solve capture costs about 0.5 ms. No reference counterpart
+ * (launch mechanics). */
+int cb_context_set_graphs(cb_context* ctx, int enabled);
 int cb_context_reset_profile(cb_context* ctx);
 /* Total milliseconds and launch count booked under `kernel` since the last
  * reset (0 / 0 when the name is unknown). */

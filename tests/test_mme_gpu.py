@@ -270,3 +270,46 @@ def test_dense_exact_score_and_run_match_oracle(cb, ctx):
     np.testing.assert_allclose([R.var_u, R.var_e], [o["vu"], o["ve"]], rtol=1e-6)
     np.testing.assert_allclose([R.getSEVarsU()[0], R.getSEVarE()], o["se"], rtol=1e-5)
     assert ebv_err(R.solution[M.p:], o["sol"][M.p:]) < 1e-6
+
+
+@pytest.mark.parametrize("precision", [_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED])
+@pytest.mark.parametrize("qkind", ["dense", "csr"])
+def test_graph_replay_equals_plain_launches(cb, ctx, precision, qkind):
+    """PCG iteration blocks replayed as HIP graphs give the same iterates,
+    bit for bit, and the same iteration counts as one launch per kernel.
+    Each solve replays its graph many times, with host checks (and, mixed,
+    FP64 residual replacements) between replays; solves alternate between
+    the two modes so that every graph is built afresh."""
+    if qkind == "dense":
+        fd, aid, y, Q = gblup_problem(missing_records=17)
+        M = mme.MixedModel(fd, aid, y, Q.shape[0])
+        M.set_qinv_dense(Q)
+        B = np.zeros((M.dim, 5))
+        B[:, 0] = M.rhs()
+        B[M.p:, 1:] = orl.probes(M.q, 4, seed=3)
+    else:
+        from cosmicblup_amd.pedigree import invertA_henderson
+
+        dam, sire = synth.simulate_pedigree(6, 150, seed=12)
+        A = invertA_henderson(dam, sire)
+        rng = np.random.default_rng(5)
+        rec = np.sort(rng.choice(dam.size, 600, replace=False)).astype(np.int32)
+        y = rng.standard_normal(rec.size) + 5.0
+        fd = design.build_fixed_design([[f"g{int(v):02d}"] for v in rng.integers(0, 12, rec.size)])
+        M = mme.MixedModel(fd, rec, y, dam.size)
+        M.set_qinv_csr(A.ptr, A.col, A.val)
+        B = M.rhs()
+    opts = mme.solve_options(precision=precision, tol=1e-12 if precision == _lib.CB_PREC_FP64 else 1e-10)
+    runs = []
+    try:
+        for graphs in (False, True, False, True):
+            ctx.set_graphs(graphs)
+            runs.append(M.solve(1.7, B, opts))
+    finally:
+        ctx.set_graphs(False)
+    X0, it0, rel0 = runs[0]
+    assert it0 >= 24  # several graph replays
+    for X, it, rel in runs[1:]:
+        assert it == it0
+        assert rel == rel0
+        assert np.array_equal(X, X0)
