@@ -213,6 +213,90 @@ void pedigree_inbreeding(const std::vector<int32_t>& dam, const std::vector<int3
 
 }  // namespace cbamd
 
+namespace {
+
+cb_ainv* ainv_sorted(int64_t n, const std::vector<int32_t>& d, const std::vector<int32_t>& s) {
+    auto* A = new cb_ainv;
+    A->n = n;
+    try {
+        pedigree_inbreeding(d, s, A->F, A->dii);
+        assemble(A, d, s);
+    } catch (...) {
+        delete A;
+        throw;
+    }
+    return A;
+}
+
+void ainv_renumbered(int64_t n, const std::vector<int32_t>& d, const std::vector<int32_t>& s, cb_ainv** out) {
+    // Kahn's algorithm over parent -> offspring edges, smallest index first
+    std::vector<int32_t> indeg(n, 0);
+    std::vector<int64_t> cptr(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        CB_REQUIRE(d[i] != i && s[i] != i, "cb_ainv_henderson: an animal is its own parent");
+        if (d[i] >= 0) ++cptr[d[i] + 1];
+        if (s[i] >= 0) ++cptr[s[i] + 1];
+        indeg[i] = (d[i] >= 0) + (s[i] >= 0);
+    }
+    for (int64_t i = 0; i < n; ++i) cptr[i + 1] += cptr[i];
+    std::vector<int32_t> kids(cptr[n]);
+    {
+        std::vector<int64_t> fill(cptr.begin(), cptr.end() - 1);
+        for (int64_t i = 0; i < n; ++i) {
+            if (d[i] >= 0) kids[fill[d[i]]++] = (int32_t)i;
+            if (s[i] >= 0) kids[fill[s[i]]++] = (int32_t)i;
+        }
+    }
+    std::priority_queue<int32_t, std::vector<int32_t>, std::greater<int32_t>> ready;
+    for (int64_t i = 0; i < n; ++i)
+        if (indeg[i] == 0) ready.push((int32_t)i);
+    std::vector<int32_t> order;  // new -> old
+    order.reserve(n);
+    std::vector<int32_t> pos(n, -1);  // old -> new
+    while (!ready.empty()) {
+        const int32_t v = ready.top();
+        ready.pop();
+        pos[v] = (int32_t)order.size();
+        order.push_back(v);
+        for (int64_t k = cptr[v]; k < cptr[v + 1]; ++k)
+            if (--indeg[kids[k]] == 0) ready.push(kids[k]);
+    }
+    CB_REQUIRE((int64_t)order.size() == n, "cb_ainv_henderson: the pedigree has a cycle");
+    std::vector<int32_t> d2(n), s2(n);
+    for (int64_t k = 0; k < n; ++k) {
+        const int32_t o = order[k];
+        d2[k] = d[o] >= 0 ? pos[d[o]] : -1;
+        s2[k] = s[o] >= 0 ? pos[s[o]] : -1;
+    }
+    cb_ainv* B = ainv_sorted(n, d2, s2);
+    auto* A = new cb_ainv;
+    A->n = n;
+    A->F.resize(n);
+    A->dii.resize(n);
+    A->ptr.assign(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        A->F[i] = B->F[pos[i]];
+        A->dii[i] = B->dii[pos[i]];
+        A->ptr[i + 1] = A->ptr[i] + (B->ptr[pos[i] + 1] - B->ptr[pos[i]]);
+    }
+    A->col.resize(B->col.size());
+    A->val.resize(B->val.size());
+    std::vector<std::pair<int32_t, double>> row;
+    for (int64_t i = 0; i < n; ++i) {
+        row.clear();
+        for (int64_t k = B->ptr[pos[i]]; k < B->ptr[pos[i] + 1]; ++k) row.emplace_back(order[B->col[k]], B->val[k]);
+        std::sort(row.begin(), row.end());
+        for (size_t k = 0; k < row.size(); ++k) {
+            A->col[A->ptr[i] + k] = row[k].first;
+            A->val[A->ptr[i] + k] = row[k].second;
+        }
+    }
+    delete B;
+    *out = A;
+}
+
+}  // namespace
+
 extern "C" {
 
 int cb_ainv_henderson(int64_t n, const int32_t* dam, const int32_t* sire, cb_ainv** out) {
@@ -224,19 +308,20 @@ int cb_ainv_henderson(int64_t n, const int32_t* dam, const int32_t* sire, cb_ain
         for (int64_t i = 0; i < n; ++i) {  // complete_missing_parents_add_phantoms (a_matrix.cpp:43-52)
             if (!(d[i] >= 0 && d[i] < n)) d[i] = -1;
             if (!(s[i] >= 0 && s[i] < n)) s[i] = -1;
-            CB_REQUIRE(d[i] < i && s[i] < i,
-                       "cb_ainv_henderson: parents must precede their offspring (sort the pedigree)");
         }
-        auto* A = new cb_ainv;
-        A->n = n;
-        try {
-            pedigree_inbreeding(d, s, A->F, A->dii);
-            assemble(A, d, s);
-        } catch (...) {
-            delete A;
-            throw;
+        bool sorted = true;
+        for (int64_t i = 0; i < n && sorted; ++i) sorted = d[i] < i && s[i] < i;
+        if (!sorted) {
+            // ml()'s descending scan (a_matrix.cpp:148-175) is exact only when
+            // every parent precedes its offspring; the reference runs it on
+            // whatever order its parser produced. Here an unsorted pedigree is
+            // renumbered parents-first (Kahn's order, lowest index first, the
+            // identity on a sorted pedigree), solved, and mapped back.
+            *out = nullptr;
+            ainv_renumbered(n, d, s, out);
+            return;
         }
-        *out = A;
+        *out = ainv_sorted(n, d, s);
     });
 }
 

@@ -309,8 +309,14 @@ typedef struct cb_ainv cb_ainv;
  * d_ii from the parents' F, the reference's duplicate-summing order,
  * symmetrisation, diagonal floor and prune (a_matrix.cpp:58-80). Parents are
  * 0-based row indices; an index outside [0, n) is an unknown parent
- * (complete_missing_parents_add_phantoms, a_matrix.cpp:43-52). Parents must
- * precede their offspring (CB_ERR_INVALID otherwise). Host code. */
+ * (complete_missing_parents_add_phantoms, a_matrix.cpp:43-52). When every
+ * parent precedes its offspring the result is bit-identical to the
+ * reference's. Otherwise (e.g. the first-appearance numbering of
+ * relationship_builder.cpp:83-125 puts parents without a row of their own
+ * after their first offspring) the pedigree is renumbered parents-first,
+ * solved and mapped back: exact F where the reference's ml() scan, which
+ * starts at max(s, d), misses ancestors numbered above it. A cycle or a
+ * self-parent is CB_ERR_INVALID. Host code. */
 int cb_ainv_henderson(int64_t n, const int32_t* dam, const int32_t* sire, cb_ainv** out);
 int cb_ainv_nnz(const cb_ainv* a, int64_t* nnz);
 /* CSR rows (ptr: n+1, col/val: nnz), columns ascending within a row. */

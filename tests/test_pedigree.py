@@ -72,9 +72,12 @@ def test_native_fixture_and_out_of_range_parents(cb):
     np.testing.assert_array_equal(got.toDense(), expect)
 
 
-def test_native_rejects_unsorted_pedigree(cb):
-    with pytest.raises(CosmicError):
-        invertA_henderson(np.array([2, -1, -1]), np.array([-1, -1, -1]))
+def test_native_parent_listed_after_offspring(cb):
+    # animal 0's dam is animal 2: A = [[1, 0, .5], [0, 1, 0], [.5, 0, 1]]
+    got = invertA_henderson(np.array([2, -1, -1]), np.array([-1, -1, -1]))
+    A = np.array([[1.0, 0.0, 0.5], [0.0, 1.0, 0.0], [0.5, 0.0, 1.0]])
+    np.testing.assert_allclose(got.toDense() @ A, np.eye(3), atol=1e-15)
+    np.testing.assert_array_equal(got.dii, [0.75, 1.0, 1.0])
 
 
 def test_native_large_pedigree_properties(cb):
@@ -118,3 +121,100 @@ def test_oracle_tune_g_matches_a22_means():
     # A22 is the genotyped block of the inverse of Henderson's A^-1
     Ai, _, _ = orl.henderson_ainv_f(dam, sire)
     np.testing.assert_allclose(A22, np.linalg.inv(Ai)[np.ix_(gmap, gmap)], atol=1e-10)
+
+
+def test_read_pedigree_fixture():
+    """tests/golden/pblup_minimal/pedigree.txt, the reference's fixture:
+    header skipped, id dam sire by position, first-appearance numbering."""
+    from pathlib import Path
+
+    from cosmicblup_amd.pedigree import read_pedigree
+
+    ids, dam, sire = read_pedigree(str(Path(__file__).parent / "golden/pblup_minimal/pedigree.txt"))
+    assert ids == ["id1", "id2", "id3", "id4"]
+    np.testing.assert_array_equal(dam, [-1, -1, 0, 0])
+    np.testing.assert_array_equal(sire, [-1, -1, 1, 1])
+
+
+def test_read_pedigree_numbering_and_missing_tokens(tmp_path):
+    from cosmicblup_amd.pedigree import read_pedigree
+
+    p = tmp_path / "ped.txt"
+    # parents without rows are numbered after their first offspring; a
+    # header whose first token is ANIMAL, NA / . / 0 parents, blank lines
+    p.write_text("ANIMAL dam sire\n\nc1 d1 s1\nc2 d1 NA\n\nc3 . s2\nd1 0 0\nc4 c1 c2\n")
+    ids, dam, sire = read_pedigree(str(p))
+    assert ids == ["c1", "d1", "s1", "c2", "c3", "s2", "c4"]
+    np.testing.assert_array_equal(dam, [1, -1, -1, 1, -1, -1, 0])
+    np.testing.assert_array_equal(sire, [2, -1, -1, -1, 5, -1, 3])
+    with pytest.raises(RuntimeError, match="Cannot open pedigree file"):
+        read_pedigree(str(tmp_path / "absent.txt"))
+
+
+def _parents_first(dam, sire):
+    """A parents-first order (Kahn, smallest index first) and its inverse."""
+    import heapq
+
+    n = dam.size
+    kids = [[] for _ in range(n)]
+    indeg = np.zeros(n, int)
+    for i in range(n):
+        for p in (dam[i], sire[i]):
+            if p >= 0:
+                kids[p].append(i)
+                indeg[i] += 1
+    heap = [i for i in range(n) if indeg[i] == 0]
+    heapq.heapify(heap)
+    order = []
+    while heap:
+        v = heapq.heappop(heap)
+        order.append(v)
+        for k in kids[v]:
+            indeg[k] -= 1
+            if indeg[k] == 0:
+                heapq.heappush(heap, k)
+    order = np.array(order)
+    pos = np.empty(n, int)
+    pos[order] = np.arange(n)
+    return order, pos
+
+
+def test_native_unsorted_pedigree_from_first_appearance(cb, tmp_path):
+    """An inbred pedigree written with offspring rows first and no rows for
+    the founders: the reader numbers parents after offspring, and the
+    library's A^-1 and F equal the oracle's on the parents-first
+    renumbering (mapped back), and A^-1 inverts the tabular A."""
+    from cosmicblup_amd.pedigree import read_pedigree
+
+    dam0, sire0 = inbred_pedigree(n_gen=5, per_gen=30, seed=7)
+    n0 = dam0.size
+    name = [f"a{i}" for i in range(n0)]
+    lines = ["id dam sire"]
+    for i in range(n0 - 1, -1, -1):  # youngest first
+        if dam0[i] >= 0 or sire0[i] >= 0:
+            lines.append(f"{name[i]} {name[dam0[i]] if dam0[i] >= 0 else 0} "
+                         f"{name[sire0[i]] if sire0[i] >= 0 else 0}")
+    p = tmp_path / "ped.txt"
+    p.write_text("\n".join(lines) + "\n")
+    ids, dam, sire = read_pedigree(str(p))
+    assert any(dam[i] > i or sire[i] > i for i in range(len(ids)))  # really unsorted
+    got = invertA_henderson(dam, sire)
+    order, pos = _parents_first(dam, sire)
+    d2 = np.where(dam[order] >= 0, pos[np.maximum(dam[order], 0)], -1)
+    s2 = np.where(sire[order] >= 0, pos[np.maximum(sire[order], 0)], -1)
+    ref2, F2, dii2 = orl.henderson_ainv_f(d2, s2)
+    np.testing.assert_array_equal(got.F, F2[pos])
+    np.testing.assert_array_equal(got.dii, dii2[pos])
+    np.testing.assert_array_equal(got.toDense(), ref2[np.ix_(pos, pos)])
+    A2 = orl.tabular_a(s2, d2)
+    np.testing.assert_allclose(got.F, (np.diag(A2) - 1.0)[pos], atol=1e-13)
+    np.testing.assert_allclose(got.toDense() @ A2[np.ix_(pos, pos)], np.eye(len(ids)), atol=1e-10)
+    for r in range(got.n):  # CSR columns ascending after the mapping back
+        assert np.all(np.diff(got.col[got.ptr[r]:got.ptr[r + 1]]) > 0)
+
+
+def test_native_rejects_cycles(cb):
+    with pytest.raises(CosmicError, match="cycle"):
+        invertA_henderson(np.array([1, 0, -1]), np.array([-1, -1, -1]))
+    with pytest.raises(CosmicError, match="own parent"):
+        invertA_henderson(np.array([-1, 1]), np.array([-1, -1]))
