@@ -17,6 +17,9 @@ CB_ERR_OOM = -2
 CB_ERR_DEVICE = -3
 CB_ERR_NOT_CONVERGED = -4
 CB_ERR_INTERNAL = -5
+CB_ERR_IO = -6
+CB_MATRIX_TXT = 0
+CB_MATRIX_BIN = 1
 
 CB_GRM_OUT_F64 = 1
 CB_GRM_OUT_F32 = 2
@@ -87,6 +90,9 @@ _SIGS = {
     "cb_context_synchronize": (C.c_int, [c_vp]),
     "cb_context_set_profiling": (C.c_int, [c_vp, C.c_int]),
     "cb_context_set_graphs": (C.c_int, [c_vp, C.c_int]),
+    "cb_grm_write": (C.c_int, [c_vp, C.c_char_p, C.c_int]),
+    "cb_matrix_write": (C.c_int, [C.c_char_p, c_i64, c_vp, c_i64, C.c_int]),
+    "cb_upper_packed_read": (C.c_int, [C.c_char_p, c_i64p, c_vp, c_i64]),
     "cb_context_reset_profile": (C.c_int, [c_vp]),
     "cb_context_kernel_time": (C.c_int, [c_vp, C.c_char_p, c_dp, c_i64p]),
     "cb_genotypes_from_bed": (C.c_int, [c_vp, c_vp, C.c_size_t, c_i64, c_i64, C.POINTER(c_vp)]),

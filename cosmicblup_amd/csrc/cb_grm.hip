@@ -1248,6 +1248,16 @@ int cb_grm_copy_int_gram(const cb_grm* grm, int32_t* out, int64_t ld) {
     return guarded([&] { CB_REQUIRE(grm, "NULL grm"); copy_out(grm, grm->Gi.p, 4, out, ld); });
 }
 
+int cb_grm_write(const cb_grm* grm, const char* path, int format) {
+    return guarded([&] {
+        CB_REQUIRE(grm && path, "cb_grm_write: NULL argument");
+        CB_REQUIRE(grm->G64.p, "cb_grm_write: FP64 G not computed");
+        set_device(grm->ctx);
+        CB_HIP(hipStreamSynchronize(grm->ctx->stream));
+        write_symmetric(grm->ctx, grm->G64.p, grm->n, grm->n, true, path, format);
+    });
+}
+
 int cb_grm_copy_grm_device(const cb_grm* grm, void* dst, int64_t ld) {
     return guarded([&] {
         CB_REQUIRE(grm && grm->G64.p, "cb_grm_copy_grm_device: FP64 G not computed");

@@ -178,6 +178,11 @@ inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Shared between translation units (defined in cb_linalg.hip / cb_pedigree.hip).
+// Symmetric n x n FP64 matrix, row-major with leading dimension ld, in device
+// (on_device) or host memory, to `path`: format 0 text triplets, 1
+// COSMIC_UPPER_PACKED (cb_matrix_io.hip).
+void write_symmetric(cb_context* ctx, const double* A, int64_t n, int64_t ld, bool on_device, const char* path,
+                     int format);
 void invert_psd(cb_context* ctx, const double* dM, int64_t ldm, int64_t n, double ridge, double* dOut,
                 int64_t ldo, hipMemcpyKind in_kind, hipMemcpyKind out_kind, double* min_eig, double* max_eig);
 void pedigree_inbreeding(const std::vector<int32_t>& dam, const std::vector<int32_t>& sire, std::vector<double>& F,

@@ -87,6 +87,14 @@ class GrmBuilder:
         _lib.check(_lib.lib().cb_grm_device_ptr(self.handle, which, C.byref(p), C.byref(ld)))
         return p.value, ld.value
 
+    def writeMatrix(self, path: str, fmt: str = "bin") -> None:
+        """The FP64 G to `path`, streamed from the device, as
+        write_matrix_txt ("txt") or write_upper_packed_binary ("bin")
+        writes it (src/core/matrix_io.cpp:464-479)."""
+        from .matrixio import FORMATS
+
+        _lib.check(_lib.lib().cb_grm_write(self.handle, str(path).encode(), FORMATS[fmt]))
+
     def getSum2pq(self) -> float:
         v = C.c_double()
         _lib.check(_lib.lib().cb_grm_sum2pq(self.handle, C.byref(v)))

@@ -32,6 +32,7 @@ extern "C" {
 #define CB_ERR_DEVICE -3        /* HIP runtime error                        */
 #define CB_ERR_NOT_CONVERGED -4 /* iterative solver hit its iteration cap   */
 #define CB_ERR_INTERNAL -5
+#define CB_ERR_IO -6            /* a file could not be opened, read or written */
 
 typedef struct cb_context cb_context;
 typedef struct cb_genotypes cb_genotypes;
@@ -123,6 +124,16 @@ int cb_grm_copy_grm_device(const cb_grm* grm, void* dst, int64_t ld);
 /* Device pointer to an output (CB_GRM_OUT_*) and its leading dimension;
  * valid until the next compute or destroy. */
 int cb_grm_device_ptr(const cb_grm* grm, int which, void** ptr, int64_t* ld);
+/* The matrix part of write_named_matrix (src/io/relationship_builder.cpp:
+ * 43-53) for the FP64 G, streamed from device memory: format
+ * CB_MATRIX_TXT writes write_matrix_txt's "i<TAB>j<TAB>G(i,j)" lines for
+ * j <= i, 1-based, 6 significant digits (src/core/matrix_io.cpp:464-472);
+ * CB_MATRIX_BIN writes COSMIC_UPPER_PACKED (write_upper_packed_binary,
+ * src/core/upper_packed_binary.cpp:54-69). The caller writes the .id file.
+ * CB_ERR_IO when the file cannot be written. */
+#define CB_MATRIX_TXT 0
+#define CB_MATRIX_BIN 1
+int cb_grm_write(const cb_grm* grm, const char* path, int format);
 /* getSum2pq(), getTotalSnps(), getAlleleFreqs() (grm_builder.h:56-61):
  * allele frequencies are (float)p for SNPs that passed the filter and -1
  * otherwise (grm_builder.cpp:111,242). */
@@ -301,6 +312,18 @@ int cb_mv_vce(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* Y,
  * dGinv: n x n G^-1 on the device; beta (c x t) and u (n x t) column-major. */
 int cb_mv_solve(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* Y, const double* X,
                 const double* dGinv, int64_t ldg, const double* Vg, const double* Ve, double* beta, double* u);
+
+/* ---- matrix files ------------------------------------------------------ */
+/* A host symmetric matrix (row-major, leading dimension lda) as
+ * write_matrix_txt (CB_MATRIX_TXT, src/core/matrix_io.cpp:464-472) or
+ * write_upper_packed_binary (CB_MATRIX_BIN, src/core/upper_packed_binary.cpp:
+ * 54-69) writes it. */
+int cb_matrix_write(const char* path, int64_t n, const double* A, int64_t lda, int format);
+/* read_upper_packed_binary (upper_packed_binary.cpp:87-138): *n from the
+ * header; when out is non-NULL the matrix is expanded symmetrically into it
+ * (n x n, leading dimension ldo). The reference's checks and messages:
+ * magic, metadata, dimension, truncation, trailing bytes (CB_ERR_IO). */
+int cb_upper_packed_read(const char* path, int64_t* n, double* out, int64_t ldo);
 
 /* ---- pedigree relationship inverse ----------------------------------- */
 typedef struct cb_ainv cb_ainv;

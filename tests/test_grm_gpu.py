@@ -122,3 +122,28 @@ def test_large_properties(cb, ctx):
     for r0 in rows:
         blk = og.grm_block_raw(bed, n, m, st, r0, 1, 0, n) / S
         assert np.abs(G[r0] - blk[0]).max() < TOL_G64
+
+
+def test_grm_written_from_device(cb, ctx, tmp_path):
+    """cb_grm_write streams the FP64 G from the device in row panels (n =
+    3000 spans three): the COSMIC_UPPER_PACKED file holds exactly getGrm(),
+    and the text file holds its lower triangle in the stream format."""
+    from cosmicblup_amd import matrixio
+
+    n, m = 3000, 2000
+    x, _ = synth.simulate_dosages(n, m, maf=(0.05, 0.5), missing_rate=0.01, seed=21)
+    gb = run_gpu(cb, synth.encode_bed(x), n, m, outputs=cb.CB_GRM_OUT_F64)
+    G = gb.getGrm()
+    pb = tmp_path / "g.GA.bin"
+    gb.writeMatrix(str(pb), "bin")
+    assert pb.stat().st_size == 32 + 28 + 8 * n * (n + 1) // 2
+    np.testing.assert_array_equal(matrixio.read_upper_packed_binary(str(pb)), G)
+    # text: spot-check rows from every panel against the stream format
+    pt = tmp_path / "g.GA.txt"
+    gb.writeMatrix(str(pt), "txt")
+    with open(pt) as f:
+        lines = f.read().splitlines()
+    assert len(lines) == n * (n + 1) // 2
+    for i in (0, 1, 1397, 1398, 2796, 2999):
+        start = i * (i + 1) // 2
+        assert lines[start:start + i + 1] == [f"{i + 1}\t{j + 1}\t{G[i, j]:g}" for j in range(i + 1)]
