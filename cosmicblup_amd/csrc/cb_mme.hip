@@ -161,45 +161,46 @@ __global__ __launch_bounds__(256) void k_dense_mfma(const T* __restrict__ Q, int
         const int64_t rr = min(row0 + w * 32 + s * 16 + li, q - 1);  // rows >= q: computed, not stored
         qa[s] = Q + rr * ldq + k0 + 8 * g;
     }
-    T xr[XE], ar[2][8], an[2][8];
+    T xr[XE], aa[2][8], ab[2][8];
     V4 acc[2][NT];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
         for (int n = 0; n < NT; ++n) acc[s][n] = V4{0, 0, 0, 0};
+    // Unbranched loads: X indices clamped into the block (values outside
+    // zeroed after the load), Q chunks clamped to the last one. A chunk past
+    // the end therefore stages zeros and adds exactly nothing, so the loop
+    // runs two chunks per trip with fixed register sets and no branches,
+    // which keeps the compiler's wait counts exact.
     auto load_x = [&](int64_t kc) {
 #pragma unroll
         for (int i = 0; i < XE; ++i) {
             const int e = tid + 256 * i, c = e >> 5;
             const int64_t k = kc + (e & 31);
-            xr[i] = (c < r && k < k1) ? X[(int64_t)c * ldx + k] : (T)0;
+            xr[i] = X[(int64_t)(c < r ? c : r - 1) * ldx + (k < k1 ? k : k1 - 1)];
         }
     };
-    auto store_x = [&](int buf) {
+    auto store_x = [&](int buf, int64_t kc) {
 #pragma unroll
         for (int i = 0; i < XE; ++i) {
-            const int e = tid + 256 * i;
-            xs[buf][e >> 5][e & 31] = xr[i];
+            const int e = tid + 256 * i, c = e >> 5;
+            const int64_t k = kc + (e & 31);
+            xs[buf][c][e & 31] = (c < r && k < k1) ? xr[i] : (T)0;
         }
     };
     // k beyond k1 reads the zero padding of Q (last split) and meets zero X
-    auto load_a = [&](int off, T (&dst)[2][8]) {
+    auto load_a = [&](int ch, T (&dst)[2][8]) {
+        const int off = (ch < nchunks ? ch : nchunks - 1) * MF_KC;
 #pragma unroll
         for (int s = 0; s < 2; ++s)
 #pragma unroll
             for (int t = 0; t < 8; ++t) dst[s][t] = __builtin_nontemporal_load(qa[s] + off + t);
     };
-    load_x(k0);
-    load_a(0, ar);
-    store_x(0);
-    __syncthreads();
-    for (int ch = 0; ch < nchunks; ++ch) {
-        const int buf = ch & 1;
-        const bool more = ch + 1 < nchunks;
-        if (more) {
-            load_x(k0 + (int64_t)(ch + 1) * MF_KC);
-            load_a((ch + 1) * MF_KC, an);
-        }
+    // one chunk from xs[buf] and register set cur; the next chunk's loads
+    // (into nxt) issue first, its X is staged into the other buffer after
+    auto half = [&](int ch, int buf, const T (&cur)[2][8], T (&nxt)[2][8]) {
+        load_x(k0 + (int64_t)(ch + 1) * MF_KC);
+        load_a(ch + 1, nxt);
         T bx[NT][8];
 #pragma unroll
         for (int n = 0; n < NT; ++n)
@@ -210,15 +211,20 @@ __global__ __launch_bounds__(256) void k_dense_mfma(const T* __restrict__ Q, int
 #pragma unroll
             for (int s = 0; s < 2; ++s)
 #pragma unroll
-                for (int n = 0; n < NT; ++n) acc[s][n] = Mfma16<T>::step(ar[s][t], bx[n][t], acc[s][n]);
-        if (more) {
-            store_x(buf ^ 1);
-#pragma unroll
-            for (int s = 0; s < 2; ++s)
-#pragma unroll
-                for (int t = 0; t < 8; ++t) ar[s][t] = an[s][t];
-        }
+                for (int n = 0; n < NT; ++n) acc[s][n] = Mfma16<T>::step(cur[s][t], bx[n][t], acc[s][n]);
+        __builtin_amdgcn_sched_group_barrier(0x020, XE + 4, 0);  // the loads first,
+        __builtin_amdgcn_sched_group_barrier(0x008, 16 * NT, 0); // then the MFMAs;
+        __builtin_amdgcn_sched_barrier(0);  // the staging (which waits for X) after them
+        store_x(buf ^ 1, k0 + (int64_t)(ch + 1) * MF_KC);
         __syncthreads();
+    };
+    load_x(k0);
+    load_a(0, aa);
+    store_x(0, k0);
+    __syncthreads();
+    for (int ch = 0; ch < nchunks; ch += 2) {
+        half(ch, 0, aa, ab);
+        half(ch + 1, 1, ab, aa);
     }
     T* pp = part + (int64_t)split * r * ldp;
 #pragma unroll
