@@ -139,15 +139,19 @@ struct Mfma16<double> {
 // Q row loaded straight from global memory (Q is read exactly once), and MFMA
 // step t multiplies k = kc + 8g + t. The X chunk is staged in LDS as
 // [column][k] (double-buffered) and read in the same k order, which is all a
-// product needs.
-template <class T, int NT>
+// product needs. EX further columns (r = 16 NT + EX) are done by VALU FMAs on
+// the same A registers, each lane group summing its k-slice, the four slices
+// added by lane shuffles at the end: cheaper than a 16-column MFMA tile for
+// one useful column.
+template <class T, int NT, int EX>
 __global__ __launch_bounds__(256) void k_dense_mfma(const T* __restrict__ Q, int64_t ldq, int64_t q,
                                                     const T* __restrict__ X, int64_t ldx, int r,
                                                     int64_t kspan, int nsplit, T* __restrict__ part,
                                                     int64_t ldp) {
     using V4 = typename Mfma16<T>::v4;
-    constexpr int XE = NT * 16 * MF_KC / 256;  // staged X elements per thread
-    __shared__ T xs[2][NT * 16][MF_KCP];
+    constexpr int NC = NT * 16 + EX;                      // staged X columns
+    constexpr int XE = (NC * MF_KC + 255) / 256;          // staged X elements per thread
+    __shared__ T xs[2][NC][MF_KCP];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, g = lane >> 4;
     const int split = (int)(blockIdx.x % (unsigned)nsplit);
@@ -163,10 +167,14 @@ __global__ __launch_bounds__(256) void k_dense_mfma(const T* __restrict__ Q, int
     }
     T xr[XE], aa[2][8], ab[2][8];
     V4 acc[2][NT];
+    T ex[2][EX > 0 ? EX : 1];  // the EX columns past the MFMA tiles, by VALU
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
+    for (int s = 0; s < 2; ++s) {
 #pragma unroll
         for (int n = 0; n < NT; ++n) acc[s][n] = V4{0, 0, 0, 0};
+#pragma unroll
+        for (int e = 0; e < (EX > 0 ? EX : 1); ++e) ex[s][e] = 0;
+    }
     // Unbranched loads: X indices clamped into the block (values outside
     // zeroed after the load), Q chunks clamped to the last one. A chunk past
     // the end therefore stages zeros and adds exactly nothing, so the loop
@@ -185,7 +193,7 @@ __global__ __launch_bounds__(256) void k_dense_mfma(const T* __restrict__ Q, int
         for (int i = 0; i < XE; ++i) {
             const int e = tid + 256 * i, c = e >> 5;
             const int64_t k = kc + (e & 31);
-            xs[buf][c][e & 31] = (c < r && k < k1) ? xr[i] : (T)0;
+            if ((NC * MF_KC) % 256 == 0 || c < NC) xs[buf][c][e & 31] = (c < r && k < k1) ? xr[i] : (T)0;
         }
     };
     // k beyond k1 reads the zero padding of Q (last split) and meets zero X
@@ -212,6 +220,16 @@ __global__ __launch_bounds__(256) void k_dense_mfma(const T* __restrict__ Q, int
             for (int s = 0; s < 2; ++s)
 #pragma unroll
                 for (int n = 0; n < NT; ++n) acc[s][n] = Mfma16<T>::step(cur[s][t], bx[n][t], acc[s][n]);
+#pragma unroll
+        for (int e = 0; e < EX; ++e) {
+            T xe[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) xe[t] = xs[buf][NT * 16 + e][8 * g + t];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int t = 0; t < 8; ++t) ex[s][e] = fma(cur[s][t], xe[t], ex[s][e]);
+        }
         __builtin_amdgcn_sched_group_barrier(0x020, XE + 4, 0);  // the loads first,
         __builtin_amdgcn_sched_group_barrier(0x008, 16 * NT, 0); // then the MFMAs;
         __builtin_amdgcn_sched_barrier(0);  // the staging (which waits for X) after them
@@ -238,6 +256,18 @@ __global__ __launch_bounds__(256) void k_dense_mfma(const T* __restrict__ Q, int
                 const int64_t row = row0 + w * 32 + s * 16 + Mfma16<T>::row(g, j);
                 if (row < q) pp[(int64_t)c * ldp + row] = acc[s][n][j];
             }
+        }
+    // extra columns: the four lane groups hold k-slices of the same rows
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int e = 0; e < EX; ++e) {
+            T v = ex[s][e];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            const int c = NT * 16 + e;
+            const int64_t row = row0 + w * 32 + s * 16 + li;
+            if (g == 0 && c < r && row < q) pp[(int64_t)c * ldp + row] = v;
         }
 }
 
@@ -735,18 +765,36 @@ void dense_apply(cb_mme* M, int64_t qn, const T* X, int64_t ldx, T* Y, int64_t l
     const int64_t ldp = round_up(qn, 64);
     for (int c0 = 0; c0 < r; c0 += MF_MAXNT * 16) {
         const int rc = std::min(MF_MAXNT * 16, r - c0);
-        const int nt = (rc + 15) / 16;
+        // FP32: whole 16-column MFMA tiles and one or two columns past them
+        // by VALU (r = 16k + 1 is the probes-plus-y block); FP64 measured
+        // faster with the padded tile
+        int nt = (rc + 15) / 16, ex = 0;
+        if (sizeof(T) == 4 && rc > 16 && rc % 16 != 0 && rc % 16 <= 2) {
+            nt = rc / 16;
+            ex = rc % 16;
+        }
         T* part = partbuf<T>(M, (size_t)nsplit * rc * ldp);
         const T* Xc = X + (int64_t)c0 * ldx;
         const unsigned grid = (unsigned)(row_tiles * nsplit);
         timed(ctx, sizeof(T) == 8 ? "k_dense_mfma_f64" : "k_dense_mfma_f32", [&] {
-            switch (nt) {
-                case 1: k_dense_mfma<T, 1><<<grid, 256, 0, st>>>(Q, ldq, qn, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
-                case 2: k_dense_mfma<T, 2><<<grid, 256, 0, st>>>(Q, ldq, qn, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
-                case 3: k_dense_mfma<T, 3><<<grid, 256, 0, st>>>(Q, ldq, qn, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
-                case 4: k_dense_mfma<T, 4><<<grid, 256, 0, st>>>(Q, ldq, qn, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
-                default: k_dense_mfma<T, 5><<<grid, 256, 0, st>>>(Q, ldq, qn, Xc, ldx, rc, kspan, nsplit, part, ldp); break;
+#define CB_DM(NT_, EX_) \
+    k_dense_mfma<T, NT_, EX_><<<grid, 256, 0, st>>>(Q, ldq, qn, Xc, ldx, rc, kspan, nsplit, part, ldp)
+            switch (nt * 4 + ex) {
+                case 1 * 4: CB_DM(1, 0); break;
+                case 1 * 4 + 1: CB_DM(1, 1); break;
+                case 1 * 4 + 2: CB_DM(1, 2); break;
+                case 2 * 4: CB_DM(2, 0); break;
+                case 2 * 4 + 1: CB_DM(2, 1); break;
+                case 2 * 4 + 2: CB_DM(2, 2); break;
+                case 3 * 4: CB_DM(3, 0); break;
+                case 3 * 4 + 1: CB_DM(3, 1); break;
+                case 3 * 4 + 2: CB_DM(3, 2); break;
+                case 4 * 4: CB_DM(4, 0); break;
+                case 4 * 4 + 1: CB_DM(4, 1); break;
+                case 4 * 4 + 2: CB_DM(4, 2); break;
+                default: CB_DM(5, 0); break;
             }
+#undef CB_DM
         });
         k_dense_reduce<T><<<dim3(blocks(qn), (unsigned)rc), 256, 0, st>>>(part, ldp, nsplit, qn, rc,
                                                                         Y + (int64_t)c0 * ldy, ldy, 0, alpha);
