@@ -271,17 +271,33 @@ __global__ __launch_bounds__(256) void k_dense_mfma(const T* __restrict__ Q, int
         }
 }
 
-// Y[row, c] += alpha * sum_split part[split][c][row]   (fixed split order)
+// Y[row, c] += alpha * sum_split part[split][c][row]   (fixed split order);
+// with sv set, then Y[row, c] += scoef * (sum of spart[c0 + c]'s chunks) *
+// sv[row]: the stiff rank-1 term of apply_C, rounded exactly as its own
+// kernel (k_stiff_axpy) rounds it.
 template <class T>
 __global__ void k_dense_reduce(const T* __restrict__ part, int64_t ldp, int nsplit, int64_t q, int r,
-                               T* __restrict__ Y, int64_t ldy, int64_t y_off, T alpha) {
+                               T* __restrict__ Y, int64_t ldy, int64_t y_off, T alpha,
+                               const double* __restrict__ sv, const double* __restrict__ spart, int c0,
+                               double scoef) {
+    __shared__ double t;
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
+    if (sv) {
+        if (threadIdx.x == 0) {
+            double a = 0;
+            for (int k = 0; k < DOT_CHUNKS; ++k) a += spart[(int64_t)(c0 + c) * DOT_CHUNKS + k];
+            t = a;
+        }
+        __syncthreads();
+    }
     if (row >= q) return;
     double s = 0;
     for (int sp = 0; sp < nsplit; ++sp) s += (double)part[((int64_t)sp * r + c) * ldp + row];
     T* y = Y + (int64_t)c * ldy + y_off + row;
-    *y = (T)((double)*y + (double)alpha * s);
+    T v = (T)((double)*y + (double)alpha * s);
+    if (sv) v = (T)((double)v + scoef * t * sv[row]);
+    *y = v;
 }
 
 // 16-byte vector of T
@@ -471,46 +487,6 @@ struct ColState {
     int active, restarted;
 };
 
-__global__ void k_cg_alpha(ColState* __restrict__ st, const double* __restrict__ pq, int r) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= r) return;
-    ColState s = st[c];
-    s.pq = pq[c];
-    if (s.active && fabs(s.pq) < 1e-300) s.active = 0;  // breakdown: freeze the column
-    s.alpha = s.active ? s.rho / s.pq : 0.0;
-    st[c] = s;
-}
-
-// x += alpha p ; r -= alpha q   (inactive columns untouched)
-template <class TX, class T>
-__global__ void k_cg_xr(TX* __restrict__ X, T* __restrict__ R, const T* __restrict__ P,
-                        const T* __restrict__ Qp, const ColState* __restrict__ st, int64_t rows,
-                        int64_t ld) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = blockIdx.y;
-    if (i >= rows || !st[c].active) return;
-    const double a = st[c].alpha;
-    const int64_t o = (int64_t)c * ld + i;
-    X[o] = (TX)((double)X[o] + a * (double)P[o]);
-    R[o] = (T)((double)R[o] - a * (double)Qp[o]);
-}
-
-// after dots: rho_new = r.z, rr = r.r ; decide convergence ; beta
-__global__ void k_cg_beta(ColState* __restrict__ st, const double* __restrict__ rz,
-                          const double* __restrict__ rr, int r, double tol2, int absolute) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= r) return;
-    ColState s = st[c];
-    if (!s.active) return;
-    s.rho_new = rz[c];
-    s.rr = rr[c];
-    const double thr = absolute ? tol2 : tol2 * s.bb;
-    if (!(s.rr >= thr)) s.active = 0;  // converged (or NaN)
-    s.beta = s.rho != 0.0 ? s.rho_new / s.rho : 0.0;
-    s.rho = s.rho_new;
-    st[c] = s;
-}
-
 template <class T>
 __global__ void k_cg_p(T* __restrict__ P, const T* __restrict__ Z, const ColState* __restrict__ st,
                        int64_t rows, int64_t ld) {
@@ -519,6 +495,94 @@ __global__ void k_cg_p(T* __restrict__ P, const T* __restrict__ Z, const ColStat
     if (i >= rows || !st[c].active) return;
     const int64_t o = (int64_t)c * ld + i;
     P[o] = (T)((double)Z[o] + st[c].beta * (double)P[o]);
+}
+
+// The iteration's scalar steps read the dot products' chunk partials
+// themselves (summed in chunk order, as k_dots_finish does), so no launch
+// finishes a dot on its own:
+//   k_col_dots2      the partials of A.B and A.C in one pass over A
+//   k_cg_alpha_parts alpha = rho / p.q (a breakdown freezes the column)
+//   k_cg_beta_parts  rho_new = r.z, rr = r.r, convergence, beta
+//   k_cg_xr_diag     x += alpha p, r -= alpha q, then the Jacobi rows of z
+template <class T>
+__global__ __launch_bounds__(256) void k_col_dots2(const T* __restrict__ A, const T* __restrict__ B,
+                                                   const T* __restrict__ C, int64_t ld, int64_t rows,
+                                                   double* __restrict__ part_ab, double* __restrict__ part_ac) {
+    __shared__ double red[2][256];
+    const int c = blockIdx.y;
+    const int64_t per = (rows + DOT_CHUNKS - 1) / DOT_CHUNKS;
+    const int64_t r0 = (int64_t)blockIdx.x * per, r1 = min(rows, r0 + per);
+    double s = 0, u = 0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) {
+        const double a = (double)A[(int64_t)c * ld + i];
+        s += a * (double)B[(int64_t)c * ld + i];
+        u += a * (double)C[(int64_t)c * ld + i];
+    }
+    red[0][threadIdx.x] = s;
+    red[1][threadIdx.x] = u;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part_ab[(int64_t)c * DOT_CHUNKS + blockIdx.x] = red[0][0];
+        part_ac[(int64_t)c * DOT_CHUNKS + blockIdx.x] = red[1][0];
+    }
+}
+
+__device__ __forceinline__ double sum_chunks(const double* __restrict__ part, int c) {
+    double s = 0;
+    for (int k = 0; k < DOT_CHUNKS; ++k) s += part[(int64_t)c * DOT_CHUNKS + k];
+    return s;
+}
+
+__global__ void k_cg_alpha_parts(ColState* __restrict__ st, const double* __restrict__ part, int r) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= r) return;
+    ColState s = st[c];
+    s.pq = sum_chunks(part, c);
+    if (s.active && fabs(s.pq) < 1e-300) s.active = 0;  // breakdown: freeze the column
+    s.alpha = s.active ? s.rho / s.pq : 0.0;
+    st[c] = s;
+}
+
+__global__ void k_cg_beta_parts(ColState* __restrict__ st, const double* __restrict__ part_rz,
+                                const double* __restrict__ part_rr, int r, double tol2, int absolute) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= r) return;
+    ColState s = st[c];
+    if (!s.active) return;
+    s.rho_new = sum_chunks(part_rz, c);
+    s.rr = sum_chunks(part_rr, c);
+    const double thr = absolute ? tol2 : tol2 * s.bb;
+    if (!(s.rr >= thr)) s.active = 0;  // converged (or NaN)
+    s.beta = s.rho != 0.0 ? s.rho_new / s.rho : 0.0;
+    s.rho = s.rho_new;
+    st[c] = s;
+}
+
+// active columns: x += alpha p, r -= alpha q; every column: z = r dinv on
+// the rows >= p0 (the block-Jacobi rows < p0 follow in their own kernel)
+template <class TX, class T>
+__global__ void k_cg_xr_diag(TX* __restrict__ X, T* __restrict__ R, const T* __restrict__ P,
+                             const T* __restrict__ Qp, T* __restrict__ Z, const double* __restrict__ dinv,
+                             const ColState* __restrict__ st, int64_t p0, int64_t rows, int64_t ld) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= rows) return;
+    const int64_t o = (int64_t)c * ld + i;
+    T rv = R[o];
+    if (st[c].active) {
+        const double a = st[c].alpha;
+        X[o] = (TX)((double)X[o] + a * (double)P[o]);
+        rv = (T)((double)rv - a * (double)Qp[o]);
+        R[o] = rv;
+    }
+    if (i >= p0) Z[o] = (T)((double)rv * dinv[i]);
 }
 
 // R = B - A   (column-major, same ld)
@@ -741,11 +805,21 @@ __global__ void k_scatter_add_rows(const T* __restrict__ yg, int64_t ldg, const 
     if (g >= 0) Y[c * ldy + g] += alpha * yg[c * ldg + i];
 }
 
+// Y[:, c] += coef (v' x_c) v, a rank-1 term: v, and the per-column partial
+// dots of v' x_c (DOT_CHUNKS per column) already on the device
+struct StiffTerm {
+    const double* v;
+    const double* part;
+    double coef;
+};
+
 // Y[0:qn, c] += alpha * Qd X[0:qn, c] for r columns: the dense block Qd
 // (qn rows, leading dimension ldq, zero-padded) of this MME. X and Y point
-// at row 0 of the block's index space.
+// at row 0 of the block's index space. With sk, the rank-1 term follows in
+// the same pass.
 template <class T>
-void dense_apply(cb_mme* M, int64_t qn, const T* X, int64_t ldx, T* Y, int64_t ldy, int r, T alpha) {
+void dense_apply(cb_mme* M, int64_t qn, const T* X, int64_t ldx, T* Y, int64_t ldy, int r, T alpha,
+                 const StiffTerm* sk = nullptr) {
     cb_context* ctx = M->ctx;
     hipStream_t st = ctx->stream;
     const T* Q = qdense<T>(M);
@@ -755,6 +829,9 @@ void dense_apply(cb_mme* M, int64_t qn, const T* X, int64_t ldx, T* Y, int64_t l
             k_gemv<T><<<(unsigned)ceil_div(qn, 4 * GEMV_ROWS), 256, 0, st>>>(Q, ldq, qn, X, ldx, r, Y, ldy, 0,
                                                                          alpha);
         });
+        if (sk)
+            k_stiff_axpy<T><<<dim3(blocks(qn), (unsigned)r), 256, 0, st>>>(Y, sk->v, sk->part, sk->coef, ldy, qn);
+        CB_HIP(hipGetLastError());
         return;
     }
     const int64_t row_tiles = ceil_div(qn, MF_ROWS);
@@ -796,8 +873,9 @@ void dense_apply(cb_mme* M, int64_t qn, const T* X, int64_t ldx, T* Y, int64_t l
             }
 #undef CB_DM
         });
-        k_dense_reduce<T><<<dim3(blocks(qn), (unsigned)rc), 256, 0, st>>>(part, ldp, nsplit, qn, rc,
-                                                                        Y + (int64_t)c0 * ldy, ldy, 0, alpha);
+        k_dense_reduce<T><<<dim3(blocks(qn), (unsigned)rc), 256, 0, st>>>(
+            part, ldp, nsplit, qn, rc, Y + (int64_t)c0 * ldy, ldy, 0, alpha, sk ? sk->v : nullptr,
+            sk ? sk->part : nullptr, c0, sk ? sk->coef : 0.0);
         CB_HIP(hipGetLastError());
     }
 }
@@ -816,11 +894,11 @@ DevBuf<float>& gbuf<float>(cb_mme* M, int which) { return which ? M->yg32 : M->x
 //           with D = tau G^-1 - omega A22^-1 held as one dense block)
 // The rank-1 stiff term of a dense Q's FP32 copy is added by apply_C.
 template <class T>
-void q_apply(cb_mme* M, const T* X, int64_t ldx, T* Y, int64_t ldy, int r, T alpha) {
+void q_apply(cb_mme* M, const T* X, int64_t ldx, T* Y, int64_t ldy, int r, T alpha, const StiffTerm* sk = nullptr) {
     hipStream_t st = M->ctx->stream;
     const int64_t p = M->p, q = M->q;
     if (M->qkind == 0) {
-        dense_apply<T>(M, q, X + p, ldx, Y + p, ldy, r, alpha);
+        dense_apply<T>(M, q, X + p, ldx, Y + p, ldy, r, alpha, sk);
         return;
     }
     k_csr_spmm<T, T><<<dim3(blocks(q), (unsigned)r), 256, 0, st>>>(M->qp.p, M->qc.p, qvals<T>(M), 0, q, p, X, ldx,
@@ -862,16 +940,17 @@ void apply_C(cb_mme* M, double lambda, const T* X, T* Y, int r) {
                 M->wp.p, M->wc.p, wvals<T>(M), M->long_rows, M->dim, 0, X, ld, Y, ld, 0, (T)1, 0);
         });
     CB_HIP(hipGetLastError());
-    // lambda Q part
-    q_apply<T>(M, X, ld, Y, ld, r, (T)lambda);
+    // lambda Q part; the FP32 copy of a dense Q holds Q - mu v v', and
+    // lambda mu v (v' x_u) is added in the same pass that adds lambda Q x_u
     if (M->qkind == 0 && M->stiff && sizeof(T) == 4) {
-        // the FP32 copy holds Q - mu v v': add lambda mu v (v' x_u) here
         if (M->stiff_part.n < (size_t)DOT_CHUNKS * r) M->stiff_part.alloc((size_t)DOT_CHUNKS * r);
         k_stiff_dot<T><<<dim3(DOT_CHUNKS, (unsigned)r), 256, 0, st>>>(X + M->p, M->stiff_v.p, ld, M->q,
                                                                     M->stiff_part.p);
-        k_stiff_axpy<T><<<dim3(blocks(M->q), (unsigned)r), 256, 0, st>>>(Y + M->p, M->stiff_v.p, M->stiff_part.p,
-                                                                       lambda * M->stiff_mu, ld, M->q);
         CB_HIP(hipGetLastError());
+        const StiffTerm sk{M->stiff_v.p, M->stiff_part.p, lambda * M->stiff_mu};
+        q_apply<T>(M, X, ld, Y, ld, r, (T)lambda, &sk);
+    } else {
+        q_apply<T>(M, X, ld, Y, ld, r, (T)lambda);
     }
 }
 
@@ -963,10 +1042,11 @@ void build_precond(cb_mme* M, double lambda, bool block) {
     M->dinv_block = (int)block;
 }
 
+// the preconditioner's block-Jacobi rows and stiff term; the Jacobi rows
+// (>= p) of Z must hold R * dinv already
 template <class T>
-void apply_precond(cb_mme* M, const T* R, T* Z, int r) {
+void apply_precond_rest(cb_mme* M, const T* R, T* Z, int r) {
     hipStream_t st = M->ctx->stream;
-    k_precond_diag<T><<<dim3(blocks(M->dim - M->p), (unsigned)r), 256, 0, st>>>(R, Z, M->dinv.p, M->p, M->dim, M->ldv);
     if (M->p > 0)
         k_precond_block<T><<<dim3(blocks(M->p, 4), (unsigned)r), 256, 0, st>>>(R, Z, M->Binv.p, M->p, M->ldv);
     if (M->stiff) {
@@ -976,6 +1056,33 @@ void apply_precond(cb_mme* M, const T* R, T* Z, int r) {
         k_stiff_axpy<T><<<dim3(blocks(M->q), (unsigned)r), 256, 0, st>>>(Z + M->p, M->stiff_y.p, M->stiff_part.p,
                                                                         -M->stiff_gamma, M->ldv, M->q);
     }
+    CB_HIP(hipGetLastError());
+}
+
+template <class T>
+void apply_precond(cb_mme* M, const T* R, T* Z, int r) {
+    hipStream_t st = M->ctx->stream;
+    k_precond_diag<T><<<dim3(blocks(M->dim - M->p), (unsigned)r), 256, 0, st>>>(R, Z, M->dinv.p, M->p, M->dim, M->ldv);
+    apply_precond_rest<T>(M, R, Z, r);
+}
+
+// One CG iteration on an r-column block after P holds the search
+// directions: Qp = C P, alpha, x and r, z = M^-1 r, beta, convergence and P.
+template <class TX, class T>
+void cg_iteration(cb_mme* M, double lambda, TX* X, T* R, T* Z, T* P, T* Qp, ColState* cs, double* part,
+                  double* part2, int r, double tol2, bool absolute) {
+    hipStream_t st = M->ctx->stream;
+    const int64_t ld = M->ldv, dim = M->dim;
+    const dim3 g2(blocks(dim), (unsigned)r);
+    const dim3 gd(DOT_CHUNKS, (unsigned)r);
+    apply_C<T>(M, lambda, P, Qp, r);
+    k_col_dots<T><<<gd, 256, 0, st>>>(P, Qp, ld, dim, part);
+    k_cg_alpha_parts<<<blocks(r, 64), 64, 0, st>>>(cs, part, r);
+    k_cg_xr_diag<TX, T><<<g2, 256, 0, st>>>(X, R, P, Qp, Z, M->dinv.p, cs, M->p, dim, ld);
+    apply_precond_rest<T>(M, R, Z, r);
+    k_col_dots2<T><<<gd, 256, 0, st>>>(R, Z, R, ld, dim, part, part2);
+    k_cg_beta_parts<<<blocks(r, 64), 64, 0, st>>>(cs, part, part2, r, tol2, absolute ? 1 : 0);
+    k_cg_p<T><<<g2, 256, 0, st>>>(P, Z, cs, dim, ld);
     CB_HIP(hipGetLastError());
 }
 
@@ -1105,17 +1212,10 @@ int64_t pcg_block(cb_mme* M, double lambda, const T* B, TX* X, int r, double tol
     int64_t it = 0;
     const int check_every = 8;
     bool any_active = std::any_of(hs.begin(), hs.end(), [](const ColState& s) { return s.active; });
+    DevBuf<double> part2;
+    part2.alloc((size_t)DOT_CHUNKS * r);
     auto iteration = [&] {
-        apply_C<T>(M, lambda, P.p, Qp.p, r);
-        col_dots<T>(M, P.p, Qp.p, r, part.p, d1.p);
-        k_cg_alpha<<<blocks(r, 64), 64, 0, st>>>(cs.p, d1.p, r);
-        k_cg_xr<TX, T><<<g2, 256, 0, st>>>(X, R.p, P.p, Qp.p, cs.p, dim, ld);
-        apply_precond<T>(M, R.p, Z.p, r);
-        col_dots<T>(M, R.p, Z.p, r, part.p, d1.p);
-        col_dots<T>(M, R.p, R.p, r, part.p, d2.p);
-        k_cg_beta<<<blocks(r, 64), 64, 0, st>>>(cs.p, d1.p, d2.p, r, tol2, absolute ? 1 : 0);
-        k_cg_p<T><<<g2, 256, 0, st>>>(P.p, Z.p, cs.p, dim, ld);
-        CB_HIP(hipGetLastError());
+        cg_iteration<TX, T>(M, lambda, X, R.p, Z.p, P.p, Qp.p, cs.p, part.p, part2.p, r, tol2, absolute);
     };
     // The first check_every iterations run as plain launches (they size the
     // operator's scratch); later blocks of check_every iterations replay one
@@ -1255,18 +1355,12 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
     int64_t it = 0;
     const int check_every = 4;
     int64_t replacements = 0;
+    DevBuf<double> part2;
+    part2.alloc((size_t)DOT_CHUNKS * r);
+    // the recursive test in the iteration only pauses a column until the
+    // next replacement
     auto iteration = [&] {
-        apply_C<float>(M, lambda, P.p, Qp.p, r);
-        col_dots<float>(M, P.p, Qp.p, r, part.p, d1.p);
-        k_cg_alpha<<<blocks(r, 64), 64, 0, st>>>(cs.p, d1.p, r);
-        k_cg_xr<double, float><<<g2, 256, 0, st>>>(X, R.p, P.p, Qp.p, cs.p, dim, ld);
-        apply_precond<float>(M, R.p, Z.p, r);
-        col_dots<float>(M, R.p, Z.p, r, part.p, d1.p);
-        col_dots<float>(M, R.p, R.p, r, part.p, d2.p);
-        // the recursive test only pauses a column until the next replacement
-        k_cg_beta<<<blocks(r, 64), 64, 0, st>>>(cs.p, d1.p, d2.p, r, tol2, absolute ? 1 : 0);
-        k_cg_p<float><<<g2, 256, 0, st>>>(P.p, Z.p, cs.p, dim, ld);
-        CB_HIP(hipGetLastError());
+        cg_iteration<double, float>(M, lambda, X, R.p, Z.p, P.p, Qp.p, cs.p, part.p, part2.p, r, tol2, absolute);
     };
     // as in pcg_block: plain launches for the first block, then a captured
     // block of check_every iterations replayed between the host checks (the
