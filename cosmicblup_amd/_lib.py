@@ -54,7 +54,7 @@ class MvOptions(C.Structure):
 class RemlOptions(C.Structure):
     _fields_ = [("algorithm", C.c_int), ("max_iter", C.c_int64), ("tol", C.c_double),
                 ("n_probes", C.c_int64), ("seed", C.c_uint64), ("trace_exact", C.c_int),
-                ("solve", SolveOptions)]
+                ("solve", SolveOptions), ("block_cols", C.c_int64)]
 
 
 CB_PREC_FP64 = 0

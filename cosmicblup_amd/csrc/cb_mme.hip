@@ -318,7 +318,11 @@ struct Vec16<double> {
 // rows: each lane streams 16-byte runs of those rows (Q rows start 128-byte
 // aligned, leading dimension a multiple of 32) and loads x once for all of
 // them; FP64 accumulation. HBM-bound: q^2 * sizeof(T) bytes per launch.
-constexpr int GEMV_ROWS = 4;
+// GEMV_PF runs of every row, with their x, are loaded before the first is
+// used, so that a lane has GEMV_PF * GEMV_ROWS loads in flight (FP32 r = 2 at
+// q = 20 000: 0.46 -> 0.24 ms; one run per trip left the FP64 conversions
+// and FMAs of each run waiting on its own load).
+constexpr int GEMV_ROWS = 4, GEMV_PF = 4;
 template <class T>
 __global__ __launch_bounds__(256) void k_gemv(const T* __restrict__ Q, int64_t ldq, int64_t q,
                                               const T* __restrict__ X, int64_t ldx, int r,
@@ -336,21 +340,30 @@ __global__ __launch_bounds__(256) void k_gemv(const T* __restrict__ Q, int64_t l
     for (int a = 0; a < GEMV_ROWS; ++a) acc0[a] = acc1[a] = 0;
     const int64_t nv = q / VW;
     const bool two = r > 1;
-#pragma unroll 2
-    for (int64_t v = lane; v < nv; v += 64) {
-        double x0[VW], x1[VW];
+    for (int64_t v0 = lane; v0 < nv; v0 += 64 * GEMV_PF) {
+        VT w[GEMV_PF][GEMV_ROWS];
+        T x0[GEMV_PF][VW], x1[GEMV_PF][VW];
 #pragma unroll
-        for (int e = 0; e < VW; ++e) {
-            x0[e] = (double)X[v * VW + e];
-            x1[e] = two ? (double)X[ldx + v * VW + e] : 0.0;
-        }
-#pragma unroll
-        for (int a = 0; a < GEMV_ROWS; ++a) {
-            const VT w = __builtin_nontemporal_load(qr[a] + v);
+        for (int t = 0; t < GEMV_PF; ++t) {
+            const int64_t v = min(v0 + 64 * t, nv - 1);  // a run past the end is loaded (clamped), not used
 #pragma unroll
             for (int e = 0; e < VW; ++e) {
-                acc0[a] += (double)w[e] * x0[e];
-                acc1[a] += (double)w[e] * x1[e];
+                x0[t][e] = X[v * VW + e];
+                x1[t][e] = two ? X[ldx + v * VW + e] : (T)0;
+            }
+#pragma unroll
+            for (int a = 0; a < GEMV_ROWS; ++a) w[t][a] = __builtin_nontemporal_load(qr[a] + v);
+        }
+#pragma unroll
+        for (int t = 0; t < GEMV_PF; ++t) {
+            if (v0 + 64 * t < nv) {
+#pragma unroll
+                for (int a = 0; a < GEMV_ROWS; ++a)
+#pragma unroll
+                    for (int e = 0; e < VW; ++e) {
+                        acc0[a] += (double)w[t][a][e] * (double)x0[t][e];
+                        acc1[a] += (double)w[t][a][e] * (double)x1[t][e];
+                    }
             }
         }
     }
@@ -688,6 +701,58 @@ __global__ void k_axpby(const double* __restrict__ a, double alpha, const double
                         double beta, double* __restrict__ out, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = alpha * a[i] + beta * b[i];
+}
+
+// The REML score's data terms from one block of MME solutions, in one pass
+// (vce.cpp:2655-2675 uses u'Qu, e'e and tr(Q C^uu)). Chunk partials in the
+// layout of k_col_dots, block (chunk, column):
+//   with_y, column 0   e = y - W s_0 over the n records (stored), e'e
+//   with_y, column 1   u'Qu with u = s_0[p:], Qu given
+//   trace column j     z_j' Q C^uu z_j = (Q z_j)' s_{t0+j}[p:]
+// (t0 = 1 with the y column, else 0). Each partial sums its chunk's rows in
+// thread-strided order and a fixed tree, so the terms do not depend on the
+// launch.
+__global__ __launch_bounds__(256) void k_reml_score_terms(
+    const int64_t* __restrict__ xp, const int32_t* __restrict__ xc, const double* __restrict__ xv,
+    const int32_t* __restrict__ aid, const double* __restrict__ y, int64_t n, int64_t p, int64_t q,
+    const double* __restrict__ S, int64_t ld, const double* __restrict__ Qu, const double* __restrict__ Qz,
+    int with_y, double* __restrict__ e, double* __restrict__ part) {
+    __shared__ double red[256];
+    const int c = blockIdx.y;
+    const bool resid = with_y && c == 0;
+    const int64_t rows = resid ? n : q;
+    const int64_t per = (rows + DOT_CHUNKS - 1) / DOT_CHUNKS;
+    const int64_t r0 = (int64_t)blockIdx.x * per, r1 = min(rows, r0 + per);
+    double acc = 0;
+    if (resid) {
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += 256) {
+            double t = 0;
+            for (int64_t k = xp[r]; k < xp[r + 1]; ++k) t += xv[k] * S[xc[k]];
+            if (aid[r] >= 0) t += S[p + aid[r]];
+            const double ev = y[r] - t;
+            e[r] = ev;
+            acc += ev * ev;
+        }
+    } else {
+        const double* a;
+        const double* b;
+        if (with_y && c == 1) {
+            a = S + p;
+            b = Qu + p;
+        } else {
+            const int j = c - (with_y ? 2 : 0);
+            a = S + (int64_t)(j + (with_y ? 1 : 0)) * ld + p;
+            b = Qz + (int64_t)j * ld + p;
+        }
+        for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) acc += a[i] * b[i];
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)c * DOT_CHUNKS + blockIdx.x] = red[0];
 }
 
 inline unsigned blocks(int64_t n, int t = 256) { return (unsigned)std::max<int64_t>(1, ceil_div(n, t)); }
@@ -1573,9 +1638,49 @@ void project_spd_2x2(double A[4]) {
 
 }  // namespace
 
+// Device state of one REML run. The block solve takes y's right-hand side
+// W'y and the k probe columns [0; z] together, bw columns at a time: bw =
+// 1 + k when the block fits (probes and Q z generated once for the run),
+// else the probes go through in chunks of bw columns whose vectors and Q z are
+// regenerated at every evaluation instead of being stored, y's column riding
+// with the last chunk.
 struct cb_reml_work {
-    DevBuf<double> B, S, Qz, B2, S2, V, T, part, dots, e;
+    DevBuf<double> B, S, Qz, B2, S2, V, T, part, dots, e, rhs;
+    int k = 0, bw = 0;
+    bool chunked = false;
 };
+
+namespace {
+
+// device bytes per block column of a REML solve: B, S and Q z (FP64), the PCG
+// vectors (four FP32 and two FP64, or four FP64) and the split-K partials
+constexpr int64_t REML_BYTES_PER_COL_ROW = 128;
+
+int reml_block_width(cb_mme* M, const cb_reml_options& o, int r) {
+    if (o.block_cols > 0) return (int)std::min<int64_t>(r, std::max<int64_t>(2, o.block_cols));
+    size_t free_b = 0, total_b = 0;
+    CB_HIP(hipMemGetInfo(&free_b, &total_b));
+    const int64_t fit = (int64_t)(free_b / 2) / (REML_BYTES_PER_COL_ROW * M->ldv);
+    return (int)std::min<int64_t>(r, std::max<int64_t>(2, fit));
+}
+
+// probe columns [first, first + count) into B's columns dst.. (rows p.. ; the
+// fixed-effect rows stay zero) and, with Qz, Q z into Qz's columns 0..count
+void make_probes(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, int64_t first, int count, int dst,
+                 bool with_qz) {
+    hipStream_t st = M->ctx->stream;
+    const int64_t ld = M->ldv, p = M->p, q = M->q;
+    CB_HIP(hipMemsetAsync(w.B.p + (size_t)dst * ld, 0, (size_t)count * ld * 8, st));
+    k_probes<<<dim3(blocks(q), (unsigned)count), 256, 0, st>>>(w.B.p + (size_t)dst * ld, ld, p, q, count, o.seed,
+                                                              o.trace_exact, first);
+    CB_HIP(hipGetLastError());
+    if (with_qz) {
+        CB_HIP(hipMemsetAsync(w.Qz.p, 0, (size_t)count * ld * 8, st));
+        q_apply<double>(M, w.B.p + (size_t)dst * ld, ld, w.Qz.p, ld, count, 1.0);
+    }
+}
+
+}  // namespace
 
 // One evaluation at (vu, ve): solves, score terms, optionally the AI matrix.
 static void reml_eval(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, double vu, double ve,
@@ -1583,50 +1688,55 @@ static void reml_eval(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, doub
     cb_context* ctx = M->ctx;
     hipStream_t st = ctx->stream;
     const int64_t ld = M->ldv, p = M->p, q = M->q, n = M->n, dim = M->dim;
-    const int k = (int)(o.trace_exact ? q : o.n_probes);
-    const int r = 1 + k;
+    const int k = w.k, bw = w.bw;
     const double lambda = ve / std::max(vu, 1e-9);
-    // block: column 0 = W'y (already in B), columns 1..k = [0; z]
-    SolveStats S1 = solve_block(M, lambda, o.solve, w.B.p, w.S.p, r);
-    (void)S1;
-    // score terms ("adjoint" pass over the solution block)
+    // probe chunks: [first, first + count), y's column with the last one
+    std::vector<std::pair<int64_t, int>> chunks;
+    if (!w.chunked) {
+        chunks.push_back({0, k});
+    } else {
+        const int64_t last = std::min<int64_t>(k, bw - 1);
+        for (int64_t f = 0; f < k - last; f += bw) chunks.push_back({f, (int)std::min<int64_t>(bw, k - last - f)});
+        chunks.push_back({k - last, (int)last});
+    }
+    std::vector<double> traces(k), hd(2 + bw);
     DevBuf<double> Qu;
     Qu.alloc(ld);
-    CB_HIP(hipMemsetAsync(Qu.p, 0, ld * 8, st));
-    q_apply<double>(M, w.S.p, ld, Qu.p, ld, 1, 1.0);
-    // e = y - W s ; e_sse ; u'Qu ; traces t_i = (Q z_i)' S_u,i
-    k_w_apply<<<dim3(blocks(n), 1), 256, 0, st>>>(M->xp.p, M->xc.p, M->xv.p, M->aid.p, n, p, w.S.p, ld, w.T.p, round_up(n, 64));
-    k_axpby<<<blocks(n), 256, 0, st>>>(M->y.p, 1.0, w.T.p, -1.0, w.e.p, n);
-    CB_HIP(hipGetLastError());
-    std::vector<double> hd(r + 2);
-    {
-        // e'e over n rows: reuse col_dots with ld = n-padded by a temporary view
-        const int64_t ldn = round_up(n, 64);
-        DevBuf<double> tmp;
-        tmp.alloc(ldn);
-        CB_HIP(hipMemsetAsync(tmp.p, 0, ldn * 8, st));
-        CB_HIP(hipMemcpyAsync(tmp.p, w.e.p, n * 8, hipMemcpyDeviceToDevice, st));
-        k_col_dots<double><<<dim3(DOT_CHUNKS, 1), 256, 0, st>>>(tmp.p, tmp.p, ldn, n, w.part.p);
-        k_dots_finish<<<1, 64, 0, st>>>(w.part.p, 1, w.dots.p);
-        // u'Qu (rows p..dim of S col 0 against Qu)
-        k_col_dots<double><<<dim3(DOT_CHUNKS, 1), 256, 0, st>>>(w.S.p + p, Qu.p + p, ld, q, w.part.p);
-        k_dots_finish<<<1, 64, 0, st>>>(w.part.p, 1, w.dots.p + 1);
-        // traces: columns 1..k of S_u against Qz
-        if (k > 0) {
-            k_col_dots<double><<<dim3(DOT_CHUNKS, (unsigned)k), 256, 0, st>>>(w.S.p + ld + p, w.Qz.p + p, ld, q, w.part.p);
-            k_dots_finish<<<blocks(k, 64), 64, 0, st>>>(w.part.p, k, w.dots.p + 2);
+    for (size_t ci = 0; ci < chunks.size(); ++ci) {
+        const bool with_y = ci + 1 == chunks.size();
+        const int64_t first = chunks[ci].first;
+        const int count = chunks[ci].second;
+        const int t0 = with_y ? 1 : 0;
+        if (w.chunked) {
+            if (with_y) CB_HIP(hipMemcpyAsync(w.B.p, w.rhs.p, ld * 8, hipMemcpyDeviceToDevice, st));
+            if (count > 0) make_probes(M, o, w, first, count, t0, true);
         }
+        solve_block(M, lambda, o.solve, w.B.p, w.S.p, t0 + count);
+        // the score terms of this block ("adjoint" pass over its solutions)
+        if (with_y) {
+            CB_HIP(hipMemsetAsync(Qu.p, 0, ld * 8, st));
+            q_apply<double>(M, w.S.p, ld, Qu.p, ld, 1, 1.0);
+        }
+        const int ncol = (with_y ? 2 : 0) + count;
+        k_reml_score_terms<<<dim3(DOT_CHUNKS, (unsigned)ncol), 256, 0, st>>>(
+            M->xp.p, M->xc.p, M->xv.p, M->aid.p, M->y.p, n, p, q, w.S.p, ld, Qu.p, w.Qz.p, with_y ? 1 : 0, w.e.p,
+            w.part.p);
+        k_dots_finish<<<blocks(ncol, 64), 64, 0, st>>>(w.part.p, ncol, w.dots.p);
         CB_HIP(hipGetLastError());
-        CB_HIP(hipMemcpyAsync(hd.data(), w.dots.p, (2 + k) * 8, hipMemcpyDeviceToHost, st));
+        CB_HIP(hipMemcpyAsync(hd.data(), w.dots.p, ncol * 8, hipMemcpyDeviceToHost, st));
         CB_HIP(hipStreamSynchronize(st));
+        const double* th = hd.data() + (with_y ? 2 : 0);
+        for (int j = 0; j < count; ++j) traces[first + j] = th[j];
+        if (with_y) {
+            out.esse = hd[0];
+            out.uQu = hd[1];
+        }
     }
     double tr = 0;
-    for (int i = 0; i < k; ++i) tr += hd[2 + i];
+    for (int i = 0; i < k; ++i) tr += traces[i];
     if (!o.trace_exact && k > 0) tr /= k;
     out.vu = vu;
     out.ve = ve;
-    out.esse = hd[0];
-    out.uQu = hd[1];
     out.tr = tr;
     // score, vce.cpp:2655 and :2661-2675 (lambda*tr capped below q)
     double lambda_tr = lambda * tr;
@@ -1662,31 +1772,33 @@ static void reml_eval(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, doub
 static void reml_setup(cb_mme* M, const cb_reml_options& o, cb_reml_work& w) {
     cb_context* ctx = M->ctx;
     hipStream_t st = ctx->stream;
-    const int64_t ld = M->ldv, p = M->p, q = M->q, n = M->n;
-    const int k = (int)(o.trace_exact ? q : o.n_probes);
-    const int r = 1 + k;
-    w.B.alloc((size_t)ld * r);
-    w.S.alloc((size_t)ld * r);
-    w.Qz.alloc((size_t)ld * std::max(1, k));
+    const int64_t ld = M->ldv, n = M->n;
+    CB_REQUIRE(o.block_cols >= 0, "REML: block_cols must be >= 0");
+    w.k = (int)(o.trace_exact ? M->q : o.n_probes);
+    const int r = 1 + w.k;
+    w.bw = reml_block_width(M, o, r);
+    w.chunked = w.bw < r;
+    const int bw = w.bw;
+    w.B.alloc((size_t)ld * bw);
+    w.S.alloc((size_t)ld * bw);
+    w.Qz.alloc((size_t)ld * std::max(1, std::min(bw, w.k)));
+    w.rhs.alloc(ld);
     w.B2.alloc((size_t)ld * 2);
     w.S2.alloc((size_t)ld * 2);
     const int64_t ldn = round_up(n, 64);
     w.V.alloc((size_t)ldn * 2);
     w.T.alloc((size_t)ldn * 2);
     w.e.alloc(ldn);
-    w.part.alloc((size_t)DOT_CHUNKS * std::max(2, r));
-    w.dots.alloc(2 + r);
+    w.part.alloc((size_t)DOT_CHUNKS * (bw + 1));
+    w.dots.alloc(bw + 1);
     CB_HIP(hipMemsetAsync(w.B.p, 0, w.B.bytes(), st));
     CB_HIP(hipMemsetAsync(w.Qz.p, 0, w.Qz.bytes(), st));
     CB_HIP(hipMemsetAsync(w.B2.p, 0, w.B2.bytes(), st));
-    CB_HIP(hipMemcpyAsync(w.B.p, M->rhs.data(), M->dim * 8, hipMemcpyHostToDevice, st));
-    if (k > 0) {
-        k_probes<<<dim3(blocks(q), (unsigned)k), 256, 0, st>>>(w.B.p + ld, ld, p, q, k, o.seed, o.trace_exact, 0);
-        CB_HIP(hipGetLastError());
-        // Qz (fixed for the run): Q applied to the probe columns
-        q_apply<double>(M, w.B.p + ld, ld, w.Qz.p, ld, k, 1.0);
-        CB_HIP(hipGetLastError());
-    }
+    CB_HIP(hipMemsetAsync(w.rhs.p, 0, w.rhs.bytes(), st));
+    CB_HIP(hipMemcpyAsync(w.rhs.p, M->rhs.data(), M->dim * 8, hipMemcpyHostToDevice, st));
+    CB_HIP(hipMemcpyAsync(w.B.p, w.rhs.p, ld * 8, hipMemcpyDeviceToDevice, st));
+    // one block: the probes and Q z (fixed for the run) are made once
+    if (!w.chunked && w.k > 0) make_probes(M, o, w, 0, w.k, 1, true);
     CB_HIP(hipStreamSynchronize(st));
 }
 
@@ -1712,6 +1824,7 @@ void cb_reml_options_default(cb_reml_options* o) {
     o->seed = 42;
     o->trace_exact = 0;
     cb_solve_options_default(&o->solve);
+    o->block_cols = 0;
 }
 
 int cb_mme_create(cb_context* ctx, int64_t n, int64_t p, int64_t q, const int64_t* x_ptr,
@@ -2447,6 +2560,7 @@ int cb_reml_run(cb_mme* M, const cb_reml_options* opts, double var_u0, double va
         if (solution) {
             // final solve at the estimates (GSSolve uses lambda = ve/vu)
             const double lambda = ve / std::max(vu, 1e-9);
+            CB_HIP(hipMemcpyAsync(w.B.p, w.rhs.p, M->ldv * 8, hipMemcpyDeviceToDevice, M->ctx->stream));
             solve_block(M, lambda, o.solve, w.B.p, w.S.p, 1);
             CB_HIP(hipMemcpy(solution, w.S.p, M->dim * 8, hipMemcpyDeviceToHost));
         }
@@ -2512,8 +2626,14 @@ int cb_reml_copy_probes(cb_mme* M, const cb_reml_options* opts, double* out, int
         set_device(M->ctx);
         cb_reml_work w;
         reml_setup(M, *opts, w);
-        const int k = (int)(opts->trace_exact ? M->q : opts->n_probes);
-        CB_HIP(hipMemcpy2D(out, ld * 8, w.B.p + M->ldv + M->p, M->ldv * 8, M->q * 8, k, hipMemcpyDeviceToHost));
+        // probes bw at a time through B's columns (the run's own stream)
+        for (int64_t f = 0; f < w.k; f += w.bw) {
+            const int cnt = (int)std::min<int64_t>(w.bw, w.k - f);
+            make_probes(M, *opts, w, f, cnt, 0, false);
+            CB_HIP(hipMemcpy2DAsync(out + f * ld, ld * 8, w.B.p + M->p, M->ldv * 8, M->q * 8, cnt,
+                                    hipMemcpyDeviceToHost, M->ctx->stream));
+            CB_HIP(hipStreamSynchronize(M->ctx->stream));
+        }
     });
 }
 

@@ -30,11 +30,13 @@ def solve_options(precision=_lib.CB_PREC_MIXED, precond=_lib.CB_PRECOND_BLOCK_JA
 
 
 def reml_options(algorithm=_lib.CB_VCE_AI, max_iter=20, tol=1e-4, n_probes=64, seed=42,
-                 trace_exact=False, solve: _lib.SolveOptions | None = None) -> _lib.RemlOptions:
+                 trace_exact=False, solve: _lib.SolveOptions | None = None,
+                 block_cols: int = 0) -> _lib.RemlOptions:
     o = _lib.RemlOptions()
     _lib.lib().cb_reml_options_default(C.byref(o))
     o.algorithm, o.max_iter, o.tol, o.n_probes, o.seed = algorithm, max_iter, tol, n_probes, seed
     o.trace_exact = int(trace_exact)
+    o.block_cols = block_cols
     if solve is not None:
         o.solve = solve
     return o

@@ -229,6 +229,11 @@ typedef struct {
     uint64_t seed;      /* probe stream: sign of splitmix64(seed + g*(c*q + j + 1)), g = 0x9E3779B97F4A7C15 */
     int trace_exact;    /* 1: probes are the q unit vectors (exact trace) */
     cb_solve_options solve;
+    int64_t block_cols; /* right-hand sides per block solve (y + probes); 0 (default): as many as
+                           fit in half the free device memory. A smaller block takes the probes in
+                           chunks whose vectors and Q z are regenerated at each evaluation rather
+                           than stored for the run; the estimator is unchanged (the reference
+                           solves probes in batches too, adaptive_trace_estimator.cpp:11-83) */
 } cb_reml_options;
 
 void cb_reml_options_default(cb_reml_options* opts);

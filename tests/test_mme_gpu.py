@@ -131,6 +131,67 @@ def test_score_and_ai_match_oracle(cb, ctx, precision):
     assert ebv_err(sol[M.p:], ref["sol"][M.p:]) < EBV_TOL
 
 
+@pytest.mark.parametrize("qkind", ["dense", "csr"])
+def test_probe_chunks_match_one_block(cb, ctx, qkind):
+    """A block too narrow for y and every probe takes the probes in chunks,
+    regenerating them and their Q z at each evaluation: the same score, AI
+    and trace as one block, to the solver's tolerance, against the oracle
+    on the same probes; probes copied out chunk by chunk are the same."""
+    if qkind == "dense":
+        fd, aid, y, Q = gblup_problem()
+        M = mme.MixedModel(fd, aid, y, Q.shape[0])
+        M.set_qinv_dense(Q)
+    else:
+        from cosmicblup_amd.pedigree import invertA_henderson
+
+        dam, sire = synth.simulate_pedigree(4, 100, seed=21)
+        A = invertA_henderson(dam, sire)
+        rng = np.random.default_rng(8)
+        aid = np.sort(rng.choice(dam.size, 300, replace=False)).astype(np.int32)
+        y = rng.standard_normal(aid.size) + 3.0
+        fd = design.build_fixed_design([[f"g{int(v)}"] for v in rng.integers(0, 5, aid.size)])
+        M = mme.MixedModel(fd, aid, y, dam.size)
+        M.set_qinv_csr(A.ptr, A.col, A.val)
+    solve = mme.solve_options(precision=_lib.CB_PREC_FP64, tol=1e-12)
+    one = mme.AIREML(M, mme.reml_options(n_probes=21, seed=5, solve=solve))
+    g1, ai1, tr1, sol1 = one.score(0.6, 0.9)
+    for bc in (2, 5, 8):
+        ch = mme.AIREML(M, mme.reml_options(n_probes=21, seed=5, solve=solve, block_cols=bc))
+        assert np.array_equal(ch.probes(), one.probes())
+        g, ai, tr, sol = ch.score(0.6, 0.9)
+        np.testing.assert_allclose(tr, tr1, rtol=1e-9)
+        np.testing.assert_allclose(g, g1, rtol=1e-8, atol=1e-10 * np.abs(g1).max())
+        np.testing.assert_allclose(ai, ai1, rtol=1e-9)
+        assert ebv_err(sol[M.p:], sol1[M.p:]) < 1e-9
+    if qkind == "dense":
+        ref = orl.reml_eval(design.dense_x(fd), aid, y, Q, 0.6, 0.9, orl.probes(M.q, 21, 5))
+        np.testing.assert_allclose(tr1, ref["tr"], rtol=1e-8)
+        np.testing.assert_allclose(g1, ref["grad"], rtol=1e-7, atol=1e-9 * np.abs(ref["grad"]).max())
+
+
+def test_exact_trace_in_chunks_and_run(cb, ctx):
+    """Exact trace (q unit probes) through 7-column chunks equals the
+    one-block exact trace; a chunked AI-REML run reaches the same estimates."""
+    fd, aid, y, Q = gblup_problem(n=120, m=600, ngroups=3)
+    M = mme.MixedModel(fd, aid, y, Q.shape[0])
+    M.set_qinv_dense(Q)
+    solve = mme.solve_options(precision=_lib.CB_PREC_FP64, tol=1e-12)
+    g1, _, tr1, _ = mme.AIREML(M, mme.reml_options(trace_exact=True, solve=solve)).score(0.45, 0.7, want_ai=False)
+    g2, _, tr2, _ = mme.AIREML(M, mme.reml_options(trace_exact=True, solve=solve, block_cols=7)).score(
+        0.45, 0.7, want_ai=False)
+    np.testing.assert_allclose(tr2, tr1, rtol=1e-9)
+    np.testing.assert_allclose(g2, g1, rtol=1e-8, atol=1e-10 * np.abs(g1).max())
+    runs = []
+    for bc in (0, 6):
+        R = mme.AIREML(M, mme.reml_options(n_probes=16, seed=3, solve=solve, block_cols=bc))
+        R.setInitialVars(0.5, 0.5)
+        R.solve()
+        runs.append(R)
+    assert runs[0].iterations == runs[1].iterations
+    np.testing.assert_allclose([runs[1].var_u, runs[1].var_e], [runs[0].var_u, runs[0].var_e], rtol=1e-8)
+    assert ebv_err(runs[1].solution[M.p:], runs[0].solution[M.p:]) < 1e-8
+
+
 def test_exact_trace_score_matches_fd_of_logL(cb, ctx):
     fd, aid, y, Q = gblup_problem(n=120, m=600, ngroups=3)
     M = mme.MixedModel(fd, aid, y, Q.shape[0])
