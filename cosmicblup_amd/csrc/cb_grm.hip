@@ -1025,10 +1025,10 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
     std::vector<uint8_t> hvalid(m);
     std::vector<float> hmean(m), h2pq(m);
     std::vector<int32_t> hnobs(m);
-    CB_HIP(hipMemcpyAsync(hvalid.data(), grm->valid.p, m, hipMemcpyDeviceToHost, st));
-    CB_HIP(hipMemcpyAsync(hmean.data(), grm->mean.p, m * 4, hipMemcpyDeviceToHost, st));
-    CB_HIP(hipMemcpyAsync(h2pq.data(), grm->twopq.p, m * 4, hipMemcpyDeviceToHost, st));
-    CB_HIP(hipMemcpyAsync(hnobs.data(), grm->nobs.p, m * 4, hipMemcpyDeviceToHost, st));
+    copy_host(st, hvalid.data(), grm->valid.p, m, hipMemcpyDeviceToHost);
+    copy_host(st, hmean.data(), grm->mean.p, m * 4, hipMemcpyDeviceToHost);
+    copy_host(st, h2pq.data(), grm->twopq.p, m * 4, hipMemcpyDeviceToHost);
+    copy_host(st, hnobs.data(), grm->nobs.p, m * 4, hipMemcpyDeviceToHost);
     CB_HIP(hipStreamSynchronize(st));
 
     // sum(2pq) in the reference's order: double sum of float 2pq per SNP
@@ -1076,7 +1076,7 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
         for (int64_t j = 0; j < m; ++j) snp_off[j + 1] = snp_off[j] + (hvalid[j] ? n - hnobs[j] : 0);
         DevBuf<int64_t> d_snp_off;
         d_snp_off.alloc(m + 1);
-        CB_HIP(hipMemcpyAsync(d_snp_off.p, snp_off.data(), (m + 1) * 8, hipMemcpyHostToDevice, st));
+        copy_host(st, d_snp_off.p, snp_off.data(), (m + 1) * 8, hipMemcpyHostToDevice);
         DevBuf<int32_t> snp_list, smp_cnt;
         snp_list.alloc(std::max<int64_t>(1, nmiss_total));
         smp_cnt.alloc(n);
@@ -1086,13 +1086,13 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
                 g->bed.p, g->pitch, n, m, grm->valid.p, d_snp_off.p, snp_list.p, smp_cnt.p);
         });
         std::vector<int32_t> hcnt(n);
-        CB_HIP(hipMemcpyAsync(hcnt.data(), smp_cnt.p, n * 4, hipMemcpyDeviceToHost, st));
+        copy_host(st, hcnt.data(), smp_cnt.p, n * 4, hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
         std::vector<int64_t> smp_off(n + 1, 0);
         for (int64_t i = 0; i < n; ++i) smp_off[i + 1] = smp_off[i] + round_up(hcnt[i], 8);
         DevBuf<int64_t> d_smp_off;
         d_smp_off.alloc(n + 1);
-        CB_HIP(hipMemcpyAsync(d_smp_off.p, smp_off.data(), (n + 1) * 8, hipMemcpyHostToDevice, st));
+        copy_host(st, d_smp_off.p, smp_off.data(), (n + 1) * 8, hipMemcpyHostToDevice);
         const int64_t nlist = std::max<int64_t>(8, smp_off[n]);
         DevBuf<int32_t> lj, cursor;
         DevBuf<uint32_t> lk;
@@ -1115,8 +1115,8 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
         // u_i += sum_{j miss i} mu_j^2
         std::vector<unsigned long long> hsq(2 * n);
         std::vector<double> hu(n);
-        CB_HIP(hipMemcpyAsync(hsq.data(), sq.p, 2 * n * 8, hipMemcpyDeviceToHost, st));
-        CB_HIP(hipMemcpyAsync(hu.data(), u.p, n * 8, hipMemcpyDeviceToHost, st));
+        copy_host(st, hsq.data(), sq.p, 2 * n * 8, hipMemcpyDeviceToHost);
+        copy_host(st, hu.data(), u.p, n * 8, hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
         // s_i = sum_{j miss i} mu_j^2, an exact integer in 2^-52 units, converted once
         std::vector<double> hsqd(n);
@@ -1127,8 +1127,8 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
         }
         DevBuf<double> sqd;
         sqd.alloc(n);
-        CB_HIP(hipMemcpyAsync(sqd.p, hsqd.data(), n * 8, hipMemcpyHostToDevice, st));
-        CB_HIP(hipMemcpyAsync(u.p, hu.data(), n * 8, hipMemcpyHostToDevice, st));
+        copy_host(st, sqd.p, hsqd.data(), n * 8, hipMemcpyHostToDevice);
+        copy_host(st, u.p, hu.data(), n * 8, hipMemcpyHostToDevice);
 
         E.alloc((size_t)n * n);
         DevBuf<uint8_t> bedd;
@@ -1233,8 +1233,7 @@ static void copy_out(const cb_grm* grm, const void* src, size_t esz, void* dst, 
     CB_REQUIRE(src, "requested GRM output was not computed");
     CB_REQUIRE(dst && ld >= grm->n, "bad destination / leading dimension");
     CB_HIP(hipSetDevice(grm->ctx->device));
-    CB_HIP(hipMemcpy2DAsync(dst, ld * esz, src, grm->n * esz, grm->n * esz, grm->n,
-                            hipMemcpyDeviceToHost, grm->ctx->stream));
+    copy_host_2d(grm->ctx->stream, dst, ld * esz, src, grm->n * esz, grm->n * esz, grm->n, hipMemcpyDeviceToHost);
     CB_HIP(hipStreamSynchronize(grm->ctx->stream));
 }
 
@@ -1296,7 +1295,7 @@ int cb_grm_copy_allele_freqs(const cb_grm* grm, float* out, int64_t m) {
     return guarded([&] {
         CB_REQUIRE(grm && out && m == grm->m && grm->freq.p, "cb_grm_copy_allele_freqs: bad argument");
         CB_HIP(hipSetDevice(grm->ctx->device));
-        CB_HIP(hipMemcpy(out, grm->freq.p, m * 4, hipMemcpyDeviceToHost));
+        copy_host(grm->ctx->stream, out, grm->freq.p, m * 4, hipMemcpyDeviceToHost);
     });
 }
 
@@ -1306,14 +1305,14 @@ int cb_grm_copy_snp_stats(const cb_grm* grm, int64_t m, int32_t* n_obs, int32_t*
     return guarded([&] {
         CB_REQUIRE(grm && m == grm->m && grm->nobs.p, "cb_grm_copy_snp_stats: bad argument");
         CB_HIP(hipSetDevice(grm->ctx->device));
-        if (n_obs) CB_HIP(hipMemcpy(n_obs, grm->nobs.p, m * 4, hipMemcpyDeviceToHost));
-        if (sum_x) CB_HIP(hipMemcpy(sum_x, grm->sumx.p, m * 4, hipMemcpyDeviceToHost));
-        if (n_hom1) CB_HIP(hipMemcpy(n_hom1, grm->h1.p, m * 4, hipMemcpyDeviceToHost));
-        if (n_het) CB_HIP(hipMemcpy(n_het, grm->het.p, m * 4, hipMemcpyDeviceToHost));
-        if (n_hom2) CB_HIP(hipMemcpy(n_hom2, grm->h2.p, m * 4, hipMemcpyDeviceToHost));
-        if (valid) CB_HIP(hipMemcpy(valid, grm->valid.p, m, hipMemcpyDeviceToHost));
-        if (mean_f) CB_HIP(hipMemcpy(mean_f, grm->mean.p, m * 4, hipMemcpyDeviceToHost));
-        if (twopq_f) CB_HIP(hipMemcpy(twopq_f, grm->twopq.p, m * 4, hipMemcpyDeviceToHost));
+        if (n_obs) copy_host(grm->ctx->stream, n_obs, grm->nobs.p, m * 4, hipMemcpyDeviceToHost);
+        if (sum_x) copy_host(grm->ctx->stream, sum_x, grm->sumx.p, m * 4, hipMemcpyDeviceToHost);
+        if (n_hom1) copy_host(grm->ctx->stream, n_hom1, grm->h1.p, m * 4, hipMemcpyDeviceToHost);
+        if (n_het) copy_host(grm->ctx->stream, n_het, grm->het.p, m * 4, hipMemcpyDeviceToHost);
+        if (n_hom2) copy_host(grm->ctx->stream, n_hom2, grm->h2.p, m * 4, hipMemcpyDeviceToHost);
+        if (valid) copy_host(grm->ctx->stream, valid, grm->valid.p, m, hipMemcpyDeviceToHost);
+        if (mean_f) copy_host(grm->ctx->stream, mean_f, grm->mean.p, m * 4, hipMemcpyDeviceToHost);
+        if (twopq_f) copy_host(grm->ctx->stream, twopq_f, grm->twopq.p, m * 4, hipMemcpyDeviceToHost);
     });
 }
 

@@ -177,6 +177,16 @@ struct LaunchGraph {
 inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Copies between the device and host memory the library does not own
+// (pageable), ordered on stream st: the data go through a pinned staging
+// buffer of the library's, so that every transfer is a DMA queued on st
+// behind the work before it, never a copy the runtime makes on the host at
+// the call. Both return once the copy has completed. A device-to-device copy
+// is queued as it is.
+void copy_host(hipStream_t st, void* dst, const void* src, size_t bytes, hipMemcpyKind kind);
+void copy_host_2d(hipStream_t st, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
+                  size_t rows, hipMemcpyKind kind);
+
 // Shared between translation units (defined in cb_linalg.hip / cb_pedigree.hip).
 // Symmetric n x n FP64 matrix, row-major with leading dimension ld, in device
 // (on_device) or host memory, to `path`: format 0 text triplets, 1

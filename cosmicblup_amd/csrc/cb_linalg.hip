@@ -64,7 +64,7 @@ void invert_psd(cb_context* ctx, const double* dM, int64_t ldm, int64_t n, doubl
     e.alloc(n);
     info.alloc(1);
     // column-major n x n copy (M is symmetric, so row- or column-major input is the same matrix)
-    CB_HIP(hipMemcpy2DAsync(U.p, n * 8, dM, ldm * 8, n * 8, n, in_kind, st));
+    copy_host_2d(st, U.p, n * 8, dM, ldm * 8, n * 8, n, in_kind);
     if (ridge > 0.0) k_add_diag<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(U.p, n, n, ridge);
     CB_HIP(hipGetLastError());
     Handle h;
@@ -74,9 +74,9 @@ void invert_psd(cb_context* ctx, const double* dM, int64_t ldm, int64_t n, doubl
                                 (rocblas_int)n, l.p, e.p, info.p));
     int hinfo = 0;
     double lo = 0, hi = 0;
-    CB_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    CB_HIP(hipMemcpyAsync(&lo, l.p, 8, hipMemcpyDeviceToHost, st));
-    CB_HIP(hipMemcpyAsync(&hi, l.p + (n - 1), 8, hipMemcpyDeviceToHost, st));
+    copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
+    copy_host(st, &lo, l.p, 8, hipMemcpyDeviceToHost);
+    copy_host(st, &hi, l.p + (n - 1), 8, hipMemcpyDeviceToHost);
     CB_HIP(hipStreamSynchronize(st));
     CB_REQUIRE(hinfo == 0, "cb_invert_psd: eigensolver did not converge");
     if (min_eig) *min_eig = lo;
@@ -98,7 +98,7 @@ void invert_psd(cb_context* ctx, const double* dM, int64_t ldm, int64_t n, doubl
                              (rocblas_int)n, (rocblas_int)n, &one, V.p, (rocblas_int)n, U.p, (rocblas_int)n, &zero,
                              dst, (rocblas_int)ldd));
     if (out_kind != hipMemcpyDeviceToDevice)
-        CB_HIP(hipMemcpy2DAsync(dOut, ldo * 8, O.p, n * 8, n * 8, n, out_kind, st));
+        copy_host_2d(st, dOut, ldo * 8, O.p, n * 8, n * 8, n, out_kind);
     CB_HIP(hipStreamSynchronize(st));
 }
 

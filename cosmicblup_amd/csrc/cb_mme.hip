@@ -1028,7 +1028,7 @@ void build_precond(cb_mme* M, double lambda, bool block) {
     std::vector<double> hv;
     if (M->stiff) {
         hv.resize(M->q);
-        CB_HIP(hipMemcpy(hv.data(), M->stiff_v.p, M->q * 8, hipMemcpyDeviceToHost));
+        copy_host(M->ctx->stream, hv.data(), M->stiff_v.p, M->q * 8, hipMemcpyDeviceToHost);
     }
     for (int64_t i = 0; i < dim; ++i) {
         double d = M->wtw_diag[i] + (i >= p ? lambda * M->q_diag[i - p] : 0.0);
@@ -1046,7 +1046,7 @@ void build_precond(cb_mme* M, double lambda, bool block) {
         }
         M->stiff_gamma = 1.0 / (1.0 / (lambda * M->stiff_mu) + vy);
         M->stiff_y.alloc(M->q);
-        CB_HIP(hipMemcpy(M->stiff_y.p, y.data(), M->q * 8, hipMemcpyHostToDevice));
+        copy_host(M->ctx->stream, M->stiff_y.p, y.data(), M->q * 8, hipMemcpyHostToDevice);
     }
     std::vector<double> binv((size_t)std::max<int64_t>(1, p * p), 0.0);
     if (p > 0) {
@@ -1100,8 +1100,8 @@ void build_precond(cb_mme* M, double lambda, bool block) {
     }
     M->Binv.alloc(binv.size());
     M->dinv.alloc(dim);
-    CB_HIP(hipMemcpyAsync(M->Binv.p, binv.data(), binv.size() * 8, hipMemcpyHostToDevice, M->ctx->stream));
-    CB_HIP(hipMemcpyAsync(M->dinv.p, dinv.data(), dim * 8, hipMemcpyHostToDevice, M->ctx->stream));
+    copy_host(M->ctx->stream, M->Binv.p, binv.data(), binv.size() * 8, hipMemcpyHostToDevice);
+    copy_host(M->ctx->stream, M->dinv.p, dinv.data(), dim * 8, hipMemcpyHostToDevice);
     CB_HIP(hipStreamSynchronize(M->ctx->stream));
     M->dinv_lambda = lambda;
     M->dinv_block = (int)block;
@@ -1252,8 +1252,8 @@ int64_t pcg_block(cb_mme* M, double lambda, const T* B, TX* X, int r, double tol
     col_dots<T>(M, R.p, Z.p, r, part.p, d1.p);
     col_dots<T>(M, B, B, r, part.p, d2.p);
     std::vector<double> rz(r), bb(r);
-    CB_HIP(hipMemcpyAsync(rz.data(), d1.p, r * 8, hipMemcpyDeviceToHost, st));
-    CB_HIP(hipMemcpyAsync(bb.data(), d2.p, r * 8, hipMemcpyDeviceToHost, st));
+    copy_host(st, rz.data(), d1.p, r * 8, hipMemcpyDeviceToHost);
+    copy_host(st, bb.data(), d2.p, r * 8, hipMemcpyDeviceToHost);
     CB_HIP(hipStreamSynchronize(st));
     std::vector<ColState> hs(r);
     for (int c = 0; c < r; ++c) {
@@ -1265,7 +1265,7 @@ int64_t pcg_block(cb_mme* M, double lambda, const T* B, TX* X, int r, double tol
     // initial residual check (a zero RHS column is solved by x = 0)
     col_dots<T>(M, R.p, R.p, r, part.p, d1.p);
     std::vector<double> rr0(r);
-    CB_HIP(hipMemcpyAsync(rr0.data(), d1.p, r * 8, hipMemcpyDeviceToHost, st));
+    copy_host(st, rr0.data(), d1.p, r * 8, hipMemcpyDeviceToHost);
     CB_HIP(hipStreamSynchronize(st));
     const double tol2 = tol * tol;
     for (int c = 0; c < r; ++c) {
@@ -1273,7 +1273,7 @@ int64_t pcg_block(cb_mme* M, double lambda, const T* B, TX* X, int r, double tol
         hs[c].rr = rr0[c];
         if (!(rr0[c] >= thr)) hs[c].active = 0;
     }
-    CB_HIP(hipMemcpyAsync(cs.p, hs.data(), r * sizeof(ColState), hipMemcpyHostToDevice, st));
+    copy_host(st, cs.p, hs.data(), r * sizeof(ColState), hipMemcpyHostToDevice);
     int64_t it = 0;
     const int check_every = 8;
     bool any_active = std::any_of(hs.begin(), hs.end(), [](const ColState& s) { return s.active; });
@@ -1301,12 +1301,12 @@ int64_t pcg_block(cb_mme* M, double lambda, const T* B, TX* X, int r, double tol
             ++it;
         }
         if (it % check_every == 0 || it == max_iter) {
-            CB_HIP(hipMemcpyAsync(hs.data(), cs.p, r * sizeof(ColState), hipMemcpyDeviceToHost, st));
+            copy_host(st, hs.data(), cs.p, r * sizeof(ColState), hipMemcpyDeviceToHost);
             CB_HIP(hipStreamSynchronize(st));
             any_active = std::any_of(hs.begin(), hs.end(), [](const ColState& s) { return s.active; });
         }
     }
-    CB_HIP(hipMemcpyAsync(hs.data(), cs.p, r * sizeof(ColState), hipMemcpyDeviceToHost, st));
+    copy_host(st, hs.data(), cs.p, r * sizeof(ColState), hipMemcpyDeviceToHost);
     CB_HIP(hipStreamSynchronize(st));
     if (rel_out) {
         rel_out->resize(r);
@@ -1357,7 +1357,7 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
     std::vector<std::vector<double>> hist(r);  // true residual at each replacement
     std::vector<ColState> hs(r);
     col_dots<double>(M, B64, B64, r, part.p, d1.p);
-    CB_HIP(hipMemcpyAsync(bb.data(), d1.p, r * 8, hipMemcpyDeviceToHost, st));
+    copy_host(st, bb.data(), d1.p, r * 8, hipMemcpyDeviceToHost);
     CB_HIP(hipStreamSynchronize(st));
     for (int c = 0; c < r; ++c) bb[c] = bb[c] == 0.0 ? 1.0 : bb[c];
     std::vector<int> done(r, 0);  // true-residual convergence per column
@@ -1374,9 +1374,9 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
         apply_precond<float>(M, R.p, Z.p, r);
         col_dots<double>(M, R64.p, R64.p, r, part.p, d1.p);
         col_dots<float>(M, R.p, Z.p, r, part.p, d2.p);
-        CB_HIP(hipMemcpyAsync(rr.data(), d1.p, r * 8, hipMemcpyDeviceToHost, st));
-        CB_HIP(hipMemcpyAsync(rz.data(), d2.p, r * 8, hipMemcpyDeviceToHost, st));
-        if (!first) CB_HIP(hipMemcpyAsync(hs.data(), cs.p, r * sizeof(ColState), hipMemcpyDeviceToHost, st));
+        copy_host(st, rr.data(), d1.p, r * 8, hipMemcpyDeviceToHost);
+        copy_host(st, rz.data(), d2.p, r * 8, hipMemcpyDeviceToHost);
+        if (!first) copy_host(st, hs.data(), cs.p, r * sizeof(ColState), hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
         bool all = true;
         for (int c = 0; c < r; ++c) {
@@ -1413,7 +1413,7 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
                                       hipMemcpyDeviceToDevice, st));
             all = all && done[c];
         }
-        CB_HIP(hipMemcpyAsync(cs.p, hs.data(), r * sizeof(ColState), hipMemcpyHostToDevice, st));
+        copy_host(st, cs.p, hs.data(), r * sizeof(ColState), hipMemcpyHostToDevice);
         return all;
     };
     bool all_done = replace(true);
@@ -1445,7 +1445,7 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
             ++it;
         }
         if (it % check_every == 0 || it == o.max_iter) {
-            CB_HIP(hipMemcpyAsync(hs.data(), cs.p, r * sizeof(ColState), hipMemcpyDeviceToHost, st));
+            copy_host(st, hs.data(), cs.p, r * sizeof(ColState), hipMemcpyDeviceToHost);
             CB_HIP(hipStreamSynchronize(st));
             bool need = false;
             for (int c = 0; c < r; ++c) {
@@ -1589,7 +1589,7 @@ static void assemble(cb_mme* M, const int64_t* x_ptr, const int32_t* x_col, cons
     auto up = [&](auto& buf, const auto& vec) {
         using E = std::remove_reference_t<decltype(*buf.p)>;
         buf.alloc(std::max<size_t>(1, vec.size()));
-        if (!vec.empty()) CB_HIP(hipMemcpyAsync(buf.p, vec.data(), vec.size() * sizeof(E), hipMemcpyHostToDevice, st));
+        if (!vec.empty()) copy_host(st, buf.p, vec.data(), vec.size() * sizeof(E), hipMemcpyHostToDevice);
     };
     std::vector<int64_t> xp(x_ptr, x_ptr + n + 1);
     std::vector<int32_t> xc(x_col, x_col + x_ptr[n]), av(aid, aid + n);
@@ -1723,7 +1723,7 @@ static void reml_eval(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, doub
             w.part.p);
         k_dots_finish<<<blocks(ncol, 64), 64, 0, st>>>(w.part.p, ncol, w.dots.p);
         CB_HIP(hipGetLastError());
-        CB_HIP(hipMemcpyAsync(hd.data(), w.dots.p, ncol * 8, hipMemcpyDeviceToHost, st));
+        copy_host(st, hd.data(), w.dots.p, ncol * 8, hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
         const double* th = hd.data() + (with_y ? 2 : 0);
         for (int j = 0; j < count; ++j) traces[first + j] = th[j];
@@ -1743,7 +1743,7 @@ static void reml_eval(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, doub
     if (lambda_tr > q - 1e-6) lambda_tr = q - 1e-6;
     out.grad[0] = 0.5 * (out.uQu + tr * ve - q * vu) / (vu * vu);
     out.grad[1] = 0.5 * (out.esse - ve * ((double)n - p - q + lambda_tr)) / (ve * ve);
-    if (sol_out) CB_HIP(hipMemcpy(sol_out, w.S.p, dim * 8, hipMemcpyDeviceToHost));
+    if (sol_out) copy_host(st, sol_out, w.S.p, dim * 8, hipMemcpyDeviceToHost);
     if (!want_ai) return;
     // AI matrix, vce.cpp:2596-2645: v_u = Z u / vu, v_e = e / ve; P v = (v - W C^-1 W'v) / ve
     const int64_t ldn = round_up(n, 64);
@@ -1753,8 +1753,8 @@ static void reml_eval(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, doub
     solve_block(M, lambda, o.solve, w.B2.p, w.S2.p, 2);
     k_w_apply<<<dim3(blocks(n), 2), 256, 0, st>>>(M->xp.p, M->xc.p, M->xv.p, M->aid.p, n, p, w.S2.p, ld, w.T.p, ldn);
     std::vector<double> hV((size_t)ldn * 2), hT((size_t)ldn * 2);
-    CB_HIP(hipMemcpyAsync(hV.data(), w.V.p, ldn * 2 * 8, hipMemcpyDeviceToHost, st));
-    CB_HIP(hipMemcpyAsync(hT.data(), w.T.p, ldn * 2 * 8, hipMemcpyDeviceToHost, st));
+    copy_host(st, hV.data(), w.V.p, ldn * 2 * 8, hipMemcpyDeviceToHost);
+    copy_host(st, hT.data(), w.T.p, ldn * 2 * 8, hipMemcpyDeviceToHost);
     CB_HIP(hipStreamSynchronize(st));
     double a_uu = 0, a_ue = 0, a_ee = 0;
     for (int64_t i = 0; i < n; ++i) {
@@ -1795,7 +1795,7 @@ static void reml_setup(cb_mme* M, const cb_reml_options& o, cb_reml_work& w) {
     CB_HIP(hipMemsetAsync(w.Qz.p, 0, w.Qz.bytes(), st));
     CB_HIP(hipMemsetAsync(w.B2.p, 0, w.B2.bytes(), st));
     CB_HIP(hipMemsetAsync(w.rhs.p, 0, w.rhs.bytes(), st));
-    CB_HIP(hipMemcpyAsync(w.rhs.p, M->rhs.data(), M->dim * 8, hipMemcpyHostToDevice, st));
+    copy_host(st, w.rhs.p, M->rhs.data(), M->dim * 8, hipMemcpyHostToDevice);
     CB_HIP(hipMemcpyAsync(w.B.p, w.rhs.p, ld * 8, hipMemcpyDeviceToDevice, st));
     // one block: the probes and Q z (fixed for the run) are made once
     if (!w.chunked && w.k > 0) make_probes(M, o, w, 0, w.k, 1, true);
@@ -1880,12 +1880,12 @@ static void find_stiff_direction(cb_mme* M) {
     dy.alloc(q);
     double mu = 0, mu_prev = 0;
     for (int it = 0; it < 100; ++it) {
-        CB_HIP(hipMemcpyAsync(dv.p, v.data(), q * 8, hipMemcpyHostToDevice, st));
+        copy_host(st, dv.p, v.data(), q * 8, hipMemcpyHostToDevice);
         CB_HIP(hipMemsetAsync(dy.p, 0, q * 8, st));
         k_gemv<double><<<(unsigned)ceil_div(q, 4 * GEMV_ROWS), 256, 0, st>>>(M->Qd64.p, M->ldq, q, dv.p, q, 1, dy.p, q,
                                                                              0, 1.0);
         CB_HIP(hipGetLastError());
-        CB_HIP(hipMemcpyAsync(y.data(), dy.p, q * 8, hipMemcpyDeviceToHost, st));
+        copy_host(st, y.data(), dy.p, q * 8, hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
         double vy = 0, yy = 0;
         for (int64_t i = 0; i < q; ++i) { vy += v[i] * y[i]; yy += y[i] * y[i]; }
@@ -1903,7 +1903,7 @@ static void find_stiff_direction(cb_mme* M) {
     M->stiff = true;
     M->stiff_mu = mu;
     M->stiff_v.alloc(q);
-    CB_HIP(hipMemcpy(M->stiff_v.p, v.data(), q * 8, hipMemcpyHostToDevice));
+    copy_host(M->ctx->stream, M->stiff_v.p, v.data(), q * 8, hipMemcpyHostToDevice);
     // FP32 copy of the bulk only: Q32 = fp32(Q - mu v v'); the rank-1 part is
     // applied separately with FP64 accumulation (apply_C<float>)
     k_q32_bulk<<<blocks(q * M->ldq), 256, 0, st>>>(M->Qd64.p, M->Qd32.p, M->stiff_v.p, mu, q, M->ldq);
@@ -1926,9 +1926,9 @@ static void set_q_csr_common(cb_mme* M, const int64_t* ptr, const int32_t* col, 
     M->qc.alloc(std::max<int64_t>(1, nnz));
     M->qv64.alloc(std::max<int64_t>(1, nnz));
     M->qv32.alloc(std::max<int64_t>(1, nnz));
-    CB_HIP(hipMemcpyAsync(M->qp.p, ptr, (q + 1) * 8, hipMemcpyHostToDevice, st));
-    CB_HIP(hipMemcpyAsync(M->qc.p, col, nnz * 4, hipMemcpyHostToDevice, st));
-    CB_HIP(hipMemcpyAsync(M->qv64.p, val, nnz * 8, hipMemcpyHostToDevice, st));
+    copy_host(st, M->qp.p, ptr, (q + 1) * 8, hipMemcpyHostToDevice);
+    copy_host(st, M->qc.p, col, nnz * 4, hipMemcpyHostToDevice);
+    copy_host(st, M->qv64.p, val, nnz * 8, hipMemcpyHostToDevice);
     k_convert<float, double><<<blocks(nnz), 256, 0, st>>>(M->qv64.p, M->qv32.p, nnz);
     CB_HIP(hipGetLastError());
     CB_HIP(hipStreamSynchronize(st));
@@ -1951,14 +1951,14 @@ static void set_q_dense_common(cb_mme* M, const double* Q, int64_t ld, hipMemcpy
     M->Qd64.alloc((size_t)q * ldq);
     M->Qd32.alloc((size_t)q * ldq);
     if (ldq > q) CB_HIP(hipMemsetAsync(M->Qd64.p, 0, M->Qd64.bytes(), st));
-    CB_HIP(hipMemcpy2DAsync(M->Qd64.p, ldq * 8, Q, ld * 8, q * 8, q, kind, st));
+    copy_host_2d(st, M->Qd64.p, ldq * 8, Q, ld * 8, q * 8, q, kind);
     k_convert<float, double><<<blocks(q * ldq), 256, 0, st>>>(M->Qd64.p, M->Qd32.p, q * ldq);
     CB_HIP(hipGetLastError());
     M->q_diag.resize(q);
     DevBuf<double> dg;
     dg.alloc(q);
     CB_HIP(hipMemcpy2DAsync(dg.p, 8, M->Qd64.p, (ldq + 1) * 8, 8, q, hipMemcpyDeviceToDevice, st));
-    CB_HIP(hipMemcpyAsync(M->q_diag.data(), dg.p, q * 8, hipMemcpyDeviceToHost, st));
+    copy_host(st, M->q_diag.data(), dg.p, q * 8, hipMemcpyDeviceToHost);
     CB_HIP(hipStreamSynchronize(st));
     M->qkind = 0;
     M->qp.release(); M->qc.release(); M->qv64.release(); M->qv32.release();
@@ -2013,9 +2013,9 @@ int cb_mme_set_qinv_split(cb_mme* M, const int64_t* ptr, const int32_t* col, con
         k_convert<float, double><<<blocks(ng * ldq), 256, 0, st>>>(M->Qd64.p, M->Qd32.p, ng * ldq);
         CB_HIP(hipGetLastError());
         M->gmap.alloc(ng);
-        CB_HIP(hipMemcpyAsync(M->gmap.p, gmap, ng * 4, hipMemcpyHostToDevice, st));
+        copy_host(st, M->gmap.p, gmap, ng * 4, hipMemcpyHostToDevice);
         std::vector<double> dg(ng);
-        CB_HIP(hipMemcpy2DAsync(dg.data(), 8, M->Qd64.p, (ldq + 1) * 8, 8, ng, hipMemcpyDeviceToHost, st));
+        copy_host_2d(st, dg.data(), 8, M->Qd64.p, (ldq + 1) * 8, 8, ng, hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
         for (int64_t i = 0; i < ng; ++i)
             if (gmap[i] >= 0) M->q_diag[gmap[i]] += dg[i];  // add_diagonal_to (matrix_adapter.h:280-300)
@@ -2041,9 +2041,9 @@ int cb_mme_apply(cb_mme* M, double lambda, int64_t r, const double* X, double* Y
         dX.alloc((size_t)ld * r);
         dY.alloc((size_t)ld * r);
         CB_HIP(hipMemsetAsync(dX.p, 0, dX.bytes(), st));
-        CB_HIP(hipMemcpy2DAsync(dX.p, ld * 8, X, M->dim * 8, M->dim * 8, r, hipMemcpyHostToDevice, st));
+        copy_host_2d(st, dX.p, ld * 8, X, M->dim * 8, M->dim * 8, r, hipMemcpyHostToDevice);
         apply_C<double>(M, lambda, dX.p, dY.p, (int)r);
-        CB_HIP(hipMemcpy2DAsync(Y, M->dim * 8, dY.p, ld * 8, M->dim * 8, r, hipMemcpyDeviceToHost, st));
+        copy_host_2d(st, Y, M->dim * 8, dY.p, ld * 8, M->dim * 8, r, hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
     });
 }
@@ -2066,18 +2066,18 @@ int cb_mme_solve(cb_mme* M, double lambda, const cb_solve_options* opts, int64_t
         dB.alloc((size_t)ld * r);
         dX.alloc((size_t)ld * r);
         CB_HIP(hipMemsetAsync(dB.p, 0, dB.bytes(), s));
-        CB_HIP(hipMemcpy2DAsync(dB.p, ld * 8, B, M->dim * 8, M->dim * 8, r, hipMemcpyHostToDevice, s));
+        copy_host_2d(s, dB.p, ld * 8, B, M->dim * 8, M->dim * 8, r, hipMemcpyHostToDevice);
         try {
             SolveStats S = solve_block(M, lambda, o, dB.p, dX.p, (int)r);
             its = S.iterations;
             rel = S.max_rel;
         } catch (const Error& e) {
             // hand back the unconverged iterate (matrix_free_pcg.h:64-65 returns it silently)
-            CB_HIP(hipMemcpy2DAsync(X, M->dim * 8, dX.p, ld * 8, M->dim * 8, r, hipMemcpyDeviceToHost, s));
+            copy_host_2d(s, X, M->dim * 8, dX.p, ld * 8, M->dim * 8, r, hipMemcpyDeviceToHost);
             CB_HIP(hipStreamSynchronize(s));
             throw;
         }
-        CB_HIP(hipMemcpy2DAsync(X, M->dim * 8, dX.p, ld * 8, M->dim * 8, r, hipMemcpyDeviceToHost, s));
+        copy_host_2d(s, X, M->dim * 8, dX.p, ld * 8, M->dim * 8, r, hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(s));
     });
     if (iterations) *iterations = its;
@@ -2247,12 +2247,12 @@ struct DenseExact {
         t.alloc(n);
         part.alloc((size_t)DOT_CHUNKS * 5);
         y.resize(n);
-        CB_HIP(hipMemcpyAsync(y.data(), M->y.p, n * 8, hipMemcpyDeviceToHost, st()));
+        copy_host(st(), y.data(), M->y.p, n * 8, hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st()));
     }
     void check_info(const char* msg) {
         int hi = 0;
-        CB_HIP(hipMemcpyAsync(&hi, info.p, sizeof(int), hipMemcpyDeviceToHost, st()));
+        copy_host(st(), &hi, info.p, sizeof(int), hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st()));
         CB_REQUIRE(hi == 0, msg);
     }
@@ -2262,7 +2262,7 @@ struct DenseExact {
         pp.alloc(DOT_CHUNKS);
         k_logdiag_part<<<DOT_CHUNKS, 256, 0, st()>>>(L, m, pp.p);
         std::vector<double> hp(DOT_CHUNKS);
-        CB_HIP(hipMemcpyAsync(hp.data(), pp.p, DOT_CHUNKS * 8, hipMemcpyDeviceToHost, st()));
+        copy_host(st(), hp.data(), pp.p, DOT_CHUNKS * 8, hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st()));
         double sum = 0;
         for (double v : hp) sum += v;
@@ -2274,11 +2274,11 @@ struct DenseExact {
         assemble_dense_C(M, ve / vu, C.p);
         CB_RB(rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)dim, C.p, (rocblas_int)dim, info.p), "dpotrf(C)");
         int hi = 0;
-        CB_HIP(hipMemcpyAsync(&hi, info.p, sizeof(int), hipMemcpyDeviceToHost, st()));
+        copy_host(st(), &hi, info.p, sizeof(int), hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st()));
         if (hi != 0) return false;
         const double ldC = logdiag2(C.p, dim);
-        CB_HIP(hipMemcpyAsync(b.p, M->rhs.data(), dim * 8, hipMemcpyHostToDevice, st()));
+        copy_host(st(), b.p, M->rhs.data(), dim * 8, hipMemcpyHostToDevice);
         CB_RB(rocsolver_dpotrs(h, rocblas_fill_lower, (rocblas_int)dim, 1, C.p, (rocblas_int)dim, b.p,
                                (rocblas_int)dim), "dpotrs(C, rhs)");
         // e = y - W sol
@@ -2288,10 +2288,10 @@ struct DenseExact {
         CB_RB(rocblas_dgemv(h, rocblas_operation_none, (rocblas_int)n, (rocblas_int)dim, &mone, W.p, (rocblas_int)n,
                             b.p, 1, &one, e.p, 1), "dgemv(W sol)");
         std::vector<double> he(n);
-        CB_HIP(hipMemcpyAsync(he.data(), e.p, n * 8, hipMemcpyDeviceToHost, st()));
+        copy_host(st(), he.data(), e.p, n * 8, hipMemcpyDeviceToHost);
         if (sol) {
             sol->resize(dim);
-            CB_HIP(hipMemcpyAsync(sol->data(), b.p, dim * 8, hipMemcpyDeviceToHost, st()));
+            copy_host(st(), sol->data(), b.p, dim * 8, hipMemcpyDeviceToHost);
         }
         CB_HIP(hipStreamSynchronize(st()));
         double ye = 0;
@@ -2319,14 +2319,14 @@ struct DenseExact {
         CB_HIP(hipGetLastError());
         // Py = e / ve ; t = ZAZ' Py
         std::vector<double> he(n), py(n), ht(n), hp((size_t)DOT_CHUNKS * 5);
-        CB_HIP(hipMemcpyAsync(he.data(), e.p, n * 8, hipMemcpyDeviceToHost, st()));
+        copy_host(st(), he.data(), e.p, n * 8, hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st()));
         for (int64_t i = 0; i < n; ++i) py[i] = he[i] / ve;
-        CB_HIP(hipMemcpyAsync(e.p, py.data(), n * 8, hipMemcpyHostToDevice, st()));
+        copy_host(st(), e.p, py.data(), n * 8, hipMemcpyHostToDevice);
         CB_RB(rocblas_dgemv(h, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n, &one, ZAZ.p, (rocblas_int)n,
                             e.p, 1, &zero, t.p, 1), "dgemv(ZAZ' Py)");
-        CB_HIP(hipMemcpyAsync(ht.data(), t.p, n * 8, hipMemcpyDeviceToHost, st()));
-        CB_HIP(hipMemcpyAsync(hp.data(), part.p, hp.size() * 8, hipMemcpyDeviceToHost, st()));
+        copy_host(st(), ht.data(), t.p, n * 8, hipMemcpyDeviceToHost);
+        copy_host(st(), hp.data(), part.p, hp.size() * 8, hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st()));
         double s[5] = {0, 0, 0, 0, 0};
         for (int k = 0; k < DOT_CHUNKS; ++k)
@@ -2465,7 +2465,7 @@ int cb_reml_run(cb_mme* M, const cb_reml_options* opts, double var_u0, double va
         // variance stabiliser of vce.cpp:1370-1389
         const int64_t n = M->n;
         std::vector<double> hy(n);
-        CB_HIP(hipMemcpy(hy.data(), M->y.p, n * 8, hipMemcpyDeviceToHost));
+        copy_host(M->ctx->stream, hy.data(), M->y.p, n * 8, hipMemcpyDeviceToHost);
         double ymean = 0;
         for (double v : hy) ymean += v;
         ymean = n > 0 ? ymean / n : 0.0;
@@ -2562,7 +2562,7 @@ int cb_reml_run(cb_mme* M, const cb_reml_options* opts, double var_u0, double va
             const double lambda = ve / std::max(vu, 1e-9);
             CB_HIP(hipMemcpyAsync(w.B.p, w.rhs.p, M->ldv * 8, hipMemcpyDeviceToDevice, M->ctx->stream));
             solve_block(M, lambda, o.solve, w.B.p, w.S.p, 1);
-            CB_HIP(hipMemcpy(solution, w.S.p, M->dim * 8, hipMemcpyDeviceToHost));
+            copy_host(M->ctx->stream, solution, w.S.p, M->dim * 8, hipMemcpyDeviceToHost);
         }
     });
 }
@@ -2581,7 +2581,7 @@ int cb_reml_logl(cb_mme* M, double var_u, double var_e, double* logl, double* lo
         C.alloc((size_t)dim * dim);
         b.alloc(dim);
         assemble_dense_C(M, lambda, C.p);
-        CB_HIP(hipMemcpyAsync(b.p, M->rhs.data(), dim * 8, hipMemcpyHostToDevice, st));
+        copy_host(st, b.p, M->rhs.data(), dim * 8, hipMemcpyHostToDevice);
         rocblas_handle h = nullptr;
         CB_REQUIRE(rocblas_create_handle(&h) == rocblas_status_success, "cb_reml_logl: rocblas handle");
         struct HG { rocblas_handle h; ~HG() { rocblas_destroy_handle(h); } } hg{h};
@@ -2591,7 +2591,7 @@ int cb_reml_logl(cb_mme* M, double var_u, double var_e, double* logl, double* lo
         CB_REQUIRE(rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)dim, C.p, (rocblas_int)dim, info.p) ==
                        rocblas_status_success, "cb_reml_logl: dpotrf failed");
         int hinfo = 0;
-        CB_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
         if (hinfo != 0) {  // not positive definite: the reference's failed-determinant value
             *logl = -1e20;
@@ -2604,8 +2604,8 @@ int cb_reml_logl(cb_mme* M, double var_u, double var_e, double* logl, double* lo
         part.alloc(DOT_CHUNKS);
         k_logdiag_part<<<DOT_CHUNKS, 256, 0, st>>>(C.p, dim, part.p);
         std::vector<double> hp(DOT_CHUNKS), sol(dim);
-        CB_HIP(hipMemcpyAsync(hp.data(), part.p, DOT_CHUNKS * 8, hipMemcpyDeviceToHost, st));
-        CB_HIP(hipMemcpyAsync(sol.data(), b.p, dim * 8, hipMemcpyDeviceToHost, st));
+        copy_host(st, hp.data(), part.p, DOT_CHUNKS * 8, hipMemcpyDeviceToHost);
+        copy_host(st, sol.data(), b.p, dim * 8, hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
         double ld = 0;
         for (double v : hp) ld += v;
@@ -2630,8 +2630,7 @@ int cb_reml_copy_probes(cb_mme* M, const cb_reml_options* opts, double* out, int
         for (int64_t f = 0; f < w.k; f += w.bw) {
             const int cnt = (int)std::min<int64_t>(w.bw, w.k - f);
             make_probes(M, *opts, w, f, cnt, 0, false);
-            CB_HIP(hipMemcpy2DAsync(out + f * ld, ld * 8, w.B.p + M->p, M->ldv * 8, M->q * 8, cnt,
-                                    hipMemcpyDeviceToHost, M->ctx->stream));
+            copy_host_2d(M->ctx->stream, out + f * ld, ld * 8, w.B.p + M->p, M->ldv * 8, M->q * 8, cnt, hipMemcpyDeviceToHost);
             CB_HIP(hipStreamSynchronize(M->ctx->stream));
         }
     });

@@ -252,7 +252,7 @@ int cb_mv_vce(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* Y,
         DevBuf<int> info;
         info.alloc(1);
         dY.alloc((size_t)n * t);
-        CB_HIP(hipMemcpyAsync(dY.p, Y, (size_t)n * t * 8, hipMemcpyHostToDevice, st));
+        copy_host(st, dY.p, Y, (size_t)n * t * 8, hipMemcpyHostToDevice);
         const double one = 1.0, zero = 0.0;
         // A = null-space basis of X' (the last n - c columns of the full Q)
         Qf.alloc((size_t)n * n);
@@ -260,7 +260,7 @@ int cb_mv_vce(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* Y,
         const double* A;
         if (c > 0) {
             tau.alloc(c);
-            CB_HIP(hipMemcpyAsync(Qf.p, X, (size_t)n * c * 8, hipMemcpyHostToDevice, st));
+            copy_host(st, Qf.p, X, (size_t)n * c * 8, hipMemcpyHostToDevice);
             CB_RBS(rocsolver_dgeqrf(h.h, (rocblas_int)n, (rocblas_int)c, Qf.p, (rocblas_int)n, tau.p), "dgeqrf");
             CB_RBS(rocsolver_dorgqr(h.h, (rocblas_int)n, (rocblas_int)n, (rocblas_int)c, Qf.p, (rocblas_int)n, tau.p),
                    "dorgqr");
@@ -268,7 +268,7 @@ int cb_mv_vce(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* Y,
         } else {
             std::vector<double> I((size_t)n * n, 0.0);
             for (int64_t i = 0; i < n; ++i) I[(size_t)i * n + i] = 1.0;
-            CB_HIP(hipMemcpyAsync(Qf.p, I.data(), Qf.bytes(), hipMemcpyHostToDevice, st));
+            copy_host(st, Qf.p, I.data(), Qf.bytes(), hipMemcpyHostToDevice);
             CB_HIP(hipStreamSynchronize(st));
             A = Qf.p;
         }
@@ -291,7 +291,7 @@ int cb_mv_vce(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* Y,
         CB_RBS(rocsolver_dsyevd(h.h, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)ns, Gr.p,
                                 (rocblas_int)ns, dd.p, ee.p, info.p), "dsyevd(A' G A)");
         int hinfo = 0;
-        CB_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
         CB_REQUIRE(hinfo == 0, "cb_mv_vce: eigensolver did not converge");
         Uy.alloc((size_t)ns * t);
@@ -300,11 +300,11 @@ int cb_mv_vce(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* Y,
                              &zero, Uy.p, (rocblas_int)ns), "dgemm(U' Y_r)");
         // eigenvalues clamped at 0 (eigen_vce.cpp:127)
         std::vector<double> hd(ns);
-        CB_HIP(hipMemcpyAsync(hd.data(), dd.p, ns * 8, hipMemcpyDeviceToHost, st));
+        copy_host(st, hd.data(), dd.p, ns * 8, hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
         for (double& v : hd)
             if (v < 0) v = 0;
-        CB_HIP(hipMemcpyAsync(dd.p, hd.data(), ns * 8, hipMemcpyHostToDevice, st));
+        copy_host(st, dd.p, hd.data(), ns * 8, hipMemcpyHostToDevice);
         // start: Vg = Ve = Cov(Y) / 2 with the divisor n - c - 1 (eigen_vce.cpp:152-156)
         const int T = (int)t;
         std::vector<double> mean(T, 0.0), vg((size_t)T * T), ve((size_t)T * T);
@@ -335,7 +335,7 @@ int cb_mv_vce(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* Y,
         while (it < opts->max_iter) {
             std::vector<double> both(vg);
             both.insert(both.end(), ve.begin(), ve.end());
-            CB_HIP(hipMemcpyAsync(vgve.p, both.data(), both.size() * 8, hipMemcpyHostToDevice, st));
+            copy_host(st, vgve.p, both.data(), both.size() * 8, hipMemcpyHostToDevice);
             switch (T) {
                 case 1: launch_em<1>(dd.p, Uy.p, ns, ns, vgve.p, part.p, st); break;
                 case 2: launch_em<2>(dd.p, Uy.p, ns, ns, vgve.p, part.p, st); break;
@@ -343,7 +343,7 @@ int cb_mv_vce(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* Y,
                 default: launch_em<4>(dd.p, Uy.p, ns, ns, vgve.p, part.p, st); break;
             }
             CB_HIP(hipGetLastError());
-            CB_HIP(hipMemcpyAsync(hp.data(), part.p, hp.size() * 8, hipMemcpyDeviceToHost, st));
+            copy_host(st, hp.data(), part.p, hp.size() * 8, hipMemcpyDeviceToHost);
             CB_HIP(hipStreamSynchronize(st));
             std::fill(sg.begin(), sg.end(), 0.0);
             std::fill(se.begin(), se.end(), 0.0);
@@ -411,13 +411,13 @@ int cb_mv_solve(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* 
         dvi.alloc(2 * T * T);
         b.alloc(dim);
         if (c > 0) {
-            CB_HIP(hipMemcpyAsync(dX.p, X, (size_t)n * c * 8, hipMemcpyHostToDevice, st));
-            CB_HIP(hipMemcpyAsync(dxtx.p, xtx.data(), (size_t)c * c * 8, hipMemcpyHostToDevice, st));
+            copy_host(st, dX.p, X, (size_t)n * c * 8, hipMemcpyHostToDevice);
+            copy_host(st, dxtx.p, xtx.data(), (size_t)c * c * 8, hipMemcpyHostToDevice);
         }
         std::vector<double> vv(vei);
         vv.insert(vv.end(), vgi.begin(), vgi.end());
-        CB_HIP(hipMemcpyAsync(dvi.p, vv.data(), vv.size() * 8, hipMemcpyHostToDevice, st));
-        CB_HIP(hipMemcpyAsync(b.p, rhs.data(), dim * 8, hipMemcpyHostToDevice, st));
+        copy_host(st, dvi.p, vv.data(), vv.size() * 8, hipMemcpyHostToDevice);
+        copy_host(st, b.p, rhs.data(), dim * 8, hipMemcpyHostToDevice);
         k_mv_lhs<<<dim3((unsigned)ceil_div(dim, 256), (unsigned)dim), 256, 0, st>>>(dxtx.p, dX.p, dGinv, ldg, dvi.p, n,
                                                                                     c, T, L.p, dim);
         CB_HIP(hipGetLastError());
@@ -426,13 +426,13 @@ int cb_mv_solve(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* 
         CB_RBS(rocblas_set_stream(h.h, st), "rocblas_set_stream");
         CB_RBS(rocsolver_dpotrf(h.h, rocblas_fill_lower, (rocblas_int)dim, L.p, (rocblas_int)dim, info.p), "dpotrf");
         int hinfo = 0;
-        CB_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
         CB_REQUIRE(hinfo == 0, "cb_mv_solve: the multi-trait MME is not positive definite");
         CB_RBS(rocsolver_dpotrs(h.h, rocblas_fill_lower, (rocblas_int)dim, 1, L.p, (rocblas_int)dim, b.p,
                                 (rocblas_int)dim), "dpotrs");
         std::vector<double> sol(dim);
-        CB_HIP(hipMemcpyAsync(sol.data(), b.p, dim * 8, hipMemcpyDeviceToHost, st));
+        copy_host(st, sol.data(), b.p, dim * 8, hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
         if (c > 0) std::copy(sol.begin(), sol.begin() + ct, beta);  // c x t column-major
         std::copy(sol.begin() + ct, sol.end(), u);                  // n x t column-major

@@ -3,9 +3,75 @@
 #include "cb_internal.h"
 #include "cb_genotypes.h"
 
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+
 namespace cbamd {
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
+
+// Pinned staging buffer for copies of caller-owned host memory, allocated on
+// first use and kept for the life of the process (pinned memory serves every
+// device). One copy uses it at a time.
+namespace {
+constexpr size_t kStageBytes = size_t(64) << 20;
+std::mutex g_stage_mu;
+char* g_stage = nullptr;
+
+char* staging() {
+    if (!g_stage) CB_HIP(hipHostMalloc((void**)&g_stage, kStageBytes, hipHostMallocDefault));
+    return g_stage;
+}
+}  // namespace
+
+void copy_host_2d(hipStream_t st, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
+                  size_t rows, hipMemcpyKind kind) {
+    if (kind == hipMemcpyDeviceToDevice) {
+        CB_HIP(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, kind, st));
+        return;
+    }
+    CB_REQUIRE(kind == hipMemcpyHostToDevice || kind == hipMemcpyDeviceToHost, "copy_host: unsupported copy kind");
+    if (width == 0 || rows == 0) return;
+    const bool up = kind == hipMemcpyHostToDevice;
+    std::lock_guard<std::mutex> lock(g_stage_mu);
+    char* stage = staging();
+    CB_HIP(hipStreamSynchronize(st));  // the work queued before the copy is done
+    // (row, byte offset) pieces of at most kStageBytes; whole rows when they fit
+    const size_t seg = std::min(width, kStageBytes);
+    const size_t per = width <= kStageBytes ? kStageBytes / width : 1;
+    for (size_t r0 = 0; r0 < rows; r0 += per) {
+        const size_t nr = std::min(per, rows - r0);
+        for (size_t off = 0; off < width; off += seg) {
+            const size_t len = std::min(seg, width - off);
+            char* d = (char*)dst + r0 * dpitch + off;
+            const char* s = (const char*)src + r0 * spitch + off;
+            if (up) {
+                // the previous piece's DMA has finished (synchronised below)
+                for (size_t r = 0; r < nr; ++r) std::memcpy(stage + r * len, s + r * spitch, len);
+                CB_HIP(hipMemcpy2DAsync(d, dpitch, stage, len, len, nr, kind, st));
+                CB_HIP(hipStreamSynchronize(st));
+            } else {
+                CB_HIP(hipMemcpy2DAsync(stage, len, s, spitch, len, nr, kind, st));
+                CB_HIP(hipStreamSynchronize(st));
+                for (size_t r = 0; r < nr; ++r) std::memcpy(d + r * dpitch, stage + r * len, len);
+            }
+        }
+    }
+}
+
+void copy_host(hipStream_t st, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    if (kind == hipMemcpyDeviceToDevice) {
+        CB_HIP(hipMemcpyAsync(dst, src, bytes, kind, st));
+        return;
+    }
+    // one row split into pieces, or rows of kStageBytes
+    const size_t w = std::min(bytes, kStageBytes);
+    if (w == 0) return;
+    const size_t rows = bytes / w, tail = bytes - rows * w;
+    copy_host_2d(st, dst, w, src, w, w, rows, kind);
+    if (tail) copy_host_2d(st, (char*)dst + rows * w, tail, (const char*)src + rows * w, tail, tail, 1, kind);
+}
 }  // namespace cbamd
 
 using namespace cbamd;
@@ -158,12 +224,11 @@ int cb_genotypes_from_bed(cb_context* ctx, const uint8_t* bed, size_t bed_size, 
                    ".bed file size (" + std::to_string(bed_size) + ") < expected (" +
                        std::to_string(3 + n_snps * bps) + ")");
         cb_genotypes* g = new_genotypes(ctx, n_samples, n_snps);
-        hipError_t e = hipMemcpy2DAsync(g->bed.p, g->pitch, bed + 3, bps, bps, n_snps,
-                                        hipMemcpyHostToDevice, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) {
+        try {
+            copy_host_2d(ctx->stream, g->bed.p, g->pitch, bed + 3, bps, bps, n_snps, hipMemcpyHostToDevice);
+        } catch (...) {
             delete g;
-            throw Error(CB_ERR_DEVICE, std::string("BED upload failed: ") + hipGetErrorString(e));
+            throw;
         }
         *out = g;
     });

@@ -219,13 +219,13 @@ int cb_hinv_blocks(cb_context* ctx, int64_t n, const int32_t* dam, const int32_t
         DevBuf<double> ddii;
         ds.alloc(n); dd.alloc(n); dorder.alloc(n); doptr.alloc(n + 1); doff.alloc(oidx.size()); ddii.alloc(n);
         dmap.alloc(ng);
-        CB_HIP(hipMemcpyAsync(ds.p, s.data(), n * 4, hipMemcpyHostToDevice, st));
-        CB_HIP(hipMemcpyAsync(dd.p, d.data(), n * 4, hipMemcpyHostToDevice, st));
-        CB_HIP(hipMemcpyAsync(dorder.p, order.data(), n * 8, hipMemcpyHostToDevice, st));
-        CB_HIP(hipMemcpyAsync(doptr.p, optr.data(), (n + 1) * 8, hipMemcpyHostToDevice, st));
-        CB_HIP(hipMemcpyAsync(doff.p, oidx.data(), oidx.size() * 4, hipMemcpyHostToDevice, st));
-        CB_HIP(hipMemcpyAsync(ddii.p, dii.data(), n * 8, hipMemcpyHostToDevice, st));
-        CB_HIP(hipMemcpyAsync(dmap.p, gmap, ng * 4, hipMemcpyHostToDevice, st));
+        copy_host(st, ds.p, s.data(), n * 4, hipMemcpyHostToDevice);
+        copy_host(st, dd.p, d.data(), n * 4, hipMemcpyHostToDevice);
+        copy_host(st, dorder.p, order.data(), n * 8, hipMemcpyHostToDevice);
+        copy_host(st, doptr.p, optr.data(), (n + 1) * 8, hipMemcpyHostToDevice);
+        copy_host(st, doff.p, oidx.data(), oidx.size() * 4, hipMemcpyHostToDevice);
+        copy_host(st, ddii.p, dii.data(), n * 8, hipMemcpyHostToDevice);
+        copy_host(st, dmap.p, gmap, ng * 4, hipMemcpyHostToDevice);
         // A22 in batches of b unit columns
         DevBuf<double> A22;
         A22.alloc((size_t)ng * ng);
@@ -262,7 +262,7 @@ int cb_hinv_blocks(cb_context* ctx, int64_t n, const int32_t* dam, const int32_t
             part.alloc(RED_CHUNKS * 4);
             k_means_part<<<RED_CHUNKS, 256, 0, st>>>(Gb.p, ng, A22.p, ng, ng, part.p);
             std::vector<double> hp(RED_CHUNKS * 4);
-            CB_HIP(hipMemcpyAsync(hp.data(), part.p, hp.size() * 8, hipMemcpyDeviceToHost, st));
+            copy_host(st, hp.data(), part.p, hp.size() * 8, hipMemcpyDeviceToHost);
             CB_HIP(hipStreamSynchronize(st));
             double sgd = 0, sgo = 0, sad = 0, sao = 0;
             for (int k = 0; k < RED_CHUNKS; ++k) {
@@ -295,7 +295,7 @@ int cb_hinv_blocks(cb_context* ctx, int64_t n, const int32_t* dam, const int32_t
         int hinfo = 0;
         CB_REQUIRE(rocsolver_dpotrf(h.h, rocblas_fill_lower, (rocblas_int)ng, A22.p, (rocblas_int)ng, info.p) ==
                        rocblas_status_success, "cb_hinv_blocks: dpotrf failed");
-        CB_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
         CB_REQUIRE(hinfo == 0, "cb_hinv_blocks: A22 is not positive definite");
         CB_REQUIRE(rocsolver_dpotri(h.h, rocblas_fill_lower, (rocblas_int)ng, A22.p, (rocblas_int)ng, info.p) ==
