@@ -1331,7 +1331,7 @@ __global__ void k_to_f32(const double* __restrict__ in, float* __restrict__ out,
 // Returns iterations of the slowest column; *max_rel = worst true relative
 // (or absolute) residual at exit.
 int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int r,
-                     const cb_solve_options& o, double* max_rel) {
+                     const cb_solve_options& o, double* max_rel, bool warm) {
     cb_context* ctx = M->ctx;
     hipStream_t st = ctx->stream;
     const int64_t ld = M->ldv, dim = M->dim;
@@ -1364,7 +1364,7 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
     // (re)start from the true residual: R64 = B - C x, R = (float)R64,
     // Z = M^-1 R, rho = R.Z; returns true when every column has converged
     auto replace = [&](bool first) -> bool {
-        if (first) {
+        if (first && !warm) {
             CB_HIP(hipMemcpyAsync(R64.p, B64, nv * 8, hipMemcpyDeviceToDevice, st));
         } else {
             apply_C<double>(M, lambda, X, CX.p, r);
@@ -1476,8 +1476,9 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
 
 // Solve driver shared by cb_mme_solve and the REML loop. B, X device FP64
 // blocks [ldv x r]; X is the output (zero start).
+// warm: X holds the initial guess (else the solve starts from 0)
 static SolveStats solve_block(cb_mme* M, double lambda, const cb_solve_options& o, const double* B,
-                              double* X, int r) {
+                              double* X, int r, bool warm = false) {
     cb_context* ctx = M->ctx;
     hipStream_t st = ctx->stream;
     const int64_t ld = M->ldv, dim = M->dim;
@@ -1485,17 +1486,17 @@ static SolveStats solve_block(cb_mme* M, double lambda, const cb_solve_options& 
     const bool absolute = o.stop_rule == CB_STOP_ABSOLUTE;
     const bool block = o.precond == CB_PRECOND_BLOCK_JACOBI;
     SolveStats S;
-    CB_HIP(hipMemsetAsync(X, 0, nv * 8, st));
+    if (!warm) CB_HIP(hipMemsetAsync(X, 0, nv * 8, st));
     std::vector<double> rel;
     if (o.precision == CB_PREC_FP64) {
         if (o.batch_rhs || r == 1) {
-            S.iterations = pcg_block<double, double>(M, lambda, B, X, r, o.tol, absolute, o.max_iter, block, &rel, true);
+            S.iterations = pcg_block<double, double>(M, lambda, B, X, r, o.tol, absolute, o.max_iter, block, &rel, !warm);
             for (double v : rel) S.max_rel = std::max(S.max_rel, v);
         } else {
             // the plain path: one right-hand side at a time
             for (int c = 0; c < r; ++c) {
                 const int64_t itc = pcg_block<double, double>(M, lambda, B + (size_t)c * ld, X + (size_t)c * ld, 1,
-                                                              o.tol, absolute, o.max_iter, block, &rel, true);
+                                                              o.tol, absolute, o.max_iter, block, &rel, !warm);
                 S.iterations += itc;
                 S.max_rel = std::max(S.max_rel, rel[0]);
             }
@@ -1510,7 +1511,7 @@ static SolveStats solve_block(cb_mme* M, double lambda, const cb_solve_options& 
     // (and before any column is declared converged). The search directions are
     // kept across replacements.
     CB_REQUIRE(M->qkind == 1 || M->Qd32.p, "FP32 copy of Q missing");
-    S.iterations = pcg_reliable(M, lambda, B, X, r, o, &S.max_rel);
+    S.iterations = pcg_reliable(M, lambda, B, X, r, o, &S.max_rel, warm);
     return S;
 }
 
@@ -1644,10 +1645,15 @@ void project_spd_2x2(double A[4]) {
 // else the probes go through in chunks of bw columns whose vectors and Q z are
 // regenerated at every evaluation instead of being stored, y's column riding
 // with the last chunk.
+// The solutions of one evaluation start the next one's solves (the probe
+// block when it is one block, and the AI pair; y's column, which the last
+// chunk leaves in S, starts the final solve), as AI_REML::solve_lhs starts
+// PCG from last_solution (vce.cpp:1225-1234).
 struct cb_reml_work {
     DevBuf<double> B, S, Qz, B2, S2, V, T, part, dots, e, rhs;
     int k = 0, bw = 0;
     bool chunked = false;
+    bool have_s = false, have_s2 = false;  // S (last block) and S2 hold solutions
 };
 
 namespace {
@@ -1711,7 +1717,8 @@ static void reml_eval(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, doub
             if (with_y) CB_HIP(hipMemcpyAsync(w.B.p, w.rhs.p, ld * 8, hipMemcpyDeviceToDevice, st));
             if (count > 0) make_probes(M, o, w, first, count, t0, true);
         }
-        solve_block(M, lambda, o.solve, w.B.p, w.S.p, t0 + count);
+        // (chunks share S, so only one block keeps its solutions to the next evaluation)
+        solve_block(M, lambda, o.solve, w.B.p, w.S.p, t0 + count, !w.chunked && w.have_s);
         // the score terms of this block ("adjoint" pass over its solutions)
         if (with_y) {
             CB_HIP(hipMemsetAsync(Qu.p, 0, ld * 8, st));
@@ -1732,6 +1739,7 @@ static void reml_eval(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, doub
             out.uQu = hd[1];
         }
     }
+    w.have_s = true;
     double tr = 0;
     for (int i = 0; i < k; ++i) tr += traces[i];
     if (!o.trace_exact && k > 0) tr /= k;
@@ -1750,7 +1758,8 @@ static void reml_eval(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, doub
     k_ai_vectors<<<blocks(n), 256, 0, st>>>(M->aid.p, n, p, w.S.p, w.e.p, 1.0 / vu, 1.0 / ve, w.V.p, ldn);
     k_wt_apply<<<dim3(blocks(dim), 2), 256, 0, st>>>(M->tp.p, M->tc.p, M->tv.p, dim, w.V.p, ldn, w.B2.p, ld);
     CB_HIP(hipGetLastError());
-    solve_block(M, lambda, o.solve, w.B2.p, w.S2.p, 2);
+    solve_block(M, lambda, o.solve, w.B2.p, w.S2.p, 2, w.have_s2);
+    w.have_s2 = true;
     k_w_apply<<<dim3(blocks(n), 2), 256, 0, st>>>(M->xp.p, M->xc.p, M->xv.p, M->aid.p, n, p, w.S2.p, ld, w.T.p, ldn);
     std::vector<double> hV((size_t)ldn * 2), hT((size_t)ldn * 2);
     copy_host(st, hV.data(), w.V.p, ldn * 2 * 8, hipMemcpyDeviceToHost);
@@ -2561,7 +2570,7 @@ int cb_reml_run(cb_mme* M, const cb_reml_options* opts, double var_u0, double va
             // final solve at the estimates (GSSolve uses lambda = ve/vu)
             const double lambda = ve / std::max(vu, 1e-9);
             CB_HIP(hipMemcpyAsync(w.B.p, w.rhs.p, M->ldv * 8, hipMemcpyDeviceToDevice, M->ctx->stream));
-            solve_block(M, lambda, o.solve, w.B.p, w.S.p, 1);
+            solve_block(M, lambda, o.solve, w.B.p, w.S.p, 1, w.have_s);
             copy_host(M->ctx->stream, solution, w.S.p, M->dim * 8, hipMemcpyDeviceToHost);
         }
     });

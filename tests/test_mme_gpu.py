@@ -187,9 +187,12 @@ def test_exact_trace_in_chunks_and_run(cb, ctx):
         R.setInitialVars(0.5, 0.5)
         R.solve()
         runs.append(R)
+    # the one-block run starts each evaluation's solves from the previous
+    # solutions, the chunked one from zero: equal to the solves' tolerance
     assert runs[0].iterations == runs[1].iterations
-    np.testing.assert_allclose([runs[1].var_u, runs[1].var_e], [runs[0].var_u, runs[0].var_e], rtol=1e-8)
-    assert ebv_err(runs[1].solution[M.p:], runs[0].solution[M.p:]) < 1e-8
+    np.testing.assert_allclose([runs[1].var_u, runs[1].var_e], [runs[0].var_u, runs[0].var_e], rtol=1e-6,
+                               atol=1e-9)
+    assert ebv_err(runs[1].solution[M.p:], runs[0].solution[M.p:]) < EBV_TOL
 
 
 def test_exact_trace_score_matches_fd_of_logL(cb, ctx):
