@@ -812,6 +812,10 @@ struct cb_mme {
     // scratch for dense products
     DevBuf<double> part64;
     DevBuf<float> part32;
+    // work vectors of the PCG solves and of cb_mme_solve's blocks, kept from
+    // one solve to the next (grown, never shrunk): repeated solves allocate
+    // and free nothing, and so never wait on the device for a free
+    DevBuf<uint8_t> ws[16];
 };
 
 namespace {
@@ -834,6 +838,23 @@ template <>
 const double* qvals<double>(cb_mme* M) { return M->qv64.p; }
 template <>
 const float* qvals<float>(cb_mme* M) { return M->qv32.p; }
+// A typed view of one of M's work buffers, grown to hold count elements.
+enum WsSlot { WS_R, WS_Z, WS_P, WS_QP, WS_R64, WS_CX, WS_PART, WS_PART2, WS_D1, WS_D2, WS_CS, WS_TMPX, WS_XT,
+              WS_IO_B, WS_IO_X, WS_COUNT };
+template <class T>
+struct WsPtr {
+    T* p;
+    size_t n;
+    size_t bytes() const { return n * sizeof(T); }
+};
+template <class T>
+WsPtr<T> ws(cb_mme* M, WsSlot slot, size_t count) {
+    static_assert(WS_COUNT <= 16, "work buffer slots");
+    DevBuf<uint8_t>& b = M->ws[slot];
+    if (b.n < count * sizeof(T)) b.alloc(count * sizeof(T));
+    return WsPtr<T>{reinterpret_cast<T*>(b.p), count};
+}
+
 template <class T>
 T* partbuf(cb_mme* M, size_t need);
 template <>
@@ -1221,14 +1242,10 @@ int64_t pcg_block(cb_mme* M, double lambda, const T* B, TX* X, int r, double tol
     const int64_t ld = M->ldv, dim = M->dim;
     const size_t nv = (size_t)ld * r;
     build_precond(M, lambda, block_precond);
-    DevBuf<T> R, Z, P, Qp;
-    DevBuf<TX> tmpX;
-    R.alloc(nv); Z.alloc(nv); P.alloc(nv); Qp.alloc(nv);
-    DevBuf<double> part, d1, d2;
-    part.alloc((size_t)DOT_CHUNKS * r);
-    d1.alloc(r); d2.alloc(r);
-    DevBuf<ColState> cs;
-    cs.alloc(r);
+    const WsPtr<T> R = ws<T>(M, WS_R, nv), Z = ws<T>(M, WS_Z, nv), P = ws<T>(M, WS_P, nv), Qp = ws<T>(M, WS_QP, nv);
+    const WsPtr<double> part = ws<double>(M, WS_PART, (size_t)DOT_CHUNKS * r), d1 = ws<double>(M, WS_D1, r),
+                        d2 = ws<double>(M, WS_D2, r);
+    const WsPtr<ColState> cs = ws<ColState>(M, WS_CS, r);
     CB_HIP(hipMemsetAsync(R.p, 0, R.bytes(), st));
     CB_HIP(hipMemsetAsync(Z.p, 0, Z.bytes(), st));
     CB_HIP(hipMemsetAsync(P.p, 0, P.bytes(), st));
@@ -1238,10 +1255,9 @@ int64_t pcg_block(cb_mme* M, double lambda, const T* B, TX* X, int r, double tol
     if (x_zero) {
         CB_HIP(hipMemcpyAsync(R.p, B, nv * sizeof(T), hipMemcpyDeviceToDevice, st));
     } else {
-        tmpX.alloc(nv);
+        const WsPtr<TX> tmpX = ws<TX>(M, WS_TMPX, nv);
         CB_HIP(hipMemcpyAsync(tmpX.p, X, nv * sizeof(TX), hipMemcpyDeviceToDevice, st));
-        DevBuf<T> xt;
-        xt.alloc(nv);
+        const WsPtr<T> xt = ws<T>(M, WS_XT, nv);
         k_convert<T, TX><<<blocks((int64_t)nv), 256, 0, st>>>(tmpX.p, xt.p, (int64_t)nv);
         apply_C<T>(M, lambda, xt.p, Qp.p, r);
         k_sub<T, T><<<g2, 256, 0, st>>>(B, Qp.p, R.p, dim, ld);
@@ -1277,8 +1293,7 @@ int64_t pcg_block(cb_mme* M, double lambda, const T* B, TX* X, int r, double tol
     int64_t it = 0;
     const int check_every = 8;
     bool any_active = std::any_of(hs.begin(), hs.end(), [](const ColState& s) { return s.active; });
-    DevBuf<double> part2;
-    part2.alloc((size_t)DOT_CHUNKS * r);
+    const WsPtr<double> part2 = ws<double>(M, WS_PART2, (size_t)DOT_CHUNKS * r);
     auto iteration = [&] {
         cg_iteration<TX, T>(M, lambda, X, R.p, Z.p, P.p, Qp.p, cs.p, part.p, part2.p, r, tol2, absolute);
     };
@@ -1338,14 +1353,12 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
     const size_t nv = (size_t)ld * r;
     const bool absolute = o.stop_rule == CB_STOP_ABSOLUTE;
     build_precond(M, lambda, o.precond == CB_PRECOND_BLOCK_JACOBI);
-    DevBuf<float> R, Z, P, Qp;
-    DevBuf<double> R64, CX, part, d1, d2;
-    R.alloc(nv); Z.alloc(nv); P.alloc(nv); Qp.alloc(nv);
-    R64.alloc(nv); CX.alloc(nv);
-    part.alloc((size_t)DOT_CHUNKS * r);
-    d1.alloc(r); d2.alloc(r);
-    DevBuf<ColState> cs;
-    cs.alloc(r);
+    const WsPtr<float> R = ws<float>(M, WS_R, nv), Z = ws<float>(M, WS_Z, nv), P = ws<float>(M, WS_P, nv),
+                       Qp = ws<float>(M, WS_QP, nv);
+    const WsPtr<double> R64 = ws<double>(M, WS_R64, nv), CX = ws<double>(M, WS_CX, nv),
+                        part = ws<double>(M, WS_PART, (size_t)DOT_CHUNKS * r), d1 = ws<double>(M, WS_D1, r),
+                        d2 = ws<double>(M, WS_D2, r);
+    const WsPtr<ColState> cs = ws<ColState>(M, WS_CS, r);
     CB_HIP(hipMemsetAsync(R.p, 0, R.bytes(), st));
     CB_HIP(hipMemsetAsync(Z.p, 0, Z.bytes(), st));
     CB_HIP(hipMemsetAsync(P.p, 0, P.bytes(), st));
@@ -1420,8 +1433,7 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
     int64_t it = 0;
     const int check_every = 4;
     int64_t replacements = 0;
-    DevBuf<double> part2;
-    part2.alloc((size_t)DOT_CHUNKS * r);
+    const WsPtr<double> part2 = ws<double>(M, WS_PART2, (size_t)DOT_CHUNKS * r);
     // the recursive test in the iteration only pauses a column until the
     // next replacement
     auto iteration = [&] {
@@ -1650,7 +1662,7 @@ void project_spd_2x2(double A[4]) {
 // chunk leaves in S, starts the final solve), as AI_REML::solve_lhs starts
 // PCG from last_solution (vce.cpp:1225-1234).
 struct cb_reml_work {
-    DevBuf<double> B, S, Qz, B2, S2, V, T, part, dots, e, rhs;
+    DevBuf<double> B, S, Qz, B2, S2, V, T, part, dots, e, rhs, Qu;
     int k = 0, bw = 0;
     bool chunked = false;
     bool have_s = false, have_s2 = false;  // S (last block) and S2 hold solutions
@@ -1706,8 +1718,7 @@ static void reml_eval(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, doub
         chunks.push_back({k - last, (int)last});
     }
     std::vector<double> traces(k), hd(2 + bw);
-    DevBuf<double> Qu;
-    Qu.alloc(ld);
+    DevBuf<double>& Qu = w.Qu;
     for (size_t ci = 0; ci < chunks.size(); ++ci) {
         const bool with_y = ci + 1 == chunks.size();
         const int64_t first = chunks[ci].first;
@@ -1792,6 +1803,7 @@ static void reml_setup(cb_mme* M, const cb_reml_options& o, cb_reml_work& w) {
     w.S.alloc((size_t)ld * bw);
     w.Qz.alloc((size_t)ld * std::max(1, std::min(bw, w.k)));
     w.rhs.alloc(ld);
+    w.Qu.alloc(ld);
     w.B2.alloc((size_t)ld * 2);
     w.S2.alloc((size_t)ld * 2);
     const int64_t ldn = round_up(n, 64);
@@ -2071,9 +2083,7 @@ int cb_mme_solve(cb_mme* M, double lambda, const cb_solve_options* opts, int64_t
         set_device(M->ctx);
         hipStream_t s = M->ctx->stream;
         const int64_t ld = M->ldv;
-        DevBuf<double> dB, dX;
-        dB.alloc((size_t)ld * r);
-        dX.alloc((size_t)ld * r);
+        const WsPtr<double> dB = ws<double>(M, WS_IO_B, (size_t)ld * r), dX = ws<double>(M, WS_IO_X, (size_t)ld * r);
         CB_HIP(hipMemsetAsync(dB.p, 0, dB.bytes(), s));
         copy_host_2d(s, dB.p, ld * 8, B, M->dim * 8, M->dim * 8, r, hipMemcpyHostToDevice);
         try {
