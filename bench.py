@@ -205,14 +205,18 @@ def c1_leg(ctx, n: int = 2000, m: int = 10000, h2: float = 0.3, ngroups: int = 2
 
 
 def c5_leg(ctx, n_inst: int = 96, m: int = 20000, n_lo: int = 500, n_hi: int = 8000, k: int = 32,
-           seed: int = 20265203) -> dict:
+           seed: int = 20265203, workers: int = 4) -> dict:
     """Batched uneven instances: n log-uniform in [n_lo, n_hi], m SNPs,
     MAF U(0.05, 0.5); even instances single-trait (AI-REML, mixed batched
     PCG with k probes + y), odd instances 3-trait (r_g ~ U(-0.5, 0.8), h2 ~
     U(0.1, 0.6), r_e = 0.2; projected eigen-rotation EM, then the Kronecker
     MME solve). Per instance: GRM and G^-1 on the device (set-up, untimed),
     then the timed estimation. Reports REML (or EM) iterations/s per size
-    bucket."""
+    bucket. Then the single-trait instances are served again together: each
+    of `workers` host threads drives its own context (stream) through its
+    share of the instances (balanced on n^2, largest first), and the wall
+    time of the batch is compared with the serial sum; the estimates must be
+    bit-identical to the serial ones."""
     import math
     import time
 
@@ -222,9 +226,21 @@ def c5_leg(ctx, n_inst: int = 96, m: int = 20000, n_lo: int = 500, n_hi: int = 8
     import cosmicblup_amd as cb
     from cosmicblup_amd import _lib, design, linalg, mme, multitrait, synth
 
+    from concurrent.futures import ThreadPoolExecutor
+
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(math.log(n_lo), math.log(n_hi), n_inst)).astype(int)
     print(f"[bench] C5 leg: {n_inst} instances, n in [{sizes.min()}, {sizes.max()}], m={m}", flush=True)
+    # single-trait instances to serving contexts: largest n^2 first, each to
+    # the least loaded worker
+    ctxs = [ctx] + [_lib.Context(0) for _ in range(workers - 1)]
+    load = [0.0] * workers
+    owner = {}
+    for idx in sorted(range(0, n_inst, 2), key=lambda i: -int(sizes[i])):
+        w = min(range(workers), key=lambda j: load[j])
+        owner[idx] = w
+        load[w] += float(sizes[idx]) ** 2
+    served = []  # (worker, idx, model, reml options, serial result)
     recs = []
     for idx, n in enumerate(sizes):
         n = int(n)
@@ -249,17 +265,19 @@ def c5_leg(ctx, n_inst: int = 96, m: int = 20000, n_lo: int = 500, n_hi: int = 8
             g = Xc @ rng.standard_normal(200)
             g *= math.sqrt(h2) / g.std()
             y = 10.0 + rng.standard_normal(10)[grp] + g + rng.standard_normal(n) * math.sqrt(1 - h2)
-            model = mme.MixedModel(fd, np.arange(n, dtype=np.int32), y, n, ctx=ctx)
-            model.set_qinv_dense_device(Q.data_ptr(), n)
-            R = mme.AIREML(model, mme.reml_options(n_probes=k, seed=idx, max_iter=20, tol=1e-4,
-                                                   solve=mme.solve_options(tol=1e-8)))
+            mctx = ctxs[owner[idx]]
             ctx.synchronize()
+            model = mme.MixedModel(fd, np.arange(n, dtype=np.int32), y, n, ctx=mctx)
+            model.set_qinv_dense_device(Q.data_ptr(), n)
+            ropts = mme.reml_options(n_probes=k, seed=idx, max_iter=20, tol=1e-4, solve=mme.solve_options(tol=1e-8))
+            R = mme.AIREML(model, ropts)
+            mctx.synchronize()
             t0 = time.perf_counter()
             R.solve()
-            ctx.synchronize()
+            mctx.synchronize()
             dt = time.perf_counter() - t0
             recs.append({"n": n, "traits": 1, "iterations": R.iterations, "seconds": dt, "converged": R.converged})
-            del model
+            served.append((owner[idx], idx, model, ropts, (R.iterations, R.var_u, R.var_e)))
         else:
             t = 3
             h2 = rng.uniform(0.1, 0.6, t)
@@ -303,6 +321,33 @@ def c5_leg(ctx, n_inst: int = 96, m: int = 20000, n_lo: int = 500, n_hi: int = 8
                                       "iterations": it, "seconds": sec, "iters_per_s": it / sec,
                                       "converged": sum(r["converged"] for r in sel)})
     out["total_seconds"] = sum(r["seconds"] for r in recs)
+
+    def serve(w):
+        res = []
+        for wk, idx, model, ropts, _ in served:
+            if wk != w:
+                continue
+            R = mme.AIREML(model, ropts)
+            R.solve()
+            model.ctx.synchronize()
+            res.append((idx, (R.iterations, R.var_u, R.var_e)))
+        return res
+
+    for c in ctxs:
+        c.synchronize()
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(workers) as ex:
+        got = dict(sum(ex.map(serve, range(workers)), []))
+    wall = time.perf_counter() - t0
+    serial = sum(r["seconds"] for r in recs if r["traits"] == 1)
+    its = sum(r["iterations"] for r in recs if r["traits"] == 1)
+    out["served_single_trait"] = {
+        "instances": len(served), "workers": workers, "reml_iterations": its, "wall_seconds": wall,
+        "serial_seconds": serial, "speedup_vs_serial": serial / wall, "reml_iters_per_s": its / wall,
+        "identical_to_serial": all(got[idx] == ref for _, idx, _, _, ref in served)}
+    print(f"[bench] C5 served: {len(served)} single-trait instances on {workers} streams in {wall:.3f} s "
+          f"(serial {serial:.3f} s)", flush=True)
+    del served
     return out
 
 
