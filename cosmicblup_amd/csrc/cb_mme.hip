@@ -36,6 +36,7 @@
 #include <numeric>
 #include <vector>
 
+#include "cb_dense_tuned.h"
 #include "cb_internal.h"
 
 using namespace cbamd;
@@ -659,6 +660,13 @@ __global__ void k_probes(double* __restrict__ Z, int64_t ld, int64_t off, int64_
     Z[(int64_t)c * ld + off + j] = v;
 }
 
+// synthetic operands for the tuner: uniform in [-0.5, 0.5) from a hash of the index
+template <class T>
+__global__ void k_tune_fill(T* __restrict__ a, int64_t n, uint64_t seed) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = (T)((double)(splitmix64(seed + (uint64_t)i) >> 11) * 0x1.0p-53 - 0.5);
+}
+
 // t = W s (records): t[r] = sum x_val * s[x_col] + (aid>=0 ? s[p+aid] : 0)
 __global__ void k_w_apply(const int64_t* __restrict__ xp, const int32_t* __restrict__ xc,
                           const double* __restrict__ xv, const int32_t* __restrict__ aid, int64_t n,
@@ -891,6 +899,19 @@ __global__ void k_scatter_add_rows(const T* __restrict__ yg, int64_t ldg, const 
     if (g >= 0) Y[c * ldy + g] += alpha * yg[c * ldg + i];
 }
 
+// Split-K shape of one k_dense_mfma launch: from the tuned table, or a
+// shape being timed by the tuner on this thread
+struct DenseSplit {
+    int target_blocks, min_kspan;
+};
+thread_local const DenseSplit* g_split_probe = nullptr;
+DenseSplit dense_split(int elem, int64_t qn, int rc) {
+    if (g_split_probe) return *g_split_probe;
+    for (const DenseTunedEntry& e : kDenseTuned)
+        if (e.elem == elem && qn <= e.q_max && rc <= e.r_max) return {e.target_blocks, e.min_kspan};
+    return {2048, 256};
+}
+
 // Y[:, c] += coef (v' x_c) v, a rank-1 term: v, and the per-column partial
 // dots of v' x_c (DOT_CHUNKS per column) already on the device
 struct StiffTerm {
@@ -921,8 +942,9 @@ void dense_apply(cb_mme* M, int64_t qn, const T* X, int64_t ldx, T* Y, int64_t l
         return;
     }
     const int64_t row_tiles = ceil_div(qn, MF_ROWS);
-    // about 2048 blocks, splits of at least 256 k
-    int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(2048, row_tiles), qn / 256));
+    // about target_blocks blocks, splits of at least min_kspan k (tuned table)
+    const DenseSplit sp = dense_split((int)sizeof(T), qn, std::min(r, MF_MAXNT * 16));
+    int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(sp.target_blocks, row_tiles), qn / sp.min_kspan));
     const int64_t kspan = round_up(ceil_div(qn, nsplit), MF_KC);
     nsplit = (int)ceil_div(qn, kspan);
     const int64_t ldp = round_up(qn, 64);
@@ -2066,6 +2088,75 @@ int cb_mme_apply(cb_mme* M, double lambda, int64_t r, const double* X, double* Y
         apply_C<double>(M, lambda, dX.p, dY.p, (int)r);
         copy_host_2d(st, Y, M->dim * 8, dY.p, ld * 8, M->dim * 8, r, hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
+    });
+}
+
+}  // extern "C"
+
+// The tuner's probe: one dense operator shape with a given split.
+template <class T>
+static double tune_dense_time(cb_context* ctx, int64_t q, int r, const DenseSplit& sp, int reps) {
+    hipStream_t st = ctx->stream;
+    cb_mme M;
+    M.ctx = ctx;
+    M.q = q;
+    M.ldq = round_up(q, 32);
+    M.qkind = 0;
+    const int64_t ldx = round_up(q, 64);
+    DevBuf<T> X, Y;
+    X.alloc((size_t)ldx * r);
+    Y.alloc((size_t)ldx * r);
+    DevBuf<T>& Qb = [&]() -> DevBuf<T>& {
+        if constexpr (sizeof(T) == 8) return M.Qd64;
+        else return M.Qd32;
+    }();
+    Qb.alloc((size_t)q * M.ldq);
+    k_tune_fill<T><<<blocks((int64_t)Qb.n), 256, 0, st>>>(Qb.p, (int64_t)Qb.n, 1);
+    k_tune_fill<T><<<blocks((int64_t)X.n), 256, 0, st>>>(X.p, (int64_t)X.n, 2);
+    CB_HIP(hipMemsetAsync(Y.p, 0, Y.bytes(), st));
+    CB_HIP(hipGetLastError());
+    struct Probe {
+        Probe(const DenseSplit* s) { g_split_probe = s; }
+        ~Probe() { g_split_probe = nullptr; }
+    } probe(&sp);
+    dense_apply<T>(&M, q, X.p, ldx, Y.p, ldx, r, (T)1);  // warm-up (sizes the partial-sum buffer)
+    hipEvent_t e0, e1;
+    CB_HIP(hipEventCreate(&e0));
+    CB_HIP(hipEventCreate(&e1));
+    float ms = 0.f;
+    CB_HIP(hipEventRecord(e0, st));
+    for (int i = 0; i < reps; ++i) dense_apply<T>(&M, q, X.p, ldx, Y.p, ldx, r, (T)1);
+    CB_HIP(hipEventRecord(e1, st));
+    const hipError_t e = hipEventSynchronize(e1);
+    if (e == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    CB_HIP(e);
+    return (double)ms / reps;
+}
+
+extern "C" {
+
+int cb_tune_dense_time(cb_context* ctx, int elem_bytes, int64_t q, int r, int target_blocks, int min_kspan, int reps,
+                       double* ms) {
+    return guarded([&] {
+        CB_REQUIRE(ctx && ms, "cb_tune_dense_time: NULL argument");
+        CB_REQUIRE(elem_bytes == 4 || elem_bytes == 8, "cb_tune_dense_time: elem_bytes must be 4 or 8");
+        CB_REQUIRE(q >= 64 && q <= 200000 && r >= 3 && r <= MF_MAXNT * 16 && reps >= 1,
+                   "cb_tune_dense_time: q in [64, 200000], r in [3, 80], reps >= 1");
+        CB_REQUIRE(target_blocks >= 1 && min_kspan >= MF_KC, "cb_tune_dense_time: target_blocks >= 1, min_kspan >= 32");
+        set_device(ctx);
+        const DenseSplit sp{target_blocks, min_kspan};
+        *ms = elem_bytes == 8 ? tune_dense_time<double>(ctx, q, r, sp, reps) : tune_dense_time<float>(ctx, q, r, sp, reps);
+    });
+}
+
+int cb_tune_dense_lookup(int elem_bytes, int64_t q, int r, int* target_blocks, int* min_kspan) {
+    return guarded([&] {
+        CB_REQUIRE(target_blocks && min_kspan, "cb_tune_dense_lookup: NULL argument");
+        const DenseSplit sp = dense_split(elem_bytes, q, std::min(r, MF_MAXNT * 16));
+        *target_blocks = sp.target_blocks;
+        *min_kspan = sp.min_kspan;
     });
 }
 

@@ -1,0 +1,135 @@
+"""Autotuner for the split-K shape of the dense operator kernel.
+
+`k_dense_mfma` computes Q X for a dense q x q block Q (G^-1, or the genotyped
+block of H^-1) and an r-column block X. Its grid is (row tiles of 128) x
+(k splits); a split's partial products are summed in a fixed order by
+`k_dense_reduce`. How many splits pay off depends on q (row tiles fill the
+chip at large q only), on r and on the element size, so the library reads
+the split from a table, `csrc/cb_dense_tuned.h`, that this tuner writes:
+
+    python -m cosmicblup_amd.tune            # on an MI355X; rewrites the table
+    python -m cosmicblup_amd.tune --dry-run  # print the table only
+
+For each element size (FP32, FP64), each representative q and r, every
+(target blocks, minimum k span) candidate is timed through
+`cb_tune_dense_time` (HIP events on the library's stream, synthetic
+operands), and the fastest is kept. A table entry covers q up to the
+geometric midpoint to the next representative q and r up to the next r
+bucket. The library is rebuilt with the new table (`__graft_entry__.build()`).
+
+The split changes which partial products are summed together, so results
+depend on the table in the last bits; for a given table they are
+deterministic (the table is a build input, never consulted at run time
+against timings)."""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import math
+import time
+from pathlib import Path
+
+from . import _lib
+
+TABLE = Path(__file__).resolve().parent / "csrc" / "cb_dense_tuned.h"
+Q_POINTS = (600, 1200, 2500, 5000, 10000, 20000, 40000)
+R_BUCKETS = ((5, 8), (17, 24), (33, 48), (65, 80))  # (r timed, r_max of the entry)
+TARGETS = (512, 1024, 2048, 4096, 8192)
+KSPANS = (32, 64, 128, 256, 512)
+DEFAULT = (2048, 256)
+
+
+def time_shape(ctx: _lib.Context, elem: int, q: int, r: int, target: int, kspan: int, reps: int) -> float:
+    ms = C.c_double()
+    _lib.check(_lib.lib().cb_tune_dense_time(ctx.handle, elem, q, r, target, kspan, reps, C.byref(ms)))
+    return ms.value
+
+
+def lookup(elem: int, q: int, r: int) -> tuple[int, int]:
+    tb, mk = C.c_int(), C.c_int()
+    _lib.check(_lib.lib().cb_tune_dense_lookup(elem, q, r, C.byref(tb), C.byref(mk)))
+    return tb.value, mk.value
+
+
+def reps_for(elem: int, q: int, r: int) -> int:
+    # about 20 ms of work per candidate at the FP32 MFMA rate, 3..50 launches
+    flops = 2.0 * q * q * r
+    per = flops / (60e12 if elem == 4 else 30e12)
+    return int(min(50, max(3, 0.02 / max(per, 1e-6))))
+
+
+def tune(ctx: _lib.Context, log=print) -> list[dict]:
+    entries = []
+    for elem in (4, 8):
+        for qi, q in enumerate(Q_POINTS):
+            q_max = int(math.sqrt(q * Q_POINTS[qi + 1])) if qi + 1 < len(Q_POINTS) else 1 << 40
+            for r, r_max in R_BUCKETS:
+                reps = reps_for(elem, q, r)
+                base = time_shape(ctx, elem, q, r, *DEFAULT, reps)
+                best = (base, DEFAULT)
+                for tb in TARGETS:
+                    for mk in KSPANS:
+                        if (tb, mk) == DEFAULT:
+                            continue
+                        t = time_shape(ctx, elem, q, r, tb, mk, reps)
+                        if t < best[0]:
+                            best = (t, (tb, mk))
+                # keep the default unless a candidate is clearly faster (timing noise)
+                t_best, (tb, mk) = best if best[0] < 0.97 * base else (base, DEFAULT)
+                entries.append({"elem": elem, "q": q, "q_max": q_max, "r": r, "r_max": r_max,
+                                "target_blocks": tb, "min_kspan": mk, "ms": t_best, "ms_default": base})
+                log(f"[tune] {'FP32' if elem == 4 else 'FP64'} q={q} r={r}: blocks {tb}, k span {mk}: "
+                    f"{t_best:.4f} ms (default {base:.4f} ms)")
+    return entries
+
+
+def render(entries: list[dict], device: str) -> str:
+    lines = [
+        "// Split-K shapes of k_dense_mfma (dense Q X), picked per (element size, q,",
+        "// r) by cosmicblup_amd/tune.py from timings on an MI355X. Generated file:",
+        "// rerun the tuner to replace it. An entry applies to q <= q_max and r <=",
+        "// r_max (first match in this order); shapes past the table use 2048 blocks",
+        "// and k spans of at least 256.",
+        f"// Tuned on {device}, {time.strftime('%Y-%m-%d')}; ms = tuned / default time of one launch.",
+        "#pragma once",
+        "#include <cstdint>",
+        "",
+        "struct DenseTunedEntry {",
+        "    int elem;            // bytes per element of Q and X (4: FP32, 8: FP64)",
+        "    int64_t q_max;",
+        "    int r_max;",
+        "    int target_blocks;   // blocks aimed at (row tiles x k splits)",
+        "    int min_kspan;       // smallest k range of one split",
+        "};",
+        "",
+        "static const DenseTunedEntry kDenseTuned[] = {",
+    ]
+    for e in entries:
+        lines.append(f"    {{{e['elem']}, {e['q_max']}LL, {e['r_max']}, {e['target_blocks']}, {e['min_kspan']}}},"
+                     f"  // q {e['q']}, r {e['r']}: {e['ms']:.4f} / {e['ms_default']:.4f} ms")
+    lines += ["};", ""]
+    return "\n".join(lines)
+
+
+def main(argv=None) -> None:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--dry-run", action="store_true", help="print the table instead of writing it")
+    ap.add_argument("--out", type=Path, default=TABLE)
+    args = ap.parse_args(argv)
+    import torch
+
+    ctx = _lib.Context(0)
+    device = "device 0"
+    if torch.cuda.is_available():
+        props = torch.cuda.get_device_properties(0)
+        device = f"{props.name} ({getattr(props, 'gcnArchName', 'unknown arch')})"
+    text = render(tune(ctx), device)
+    if args.dry_run:
+        print(text)
+    else:
+        args.out.write_text(text)
+        print(f"[tune] wrote {args.out}")
+
+
+if __name__ == "__main__":
+    main()

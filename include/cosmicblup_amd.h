@@ -353,6 +353,20 @@ int cb_ainv_copy_csr(const cb_ainv* a, int64_t* ptr, int32_t* col, double* val);
 int cb_ainv_copy_inbreeding(const cb_ainv* a, double* F, double* dii);
 void cb_ainv_destroy(cb_ainv* a);
 
+/* ---- operator tuning -------------------------------------------------- */
+/* No counterpart in the reference: the autotuner's interface
+ * (cosmicblup_amd/tune.py). The dense operator Q X (k_dense_mfma) splits the
+ * k range of Q over blocks; the split is looked up per (element size, q, r)
+ * in a table the tuner generates (cb_dense_tuned.h).
+ * cb_tune_dense_time: average ms of one Q X on synthetic q x q Q and r
+ * columns (elem_bytes 4: FP32, 8: FP64; q in [64, 200000], r in [3, 80])
+ * with about target_blocks blocks and splits of at least min_kspan k, over
+ * reps launches after one warm-up.
+ * cb_tune_dense_lookup: the split the library uses for that shape. */
+int cb_tune_dense_time(cb_context* ctx, int elem_bytes, int64_t q, int r, int target_blocks, int min_kspan, int reps,
+                       double* ms);
+int cb_tune_dense_lookup(int elem_bytes, int64_t q, int r, int* target_blocks, int* min_kspan);
+
 #ifdef __cplusplus
 }
 #endif

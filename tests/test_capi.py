@@ -45,3 +45,28 @@ def test_last_error_without_device():
     # invalid arguments are rejected before any device work
     assert L.cb_grm_create(None, None, None) == _lib.CB_ERR_INVALID
     assert b"NULL" in L.cb_last_error()
+
+
+def test_dense_split_lookup_follows_tuned_table():
+    """cb_tune_dense_lookup returns the first entry of the generated table
+    (csrc/cb_dense_tuned.h) whose element size matches and whose q_max and
+    r_max cover the shape, else the built-in 2048 blocks / 256 k."""
+    import re
+
+    from cosmicblup_amd import tune
+
+    rows = [tuple(int(v) for v in m.groups()) for m in
+            re.finditer(r"\{(\d+), (\d+)LL, (\d+), (\d+), (\d+)\}", tune.TABLE.read_text())]
+    assert rows, "no tuned entries"
+
+    def expect(elem, q, r):
+        for e, q_max, r_max, tb, mk in rows:
+            if e == elem and q <= q_max and min(r, 80) <= r_max:
+                return tb, mk
+        return tune.DEFAULT
+
+    for elem in (4, 8):
+        for q in (64, 599, 600, 849, 850, 3000, 19999, 20000, 28284, 28285, 10 ** 6):
+            for r in (3, 8, 9, 24, 25, 48, 49, 65, 80, 200):
+                assert tune.lookup(elem, q, r) == expect(elem, q, r), (elem, q, r)
+    assert tune.lookup(2, 1000, 10) == tune.DEFAULT

@@ -377,3 +377,38 @@ def test_graph_replay_equals_plain_launches(cb, ctx, precision, qkind):
         assert it == it0
         assert rel == rel0
         assert np.array_equal(X, X0)
+
+
+def test_contexts_on_threads_match_serial(cb, ctx):
+    """One context per host thread, each with its own models: the same
+    estimates, bit for bit, as the same calls made one after another (the
+    library's host copies share one staging buffer; its device work shares
+    nothing between contexts)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    ctxs = [_lib.Context(0) for _ in range(3)]
+    models = []
+    for w, c in enumerate(ctxs):
+        for j in range(2):
+            fd, aid, y, Q = gblup_problem(n=250 + 90 * (2 * w + j), m=1200, seed=40 + 2 * w + j)
+            M = mme.MixedModel(fd, aid, y, Q.shape[0], ctx=c)
+            M.set_qinv_dense(Q)
+            models.append((w, M))
+    opts = mme.reml_options(n_probes=12, seed=9, solve=mme.solve_options(tol=1e-10))
+
+    def run(M):
+        R = mme.AIREML(M, opts)
+        R.setInitialVars(0.5, 0.5)
+        R.solve()
+        X, it, _ = M.solve(1.3, M.rhs(), mme.solve_options(precision=_lib.CB_PREC_FP64, tol=1e-12))
+        return R.iterations, R.var_u, R.var_e, R.solution.tobytes(), X.tobytes(), it
+
+    serial = [run(M) for _, M in models]
+
+    def worker(w):
+        return [(i, run(M)) for i, (wk, M) in enumerate(models) if wk == w]
+
+    with ThreadPoolExecutor(len(ctxs)) as ex:
+        par = dict(sum(ex.map(worker, range(len(ctxs))), []))
+    for i, ref in enumerate(serial):
+        assert par[i] == ref
