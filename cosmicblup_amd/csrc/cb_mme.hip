@@ -343,15 +343,18 @@ __global__ __launch_bounds__(256) void k_gemv(const T* __restrict__ Q, int64_t l
     const bool two = r > 1;
     for (int64_t v0 = lane; v0 < nv; v0 += 64 * GEMV_PF) {
         VT w[GEMV_PF][GEMV_ROWS];
-        T x0[GEMV_PF][VW], x1[GEMV_PF][VW];
+        VT x0[GEMV_PF], x1[GEMV_PF];
 #pragma unroll
         for (int t = 0; t < GEMV_PF; ++t) {
             const int64_t v = min(v0 + 64 * t, nv - 1);  // a run past the end is loaded (clamped), not used
-#pragma unroll
-            for (int e = 0; e < VW; ++e) {
-                x0[t][e] = X[v * VW + e];
-                x1[t][e] = two ? X[ldx + v * VW + e] : (T)0;
-            }
+            // x runs as 16-byte loads: X sits at the random-effect offset p of
+            // a block, so only element alignment is known (one unaligned
+            // dwordx4 rather than VW dword loads per run and column)
+            __builtin_memcpy(&x0[t], X + v * VW, sizeof(VT));
+            if (two)
+                __builtin_memcpy(&x1[t], X + ldx + v * VW, sizeof(VT));
+            else
+                x1[t] = VT{};
 #pragma unroll
             for (int a = 0; a < GEMV_ROWS; ++a) w[t][a] = __builtin_nontemporal_load(qr[a] + v);
         }
