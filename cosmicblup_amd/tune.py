@@ -13,9 +13,10 @@ the split from a table, `csrc/cb_dense_tuned.h`, that this tuner writes:
 For each element size (FP32, FP64), each representative q and r, every
 (target blocks, minimum k span) candidate is timed through
 `cb_tune_dense_time` (HIP events on the library's stream, synthetic
-operands), and the fastest is kept. A table entry covers q up to the
-geometric midpoint to the next representative q and r up to the next r
-bucket. The library is rebuilt with the new table (`__graft_entry__.build()`).
+operands) in three interleaved passes, and the fastest (best of its three)
+is kept when it beats the built-in split by more than 5%. A table entry
+covers q up to the geometric midpoint to the next representative q and r up
+to the next r bucket. The library is rebuilt with the new table (`__graft_entry__.build()`).
 
 The split changes which partial products are summed together, so results
 depend on the table in the last bits; for a given table they are
@@ -37,6 +38,8 @@ R_BUCKETS = ((5, 8), (17, 24), (33, 48), (65, 80))  # (r timed, r_max of the ent
 TARGETS = (512, 1024, 2048, 4096, 8192)
 KSPANS = (32, 64, 128, 256, 512)
 DEFAULT = (2048, 256)
+ROUNDS = 3       # interleaved timing passes per shape
+MARGIN = 0.05    # a candidate must beat the default by this fraction
 
 
 def time_shape(ctx: _lib.Context, elem: int, q: int, r: int, target: int, kspan: int, reps: int) -> float:
@@ -65,17 +68,18 @@ def tune(ctx: _lib.Context, log=print) -> list[dict]:
             q_max = int(math.sqrt(q * Q_POINTS[qi + 1])) if qi + 1 < len(Q_POINTS) else 1 << 40
             for r, r_max in R_BUCKETS:
                 reps = reps_for(elem, q, r)
-                base = time_shape(ctx, elem, q, r, *DEFAULT, reps)
-                best = (base, DEFAULT)
-                for tb in TARGETS:
-                    for mk in KSPANS:
-                        if (tb, mk) == DEFAULT:
-                            continue
-                        t = time_shape(ctx, elem, q, r, tb, mk, reps)
-                        if t < best[0]:
-                            best = (t, (tb, mk))
-                # keep the default unless a candidate is clearly faster (timing noise)
-                t_best, (tb, mk) = best if best[0] < 0.97 * base else (base, DEFAULT)
+                cands = [DEFAULT] + [(tb, mk) for tb in TARGETS for mk in KSPANS if (tb, mk) != DEFAULT]
+                # ROUNDS interleaved passes over all candidates, best time of each:
+                # clock drift during a sweep then favours no candidate
+                times = {c: math.inf for c in cands}
+                for _ in range(ROUNDS):
+                    for c in cands:
+                        times[c] = min(times[c], time_shape(ctx, elem, q, r, *c, reps))
+                base = times[DEFAULT]
+                (tb, mk), t_best = min(times.items(), key=lambda kv: kv[1])
+                # keep the default unless a candidate is clearly faster
+                if not t_best < (1.0 - MARGIN) * base:
+                    (tb, mk), t_best = DEFAULT, base
                 entries.append({"elem": elem, "q": q, "q_max": q_max, "r": r, "r_max": r_max,
                                 "target_blocks": tb, "min_kspan": mk, "ms": t_best, "ms_default": base})
                 log(f"[tune] {'FP32' if elem == 4 else 'FP64'} q={q} r={r}: blocks {tb}, k span {mk}: "
