@@ -123,6 +123,8 @@ _SIGS = {
     "cb_mme_apply": (C.c_int, [c_vp, C.c_double, c_i64, c_vp, c_vp]),
     "cb_solve_options_default": (None, [C.POINTER(SolveOptions)]),
     "cb_mme_solve": (C.c_int, [c_vp, C.c_double, C.POINTER(SolveOptions), c_i64, c_vp, c_vp, c_i64p, c_dp]),
+    "cb_mme_solve_device": (C.c_int, [c_vp, C.c_double, C.POINTER(SolveOptions), c_i64, c_vp, c_i64, c_vp, c_i64,
+                                      c_i64p, c_dp]),
     "cb_tune_dense_time": (C.c_int, [c_vp, C.c_int, c_i64, C.c_int, C.c_int, C.c_int, C.c_int, c_dp]),
     "cb_tune_dense_lookup": (C.c_int, [C.c_int, c_i64, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cb_reml_options_default": (None, [C.POINTER(RemlOptions)]),

@@ -54,21 +54,61 @@ constexpr int MF_MAXNT = 5;       // 16-column subtiles per launch (80 columns)
 // ---------------------------------------------------------------------------
 // Generic device kernels
 
-// Y[row, c] = alpha * sum_k val(row,k) X[col(k), c] (+ beta_acc * Y[row, c])
-// thread per (row, c) for rows in [row0, row1)
-template <class TM, class TV>
-__global__ void k_csr_spmm(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
-                           const TM* __restrict__ val, int64_t row0, int64_t row1,
-                           int64_t col_off, const TV* __restrict__ X, int64_t ldx,
-                           TV* __restrict__ Y, int64_t ldy, int64_t y_off, TV alpha, int accumulate) {
-    const int64_t row = row0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Y[row, c] = alpha * sum_k val(row,k) X[col(k), c] (+ Y[row, c] when
+// accumulating) for rows in [row0, row1): S lanes per (row, c), lane j
+// taking nonzeros j, j + S, ... of the row (consecutive lanes read
+// consecutive values and column indices), then a butterfly over the S
+// lanes. S follows the matrix's mean row length (csr_lanes); S = 1 is a
+// thread per row.
+template <class TM, class TV, int S>
+__global__ __launch_bounds__(256) void k_csr_spmm(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                                  const TM* __restrict__ val, int64_t row0, int64_t row1,
+                                                  int64_t col_off, const TV* __restrict__ X, int64_t ldx,
+                                                  TV* __restrict__ Y, int64_t ldy, int64_t y_off, TV alpha,
+                                                  int accumulate) {
+    const int sub = (int)threadIdx.x & (S - 1);
+    const int64_t row = row0 + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / S;
     const int64_t c = blockIdx.y;
-    if (row >= row1) return;
-    const TV* x = X + c * ldx + col_off;
     TV acc = 0;
-    for (int64_t k = ptr[row]; k < ptr[row + 1]; ++k) acc += (TV)val[k] * x[col[k]];
-    TV* y = Y + c * ldy + y_off + row;
-    *y = accumulate ? *y + alpha * acc : alpha * acc;
+    if (row < row1) {  // no return before the butterfly: every lane of a group takes part
+        const TV* x = X + c * ldx + col_off;
+        const int64_t e = ptr[row + 1];
+        for (int64_t k = ptr[row] + sub; k < e; k += S) acc += (TV)val[k] * x[col[k]];
+    }
+#pragma unroll
+    for (int off = S / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, S);
+    if (row < row1 && sub == 0) {
+        TV* y = Y + c * ldy + y_off + row;
+        *y = accumulate ? *y + alpha * acc : alpha * acc;
+    }
+}
+
+// lanes per row for a CSR block: the mean row length rounded up to a power
+// of two, at most 16
+inline int csr_lanes(int64_t nnz, int64_t rows) {
+    int S = 1;
+    while (S < 16 && (int64_t)S * rows < nnz) S *= 2;
+    return S;
+}
+
+template <class TM, class TV>
+void csr_spmm(hipStream_t st, int S, int r, const int64_t* ptr, const int32_t* col, const TM* val, int64_t row0,
+              int64_t row1, int64_t col_off, const TV* X, int64_t ldx, TV* Y, int64_t ldy, int64_t y_off, TV alpha,
+              int accumulate) {
+    const int64_t rows = row1 - row0;
+    if (rows <= 0) return;
+    const dim3 grid((unsigned)std::max<int64_t>(1, (rows * S + 255) / 256), (unsigned)r);
+#define CB_CSR(S_) \
+    k_csr_spmm<TM, TV, S_><<<grid, 256, 0, st>>>(ptr, col, val, row0, row1, col_off, X, ldx, Y, ldy, y_off, alpha, accumulate)
+    switch (S) {
+        case 1: CB_CSR(1); break;
+        case 2: CB_CSR(2); break;
+        case 4: CB_CSR(4); break;
+        case 8: CB_CSR(8); break;
+        default: CB_CSR(16); break;
+    }
+#undef CB_CSR
+    CB_HIP(hipGetLastError());
 }
 
 // The few long rows of W'W (fixed effects; the mean row touches every
@@ -784,6 +824,7 @@ struct cb_mme {
     DevBuf<double> wv64;
     DevBuf<float> wv32;
     int64_t long_rows = 0;  // rows [0, long_rows) use the chunked two-stage SpMM
+    int ww_lanes = 1, q_lanes = 1;  // lanes per row of the W'W rows past long_rows and of a CSR Q
     // chunks of at most LONG_CHUNK nonzeros of the long rows, in row order
     int64_t n_lchunks = 0;
     DevBuf<int64_t> lc_beg, lc_end, lr_start;
@@ -1012,8 +1053,7 @@ void q_apply(cb_mme* M, const T* X, int64_t ldx, T* Y, int64_t ldy, int r, T alp
         dense_apply<T>(M, q, X + p, ldx, Y + p, ldy, r, alpha, sk);
         return;
     }
-    k_csr_spmm<T, T><<<dim3(blocks(q), (unsigned)r), 256, 0, st>>>(M->qp.p, M->qc.p, qvals<T>(M), 0, q, p, X, ldx,
-                                                                   Y, ldy, p, alpha, 1);
+    csr_spmm<T, T>(st, M->q_lanes, r, M->qp.p, M->qc.p, qvals<T>(M), 0, q, p, X, ldx, Y, ldy, p, alpha, 1);
     CB_HIP(hipGetLastError());
     if (M->qkind == 2) {
         const int64_t ng = M->ng, ldg = round_up(ng, 64);
@@ -1047,8 +1087,8 @@ void apply_C(cb_mme* M, double lambda, const T* X, T* Y, int r) {
     }
     if (M->dim > M->long_rows)
         timed(ctx, sizeof(T) == 8 ? "k_csr_spmm_f64" : "k_csr_spmm_f32", [&] {
-            k_csr_spmm<T, T><<<dim3(blocks(M->dim - M->long_rows), (unsigned)r), 256, 0, st>>>(
-                M->wp.p, M->wc.p, wvals<T>(M), M->long_rows, M->dim, 0, X, ld, Y, ld, 0, (T)1, 0);
+            csr_spmm<T, T>(st, M->ww_lanes, r, M->wp.p, M->wc.p, wvals<T>(M), M->long_rows, M->dim, 0, X, ld, Y, ld, 0,
+                           (T)1, 0);
         });
     CB_HIP(hipGetLastError());
     // lambda Q part; the FP32 copy of a dense Q holds Q - mu v v', and
@@ -1607,6 +1647,7 @@ static void assemble(cb_mme* M, const int64_t* x_ptr, const int32_t* x_col, cons
     for (int64_t r = 0; r < p; ++r)
         if (wp[r + 1] - wp[r] > 32) long_rows = r + 1;
     M->long_rows = long_rows;
+    M->ww_lanes = csr_lanes(wp[dim] - wp[long_rows], dim - long_rows);
     std::vector<int64_t> lcb, lce, lrs(long_rows + 1, 0);
     for (int64_t r = 0; r < long_rows; ++r) {
         for (int64_t k = wp[r]; k < wp[r + 1]; k += LONG_CHUNK) {
@@ -1968,6 +2009,7 @@ static void set_q_csr_common(cb_mme* M, const int64_t* ptr, const int32_t* col, 
             if (col[k] == i) M->q_diag[i] += val[k];
         }
     hipStream_t st = M->ctx->stream;
+    M->q_lanes = csr_lanes(nnz, q);
     M->qp.alloc(q + 1);
     M->qc.alloc(std::max<int64_t>(1, nnz));
     M->qv64.alloc(std::max<int64_t>(1, nnz));
@@ -2163,39 +2205,62 @@ int cb_tune_dense_lookup(int elem_bytes, int64_t q, int r, int* target_blocks, i
     });
 }
 
-int cb_mme_solve(cb_mme* M, double lambda, const cb_solve_options* opts, int64_t r, const double* B,
-                 double* X, int64_t* iterations, double* rel_residual) {
+}  // extern "C"
+
+// cb_mme_solve / cb_mme_solve_device: B and X are dim x r column blocks with
+// leading dimensions ldb, ldx, in host memory (kind H2D / D2H) or in device
+// memory (D2D). The work blocks pad each column to ldv rows with zeros.
+static int mme_solve_blocks(cb_mme* M, double lambda, const cb_solve_options* opts, int64_t r, const double* B,
+                            int64_t ldb, double* X, int64_t ldx, bool on_device, int64_t* iterations,
+                            double* rel_residual, const char* who) {
     cb_solve_options o;
     cb_solve_options_default(&o);
     if (opts) o = *opts;
     int64_t its = 0;
     double rel = 0;
     const int st = guarded([&] {
-        CB_REQUIRE(M && B && X && r > 0, "cb_mme_solve: bad argument");
-        CB_REQUIRE(M->qkind >= 0, "cb_mme_solve: Qinv not set");
-        CB_REQUIRE(lambda >= 0 && std::isfinite(lambda), "cb_mme_solve: lambda must be finite and >= 0");
+        CB_REQUIRE(M && B && X && r > 0, std::string(who) + ": bad argument");
+        CB_REQUIRE(ldb >= M->dim && ldx >= M->dim, std::string(who) + ": leading dimension < p + q");
+        CB_REQUIRE(M->qkind >= 0, std::string(who) + ": Qinv not set");
+        CB_REQUIRE(lambda >= 0 && std::isfinite(lambda), std::string(who) + ": lambda must be finite and >= 0");
         set_device(M->ctx);
         hipStream_t s = M->ctx->stream;
         const int64_t ld = M->ldv;
+        const hipMemcpyKind in = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        const hipMemcpyKind out = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
         const WsPtr<double> dB = ws<double>(M, WS_IO_B, (size_t)ld * r), dX = ws<double>(M, WS_IO_X, (size_t)ld * r);
         CB_HIP(hipMemsetAsync(dB.p, 0, dB.bytes(), s));
-        copy_host_2d(s, dB.p, ld * 8, B, M->dim * 8, M->dim * 8, r, hipMemcpyHostToDevice);
+        copy_host_2d(s, dB.p, ld * 8, B, ldb * 8, M->dim * 8, r, in);
         try {
             SolveStats S = solve_block(M, lambda, o, dB.p, dX.p, (int)r);
             its = S.iterations;
             rel = S.max_rel;
         } catch (const Error& e) {
             // hand back the unconverged iterate (matrix_free_pcg.h:64-65 returns it silently)
-            copy_host_2d(s, X, M->dim * 8, dX.p, ld * 8, M->dim * 8, r, hipMemcpyDeviceToHost);
+            copy_host_2d(s, X, ldx * 8, dX.p, ld * 8, M->dim * 8, r, out);
             CB_HIP(hipStreamSynchronize(s));
             throw;
         }
-        copy_host_2d(s, X, M->dim * 8, dX.p, ld * 8, M->dim * 8, r, hipMemcpyDeviceToHost);
+        copy_host_2d(s, X, ldx * 8, dX.p, ld * 8, M->dim * 8, r, out);
         CB_HIP(hipStreamSynchronize(s));
     });
     if (iterations) *iterations = its;
     if (rel_residual) *rel_residual = rel;
     return st;
+}
+
+extern "C" {
+
+int cb_mme_solve(cb_mme* M, double lambda, const cb_solve_options* opts, int64_t r, const double* B,
+                 double* X, int64_t* iterations, double* rel_residual) {
+    const int64_t dim = M ? M->dim : 0;
+    return mme_solve_blocks(M, lambda, opts, r, B, dim, X, dim, false, iterations, rel_residual, "cb_mme_solve");
+}
+
+int cb_mme_solve_device(cb_mme* M, double lambda, const cb_solve_options* opts, int64_t r, const double* dB,
+                        int64_t ldb, double* dX, int64_t ldx, int64_t* iterations, double* rel_residual) {
+    return mme_solve_blocks(M, lambda, opts, r, dB, ldb, dX, ldx, true, iterations, rel_residual,
+                            "cb_mme_solve_device");
 }
 
 }  // extern "C"

@@ -124,6 +124,18 @@ class MixedModel:
                                            C.byref(it), C.byref(rel)))
         return X.T.reshape(np.asarray(B).shape), it.value, rel.value
 
+    def solve_device(self, lam: float, dB: int, ldb: int, dX: int, ldx: int, r: int,
+                     opts: _lib.SolveOptions | None = None):
+        """solve() on device-resident column blocks (device pointers with
+        leading dimensions >= dim, e.g. torch tensors of shape (r, ld));
+        returns (iterations, max relative residual)."""
+        opts = opts or solve_options()
+        it = C.c_int64()
+        rel = C.c_double()
+        _lib.check(_lib.lib().cb_mme_solve_device(self.handle, lam, C.byref(opts), r, C.c_void_p(dB), ldb,
+                                                  C.c_void_p(dX), ldx, C.byref(it), C.byref(rel)))
+        return it.value, rel.value
+
 
 class AIREML:
     """VCESolver for the single-component model (vce.h:223-390)."""

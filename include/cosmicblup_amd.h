@@ -213,6 +213,13 @@ void cb_solve_options_default(cb_solve_options* opts);
  * iteration cap is hit. */
 int cb_mme_solve(cb_mme* mme, double lambda, const cb_solve_options* opts, int64_t r,
                  const double* B, double* X, int64_t* iterations, double* rel_residual);
+/* cb_mme_solve with B and X in device memory on this context's device
+ * ((p+q) x r column blocks, leading dimensions ldb, ldx >= p+q): for callers
+ * that keep right-hand sides and solutions on the GPU (no host transfer).
+ * The work that produced B must be finished before the call. */
+int cb_mme_solve_device(cb_mme* mme, double lambda, const cb_solve_options* opts, int64_t r,
+                        const double* dB, int64_t ldb, double* dX, int64_t ldx, int64_t* iterations,
+                        double* rel_residual);
 
 /* ---- REML -------------------------------------------------------------- */
 #define CB_VCE_AI 0  /* data-driven AI step, vce.cpp:2414-2857 */

@@ -412,3 +412,30 @@ def test_contexts_on_threads_match_serial(cb, ctx):
         par = dict(sum(ex.map(worker, range(len(ctxs))), []))
     for i, ref in enumerate(serial):
         assert par[i] == ref
+
+
+@pytest.mark.parametrize("precision", [_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED])
+def test_device_resident_solve_equals_host_solve(cb, ctx, precision):
+    """cb_mme_solve_device on torch-owned blocks with padded leading
+    dimensions gives the host-array solve's iterates bit for bit, and leaves
+    the padding rows of X alone."""
+    import torch
+
+    fd, aid, y, Q = gblup_problem(n=300, m=1500, missing_records=9)
+    M = mme.MixedModel(fd, aid, y, Q.shape[0])
+    M.set_qinv_dense(Q)
+    B = np.zeros((M.dim, 3))
+    B[:, 0] = M.rhs()
+    B[M.p:, 1:] = orl.probes(M.q, 2, seed=4)
+    opts = mme.solve_options(precision=precision, tol=1e-12 if precision == _lib.CB_PREC_FP64 else 1e-10)
+    X, it, rel = M.solve(1.4, B, opts)
+    ldb, ldx = M.dim + 5, M.dim + 11
+    dB = torch.zeros((3, ldb), dtype=torch.float64, device="cuda")
+    dB[:, :M.dim] = torch.from_numpy(B.T.copy()).cuda()
+    dX = torch.full((3, ldx), -7.0, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    it_d, rel_d = M.solve_device(1.4, dB.data_ptr(), ldb, dX.data_ptr(), ldx, 3, opts)
+    Xd = dX.cpu().numpy()
+    assert it_d == it and rel_d == rel
+    assert np.array_equal(Xd[:, :M.dim].T, X)
+    assert np.all(Xd[:, M.dim:] == -7.0)
