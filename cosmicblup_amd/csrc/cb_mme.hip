@@ -133,15 +133,22 @@ __global__ __launch_bounds__(256) void k_csr_spmm_chunks(const int64_t* __restri
     if (threadIdx.x == 0) part[c * nchunks + ch] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// a wave per (row, column): lane l sums the row's chunks l, l + 64, ... in
+// order, then a fixed butterfly over the wave (the mean row of W'W has
+// hundreds of chunks, which one thread would add one dependent load at a time)
 template <class TV>
-__global__ void k_csr_chunks_finish(const int64_t* __restrict__ rstart, int64_t rows, int64_t nchunks,
-                                    const double* __restrict__ part, TV* __restrict__ Y, int64_t ldy) {
-    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_csr_chunks_finish(const int64_t* __restrict__ rstart, int64_t rows,
+                                                           int64_t nchunks, const double* __restrict__ part,
+                                                           TV* __restrict__ Y, int64_t ldy) {
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int64_t c = blockIdx.y;
-    if (row >= rows) return;
+    if (row >= rows) return;  // whole waves
     double s = 0;
-    for (int64_t k = rstart[row]; k < rstart[row + 1]; ++k) s += part[c * nchunks + k];
-    Y[c * ldy + row] = (TV)s;
+    for (int64_t k = rstart[row] + lane; k < rstart[row + 1]; k += 64) s += part[c * nchunks + k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) Y[c * ldy + row] = (TV)s;
 }
 
 // MFMA 16x16x4 (FP32 or FP64). Both take lane l's operands as row (l & 15)
@@ -437,26 +444,31 @@ __global__ __launch_bounds__(256) void k_gemv(const T* __restrict__ Q, int64_t l
     }
 }
 
-// part[c][chunk] = sum over the chunk's rows of A[row,c] * B[row,c] (FP64)
-template <class T>
-__global__ __launch_bounds__(256) void k_col_dots(const T* __restrict__ A, const T* __restrict__ B,
+// part[c][chunk] = sum over the chunk's rows of A[row,c] * B[row,c] (FP64),
+// NTH threads per chunk (1024 for long vectors, so that 128 chunks still
+// fill the chip)
+template <class T, int NTH = 256>
+__global__ __launch_bounds__(NTH) void k_col_dots(const T* __restrict__ A, const T* __restrict__ B,
                                                   int64_t ld, int64_t rows,
                                                   double* __restrict__ part) {
-    __shared__ double red[256];
+    __shared__ double red[NTH];
     const int c = blockIdx.y;
     const int64_t per = (rows + DOT_CHUNKS - 1) / DOT_CHUNKS;
     const int64_t r0 = (int64_t)blockIdx.x * per, r1 = min(rows, r0 + per);
     double s = 0;
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256)
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += NTH)
         s += (double)A[(int64_t)c * ld + i] * (double)B[(int64_t)c * ld + i];
     red[threadIdx.x] = s;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = NTH / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0) part[(int64_t)c * DOT_CHUNKS + blockIdx.x] = red[0];
 }
+
+// vectors at least this long take 1024 threads per dot chunk
+constexpr int64_t DOT_LONG_ROWS = int64_t(1) << 18;
 
 __global__ void k_dots_finish(const double* __restrict__ part, int r, double* __restrict__ out) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -561,16 +573,16 @@ __global__ void k_cg_p(T* __restrict__ P, const T* __restrict__ Z, const ColStat
 //   k_cg_alpha_parts alpha = rho / p.q (a breakdown freezes the column)
 //   k_cg_beta_parts  rho_new = r.z, rr = r.r, convergence, beta
 //   k_cg_xr_diag     x += alpha p, r -= alpha q, then the Jacobi rows of z
-template <class T>
-__global__ __launch_bounds__(256) void k_col_dots2(const T* __restrict__ A, const T* __restrict__ B,
+template <class T, int NTH = 256>
+__global__ __launch_bounds__(NTH) void k_col_dots2(const T* __restrict__ A, const T* __restrict__ B,
                                                    const T* __restrict__ C, int64_t ld, int64_t rows,
                                                    double* __restrict__ part_ab, double* __restrict__ part_ac) {
-    __shared__ double red[2][256];
+    __shared__ double red[2][NTH];
     const int c = blockIdx.y;
     const int64_t per = (rows + DOT_CHUNKS - 1) / DOT_CHUNKS;
     const int64_t r0 = (int64_t)blockIdx.x * per, r1 = min(rows, r0 + per);
     double s = 0, u = 0;
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) {
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += NTH) {
         const double a = (double)A[(int64_t)c * ld + i];
         s += a * (double)B[(int64_t)c * ld + i];
         u += a * (double)C[(int64_t)c * ld + i];
@@ -578,7 +590,7 @@ __global__ __launch_bounds__(256) void k_col_dots2(const T* __restrict__ A, cons
     red[0][threadIdx.x] = s;
     red[1][threadIdx.x] = u;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = NTH / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
             red[0][threadIdx.x] += red[0][threadIdx.x + w];
             red[1][threadIdx.x] += red[1][threadIdx.x + w];
@@ -1082,7 +1094,7 @@ void apply_C(cb_mme* M, double lambda, const T* X, T* Y, int r) {
         if (M->n_lchunks > 0)
             k_csr_spmm_chunks<T, T><<<dim3((unsigned)M->n_lchunks, (unsigned)r), 256, 0, st>>>(
             M->lc_beg.p, M->lc_end.p, M->n_lchunks, M->wc.p, wvals<T>(M), X, ld, M->lc_part.p);
-        k_csr_chunks_finish<T><<<dim3(blocks(M->long_rows), (unsigned)r), 256, 0, st>>>(
+        k_csr_chunks_finish<T><<<dim3(blocks(M->long_rows, 4), (unsigned)r), 256, 0, st>>>(
             M->lr_start.p, M->long_rows, M->n_lchunks, M->lc_part.p, Y, ld);
     }
     if (M->dim > M->long_rows)
@@ -1227,11 +1239,18 @@ void cg_iteration(cb_mme* M, double lambda, TX* X, T* R, T* Z, T* P, T* Qp, ColS
     const dim3 g2(blocks(dim), (unsigned)r);
     const dim3 gd(DOT_CHUNKS, (unsigned)r);
     apply_C<T>(M, lambda, P, Qp, r);
-    k_col_dots<T><<<gd, 256, 0, st>>>(P, Qp, ld, dim, part);
+    const bool long_dots = dim >= DOT_LONG_ROWS;
+    if (long_dots)
+        k_col_dots<T, 1024><<<gd, 1024, 0, st>>>(P, Qp, ld, dim, part);
+    else
+        k_col_dots<T><<<gd, 256, 0, st>>>(P, Qp, ld, dim, part);
     k_cg_alpha_parts<<<blocks(r, 64), 64, 0, st>>>(cs, part, r);
     k_cg_xr_diag<TX, T><<<g2, 256, 0, st>>>(X, R, P, Qp, Z, M->dinv.p, cs, M->p, dim, ld);
     apply_precond_rest<T>(M, R, Z, r);
-    k_col_dots2<T><<<gd, 256, 0, st>>>(R, Z, R, ld, dim, part, part2);
+    if (long_dots)
+        k_col_dots2<T, 1024><<<gd, 1024, 0, st>>>(R, Z, R, ld, dim, part, part2);
+    else
+        k_col_dots2<T><<<gd, 256, 0, st>>>(R, Z, R, ld, dim, part, part2);
     k_cg_beta_parts<<<blocks(r, 64), 64, 0, st>>>(cs, part, part2, r, tol2, absolute ? 1 : 0);
     k_cg_p<T><<<g2, 256, 0, st>>>(P, Z, cs, dim, ld);
     CB_HIP(hipGetLastError());
