@@ -85,6 +85,34 @@ def test_config1_shape(cb, ctx):
     check_case(cb, x)
 
 
+def test_maf_boundary_and_dropped_snps(cb, ctx):
+    """SNPs whose p straddles the 0.01f MAF literal from both sides (minor
+    and major allele counted), an all-missing SNP and a monomorphic one,
+    among random SNPs with missing calls: the device keeps exactly the SNPs
+    the oracle keeps, and G matches."""
+    n = 1000
+    rng = np.random.default_rng(17)
+    sums = [19, 20, 21, 1979, 1980, 1981, 0, 2000]
+    xb = np.zeros((n, len(sums)), dtype=np.int8)
+    for j, sm in enumerate(sums):
+        xb[: sm // 2, j] = 2
+        if sm % 2:
+            xb[sm // 2, j] = 1
+        rng.shuffle(xb[:, j])
+    x_rand, _ = synth.simulate_dosages(n, 120, maf=(0.02, 0.5), missing_rate=0.02, seed=5)
+    x_miss = np.full((n, 1), -1, dtype=np.int8)
+    x = np.concatenate([xb[:, :3], x_rand[:, :60], x_miss, xb[:, 3:], x_rand[:, 60:]], axis=1)
+    check_case(cb, x)
+    st = og.snp_stats(synth.encode_bed(x), n, x.shape[1])
+    kept = st["valid"].astype(bool)
+    # p = 19/2000 drops, 20/2000 (exactly 0.01 > (double)0.01f) and 21/2000
+    # stay; the same counted as the major allele (1979, 1980, 1981 of 2000);
+    # monomorphic 0 and 2000; the all-missing SNP drops
+    assert list(kept[:3]) == [False, True, True]
+    assert not kept[63]
+    assert list(kept[64:69]) == [True, True, False, False, False]
+
+
 def test_all_snps_filtered(cb, ctx):
     x = np.full((64, 40), 2, dtype=np.int8)  # monomorphic everywhere
     bed = synth.encode_bed(x)
