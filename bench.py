@@ -475,8 +475,8 @@ def main() -> None:
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(one GPU's GRM is the unit; see DESIGN.md for how it would shard)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path; one GPU's share "
+              "(the GRM tiles, the probe columns of a REML block) is the unit of work", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -2,8 +2,9 @@
 FP64 refinement) and AI/EM-REML against the dense FP64 oracle on the same
 inputs and the same probe vectors.
 
-Tolerances (BASELINE metric): EBV error max|du| / max|u| <= 1e-6; variance
-components relative error <= 1e-4; the operator itself to 1e-12 relative.
+Tolerances (the accuracy bar of the mixed-precision path): EBV error
+max|du| / max|u| <= 1e-6; variance components relative error <= 1e-4; the
+operator itself to 1e-12 relative.
 """
 import numpy as np
 import pytest
