@@ -214,9 +214,11 @@ def c5_leg(ctx, n_inst: int = 96, m: int = 20000, n_lo: int = 500, n_hi: int = 8
     then the timed estimation. Reports REML (or EM) iterations/s per size
     bucket. Then the single-trait instances are served again together: each
     of `workers` host threads drives its own context (stream) through its
-    share of the instances (balanced on n^2, largest first), and the wall
-    time of the batch is compared with the serial sum; the estimates must be
-    bit-identical to the serial ones."""
+    share of them (balanced on n^2, largest first), and the wall time of the
+    batch is compared with the serial sum; the estimates must be
+    bit-identical to the serial ones. The 3-trait instances stay serial:
+    their dense solve (rocSOLVER Cholesky) gave different solutions when
+    other work ran on the GPU beside it, even a plain torch matmul loop."""
     import math
     import time
 
@@ -240,7 +242,7 @@ def c5_leg(ctx, n_inst: int = 96, m: int = 20000, n_lo: int = 500, n_hi: int = 8
         w = min(range(workers), key=lambda j: load[j])
         owner[idx] = w
         load[w] += float(sizes[idx]) ** 2
-    served = []  # (worker, idx, model, reml options, serial result)
+    served = []  # (worker, idx, job, serial result); job() reruns the timed estimation
     recs = []
     for idx, n in enumerate(sizes):
         n = int(n)
@@ -277,7 +279,14 @@ def c5_leg(ctx, n_inst: int = 96, m: int = 20000, n_lo: int = 500, n_hi: int = 8
             mctx.synchronize()
             dt = time.perf_counter() - t0
             recs.append({"n": n, "traits": 1, "iterations": R.iterations, "seconds": dt, "converged": R.converged})
-            served.append((owner[idx], idx, model, ropts, (R.iterations, R.var_u, R.var_e)))
+
+            def job(model=model, ropts=ropts):
+                R = mme.AIREML(model, ropts)
+                R.solve()
+                model.ctx.synchronize()
+                return R.iterations, R.var_u, R.var_e
+
+            served.append((owner[idx], idx, job, (R.iterations, R.var_u, R.var_e)))
         else:
             t = 3
             h2 = rng.uniform(0.1, 0.6, t)
@@ -295,15 +304,17 @@ def c5_leg(ctx, n_inst: int = 96, m: int = 20000, n_lo: int = 500, n_hi: int = 8
             E = rng.standard_normal((n, t)) @ np.linalg.cholesky(Ve).T
             X = design.dense_x(fd)
             Y = X @ rng.standard_normal((X.shape[1], t)) + U + E
-            v = multitrait.MvEigenVce(multitrait.MvEigenVce.Options(max_iter=500, tol=1e-4), ctx=ctx)
+            mctx = ctx
+            v = multitrait.MvEigenVce(multitrait.MvEigenVce.Options(max_iter=500, tol=1e-4), ctx=mctx)
             v.prepare(Y, X, G)
             ctx.synchronize()
             t0 = time.perf_counter()
             v.runNullModel()
-            beta, u = multitrait.solveMultiTrait(Y, X, Q, v.getVg(), v.getVe(), ctx=ctx)
-            ctx.synchronize()
+            beta, u = multitrait.solveMultiTrait(Y, X, Q, v.getVg(), v.getVe(), ctx=mctx)
+            mctx.synchronize()
             dt = time.perf_counter() - t0
             recs.append({"n": n, "traits": 3, "iterations": v.iterations, "seconds": dt, "converged": v.converged})
+
         del G, Q
         torch.cuda.empty_cache()
         print(f"[bench] C5 instance {idx + 1}/{n_inst}: n={n} traits={recs[-1]['traits']} "
@@ -323,15 +334,7 @@ def c5_leg(ctx, n_inst: int = 96, m: int = 20000, n_lo: int = 500, n_hi: int = 8
     out["total_seconds"] = sum(r["seconds"] for r in recs)
 
     def serve(w):
-        res = []
-        for wk, idx, model, ropts, _ in served:
-            if wk != w:
-                continue
-            R = mme.AIREML(model, ropts)
-            R.solve()
-            model.ctx.synchronize()
-            res.append((idx, (R.iterations, R.var_u, R.var_e)))
-        return res
+        return [(idx, job()) for wk, idx, job, _ in served if wk == w]
 
     for c in ctxs:
         c.synchronize()
@@ -344,7 +347,8 @@ def c5_leg(ctx, n_inst: int = 96, m: int = 20000, n_lo: int = 500, n_hi: int = 8
     out["served_single_trait"] = {
         "instances": len(served), "workers": workers, "reml_iterations": its, "wall_seconds": wall,
         "serial_seconds": serial, "speedup_vs_serial": serial / wall, "reml_iters_per_s": its / wall,
-        "identical_to_serial": all(got[idx] == ref for _, idx, _, _, ref in served)}
+        "identical_to_serial": all(got[idx] == ref for _, idx, _, ref in served),
+        "differing_instances": [idx for _, idx, _, ref in served if got[idx] != ref]}
     print(f"[bench] C5 served: {len(served)} single-trait instances on {workers} streams in {wall:.3f} s "
           f"(serial {serial:.3f} s)", flush=True)
     del served

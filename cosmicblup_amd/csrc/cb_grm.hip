@@ -1216,6 +1216,7 @@ int cb_grm_create(cb_context* ctx, const cb_grm_options* opts, cb_grm** out) {
 void cb_grm_destroy(cb_grm* grm) {
     if (!grm) return;
     (void)hipSetDevice(grm->ctx->device);
+    tls_stream = grm->ctx->stream;  // its buffers are released behind the context's queued work
     delete grm;
 }
 

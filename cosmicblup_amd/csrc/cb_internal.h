@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -56,7 +57,26 @@ int guarded(F&& f) {
     }
 }
 
-// Device buffer owned by a handle.
+// The stream of the context the calling thread's current library call works
+// on (set by set_device and the destroy calls).
+inline thread_local hipStream_t tls_stream = nullptr;
+
+// Device buffer owned by a handle. A buffer is released only after the
+// calling context's stream has drained, so its memory is never handed to
+// another context (another thread's stream) while a kernel queued on this
+// one still reads or writes it, whatever hipFree itself waits for.
+//
+// With CB_DEVICE_GUARDS set in the environment every buffer carries a
+// 256-byte guard before it and a 4 KB guard after it, filled with 0xA5 at
+// allocation and checked at release; a write outside the buffer is reported
+// on stderr with the buffer's size (a debugging aid; off by default).
+inline bool device_guards() {
+    static const bool on = std::getenv("CB_DEVICE_GUARDS") != nullptr;
+    return on;
+}
+constexpr size_t GUARD_FRONT = 256, GUARD_BACK = 4096;
+void check_guards(const void* user, size_t bytes);  // cb_runtime.hip
+
 template <class T>
 struct DevBuf {
     T* p = nullptr;
@@ -66,7 +86,15 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (tls_stream) (void)hipStreamSynchronize(tls_stream);
+            if (device_guards()) {
+                check_guards(p, n * sizeof(T));
+                (void)hipFree(reinterpret_cast<char*>(p) - GUARD_FRONT);
+            } else {
+                (void)hipFree(p);
+            }
+        }
         p = nullptr;
         n = 0;
     }
@@ -74,11 +102,22 @@ struct DevBuf {
         if (count == n && p) return;
         release();
         if (count == 0) return;
-        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        const bool g = device_guards();
+        const size_t bytes = count * sizeof(T) + (g ? GUARD_FRONT + GUARD_BACK : 0);
+        void* base = nullptr;
+        hipError_t e = hipMalloc(&base, bytes);
         if (e != hipSuccess) {
             p = nullptr;
-            throw Error(CB_ERR_OOM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+            throw Error(CB_ERR_OOM, "hipMalloc of " + std::to_string(bytes) +
                                         " bytes failed: " + hipGetErrorString(e));
+        }
+        if (g) {
+            (void)hipMemset(base, 0xA5, GUARD_FRONT);
+            (void)hipMemset(static_cast<char*>(base) + GUARD_FRONT + count * sizeof(T), 0xA5, GUARD_BACK);
+            (void)hipDeviceSynchronize();
+            p = reinterpret_cast<T*>(static_cast<char*>(base) + GUARD_FRONT);
+        } else {
+            p = static_cast<T*>(base);
         }
         n = count;
     }
@@ -127,7 +166,17 @@ void timed(cb_context* ctx, const char* name, F&& launch) {
     ctx->pending.push_back(t);
 }
 
-inline void set_device(cb_context* ctx) { CB_HIP(hipSetDevice(ctx->device)); }
+// Waits for the calling context's stream after a rocBLAS/rocSOLVER call:
+// the call's work has finished before the host goes on to read its results
+// or release its arguments.
+inline void blas_drain() {
+    if (tls_stream) CB_HIP(hipStreamSynchronize(tls_stream));
+}
+
+inline void set_device(cb_context* ctx) {
+    CB_HIP(hipSetDevice(ctx->device));
+    tls_stream = ctx->stream;
+}
 
 // A fixed sequence of launches on ctx->stream, captured once and replayed
 // with hipGraphLaunch. The sequence must not allocate, copy to or from the

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "cb_blas.h"
 #include "cb_internal.h"
 
 using namespace cbamd;
@@ -22,6 +23,7 @@ namespace {
         rocblas_status s_ = (call);                                                          \
         if (s_ != rocblas_status_success)                                                    \
             throw Error(CB_ERR_DEVICE, std::string(#call " failed: ") + rocblas_status_to_string(s_)); \
+        blas_drain();                                                                        \
     } while (0)
 
 __global__ void k_add_diag(double* A, int64_t n, int64_t ld, double v) {
@@ -40,12 +42,6 @@ __global__ void k_scale_cols(const double* __restrict__ U, double* __restrict__ 
     V[j * ld + i] = U[j * ld + i] * (1.0 / lj);
 }
 
-struct Handle {
-    rocblas_handle h = nullptr;
-    ~Handle() {
-        if (h) rocblas_destroy_handle(h);
-    }
-};
 
 }  // namespace
 
@@ -67,11 +63,9 @@ void invert_psd(cb_context* ctx, const double* dM, int64_t ldm, int64_t n, doubl
     copy_host_2d(st, U.p, n * 8, dM, ldm * 8, n * 8, n, in_kind);
     if (ridge > 0.0) k_add_diag<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(U.p, n, n, ridge);
     CB_HIP(hipGetLastError());
-    Handle h;
-    CB_ROCBLAS(rocblas_create_handle(&h.h));
-    CB_ROCBLAS(rocblas_set_stream(h.h, st));
-    CB_ROCBLAS(rocsolver_dsyevd(h.h, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n, U.p,
-                                (rocblas_int)n, l.p, e.p, info.p));
+    BlasHandle h(st);
+    CB_BLAS(h, rocsolver_dsyevd(h.h, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n, U.p,
+                                (rocblas_int)n, l.p, e.p, info.p), "rocsolver_dsyevd");
     int hinfo = 0;
     double lo = 0, hi = 0;
     copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
@@ -94,9 +88,9 @@ void invert_psd(cb_context* ctx, const double* dM, int64_t ldm, int64_t n, doubl
         ldd = n;
     }
     CB_ROCBLAS(rocblas_set_pointer_mode(h.h, rocblas_pointer_mode_host));
-    CB_ROCBLAS(rocblas_dgemm(h.h, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)n,
+    CB_BLAS(h, rocblas_dgemm(h.h, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)n,
                              (rocblas_int)n, (rocblas_int)n, &one, V.p, (rocblas_int)n, U.p, (rocblas_int)n, &zero,
-                             dst, (rocblas_int)ldd));
+                             dst, (rocblas_int)ldd), "rocblas_dgemm");
     if (out_kind != hipMemcpyDeviceToDevice)
         copy_host_2d(st, dOut, ldo * 8, O.p, n * 8, n * 8, n, out_kind);
     CB_HIP(hipStreamSynchronize(st));

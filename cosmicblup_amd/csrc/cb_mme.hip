@@ -33,9 +33,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <memory>
 #include <numeric>
 #include <vector>
 
+#include "cb_blas.h"
 #include "cb_dense_tuned.h"
 #include "cb_internal.h"
 
@@ -1962,6 +1964,7 @@ int cb_mme_create(cb_context* ctx, int64_t n, int64_t p, int64_t q, const int64_
 void cb_mme_destroy(cb_mme* M) {
     if (!M) return;
     (void)hipSetDevice(M->ctx->device);
+    tls_stream = M->ctx->stream;  // its buffers are released behind the context's queued work
     delete M;
 }
 
@@ -2388,27 +2391,27 @@ __global__ __launch_bounds__(256) void k_exact_traces(const double* __restrict__
 #define CB_RB(call, what)                                                                              \
     do {                                                                                               \
         if ((call) != rocblas_status_success) throw Error(CB_ERR_DEVICE, std::string(what " failed")); \
+        blas_drain();                                                                                  \
     } while (0)
 
 // run_dense_exact_step (vce.cpp:2859-3195) on dense device matrices.
 struct DenseExact {
     cb_mme* M;
     int64_t n, p, q, dim;
-    rocblas_handle h = nullptr;
+    std::unique_ptr<BlasHandle> bh;  // workspace reserved per call (CB_BLAS)
+    rocblas_handle h = nullptr;      // bh->h
     DevBuf<double> W, Wt, ZAZ, C, B, H, Mu, b, e, t, part;
     DevBuf<int> info;
     std::vector<double> y;
     double logdetA = 0;
     explicit DenseExact(cb_mme* m) : M(m), n(m->n), p(m->p), q(m->q), dim(m->dim) {}
-    ~DenseExact() {
-        if (h) rocblas_destroy_handle(h);
-    }
+
     hipStream_t st() const { return M->ctx->stream; }
 
     void init() {
         CB_REQUIRE(n <= 40000 && dim <= 40000, "dense-exact AI-REML: n and p + q are limited to 40000");
-        CB_RB(rocblas_create_handle(&h), "rocblas_create_handle");
-        CB_RB(rocblas_set_stream(h, st()), "rocblas_set_stream");
+        bh = std::make_unique<BlasHandle>(st());
+        h = bh->h;
         info.alloc(1);
         // W and W'
         W.alloc((size_t)n * dim);
@@ -2430,10 +2433,10 @@ struct DenseExact {
                                                                                         M->gmap.p, 1.0, A.p, q);
         }
         CB_HIP(hipGetLastError());
-        CB_RB(rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)q, A.p, (rocblas_int)q, info.p), "dpotrf(Q)");
+        CB_BLAS(*bh, rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)q, A.p, (rocblas_int)q, info.p), "dpotrf(Q)");
         check_info("Qinv is not positive definite");
         logdetA = -logdiag2(A.p, q);
-        CB_RB(rocsolver_dpotri(h, rocblas_fill_lower, (rocblas_int)q, A.p, (rocblas_int)q, info.p), "dpotri(Q)");
+        CB_BLAS(*bh, rocsolver_dpotri(h, rocblas_fill_lower, (rocblas_int)q, A.p, (rocblas_int)q, info.p), "dpotri(Q)");
         k_sym_from_lower<<<dim3(blocks(q), (unsigned)q), 256, 0, st()>>>(A.p, q);
         ZAZ.alloc((size_t)n * n);
         k_zazt<<<dim3(blocks(n), (unsigned)n), 256, 0, st()>>>(A.p, q, M->aid.p, n, ZAZ.p);
@@ -2472,20 +2475,20 @@ struct DenseExact {
     // not positive definite. logL of the dense-exact step (vce.cpp:2972-3003).
     bool solve_logl(double vu, double ve, double* logl, std::vector<double>* sol) {
         assemble_dense_C(M, ve / vu, C.p);
-        CB_RB(rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)dim, C.p, (rocblas_int)dim, info.p), "dpotrf(C)");
+        CB_BLAS(*bh, rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)dim, C.p, (rocblas_int)dim, info.p), "dpotrf(C)");
         int hi = 0;
         copy_host(st(), &hi, info.p, sizeof(int), hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st()));
         if (hi != 0) return false;
         const double ldC = logdiag2(C.p, dim);
         copy_host(st(), b.p, M->rhs.data(), dim * 8, hipMemcpyHostToDevice);
-        CB_RB(rocsolver_dpotrs(h, rocblas_fill_lower, (rocblas_int)dim, 1, C.p, (rocblas_int)dim, b.p,
+        CB_BLAS(*bh, rocsolver_dpotrs(h, rocblas_fill_lower, (rocblas_int)dim, 1, C.p, (rocblas_int)dim, b.p,
                                (rocblas_int)dim), "dpotrs(C, rhs)");
         // e = y - W sol
         CB_HIP(hipMemcpyAsync(e.p, M->y.p, n * 8, hipMemcpyDeviceToDevice, st()));
         const double one = 1.0, mone = -1.0;
         CB_RB(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host), "pointer mode");
-        CB_RB(rocblas_dgemv(h, rocblas_operation_none, (rocblas_int)n, (rocblas_int)dim, &mone, W.p, (rocblas_int)n,
+        CB_BLAS(*bh, rocblas_dgemv(h, rocblas_operation_none, (rocblas_int)n, (rocblas_int)dim, &mone, W.p, (rocblas_int)n,
                             b.p, 1, &one, e.p, 1), "dgemv(W sol)");
         std::vector<double> he(n);
         copy_host(st(), he.data(), e.p, n * 8, hipMemcpyDeviceToHost);
@@ -2504,15 +2507,15 @@ struct DenseExact {
     // same point must have run just before (C holds its factor, e its residuals)
     void terms(double ve, double grad[2], double ai[4]) {
         CB_HIP(hipMemcpyAsync(B.p, Wt.p, B.bytes(), hipMemcpyDeviceToDevice, st()));
-        CB_RB(rocsolver_dpotrs(h, rocblas_fill_lower, (rocblas_int)dim, (rocblas_int)n, C.p, (rocblas_int)dim, B.p,
+        CB_BLAS(*bh, rocsolver_dpotrs(h, rocblas_fill_lower, (rocblas_int)dim, (rocblas_int)n, C.p, (rocblas_int)dim, B.p,
                                (rocblas_int)dim), "dpotrs(C, W')");
         const double one = 1.0, zero = 0.0;
-        CB_RB(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
+        CB_BLAS(*bh, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
                             (rocblas_int)dim, &one, W.p, (rocblas_int)n, B.p, (rocblas_int)dim, &zero, H.p,
                             (rocblas_int)n), "dgemm(W C^-1 W')");
         k_p_from_h<<<dim3(blocks(n), (unsigned)n), 256, 0, st()>>>(H.p, n, ve);  // H now holds P
         CB_HIP(hipGetLastError());
-        CB_RB(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
+        CB_BLAS(*bh, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
                             (rocblas_int)n, &one, H.p, (rocblas_int)n, ZAZ.p, (rocblas_int)n, &zero, Mu.p,
                             (rocblas_int)n), "dgemm(P ZAZ')");
         k_exact_traces<<<DOT_CHUNKS, 256, 0, st()>>>(Mu.p, H.p, n, part.p);
@@ -2523,7 +2526,7 @@ struct DenseExact {
         CB_HIP(hipStreamSynchronize(st()));
         for (int64_t i = 0; i < n; ++i) py[i] = he[i] / ve;
         copy_host(st(), e.p, py.data(), n * 8, hipMemcpyHostToDevice);
-        CB_RB(rocblas_dgemv(h, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n, &one, ZAZ.p, (rocblas_int)n,
+        CB_BLAS(*bh, rocblas_dgemv(h, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n, &one, ZAZ.p, (rocblas_int)n,
                             e.p, 1, &zero, t.p, 1), "dgemv(ZAZ' Py)");
         copy_host(st(), ht.data(), t.p, n * 8, hipMemcpyDeviceToHost);
         copy_host(st(), hp.data(), part.p, hp.size() * 8, hipMemcpyDeviceToHost);
@@ -2782,14 +2785,12 @@ int cb_reml_logl(cb_mme* M, double var_u, double var_e, double* logl, double* lo
         b.alloc(dim);
         assemble_dense_C(M, lambda, C.p);
         copy_host(st, b.p, M->rhs.data(), dim * 8, hipMemcpyHostToDevice);
-        rocblas_handle h = nullptr;
-        CB_REQUIRE(rocblas_create_handle(&h) == rocblas_status_success, "cb_reml_logl: rocblas handle");
-        struct HG { rocblas_handle h; ~HG() { rocblas_destroy_handle(h); } } hg{h};
         DevBuf<int> info;
         info.alloc(1);
-        CB_REQUIRE(rocblas_set_stream(h, st) == rocblas_status_success, "cb_reml_logl: rocblas stream");
-        CB_REQUIRE(rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)dim, C.p, (rocblas_int)dim, info.p) ==
-                       rocblas_status_success, "cb_reml_logl: dpotrf failed");
+        BlasHandle bh(st);
+        const rocblas_handle h = bh.h;
+        CB_BLAS(bh, rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)dim, C.p, (rocblas_int)dim, info.p),
+                "cb_reml_logl: dpotrf");
         int hinfo = 0;
         copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
@@ -2798,8 +2799,9 @@ int cb_reml_logl(cb_mme* M, double var_u, double var_e, double* logl, double* lo
             if (logdet_c) *logdet_c = NAN;
             return;
         }
-        CB_REQUIRE(rocsolver_dpotrs(h, rocblas_fill_lower, (rocblas_int)dim, 1, C.p, (rocblas_int)dim, b.p,
-                                    (rocblas_int)dim) == rocblas_status_success, "cb_reml_logl: dpotrs failed");
+        CB_BLAS(bh, rocsolver_dpotrs(h, rocblas_fill_lower, (rocblas_int)dim, 1, C.p, (rocblas_int)dim, b.p,
+                                     (rocblas_int)dim),
+                "cb_reml_logl: dpotrs");
         DevBuf<double> part;
         part.alloc(DOT_CHUNKS);
         k_logdiag_part<<<DOT_CHUNKS, 256, 0, st>>>(C.p, dim, part.p);

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <vector>
 
+#include "cb_blas.h"
 #include "cb_internal.h"
 
 using namespace cbamd;
@@ -32,14 +33,9 @@ constexpr int MV_MAXT = 4;
 #define CB_RBS(call, what)                                                                             \
     do {                                                                                               \
         if ((call) != rocblas_status_success) throw Error(CB_ERR_DEVICE, std::string(what " failed")); \
+        blas_drain();                                                                                  \
     } while (0)
 
-struct Handle {
-    rocblas_handle h = nullptr;
-    ~Handle() {
-        if (h) rocblas_destroy_handle(h);
-    }
-};
 
 // in-register t x t inverse of an SPD matrix by Gauss-Jordan (t <= 4)
 template <int T>
@@ -243,9 +239,7 @@ int cb_mv_vce(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* Y,
         CB_REQUIRE(n > c + 1 && c >= 0 && (c == 0 || X) && ldg >= n && n < INT32_MAX, "cb_mv_vce: bad dimensions");
         set_device(ctx);
         hipStream_t st = ctx->stream;
-        Handle h;
-        CB_RBS(rocblas_create_handle(&h.h), "rocblas_create_handle");
-        CB_RBS(rocblas_set_stream(h.h, st), "rocblas_set_stream");
+        BlasHandle h(st);
         CB_RBS(rocblas_set_pointer_mode(h.h, rocblas_pointer_mode_host), "pointer mode");
         const int64_t ns = n - c;
         DevBuf<double> Qf, tau, T1, Gr, dY, Yr, Uy, dd, ee;
@@ -261,8 +255,8 @@ int cb_mv_vce(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* Y,
         if (c > 0) {
             tau.alloc(c);
             copy_host(st, Qf.p, X, (size_t)n * c * 8, hipMemcpyHostToDevice);
-            CB_RBS(rocsolver_dgeqrf(h.h, (rocblas_int)n, (rocblas_int)c, Qf.p, (rocblas_int)n, tau.p), "dgeqrf");
-            CB_RBS(rocsolver_dorgqr(h.h, (rocblas_int)n, (rocblas_int)n, (rocblas_int)c, Qf.p, (rocblas_int)n, tau.p),
+            CB_BLAS(h, rocsolver_dgeqrf(h.h, (rocblas_int)n, (rocblas_int)c, Qf.p, (rocblas_int)n, tau.p), "dgeqrf");
+            CB_BLAS(h, rocsolver_dorgqr(h.h, (rocblas_int)n, (rocblas_int)n, (rocblas_int)c, Qf.p, (rocblas_int)n, tau.p),
                    "dorgqr");
             A = Qf.p + (size_t)c * n;
         } else {
@@ -275,27 +269,27 @@ int cb_mv_vce(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* Y,
         // G_r = A' G A, Y_r = A' Y
         T1.alloc((size_t)n * ns);
         Gr.alloc((size_t)ns * ns);
-        CB_RBS(rocblas_dgemm(h.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)ns,
+        CB_BLAS(h, rocblas_dgemm(h.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)ns,
                              (rocblas_int)n, &one, dG, (rocblas_int)ldg, A, (rocblas_int)n, &zero, T1.p,
                              (rocblas_int)n), "dgemm(G A)");
-        CB_RBS(rocblas_dgemm(h.h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)ns,
+        CB_BLAS(h, rocblas_dgemm(h.h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)ns,
                              (rocblas_int)ns, (rocblas_int)n, &one, A, (rocblas_int)n, T1.p, (rocblas_int)n, &zero,
                              Gr.p, (rocblas_int)ns), "dgemm(A' G A)");
         Yr.alloc((size_t)ns * t);
-        CB_RBS(rocblas_dgemm(h.h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)ns,
+        CB_BLAS(h, rocblas_dgemm(h.h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)ns,
                              (rocblas_int)t, (rocblas_int)n, &one, A, (rocblas_int)n, dY.p, (rocblas_int)n, &zero,
                              Yr.p, (rocblas_int)ns), "dgemm(A' Y)");
         T1.release();
         dd.alloc(ns);
         ee.alloc(ns);
-        CB_RBS(rocsolver_dsyevd(h.h, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)ns, Gr.p,
+        CB_BLAS(h, rocsolver_dsyevd(h.h, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)ns, Gr.p,
                                 (rocblas_int)ns, dd.p, ee.p, info.p), "dsyevd(A' G A)");
         int hinfo = 0;
         copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
         CB_REQUIRE(hinfo == 0, "cb_mv_vce: eigensolver did not converge");
         Uy.alloc((size_t)ns * t);
-        CB_RBS(rocblas_dgemm(h.h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)ns,
+        CB_BLAS(h, rocblas_dgemm(h.h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)ns,
                              (rocblas_int)t, (rocblas_int)ns, &one, Gr.p, (rocblas_int)ns, Yr.p, (rocblas_int)ns,
                              &zero, Uy.p, (rocblas_int)ns), "dgemm(U' Y_r)");
         // eigenvalues clamped at 0 (eigen_vce.cpp:127)
@@ -421,15 +415,13 @@ int cb_mv_solve(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* 
         k_mv_lhs<<<dim3((unsigned)ceil_div(dim, 256), (unsigned)dim), 256, 0, st>>>(dxtx.p, dX.p, dGinv, ldg, dvi.p, n,
                                                                                     c, T, L.p, dim);
         CB_HIP(hipGetLastError());
-        Handle h;
-        CB_RBS(rocblas_create_handle(&h.h), "rocblas_create_handle");
-        CB_RBS(rocblas_set_stream(h.h, st), "rocblas_set_stream");
-        CB_RBS(rocsolver_dpotrf(h.h, rocblas_fill_lower, (rocblas_int)dim, L.p, (rocblas_int)dim, info.p), "dpotrf");
+        BlasHandle h(st);
+        CB_BLAS(h, rocsolver_dpotrf(h.h, rocblas_fill_lower, (rocblas_int)dim, L.p, (rocblas_int)dim, info.p), "dpotrf");
         int hinfo = 0;
         copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
         CB_REQUIRE(hinfo == 0, "cb_mv_solve: the multi-trait MME is not positive definite");
-        CB_RBS(rocsolver_dpotrs(h.h, rocblas_fill_lower, (rocblas_int)dim, 1, L.p, (rocblas_int)dim, b.p,
+        CB_BLAS(h, rocsolver_dpotrs(h.h, rocblas_fill_lower, (rocblas_int)dim, 1, L.p, (rocblas_int)dim, b.p,
                                 (rocblas_int)dim), "dpotrs");
         std::vector<double> sol(dim);
         copy_host(st, sol.data(), b.p, dim * 8, hipMemcpyDeviceToHost);

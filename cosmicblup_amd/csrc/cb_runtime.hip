@@ -60,6 +60,26 @@ void copy_host_2d(hipStream_t st, void* dst, size_t dpitch, const void* src, siz
     }
 }
 
+void check_guards(const void* user, size_t bytes) {
+    std::vector<unsigned char> f(GUARD_FRONT), b(GUARD_BACK);
+    const char* u = static_cast<const char*>(user);
+    if (hipMemcpy(f.data(), u - GUARD_FRONT, GUARD_FRONT, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(b.data(), u + bytes, GUARD_BACK, hipMemcpyDeviceToHost) != hipSuccess) {
+        std::fprintf(stderr, "[guard] could not read the guards of a %zu-byte buffer\n", bytes);
+        return;
+    }
+    size_t bad_f = 0, bad_b = 0, first_b = GUARD_BACK;
+    for (size_t i = 0; i < GUARD_FRONT; ++i) bad_f += f[i] != 0xA5;
+    for (size_t i = 0; i < GUARD_BACK; ++i)
+        if (b[i] != 0xA5) {
+            ++bad_b;
+            if (first_b == GUARD_BACK) first_b = i;
+        }
+    if (bad_f || bad_b)
+        std::fprintf(stderr, "[guard] %zu-byte buffer: %zu bytes written before it, %zu after it (first at +%zu)\n",
+                     bytes, bad_f, bad_b, first_b);
+}
+
 void copy_host(hipStream_t st, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
     if (kind == hipMemcpyDeviceToDevice) {
         CB_HIP(hipMemcpyAsync(dst, src, bytes, kind, st));
@@ -120,6 +140,8 @@ void cb_context_destroy(cb_context* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     ctx->resolve_profile();
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (tls_stream == ctx->stream) tls_stream = nullptr;
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -132,6 +154,7 @@ int cb_context_set_stream(cb_context* ctx, void* hip_stream) {
             CB_HIP(hipStreamSynchronize(ctx->stream));
             CB_HIP(hipStreamDestroy(ctx->stream));
             ctx->stream = nullptr;
+            tls_stream = nullptr;
             ctx->own_stream = false;
         }
         if (hip_stream) {
@@ -140,6 +163,7 @@ int cb_context_set_stream(cb_context* ctx, void* hip_stream) {
             CB_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
             ctx->own_stream = true;
         }
+        tls_stream = ctx->stream;
     });
 }
 
@@ -256,6 +280,7 @@ int cb_genotypes_from_device_rows(cb_context* ctx, const void* d_rows, size_t ro
 void cb_genotypes_destroy(cb_genotypes* g) {
     if (!g) return;
     (void)hipSetDevice(g->ctx->device);
+    tls_stream = g->ctx->stream;  // its buffers are released behind the context's queued work
     delete g;
 }
 

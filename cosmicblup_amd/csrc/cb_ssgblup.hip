@@ -25,6 +25,7 @@
 #include <cmath>
 #include <vector>
 
+#include "cb_blas.h"
 #include "cb_internal.h"
 
 using namespace cbamd;
@@ -144,13 +145,6 @@ __global__ void k_combine(const double* __restrict__ Gi, int64_t ldg, const doub
     const double a = i >= j ? Ai[j * lda + i] : Ai[i * lda + j];
     D[j * ldd + i] = tau * Gi[j * ldg + i] - omega * a;
 }
-
-struct Handle {
-    rocblas_handle h = nullptr;
-    ~Handle() {
-        if (h) rocblas_destroy_handle(h);
-    }
-};
 
 }  // namespace
 
@@ -287,19 +281,17 @@ int cb_hinv_blocks(cb_context* ctx, int64_t n, const int32_t* dam, const int32_t
                    nullptr);
         // A22^-1 by Cholesky (relationship_builder.cpp:574-581)
         if (o.ridge > 0.0) k_add_diag2<<<(unsigned)ceil_div(ng, 256), 256, 0, st>>>(A22.p, ng, ng, o.ridge);
-        Handle h;
-        CB_REQUIRE(rocblas_create_handle(&h.h) == rocblas_status_success, "cb_hinv_blocks: rocblas handle");
-        CB_REQUIRE(rocblas_set_stream(h.h, st) == rocblas_status_success, "cb_hinv_blocks: rocblas stream");
+        BlasHandle h(st);
         DevBuf<int> info;
         info.alloc(1);
         int hinfo = 0;
-        CB_REQUIRE(rocsolver_dpotrf(h.h, rocblas_fill_lower, (rocblas_int)ng, A22.p, (rocblas_int)ng, info.p) ==
-                       rocblas_status_success, "cb_hinv_blocks: dpotrf failed");
+        CB_BLAS(h, rocsolver_dpotrf(h.h, rocblas_fill_lower, (rocblas_int)ng, A22.p, (rocblas_int)ng, info.p),
+                "cb_hinv_blocks: dpotrf");
         copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
         CB_HIP(hipStreamSynchronize(st));
         CB_REQUIRE(hinfo == 0, "cb_hinv_blocks: A22 is not positive definite");
-        CB_REQUIRE(rocsolver_dpotri(h.h, rocblas_fill_lower, (rocblas_int)ng, A22.p, (rocblas_int)ng, info.p) ==
-                       rocblas_status_success, "cb_hinv_blocks: dpotri failed");
+        CB_BLAS(h, rocsolver_dpotri(h.h, rocblas_fill_lower, (rocblas_int)ng, A22.p, (rocblas_int)ng, info.p),
+                "cb_hinv_blocks: dpotri");
         k_combine<<<dim3((unsigned)ceil_div(ng, 256), (unsigned)ng), 256, 0, st>>>(Gb.p, ng, A22.p, ng, ng, o.tau,
                                                                                   o.omega, dD, ldd);
         CB_HIP(hipGetLastError());

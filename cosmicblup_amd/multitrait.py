@@ -57,7 +57,7 @@ class MvEigenVce:
         conv = C.c_int()
         Yf = np.ascontiguousarray(self.Y.T)  # column-major n x t
         Xf = np.ascontiguousarray(self.X.T)
-        torch.cuda.synchronize()
+        torch.cuda.current_stream().synchronize()  # torch-made inputs ready (not other streams)
         _lib.check(_lib.lib().cb_mv_vce(self.ctx.handle, n, t, c, _lib.ptr(Yf), _lib.ptr(Xf), self.G.data_ptr(),
                                         n, C.byref(o), _lib.ptr(Vg), _lib.ptr(Ve), C.byref(it), C.byref(conv)))
         self.Vg, self.Ve = Vg, Ve
@@ -87,7 +87,7 @@ def solveMultiTrait(Y, X, Ginv, Vg, Ve, ctx: _lib.Context | None = None):
     Xf = np.ascontiguousarray(X.T)
     Vg = np.ascontiguousarray(Vg, dtype=np.float64)
     Ve = np.ascontiguousarray(Ve, dtype=np.float64)
-    torch.cuda.synchronize()
+    torch.cuda.current_stream().synchronize()  # torch-made inputs ready (not other streams)
     _lib.check(_lib.lib().cb_mv_solve(ctx.handle, n, t, c, _lib.ptr(Yf), _lib.ptr(Xf), dG.data_ptr(), n, _lib.ptr(Vg),
                                       _lib.ptr(Ve), _lib.ptr(beta), _lib.ptr(u)))
     return beta.T, u.T

@@ -45,3 +45,39 @@ def test_mv_solve_matches_oracle(cb, ctx):
     br, ur = orl.mv_solve(Y, X, Gi, Vg, Ve)
     np.testing.assert_allclose(beta, br, atol=1e-9 * np.abs(br).max())
     np.testing.assert_allclose(u, ur, atol=1e-9 * np.abs(ur).max())
+
+
+def test_contexts_on_threads_match_serial(cb, ctx):
+    """Multi-trait estimation and solve on three contexts driven by three
+    host threads give the serial results bit for bit. These calls allocate
+    and free device buffers while other contexts' work is queued: a buffer
+    is released only behind its own context's work, so its memory is never
+    handed to another stream early."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from cosmicblup_amd import _lib
+
+    ctxs = [_lib.Context(0) for _ in range(3)]
+    jobs = []
+    for w, c in enumerate(ctxs):
+        for j in range(2):
+            Y, X, G = mv_problem(n=300 + 60 * (2 * w + j), seed=20 + 2 * w + j)
+            jobs.append((w, c, Y, X, G, np.linalg.inv(G)))
+
+    def run(c, Y, X, G, Gi):
+        v = multitrait.MvEigenVce(multitrait.MvEigenVce.Options(max_iter=300, tol=1e-7), ctx=c)
+        v.prepare(Y, X, G)
+        v.runNullModel()
+        beta, u = multitrait.solveMultiTrait(Y, X, Gi, v.getVg(), v.getVe(), ctx=c)
+        return v.iterations, v.getVg().tobytes(), v.getVe().tobytes(), beta.tobytes(), u.tobytes()
+
+    serial = [run(c, Y, X, G, Gi) for _, c, Y, X, G, Gi in jobs]
+
+    def worker(w):
+        return [(i, run(c, Y, X, G, Gi)) for i, (wk, c, Y, X, G, Gi) in enumerate(jobs) if wk == w]
+
+    for _ in range(2):
+        with ThreadPoolExecutor(len(ctxs)) as ex:
+            par = dict(sum(ex.map(worker, range(len(ctxs))), []))
+        for i, ref in enumerate(serial):
+            assert par[i] == ref, i
