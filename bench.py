@@ -129,6 +129,15 @@ def reml_leg(ctx, n: int = 20000, m: int = 50000, k: int = 64, h2: float = 0.4, 
     ctx.set_profiling(False)
     print(f"[bench] mixed: {Rm.iterations} iterations in {tm:.2f} s, vu={Rm.var_u:.5f} ve={Rm.var_e:.5f}", flush=True)
     symm_ms, symm_n = ctx.kernel_time("k_dense_mfma_f32")
+    # the same run with the dense product of the FP32 recursion on FP16
+    # matrix cores (hi/lo halves)
+    ctx.reset_profile()
+    ctx.set_profiling(True)
+    Rh, th = run(_lib.CB_PREC_MIXED_F16, True, max_iter)
+    ctx.set_profiling(False)
+    print(f"[bench] mixed f16: {Rh.iterations} iterations in {th:.2f} s, vu={Rh.var_u:.5f} ve={Rh.var_e:.5f}",
+          flush=True)
+    h_ms, h_n = ctx.kernel_time("k_dense_h")
     Rp, tp = run(_lib.CB_PREC_FP64, False, plain_iters)
     print(f"[bench] fp64 plain: {Rp.iterations} iterations in {tp:.2f} s", flush=True)
     # FP64 with the same batching: the variance-component reference for the
@@ -136,13 +145,17 @@ def reml_leg(ctx, n: int = 20000, m: int = 50000, k: int = 64, h2: float = 0.4, 
     Rb, tb = run(_lib.CB_PREC_FP64, True, max_iter)
     print(f"[bench] fp64 batched: {Rb.iterations} iterations in {tb:.2f} s", flush=True)
     vc_err = max(abs(Rm.var_u - Rb.var_u) / Rb.var_u, abs(Rm.var_e - Rb.var_e) / Rb.var_e)
+    vc_err_h = max(abs(Rh.var_u - Rb.var_u) / Rb.var_u, abs(Rh.var_e - Rb.var_e) / Rb.var_e)
     # accuracy of the mixed path against FP64 at identical variances
     opts64 = mme.reml_options(n_probes=k, seed=7, solve=mme.solve_options(precision=_lib.CB_PREC_FP64, tol=1e-12))
     opts32 = mme.reml_options(n_probes=k, seed=7, solve=mme.solve_options(precision=_lib.CB_PREC_MIXED, tol=1e-10))
     g64, _, tr64, s64 = mme.AIREML(model, opts64).score(Rm.var_u, Rm.var_e, want_ai=False)
     g32, _, tr32, s32 = mme.AIREML(model, opts32).score(Rm.var_u, Rm.var_e, want_ai=False)
+    opts16 = mme.reml_options(n_probes=k, seed=7, solve=mme.solve_options(precision=_lib.CB_PREC_MIXED_F16, tol=1e-10))
+    g16, _, tr16, s16 = mme.AIREML(model, opts16).score(Rm.var_u, Rm.var_e, want_ai=False)
     p = model.p
     ebv_err = float(np.abs(s32[p:] - s64[p:]).max() / np.abs(s64[p:]).max())
+    ebv_err_h = float(np.abs(s16[p:] - s64[p:]).max() / np.abs(s64[p:]).max())
     rate_m = Rm.iterations / tm
     rate_p = Rp.iterations / tp
     # per launch of the dominant REML kernel: q^2 FP32 of Q read, 2 q^2 r flops
@@ -153,6 +166,11 @@ def reml_leg(ctx, n: int = 20000, m: int = 50000, k: int = 64, h2: float = 0.4, 
             "ebv_rel_err_mixed_vs_fp64": ebv_err,
             "trace_rel_err_mixed_vs_fp64": abs(tr32 - tr64) / abs(tr64),
             "k_dense_mfma_f32_avg_ms": symm_ms / max(symm_n, 1),
+            "mixed_f16": {"iters_per_s": Rh.iterations / th, "speedup_vs_fp64_plain": Rh.iterations / th / rate_p,
+                          "iterations": Rh.iterations, "converged": Rh.converged,
+                          "vc_rel_err_vs_fp64": vc_err_h, "ebv_rel_err_vs_fp64": ebv_err_h,
+                          "trace_rel_err_vs_fp64": abs(tr16 - tr64) / abs(tr64),
+                          "k_dense_h_avg_ms": h_ms / max(h_n, 1)},
             "qinv": "eigen-bent G^-1 (invert_psd_matrix semantics)", "qinv_setup_s": t_inv}
 
 

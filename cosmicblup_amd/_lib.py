@@ -59,6 +59,7 @@ class RemlOptions(C.Structure):
 
 CB_PREC_FP64 = 0
 CB_PREC_MIXED = 1
+CB_PREC_MIXED_F16 = 2
 CB_PRECOND_JACOBI = 0
 CB_PRECOND_BLOCK_JACOBI = 1
 CB_STOP_RELATIVE = 0

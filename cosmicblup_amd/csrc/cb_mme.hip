@@ -324,15 +324,223 @@ __global__ __launch_bounds__(256) void k_dense_mfma(const T* __restrict__ Q, int
         }
 }
 
+// Split-FP16 variant of k_dense_mfma for the FP32 recursion of the mixed
+// PCG. Q and X are each held as two FP16 halves, hi = fp16(v) and lo =
+// fp16(v - hi), of power-of-two scaled values (Q by one scale, each X
+// column by its own, so that the largest magnitude sits near 2^14): hi + lo
+// carries 22 significant bits. Q X ~ Qh Xh + Qh Xl + Ql Xh on
+// v_mfma_f32_16x16x32_f16 with FP32 accumulation (the dropped Ql Xl term is
+// 2^-22 relative). Q's halves take the same 4 bytes per element as FP32, so
+// the launch reads q^2 * 4 bytes, and three FP16 MFMAs do the work of 32
+// FP32 ones: HBM-bound instead of MFMA-bound. Q's halves are stored in
+// MFMA fragment order: fragment (strip, kb) = rows [16 strip, 16 strip + 16)
+// x k [32 kb, 32 kb + 32) is one contiguous 2 KB run, the hi halves (1 KB)
+// then the lo halves, each with lane l = li + 16 g's 8 k of group g of row
+// li at 16 l bytes, and a strip's fragments follow each other along k
+// (ldq / 32 per strip). A wave thus streams each of its subtiles
+// sequentially, and each of its 16-byte loads covers whole 128-byte lines. The partials stay scaled; k_dense_reduce applies each column's
+// power of two in FP64.
+// Tiling as k_dense_mfma: 128-row tiles, split-K over a 1-D grid, wave w
+// owns rows [32w, 32w + 32) as two 16-row subtiles; in each 32-k chunk lane
+// l holds row (l & 15) and the 8 consecutive k of group g = l >> 4 (one
+// 16-byte load per half), and its B operand is the same 8 k of column
+// (l & 15) of the staged X chunk, so A and B agree on k slot by slot.
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+constexpr int MH_KCP = MF_KC + 8;  // LDS pitch (halves) of a staged X column: 16-byte aligned runs
+template <int NT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_dense_h(const _Float16* __restrict__ Qhl, int64_t ldq, int64_t q,
+                                                 const _Float16* __restrict__ Xh,
+                                                 const _Float16* __restrict__ Xl, int64_t ldh, int r, int64_t kspan,
+                                                 int nsplit, float* __restrict__ part, int64_t ldp) {
+    constexpr int NC = NT * 16;
+    constexpr int FRAG = 64 * 16;             // halves per fragment
+    constexpr int XP = NC * MF_KC / 2;        // staged half pairs per buffer and half
+    constexpr int XE = (XP + 255) / 256;      // pairs per thread
+    __shared__ __attribute__((aligned(16))) _Float16 xs[2][2][NC][MH_KCP];  // [buf][hi, lo][column][k]
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int li = lane & 15, g = lane >> 4;
+    const int split = (int)(blockIdx.x % (unsigned)nsplit);
+    const int64_t row0 = (int64_t)(blockIdx.x / (unsigned)nsplit) * MF_ROWS;
+    const int64_t k0 = (int64_t)split * kspan;
+    const int64_t k1 = min(q, k0 + kspan);
+    const int nchunks = (int)((k1 - k0 + MF_KC - 1) / MF_KC);
+    int64_t qoff[2];
+    const int64_t kb_row = ldq / MF_KC, last_strip = (q - 1) / 16;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        // strips past the last: computed from the last one, not stored
+        const int64_t strip = min((row0 + w * 32 + s * 16) / 16, last_strip);
+        qoff[s] = (strip * kb_row + k0 / MF_KC) * FRAG + (int64_t)lane * 8;
+    }
+    uint32_t xr[2][2][XE];                       // [register set][hi, lo][pair]
+    h8v ah[3][2], al[3][2];                      // [register set][subtile]
+    f4v acc[2][NT];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[s][n] = f4v{0, 0, 0, 0};
+    // X half pairs (k, k+1) of column c; pairs at or past k1 stage zeros
+    // (the padding rows of Xh/Xl past q are zero, so a pair that straddles
+    // q = k1 reads zero for its second half)
+    auto load_x = [&](int64_t kc, int set) {
+#pragma unroll
+        for (int i = 0; i < XE; ++i) {
+            const int e = tid + 256 * i, c = e >> 4;
+            const int64_t k = kc + 2 * (e & 15);
+            const int64_t off = (int64_t)(c < r ? c : r - 1) * ldh + (k < k1 ? k : k0);
+            xr[set][0][i] = *reinterpret_cast<const uint32_t*>(Xh + off);
+            xr[set][1][i] = *reinterpret_cast<const uint32_t*>(Xl + off);
+        }
+    };
+    auto store_x = [&](int buf, int64_t kc, int set) {
+#pragma unroll
+        for (int i = 0; i < XE; ++i) {
+            const int e = tid + 256 * i, c = e >> 4;
+            const int64_t k = kc + 2 * (e & 15);
+            if (XP % 256 == 0 || c < NC) {
+                const bool ok = c < r && k < k1;
+#pragma unroll
+                for (int hl = 0; hl < 2; ++hl)
+                    *reinterpret_cast<uint32_t*>(&xs[buf][hl][c][2 * (e & 15)]) = ok ? xr[set][hl][i] : 0u;
+            }
+        }
+    };
+    // k beyond k1 reads the zero padding of Q (last split) and meets zero X
+    auto load_a = [&](int ch, int set) {
+        const int off = (ch < nchunks ? ch : nchunks - 1) * FRAG;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            ah[set][s] = __builtin_nontemporal_load(reinterpret_cast<const h8v*>(Qhl + qoff[s] + off));
+            al[set][s] = __builtin_nontemporal_load(reinterpret_cast<const h8v*>(Qhl + qoff[s] + off + FRAG / 2));
+        }
+    };
+    // chunk ch: its Q registers (set qs = ch % 3) were loaded two chunks
+    // ahead, its X staged in xs[buf]; X of chunk ch + 1 (register set
+    // xset ^ 1, loaded one chunk ago) is staged after the products, and X of
+    // chunk ch + 2 loads into set xset: a Q load has two chunks to arrive, an
+    // X load (from L2, X being shared by all row tiles) one
+    auto half = [&](int ch, int buf, int qs, int xset) {
+        load_a(ch + 2, (qs + 2) % 3);
+        load_x(k0 + (int64_t)(ch + 2) * MF_KC, xset);
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            const h8v xh = *reinterpret_cast<const h8v*>(&xs[buf][0][n * 16 + li][8 * g]);
+            const h8v xl = *reinterpret_cast<const h8v*>(&xs[buf][1][n * 16 + li][8 * g]);
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                acc[s][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[qs][s], xh, acc[s][n], 0, 0, 0);
+                acc[s][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[qs][s], xl, acc[s][n], 0, 0, 0);
+                acc[s][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[qs][s], xh, acc[s][n], 0, 0, 0);
+            }
+        }
+        // xs[buf ^ 1] was last read before the previous chunk's barrier
+        store_x(buf ^ 1, k0 + (int64_t)(ch + 1) * MF_KC, xset ^ 1);
+        __syncthreads();
+    };
+    load_x(k0, 0);
+    load_a(0, 0);
+    load_a(1, 1);
+    store_x(0, k0, 0);
+    load_x(k0 + MF_KC, 1);
+    __syncthreads();
+    // six chunks per trip keep every register set index static; chunks past
+    // the split's end stage zero X (kspan is a multiple of 6 chunks except in
+    // the last split)
+    for (int ch = 0; ch < nchunks; ch += 6) {
+        half(ch, 0, 0, 0);
+        half(ch + 1, 1, 1, 1);
+        half(ch + 2, 0, 2, 0);
+        half(ch + 3, 1, 0, 1);
+        half(ch + 4, 0, 1, 0);
+        half(ch + 5, 1, 2, 1);
+    }
+    float* pp = part + (int64_t)split * r * ldp;
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            const int c = n * 16 + li;
+            if (c >= r) continue;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int64_t row = row0 + w * 32 + s * 16 + 4 * g + j;
+                if (row < q) pp[(int64_t)c * ldp + row] = acc[s][n][j];
+            }
+        }
+}
+
+// Largest |Q| of n elements as float bits into *out (non-negative floats
+// order as their bit patterns); *out starts at 0.
+__global__ __launch_bounds__(256) void k_absmax_bits(const float* __restrict__ a, int64_t n, unsigned* out) {
+    __shared__ float red[4];
+    float m = 0.0f;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, fabsf(a[e]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(out, __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))));
+}
+
+__device__ __forceinline__ void split_h(float v, _Float16* hi, _Float16* lo) {
+    const _Float16 h = (_Float16)v;
+    *hi = h;
+    *lo = (_Float16)(v - (float)h);  // v - h is exact in FP32
+}
+
+// hi + lo = Q32 * 2^sq to 22 bits, written in k_dense_h's fragment order
+// (ldq a multiple of 32; Q32 padding zeros stay zero, and rows past the
+// last up to a multiple of 16 are zeroed by the caller)
+__global__ void k_split_q(const float* __restrict__ Q, int64_t n, int64_t ldq, int sq, _Float16* __restrict__ Qhl) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int64_t i = e / ldq, k = e - i * ldq;
+    const int64_t frag = (i >> 4) * (ldq / MF_KC) + (k >> 5);
+    const int lane = (int)(i & 15) + 16 * (int)((k & 31) >> 3);
+    _Float16* d = Qhl + frag * (64 * 16) + lane * 8 + (k & 7);
+    split_h(ldexpf(Q[e], sq), d, d + 64 * 8);
+}
+
+// One block per column c of X (q rows): its largest |x| sets sx so that the
+// column's values times 2^sx stay below 2^14; Xh + Xl = x * 2^sx, rows
+// [q, ldh) zero; cscale[c] = 2^-(sq + sx) undoes both scales of Q X.
+constexpr int SPLIT_X_THREADS = 1024;
+__global__ __launch_bounds__(SPLIT_X_THREADS) void k_split_x(const float* __restrict__ X, int64_t ldx, int64_t q,
+                                                             int64_t ldh, int sq, _Float16* __restrict__ Xh,
+                                                             _Float16* __restrict__ Xl, double* __restrict__ cscale) {
+    constexpr int NW = SPLIT_X_THREADS / 64;
+    __shared__ float red[NW];
+    const int c = blockIdx.x;
+    const float* x = X + (int64_t)c * ldx;
+    float m = 0.0f;
+    for (int64_t i = threadIdx.x; i < q; i += SPLIT_X_THREADS) m = fmaxf(m, fabsf(x[i]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = red[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) m = fmaxf(m, red[w]);
+    int e = 0;
+    if (m > 0.0f && m <= 3.4e38f) (void)frexpf(m, &e);  // m < 2^e
+    const int sx = m > 0.0f && m <= 3.4e38f ? 14 - e : 0;
+    for (int64_t i = threadIdx.x; i < ldh; i += SPLIT_X_THREADS)
+        split_h(i < q ? ldexpf(x[i], sx) : 0.0f, Xh + (int64_t)c * ldh + i, Xl + (int64_t)c * ldh + i);
+    if (threadIdx.x == 0) cscale[c] = ldexp(1.0, -(sq + sx));
+}
+
 // Y[row, c] += alpha * sum_split part[split][c][row]   (fixed split order);
-// with sv set, then Y[row, c] += scoef * (sum of spart[c0 + c]'s chunks) *
+// with cscale set, the split sum is first scaled by cscale[c] (the split-FP16
+// path's power of two); with sv set, then Y[row, c] += scoef * (sum of spart[c0 + c]'s chunks) *
 // sv[row]: the stiff rank-1 term of apply_C, rounded exactly as its own
 // kernel (k_stiff_axpy) rounds it.
 template <class T>
 __global__ void k_dense_reduce(const T* __restrict__ part, int64_t ldp, int nsplit, int64_t q, int r,
                                T* __restrict__ Y, int64_t ldy, int64_t y_off, T alpha,
                                const double* __restrict__ sv, const double* __restrict__ spart, int c0,
-                               double scoef) {
+                               double scoef, const double* __restrict__ cscale = nullptr) {
     __shared__ double t;
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
@@ -347,6 +555,7 @@ __global__ void k_dense_reduce(const T* __restrict__ part, int64_t ldp, int nspl
     if (row >= q) return;
     double s = 0;
     for (int sp = 0; sp < nsplit; ++sp) s += (double)part[((int64_t)sp * r + c) * ldp + row];
+    if (cscale) s *= cscale[c];  // a power of two: exact
     T* y = Y + (int64_t)c * ldy + y_off + row;
     T v = (T)((double)*y + (double)alpha * s);
     if (sv) v = (T)((double)v + scoef * t * sv[row]);
@@ -858,6 +1067,14 @@ struct cb_mme {
     DevBuf<float> xg32, yg32;
     DevBuf<double> Qd64;  // q rows, leading dimension ldq (zero-padded)
     DevBuf<float> Qd32;
+    // FP16 halves of Qd32 * 2^q16_scale for CB_PREC_MIXED_F16 solves, built
+    // before such a solve when Qd32 has changed (q16_rows == 0: not built)
+    DevBuf<_Float16> Qhl16;        // hi and lo halves in k_dense_h's fragment order
+    int64_t q16_rows = 0;
+    int q16_scale = 0;
+    bool use_f16 = false;          // set for the duration of a CB_PREC_MIXED_F16 solve
+    DevBuf<_Float16> xh16, xl16;   // the operand block's halves
+    DevBuf<double> xscale;         // per column: 2^-(q16_scale + its own scale)
     int64_t ldq = 0;
     DevBuf<int64_t> qp;
     DevBuf<int32_t> qc;
@@ -985,6 +1202,98 @@ struct StiffTerm {
 // (qn rows, leading dimension ldq, zero-padded) of this MME. X and Y point
 // at row 0 of the block's index space. With sk, the rank-1 term follows in
 // the same pass.
+// Split shape of one k_dense_mfma / k_dense_h launch over qn rows
+struct DenseShape {
+    int64_t row_tiles, kspan, ldp;
+    int nsplit;
+};
+DenseShape dense_shape(int elem, int64_t qn, int r) {
+    // k_dense_h runs six 32-k chunks per loop trip
+    const int64_t kq = elem == 2 ? 6 * MF_KC : MF_KC;
+    DenseShape d;
+    d.row_tiles = ceil_div(qn, MF_ROWS);
+    // about target_blocks blocks, splits of at least min_kspan k (tuned table)
+    const DenseSplit sp = dense_split(elem, qn, std::min(r, MF_MAXNT * 16));
+    d.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(sp.target_blocks, d.row_tiles), qn / sp.min_kspan));
+    d.kspan = round_up(ceil_div(qn, d.nsplit), kq);
+    d.nsplit = (int)ceil_div(qn, d.kspan);
+    d.ldp = round_up(qn, 64);
+    return d;
+}
+
+// Y[0:qn, c] += alpha * Qd X[0:qn, c] through the FP16 halves of Qd32
+// (CB_PREC_MIXED_F16): X split per column (k_split_x), the product on FP16
+// MFMA (k_dense_h), then the FP64 split sum, scale and stiff term as in the
+// FP32 path.
+void dense_apply_h(cb_mme* M, int64_t qn, const float* X, int64_t ldx, float* Y, int64_t ldy, int r, float alpha,
+                   const StiffTerm* sk) {
+    cb_context* ctx = M->ctx;
+    hipStream_t st = ctx->stream;
+    CB_REQUIRE(M->q16_rows == qn && M->Qhl16.p, "FP16 halves of Q missing");
+    const int64_t ldh = round_up(qn, 64);
+    if (M->xh16.n < (size_t)ldh * r) {
+        M->xh16.alloc((size_t)ldh * r);
+        M->xl16.alloc((size_t)ldh * r);
+    }
+    if (M->xscale.n < (size_t)r) M->xscale.alloc(r);
+    k_split_x<<<(unsigned)r, SPLIT_X_THREADS, 0, st>>>(X, ldx, qn, ldh, M->q16_scale, M->xh16.p, M->xl16.p,
+                                                       M->xscale.p);
+    const DenseShape d = dense_shape(2, qn, r);
+    for (int c0 = 0; c0 < r; c0 += MF_MAXNT * 16) {
+        const int rc = std::min(MF_MAXNT * 16, r - c0);
+        float* part = partbuf<float>(M, (size_t)d.nsplit * rc * d.ldp);
+        const unsigned grid = (unsigned)(d.row_tiles * d.nsplit);
+        const _Float16* xh = M->xh16.p + (int64_t)c0 * ldh;
+        const _Float16* xl = M->xl16.p + (int64_t)c0 * ldh;
+        timed(ctx, "k_dense_h", [&] {
+#define CB_DH(NT_)                                                                                            \
+    k_dense_h<NT_><<<grid, 256, 0, st>>>(M->Qhl16.p, M->ldq, qn, xh, xl, ldh, rc, d.kspan, d.nsplit,          \
+                                         part, d.ldp)
+            switch ((rc + 15) / 16) {
+                case 1: CB_DH(1); break;
+                case 2: CB_DH(2); break;
+                case 3: CB_DH(3); break;
+                case 4: CB_DH(4); break;
+                default: CB_DH(5); break;
+            }
+#undef CB_DH
+        });
+        k_dense_reduce<float><<<dim3(blocks(qn), (unsigned)rc), 256, 0, st>>>(
+            part, d.ldp, d.nsplit, qn, rc, Y + (int64_t)c0 * ldy, ldy, 0, alpha, sk ? sk->v : nullptr,
+            sk ? sk->part : nullptr, c0, sk ? sk->coef : 0.0, M->xscale.p + c0);
+        CB_HIP(hipGetLastError());
+    }
+}
+
+// Builds the FP16 halves of Qd32 (rows x ldq) for CB_PREC_MIXED_F16, unless
+// they are current: one scale for the whole block puts its largest
+// magnitude just below 2^14.
+void ensure_q16(cb_mme* M, int64_t rows) {
+    if (M->q16_rows == rows && M->Qhl16.p) return;
+    hipStream_t st = M->ctx->stream;
+    const int64_t n = rows * M->ldq;
+    CB_REQUIRE(M->ldq % MF_KC == 0, "FP16 halves of Q: leading dimension not a multiple of 32");
+    M->Qhl16.alloc((size_t)2 * round_up(rows, 16) * M->ldq);
+    CB_HIP(hipMemsetAsync(M->Qhl16.p, 0, M->Qhl16.bytes(), st));
+    DevBuf<unsigned> mx;
+    mx.alloc(1);
+    CB_HIP(hipMemsetAsync(mx.p, 0, sizeof(unsigned), st));
+    k_absmax_bits<<<1024, 256, 0, st>>>(M->Qd32.p, n, mx.p);
+    CB_HIP(hipGetLastError());
+    unsigned bits = 0;
+    copy_host(st, &bits, mx.p, sizeof(unsigned), hipMemcpyDeviceToHost);
+    CB_HIP(hipStreamSynchronize(st));
+    float m;
+    std::memcpy(&m, &bits, sizeof m);
+    CB_REQUIRE(std::isfinite(m), "FP16 halves of Q: non-finite entries");
+    int e = 0;
+    if (m > 0) (void)std::frexp(m, &e);
+    M->q16_scale = m > 0 ? 14 - e : 0;
+    k_split_q<<<blocks(n), 256, 0, st>>>(M->Qd32.p, n, M->ldq, M->q16_scale, M->Qhl16.p);
+    CB_HIP(hipGetLastError());
+    M->q16_rows = rows;
+}
+
 template <class T>
 void dense_apply(cb_mme* M, int64_t qn, const T* X, int64_t ldx, T* Y, int64_t ldy, int r, T alpha,
                  const StiffTerm* sk = nullptr) {
@@ -1002,13 +1311,15 @@ void dense_apply(cb_mme* M, int64_t qn, const T* X, int64_t ldx, T* Y, int64_t l
         CB_HIP(hipGetLastError());
         return;
     }
-    const int64_t row_tiles = ceil_div(qn, MF_ROWS);
-    // about target_blocks blocks, splits of at least min_kspan k (tuned table)
-    const DenseSplit sp = dense_split((int)sizeof(T), qn, std::min(r, MF_MAXNT * 16));
-    int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(sp.target_blocks, row_tiles), qn / sp.min_kspan));
-    const int64_t kspan = round_up(ceil_div(qn, nsplit), MF_KC);
-    nsplit = (int)ceil_div(qn, kspan);
-    const int64_t ldp = round_up(qn, 64);
+    if constexpr (sizeof(T) == 4) {
+        if (M->use_f16) {
+            dense_apply_h(M, qn, X, ldx, Y, ldy, r, alpha, sk);
+            return;
+        }
+    }
+    const DenseShape d = dense_shape((int)sizeof(T), qn, r);
+    const int64_t row_tiles = d.row_tiles, kspan = d.kspan, ldp = d.ldp;
+    const int nsplit = d.nsplit;
     for (int c0 = 0; c0 < r; c0 += MF_MAXNT * 16) {
         const int rc = std::min(MF_MAXNT * 16, r - c0);
         // FP32: whole 16-column MFMA tiles and one or two columns past them
@@ -1589,6 +1900,7 @@ static SolveStats solve_block(cb_mme* M, double lambda, const cb_solve_options& 
     SolveStats S;
     if (!warm) CB_HIP(hipMemsetAsync(X, 0, nv * 8, st));
     std::vector<double> rel;
+    CB_REQUIRE(o.precision >= CB_PREC_FP64 && o.precision <= CB_PREC_MIXED_F16, "unknown precision");
     if (o.precision == CB_PREC_FP64) {
         if (o.batch_rhs || r == 1) {
             S.iterations = pcg_block<double, double>(M, lambda, B, X, r, o.tol, absolute, o.max_iter, block, &rel, !warm);
@@ -1612,6 +1924,14 @@ static SolveStats solve_block(cb_mme* M, double lambda, const cb_solve_options& 
     // (and before any column is declared converged). The search directions are
     // kept across replacements.
     CB_REQUIRE(M->qkind == 1 || M->Qd32.p, "FP32 copy of Q missing");
+    struct F16Scope {
+        cb_mme* M;
+        ~F16Scope() { M->use_f16 = false; }
+    } f16{M};
+    if (o.precision == CB_PREC_MIXED_F16 && M->qkind != 1) {
+        ensure_q16(M, M->qkind == 0 ? M->q : M->ng);
+        M->use_f16 = true;
+    }
     S.iterations = pcg_reliable(M, lambda, B, X, r, o, &S.max_rel, warm);
     return S;
 }
@@ -2019,6 +2339,7 @@ static void find_stiff_direction(cb_mme* M) {
     // FP32 copy of the bulk only: Q32 = fp32(Q - mu v v'); the rank-1 part is
     // applied separately with FP64 accumulation (apply_C<float>)
     k_q32_bulk<<<blocks(q * M->ldq), 256, 0, st>>>(M->Qd64.p, M->Qd32.p, M->stiff_v.p, mu, q, M->ldq);
+    M->q16_rows = 0;
     CB_HIP(hipGetLastError());
     CB_HIP(hipStreamSynchronize(st));
 }
@@ -2066,6 +2387,7 @@ static void set_q_dense_common(cb_mme* M, const double* Q, int64_t ld, hipMemcpy
     if (ldq > q) CB_HIP(hipMemsetAsync(M->Qd64.p, 0, M->Qd64.bytes(), st));
     copy_host_2d(st, M->Qd64.p, ldq * 8, Q, ld * 8, q * 8, q, kind);
     k_convert<float, double><<<blocks(q * ldq), 256, 0, st>>>(M->Qd64.p, M->Qd32.p, q * ldq);
+    M->q16_rows = 0;
     CB_HIP(hipGetLastError());
     M->q_diag.resize(q);
     DevBuf<double> dg;
@@ -2124,6 +2446,7 @@ int cb_mme_set_qinv_split(cb_mme* M, const int64_t* ptr, const int32_t* col, con
         if (ldq > ng) CB_HIP(hipMemsetAsync(M->Qd64.p, 0, M->Qd64.bytes(), st));
         CB_HIP(hipMemcpy2DAsync(M->Qd64.p, ldq * 8, dD, ldd * 8, ng * 8, ng, hipMemcpyDeviceToDevice, st));
         k_convert<float, double><<<blocks(ng * ldq), 256, 0, st>>>(M->Qd64.p, M->Qd32.p, ng * ldq);
+        M->q16_rows = 0;
         CB_HIP(hipGetLastError());
         M->gmap.alloc(ng);
         copy_host(st, M->gmap.p, gmap, ng * 4, hipMemcpyHostToDevice);
@@ -2165,7 +2488,7 @@ int cb_mme_apply(cb_mme* M, double lambda, int64_t r, const double* X, double* Y
 
 // The tuner's probe: one dense operator shape with a given split.
 template <class T>
-static double tune_dense_time(cb_context* ctx, int64_t q, int r, const DenseSplit& sp, int reps) {
+static double tune_dense_time(cb_context* ctx, int64_t q, int r, const DenseSplit& sp, int reps, bool f16 = false) {
     hipStream_t st = ctx->stream;
     cb_mme M;
     M.ctx = ctx;
@@ -2185,6 +2508,12 @@ static double tune_dense_time(cb_context* ctx, int64_t q, int r, const DenseSpli
     k_tune_fill<T><<<blocks((int64_t)X.n), 256, 0, st>>>(X.p, (int64_t)X.n, 2);
     CB_HIP(hipMemsetAsync(Y.p, 0, Y.bytes(), st));
     CB_HIP(hipGetLastError());
+    if constexpr (sizeof(T) == 4) {
+        if (f16) {  // the FP16-halves product of CB_PREC_MIXED_F16 solves
+            ensure_q16(&M, q);
+            M.use_f16 = true;
+        }
+    }
     struct Probe {
         Probe(const DenseSplit* s) { g_split_probe = s; }
         ~Probe() { g_split_probe = nullptr; }
@@ -2211,13 +2540,15 @@ int cb_tune_dense_time(cb_context* ctx, int elem_bytes, int64_t q, int r, int ta
                        double* ms) {
     return guarded([&] {
         CB_REQUIRE(ctx && ms, "cb_tune_dense_time: NULL argument");
-        CB_REQUIRE(elem_bytes == 4 || elem_bytes == 8, "cb_tune_dense_time: elem_bytes must be 4 or 8");
+        CB_REQUIRE(elem_bytes == 2 || elem_bytes == 4 || elem_bytes == 8,
+                   "cb_tune_dense_time: elem_bytes must be 2 (FP16 halves), 4 or 8");
         CB_REQUIRE(q >= 64 && q <= 200000 && r >= 3 && r <= MF_MAXNT * 16 && reps >= 1,
                    "cb_tune_dense_time: q in [64, 200000], r in [3, 80], reps >= 1");
         CB_REQUIRE(target_blocks >= 1 && min_kspan >= MF_KC, "cb_tune_dense_time: target_blocks >= 1, min_kspan >= 32");
         set_device(ctx);
         const DenseSplit sp{target_blocks, min_kspan};
-        *ms = elem_bytes == 8 ? tune_dense_time<double>(ctx, q, r, sp, reps) : tune_dense_time<float>(ctx, q, r, sp, reps);
+        *ms = elem_bytes == 8 ? tune_dense_time<double>(ctx, q, r, sp, reps)
+                              : tune_dense_time<float>(ctx, q, r, sp, reps, elem_bytes == 2);
     });
 }
 

@@ -184,6 +184,11 @@ int cb_mme_apply(cb_mme* mme, double lambda, int64_t r, const double* X, double*
 
 #define CB_PREC_FP64 0   /* FP64 operator and vectors ("plain") */
 #define CB_PREC_MIXED 1  /* FP32 block PCG inside FP64 residual refinement */
+/* As MIXED, with the dense Q X of the FP32 recursion (r > 2 columns) on FP16
+   matrix cores: Q and X as hi + lo FP16 halves of power-of-two scaled values
+   (22 significant bits), three products with FP32 accumulation. No
+   counterpart in the reference (its operator is FP64, implicit_mme.h). */
+#define CB_PREC_MIXED_F16 2
 #define CB_PRECOND_JACOBI 0        /* JacobiPreconditioner, pcg_solver.h:35-57 */
 #define CB_PRECOND_BLOCK_JACOBI 1  /* BlockJacobiPreconditioner, pcg_solver.h:60-128 */
 #define CB_STOP_RELATIVE 0  /* ||r|| / ||b|| < tol, PCGSolver (pcg_solver.h:501) */
@@ -366,7 +371,8 @@ void cb_ainv_destroy(cb_ainv* a);
  * k range of Q over blocks; the split is looked up per (element size, q, r)
  * in a table the tuner generates (cb_dense_tuned.h).
  * cb_tune_dense_time: average ms of one Q X on synthetic q x q Q and r
- * columns (elem_bytes 4: FP32, 8: FP64; q in [64, 200000], r in [3, 80])
+ * columns (elem_bytes 2: the FP16 halves of CB_PREC_MIXED_F16, 4: FP32,
+ * 8: FP64; q in [64, 200000], r in [3, 80])
  * with about target_blocks blocks and splits of at least min_kspan k, over
  * reps launches after one warm-up.
  * cb_tune_dense_lookup: the split the library uses for that shape. */

@@ -65,8 +65,20 @@ def test_dense_split_lookup_follows_tuned_table():
                 return tb, mk
         return tune.DEFAULT
 
-    for elem in (4, 8):
+    for elem in (2, 4, 8):
         for q in (64, 599, 600, 849, 850, 3000, 19999, 20000, 28284, 28285, 10 ** 6):
             for r in (3, 8, 9, 24, 25, 48, 49, 65, 80, 200):
                 assert tune.lookup(elem, q, r) == expect(elem, q, r), (elem, q, r)
-    assert tune.lookup(2, 1000, 10) == tune.DEFAULT
+    assert tune.lookup(16, 1000, 10) == tune.DEFAULT
+
+
+def test_tuned_table_rows_round_trip():
+    """The tuner re-reads the rows it keeps (`--elems` tunes a subset) and
+    writes them back unchanged."""
+    from cosmicblup_amd import tune
+
+    rows = tune.read_table()
+    assert {e["elem"] for e in rows} == {2, 4, 8}
+    old = [l for l in tune.TABLE.read_text().splitlines() if l.startswith("    {")]
+    new = [l for l in tune.render(rows, "device").splitlines() if l.startswith("    {")]
+    assert new == old

@@ -52,7 +52,7 @@ def test_operator_matches_dense(cb, ctx):
         assert np.abs(Y - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("precision", [_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED])
+@pytest.mark.parametrize("precision", [_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED, _lib.CB_PREC_MIXED_F16])
 @pytest.mark.parametrize("precond", [_lib.CB_PRECOND_JACOBI, _lib.CB_PRECOND_BLOCK_JACOBI])
 def test_pcg_solves_match_dense(cb, ctx, precision, precond):
     fd, aid, y, Q = gblup_problem(missing_records=17)
@@ -115,7 +115,7 @@ def test_probes_match_oracle(cb, ctx):
     assert np.array_equal(R.probes(), orl.probes(M.q, 9, 1234))
 
 
-@pytest.mark.parametrize("precision", [_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED])
+@pytest.mark.parametrize("precision", [_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED, _lib.CB_PREC_MIXED_F16])
 def test_score_and_ai_match_oracle(cb, ctx, precision):
     fd, aid, y, Q = gblup_problem()
     M = mme.MixedModel(fd, aid, y, Q.shape[0])
@@ -272,7 +272,7 @@ def test_pblup_native_ainv_with_inbreeding_mixed_vs_dense(cb, ctx):
     Ad = A.toDense()
     C, rhs, _ = orl.mme_dense(design.dense_x(fd), rec, y, Ad, 2.0)
     ref = np.linalg.solve(C, rhs)
-    for prec in (_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED):
+    for prec in (_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED, _lib.CB_PREC_MIXED_F16):
         X, _, _ = M.solve(2.0, rhs, mme.solve_options(precision=prec, tol=1e-12 if prec == 0 else 1e-10))
         assert ebv_err(X[M.p:], ref[M.p:]) < EBV_TOL
     R = mme.AIREML(M, mme.reml_options(n_probes=16, seed=4, solve=mme.solve_options(tol=1e-10)))
@@ -337,7 +337,7 @@ def test_dense_exact_score_and_run_match_oracle(cb, ctx):
     assert ebv_err(R.solution[M.p:], o["sol"][M.p:]) < 1e-6
 
 
-@pytest.mark.parametrize("precision", [_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED])
+@pytest.mark.parametrize("precision", [_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED, _lib.CB_PREC_MIXED_F16])
 @pytest.mark.parametrize("qkind", ["dense", "csr"])
 def test_graph_replay_equals_plain_launches(cb, ctx, precision, qkind):
     """PCG iteration blocks replayed as HIP graphs give the same iterates,
@@ -415,7 +415,7 @@ def test_contexts_on_threads_match_serial(cb, ctx):
         assert par[i] == ref
 
 
-@pytest.mark.parametrize("precision", [_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED])
+@pytest.mark.parametrize("precision", [_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED, _lib.CB_PREC_MIXED_F16])
 def test_device_resident_solve_equals_host_solve(cb, ctx, precision):
     """cb_mme_solve_device on torch-owned blocks with padded leading
     dimensions gives the host-array solve's iterates bit for bit, and leaves

@@ -61,7 +61,7 @@ def test_split_operator_solve_and_reml(cb, ctx):
         Y = M.apply(1.7, Xb[:, :r])
         assert np.abs(Y - C @ Xb[:, :r]).max() <= 1e-12 * np.abs(C @ Xb[:, :r]).max()
     ref = np.linalg.solve(C, rhs)
-    for prec in (_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED):
+    for prec in (_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED, _lib.CB_PREC_MIXED_F16):
         X, _, _ = M.solve(1.7, rhs, mme.solve_options(precision=prec, tol=1e-12 if prec == 0 else 1e-10))
         assert np.abs(X[M.p:] - ref[M.p:]).max() <= 1e-6 * np.abs(ref[M.p:]).max()
     R = mme.AIREML(M, mme.reml_options(n_probes=16, seed=2, solve=mme.solve_options(tol=1e-10)))
