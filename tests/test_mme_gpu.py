@@ -52,6 +52,32 @@ def test_operator_matches_dense(cb, ctx):
         assert np.abs(Y - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("r", [20, 65])
+def test_f16_operator_solves_match_fp64_at_several_splits(cb, ctx, r):
+    """The FP16-halves product at q = 2999: 24 row tiles (the last partial),
+    a last 16-row strip half padding, several k splits with a short last
+    one, and a padded 16-column tile (r = 20) or five tiles (r = 65). Its
+    solves must match FP64 ones; a wrong fragment, split or scale would
+    leave errors far above the FP64 tolerance."""
+    n, rng = 2999, np.random.default_rng(29)
+    Z = rng.standard_normal((n, 400))
+    G = Z @ Z.T / 400 + 0.1 * np.eye(n)
+    Q = np.linalg.inv(G)
+    Q = (Q + Q.T) / 2
+    fd = design.build_fixed_design([[f"g{i % 5}"] for i in range(n)])
+    y = rng.standard_normal(n) + np.arange(n) % 5
+    M = mme.MixedModel(fd, np.arange(n, dtype=np.int32), y, n)
+    M.set_qinv_dense(Q)
+    B = np.zeros((M.dim, r))
+    B[:, 0] = M.rhs()
+    B[M.p:, 1:] = orl.probes(M.q, r - 1, seed=5)
+    X64, _, _ = M.solve(0.8, B, mme.solve_options(precision=_lib.CB_PREC_FP64, tol=1e-12))
+    X16, _, rel = M.solve(0.8, B, mme.solve_options(precision=_lib.CB_PREC_MIXED_F16, tol=1e-11))
+    assert rel < 1e-11
+    for c in range(r):
+        assert ebv_err(X16[M.p:, c], X64[M.p:, c]) < 1e-8, c
+
+
 @pytest.mark.parametrize("precision", [_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED, _lib.CB_PREC_MIXED_F16])
 @pytest.mark.parametrize("precond", [_lib.CB_PRECOND_JACOBI, _lib.CB_PRECOND_BLOCK_JACOBI])
 def test_pcg_solves_match_dense(cb, ctx, precision, precond):
