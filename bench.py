@@ -67,7 +67,7 @@ def decode_rows_dosage(rows, pitch: int, n: int, snps):
 
 
 def reml_leg(ctx, n: int = 20000, m: int = 50000, k: int = 64, h2: float = 0.4, ngroups: int = 20,
-             max_iter: int = 20, plain_iters: int = 2, seed: int = 20264203) -> dict:
+             max_iter: int = 20, plain_iters: int = 0, seed: int = 20264203) -> dict:
     """REML iterations/s, GPU mixed (FP32 block PCG of 64 probes + y with FP64
     refinement) against GPU FP64 plain (FP64 PCG, one right-hand side at a
     time), same AI-REML algorithm and probes."""
@@ -138,7 +138,8 @@ def reml_leg(ctx, n: int = 20000, m: int = 50000, k: int = 64, h2: float = 0.4, 
     print(f"[bench] mixed f16: {Rh.iterations} iterations in {th:.2f} s, vu={Rh.var_u:.5f} ve={Rh.var_e:.5f}",
           flush=True)
     h_ms, h_n = ctx.kernel_time("k_dense_h")
-    Rp, tp = run(_lib.CB_PREC_FP64, False, plain_iters)
+    # FP64 plain over the same run as mixed (plain_iters > 0 caps it)
+    Rp, tp = run(_lib.CB_PREC_FP64, False, plain_iters or max_iter)
     print(f"[bench] fp64 plain: {Rp.iterations} iterations in {tp:.2f} s", flush=True)
     # FP64 with the same batching: the variance-component reference for the
     # mixed path (same probes, same AI-REML) and a second speed point
@@ -228,15 +229,13 @@ def c5_leg(ctx, n_inst: int = 96, m: int = 20000, n_lo: int = 500, n_hi: int = 8
     MAF U(0.05, 0.5); even instances single-trait (AI-REML, mixed batched
     PCG with k probes + y), odd instances 3-trait (r_g ~ U(-0.5, 0.8), h2 ~
     U(0.1, 0.6), r_e = 0.2; projected eigen-rotation EM, then the Kronecker
-    MME solve). Per instance: GRM and G^-1 on the device (set-up, untimed),
-    then the timed estimation. Reports REML (or EM) iterations/s per size
-    bucket. Then the single-trait instances are served again together: each
-    of `workers` host threads drives its own context (stream) through its
-    share of them (balanced on n^2, largest first), and the wall time of the
-    batch is compared with the serial sum; the estimates must be
-    bit-identical to the serial ones. The 3-trait instances stay serial:
-    their dense solve (rocSOLVER Cholesky) gave different solutions when
-    other work ran on the GPU beside it, even a plain torch matmul loop."""
+    MME solution by the canonical transformation and PCG). Per instance:
+    GRM and G^-1 on the device (set-up, untimed), then the timed estimation,
+    serially. Reports REML (or EM) iterations/s per size bucket. Then every
+    instance is served again together: each of `workers` host threads
+    drives its own context (stream) through its share (balanced on n^2,
+    largest first), and the batch's wall time is compared with the serial
+    sum; the estimates must be bit-identical to the serial ones."""
     import math
     import time
 
@@ -251,17 +250,18 @@ def c5_leg(ctx, n_inst: int = 96, m: int = 20000, n_lo: int = 500, n_hi: int = 8
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(math.log(n_lo), math.log(n_hi), n_inst)).astype(int)
     print(f"[bench] C5 leg: {n_inst} instances, n in [{sizes.min()}, {sizes.max()}], m={m}", flush=True)
-    # single-trait instances to serving contexts: largest n^2 first, each to
-    # the least loaded worker
+    # instances to serving contexts: largest n^2 first, each to the least
+    # loaded worker
     ctxs = [ctx] + [_lib.Context(0) for _ in range(workers - 1)]
     load = [0.0] * workers
     owner = {}
-    for idx in sorted(range(0, n_inst, 2), key=lambda i: -int(sizes[i])):
+    for idx in sorted(range(n_inst), key=lambda i: -int(sizes[i])):
         w = min(range(workers), key=lambda j: load[j])
         owner[idx] = w
         load[w] += float(sizes[idx]) ** 2
     served = []  # (worker, idx, job, serial result); job() reruns the timed estimation
     recs = []
+    keep = []  # device inputs the served jobs read again
     for idx, n in enumerate(sizes):
         n = int(n)
         rows, pitch = synth.bed_on_device(n, m, maf=(0.05, 0.5), missing_rate=0.0, seed=seed + 1 + idx)
@@ -280,31 +280,23 @@ def c5_leg(ctx, n_inst: int = 96, m: int = 20000, n_lo: int = 500, n_hi: int = 8
         Xq = decode_rows_dosage(rows, pitch, n, qtl)
         Xc = (Xq - Xq.mean(0)).cpu().numpy()
         del rows, gb, reader
+        mctx = ctxs[owner[idx]]
         if idx % 2 == 0:
             h2 = float(rng.uniform(0.1, 0.6))
             g = Xc @ rng.standard_normal(200)
             g *= math.sqrt(h2) / g.std()
             y = 10.0 + rng.standard_normal(10)[grp] + g + rng.standard_normal(n) * math.sqrt(1 - h2)
-            mctx = ctxs[owner[idx]]
             ctx.synchronize()
             model = mme.MixedModel(fd, np.arange(n, dtype=np.int32), y, n, ctx=mctx)
             model.set_qinv_dense_device(Q.data_ptr(), n)
             ropts = mme.reml_options(n_probes=k, seed=idx, max_iter=20, tol=1e-4, solve=mme.solve_options(tol=1e-8))
-            R = mme.AIREML(model, ropts)
-            mctx.synchronize()
-            t0 = time.perf_counter()
-            R.solve()
-            mctx.synchronize()
-            dt = time.perf_counter() - t0
-            recs.append({"n": n, "traits": 1, "iterations": R.iterations, "seconds": dt, "converged": R.converged})
 
             def job(model=model, ropts=ropts):
                 R = mme.AIREML(model, ropts)
                 R.solve()
                 model.ctx.synchronize()
-                return R.iterations, R.var_u, R.var_e
-
-            served.append((owner[idx], idx, job, (R.iterations, R.var_u, R.var_e)))
+                return R, (R.iterations, R.var_u, R.var_e, R.solution.tobytes())
+            del G, Q
         else:
             t = 3
             h2 = rng.uniform(0.1, 0.6, t)
@@ -322,21 +314,27 @@ def c5_leg(ctx, n_inst: int = 96, m: int = 20000, n_lo: int = 500, n_hi: int = 8
             E = rng.standard_normal((n, t)) @ np.linalg.cholesky(Ve).T
             X = design.dense_x(fd)
             Y = X @ rng.standard_normal((X.shape[1], t)) + U + E
-            mctx = ctx
-            v = multitrait.MvEigenVce(multitrait.MvEigenVce.Options(max_iter=500, tol=1e-4), ctx=mctx)
-            v.prepare(Y, X, G)
-            ctx.synchronize()
-            t0 = time.perf_counter()
-            v.runNullModel()
-            beta, u = multitrait.solveMultiTrait(Y, X, Q, v.getVg(), v.getVe(), ctx=mctx)
-            mctx.synchronize()
-            dt = time.perf_counter() - t0
-            recs.append({"n": n, "traits": 3, "iterations": v.iterations, "seconds": dt, "converged": v.converged})
+            keep.append((G, Q))
 
-        del G, Q
+            def job(Y=Y, X=X, G=G, Q=Q, mctx=mctx):
+                v = multitrait.MvEigenVce(multitrait.MvEigenVce.Options(max_iter=500, tol=1e-4), ctx=mctx)
+                v.prepare(Y, X, G)
+                v.runNullModel()
+                beta, u = multitrait.solveMultiTrait(Y, X, Q, v.getVg(), v.getVe(), ctx=mctx)
+                mctx.synchronize()
+                return v, (v.iterations, v.getVg().tobytes(), v.getVe().tobytes(), beta.tobytes(), u.tobytes())
+        ctx.synchronize()
+        mctx.synchronize()
+        t0 = time.perf_counter()
+        est, bits = job()
+        dt = time.perf_counter() - t0
+        traits = 1 if idx % 2 == 0 else 3
+        recs.append({"n": n, "traits": traits, "iterations": est.iterations, "seconds": dt,
+                     "converged": bool(est.converged)})
+        served.append((owner[idx], idx, job, bits))
         torch.cuda.empty_cache()
-        print(f"[bench] C5 instance {idx + 1}/{n_inst}: n={n} traits={recs[-1]['traits']} "
-              f"{recs[-1]['iterations']} iterations in {recs[-1]['seconds']:.3f} s", flush=True)
+        print(f"[bench] C5 instance {idx + 1}/{n_inst}: n={n} traits={traits} "
+              f"{recs[-1]['iterations']} iterations in {dt:.3f} s", flush=True)
     buckets = [(500, 1000), (1000, 2000), (2000, 4000), (4000, 8001)]
     out = {"instances": n_inst, "m": m, "probes": k, "per_bucket": []}
     for lo, hi in buckets:
@@ -352,7 +350,7 @@ def c5_leg(ctx, n_inst: int = 96, m: int = 20000, n_lo: int = 500, n_hi: int = 8
     out["total_seconds"] = sum(r["seconds"] for r in recs)
 
     def serve(w):
-        return [(idx, job()) for wk, idx, job, _ in served if wk == w]
+        return [(idx, job()[1]) for wk, idx, job, _ in served if wk == w]
 
     for c in ctxs:
         c.synchronize()
@@ -360,16 +358,16 @@ def c5_leg(ctx, n_inst: int = 96, m: int = 20000, n_lo: int = 500, n_hi: int = 8
     with ThreadPoolExecutor(workers) as ex:
         got = dict(sum(ex.map(serve, range(workers)), []))
     wall = time.perf_counter() - t0
-    serial = sum(r["seconds"] for r in recs if r["traits"] == 1)
-    its = sum(r["iterations"] for r in recs if r["traits"] == 1)
-    out["served_single_trait"] = {
-        "instances": len(served), "workers": workers, "reml_iterations": its, "wall_seconds": wall,
-        "serial_seconds": serial, "speedup_vs_serial": serial / wall, "reml_iters_per_s": its / wall,
+    serial = out["total_seconds"]
+    its = sum(r["iterations"] for r in recs)
+    out["served"] = {
+        "instances": len(served), "workers": workers, "iterations": its, "wall_seconds": wall,
+        "serial_seconds": serial, "speedup_vs_serial": serial / wall,
         "identical_to_serial": all(got[idx] == ref for _, idx, _, ref in served),
         "differing_instances": [idx for _, idx, _, ref in served if got[idx] != ref]}
-    print(f"[bench] C5 served: {len(served)} single-trait instances on {workers} streams in {wall:.3f} s "
-          f"(serial {serial:.3f} s)", flush=True)
-    del served
+    print(f"[bench] C5 served: {len(served)} instances on {workers} streams in {wall:.3f} s "
+          f"(serial {serial:.3f} s), identical {out['served']['identical_to_serial']}", flush=True)
+    del served, keep
     return out
 
 
@@ -475,8 +473,8 @@ def pcg_leg(ctx, n_gen: int = 10, per_gen: int = 100000, n_rec: int = 600000, n_
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=50000)
     ap.add_argument("--m", type=int, default=50000)
     ap.add_argument("--missing", type=float, default=0.01)
@@ -484,7 +482,8 @@ def main() -> None:
     ap.add_argument("--no-reml", action="store_true")
     ap.add_argument("--reml-n", type=int, default=20000)
     ap.add_argument("--reml-m", type=int, default=50000)
-    ap.add_argument("--plain-iters", type=int, default=2)
+    ap.add_argument("--plain-iters", type=int, default=0,
+                    help="cap the FP64 plain REML run (0: the same max_iter as the mixed run)")
     ap.add_argument("--reml-only", action="store_true",
                     help="only run the REML leg and print its JSON object")
     ap.add_argument("--no-pcg", action="store_true", help="skip the 1M-animal PBLUP PCG leg")
@@ -558,12 +557,14 @@ def main() -> None:
     ops = float(m) * n * (n + 1)
     kern = {}
     for k in ("k_snp_stats", "k_pack", "k_row_sums", "k_miss_snp_lists", "k_miss_sample_lists", "k_sample_sq",
+              "k_encode_d3",
               "k_miss_f", "k_miss_cross", "k_gram"):
         ms, cnt = ctx.kernel_time(k)
         if cnt:
             kern[k] = ms / cnt
     gram_ms = kern["k_gram"]
     achieved = ops / (gram_ms * 1e-3) / 1e12
+    kern_sum = sum(kern.values())
     out = {
         "metric": METRIC,
         "value": ops / dt / 1e12,
@@ -584,6 +585,9 @@ def main() -> None:
                      "frac": achieved / PEAK_INT8_TOPS,
                      "traffic": K_GRAM_TRAFFIC_BYTES if (n, m, args.missing) == (50000, 50000, 0.01) else None},
         "kernel_ms": kern,
+        # per step: the kernels' own time and what lies outside them (host
+        # statistics, list building, copies, launch gaps)
+        "step_breakdown_ms": {"kernels": kern_sum, "outside_kernels": dt * 1e3 - kern_sum},
     }
     del gb, reader
     torch.cuda.empty_cache()

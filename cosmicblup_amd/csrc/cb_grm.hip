@@ -998,7 +998,26 @@ struct cb_grm {
     DevBuf<double> G64;
     DevBuf<float> G32;
     DevBuf<int32_t> Gi;
+    // work buffers of compute, kept from one call to the next and only grown,
+    // so a repeated compute at a fixed shape allocates and frees nothing
+    DevBuf<uint32_t> X2;              // sample-major packed dosages
+    DevBuf<double> u;                 // per-sample centring terms
+    DevBuf<double> E;                 // missing-call correction, n x n
+    DevBuf<int64_t> snp_off, smp_off;
+    DevBuf<int32_t> snp_list, smp_cnt, lj, cursor;
+    DevBuf<uint32_t> lk, lg;
+    DevBuf<unsigned long long> sq;
+    DevBuf<double> sqd;
+    DevBuf<uint8_t> bedd;             // re-encoded rows for the missing-call kernel
 };
+
+namespace {
+// grow-only allocation of a work buffer (contents undefined)
+template <class T>
+void grow(DevBuf<T>& b, size_t count) {
+    if (b.n < count) b.alloc(count);
+}
+}  // namespace
 
 static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
     cb_context* ctx = grm->ctx;
@@ -1053,34 +1072,36 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
     grm->total_valid = nvalid;
 
     // 2. sample-major packed dosages
-    DevBuf<uint32_t> X2;
-    X2.alloc((size_t)n_stages * n_pad * 16);
+    DevBuf<uint32_t>& X2 = grm->X2;
+    grow(X2, (size_t)n_stages * n_pad * 16);
     timed(ctx, "k_pack", [&] {
         k_pack<<<dim3((unsigned)(n_pad / TILE), (unsigned)n_stages), dim3(256), 0, st>>>(
             g->bed.p, g->pitch, n, m, n_pad, grm->valid.p, X2.p);
     });
 
     // 3. per-sample centring terms
-    DevBuf<double> u;
-    u.alloc(n_pad);
-    CB_HIP(hipMemsetAsync(u.p, 0, u.bytes(), st));
+    DevBuf<double>& u = grm->u;
+    grow(u, n_pad);
+    CB_HIP(hipMemsetAsync(u.p, 0, n_pad * 8, st));
     timed(ctx, "k_row_sums", [&] {
         k_row_sums<<<dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st>>>(X2.p, n, n_pad, n_stages,
                                                                         grm->kfix.p, u.p);
     });
 
     // 4. missing calls among the kept SNPs
-    DevBuf<double> E;
-    if (nmiss_total > 0 && (outputs & (CB_GRM_OUT_F64 | CB_GRM_OUT_F32))) {
+    const bool has_e = nmiss_total > 0 && (outputs & (CB_GRM_OUT_F64 | CB_GRM_OUT_F32));
+    DevBuf<double>& E = grm->E;
+    if (has_e) {
         std::vector<int64_t> snp_off(m + 1, 0);
         for (int64_t j = 0; j < m; ++j) snp_off[j + 1] = snp_off[j] + (hvalid[j] ? n - hnobs[j] : 0);
-        DevBuf<int64_t> d_snp_off;
-        d_snp_off.alloc(m + 1);
+        DevBuf<int64_t>& d_snp_off = grm->snp_off;
+        grow(d_snp_off, m + 1);
         copy_host(st, d_snp_off.p, snp_off.data(), (m + 1) * 8, hipMemcpyHostToDevice);
-        DevBuf<int32_t> snp_list, smp_cnt;
-        snp_list.alloc(std::max<int64_t>(1, nmiss_total));
-        smp_cnt.alloc(n);
-        CB_HIP(hipMemsetAsync(smp_cnt.p, 0, smp_cnt.bytes(), st));
+        DevBuf<int32_t>& snp_list = grm->snp_list;
+        DevBuf<int32_t>& smp_cnt = grm->smp_cnt;
+        grow(snp_list, std::max<int64_t>(1, nmiss_total));
+        grow(smp_cnt, n);
+        CB_HIP(hipMemsetAsync(smp_cnt.p, 0, n * 4, st));
         timed(ctx, "k_miss_snp_lists", [&] {
             k_miss_snp_lists<<<dim3((unsigned)ceil_div(m, 4)), dim3(256), 0, st>>>(
                 g->bed.p, g->pitch, n, m, grm->valid.p, d_snp_off.p, snp_list.p, smp_cnt.p);
@@ -1090,20 +1111,21 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
         CB_HIP(hipStreamSynchronize(st));
         std::vector<int64_t> smp_off(n + 1, 0);
         for (int64_t i = 0; i < n; ++i) smp_off[i + 1] = smp_off[i] + round_up(hcnt[i], 8);
-        DevBuf<int64_t> d_smp_off;
-        d_smp_off.alloc(n + 1);
+        DevBuf<int64_t>& d_smp_off = grm->smp_off;
+        grow(d_smp_off, n + 1);
         copy_host(st, d_smp_off.p, smp_off.data(), (n + 1) * 8, hipMemcpyHostToDevice);
         const int64_t nlist = std::max<int64_t>(8, smp_off[n]);
-        DevBuf<int32_t> lj, cursor;
-        DevBuf<uint32_t> lk;
-        DevBuf<unsigned long long> sq;  // per sample: low words, then high words
-        lj.alloc(nlist);
-        lk.alloc(nlist);
-        cursor.alloc(n);
-        sq.alloc(2 * n);
-        CB_HIP(hipMemsetAsync(lj.p, 0, lj.bytes(), st));
-        CB_HIP(hipMemsetAsync(lk.p, 0, lk.bytes(), st));
-        CB_HIP(hipMemsetAsync(cursor.p, 0, cursor.bytes(), st));
+        DevBuf<int32_t>& lj = grm->lj;
+        DevBuf<int32_t>& cursor = grm->cursor;
+        DevBuf<uint32_t>& lk = grm->lk;
+        DevBuf<unsigned long long>& sq = grm->sq;  // per sample: low words, then high words
+        grow(lj, nlist);
+        grow(lk, nlist);
+        grow(cursor, n);
+        grow(sq, 2 * n);
+        CB_HIP(hipMemsetAsync(lj.p, 0, nlist * 4, st));
+        CB_HIP(hipMemsetAsync(lk.p, 0, nlist * 4, st));
+        CB_HIP(hipMemsetAsync(cursor.p, 0, n * 4, st));
         timed(ctx, "k_miss_sample_lists", [&] {
             k_miss_sample_lists<<<dim3((unsigned)ceil_div(m, 4)), dim3(256), 0, st>>>(
                 d_snp_off.p, snp_list.p, m, grm->valid.p, grm->kfix.p, d_smp_off.p, cursor.p, lj.p,
@@ -1125,15 +1147,15 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
             hsqd[i] = (double)s_i * (1.0 / 4503599627370496.0);
             hu[i] += hsqd[i];
         }
-        DevBuf<double> sqd;
-        sqd.alloc(n);
+        DevBuf<double>& sqd = grm->sqd;
+        grow(sqd, n);
         copy_host(st, sqd.p, hsqd.data(), n * 8, hipMemcpyHostToDevice);
         copy_host(st, u.p, hu.data(), n * 8, hipMemcpyHostToDevice);
 
-        E.alloc((size_t)n * n);
-        DevBuf<uint8_t> bedd;
+        grow(E, (size_t)n * n);
+        DevBuf<uint8_t>& bedd = grm->bedd;
         const int64_t dpitch = round_up(g->pitch, 128);
-        bedd.alloc((size_t)dpitch * m);
+        grow(bedd, (size_t)dpitch * m);
         timed(ctx, "k_encode_d3", [&] {
             k_encode_d3<<<dim3((unsigned)ceil_div(dpitch / 4, 256), (unsigned)m), 256, 0, st>>>(
                 g->bed.p, bedd.p, g->pitch, dpitch, n, m);
@@ -1147,16 +1169,14 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
             });
         } else {
             const int kw = 64;
-            DevBuf<uint32_t> lg;
-            lg.alloc((size_t)nlist * 2);  // nlist / 4 groups of 8 words
+            DevBuf<uint32_t>& lg = grm->lg;
+            grow(lg, (size_t)nlist * 2);  // nlist / 4 groups of 8 words
             k_miss_groups<<<dim3((unsigned)ceil_div(nlist / 4, 256)), dim3(256), 0, st>>>(lj.p, lk.p, nlist / 4, lg.p);
             timed(ctx, "k_miss_cross", [&] {
                 k_miss_cross_h<<<dim3((unsigned)ceil_div(n, kw), (unsigned)ceil_div(n, MC_I)), dim3(256), 0, st>>>(
                     bedd.p, dpitch, n, dpitch * m, d_smp_off.p, lg.p, lk.p, sqd.p, kw, E.p, n);
             });
-            CB_HIP(hipStreamSynchronize(st));
         }
-        CB_HIP(hipStreamSynchronize(st));  // list buffers die at scope end
     }
 
     // 5. Gram + fused correction
@@ -1168,7 +1188,7 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
     if (outputs & CB_GRM_OUT_INT) grm->Gi.alloc((size_t)n * n); else grm->Gi.release();
     if (outputs & (CB_GRM_OUT_F64 | CB_GRM_OUT_F32)) {
         timed(ctx, "k_gram", [&] {
-            if (E.p)
+            if (has_e)
                 k_gram<0><<<dim3((unsigned)ntiles), dim3(256), 0, st>>>(
                     X2.p, n, n_pad, n_stages, u.p, Cc, total, E.p, n, grm->G64.p, grm->G32.p, nullptr, n);
             else
@@ -1182,7 +1202,7 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
                 X2.p, n, n_pad, n_stages, u.p, Cc, total, nullptr, n, nullptr, nullptr, grm->Gi.p, n);
         });
     }
-    CB_HIP(hipStreamSynchronize(st));  // X2 / u / E are released at scope end
+    CB_HIP(hipStreamSynchronize(st));
 }
 
 extern "C" {
@@ -1233,7 +1253,7 @@ int cb_grm_compute(cb_grm* grm, const cb_genotypes* g, int outputs) {
 static void copy_out(const cb_grm* grm, const void* src, size_t esz, void* dst, int64_t ld) {
     CB_REQUIRE(src, "requested GRM output was not computed");
     CB_REQUIRE(dst && ld >= grm->n, "bad destination / leading dimension");
-    CB_HIP(hipSetDevice(grm->ctx->device));
+    set_device(grm->ctx);
     copy_host_2d(grm->ctx->stream, dst, ld * esz, src, grm->n * esz, grm->n * esz, grm->n, hipMemcpyDeviceToHost);
     CB_HIP(hipStreamSynchronize(grm->ctx->stream));
 }
@@ -1262,7 +1282,7 @@ int cb_grm_copy_grm_device(const cb_grm* grm, void* dst, int64_t ld) {
     return guarded([&] {
         CB_REQUIRE(grm && grm->G64.p, "cb_grm_copy_grm_device: FP64 G not computed");
         CB_REQUIRE(dst && ld >= grm->n, "cb_grm_copy_grm_device: bad destination");
-        CB_HIP(hipSetDevice(grm->ctx->device));
+        set_device(grm->ctx);
         CB_HIP(hipMemcpy2DAsync(dst, ld * 8, grm->G64.p, grm->n * 8, grm->n * 8, grm->n,
                                 hipMemcpyDeviceToDevice, grm->ctx->stream));
         CB_HIP(hipStreamSynchronize(grm->ctx->stream));
@@ -1295,7 +1315,7 @@ int cb_grm_num_samples(const cb_grm* grm, int64_t* out) {
 int cb_grm_copy_allele_freqs(const cb_grm* grm, float* out, int64_t m) {
     return guarded([&] {
         CB_REQUIRE(grm && out && m == grm->m && grm->freq.p, "cb_grm_copy_allele_freqs: bad argument");
-        CB_HIP(hipSetDevice(grm->ctx->device));
+        set_device(grm->ctx);
         copy_host(grm->ctx->stream, out, grm->freq.p, m * 4, hipMemcpyDeviceToHost);
     });
 }
@@ -1305,7 +1325,7 @@ int cb_grm_copy_snp_stats(const cb_grm* grm, int64_t m, int32_t* n_obs, int32_t*
                           float* mean_f, float* twopq_f) {
     return guarded([&] {
         CB_REQUIRE(grm && m == grm->m && grm->nobs.p, "cb_grm_copy_snp_stats: bad argument");
-        CB_HIP(hipSetDevice(grm->ctx->device));
+        set_device(grm->ctx);
         if (n_obs) copy_host(grm->ctx->stream, n_obs, grm->nobs.p, m * 4, hipMemcpyDeviceToHost);
         if (sum_x) copy_host(grm->ctx->stream, sum_x, grm->sumx.p, m * 4, hipMemcpyDeviceToHost);
         if (n_hom1) copy_host(grm->ctx->stream, n_hom1, grm->h1.p, m * 4, hipMemcpyDeviceToHost);

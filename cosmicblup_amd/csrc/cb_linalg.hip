@@ -50,8 +50,8 @@ namespace cbamd {
 void invert_psd(cb_context* ctx, const double* dM, int64_t ldm, int64_t n, double ridge, double* dOut,
                 int64_t ldo, hipMemcpyKind in_kind, hipMemcpyKind out_kind, double* min_eig, double* max_eig) {
     CB_REQUIRE(n > 0 && n < (int64_t)1 << 31 && ldm >= n && ldo >= n, "cb_invert_psd: bad dimensions");
+    set_device(ctx);  // also points tls_stream at this context's stream (drains, releases)
     hipStream_t st = ctx->stream;
-    CB_HIP(hipSetDevice(ctx->device));
     DevBuf<double> U, V, l, e;
     DevBuf<int> info;
     U.alloc((size_t)n * n);

@@ -482,12 +482,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 }
 
 // Largest |Q| of n elements as float bits into *out (non-negative floats
-// order as their bit patterns); *out starts at 0.
+// order as their bit patterns); *out starts at 0. A NaN counts as +inf, so
+// it reaches the result (fmaxf alone would drop it).
 __global__ __launch_bounds__(256) void k_absmax_bits(const float* __restrict__ a, int64_t n, unsigned* out) {
     __shared__ float red[4];
     float m = 0.0f;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
-        m = fmaxf(m, fabsf(a[e]));
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const float v = fabsf(a[e]);
+        m = fmaxf(m, v == v ? v : __builtin_inff());
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
@@ -1829,7 +1832,13 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
                     for (size_t k = 1; k + 7 < h; ++k) before = std::min(before, hist[c][k]);
                     for (size_t k = h - 7; k < h; ++k) recent = std::min(recent, hist[c][k]);
                     const double rel = absolute ? std::sqrt(rr[c]) : std::sqrt(rr[c] / bb[c]);
-                    if (recent > 0.25 * before && rel < o.stall_tol) done[c] = 1;
+                    if (recent > 0.25 * before) {
+                        if (!(rel < o.stall_tol))
+                            throw Error(CB_ERR_NOT_CONVERGED,
+                                        "mixed-precision PCG stalled at the FP64 rounding floor (residual " +
+                                            std::to_string(rel) + ") above the tolerance; stall_tol accepts it");
+                        done[c] = 1;
+                    }
                 }
             }
             // a column paused by the recursive test (or one whose recursion
@@ -2263,7 +2272,7 @@ void cb_solve_options_default(cb_solve_options* o) {
     o->batch_rhs = 1;
     o->tol = 1e-10;
     o->max_iter = 10000;
-    o->stall_tol = 1e-8;
+    o->stall_tol = 0.0;  // strict: a stall above tol is CB_ERR_NOT_CONVERGED
 }
 
 void cb_reml_options_default(cb_reml_options* o) {
@@ -2275,6 +2284,9 @@ void cb_reml_options_default(cb_reml_options* o) {
     o->seed = 42;
     o->trace_exact = 0;
     cb_solve_options_default(&o->solve);
+    // REML solves accept the FP64 rounding floor up to 1e-8: the probes'
+    // sampling noise in the trace is orders of magnitude larger
+    o->solve.stall_tol = 1e-8;
     o->block_cols = 0;
 }
 

@@ -10,9 +10,16 @@
 // (t x t, t <= 4) in registers and reduces them over fixed row chunks; the
 // chunks are summed in order on the host, so every run gives the same bits.
 //
-// Solve. The Kronecker MME [Ve^-1 (x) X'X, Ve^-1 (x) X'; ., Ve^-1 (x) I +
-// Vg^-1 (x) G^-1] (unknowns trait-major) is assembled densely on the device
-// by one kernel and solved by Cholesky (dpotrf + dpotrs).
+// Solve. The reference assembles the Kronecker MME [Ve^-1 (x) X'X,
+// Ve^-1 (x) X'; ., Ve^-1 (x) I + Vg^-1 (x) G^-1] (unknowns trait-major) and
+// factorises it by dense LDLT, O((n t)^3). With complete records and one
+// design for every trait that system is t independent single-trait MMEs
+// after the canonical transformation of the traits (canonical() below), so
+// it is solved here as t single-trait PCG solves on the library's own
+// operator kernels (the dense G^-1 product, W'W, block-Jacobi with the stiff
+// direction deflated), FP64 to a relative residual of MV_SOLVE_TOL, and
+// transformed back: the same solution to rounding, deterministic, and
+// O(iterations n^2).
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -29,6 +36,7 @@ namespace {
 
 constexpr int MV_CHUNKS = 256;
 constexpr int MV_MAXT = 4;
+constexpr double MV_SOLVE_TOL = 1e-12;  // relative residual of each transformed-trait solve
 
 #define CB_RBS(call, what)                                                                             \
     do {                                                                                               \
@@ -157,61 +165,99 @@ __global__ __launch_bounds__(256) void k_mv_em(const double* __restrict__ d, con
     }
 }
 
-// Dense Kronecker MME of solveMultiTrait (column-major, ld = dim), unknowns
-// [beta: trait-major blocks of c | u: trait-major blocks of n].
-__global__ void k_mv_lhs(const double* __restrict__ XtX, const double* __restrict__ X, const double* __restrict__ Gi,
-                         int64_t ldg, const double* __restrict__ vi, int64_t n, int64_t c, int t,
-                         double* __restrict__ L, int64_t dim) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t s = blockIdx.y;
-    if (r >= dim) return;
-    const int64_t ct = c * t;
-    const double* Vei = vi;
-    const double* Vgi = vi + t * t;
-    double v;
-    if (r < ct && s < ct) {
-        const int64_t a = r / c, i = r % c, b = s / c, j = s % c;
-        v = Vei[a * t + b] * XtX[j * c + i];
-    } else if (r < ct) {
-        const int64_t a = r / c, i = r % c, b = (s - ct) / n, j = (s - ct) % n;
-        v = Vei[a * t + b] * X[i * n + j];
-    } else if (s < ct) {
-        const int64_t a = (r - ct) / n, i = (r - ct) % n, b = s / c, j = s % c;
-        v = Vei[a * t + b] * X[j * n + i];
-    } else {
-        const int64_t a = (r - ct) / n, i = (r - ct) % n, b = (s - ct) / n, j = (s - ct) % n;
-        v = (i == j ? Vei[a * t + b] : 0.0) + Vgi[a * t + b] * Gi[j * ldg + i];
-    }
-    L[s * dim + r] = v;
-}
+// Canonical transformation of a t-trait model with complete records and one
+// design for every trait: with Ve = L L' and L^-1 Vg L^-T = W diag(d) W',
+// T = L^-T W gives T' Ve T = I and T' Vg T = diag(d), so Y T splits into t
+// independent single-trait models with lambda_a = 1 / d_a, and the
+// Kronecker MME's solution is the transformed traits' solutions times T^-1
+// (T^-1 = W' L'). Matrices row-major t x t.
+struct Canonical {
+    std::vector<double> d, T, Tinv;
+};
 
-std::vector<double> inv_host(const std::vector<double>& A, int t) {
-    std::vector<double> m(A), inv((size_t)t * t, 0.0);
-    for (int i = 0; i < t; ++i) inv[i * t + i] = 1.0;
-    for (int k = 0; k < t; ++k) {
-        int pr = k;
-        for (int i = k + 1; i < t; ++i)
-            if (std::fabs(m[i * t + k]) > std::fabs(m[pr * t + k])) pr = i;
-        CB_REQUIRE(std::fabs(m[pr * t + k]) > 0, "multi-trait: singular covariance matrix");
-        for (int j = 0; j < t; ++j) {
-            std::swap(m[k * t + j], m[pr * t + j]);
-            std::swap(inv[k * t + j], inv[pr * t + j]);
-        }
-        const double piv = 1.0 / m[k * t + k];
-        for (int j = 0; j < t; ++j) {
-            m[k * t + j] *= piv;
-            inv[k * t + j] *= piv;
-        }
-        for (int i = 0; i < t; ++i) {
-            if (i == k) continue;
-            const double f = m[i * t + k];
-            for (int j = 0; j < t; ++j) {
-                m[i * t + j] -= f * m[k * t + j];
-                inv[i * t + j] -= f * inv[k * t + j];
-            }
+Canonical canonical(const std::vector<double>& Vg, const std::vector<double>& Ve, int t) {
+    // Cholesky of Ve (row-major lower L)
+    std::vector<double> L((size_t)t * t, 0.0);
+    for (int j = 0; j < t; ++j) {
+        double s = Ve[j * t + j];
+        for (int k = 0; k < j; ++k) s -= L[j * t + k] * L[j * t + k];
+        CB_REQUIRE(s > 0 && std::isfinite(s), "multi-trait: Ve is not positive definite");
+        L[j * t + j] = std::sqrt(s);
+        for (int i = j + 1; i < t; ++i) {
+            double v = Ve[i * t + j];
+            for (int k = 0; k < j; ++k) v -= L[i * t + k] * L[j * t + k];
+            L[i * t + j] = v / L[j * t + j];
         }
     }
-    return inv;
+    // Li = L^-1 (lower, forward substitution on unit columns)
+    std::vector<double> Li((size_t)t * t, 0.0);
+    for (int cidx = 0; cidx < t; ++cidx)
+        for (int i = 0; i < t; ++i) {
+            double v = i == cidx ? 1.0 : 0.0;
+            for (int k = 0; k < i; ++k) v -= L[i * t + k] * Li[k * t + cidx];
+            Li[i * t + cidx] = v / L[i * t + i];
+        }
+    // S = Li Vg Li'
+    std::vector<double> A((size_t)t * t, 0.0), S((size_t)t * t, 0.0);
+    for (int i = 0; i < t; ++i)
+        for (int j = 0; j < t; ++j)
+            for (int k = 0; k < t; ++k) A[i * t + j] += Li[i * t + k] * Vg[k * t + j];
+    for (int i = 0; i < t; ++i)
+        for (int j = 0; j < t; ++j)
+            for (int k = 0; k < t; ++k) S[i * t + j] += A[i * t + k] * Li[j * t + k];
+    for (int i = 0; i < t; ++i)
+        for (int j = 0; j < i; ++j) S[i * t + j] = S[j * t + i] = 0.5 * (S[i * t + j] + S[j * t + i]);
+    // cyclic Jacobi eigen-decomposition S = W diag(d) W'
+    std::vector<double> W((size_t)t * t, 0.0);
+    for (int i = 0; i < t; ++i) W[i * t + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0, tot = 0;
+        for (int i = 0; i < t; ++i)
+            for (int j = 0; j < t; ++j) (i == j ? tot : off) += S[i * t + j] * S[i * t + j];
+        if (off <= 1e-30 * tot) break;
+        for (int pi = 0; pi < t; ++pi)
+            for (int qi = pi + 1; qi < t; ++qi) {
+                const double apq = S[pi * t + qi];
+                if (apq == 0.0) continue;
+                const double th = (S[qi * t + qi] - S[pi * t + pi]) / (2.0 * apq);
+                const double tn = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double cs = 1.0 / std::sqrt(tn * tn + 1.0), sn = tn * cs;
+                for (int k = 0; k < t; ++k) {  // S <- S J (columns p, q)
+                    const double skp = S[k * t + pi], skq = S[k * t + qi];
+                    S[k * t + pi] = cs * skp - sn * skq;
+                    S[k * t + qi] = sn * skp + cs * skq;
+                }
+                for (int k = 0; k < t; ++k) {  // S <- J' S (rows p, q)
+                    const double spk = S[pi * t + k], sqk = S[qi * t + k];
+                    S[pi * t + k] = cs * spk - sn * sqk;
+                    S[qi * t + k] = sn * spk + cs * sqk;
+                }
+                for (int k = 0; k < t; ++k) {  // W <- W J
+                    const double wkp = W[k * t + pi], wkq = W[k * t + qi];
+                    W[k * t + pi] = cs * wkp - sn * wkq;
+                    W[k * t + qi] = sn * wkp + cs * wkq;
+                }
+            }
+    }
+    Canonical cz;
+    cz.d.resize(t);
+    double dmax = 0;
+    for (int i = 0; i < t; ++i) dmax = std::max(dmax, std::fabs(S[i * t + i]));
+    for (int i = 0; i < t; ++i) {
+        cz.d[i] = S[i * t + i];
+        CB_REQUIRE(cz.d[i] > 1e-12 * dmax && std::isfinite(cz.d[i]),
+                   "multi-trait: Vg is not positive definite (the MME needs Vg^-1)");
+    }
+    // T = Li' W, Tinv = W' L'
+    cz.T.assign((size_t)t * t, 0.0);
+    cz.Tinv.assign((size_t)t * t, 0.0);
+    for (int i = 0; i < t; ++i)
+        for (int j = 0; j < t; ++j)
+            for (int k = 0; k < t; ++k) {
+                cz.T[i * t + j] += Li[k * t + i] * W[k * t + j];
+                cz.Tinv[i * t + j] += W[k * t + i] * L[j * t + k];
+            }
+    return cz;
 }
 
 template <int T>
@@ -372,62 +418,72 @@ int cb_mv_solve(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* 
                 const double* dGinv, int64_t ldg, const double* Vg, const double* Ve, double* beta, double* u) {
     return guarded([&] {
         CB_REQUIRE(ctx && Y && dGinv && Vg && Ve && u && (c == 0 || (X && beta)), "cb_mv_solve: NULL argument");
-        CB_REQUIRE(t >= 1 && n > 0 && c >= 0 && ldg >= n, "cb_mv_solve: bad dimensions");
-        const int64_t ct = c * t, dim = ct + n * t;
-        CB_REQUIRE(dim <= 60000, "cb_mv_solve: dense MME limited to (n + c) t <= 60000");
+        CB_REQUIRE(t >= 1 && t <= MV_MAXT && n > 0 && c >= 0 && ldg >= n && n + c < INT32_MAX,
+                   "cb_mv_solve: bad dimensions");
         set_device(ctx);
-        hipStream_t st = ctx->stream;
         const int T = (int)t;
-        std::vector<double> vg(Vg, Vg + T * T), ve(Ve, Ve + T * T);
-        std::vector<double> vei = inv_host(ve, T), vgi = inv_host(vg, T);
-        // X'X and the RHS on the host (n x c and n x t inputs)
-        std::vector<double> xtx((size_t)c * c, 0.0), rhs(dim, 0.0);
-        for (int64_t i = 0; i < c; ++i)
+        const Canonical cz = canonical(std::vector<double>(Vg, Vg + T * T), std::vector<double>(Ve, Ve + T * T), T);
+        // the single-trait model [X'X X'; X I + lambda G^-1] shared by the
+        // transformed traits: one record per individual, fixed row X[i, :]
+        std::vector<int64_t> xp(n + 1);
+        std::vector<int32_t> xc((size_t)n * c), aid(n);
+        std::vector<double> xv((size_t)n * c), y0(n, 0.0);
+        for (int64_t i = 0; i < n; ++i) {
+            xp[i] = i * c;
+            aid[i] = (int32_t)i;
             for (int64_t j = 0; j < c; ++j) {
-                double sum = 0;
-                for (int64_t r = 0; r < n; ++r) sum += X[i * n + r] * X[j * n + r];
-                xtx[j * c + i] = sum;
+                xc[(size_t)i * c + j] = (int32_t)j;
+                xv[(size_t)i * c + j] = X[j * n + i];
             }
-        // RHS1 = vec(X' Y Ve^-1'), RHS2 = vec(Y Ve^-1') (trait-major)
-        for (int b = 0; b < T; ++b)
-            for (int64_t r = 0; r < n; ++r) {
-                double yv = 0;
-                for (int a = 0; a < T; ++a) yv += Y[(size_t)a * n + r] * vei[b * T + a];
-                rhs[ct + (size_t)b * n + r] = yv;
-                for (int64_t i = 0; i < c; ++i) rhs[(size_t)b * c + i] += X[i * n + r] * yv;
-            }
-        DevBuf<double> L, dX, dxtx, dvi, b;
-        DevBuf<int> info;
-        info.alloc(1);
-        L.alloc((size_t)dim * dim);
-        dX.alloc(std::max<int64_t>(1, n * c));
-        dxtx.alloc(std::max<int64_t>(1, c * c));
-        dvi.alloc(2 * T * T);
-        b.alloc(dim);
-        if (c > 0) {
-            copy_host(st, dX.p, X, (size_t)n * c * 8, hipMemcpyHostToDevice);
-            copy_host(st, dxtx.p, xtx.data(), (size_t)c * c * 8, hipMemcpyHostToDevice);
         }
-        std::vector<double> vv(vei);
-        vv.insert(vv.end(), vgi.begin(), vgi.end());
-        copy_host(st, dvi.p, vv.data(), vv.size() * 8, hipMemcpyHostToDevice);
-        copy_host(st, b.p, rhs.data(), dim * 8, hipMemcpyHostToDevice);
-        k_mv_lhs<<<dim3((unsigned)ceil_div(dim, 256), (unsigned)dim), 256, 0, st>>>(dxtx.p, dX.p, dGinv, ldg, dvi.p, n,
-                                                                                    c, T, L.p, dim);
-        CB_HIP(hipGetLastError());
-        BlasHandle h(st);
-        CB_BLAS(h, rocsolver_dpotrf(h.h, rocblas_fill_lower, (rocblas_int)dim, L.p, (rocblas_int)dim, info.p), "dpotrf");
-        int hinfo = 0;
-        copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
-        CB_HIP(hipStreamSynchronize(st));
-        CB_REQUIRE(hinfo == 0, "cb_mv_solve: the multi-trait MME is not positive definite");
-        CB_BLAS(h, rocsolver_dpotrs(h.h, rocblas_fill_lower, (rocblas_int)dim, 1, L.p, (rocblas_int)dim, b.p,
-                                (rocblas_int)dim), "dpotrs");
-        std::vector<double> sol(dim);
-        copy_host(st, sol.data(), b.p, dim * 8, hipMemcpyDeviceToHost);
-        CB_HIP(hipStreamSynchronize(st));
-        if (c > 0) std::copy(sol.begin(), sol.begin() + ct, beta);  // c x t column-major
-        std::copy(sol.begin() + ct, sol.end(), u);                  // n x t column-major
+        xp[n] = n * c;
+        cb_mme* raw = nullptr;
+        if (cb_mme_create(ctx, n, c, n, xp.data(), xc.data(), xv.data(), aid.data(), y0.data(), &raw) != CB_OK)
+            throw Error(CB_ERR_DEVICE, std::string("cb_mv_solve: ") + cb_last_error());
+        struct Own {
+            cb_mme* m;
+            ~Own() { cb_mme_destroy(m); }
+        } own{raw};
+        if (cb_mme_set_qinv_dense_device(raw, dGinv, ldg) != CB_OK)
+            throw Error(CB_ERR_DEVICE, std::string("cb_mv_solve: ") + cb_last_error());
+        // transformed traits Y* = Y T, each solved with lambda_a = 1 / d_a
+        const int64_t dim = c + n;
+        std::vector<double> ys((size_t)n), rhs(dim), sol((size_t)dim * T);
+        cb_solve_options so;
+        cb_solve_options_default(&so);
+        so.precision = CB_PREC_FP64;
+        so.precond = CB_PRECOND_BLOCK_JACOBI;
+        so.stop_rule = CB_STOP_RELATIVE;
+        so.tol = MV_SOLVE_TOL;
+        so.max_iter = 20000;
+        for (int a = 0; a < T; ++a) {
+            for (int64_t i = 0; i < n; ++i) {
+                double v = 0;
+                for (int b = 0; b < T; ++b) v += Y[(size_t)b * n + i] * cz.T[b * T + a];
+                ys[i] = v;
+            }
+            for (int64_t j = 0; j < c; ++j) {
+                double v = 0;
+                for (int64_t i = 0; i < n; ++i) v += X[j * n + i] * ys[i];
+                rhs[j] = v;
+            }
+            std::copy(ys.begin(), ys.end(), rhs.begin() + c);
+            int64_t its = 0;
+            double rel = 0;
+            if (cb_mme_solve(raw, 1.0 / cz.d[a], &so, 1, rhs.data(), sol.data() + (size_t)a * dim, &its, &rel) != CB_OK)
+                throw Error(CB_ERR_NOT_CONVERGED, std::string("cb_mv_solve: ") + cb_last_error());
+        }
+        // back to the original traits: [beta; u] = [beta*; u*] T^-1
+        for (int b = 0; b < T; ++b) {
+            for (int64_t r = 0; r < dim; ++r) {
+                double v = 0;
+                for (int a = 0; a < T; ++a) v += sol[(size_t)a * dim + r] * cz.Tinv[a * T + b];
+                if (r < c)
+                    beta[(size_t)b * c + r] = v;  // c x t column-major
+                else
+                    u[(size_t)b * n + (r - c)] = v;  // n x t column-major
+            }
+        }
     });
 }
 

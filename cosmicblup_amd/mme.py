@@ -20,7 +20,9 @@ from .design import FixedDesignG
 
 def solve_options(precision=_lib.CB_PREC_MIXED, precond=_lib.CB_PRECOND_BLOCK_JACOBI,
                   stop_rule=_lib.CB_STOP_RELATIVE, tol=1e-10, max_iter=10000,
-                  batch_rhs=True, stall_tol=1e-8) -> _lib.SolveOptions:
+                  batch_rhs=True, stall_tol=0.0) -> _lib.SolveOptions:
+    """stall_tol 0 (the library default): a mixed-precision solve that stalls
+    at the FP64 rounding floor above tol raises NotConverged."""
     o = _lib.SolveOptions()
     _lib.lib().cb_solve_options_default(C.byref(o))
     o.precision, o.precond, o.stop_rule = precision, precond, stop_rule
@@ -31,7 +33,9 @@ def solve_options(precision=_lib.CB_PREC_MIXED, precond=_lib.CB_PRECOND_BLOCK_JA
 
 def reml_options(algorithm=_lib.CB_VCE_AI, max_iter=20, tol=1e-4, n_probes=64, seed=42,
                  trace_exact=False, solve: _lib.SolveOptions | None = None,
-                 block_cols: int = 0) -> _lib.RemlOptions:
+                 block_cols: int = 0, stall_tol: float = 1e-8) -> _lib.RemlOptions:
+    """stall_tol (default 1e-8, as cb_reml_options_default) is applied to the
+    REML solves, also to a `solve` given here."""
     o = _lib.RemlOptions()
     _lib.lib().cb_reml_options_default(C.byref(o))
     o.algorithm, o.max_iter, o.tol, o.n_probes, o.seed = algorithm, max_iter, tol, n_probes, seed
@@ -39,6 +43,7 @@ def reml_options(algorithm=_lib.CB_VCE_AI, max_iter=20, tol=1e-4, n_probes=64, s
     o.block_cols = block_cols
     if solve is not None:
         o.solve = solve
+    o.solve.stall_tol = stall_tol
     return o
 
 

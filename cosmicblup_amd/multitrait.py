@@ -72,7 +72,9 @@ class MvEigenVce:
 
 
 def solveMultiTrait(Y, X, Ginv, Vg, Ve, ctx: _lib.Context | None = None):
-    """Returns (beta c x t, u n x t)."""
+    """Solution of the Kronecker MME of GSSolve::solveMultiTrait
+    (mixed_model_solver.cpp:425-524) by the canonical transformation and t
+    single-trait PCG solves on the device. Returns (beta c x t, u n x t)."""
     import torch
 
     ctx = ctx or _lib.default_context()

@@ -202,10 +202,14 @@ typedef struct {
     double tol;         /* default 1e-10 (config.h:41) */
     int64_t max_iter;   /* default 10000 (config.h:42) */
     double stall_tol;   /* MIXED: the stop test is on the true FP64 residual b - C x,
-                           which has a rounding floor near cond(C) * eps64; a column
+                           which has a rounding floor near cond(C) * eps64. A column
                            whose best true residual has not halved over eight
-                           residual replacements is accepted when it is below
-                           stall_tol. Default 1e-8. (The reference's 1e-10 tests the
+                           residual replacements has stalled there: it is accepted
+                           when its residual is below stall_tol (the call then
+                           returns CB_OK with *rel_residual above tol), else the
+                           solve returns CB_ERR_NOT_CONVERGED at once. Default 0
+                           (no stall accepted); cb_reml_options_default sets 1e-8
+                           for the REML solves. (The reference's tol tests the
                            recursive residual, pcg_solver.h:501.) */
 } cb_solve_options;
 
@@ -325,8 +329,13 @@ void cb_mv_options_default(cb_mv_options* opts);
 int cb_mv_vce(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* Y, const double* X, const double* dG,
               int64_t ldg, const cb_mv_options* opts, double* Vg, double* Ve, int64_t* iterations, int* converged);
 /* GSSolve::solveMultiTrait (src/solve/mixed_model_solver.cpp:425-524): the
- * dense Kronecker MME solved by Cholesky on the device ((n + c) t <= 60000).
- * dGinv: n x n G^-1 on the device; beta (c x t) and u (n x t) column-major. */
+ * solution of the Kronecker MME [Ve^-1 (x) X'X, Ve^-1 (x) X'; .,
+ * Ve^-1 (x) I + Vg^-1 (x) G^-1] (the reference factorises it by dense
+ * LDLT). Here: canonical transformation of the t traits (complete records,
+ * one design), t single-trait FP64 PCG solves with lambda_a = 1/d_a on the
+ * device to relative residual 1e-12, transformed back. Vg and Ve (t x t,
+ * t <= 4) must be positive definite. dGinv: n x n G^-1 on the device;
+ * beta (c x t) and u (n x t) column-major. */
 int cb_mv_solve(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* Y, const double* X,
                 const double* dGinv, int64_t ldg, const double* Vg, const double* Ve, double* beta, double* u);
 

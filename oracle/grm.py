@@ -35,6 +35,7 @@ def lib():
         L.orc_int_gram.argtypes = [vp, i64, i64, i64, vp, vp]
         L.orc_grm_vanraden.argtypes = [vp, i64, i64, i64, i64, C.c_float, vp, vp, vp]
         L.orc_grm_block_raw.argtypes = [vp, i64, i64, i64, vp, vp, i64, i64, i64, i64, i64, vp]
+        L.orc_grm_rows_raw.argtypes = [vp, i64, i64, i64, vp, vp, vp, i64, i64, i64, i64, vp]
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -50,8 +51,12 @@ def _rows(bed: np.ndarray) -> np.ndarray:
 
 
 def snp_stats(bed: np.ndarray, n: int, m: int, min_maf: float = 0.01) -> dict[str, np.ndarray]:
-    rows = _rows(bed)
-    bps = (n + 3) // 4
+    return snp_stats_rows(_rows(bed), (n + 3) // 4, n, m, min_maf)
+
+
+def snp_stats_rows(rows: np.ndarray, bps: int, n: int, m: int, min_maf: float = 0.01) -> dict[str, np.ndarray]:
+    """snp_stats on SNP-major rows of `bps` bytes each (bps >= ceil(n/4))."""
+    rows = np.ascontiguousarray(rows, dtype=np.uint8)
     out = {k: np.zeros(m, dtype=np.int64) for k in ("n_obs", "sum_x", "n_hom1", "n_het", "n_hom2")}
     out["valid"] = np.zeros(m, dtype=np.uint8)
     for k in ("freq_f", "mean_f", "twopq_f"):
@@ -88,6 +93,22 @@ def grm_block_raw(bed: np.ndarray, n: int, m: int, stats: dict, r0: int, nr: int
     lib().orc_grm_block_raw(_p(rows), n, m, (n + 3) // 4, _p(np.ascontiguousarray(stats["valid"])),
                             _p(np.ascontiguousarray(stats["mean_f"])), r0, nr, c0, nc, block_size,
                             _p(out))
+    return out
+
+
+def grm_rows_raw(rows: np.ndarray, nbytes: int, n: int, m: int, stats: dict, sel, c0: int = 0,
+                 nc: int | None = None, block_size: int = 2048) -> np.ndarray:
+    """Unnormalised Z Z^T of the listed samples `sel` against columns
+    [c0, c0+nc), from SNP-major rows (m x nbytes uint8, nbytes >= ceil(n/4),
+    e.g. a device BED copied to the host)."""
+    nc = n - c0 if nc is None else nc
+    rows = np.ascontiguousarray(rows, dtype=np.uint8)
+    assert rows.shape[0] == m and rows.shape[1] == nbytes and nbytes >= (n + 3) // 4
+    sel = np.ascontiguousarray(sel, dtype=np.int64)
+    out = np.zeros((sel.size, nc), dtype=np.float64)
+    lib().orc_grm_rows_raw(_p(rows), n, m, nbytes, _p(np.ascontiguousarray(stats["valid"])),
+                           _p(np.ascontiguousarray(stats["mean_f"])), _p(sel), sel.size, c0, nc, block_size,
+                           _p(out))
     return out
 
 

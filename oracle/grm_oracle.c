@@ -15,6 +15,11 @@
  *   orc_grm_vanraden   pinned: brute-force numpy Z Z^T / sum(2pq); invariant
  *                      G 1 ~ 0 without missing calls; the identity
  *                      G = (XX^T - r1^T - 1r^T + c11^T)/S.
+ *   orc_grm_block_raw,
+ *   orc_grm_rows_raw   pinned: sub-blocks and row lists of S * G from
+ *                      orc_grm_vanraden, whose pins are above, and a brute-
+ *                      force numpy Z Z^T with missing calls, bit for bit on
+ *                      the same block order (tests/test_oracle_grm.py).
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -203,6 +208,54 @@ void orc_grm_block_raw(const uint8_t *bed, int64_t n, int64_t m, int64_t bytes_p
         }
     }
     free(Zr); free(Zc);
+}
+
+/*
+ * The same unnormalised Z Z^T sum for an explicit list of rows against the
+ * column range [c0, c0+nc): out[a*nc + b] = sum_j Z[rows[a], j] Z[c0+b, j],
+ * SNP blocks in the reference's order. The column panel of each SNP block
+ * is filled in parallel, so a few rows of a 50,000-sample G take seconds.
+ */
+void orc_grm_rows_raw(const uint8_t *bed, int64_t n, int64_t m, int64_t bytes_per_snp,
+                      const uint8_t *valid, const float *mean_f, const int64_t *rows, int64_t nr,
+                      int64_t c0, int64_t nc, int64_t block_size, double *out) {
+    (void)n;
+    memset(out, 0, sizeof(double) * (size_t)nr * (size_t)nc);
+    double *Zr = (double *)malloc(sizeof(double) * (size_t)nr * (size_t)block_size);
+    double *Zc = (double *)malloc(sizeof(double) * (size_t)nc * (size_t)block_size);
+    int64_t *cols = (int64_t *)malloc(sizeof(int64_t) * (size_t)block_size);
+    for (int64_t start = 0; start < m; start += block_size) {
+        int64_t end = start + block_size < m ? start + block_size : m;
+        int64_t nv = 0;
+        for (int64_t j = start; j < end; ++j)
+            if (valid[j]) cols[nv++] = j;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nc; ++i)
+            for (int64_t c = 0; c < nv; ++c) {
+                const int64_t j = cols[c];
+                const double mu = (double)mean_f[j];
+                const double map[4] = {2.0 - mu, 0.0, 1.0 - mu, 0.0 - mu};
+                Zc[i * block_size + c] = map[code_at(bed + j * bytes_per_snp, c0 + i)];
+            }
+        for (int64_t a = 0; a < nr; ++a)
+            for (int64_t c = 0; c < nv; ++c) {
+                const int64_t j = cols[c];
+                const double mu = (double)mean_f[j];
+                const double map[4] = {2.0 - mu, 0.0, 1.0 - mu, 0.0 - mu};
+                Zr[a * block_size + c] = map[code_at(bed + j * bytes_per_snp, rows[a])];
+            }
+#pragma omp parallel for schedule(static)
+        for (int64_t b = 0; b < nc; ++b) {
+            const double *zb = Zc + b * block_size;
+            for (int64_t a = 0; a < nr; ++a) {
+                const double *za = Zr + a * block_size;
+                double s = 0.0;
+                for (int64_t c = 0; c < nv; ++c) s += za[c] * zb[c];
+                out[a * nc + b] += s;
+            }
+        }
+    }
+    free(Zr); free(Zc); free(cols);
 }
 
 int orc_num_threads(void) {

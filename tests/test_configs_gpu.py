@@ -1,0 +1,119 @@
+"""The configurations at their real sizes, checked against the oracle.
+
+  GRM at scale: n = m = 50,000, MAF U(0.01, 0.5), 1% MCAR missing calls.
+  Index ranges past n^2 = 2^31 (n >= 46,341) are reached only at this size.
+  The device G is checked for exact symmetry, its integer Gram diagonal
+  against sum_j x_ij^2 over the kept SNPs (exact), SNP statistics bit for
+  bit, and rows straddling 2^31 against the oracle's FP64 Z Z^T / sum(2pq).
+
+  Plain GBLUP with dense-exact AI-REML: n = 2,000, m = 10,000, MAF
+  U(0.05, 0.5), 20 contemporary groups, h2 = 0.3, from (1, 1) to
+  sum |d sigma^2| < 1e-4, against the oracle's run on the same G^-1.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from cosmicblup_amd import _lib, design, mme, synth
+from oracle import grm as og
+from oracle import reml as orl
+
+pytestmark = pytest.mark.gpu
+
+TOL_G64 = 1e-12
+
+
+def _hip():
+    return C.CDLL("libamdhip64.so")
+
+
+def _device_diag_i32(ptr: int, n: int, ld: int) -> np.ndarray:
+    """Diagonal of an n x n int32 device matrix (leading dimension ld)."""
+    out = np.empty(n, dtype=np.int32)
+    hip = _hip()
+    hip.hipMemcpy2D.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int]
+    rc = hip.hipMemcpy2D(out.ctypes.data, 4, ptr, (ld + 1) * 4, 4, n, 2)  # hipMemcpyDeviceToHost
+    assert rc == 0, f"hipMemcpy2D failed ({rc})"
+    return out
+
+
+@pytest.mark.timeout(900)
+def test_grm_n50000_m50000_missing_1pct(cb):
+    import torch
+
+    n = m = 50000
+    print(f"[grm-full] generating BED n={n} m={m}", flush=True)
+    rows, pitch = synth.bed_on_device(n, m, maf=(0.01, 0.5), missing_rate=0.01, seed=20262203)
+    ctx = _lib.default_context()
+    reader = cb.PlinkReader.from_device_rows(ctx, rows.data_ptr(), pitch, n, m)
+    gb = cb.GrmBuilder(cb.Options(), ctx=ctx)
+    gb.compute(reader, outputs=cb.CB_GRM_OUT_F64 | cb.CB_GRM_OUT_INT)
+    print("[grm-full] computed", flush=True)
+    host_rows = rows.cpu().numpy()
+    st = og.snp_stats_rows(host_rows, pitch, n, m)
+    gs = gb.getSnpStats()
+    for k_dev, k_ref in (("n_obs", "n_obs"), ("sum_x", "sum_x"), ("n_hom1", "n_hom1"), ("n_het", "n_het"),
+                         ("n_hom2", "n_hom2"), ("valid", "valid"), ("mean_f", "mean_f"), ("twopq_f", "twopq_f")):
+        assert np.array_equal(gs[k_dev], st[k_ref]), k_dev
+    valid = torch.from_numpy(st["valid"].astype(bool)).cuda()
+    # integer Gram diagonal = sum over kept SNPs of x^2 (codes 00 -> 4, 10 -> 1, else 0)
+    sq = torch.tensor([4, 0, 1, 0], dtype=torch.int32, device="cuda")
+    kept = rows[valid]
+    diag = torch.zeros(4 * pitch, dtype=torch.int64, device="cuda")
+    for t in range(4):
+        diag[t::4] = sq[((kept >> (2 * t)) & 3).long()].sum(0, dtype=torch.int64)
+    del kept
+    iptr, ild = gb.devicePtr(cb.CB_GRM_OUT_INT)
+    torch.cuda.synchronize()
+    assert np.array_equal(_device_diag_i32(iptr, n, ild).astype(np.int64), diag[:n].cpu().numpy())
+    # FP64 G: exact symmetry in row panels, then rows against the oracle
+    G = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    gb.copyGrmToDevice(G.data_ptr(), n)
+    for i0 in range(0, n, 5000):
+        assert torch.equal(G[i0:i0 + 5000], G[:, i0:i0 + 5000].T), i0
+    print("[grm-full] symmetric", flush=True)
+    sel = np.array([0, 1, 25000, 46340, 46341, 49999])
+    got = G[torch.from_numpy(sel).cuda()].cpu().numpy()
+    del G
+    # sum(2pq): double sums of float 2pq per 2048-SNP block, then across blocks
+    S_ref = 0.0
+    for b0 in range(0, m, 2048):
+        blk = 0.0
+        for j in range(b0, min(m, b0 + 2048)):
+            if st["valid"][j]:
+                blk += float(st["twopq_f"][j])
+        S_ref += blk
+    S = gb.getSum2pq()
+    assert S == S_ref
+    ref = og.grm_rows_raw(host_rows, pitch, n, m, st, sel) / S
+    err = np.abs(got - ref).max()
+    print(f"[grm-full] rows {list(sel)}: max |dG| = {err:.3e}", flush=True)
+    assert err < TOL_G64
+
+
+def c1_problem(n=2000, m=10000, ngroups=20, h2=0.3, seed=20261203):
+    x, _ = synth.simulate_dosages(n, m, maf=(0.05, 0.5), seed=seed)
+    bed = synth.encode_bed(x)
+    G, _, _ = og.grm_vanraden(bed, n, m)
+    Q = og.invert_psd_matrix(G)
+    y, cg = synth.simulate_phenotype(x, n_qtl=500, h2=h2, n_groups=ngroups, seed=seed + 1)
+    fd = design.build_fixed_design([[c] for c in cg])
+    return fd, np.arange(n, dtype=np.int32), y, Q
+
+
+@pytest.mark.timeout(900)
+def test_c1_dense_exact_reml_n2000_matches_oracle(cb, ctx):
+    fd, aid, y, Q = c1_problem()
+    M = mme.MixedModel(fd, aid, y, Q.shape[0])
+    M.set_qinv_dense(Q)
+    R = mme.AIREML(M, mme.reml_options(algorithm=_lib.CB_VCE_AI_EXACT, max_iter=20, tol=1e-4))
+    R.solve()
+    o = orl.dense_exact_run(design.dense_x(fd), aid, y, Q, max_iter=20, tol=1e-4)
+    print(f"[c1] device {R.iterations} iterations vu={R.var_u:.8f} ve={R.var_e:.8f}; "
+          f"oracle {len(o['hist'])} vu={o['vu']:.8f} ve={o['ve']:.8f}", flush=True)
+    assert R.converged and o["converged"] and R.iterations == len(o["hist"])
+    assert abs(R.var_u - o["vu"]) / o["vu"] < 1e-6 and abs(R.var_e - o["ve"]) / o["ve"] < 1e-6
+    u, ur = R.solution[M.p:], o["sol"][M.p:]
+    assert np.abs(u - ur).max() / np.abs(ur).max() < 1e-6

@@ -47,6 +47,56 @@ def test_mv_solve_matches_oracle(cb, ctx):
     np.testing.assert_allclose(u, ur, atol=1e-9 * np.abs(ur).max())
 
 
+def bent_problem(n=1200, m=4000, seed=11):
+    """A serving-workload-shaped 3-trait instance: centred VanRaden G (so its
+    eigen-bent inverse has the 1e6 eigenvalue of the floored centring
+    direction), 10 contemporary groups plus the mean, trait correlations
+    as the batched workload draws them, and Vg / Ve from the oracle's EM
+    stopped at its 500-iteration cap."""
+    from cosmicblup_amd import design
+
+    rng = np.random.default_rng(seed)
+    x, _ = synth.simulate_dosages(n, m, maf=(0.05, 0.5), seed=seed)
+    G, _, _ = og.grm_vanraden(synth.encode_bed(x), n, m)
+    Gi = og.invert_psd_matrix(G)
+    grp = rng.integers(0, 10, n)
+    X = design.dense_x(design.build_fixed_design([[f"cg{int(c)}"] for c in grp]))
+    Vg0 = np.array([[0.3, 0.1, -0.05], [0.1, 0.45, 0.2], [-0.05, 0.2, 0.2]])
+    Ve0 = np.array([[0.7, 0.12, 0.1], [0.12, 0.55, 0.13], [0.1, 0.13, 0.8]])
+    w, V = np.linalg.eigh(G)
+    Gh = (V * np.sqrt(np.maximum(w, 0))) @ V.T
+    U = Gh @ rng.standard_normal((n, 3)) @ np.linalg.cholesky(Vg0).T
+    E = rng.standard_normal((n, 3)) @ np.linalg.cholesky(Ve0).T
+    Y = X @ rng.standard_normal((X.shape[1], 3)) + U + E
+    Vg, Ve, _, _ = orl.mv_eigen_vce(Y, X, G, max_iter=500)
+    return Y, X, G, Gi, Vg, Ve
+
+
+def test_mv_solve_bent_ginv_matches_oracle(cb, ctx):
+    """solveMultiTrait on the eigen-bent G^-1 (largest eigenvalue 1e6) with
+    EM-estimated covariances: beta and u within 1e-6 (relative to their
+    largest entries) of the oracle's dense Kronecker solve, and the same bits
+    on a second call."""
+    Y, X, G, Gi, Vg, Ve = bent_problem()
+    assert np.linalg.eigvalsh(Gi).max() > 1e5
+    beta, u = multitrait.solveMultiTrait(Y, X, Gi, Vg, Ve)
+    br, ur = orl.mv_solve(Y, X, Gi, Vg, Ve)
+    assert np.abs(u - ur).max() <= 1e-6 * np.abs(ur).max()
+    assert np.abs(beta - br).max() <= 1e-6 * np.abs(br).max()
+    beta2, u2 = multitrait.solveMultiTrait(Y, X, Gi, Vg, Ve)
+    assert np.array_equal(u, u2) and np.array_equal(beta, beta2)
+
+
+def test_mv_solve_rejects_singular_vg(cb, ctx):
+    Y, X, G = mv_problem(n=120)
+    Vg = np.array([[1.0, 1.0, 0.0], [1.0, 1.0, 0.0], [0.0, 0.0, 0.5]])
+    Ve = np.eye(3)
+    from cosmicblup_amd import _lib
+
+    with pytest.raises(_lib.CosmicError, match="Vg is not positive definite"):
+        multitrait.solveMultiTrait(Y, X, np.linalg.inv(G), Vg, Ve)
+
+
 def test_contexts_on_threads_match_serial(cb, ctx):
     """Multi-trait estimation and solve on three contexts driven by three
     host threads give the serial results bit for bit. These calls allocate

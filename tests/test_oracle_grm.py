@@ -151,3 +151,33 @@ def test_invert_psd_identity_and_floor():
     assert one @ Gi @ one > 1e5
     Gb = og.bent_matrix(G)
     np.testing.assert_allclose(Gb @ Gi, np.eye(40), atol=1e-6)
+
+
+@pytest.mark.parametrize("miss", [0.0, 0.03])
+def test_block_and_row_oracles_match_the_full_grm(miss):
+    """The sub-block and row-list forms (used for single rows of a G too
+    large for the full oracle, and as the CPU baseline) against the full
+    orc_grm_vanraden times S and a brute-force numpy Z Z^T, including a
+    partial last SNP block and non-contiguous rows."""
+    n, m = 150, 4500
+    x, _ = synth.simulate_dosages(n, m, maf=(0.01, 0.5), missing_rate=miss, seed=31)
+    bed = synth.encode_bed(x)
+    st = og.snp_stats(bed, n, m)
+    G, S, _ = og.grm_vanraden(bed, n, m)
+    v = st["valid"].astype(bool)
+    mu = st["mean_f"].astype(np.float64)
+    Z = np.where(x >= 0, x - mu, 0.0)[:, v]
+    ZZ = Z @ Z.T
+    blk = og.grm_block_raw(bed, n, m, st, 17, 40, 60, 90)
+    np.testing.assert_allclose(blk, ZZ[17:57, 60:150], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(blk / S, G[17:57, 60:150], rtol=0, atol=1e-14)
+    sel = np.array([0, 1, 77, 149, 3])
+    rr = og.grm_rows_raw(bed[3:].reshape(m, -1), (n + 3) // 4, n, m, st, sel)
+    assert np.array_equal(rr, og.grm_block_raw(bed, n, m, st, 0, n, 0, n)[sel])
+    np.testing.assert_allclose(rr, ZZ[sel], rtol=0, atol=1e-9)
+    # rows padded to a wider pitch (device row layout) give the same sums
+    rows = np.zeros((m, 48), dtype=np.uint8)
+    rows[:, :(n + 3) // 4] = bed[3:].reshape(m, -1)
+    st2 = og.snp_stats_rows(rows, 48, n, m)
+    assert all(np.array_equal(st[k], st2[k]) for k in st)
+    assert np.array_equal(og.grm_rows_raw(rows, 48, n, m, st2, sel), rr)

@@ -216,3 +216,24 @@ def test_mv_solve_is_the_v_form_blup():
     uv = np.kron(Vg, G) @ Vi @ (y - Xt @ b)
     np.testing.assert_allclose(beta.T.ravel(), b, atol=1e-9)
     np.testing.assert_allclose(u.T.ravel(), uv, atol=1e-9)
+
+
+def test_logl_equals_v_form_reml_plus_log_det_a():
+    """calc_logL_internal's MME form against the V-form REML likelihood
+    -1/2 (log|V| + log|X'V^-1 X| + y'Py), V = vu Z A Z' + ve I, A = Q^-1:
+    the two differ by exactly 1/2 log|A| (the reference omits that
+    constant), at several variance pairs."""
+    X, aid, y, Q, _ = small_problem(n=90, seed=12)
+    # two records share a level and one record has none
+    aid = aid.copy()
+    aid[5], aid[6] = aid[4], -1
+    A = np.linalg.inv(Q)
+    Z = orl.design_w(np.zeros((X.shape[0], 0)), aid, Q.shape[0])
+    half_logdet_a = 0.5 * np.linalg.slogdet(A)[1]
+    for vu, ve in [(0.7, 1.3), (2.0, 0.4), (0.05, 3.0)]:
+        V = vu * Z @ A @ Z.T + ve * np.eye(X.shape[0])
+        Vi = np.linalg.inv(V)
+        XVX = X.T @ Vi @ X
+        P = Vi - Vi @ X @ np.linalg.solve(XVX, X.T @ Vi)
+        v_form = -0.5 * (np.linalg.slogdet(V)[1] + np.linalg.slogdet(XVX)[1] + y @ P @ y)
+        assert abs(orl.logL(X, aid, y, Q, vu, ve) - v_form - half_logdet_a) < 1e-8
