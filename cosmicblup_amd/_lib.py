@@ -54,7 +54,8 @@ class MvOptions(C.Structure):
 class RemlOptions(C.Structure):
     _fields_ = [("algorithm", C.c_int), ("max_iter", C.c_int64), ("tol", C.c_double),
                 ("n_probes", C.c_int64), ("seed", C.c_uint64), ("trace_exact", C.c_int),
-                ("solve", SolveOptions), ("block_cols", C.c_int64)]
+                ("solve", SolveOptions), ("block_cols", C.c_int64), ("mc_samples", C.c_int64),
+                ("adaptive", C.c_int)]
 
 
 CB_PREC_FP64 = 0
@@ -67,6 +68,7 @@ CB_STOP_ABSOLUTE = 1
 CB_VCE_AI = 0
 CB_VCE_EM = 1
 CB_VCE_AI_EXACT = 2
+CB_VCE_AUTO = 3
 
 
 class CosmicError(RuntimeError):
@@ -134,6 +136,8 @@ _SIGS = {
     "cb_reml_logl": (C.c_int, [c_vp, C.c_double, C.c_double, c_dp, c_dp]),
     "cb_reml_score": (C.c_int, [c_vp, C.POINTER(RemlOptions), C.c_double, C.c_double, c_vp, c_vp, c_dp, c_vp]),
     "cb_reml_copy_probes": (C.c_int, [c_vp, C.POINTER(RemlOptions), c_vp, c_i64]),
+    "cb_reml_branch": (C.c_int, [c_vp, C.POINTER(RemlOptions), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_int), c_i64p, c_i64p]),
     "cb_invert_psd": (C.c_int, [c_vp, c_vp, c_i64, c_i64, C.c_double, c_vp, c_i64, c_dp, c_dp]),
     "cb_invert_psd_device": (C.c_int, [c_vp, c_vp, c_i64, c_i64, C.c_double, c_vp, c_i64, c_dp, c_dp]),
     "cb_mme_set_qinv_split": (C.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_i64]),

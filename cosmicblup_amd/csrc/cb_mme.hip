@@ -2105,6 +2105,25 @@ struct cb_reml_work {
     int k = 0, bw = 0;
     bool chunked = false;
     bool have_s = false, have_s2 = false;  // S (last block) and S2 hold solutions
+    // adaptive traces: the k = max_k probe columns are solved in sub-blocks
+    // (y + probes up to the first check point, then batches of 16) until the
+    // stopping rule holds; sub_warm[b]: sub-block b holds solutions
+    bool adaptive = false;
+    bool exact = false;  // probes are the q unit vectors
+    int64_t min_k = 0, max_k = 0;
+    std::vector<char> sub_warm;
+    int64_t used_k = 0;  // probes the last evaluation used
+};
+
+// The branch of one REML run: the step of iteration 0 and of later
+// iterations (CB_VCE_AI_EXACT / AI / EM), the trace rule (0 fixed n_probes,
+// 1 adaptive, 2 exact by unit probes), whether AI steps need a logL increase,
+// and the adaptive rule's probe bounds.
+struct RemlPlan {
+    int first = CB_VCE_AI, later = CB_VCE_AI;
+    int trace = 0;
+    bool logl = false;
+    int64_t min_k = 0, max_k = 0;
 };
 
 namespace {
@@ -2129,7 +2148,7 @@ void make_probes(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, int64_t f
     const int64_t ld = M->ldv, p = M->p, q = M->q;
     CB_HIP(hipMemsetAsync(w.B.p + (size_t)dst * ld, 0, (size_t)count * ld * 8, st));
     k_probes<<<dim3(blocks(q), (unsigned)count), 256, 0, st>>>(w.B.p + (size_t)dst * ld, ld, p, q, count, o.seed,
-                                                              o.trace_exact, first);
+                                                              w.exact ? 1 : 0, first);
     CB_HIP(hipGetLastError());
     if (with_qz) {
         CB_HIP(hipMemsetAsync(w.Qz.p, 0, (size_t)count * ld * 8, st));
@@ -2138,6 +2157,82 @@ void make_probes(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, int64_t f
 }
 
 }  // namespace
+
+// AdaptiveTraceEstimator bounds for n_samples ns (vce.cpp:1312-1314)
+static void adaptive_bounds(int64_t ns, RemlPlan& pl) {
+    pl.min_k = std::max<int64_t>(30, ns / 3);
+    pl.max_k = std::max<int64_t>(100, ns);
+}
+
+// CB_VCE_AUTO: the branch AI_REML takes for a single random component
+// (run_ai_iteration, vce.cpp:1334-1574), from the solvers it would have:
+// Ainv factorised for an in-memory sparse Q with q <= 100000 (and not
+// q >= 50000 with density > 0.001, vce.cpp:877-1001); build_lhs's dense
+// direct solver for dim < 10000 or LHS density > 0.05 with dim < 15000, else
+// its sparse direct solver for dim < 200000 and density < 0.05
+// (vce.cpp:1041-1110). The LHS density is estimated from nnz(W'W) plus the
+// nonzeros of Q (q^2 dense; CSR; CSR + ng^2 split).
+static RemlPlan reml_plan(const cb_mme* M, const cb_reml_options& o) {
+    RemlPlan pl;
+    const int64_t mc = o.mc_samples > 0 ? o.mc_samples : 100;
+    if (o.algorithm != CB_VCE_AUTO) {
+        pl.first = pl.later = o.algorithm;
+        pl.trace = o.trace_exact ? 2 : (o.adaptive ? 1 : 0);
+        adaptive_bounds(std::max<int64_t>(5, mc / 2), pl);
+        return pl;
+    }
+    const int64_t dim = M->dim, q = M->q;
+    if (dim < 5000) {  // run_dense_exact_step (vce.cpp:1397-1422)
+        pl.first = pl.later = CB_VCE_AI_EXACT;
+        return pl;
+    }
+    const double ww_nnz = M->wc.n ? (double)M->wc.n : 0.0;
+    double q_nnz = 0;
+    if (M->qkind == 0) q_nnz = (double)q * (double)q;
+    else q_nnz = (double)M->qc.n + (M->qkind == 2 ? (double)M->ng * (double)M->ng : 0.0);
+    const double density = (ww_nnz + q_nnz) / ((double)dim * (double)dim);
+    const bool dense_direct = dim < 10000 || (density > 0.05 && dim < 15000);
+    const bool sparse_direct = !dense_direct && dim < 200000 && density < 0.05;
+    const double q_density = M->qkind == 1 ? (double)M->qc.n / ((double)q * (double)q) : 1.0;
+    const bool ainv_fact = M->qkind == 1 && q <= 100000 && !(q >= 50000 && q_density > 0.001);
+    if (dense_direct || sparse_direct) {
+        // iteration 0: AI needs Ainv factorised or a solver, which build_lhs
+        // creates only inside the step; EM otherwise. Later: the AI step
+        // with its logL monotonicity check.
+        pl.first = ainv_fact ? CB_VCE_AI : CB_VCE_EM;
+        pl.later = CB_VCE_AI;
+        pl.logl = dim <= 65536;
+        if (q <= 5000) {
+            pl.trace = 2;  // estimate_trace_Qinv_Cuu's direct branch (vce.cpp:1278-1302)
+        } else {
+            pl.trace = 1;
+            adaptive_bounds(std::max<int64_t>(50, mc), pl);
+        }
+        return pl;
+    }
+    // no direct solver: the AI step cannot check logL and falls back to EM
+    // every iteration (vce.cpp:2788-2795, 1441-1454)
+    pl.first = pl.later = CB_VCE_EM;
+    pl.trace = 1;
+    adaptive_bounds(std::max<int64_t>(5, mc / 2), pl);
+    return pl;
+}
+
+// The adaptive rule of AdaptiveTraceEstimator::estimate
+// (adaptive_trace_estimator.cpp:11-83) over per-probe values t[0..done):
+// checked at each batch end once done >= min_k.
+static bool adaptive_stop(const std::vector<double>& t, int64_t done, int64_t min_k) {
+    if (done < min_k || done < 2) return false;
+    double sum = 0, sq = 0;
+    for (int64_t i = 0; i < done; ++i) {
+        sum += t[i];
+        sq += t[i] * t[i];
+    }
+    const double mean = sum / (double)done;
+    const double var = (sq - (sum * sum) / (double)done) / ((double)done - 1.0);
+    const double se = std::sqrt(std::max(0.0, var) / (double)done);
+    return std::fabs(mean) > 1e-12 && se / std::fabs(mean) < 0.05;
+}
 
 // One evaluation at (vu, ve): solves, score terms, optionally the AI matrix.
 static void reml_eval(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, double vu, double ve,
@@ -2158,7 +2253,54 @@ static void reml_eval(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, doub
     }
     std::vector<double> traces(k), hd(2 + bw);
     DevBuf<double>& Qu = w.Qu;
-    for (size_t ci = 0; ci < chunks.size(); ++ci) {
+    if (w.adaptive) {
+        // sub-block 0: y and the probes up to the first check point (the rule
+        // is not checked before min_k); then batches of 16
+        int64_t done = 0;
+        int sub = 0;
+        bool stop = false;
+        while (!stop && done < w.max_k) {
+            int64_t end = done == 0 ? std::min<int64_t>(w.max_k, round_up(w.min_k, 16))
+                                    : std::min<int64_t>(w.max_k, done + 16);
+            const bool with_y = done == 0;
+            const int64_t off = with_y ? 0 : 1 + done;  // first column of the sub-block
+            const int cnt = (int)(end - done) + (with_y ? 1 : 0);
+            if ((int)w.sub_warm.size() <= sub) w.sub_warm.resize(sub + 1, 0);
+            solve_block(M, lambda, o.solve, w.B.p + (size_t)off * ld, w.S.p + (size_t)off * ld, cnt,
+                        w.sub_warm[sub] != 0);
+            w.sub_warm[sub] = 1;
+            if (with_y) {
+                CB_HIP(hipMemsetAsync(Qu.p, 0, ld * 8, st));
+                q_apply<double>(M, w.S.p, ld, Qu.p, ld, 1, 1.0);
+            }
+            const int ncol = (with_y ? 2 : 0) + (int)(end - done);
+            k_reml_score_terms<<<dim3(DOT_CHUNKS, (unsigned)ncol), 256, 0, st>>>(
+                M->xp.p, M->xc.p, M->xv.p, M->aid.p, M->y.p, n, p, q, w.S.p + (size_t)off * ld, ld, Qu.p,
+                w.Qz.p + (size_t)done * ld, with_y ? 1 : 0, w.e.p, w.part.p);
+            k_dots_finish<<<blocks(ncol, 64), 64, 0, st>>>(w.part.p, ncol, w.dots.p);
+            CB_HIP(hipGetLastError());
+            copy_host(st, hd.data(), w.dots.p, ncol * 8, hipMemcpyDeviceToHost);
+            CB_HIP(hipStreamSynchronize(st));
+            const double* th = hd.data() + (with_y ? 2 : 0);
+            for (int64_t j = done; j < end; ++j) traces[j] = th[j - done];
+            if (with_y) {
+                out.esse = hd[0];
+                out.uQu = hd[1];
+            }
+            done = end;
+            ++sub;
+            stop = adaptive_stop(traces, done, w.min_k);
+        }
+        w.used_k = done;
+        w.have_s = true;
+        double tr = 0;
+        for (int64_t i = 0; i < done; ++i) tr += traces[i];
+        tr /= (double)done;
+        out.vu = vu;
+        out.ve = ve;
+        out.tr = tr;
+    }
+    for (size_t ci = 0; ci < (w.adaptive ? 0 : chunks.size()); ++ci) {
         const bool with_y = ci + 1 == chunks.size();
         const int64_t first = chunks[ci].first;
         const int count = chunks[ci].second;
@@ -2189,13 +2331,17 @@ static void reml_eval(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, doub
             out.uQu = hd[1];
         }
     }
-    w.have_s = true;
-    double tr = 0;
-    for (int i = 0; i < k; ++i) tr += traces[i];
-    if (!o.trace_exact && k > 0) tr /= k;
-    out.vu = vu;
-    out.ve = ve;
-    out.tr = tr;
+    if (!w.adaptive) {
+        w.have_s = true;
+        double tr = 0;
+        for (int i = 0; i < k; ++i) tr += traces[i];
+        if (!w.exact && k > 0) tr /= k;
+        out.vu = vu;
+        out.ve = ve;
+        out.tr = tr;
+        w.used_k = k;
+    }
+    const double tr = out.tr;
     // score, vce.cpp:2655 and :2661-2675 (lambda*tr capped below q)
     double lambda_tr = lambda * tr;
     if (lambda_tr > q - 1e-6) lambda_tr = q - 1e-6;
@@ -2228,15 +2374,21 @@ static void reml_eval(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, doub
     out.ai[3] = 0.5 * a_ee;
 }
 
-static void reml_setup(cb_mme* M, const cb_reml_options& o, cb_reml_work& w) {
+static void reml_setup(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, const RemlPlan& pl) {
     cb_context* ctx = M->ctx;
     hipStream_t st = ctx->stream;
     const int64_t ld = M->ldv, n = M->n;
     CB_REQUIRE(o.block_cols >= 0, "REML: block_cols must be >= 0");
-    w.k = (int)(o.trace_exact ? M->q : o.n_probes);
+    w.adaptive = pl.trace == 1;
+    w.exact = pl.trace == 2;
+    w.min_k = pl.min_k;
+    w.max_k = pl.max_k;
+    w.k = (int)(pl.trace == 2 ? M->q : pl.trace == 1 ? pl.max_k : o.n_probes);
     const int r = 1 + w.k;
     w.bw = reml_block_width(M, o, r);
     w.chunked = w.bw < r;
+    CB_REQUIRE(!(w.adaptive && w.chunked),
+               "REML: adaptive traces need y and max(100, ns) probe columns in one block (raise block_cols)");
     const int bw = w.bw;
     w.B.alloc((size_t)ld * bw);
     w.S.alloc((size_t)ld * bw);
@@ -2284,6 +2436,8 @@ void cb_reml_options_default(cb_reml_options* o) {
     o->seed = 42;
     o->trace_exact = 0;
     cb_solve_options_default(&o->solve);
+    o->mc_samples = 100;  // vce_mc_samples (config.h:56)
+    o->adaptive = 0;
     // REML solves accept the FP64 rounding floor up to 1e-8: the probes'
     // sampling noise in the trace is orders of magnitude larger
     o->solve.stall_tol = 1e-8;
@@ -2910,6 +3064,57 @@ struct DenseExact {
     }
 };
 
+
+// calc_logL_internal (vce.cpp:3533-3588) with C(ve/vu) assembled densely on
+// the device and factorised by Cholesky; buffers kept in lw between calls.
+// *logl = -1e20 (the reference's failed-determinant value) when C is not
+// positive definite; returns log|C| (NaN then).
+struct LoglWork {
+    DevBuf<double> C, b, part;
+    DevBuf<int> info;
+};
+
+static double reml_logl(cb_mme* M, LoglWork& lw, double var_u, double var_e, double* logl) {
+    hipStream_t st = M->ctx->stream;
+    const int64_t dim = M->dim, p = M->p, q = M->q;
+    const double lambda = var_e / std::max(var_u, 1e-9);
+    if (lw.C.n < (size_t)dim * dim) lw.C.alloc((size_t)dim * dim);
+    if (lw.b.n < (size_t)dim) lw.b.alloc(dim);
+    if (!lw.info.p) lw.info.alloc(1);
+    if (!lw.part.p) lw.part.alloc(DOT_CHUNKS);
+    assemble_dense_C(M, lambda, lw.C.p);
+    copy_host(st, lw.b.p, M->rhs.data(), dim * 8, hipMemcpyHostToDevice);
+    BlasHandle bh(st);
+    const rocblas_handle h = bh.h;
+    CB_BLAS(bh, rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)dim, lw.C.p, (rocblas_int)dim, lw.info.p),
+            "REML logL: dpotrf");
+    int hinfo = 0;
+    copy_host(st, &hinfo, lw.info.p, sizeof(int), hipMemcpyDeviceToHost);
+    CB_HIP(hipStreamSynchronize(st));
+    if (hinfo != 0) {
+        *logl = -1e20;
+        return NAN;
+    }
+    CB_BLAS(bh, rocsolver_dpotrs(h, rocblas_fill_lower, (rocblas_int)dim, 1, lw.C.p, (rocblas_int)dim, lw.b.p,
+                                 (rocblas_int)dim),
+            "REML logL: dpotrs");
+    k_logdiag_part<<<DOT_CHUNKS, 256, 0, st>>>(lw.C.p, dim, lw.part.p);
+    std::vector<double> hp(DOT_CHUNKS), sol(dim);
+    copy_host(st, hp.data(), lw.part.p, DOT_CHUNKS * 8, hipMemcpyDeviceToHost);
+    copy_host(st, sol.data(), lw.b.p, dim * 8, hipMemcpyDeviceToHost);
+    CB_HIP(hipStreamSynchronize(st));
+    double ld = 0;
+    for (double v : hp) ld += v;
+    ld *= 2.0;  // log|C| = 2 sum log L_ii
+    double sr = 0;
+    for (int64_t i = 0; i < dim; ++i) sr += sol[i] * M->rhs[i];
+    const double yPy = (M->yty - sr) / var_e;
+    const double n = (double)M->n;
+    *logl = -0.5 * (ld + (n - (double)p - (double)q) * std::log(var_e) +
+                    (double)q * std::log(std::max(var_u, 1e-9)) + yPy);
+    return ld;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2921,7 +3126,8 @@ int cb_reml_score(cb_mme* M, const cb_reml_options* opts, double var_u, double v
         CB_REQUIRE(M->qkind >= 0, "cb_reml_score: Qinv not set");
         CB_REQUIRE(var_u > 0 && var_e > 0, "cb_reml_score: variances must be positive");
         set_device(M->ctx);
-        if (opts->algorithm == CB_VCE_AI_EXACT) {
+        const RemlPlan pl = reml_plan(M, *opts);
+        if (pl.first == CB_VCE_AI_EXACT) {
             DenseExact de(M);
             de.init();
             double lg = 0, g[2], a[4];
@@ -2936,7 +3142,7 @@ int cb_reml_score(cb_mme* M, const cb_reml_options* opts, double var_u, double v
             return;
         }
         cb_reml_work w;
-        reml_setup(M, *opts, w);
+        reml_setup(M, *opts, w, pl);
         RemlState s;
         reml_eval(M, *opts, w, var_u, var_e, ai != nullptr, s, solution);
         if (grad) { grad[0] = s.grad[0]; grad[1] = s.grad[1]; }
@@ -2946,7 +3152,9 @@ int cb_reml_score(cb_mme* M, const cb_reml_options* opts, double var_u, double v
 }
 
 // AI_REML::solve (vce.cpp:741-837) with the data-driven AI step
-// (vce.cpp:2414-2857) or the EM + Hutchinson step (vce.cpp:1490-1574).
+// (vce.cpp:2414-2857), the EM + Hutchinson step (vce.cpp:1490-1574), the
+// dense-exact step (vce.cpp:2859-3195), or the reference's own choice among
+// them per iteration (CB_VCE_AUTO, reml_plan).
 int cb_reml_run(cb_mme* M, const cb_reml_options* opts, double var_u0, double var_e0, double* var_u,
                 double* var_e, int64_t* iterations, int* converged, double* hist_vu, double* hist_ve,
                 double* solution, double* ai_last, double* se) {
@@ -2954,11 +3162,11 @@ int cb_reml_run(cb_mme* M, const cb_reml_options* opts, double var_u0, double va
         CB_REQUIRE(M && opts && var_u && var_e, "cb_reml_run: NULL argument");
         CB_REQUIRE(M->qkind >= 0, "cb_reml_run: Qinv not set");
         CB_REQUIRE(var_u0 > 0 && var_e0 > 0, "cb_reml_run: initial variances must be positive");
-        CB_REQUIRE(opts->algorithm == CB_VCE_AI || opts->algorithm == CB_VCE_EM || opts->algorithm == CB_VCE_AI_EXACT,
-                   "cb_reml_run: unknown algorithm");
+        CB_REQUIRE(opts->algorithm >= CB_VCE_AI && opts->algorithm <= CB_VCE_AUTO, "cb_reml_run: unknown algorithm");
         set_device(M->ctx);
         const cb_reml_options& o = *opts;
-        if (o.algorithm == CB_VCE_AI_EXACT) {
+        const RemlPlan pl = reml_plan(M, o);
+        if (pl.first == CB_VCE_AI_EXACT) {
             // AI_REML::solve with run_dense_exact_step (vce.cpp:741-837,2949-3195)
             DenseExact de(M);
             de.init();
@@ -3027,7 +3235,7 @@ int cb_reml_run(cb_mme* M, const cb_reml_options* opts, double var_u0, double va
             return;
         }
         cb_reml_work w;
-        reml_setup(M, o, w);
+        reml_setup(M, o, w, pl);
         // variance stabiliser of vce.cpp:1370-1389
         const int64_t n = M->n;
         std::vector<double> hy(n);
@@ -3054,17 +3262,14 @@ int cb_reml_run(cb_mme* M, const cb_reml_options* opts, double var_u0, double va
         int conv = 0;
         double last_ai[4] = {0, 0, 0, 0};
         bool have_ai = false;
+        LoglWork lw;
         for (; it < o.max_iter;) {
             const double old_u = vu, old_e = ve;
+            const int kind = it == 0 ? pl.first : pl.later;
             RemlState s;
-            reml_eval(M, o, w, vu, ve, o.algorithm == CB_VCE_AI, s, nullptr);
-            if (o.algorithm == CB_VCE_EM) {
-                const double lambda = ve / std::max(vu, 1e-9);
-                const double vu_new = (s.uQu + s.tr * ve) / (double)M->q;
-                const double ve_new = (s.esse + ve * ((double)(M->p + M->q) - lambda * s.tr)) / (double)n;
-                vu = stabilize(vu_new, old_u);
-                ve = stabilize(ve_new, old_e);
-            } else {
+            reml_eval(M, o, w, vu, ve, kind == CB_VCE_AI, s, nullptr);
+            bool em = kind == CB_VCE_EM;
+            if (!em) {
                 double A[4] = {s.ai[0], s.ai[1], s.ai[2], s.ai[3]};
                 project_spd_2x2(A);
                 std::copy(A, A + 4, last_ai);  // last_AI_mat (vce.cpp:3245-3252)
@@ -3074,8 +3279,34 @@ int cb_reml_run(cb_mme* M, const cb_reml_options* opts, double var_u0, double va
                 double d1 = (A[0] * s.grad[1] - A[2] * s.grad[0]) / det;
                 bool ok = std::isfinite(d0) && std::isfinite(d1) &&
                           std::fabs(d0) <= 100.0 * vu && std::fabs(d1) <= 100.0 * ve;
-                if (ok) {
-                    // halve until positive and within 5x (vce.cpp:2797-2813)
+                if (ok && pl.logl) {
+                    // Newton step accepted only where logL improves, <= 15
+                    // halvings (vce.cpp:2797-2848); unstabilised, as there
+                    double cur = 0;
+                    (void)reml_logl(M, lw, vu, ve, &cur);
+                    if (!std::isfinite(cur)) cur = -1e18;
+                    ok = false;
+                    double nu = vu, ne = ve;
+                    for (int h = 0; h < 15; ++h) {
+                        nu = vu + d0;
+                        ne = ve + d1;
+                        if (nu > 0.0 && ne > 0.0 && std::fabs(d0) <= 5.0 * vu && std::fabs(d1) <= 5.0 * ve) {
+                            double nl = 0;
+                            (void)reml_logl(M, lw, nu, ne, &nl);
+                            if (std::isfinite(nl) && nl > cur) {
+                                ok = true;
+                                break;
+                            }
+                        }
+                        d0 *= 0.5;
+                        d1 *= 0.5;
+                    }
+                    if (ok) {
+                        vu = nu;
+                        ve = ne;
+                    }
+                } else if (ok) {
+                    // no logL: halve until positive and within 5x (vce.cpp:2797-2813)
                     int h = 0;
                     for (; h < 15; ++h) {
                         const double nu = vu + d0, ne = ve + d1;
@@ -3084,16 +3315,17 @@ int cb_reml_run(cb_mme* M, const cb_reml_options* opts, double var_u0, double va
                         d1 *= 0.5;
                     }
                     ok = h < 15;
+                    if (ok) {
+                        vu = stabilize(vu + d0, old_u);
+                        ve = stabilize(ve + d1, old_e);
+                    }
                 }
-                if (ok) {
-                    vu = stabilize(vu + d0, old_u);
-                    ve = stabilize(ve + d1, old_e);
-                } else {
-                    // EM fallback for this iteration (vce.cpp:1441-1454, PCG path)
-                    const double lambda = ve / std::max(vu, 1e-9);
-                    vu = stabilize((s.uQu + s.tr * ve) / (double)M->q, old_u);
-                    ve = stabilize((s.esse + ve * ((double)(M->p + M->q) - lambda * s.tr)) / (double)n, old_e);
-                }
+                em = !ok;  // the EM step for this iteration (vce.cpp:1441-1454)
+            }
+            if (em) {
+                const double lambda = ve / std::max(vu, 1e-9);
+                vu = stabilize((s.uQu + s.tr * ve) / (double)M->q, old_u);
+                ve = stabilize((s.esse + ve * ((double)(M->p + M->q) - lambda * s.tr)) / (double)n, old_e);
             }
             if (hist_vu) hist_vu[it] = vu;
             if (hist_ve) hist_ve[it] = ve;
@@ -3127,9 +3359,24 @@ int cb_reml_run(cb_mme* M, const cb_reml_options* opts, double var_u0, double va
             // final solve at the estimates (GSSolve uses lambda = ve/vu)
             const double lambda = ve / std::max(vu, 1e-9);
             CB_HIP(hipMemcpyAsync(w.B.p, w.rhs.p, M->ldv * 8, hipMemcpyDeviceToDevice, M->ctx->stream));
-            solve_block(M, lambda, o.solve, w.B.p, w.S.p, 1, w.have_s);
+            solve_block(M, lambda, o.solve, w.B.p, w.S.p, 1, w.have_s && !w.adaptive);
             copy_host(M->ctx->stream, solution, w.S.p, M->dim * 8, hipMemcpyDeviceToHost);
         }
+    });
+}
+
+int cb_reml_branch(const cb_mme* M, const cb_reml_options* opts, int* first_step, int* later_step, int* trace_rule,
+                   int* logl_checked, int64_t* min_probes, int64_t* max_probes) {
+    return guarded([&] {
+        CB_REQUIRE(M && opts, "cb_reml_branch: NULL argument");
+        CB_REQUIRE(M->qkind >= 0, "cb_reml_branch: Qinv not set");
+        const RemlPlan pl = reml_plan(M, *opts);
+        if (first_step) *first_step = pl.first;
+        if (later_step) *later_step = pl.later;
+        if (trace_rule) *trace_rule = pl.first == CB_VCE_AI_EXACT ? 2 : pl.trace;
+        if (logl_checked) *logl_checked = pl.logl || pl.first == CB_VCE_AI_EXACT ? 1 : 0;
+        if (min_probes) *min_probes = pl.min_k;
+        if (max_probes) *max_probes = pl.max_k;
     });
 }
 
@@ -3140,47 +3387,8 @@ int cb_reml_logl(cb_mme* M, double var_u, double var_e, double* logl, double* lo
         CB_REQUIRE(var_u > 0 && var_e > 0, "cb_reml_logl: variances must be positive");
         CB_REQUIRE(M->dim <= 65536, "cb_reml_logl: dense factorisation limited to p + q <= 65536");
         set_device(M->ctx);
-        hipStream_t st = M->ctx->stream;
-        const int64_t dim = M->dim, p = M->p, q = M->q;
-        const double lambda = var_e / std::max(var_u, 1e-9);
-        DevBuf<double> C, b;
-        C.alloc((size_t)dim * dim);
-        b.alloc(dim);
-        assemble_dense_C(M, lambda, C.p);
-        copy_host(st, b.p, M->rhs.data(), dim * 8, hipMemcpyHostToDevice);
-        DevBuf<int> info;
-        info.alloc(1);
-        BlasHandle bh(st);
-        const rocblas_handle h = bh.h;
-        CB_BLAS(bh, rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)dim, C.p, (rocblas_int)dim, info.p),
-                "cb_reml_logl: dpotrf");
-        int hinfo = 0;
-        copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
-        CB_HIP(hipStreamSynchronize(st));
-        if (hinfo != 0) {  // not positive definite: the reference's failed-determinant value
-            *logl = -1e20;
-            if (logdet_c) *logdet_c = NAN;
-            return;
-        }
-        CB_BLAS(bh, rocsolver_dpotrs(h, rocblas_fill_lower, (rocblas_int)dim, 1, C.p, (rocblas_int)dim, b.p,
-                                     (rocblas_int)dim),
-                "cb_reml_logl: dpotrs");
-        DevBuf<double> part;
-        part.alloc(DOT_CHUNKS);
-        k_logdiag_part<<<DOT_CHUNKS, 256, 0, st>>>(C.p, dim, part.p);
-        std::vector<double> hp(DOT_CHUNKS), sol(dim);
-        copy_host(st, hp.data(), part.p, DOT_CHUNKS * 8, hipMemcpyDeviceToHost);
-        copy_host(st, sol.data(), b.p, dim * 8, hipMemcpyDeviceToHost);
-        CB_HIP(hipStreamSynchronize(st));
-        double ld = 0;
-        for (double v : hp) ld += v;
-        ld *= 2.0;  // log|C| = 2 sum log L_ii
-        double sr = 0;
-        for (int64_t i = 0; i < dim; ++i) sr += sol[i] * M->rhs[i];
-        // calc_logL_internal (vce.cpp:3533-3588)
-        const double yPy = (M->yty - sr) / var_e;
-        const double n = (double)M->n;
-        *logl = -0.5 * (ld + (n - (double)p - (double)q) * std::log(var_e) + (double)q * std::log(std::max(var_u, 1e-9)) + yPy);
+        LoglWork lw;
+        const double ld = reml_logl(M, lw, var_u, var_e, logl);
         if (logdet_c) *logdet_c = ld;
     });
 }
@@ -3190,7 +3398,7 @@ int cb_reml_copy_probes(cb_mme* M, const cb_reml_options* opts, double* out, int
         CB_REQUIRE(M && opts && out && ld >= M->q, "cb_reml_copy_probes: bad argument");
         set_device(M->ctx);
         cb_reml_work w;
-        reml_setup(M, *opts, w);
+        reml_setup(M, *opts, w, reml_plan(M, *opts));
         // probes bw at a time through B's columns (the run's own stream)
         for (int64_t f = 0; f < w.k; f += w.bw) {
             const int cnt = (int)std::min<int64_t>(w.bw, w.k - f);

@@ -33,14 +33,17 @@ def solve_options(precision=_lib.CB_PREC_MIXED, precond=_lib.CB_PRECOND_BLOCK_JA
 
 def reml_options(algorithm=_lib.CB_VCE_AI, max_iter=20, tol=1e-4, n_probes=64, seed=42,
                  trace_exact=False, solve: _lib.SolveOptions | None = None,
-                 block_cols: int = 0, stall_tol: float = 1e-8) -> _lib.RemlOptions:
+                 block_cols: int = 0, stall_tol: float = 1e-8, mc_samples: int = 100,
+                 adaptive: bool = False) -> _lib.RemlOptions:
     """stall_tol (default 1e-8, as cb_reml_options_default) is applied to the
-    REML solves, also to a `solve` given here."""
+    REML solves, also to a `solve` given here. algorithm CB_VCE_AUTO takes
+    the reference's size-based branch (AIREML.branch() reports it)."""
     o = _lib.RemlOptions()
     _lib.lib().cb_reml_options_default(C.byref(o))
     o.algorithm, o.max_iter, o.tol, o.n_probes, o.seed = algorithm, max_iter, tol, n_probes, seed
     o.trace_exact = int(trace_exact)
     o.block_cols = block_cols
+    o.mc_samples, o.adaptive = mc_samples, int(adaptive)
     if solve is not None:
         o.solve = solve
     o.solve.stall_tol = stall_tol
@@ -192,9 +195,21 @@ class AIREML:
                                             _lib.ptr(ai) if want_ai else None, C.byref(tr), _lib.ptr(sol)))
         return g, ai.reshape(2, 2) if want_ai else None, tr.value, sol
 
+    def branch(self) -> dict:
+        """The steps, trace rule and logL check this run takes
+        (cb_reml_branch; VCESolver::create's size-based choice for
+        CB_VCE_AUTO)."""
+        f, l, t, g = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        lo, hi = C.c_int64(), C.c_int64()
+        _lib.check(_lib.lib().cb_reml_branch(self.model.handle, C.byref(self.opts), C.byref(f), C.byref(l),
+                                             C.byref(t), C.byref(g), C.byref(lo), C.byref(hi)))
+        return {"first": f.value, "later": l.value, "trace": t.value, "logl_checked": bool(g.value),
+                "min_probes": lo.value, "max_probes": hi.value}
+
     def probes(self) -> np.ndarray:
         m = self.model
-        k = m.q if self.opts.trace_exact else int(self.opts.n_probes)
+        b = self.branch()
+        k = m.q if b["trace"] == 2 else b["max_probes"] if b["trace"] == 1 else int(self.opts.n_probes)
         out = np.empty((k, m.q))
         _lib.check(_lib.lib().cb_reml_copy_probes(m.handle, C.byref(self.opts), _lib.ptr(out), m.q))
         return out.T

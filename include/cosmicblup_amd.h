@@ -236,6 +236,26 @@ int cb_mme_solve_device(cb_mme* mme, double lambda, const cb_solve_options* opts
 #define CB_VCE_AI_EXACT 2  /* dense-exact AI step (run_dense_exact_step, vce.cpp:2859-3195):
                               exact traces from dense P = (I - W C^-1 W')/ve on the device,
                               <= 6 logL halvings; n, p + q <= 40000 */
+/* The reference's own choice per iteration (AI_REML::run_ai_iteration,
+ * vce.cpp:1334-1574, with the solver set-up of build_lhs, vce.cpp:1041-1110,
+ * and estimate_trace_Qinv_Cuu, vce.cpp:1266-1315), from dim = p + q, the
+ * kind of Q and q (cb_reml_branch reports it):
+ *   dim < 5000                         dense-exact AI step (as CB_VCE_AI_EXACT);
+ *   direct-solver sizes (dense or split Q with dim < 15000, CSR Q with
+ *   q <= 100000 and dim < 200000)      AI step (CSR: from the first iteration;
+ *                                      dense/split: EM first, AI after), the
+ *                                      Newton step accepted only when logL
+ *                                      improves (<= 15 halvings, vce.cpp:
+ *                                      2797-2848; p + q <= 65536) else EM;
+ *                                      traces exact when q <= 5000, else
+ *                                      adaptive with ns = max(50, mc_samples);
+ *   otherwise                          EM with adaptive traces, ns =
+ *                                      max(5, mc_samples / 2).
+ * Adaptive traces: Rademacher probes in batches of 16 until at least
+ * max(30, ns/3) and SE/|mean| < 0.05, at most max(100, ns)
+ * (AdaptiveTraceEstimator::estimate, adaptive_trace_estimator.cpp:11-83,
+ * with the batch size of one thread). */
+#define CB_VCE_AUTO 3
 
 typedef struct {
     int algorithm;      /* CB_VCE_*; default AI */
@@ -250,6 +270,11 @@ typedef struct {
                            chunks whose vectors and Q z are regenerated at each evaluation rather
                            than stored for the run; the estimator is unchanged (the reference
                            solves probes in batches too, adaptive_trace_estimator.cpp:11-83) */
+    int64_t mc_samples; /* VCEConfig.mc_samples (config.h:56), default 100: sets the adaptive
+                           probe counts (CB_VCE_AUTO, or adaptive = 1) */
+    int adaptive;       /* 1: the adaptive trace rule (see CB_VCE_AUTO) with ns = max(5,
+                           mc_samples / 2) for CB_VCE_AI / CB_VCE_EM instead of n_probes fixed
+                           probes; default 0. CB_VCE_AUTO sets its own. */
 } cb_reml_options;
 
 void cb_reml_options_default(cb_reml_options* opts);
@@ -277,6 +302,14 @@ int cb_reml_score(cb_mme* mme, const cb_reml_options* opts, double var_u, double
  * Cholesky (p + q <= 65536). A C that is not positive definite gives
  * -1e20, the reference's value for a failed determinant. logdet_c may be NULL. */
 int cb_reml_logl(cb_mme* mme, double var_u, double var_e, double* logl, double* logdet_c);
+/* The branch CB_VCE_AUTO takes for this model (VCESolver::create's size-based
+ * choice): *first_step and *later_step are CB_VCE_AI_EXACT, CB_VCE_AI or
+ * CB_VCE_EM for iteration 0 and the iterations after it; *trace_rule is 0
+ * (fixed n_probes), 1 (adaptive) or 2 (exact: q unit probes); *logl_checked
+ * is 1 when AI steps are accepted on a logL increase; *min_probes and
+ * *max_probes are the adaptive rule's bounds. Any output may be NULL. */
+int cb_reml_branch(const cb_mme* mme, const cb_reml_options* opts, int* first_step, int* later_step,
+                   int* trace_rule, int* logl_checked, int64_t* min_probes, int64_t* max_probes);
 /* The probe matrix the run uses (q x k, column-major, leading dimension ld). */
 int cb_reml_copy_probes(cb_mme* mme, const cb_reml_options* opts, double* out, int64_t ld);
 

@@ -85,8 +85,28 @@ def mme_dense(X, aid, y, Q, lam):
     return C, W.T @ y, W
 
 
-def reml_eval(X, aid, y, Q, vu, ve, Zp, want_ai=True, exact=False):
-    """Score, AI and trace at (vu, ve) with probe matrix Zp (q x k)."""
+def adaptive_trace(t, min_k, max_k):
+    """AdaptiveTraceEstimator::estimate (src/core/adaptive_trace_estimator.cpp:
+    11-83) over per-probe values t: probes in batches of 16 (one thread);
+    after each batch, once at least min_k are done, stop when SE/|mean| <
+    0.05; at most max_k. Returns (mean, probes used)."""
+    done = 0
+    while done < max_k:
+        done = min(max_k, done + 16)
+        if done >= min_k and done >= 2:
+            v = np.asarray(t[:done], dtype=np.float64)
+            ssum, sq = float(np.sum(v)), float(np.sum(v * v))
+            mean = ssum / done
+            var = (sq - ssum * ssum / done) / (done - 1.0)
+            se = np.sqrt(max(0.0, var) / done)
+            if abs(mean) > 1e-12 and se / abs(mean) < 0.05:
+                break
+    return float(np.mean(np.asarray(t[:done], dtype=np.float64))), done
+
+
+def reml_eval(X, aid, y, Q, vu, ve, Zp, want_ai=True, exact=False, adaptive=None):
+    """Score, AI and trace at (vu, ve) with probe matrix Zp (q x k);
+    adaptive = (min_k, max_k) applies adaptive_trace to Zp's columns."""
     n, p = X.shape
     q = Q.shape[0]
     lam = ve / max(vu, 1e-9)
@@ -102,7 +122,12 @@ def reml_eval(X, aid, y, Q, vu, ve, Zp, want_ai=True, exact=False):
     uQu = float(u @ Q @ u)
     QZ = Q @ Zp
     t = np.einsum("ij,ij->j", QZ, S[p:, 1:])
-    tr = float(t.sum()) if exact else float(t.mean())
+    if exact:
+        tr = float(t.sum())
+    elif adaptive is not None:
+        tr, _ = adaptive_trace(t, adaptive[0], adaptive[1])
+    else:
+        tr = float(t.mean())
     lam_tr = min(lam * tr, q - 1e-6)
     grad = np.array([0.5 * (uQu + tr * ve - q * vu) / (vu * vu),
                      0.5 * (esse - ve * (n - p - q + lam_tr)) / (ve * ve)])
@@ -131,7 +156,13 @@ def project_spd_2x2(A, eps_rel=1e-8, eps_abs=1e-12):
 
 
 def reml_run(X, aid, y, Q, vu0=1.0, ve0=1.0, algorithm="ai", max_iter=20, tol=1e-4, Zp=None,
-             exact=False):
+             exact=False, later=None, adaptive=None, logl_check=False):
+    """AI_REML::solve (vce.cpp:741-837): `algorithm` is the step of the
+    first iteration ("ai" or "em"), `later` (default: the same) that of the
+    iterations after it; adaptive = (min_k, max_k) traces (reml_eval); with
+    logl_check an AI step is accepted only where logL improves (<= 15
+    halvings, positive, within 5x, vce.cpp:2797-2848), unstabilised, and a
+    failed one is an EM step."""
     n, p = X.shape
     q = Q.shape[0]
     ymean = y.mean()
@@ -152,36 +183,55 @@ def reml_run(X, aid, y, Q, vu0=1.0, ve0=1.0, algorithm="ai", max_iter=20, tol=1e
     hist = []
     conv = False
     ai_last = None
-    for _ in range(max_iter):
+    later = later or algorithm
+    for it in range(max_iter):
         ou, oe = vu, ve
-        s = reml_eval(X, aid, y, Q, vu, ve, Zp, want_ai=(algorithm == "ai"), exact=exact)
+        step = algorithm if it == 0 else later
+        s = reml_eval(X, aid, y, Q, vu, ve, Zp, want_ai=(step == "ai"), exact=exact, adaptive=adaptive)
         lam = ve / max(vu, 1e-9)
         em_u = (s["uQu"] + s["tr"] * ve) / q
         em_e = (s["esse"] + ve * ((p + q) - lam * s["tr"])) / n
-        if algorithm == "em":
+        if step == "em":
             vu, ve = stabilize(em_u, ou), stabilize(em_e, oe)
         else:
             A = project_spd_2x2(s["ai"])
             ai_last = A
             d = np.linalg.solve(A, s["grad"])
             ok = np.all(np.isfinite(d)) and abs(d[0]) <= 100 * vu and abs(d[1]) <= 100 * ve
-            if ok:
+            if ok and logl_check:
+                cur = logL(X, aid, y, Q, vu, ve)
+                ok = False
                 for h in range(15):
-                    if vu + d[0] > 0 and ve + d[1] > 0 and abs(d[0]) <= 5 * vu and abs(d[1]) <= 5 * ve:
-                        break
+                    nu, ne = vu + d[0], ve + d[1]
+                    if nu > 0 and ne > 0 and abs(d[0]) <= 5 * vu and abs(d[1]) <= 5 * ve:
+                        nl = logL(X, aid, y, Q, nu, ne)
+                        if np.isfinite(nl) and nl > cur:
+                            ok = True
+                            break
                     d = d * 0.5
+                if ok:
+                    vu, ve = nu, ne
                 else:
-                    ok = False
-            if ok:
-                vu, ve = stabilize(vu + d[0], ou), stabilize(ve + d[1], oe)
+                    vu, ve = stabilize(em_u, ou), stabilize(em_e, oe)
             else:
-                vu, ve = stabilize(em_u, ou), stabilize(em_e, oe)
+                if ok:
+                    for h in range(15):
+                        if vu + d[0] > 0 and ve + d[1] > 0 and abs(d[0]) <= 5 * vu and abs(d[1]) <= 5 * ve:
+                            break
+                        d = d * 0.5
+                    else:
+                        ok = False
+                if ok:
+                    vu, ve = stabilize(vu + d[0], ou), stabilize(ve + d[1], oe)
+                else:
+                    vu, ve = stabilize(em_u, ou), stabilize(em_e, oe)
         hist.append((vu, ve))
         if abs(ve - oe) + abs(vu - ou) < tol:
             conv = True
             break
     if ai_last is None:  # EM run: the AI matrix at the estimates
-        ai_last = project_spd_2x2(reml_eval(X, aid, y, Q, vu, ve, Zp, want_ai=True, exact=exact)["ai"])
+        ai_last = project_spd_2x2(reml_eval(X, aid, y, Q, vu, ve, Zp, want_ai=True, exact=exact,
+                                            adaptive=adaptive)["ai"])
     # calculate_SE (vce.cpp:3342-3352): SE = sqrt(max(0, diag(AI^-1)))
     se = np.sqrt(np.maximum(0.0, np.diag(np.linalg.inv(ai_last))))
     lam = ve / max(vu, 1e-9)

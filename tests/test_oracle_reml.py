@@ -237,3 +237,26 @@ def test_logl_equals_v_form_reml_plus_log_det_a():
         P = Vi - Vi @ X @ np.linalg.solve(XVX, X.T @ Vi)
         v_form = -0.5 * (np.linalg.slogdet(V)[1] + np.linalg.slogdet(XVX)[1] + y @ P @ y)
         assert abs(orl.logL(X, aid, y, Q, vu, ve) - v_form - half_logdet_a) < 1e-8
+
+
+def test_adaptive_trace_rule_by_brute_force():
+    """adaptive_trace against the stopping rule evaluated independently with
+    numpy's two-pass standard deviation at every batch end (16, 32, ...,
+    max_k): the first batch end >= min_k with std/sqrt(n)/|mean| < 0.05."""
+    rng = np.random.default_rng(5)
+    for loc, scale, lo, hi in [(10.0, 1.0, 30, 100), (1.0, 3.0, 33, 100), (1.0, 20.0, 30, 100),
+                               (-5.0, 8.0, 30, 150), (2.0, 0.0, 30, 100)]:
+        t = loc + scale * rng.standard_normal(hi)
+        want = hi
+        for end in list(range(16, hi, 16)) + [hi]:
+            if end < lo:
+                continue
+            v = t[:end]
+            if abs(v.mean()) > 1e-12 and np.std(v, ddof=1) / np.sqrt(end) / abs(v.mean()) < 0.05:
+                want = end
+                break
+        mean, used = orl.adaptive_trace(t, lo, hi)
+        assert used == want, (loc, scale, used, want)
+        assert mean == pytest.approx(t[:want].mean(), rel=1e-14)
+    # constant values stop at the first batch end past min_k
+    assert orl.adaptive_trace(np.full(100, 3.0), 30, 100) == (3.0, 32)
