@@ -943,6 +943,12 @@ __global__ void k_probes(double* __restrict__ Z, int64_t ld, int64_t off, int64_
     Z[(int64_t)c * ld + off + j] = v;
 }
 
+// B[:, c] = e_{first + c} (B zeroed by the caller)
+__global__ void k_unit_cols(double* __restrict__ B, int64_t ld, int64_t first, int r) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < r) B[(int64_t)c * ld + first + c] = 1.0;
+}
+
 // synthetic operands for the tuner: uniform in [-0.5, 0.5) from a hash of the index
 template <class T>
 __global__ void k_tune_fill(T* __restrict__ a, int64_t n, uint64_t seed) {
@@ -2803,6 +2809,46 @@ int cb_mme_solve_device(cb_mme* M, double lambda, const cb_solve_options* opts, 
                         int64_t ldb, double* dX, int64_t ldx, int64_t* iterations, double* rel_residual) {
     return mme_solve_blocks(M, lambda, opts, r, dB, ldb, dX, ldx, true, iterations, rel_residual,
                             "cb_mme_solve_device");
+}
+
+// GSSolve's fixed-effect standard errors (mixed_model_solver.cpp:346-417):
+// column i of C^-1 for each of the p fixed effects, from C x = e_i, as
+// right-hand-side blocks of at most BETA_COV_COLS unit columns.
+int cb_mme_beta_cov(cb_mme* M, double lambda, double var_e, const cb_solve_options* opts, double* cinv_beta,
+                    double* beta_se, int64_t* iterations) {
+    cb_solve_options o;
+    cb_solve_options_default(&o);
+    if (opts) o = *opts;
+    return guarded([&] {
+        CB_REQUIRE(M && (cinv_beta || beta_se), "cb_mme_beta_cov: NULL argument");
+        CB_REQUIRE(M->qkind >= 0, "cb_mme_beta_cov: Qinv not set");
+        CB_REQUIRE(lambda >= 0 && std::isfinite(lambda) && var_e > 0, "cb_mme_beta_cov: bad lambda or var_e");
+        set_device(M->ctx);
+        hipStream_t st = M->ctx->stream;
+        const int64_t p = M->p, ld = M->ldv;
+        constexpr int64_t BETA_COV_COLS = 256;
+        std::vector<double> cinv((size_t)p * p);
+        int64_t its = 0;
+        for (int64_t c0 = 0; c0 < p; c0 += BETA_COV_COLS) {
+            const int r = (int)std::min<int64_t>(BETA_COV_COLS, p - c0);
+            const WsPtr<double> dB = ws<double>(M, WS_IO_B, (size_t)ld * r), dX = ws<double>(M, WS_IO_X, (size_t)ld * r);
+            CB_HIP(hipMemsetAsync(dB.p, 0, (size_t)ld * r * 8, st));
+            k_unit_cols<<<blocks(r), 256, 0, st>>>(dB.p, ld, c0, r);
+            CB_HIP(hipGetLastError());
+            const SolveStats S = solve_block(M, lambda, o, dB.p, dX.p, r);
+            its = std::max(its, S.iterations);
+            // rows 0..p of each solution column
+            std::vector<double> hx((size_t)p * r);
+            copy_host_2d(st, hx.data(), p * 8, dX.p, ld * 8, p * 8, r, hipMemcpyDeviceToHost);
+            CB_HIP(hipStreamSynchronize(st));
+            for (int c = 0; c < r; ++c)
+                for (int64_t j = 0; j < p; ++j) cinv[(size_t)(c0 + c) * p + j] = hx[(size_t)c * p + j];
+        }
+        if (cinv_beta) std::copy(cinv.begin(), cinv.end(), cinv_beta);  // column i = C^-1 e_i (p x p)
+        if (beta_se)
+            for (int64_t i = 0; i < p; ++i) beta_se[i] = std::sqrt(std::max(0.0, var_e * cinv[(size_t)i * p + i]));
+        if (iterations) *iterations = its;
+    });
 }
 
 }  // extern "C"

@@ -132,6 +132,16 @@ class MixedModel:
                                            C.byref(it), C.byref(rel)))
         return X.T.reshape(np.asarray(B).shape), it.value, rel.value
 
+    def beta_cov(self, lam: float, var_e: float, opts: _lib.SolveOptions | None = None):
+        """GSSolve's fixed-effect SEs (mixed_model_solver.cpp:346-417):
+        returns (Cinv_beta p x p, beta_se p)."""
+        opts = opts or solve_options()
+        cinv = np.empty((self.p, self.p))
+        se = np.empty(self.p)
+        _lib.check(_lib.lib().cb_mme_beta_cov(self.handle, lam, var_e, C.byref(opts), _lib.ptr(cinv), _lib.ptr(se),
+                                              None))
+        return cinv.T.copy(), se
+
     def solve_device(self, lam: float, dB: int, ldb: int, dX: int, ldx: int, r: int,
                      opts: _lib.SolveOptions | None = None):
         """solve() on device-resident column blocks (device pointers with

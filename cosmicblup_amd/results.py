@@ -1,8 +1,11 @@
-"""Result files of a single-trait run with one random effect — host mirror
-of GSOutput::writeResults (src/solve/result_writer.cpp:64-433): .vars
-(variances, h2 with delta-method SE and a normal-approximation p value),
-.beta (fixed effects) and .rand (random effects averaged over each
-animal's records, mean residual, record count).
+"""Result files — host mirror of GSOutput::writeResults
+(src/solve/result_writer.cpp:64-562) for a single-trait run with one random
+effect: .vars (variances, h2 with delta-method SE and a normal-approximation
+p value), .beta (fixed effects with SEs), .rand (random effects averaged
+over each animal's records, mean residual, record count) and .obsrand (per
+record: fixed part, random part, fitted value, residual); and of the
+multi-trait outputs: .mv.vars (variance_estimator.cpp:303-311) and
+GSOutput::writeMultiTrait's .beta / .rand (result_writer.cpp:672-761).
 
 Numbers are written as a C++ stream writes a double by default (%g, six
 significant digits)."""
@@ -122,3 +125,71 @@ def write_rand(out_prefix: str, model: str, ids: list[str], rec_aid, y, fd: Fixe
                 g = fmt(u[i]) if i < u.size else "nan"
                 f.write(f"{name}\t{g}\tnan\t0\n")
     return path
+
+
+def write_obsrand(out_prefix: str, model: str, rec_ids: list[str], y, fd: FixedDesignG, beta, u, rec_aid) -> str:
+    """The .obsrand table (result_writer.cpp:345-375 values, 548-562 file):
+    per record (1-based), its id, y, x'b, the random effect u[aid - 1] (0
+    for a record without a level), fitted = x'b + u and y - fitted."""
+    rec_aid = np.asarray(rec_aid, dtype=np.int64)
+    beta = np.asarray(beta, dtype=np.float64)
+    u = np.asarray(u, dtype=np.float64)
+    col = MODEL_TAGS.get(model, "PA")
+    path = out_prefix + ".obsrand"
+    with open(path, "w") as f:
+        f.write(f"Record\tID\tY\tFixed\t{col}\tFitted\tResidual\n")
+        for i in range(rec_aid.size):
+            xb = 0.0
+            for k in range(fd.ptr[i], fd.ptr[i + 1]):
+                c = int(fd.col[k])
+                if 0 <= c < fd.p:
+                    xb += float(fd.val[k]) * float(beta[c])
+            a = int(rec_aid[i]) - 1
+            val = float(u[a]) if 0 <= a < u.size else 0.0
+            fitted = xb + val
+            f.write(f"{i + 1}\t{rec_ids[i]}\t{fmt(y[i])}\t{fmt(xb)}\t{fmt(val)}\t{fmt(fitted)}\t"
+                    f"{fmt(float(y[i]) - fitted)}\n")
+    return path
+
+
+def write_mv_vars(out_prefix: str, Vg, Ve) -> str:
+    """The .mv.vars table of the multi-trait VCE (variance_estimator.cpp:
+    303-311): one line per (trait i, trait j), 1-based, with Vg and Ve."""
+    Vg, Ve = np.asarray(Vg, dtype=np.float64), np.asarray(Ve, dtype=np.float64)
+    path = out_prefix + ".mv.vars"
+    with open(path, "w") as f:
+        f.write("Trait1\tTrait2\tVg\tVe\n")
+        for i in range(Vg.shape[0]):
+            for j in range(Vg.shape[1]):
+                f.write(f"{i + 1}\t{j + 1}\t{fmt(Vg[i, j])}\t{fmt(Ve[i, j])}\n")
+    return path
+
+
+def write_multi_trait(out_prefix: str, beta, u, ids: list[str], trait_names: list[str] | None = None,
+                      effect_names: list[str] | None = None) -> tuple[str, str]:
+    """GSOutput::writeMultiTrait (result_writer.cpp:672-761): .beta with one
+    row per fixed effect and one column per trait, .rand with one row per
+    individual (EBV per trait). beta: p x t, u: n x t. Trait names default to
+    Trait_j; effect names to Intercept (p = 1) or Covar_k when no names of
+    length p are given."""
+    beta = np.asarray(beta, dtype=np.float64).reshape(-1, np.asarray(u).shape[1])
+    u = np.asarray(u, dtype=np.float64)
+    p, t = beta.shape
+    names = list(trait_names) if trait_names is not None and len(trait_names) >= t else \
+        [f"Trait_{j + 1}" for j in range(t)]
+    if effect_names is not None and len(effect_names) == p:
+        eff = list(effect_names)
+    elif p == 1:
+        eff = ["Intercept"]
+    else:
+        eff = [f"Covar_{k + 1}" for k in range(p)]
+    bpath, rpath = out_prefix + ".beta", out_prefix + ".rand"
+    with open(bpath, "w") as f:
+        f.write("Effect" + "".join(f"\t{names[j]}" for j in range(t)) + "\n")
+        for k in range(p):
+            f.write(eff[k] + "".join(f"\t{fmt(beta[k, j])}" for j in range(t)) + "\n")
+    with open(rpath, "w") as f:
+        f.write("ID" + "".join(f"\t{names[j]}" for j in range(t)) + "\n")
+        for i in range(u.shape[0]):
+            f.write(ids[i] + "".join(f"\t{fmt(u[i, j])}" for j in range(t)) + "\n")
+    return bpath, rpath

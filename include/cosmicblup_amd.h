@@ -230,6 +230,15 @@ int cb_mme_solve_device(cb_mme* mme, double lambda, const cb_solve_options* opts
                         const double* dB, int64_t ldb, double* dX, int64_t ldx, int64_t* iterations,
                         double* rel_residual);
 
+/* GSSolve's fixed-effect covariance and standard errors (calc_se, on by
+ * default: config.h:45; mixed_model_solver.cpp:346-417): C(lambda) x = e_i
+ * for each fixed effect i < p, solved as blocks of unit right-hand sides;
+ * cinv_beta (optional, p x p, column i = rows 0..p of C^-1 e_i, i.e.
+ * Cinv_beta) and beta_se (optional, p) = sqrt(max(0, var_e (C^-1)_ii)).
+ * iterations (optional): PCG iterations of the slowest block. */
+int cb_mme_beta_cov(cb_mme* mme, double lambda, double var_e, const cb_solve_options* opts, double* cinv_beta,
+                    double* beta_se, int64_t* iterations);
+
 /* ---- REML -------------------------------------------------------------- */
 #define CB_VCE_AI 0  /* data-driven AI step, vce.cpp:2414-2857 */
 #define CB_VCE_EM 1  /* EM + Hutchinson step, vce.cpp:1490-1574 */

@@ -9,6 +9,10 @@ path: dense FP64 numpy restatements of the reference algorithms.
                        + Hutchinson step (vce.cpp:1490-1574); variance
                        stabiliser vce.cpp:1370-1389
   logL                 calc_logL_internal (vce.cpp:3533-3588)
+  beta_cov             GSSolve's fixed-effect covariance / SEs
+                       (mixed_model_solver.cpp:346-417)
+  adaptive_trace       AdaptiveTraceEstimator::estimate's stopping rule
+                       (adaptive_trace_estimator.cpp:11-83)
   dense_exact_eval     run_dense_exact_step terms (vce.cpp:2859-3120): P from
                        the MME, M_u = P Z A Z', AI = 1/2 [tr(M_u M_u),
                        tr(M_u P); ., ||P||_F^2], exact gradient, logL with
@@ -77,12 +81,17 @@ def design_w(X: np.ndarray, aid: np.ndarray, q: int) -> np.ndarray:
 
 
 def mme_dense(X, aid, y, Q, lam):
-    """C = W'W + lam blockdiag(0, Q), rhs = W'y (data scale, as build_lhs)."""
+    """C = W'W + lam blockdiag(0, Q), rhs = W'y (data scale, as build_lhs).
+    W'W is formed from a sparse copy of W (one level per record), so the
+    product costs O(nnz) rather than a dense n x (p+q) GEMM."""
+    import scipy.sparse as sp
+
     p, q = X.shape[1], Q.shape[0]
     W = design_w(X, aid, q)
-    C = W.T @ W
+    Ws = sp.csr_matrix(W)
+    C = (Ws.T @ Ws).toarray()
     C[p:, p:] += lam * Q
-    return C, W.T @ y, W
+    return C, Ws.T @ y, W
 
 
 def adaptive_trace(t, min_k, max_k):
@@ -238,6 +247,18 @@ def reml_run(X, aid, y, Q, vu0=1.0, ve0=1.0, algorithm="ai", max_iter=20, tol=1e
     C, rhs, _ = mme_dense(X, aid, y, Q, lam)
     sol = np.linalg.solve(C, rhs)
     return {"vu": vu, "ve": ve, "hist": hist, "converged": conv, "sol": sol, "ai_last": ai_last, "se": se}
+
+
+def beta_cov(X, aid, y, Q, vu, ve):
+    """GSSolve's calc_se (src/solve/mixed_model_solver.cpp:346-417): column i
+    of C^-1 for each fixed effect, Cinv_beta = its first p rows, beta_se_i =
+    sqrt(max(0, ve (C^-1)_ii)), C = C(ve / vu)."""
+    p = X.shape[1]
+    C, _, _ = mme_dense(X, aid, y, Q, ve / vu)
+    E = np.zeros((C.shape[0], p))
+    E[np.arange(p), np.arange(p)] = 1.0
+    Ci = np.linalg.solve(C, E)[:p]
+    return Ci, np.sqrt(np.maximum(0.0, ve * np.diag(Ci)))
 
 
 def logL(X, aid, y, Q, vu, ve):

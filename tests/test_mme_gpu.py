@@ -466,3 +466,39 @@ def test_device_resident_solve_equals_host_solve(cb, ctx, precision):
     assert it_d == it and rel_d == rel
     assert np.array_equal(Xd[:, :M.dim].T, X)
     assert np.all(Xd[:, M.dim:] == -7.0)
+
+
+@pytest.mark.parametrize("precision", [_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED])
+@pytest.mark.parametrize("qkind", ["dense", "csr"])
+def test_beta_cov_matches_oracle(cb, ctx, precision, qkind):
+    """GSSolve's fixed-effect SEs (mixed_model_solver.cpp:346-417) from the
+    device's unit-column block solves, 300 fixed effects (two blocks of
+    right-hand sides), against the dense oracle."""
+    rng = np.random.default_rng(41)
+    n = 1500
+    if qkind == "dense":
+        x, _ = synth.simulate_dosages(n, 1500, maf=(0.05, 0.5), seed=41)
+        G, _, _ = og.grm_vanraden(synth.encode_bed(x), n, 1500)
+        Q = og.invert_psd_matrix(G)
+    else:
+        from cosmicblup_amd.pedigree import invertA_henderson
+
+        dam, sire = synth.simulate_pedigree(3, n // 3, unknown=0.05, seed=41)
+        A = invertA_henderson(dam, sire)
+        Q = A.toDense()
+    grp = rng.integers(0, 300, n)
+    fd = design.build_fixed_design([[f"g{int(c):03d}"] for c in grp])
+    aid = np.arange(n, dtype=np.int32)
+    y = rng.standard_normal(n) + grp * 0.01
+    M = mme.MixedModel(fd, aid, y, n)
+    if qkind == "dense":
+        M.set_qinv_dense(Q)
+    else:
+        M.set_qinv_csr(A.ptr, A.col, A.val)
+    vu, ve = 0.4, 0.9
+    tol = 1e-12 if precision == _lib.CB_PREC_FP64 else 1e-10
+    cinv, se = M.beta_cov(ve / vu, ve, mme.solve_options(precision=precision, tol=tol))
+    cref, sref = orl.beta_cov(design.dense_x(fd), aid, y, Q, vu, ve)
+    assert M.p > 256
+    np.testing.assert_allclose(se, sref, rtol=1e-7)
+    np.testing.assert_allclose(cinv, cref, atol=1e-8 * np.abs(cref).max())

@@ -260,3 +260,19 @@ def test_adaptive_trace_rule_by_brute_force():
         assert mean == pytest.approx(t[:want].mean(), rel=1e-14)
     # constant values stop at the first batch end past min_k
     assert orl.adaptive_trace(np.full(100, 3.0), 30, 100) == (3.0, 32)
+
+
+def test_beta_cov_is_the_gls_covariance():
+    """sigma_e^2 C^-1 restricted to the fixed effects equals the GLS
+    covariance (X'V^-1 X)^-1 with V = vu Z A Z' + ve I, A = Q^-1 (Henderson),
+    with a record lacking a level and two records sharing one."""
+    X, aid, y, Q, _ = small_problem(n=80, seed=21)
+    aid = aid.copy()
+    aid[3], aid[9] = -1, aid[8]
+    vu, ve = 0.6, 1.4
+    Ci, se = orl.beta_cov(X, aid, y, Q, vu, ve)
+    Z = orl.design_w(np.zeros((X.shape[0], 0)), aid, Q.shape[0])
+    V = vu * Z @ np.linalg.inv(Q) @ Z.T + ve * np.eye(X.shape[0])
+    gls = np.linalg.inv(X.T @ np.linalg.solve(V, X))
+    np.testing.assert_allclose(ve * Ci, gls, rtol=1e-8, atol=1e-12)
+    np.testing.assert_allclose(se, np.sqrt(np.diag(gls)), rtol=1e-8)

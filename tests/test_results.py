@@ -116,6 +116,32 @@ def test_beta_and_rand_tables(tmp_path):
     assert rows[3] == ["a", "0.125", "nan", "0"] and rows[4] == ["c", "1", "nan", "0"]
 
 
+def test_obsrand_and_multi_trait_tables(tmp_path):
+    """.obsrand: per record x'b, u of its level (0 without one), fitted and
+    residual. .mv.vars: (i, j, Vg, Ve) lines. Multi-trait .beta / .rand: one
+    column per trait, effect names of length p or Covar_k / Intercept."""
+    fd = design.build_fixed_design([["A"], ["B"], ["A"]], cat_names=["grp"])
+    beta = np.array([10.0, 2.0])
+    u = np.array([0.5, -0.25])
+    op = results.write_obsrand(str(tmp_path / "o"), "gblup", ["x1", "x2", "x3"], [11.0, 12.5, 9.0], fd, beta, u,
+                               [2, 1, 0])
+    rows = [l.split("\t") for l in Path(op).read_text().splitlines()]
+    assert rows[0] == ["Record", "ID", "Y", "Fixed", "GA", "Fitted", "Residual"]
+    assert rows[1] == ["1", "x1", "11", "10", "-0.25", "9.75", "1.25"]
+    assert rows[2] == ["2", "x2", "12.5", "12", "0.5", "12.5", "0"]
+    assert rows[3] == ["3", "x3", "9", "10", "0", "10", "-1"]
+    vp = results.write_mv_vars(str(tmp_path / "m"), [[1.0, 0.25], [0.25, 2.0]], [[3.0, 0.5], [0.5, 4.0]])
+    assert Path(vp).read_text().splitlines() == ["Trait1\tTrait2\tVg\tVe", "1\t1\t1\t3", "1\t2\t0.25\t0.5",
+                                                 "2\t1\t0.25\t0.5", "2\t2\t2\t4"]
+    bp, rp = results.write_multi_trait(str(tmp_path / "m"), [[1.5, -2.0], [0.125, 3.0]], [[0.1, 0.2], [0.3, 0.4]],
+                                       ["a", "b"], trait_names=["ht", "wt"])
+    assert Path(bp).read_text().splitlines() == ["Effect\tht\twt", "Covar_1\t1.5\t-2", "Covar_2\t0.125\t3"]
+    assert Path(rp).read_text().splitlines() == ["ID\tht\twt", "a\t0.1\t0.2", "b\t0.3\t0.4"]
+    bp, _ = results.write_multi_trait(str(tmp_path / "n"), [[7.0, 8.0]], [[0.0, 0.0]], ["a"],
+                                      effect_names=["mu"])
+    assert Path(bp).read_text().splitlines() == ["Effect\tTrait_1\tTrait_2", "mu\t7\t8"]
+
+
 @pytest.mark.gpu
 def test_pblup_minimal_end_to_end(cb, ctx, tmp_path):
     """The reference fixture's run (README: --model pblup with the given
@@ -139,10 +165,23 @@ def test_pblup_minimal_end_to_end(cb, ctx, tmp_path):
     C, rhs, _ = orl.mme_dense(design.dense_x(fd), aid, y, A.toDense(), ve / vu)
     ref = np.linalg.solve(C, rhs)
     np.testing.assert_allclose(X, ref, rtol=1e-10, atol=1e-12)
+    # fixed-effect SEs (calc_se, on by default) by unit-column solves on the device
+    _, se = M.beta_cov(ve / vu, ve, mme.solve_options(tol=1e-12, precision=cb._lib.CB_PREC_FP64))
+    _, se_ref = orl.beta_cov(design.dense_x(fd), aid, y, A.toDense(), vu, ve)
+    np.testing.assert_allclose(se, se_ref, rtol=1e-9)
     out = str(tmp_path / "pblup")
-    bl = Path(results.write_beta(out, fd, X[:M.p])).read_text().splitlines()
+    bl = Path(results.write_beta(out, fd, X[:M.p], se)).read_text().splitlines()
     assert bl[0] == "Levels\tEstimation\tSE" and bl[1].split("\t")[0] == "mu"
     assert float(bl[1].split("\t")[1]) == pytest.approx(ref[0], rel=1e-5)
+    assert float(bl[1].split("\t")[2]) == pytest.approx(se_ref[0], rel=1e-5)
+    ol = Path(results.write_obsrand(out, "pblup", [r.idstr for r in recs], y, fd, X[:M.p], X[M.p:],
+                                    aid + 1)).read_text().splitlines()
+    assert ol[0] == "Record\tID\tY\tFixed\tPA\tFitted\tResidual" and len(ol) == 1 + len(recs)
+    for k, line in enumerate(ol[1:]):
+        rec, rid, yv, fx, g, fit, res = line.split("\t")
+        assert int(rec) == k + 1 and rid == recs[k].idstr
+        assert float(fit) == pytest.approx(float(fx) + float(g), rel=1e-5, abs=1e-9)
+        assert float(res) == pytest.approx(y[k] - ref[0] - ref[M.p + aid[k]], rel=1e-5, abs=1e-9)
     rl = Path(results.write_rand(out, "pblup", ids, aid + 1, y, fd, X[:M.p], X[M.p:])).read_text().splitlines()
     assert [l.split("\t")[0] for l in rl[1:]] == ["id1", "id2", "id3", "id4"]
     for k, line in enumerate(rl[1:]):
