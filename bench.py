@@ -179,6 +179,19 @@ def reml_leg(ctx, n: int = 20000, m: int = 50000, k: int = 64, h2: float = 0.4, 
     print(f"[bench] mixed f16: {Rh.iterations} iterations in {th:.2f} s, vu={Rh.var_u:.5f} ve={Rh.var_e:.5f}",
           flush=True)
     h_ms, h_n = ctx.kernel_time("k_dense_h")
+    # the reference's own branch at this size (CB_VCE_AUTO: EM with adaptive
+    # Hutchinson probes, dense G^-1 with dim >= 15000), mixed precision
+    ra = mme.reml_options(algorithm=_lib.CB_VCE_AUTO, max_iter=max_iter,
+                          solve=mme.solve_options(precision=_lib.CB_PREC_MIXED, tol=1e-10))
+    Ra = mme.AIREML(model, ra)
+    branch = Ra.branch()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    Ra.solve()
+    ctx.synchronize()
+    ta = time.perf_counter() - t0
+    print(f"[bench] auto (reference branch {branch}): {Ra.iterations} iterations in {ta:.2f} s, "
+          f"vu={Ra.var_u:.5f} ve={Ra.var_e:.5f}", flush=True)
     # FP64 plain over the same run as mixed (plain_iters > 0 caps it)
     Rp, tp = run(_lib.CB_PREC_FP64, False, plain_iters or max_iter)
     print(f"[bench] fp64 plain: {Rp.iterations} iterations in {tp:.2f} s", flush=True)
@@ -239,6 +252,9 @@ def reml_leg(ctx, n: int = 20000, m: int = 50000, k: int = 64, h2: float = 0.4, 
                           "vc_rel_err_vs_fp64": vc_err_h, "ebv_rel_err_vs_fp64": ebv_err_h,
                           "trace_rel_err_vs_fp64": abs(tr16 - tr64) / abs(tr64),
                           "k_dense_h_avg_ms": h_ms / max(h_n, 1)},
+            "auto_reference_branch": {"branch": branch, "iterations": Ra.iterations, "converged": Ra.converged,
+                                      "seconds": ta, "iters_per_s": Ra.iterations / ta, "var_u": Ra.var_u,
+                                      "var_e": Ra.var_e},
             "qinv": "eigen-bent G^-1 (invert_psd_matrix semantics)", "qinv_setup_s": t_inv,
             "cpu_baseline": cpu_leg}
 

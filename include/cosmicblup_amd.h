@@ -206,11 +206,12 @@ typedef struct {
                            whose best true residual has not halved over eight
                            residual replacements has stalled there: it is accepted
                            when its residual is below stall_tol (the call then
-                           returns CB_OK with *rel_residual above tol), else the
-                           solve returns CB_ERR_NOT_CONVERGED at once. Default 0
-                           (no stall accepted); cb_reml_options_default sets 1e-8
-                           for the REML solves. (The reference's tol tests the
-                           recursive residual, pcg_solver.h:501.) */
+                           returns CB_OK with *rel_residual above tol), else it
+                           iterates on and the solve returns CB_ERR_NOT_CONVERGED
+                           at max_iter. Default 0 (no stall accepted);
+                           cb_reml_options_default sets 1e-8 for the REML solves.
+                           (The reference's tol tests the recursive residual,
+                           pcg_solver.h:501.) */
 } cb_solve_options;
 
 void cb_solve_options_default(cb_solve_options* opts);

@@ -477,8 +477,8 @@ def test_beta_cov_matches_oracle(cb, ctx, precision, qkind):
     rng = np.random.default_rng(41)
     n = 1500
     if qkind == "dense":
-        x, _ = synth.simulate_dosages(n, 1500, maf=(0.05, 0.5), seed=41)
-        G, _, _ = og.grm_vanraden(synth.encode_bed(x), n, 1500)
+        x, _ = synth.simulate_dosages(n, 4000, maf=(0.05, 0.5), seed=41)
+        G, _, _ = og.grm_vanraden(synth.encode_bed(x), n, 4000)
         Q = og.invert_psd_matrix(G)
     else:
         from cosmicblup_amd.pedigree import invertA_henderson
