@@ -130,6 +130,8 @@ _SIGS = {
                                       c_i64p, c_dp]),
     "cb_tune_dense_time": (C.c_int, [c_vp, C.c_int, c_i64, C.c_int, C.c_int, C.c_int, C.c_int, c_dp]),
     "cb_tune_dense_lookup": (C.c_int, [C.c_int, c_i64, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "cb_mme_solve_device_noalloc": (C.c_int, [c_vp, C.c_double, C.POINTER(SolveOptions), c_i64, c_vp, c_i64,
+                                              c_vp, c_i64, c_i64p, c_dp]),
     "cb_mme_beta_cov": (C.c_int, [c_vp, C.c_double, C.c_double, C.POINTER(SolveOptions), c_vp, c_vp, c_i64p]),
     "cb_reml_options_default": (None, [C.POINTER(RemlOptions)]),
     "cb_reml_run": (C.c_int, [c_vp, C.POINTER(RemlOptions), C.c_double, C.c_double, c_dp, c_dp, c_i64p,

@@ -33,7 +33,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -1057,7 +1059,69 @@ inline unsigned blocks(int64_t n, int t = 256) { return (unsigned)std::max<int64
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// The device scratch of cb_mme_solve_device_noalloc: one array of
+// NOALLOC_SCRATCH_BYTES on every GPU the library runs on, held until the
+// process ends. Host code never uses the array's name as an address: each
+// device's address comes from hipGetSymbolAddress, looked up once per
+// device with that device current (at cb_mme_create) and kept in
+// g_scratch_ptr.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch_ptr;  // device -> address of its scratch
+
+void init_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr.count(device)) return;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        throw cbamd::Error(CB_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<unsigned char*>(ptr);
+}
+
+unsigned char* scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    return it == g_scratch_ptr.end() ? nullptr : it->second;
+}
+}  // namespace
+
+// Scratch of an allocation-free call: sub-buffers carved from one fixed
+// device region, one per key, 256-byte aligned and reused by the same key
+// within the call. Running out is an error, never an allocation.
+struct ScratchArena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    struct Slot {
+        int key;
+        char* p;
+        size_t bytes;
+    };
+    std::vector<Slot> slots;
+    char* get(int key, size_t bytes) {
+        for (Slot& s : slots)
+            if (s.key == key && s.bytes >= bytes) return s.p;
+        const size_t at = (size_t)round_up((int64_t)used, 256);
+        if (at + bytes > cap)
+            throw cbamd::Error(CB_ERR_INVALID, "allocation-free call: its scratch (" + std::to_string(cap) +
+                                                   " bytes) is too small for this size");
+        for (Slot& s : slots)
+            if (s.key == key) {
+                s.p = base + at;
+                s.bytes = bytes;
+                used = at + bytes;
+                return base + at;
+            }
+        slots.push_back({key, base + at, bytes});
+        used = at + bytes;
+        return base + at;
+    }
+};
+
 struct cb_mme {
+    ScratchArena* arena = nullptr;  // set for the duration of an allocation-free call
     cb_context* ctx = nullptr;
     int64_t n = 0, p = 0, q = 0, dim = 0, ldv = 0;
     // design (device): W rows (records), W^T rows (unknowns)
@@ -1153,25 +1217,32 @@ struct WsPtr {
     size_t n;
     size_t bytes() const { return n * sizeof(T); }
 };
+// count elements of one of M's grow-only buffers, or, during an
+// allocation-free call, of the arena's region for `key`
+enum ScratchKey { SK_PART64 = 20, SK_PART32, SK_LCPART, SK_STIFF };
+template <class T>
+T* scratch_or(cb_mme* M, int key, DevBuf<T>& own, size_t count) {
+    if (M->arena) return reinterpret_cast<T*>(M->arena->get(key, count * sizeof(T)));
+    if (own.n < count) own.alloc(count);
+    return own.p;
+}
+
 template <class T>
 WsPtr<T> ws(cb_mme* M, WsSlot slot, size_t count) {
     static_assert(WS_COUNT <= 16, "work buffer slots");
-    DevBuf<uint8_t>& b = M->ws[slot];
-    if (b.n < count * sizeof(T)) b.alloc(count * sizeof(T));
-    return WsPtr<T>{reinterpret_cast<T*>(b.p), count};
+    uint8_t* p = scratch_or<uint8_t>(M, (int)slot, M->ws[slot], count * sizeof(T));
+    return WsPtr<T>{reinterpret_cast<T*>(p), count};
 }
 
 template <class T>
 T* partbuf(cb_mme* M, size_t need);
 template <>
 double* partbuf<double>(cb_mme* M, size_t need) {
-    if (M->part64.n < need) M->part64.alloc(need);
-    return M->part64.p;
+    return scratch_or<double>(M, SK_PART64, M->part64, need);
 }
 template <>
 float* partbuf<float>(cb_mme* M, size_t need) {
-    if (M->part32.n < need) M->part32.alloc(need);
-    return M->part32.p;
+    return scratch_or<float>(M, SK_PART32, M->part32, need);
 }
 
 // xg[i, c] = X[gmap[i], c] (0 for gmap < 0); yg[i, c] = 0
@@ -1435,12 +1506,12 @@ void apply_C(cb_mme* M, double lambda, const T* X, T* Y, int r) {
     const int64_t ld = M->ldv;
     // W'W part (overwrites Y)
     if (M->long_rows > 0) {
-        if (M->lc_part.n < (size_t)M->n_lchunks * r) M->lc_part.alloc((size_t)M->n_lchunks * r);
+        double* lcp = scratch_or<double>(M, SK_LCPART, M->lc_part, (size_t)std::max<int64_t>(1, M->n_lchunks) * r);
         if (M->n_lchunks > 0)
             k_csr_spmm_chunks<T, T><<<dim3((unsigned)M->n_lchunks, (unsigned)r), 256, 0, st>>>(
-            M->lc_beg.p, M->lc_end.p, M->n_lchunks, M->wc.p, wvals<T>(M), X, ld, M->lc_part.p);
+            M->lc_beg.p, M->lc_end.p, M->n_lchunks, M->wc.p, wvals<T>(M), X, ld, lcp);
         k_csr_chunks_finish<T><<<dim3(blocks(M->long_rows, 4), (unsigned)r), 256, 0, st>>>(
-            M->lr_start.p, M->long_rows, M->n_lchunks, M->lc_part.p, Y, ld);
+            M->lr_start.p, M->long_rows, M->n_lchunks, lcp, Y, ld);
     }
     if (M->dim > M->long_rows)
         timed(ctx, sizeof(T) == 8 ? "k_csr_spmm_f64" : "k_csr_spmm_f32", [&] {
@@ -1451,11 +1522,10 @@ void apply_C(cb_mme* M, double lambda, const T* X, T* Y, int r) {
     // lambda Q part; the FP32 copy of a dense Q holds Q - mu v v', and
     // lambda mu v (v' x_u) is added in the same pass that adds lambda Q x_u
     if (M->qkind == 0 && M->stiff && sizeof(T) == 4) {
-        if (M->stiff_part.n < (size_t)DOT_CHUNKS * r) M->stiff_part.alloc((size_t)DOT_CHUNKS * r);
-        k_stiff_dot<T><<<dim3(DOT_CHUNKS, (unsigned)r), 256, 0, st>>>(X + M->p, M->stiff_v.p, ld, M->q,
-                                                                    M->stiff_part.p);
+        double* sp = scratch_or<double>(M, SK_STIFF, M->stiff_part, (size_t)DOT_CHUNKS * r);
+        k_stiff_dot<T><<<dim3(DOT_CHUNKS, (unsigned)r), 256, 0, st>>>(X + M->p, M->stiff_v.p, ld, M->q, sp);
         CB_HIP(hipGetLastError());
-        const StiffTerm sk{M->stiff_v.p, M->stiff_part.p, lambda * M->stiff_mu};
+        const StiffTerm sk{M->stiff_v.p, sp, lambda * M->stiff_mu};
         q_apply<T>(M, X, ld, Y, ld, r, (T)lambda, &sk);
     } else {
         q_apply<T>(M, X, ld, Y, ld, r, (T)lambda);
@@ -1559,9 +1629,9 @@ void apply_precond_rest(cb_mme* M, const T* R, T* Z, int r) {
         k_precond_block<T><<<dim3(blocks(M->p, 4), (unsigned)r), 256, 0, st>>>(R, Z, M->Binv.p, M->p, M->ldv);
     if (M->stiff) {
         // z_u -= gamma (v' z_u) D'^-1 v
-        if (M->stiff_part.n < (size_t)DOT_CHUNKS * r) M->stiff_part.alloc((size_t)DOT_CHUNKS * r);
-        k_stiff_dot<T><<<dim3(DOT_CHUNKS, (unsigned)r), 256, 0, st>>>(Z + M->p, M->stiff_v.p, M->ldv, M->q, M->stiff_part.p);
-        k_stiff_axpy<T><<<dim3(blocks(M->q), (unsigned)r), 256, 0, st>>>(Z + M->p, M->stiff_y.p, M->stiff_part.p,
+        double* sp = scratch_or<double>(M, SK_STIFF, M->stiff_part, (size_t)DOT_CHUNKS * r);
+        k_stiff_dot<T><<<dim3(DOT_CHUNKS, (unsigned)r), 256, 0, st>>>(Z + M->p, M->stiff_v.p, M->ldv, M->q, sp);
+        k_stiff_axpy<T><<<dim3(blocks(M->q), (unsigned)r), 256, 0, st>>>(Z + M->p, M->stiff_y.p, sp,
                                                                         -M->stiff_gamma, M->ldv, M->q);
     }
     CB_HIP(hipGetLastError());
@@ -2456,6 +2526,7 @@ int cb_mme_create(cb_context* ctx, int64_t n, int64_t p, int64_t q, const int64_
         CB_REQUIRE(x_ptr[0] == 0 && x_ptr[n] >= 0 && (x_ptr[n] == 0 || (x_col && x_val)),
                    "cb_mme_create: bad fixed-effect CSR");
         set_device(ctx);
+        init_scratch(ctx->device);  // this device's scratch address (the allocation-free solve)
         auto* M = new cb_mme();
         M->ctx = ctx;
         M->n = n; M->p = p; M->q = q; M->dim = p + q;
@@ -2533,6 +2604,14 @@ static void find_stiff_direction(cb_mme* M) {
     CB_HIP(hipStreamSynchronize(st));
 }
 
+// The preconditioner's buffers at their fixed sizes (p x p, dim, q), so
+// that building it for a new lambda inside a solve allocates nothing.
+static void presize_precond(cb_mme* M) {
+    M->Binv.alloc((size_t)std::max<int64_t>(1, M->p * M->p));
+    M->dinv.alloc(M->dim);
+    if (M->stiff) M->stiff_y.alloc(M->q);
+}
+
 static void set_q_csr_common(cb_mme* M, const int64_t* ptr, const int32_t* col, const double* val) {
     set_device(M->ctx);
     const int64_t q = M->q, nnz = ptr[q];
@@ -2560,6 +2639,7 @@ static void set_q_csr_common(cb_mme* M, const int64_t* ptr, const int32_t* col, 
     M->Qd32.release();
     M->ng = 0;
     M->dinv_lambda = std::nan("");
+    presize_precond(M);
 }
 
 static void set_q_dense_common(cb_mme* M, const double* Q, int64_t ld, hipMemcpyKind kind) {
@@ -2588,6 +2668,7 @@ static void set_q_dense_common(cb_mme* M, const double* Q, int64_t ld, hipMemcpy
     M->qp.release(); M->qc.release(); M->qv64.release(); M->qv32.release();
     M->dinv_lambda = std::nan("");
     find_stiff_direction(M);
+    presize_precond(M);
 }
 
 int cb_mme_set_qinv_dense(cb_mme* M, const double* Q, int64_t ld) {
@@ -2806,6 +2887,73 @@ int cb_mme_solve_device(cb_mme* M, double lambda, const cb_solve_options* opts, 
                         int64_t ldb, double* dX, int64_t ldx, int64_t* iterations, double* rel_residual) {
     return mme_solve_blocks(M, lambda, opts, r, dB, ldb, dX, ldx, true, iterations, rel_residual,
                             "cb_mme_solve_device");
+}
+
+// Upper bound of the scratch bytes one solve of r columns takes (the
+// buffers solve_block and apply_C ask for, 256-byte aligned each).
+static size_t noalloc_need(const cb_mme* M, int64_t r, int precision) {
+    const size_t ld = (size_t)M->ldv, R = (size_t)r, a = 256;
+    const size_t vec = ld * R;
+    size_t need = 2 * (vec * 8 + a);  // the right-hand-side and solution blocks
+    need += (precision == CB_PREC_FP64 ? 4 * (vec * 8 + a) : 4 * (vec * 4 + a) + 2 * (vec * 8 + a));
+    need += 2 * ((size_t)DOT_CHUNKS * R * 8 + a) + 2 * (R * 8 + a) + R * sizeof(ColState) + a;
+    if (M->long_rows > 0) need += (size_t)std::max<int64_t>(1, M->n_lchunks) * R * 8 + a;
+    if (M->qkind == 0) {
+        if (M->stiff) need += (size_t)DOT_CHUNKS * R * 8 + a;
+        if (r > 2) {
+            const int rc = (int)std::min<int64_t>(r, MF_MAXNT * 16);
+            const DenseShape d8 = dense_shape(8, M->q, rc), d4 = dense_shape(4, M->q, rc);
+            need += (size_t)d8.nsplit * rc * d8.ldp * 8 + a;
+            if (precision != CB_PREC_FP64) need += (size_t)d4.nsplit * rc * d4.ldp * 4 + a;
+        }
+    }
+    return need;
+}
+
+// cb_mme_solve_device with its scratch in this device's fixed scratch array
+// instead of the model's growing buffers: no device allocation or release.
+int cb_mme_solve_device_noalloc(cb_mme* M, double lambda, const cb_solve_options* opts, int64_t r,
+                                const double* dB, int64_t ldb, double* dX, int64_t ldx, int64_t* iterations,
+                                double* rel_residual) {
+    cb_solve_options o;
+    cb_solve_options_default(&o);
+    if (opts) o = *opts;
+    ScratchArena arena;
+    bool graphs = false;
+    const int st = guarded([&] {
+        CB_REQUIRE(M && dB && dX && r > 0, "cb_mme_solve_device_noalloc: bad argument");
+        CB_REQUIRE(M->qkind == 0 || M->qkind == 1,
+                   "cb_mme_solve_device_noalloc: dense or CSR Qinv only (the split form gathers into growing buffers)");
+        CB_REQUIRE(o.precision == CB_PREC_FP64 || o.precision == CB_PREC_MIXED,
+                   "cb_mme_solve_device_noalloc: CB_PREC_FP64 or CB_PREC_MIXED only");
+        CB_REQUIRE(M->dinv.p && M->Binv.p && (!M->stiff || M->stiff_y.p),
+                   "cb_mme_solve_device_noalloc: preconditioner buffers missing (set Qinv first)");
+        unsigned char* base = scratch_ptr(M->ctx->device);
+        CB_REQUIRE(base, "cb_mme_solve_device_noalloc: no scratch on this device");
+        const size_t need = noalloc_need(M, r, o.precision);
+        if (need > NOALLOC_SCRATCH_BYTES)
+            throw Error(CB_ERR_INVALID, "cb_mme_solve_device_noalloc: " + std::to_string(r) + " columns of " +
+                                            std::to_string(M->dim) + " unknowns need about " + std::to_string(need) +
+                                            " bytes of scratch, more than its " +
+                                            std::to_string(NOALLOC_SCRATCH_BYTES) + "; use cb_mme_solve_device");
+        arena.base = reinterpret_cast<char*>(base);
+        arena.cap = NOALLOC_SCRATCH_BYTES;
+        graphs = M->ctx->graphs;
+    });
+    if (st != CB_OK) return st;
+    // scratch from the arena; no graph capture (instantiation may allocate)
+    struct Scope {
+        cb_mme* M;
+        bool graphs;
+        ~Scope() {
+            M->arena = nullptr;
+            M->ctx->graphs = graphs;
+        }
+    } scope{M, graphs};
+    M->arena = &arena;
+    M->ctx->graphs = false;
+    return mme_solve_blocks(M, lambda, &o, r, dB, ldb, dX, ldx, true, iterations, rel_residual,
+                            "cb_mme_solve_device_noalloc");
 }
 
 // GSSolve's fixed-effect standard errors (mixed_model_solver.cpp:346-417):

@@ -132,6 +132,17 @@ class MixedModel:
                                            C.byref(it), C.byref(rel)))
         return X.T.reshape(np.asarray(B).shape), it.value, rel.value
 
+    def solve_device_noalloc(self, lam: float, dB: int, ldb: int, dX: int, ldx: int, r: int,
+                             opts: _lib.SolveOptions | None = None):
+        """solve_device with the scratch in the library's fixed device array
+        (no device allocation); returns (iterations, max relative residual)."""
+        opts = opts or solve_options()
+        it = C.c_int64()
+        rel = C.c_double()
+        _lib.check(_lib.lib().cb_mme_solve_device_noalloc(self.handle, lam, C.byref(opts), r, C.c_void_p(dB), ldb,
+                                                          C.c_void_p(dX), ldx, C.byref(it), C.byref(rel)))
+        return it.value, rel.value
+
     def beta_cov(self, lam: float, var_e: float, opts: _lib.SolveOptions | None = None):
         """GSSolve's fixed-effect SEs (mixed_model_solver.cpp:346-417):
         returns (Cinv_beta p x p, beta_se p)."""

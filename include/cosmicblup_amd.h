@@ -231,6 +231,20 @@ int cb_mme_solve_device(cb_mme* mme, double lambda, const cb_solve_options* opts
                         const double* dB, int64_t ldb, double* dX, int64_t ldx, int64_t* iterations,
                         double* rel_residual);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): cb_mme_solve_device with its scratch (PCG vectors,
+ * right-hand-side and solution blocks, dot and split-K partials) taken from
+ * a fixed 64 MiB array on the device instead of the model's growing
+ * buffers; it calls no hipMalloc or hipFree. Dense or CSR Qinv,
+ * CB_PREC_FP64 or CB_PREC_MIXED; sizes whose scratch would exceed the array
+ * (about r * (p + q) * 96 bytes plus partials) return CB_ERR_INVALID. The
+ * array is shared by every call on one GPU and keeps the last call's
+ * contents: calls that overlap on one GPU (several contexts or threads)
+ * race. Results equal cb_mme_solve_device's. */
+int cb_mme_solve_device_noalloc(cb_mme* mme, double lambda, const cb_solve_options* opts, int64_t r,
+                                const double* dB, int64_t ldb, double* dX, int64_t ldx, int64_t* iterations,
+                                double* rel_residual);
 /* GSSolve's fixed-effect covariance and standard errors (calc_se, on by
  * default: config.h:45; mixed_model_solver.cpp:346-417): C(lambda) x = e_i
  * for each fixed effect i < p, solved as blocks of unit right-hand sides;
