@@ -21,6 +21,7 @@
 //   k_miss_*      missing-call lists; E with F folded in (dot4 + rare-path LDS)
 //   k_gram        INT8 MFMA Gram over lower-triangle 256x256 tiles with the
 //                 FP64 centring/scaling fused into the epilogue, MFMA-bound
+#include <algorithm>
 #include <cmath>
 #include <numeric>
 
@@ -35,6 +36,7 @@ constexpr int TILE = 256;        // samples per Gram tile side
 constexpr int STAGE_SNPS = 256;  // SNPs per LDS stage (16 packed words)
 constexpr int LDS_PITCH = 20;    // words per LDS row (16 + 4 pad: conflict-free b128 reads)
 constexpr double TWO_M29 = 1.0 / 536870912.0;  // 2^-29
+constexpr double TWO_M52 = 1.0 / 4503599627370496.0;  // 2^-52
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
@@ -273,235 +275,107 @@ __global__ __launch_bounds__(256) void k_sample_sq(const int64_t* __restrict__ s
     }
 }
 
-// Sample lists regrouped for k_miss_cross_h: every 4 consecutive entries
-// (lists are padded to a multiple of 8) become 8 words, the 4 SNP indices
-// and the 4 byte-chunk words of their K (word c, byte t = byte c of K_t),
-// the operand layout of v_dot4 against 4 SNPs' dosages.
-__global__ void k_miss_groups(const int32_t* __restrict__ lj, const uint32_t* __restrict__ lk, int64_t ngroups,
-                              uint32_t* __restrict__ lg) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups) return;
-    uint32_t K[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        lg[g * 8 + t] = (uint32_t)lj[4 * g + t];
-        K[t] = lk[4 * g + t];
-    }
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-        lg[g * 8 + 4 + c] = ((K[0] >> (8 * c)) & 0xffu) | (((K[1] >> (8 * c)) & 0xffu) << 8) |
-                            (((K[2] >> (8 * c)) & 0xffu) << 16) | (((K[3] >> (8 * c)) & 0xffu) << 24);
-}
-
-// Rows of the BED re-encoded for k_miss_cross: each 2-bit field holds the
-// dosage (0, 1, 2), or 3 for a missing call. Output rows are padded to
-// out_pitch (a multiple of 128 bytes, zero-filled) so that a wave's 128-byte
-// slice of a row is one cache line.
-__global__ void k_encode_d3(const uint8_t* __restrict__ bed, uint8_t* __restrict__ out, int64_t pitch,
-                            int64_t out_pitch, int64_t n, int64_t m) {
-    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // dword within the row
-    const int64_t j = blockIdx.y;
-    if (w * 4 >= out_pitch || j >= m) return;
-    uint32_t v = 0;
-    if (w * 4 < pitch) {
-        const uint32_t x = reinterpret_cast<const uint32_t*>(bed + j * pitch)[w];
-        const uint32_t vm = field_mask(n, w);
-        const uint32_t miss = ~(x >> 1) & x & vm;
-        const uint32_t d = dosage_swar(x) & (vm | (vm << 1));
-        v = d | miss | (miss << 1);
-    }
-    reinterpret_cast<uint32_t*>(out + j * out_pitch)[w] = v;
-}
-
-// Both-missing event sums are exact integers in 2^-48 units. Each term is
-// split into its low 32 bits and its signed rest, summed in two u64 slots,
-// so that no number of events overflows; ev_value() scales the total. While
-// the total fits in 63 bits (all but extreme cases) it is converted once,
-// exactly as a single 64-bit sum would be.
-__device__ __forceinline__ void ev_add(unsigned long long* lo, unsigned long long* hi, unsigned long long add) {
-    atomicAdd(lo, add & 0xffffffffull);
-    atomicAdd(hi, (unsigned long long)((long long)add >> 32));
-}
-__device__ __forceinline__ double ev_value(unsigned long long lo, unsigned long long hi) {
-    const long long h = (long long)hi;
-    if (h > -(1ll << 30) && h < (1ll << 30))
-        return (double)(long long)(((unsigned long long)h << 32) + lo) * (1.0 / 281474976710656.0);
-    return ((double)h * 4294967296.0 + (double)lo) * (1.0 / 281474976710656.0);
-}
-
-// E for the 1024 samples i of a block and the rows k of this wave, with
-// E(k,i) + E(i,k) = e(k,i) + e(i,k) + F(k,i) as the Gram epilogue needs,
-// e(k,i) = sum_{j miss k} mu_j X_ij and F(k,i) = sum_{j miss k and i} mu_j^2.
-// Each lane owns 16 consecutive samples = one dword of a row of the
-// re-encoded BED (field = dosage, or 3 for missing). Four list entries
-// (SNPs of miss(k)) are consumed per step: the four row dwords are
-// byte-transposed with v_perm so that W_q holds samples 4q..4q+3 of all
-// four SNPs; sample s of W_q is selected in place with 0x03030303 << 2s and
-// dotted (v_dot4_u32_u8) with the four byte chunks of K_j — exact integers
-// that carry a factor 4^s, shifted out at the end.
-// A call missing in both k and i entered the dot as 3 K_j. Such events
-// are rare (the missing rate squared) and symmetric in (k, i), so each is
-// taken once, in row k for i > k, adding mu_j^2 - 6 mu_j (signed 2^-48
-// units, lane-private LDS). The diagonal is set directly: E(k,k) = s_k/2.
-// Every element E(k, i < n) is written (this initialises E).
-__global__ __launch_bounds__(256) void k_miss_cross(const uint8_t* __restrict__ bedd,
-                                                    int64_t pitch, int64_t n,
-                                                    const int64_t* __restrict__ smp_off,
-                                                    const int32_t* __restrict__ lj,
-                                                    const uint32_t* __restrict__ lk,
-                                                    const double* __restrict__ sqd, int kw,
-                                                    double* __restrict__ E, int64_t lde) {
-    __shared__ unsigned long long fl[4 * 2 * 16 * 64];  // [wave][lo, hi][sample][lane], 64 KB
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int64_t I0 = (int64_t)blockIdx.y * 1024;
-    const int64_t dw = I0 / 16 + lane;  // dword of each SNP row owned by this lane
-    const int64_t i_lane = I0 + 16 * lane;
-    const int64_t dwc = dw * 4 < pitch ? dw : 0;  // clamped in-row dword
-    unsigned long long* myf = fl + wave * 2 * 16 * 64 + lane;  // lo slots; hi slots at + 16 * 64
-    const int per_wave = kw / 4;
-    for (int kq = 0; kq < per_wave; ++kq) {
-        const int64_t k = (int64_t)blockIdx.x * kw + wave * per_wave + kq;
-        if (k >= n) break;
-        // event masks: field positions (2s + 8t) of W_q whose sample index exceeds k
-        uint32_t mk[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            uint32_t mq = 0;
-#pragma unroll
-            for (int s = 0; s < 4; ++s)
-                if (i_lane + 4 * q + s > k) mq |= 0x01010101u << (2 * s);
-            mk[q] = mq;
-        }
-#pragma unroll
-        for (int s = 0; s < 32; ++s) myf[s * 64] = 0ull;
-        uint32_t acc[16][4];
-#pragma unroll
-        for (int s = 0; s < 16; ++s)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) acc[s][c] = 0;
-        const int64_t b = smp_off[k], e = smp_off[k + 1];
-        // One step = two groups of four list entries (lists are padded to a
-        // multiple of 8). Lanes past the row read an in-row dword instead;
-        // their samples are >= n and never written.
-        auto fetch = [&](int64_t pp, uint32_t (&c)[8]) {
-#pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const int32_t jj = lj[pp + t];
-                c[t] = reinterpret_cast<const uint32_t*>(bedd + (int64_t)jj * pitch)[dwc];
-            }
-        };
-        auto step = [&](int64_t pp, const uint32_t (&cw)[8]) {
-#pragma unroll
-            for (int g = 0; g < 2; ++g) {
-                const uint32_t c0 = cw[4 * g], c1 = cw[4 * g + 1], c2 = cw[4 * g + 2], c3 = cw[4 * g + 3];
-                const uint32_t K0 = lk[pp + 4 * g], K1 = lk[pp + 4 * g + 1], K2 = lk[pp + 4 * g + 2],
-                               K3 = lk[pp + 4 * g + 3];
-                // 4x4 byte transpose: W[q] byte t = byte q of c_t
-                const uint32_t ta = __builtin_amdgcn_perm(c1, c0, 0x05010400u);
-                const uint32_t tb = __builtin_amdgcn_perm(c1, c0, 0x07030602u);
-                const uint32_t tc = __builtin_amdgcn_perm(c3, c2, 0x05010400u);
-                const uint32_t td = __builtin_amdgcn_perm(c3, c2, 0x07030602u);
-                uint32_t W[4];
-                W[0] = __builtin_amdgcn_perm(tc, ta, 0x05040100u);
-                W[1] = __builtin_amdgcn_perm(tc, ta, 0x07060302u);
-                W[2] = __builtin_amdgcn_perm(td, tb, 0x05040100u);
-                W[3] = __builtin_amdgcn_perm(td, tb, 0x07060302u);
-                uint32_t kc[4];
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    kc[c] = ((K0 >> (8 * c)) & 0xffu) | (((K1 >> (8 * c)) & 0xffu) << 8) |
-                            (((K2 >> (8 * c)) & 0xffu) << 16) | (((K3 >> (8 * c)) & 0xffu) << 24);
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-#pragma unroll
-                    for (int s = 0; s < 4; ++s) {
-                        const uint32_t A = W[q] & (0x03030303u << (2 * s));
-#pragma unroll
-                        for (int c = 0; c < 4; ++c)
-                            acc[4 * q + s][c] = __builtin_amdgcn_udot4(A, kc[c], acc[4 * q + s][c], false);
-                    }
-                // events: field 3 (missing in both) at a sample > k; q = 0,1 in
-                // the even/odd bits of ev0, q = 2,3 in those of ev1
-                uint32_t ev0 = (W[0] & (W[0] >> 1) & mk[0]) | ((W[1] & (W[1] >> 1) & mk[1]) << 1);
-                uint32_t ev1 = (W[2] & (W[2] >> 1) & mk[2]) | ((W[3] & (W[3] >> 1) & mk[3]) << 1);
-                while (ev0 | ev1) {
-                    const bool lo = ev0 != 0;
-                    const uint32_t ev = lo ? ev0 : ev1;
-                    const int bit = __ffs(ev) - 1;
-                    if (lo) ev0 &= ev0 - 1; else ev1 &= ev1 - 1;
-                    const int t = bit >> 3, smp = (lo ? 0 : 8) + 4 * (bit & 1) + ((bit >> 1) & 3);
-                    const uint32_t Kt = t == 0 ? K0 : t == 1 ? K1 : t == 2 ? K2 : K3;
-                    // mu^2 - 6 mu in 2^-48 units (two's complement, split by ev_add)
-                    const unsigned long long add =
-                        (((unsigned long long)Kt * Kt) >> 10) - ((unsigned long long)Kt * 6 << 19);
-                    ev_add(&myf[smp * 64], &myf[(16 + smp) * 64], add);
-                }
-            }
-        };
-        // accumulator folds as in k_miss_cross_h, every 4096 16-entry iterations
-        bool folded = false;
-        auto fold = [&]() {
-#pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                const uint64_t t = (uint64_t)acc[s][0] + ((uint64_t)acc[s][1] << 8) +
-                                   ((uint64_t)acc[s][2] << 16) + ((uint64_t)acc[s][3] << 24);
-#pragma unroll
-                for (int c = 0; c < 4; ++c) acc[s][c] = 0;
-                if (i_lane + s < n) {
-                    uint64_t* slot = reinterpret_cast<uint64_t*>(E + k * lde + i_lane + s);
-                    *slot = (folded ? *slot : 0) + t;
-                }
-            }
-            folded = true;
-        };
-        int it = 0;
-        uint32_t ra[8], rb[8];
-        if (b < e) fetch(b, ra);
-        for (int64_t p = b; p < e; p += 16) {
-            if (++it == 4096) {
-                fold();
-                it = 0;
-            }
-            const bool m1 = p + 8 < e;
-            if (m1) fetch(p + 8, rb);
-            step(p, ra);
-            if (!m1) break;
-            if (p + 16 < e) fetch(p + 16, ra);
-            step(p + 8, rb);
-        }
-#pragma unroll
-        for (int s = 0; s < 16; ++s) {
-            const int64_t i = i_lane + s;
-            if (i >= n) continue;
-            double v;
-            if (i == k) {
-                v = sqd[k] * 0.5;  // s_k / 2
-            } else {
-                const uint64_t part = folded ? *reinterpret_cast<const uint64_t*>(E + k * lde + i) : 0;
-                const uint64_t T = (part + (uint64_t)acc[s][0] + ((uint64_t)acc[s][1] << 8) +
-                                    ((uint64_t)acc[s][2] << 16) + ((uint64_t)acc[s][3] << 24)) >> (2 * (s & 3));
-                v = (double)T * TWO_M29 + ev_value(myf[s * 64], myf[(16 + s) * 64]);
-            }
-            E[k * lde + i] = v;
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------
-// The same E with two lanes per row dword: lane 2w + h takes samples
-// 8h .. 8h + 7 of dword w (W_q for q = 2h, 2h + 1), so a wave covers 512
-// samples and each lane keeps 32 accumulators instead of 64 — twice the
-// resident waves to hide the row-gather latency, the same VALU work per
-// (k, i, j) term. The list comes regrouped (k_miss_groups), so the K byte
-// chunks are loaded, not assembled per step. Arithmetic as in k_miss_cross,
-// except that both-missing events are not handled inside the dot loop: a
-// lane with events (about one in four per group) queues one record, and the
-// wave applies 64 events at a time, one per lane, instead of looping as long
-// as its busiest lane has events. Row gathers run two pairs of groups ahead
-// of the dot loop to cover their latency (the rows are random SNPs).
-constexpr int MC_I = 512;    // samples per block column
-constexpr int MC_RING = 256; // event records per wave (power of two)
-constexpr int MC_FOLD_ITERS = 2048;  // 24-entry iterations per accumulator fold (< 2^32 / (24 * 48960))
+// Missing-call correction on INT8 MFMA (k_miss_mfma).
+//
+// E(k, i) = e(k, i) + F(k, i) / 2 with
+//   e(k, i) = sum_{j in miss(k)} mu_j X_ij     (X = 0 where i is missing)
+//   F(k, i) = sum_{j in miss(k) and miss(i)} mu_j^2.
+// With mu_j = K_j 2^-29 (K_j < 2^30 an integer), e is the exact integer
+// sum_j K_j X_ij and F the exact integer sum_j K_j^2 (units 2^-58). Written
+// in signed base-256 digits d_c in [-128, 127] (K = sum_c d_c 256^c: the
+// bytes of (K + 0x80808080) ^ 0x80808080; K^2 likewise with 8 digits), both
+// over a group of four samples k are int8 matrix products: A (16 rows = 4
+// samples x 4 digits, columns = the group's list entries, row (kl, c)
+// holding d_c of the entries of sample kl and 0 elsewhere) times B (64
+// entries x 256 samples i, from the entries' SNP rows: the dosage for e, the
+// missing-call indicator for F, with digits 0-3 and 4-7 of K^2 in two A
+// tiles), 64 entries per v_mfma_i32_16x16x64_i8. Row (kl, c) of the 16x16
+// output is element c of lane 16 kl + i's accumulator, so the digits
+// combine in the lane: sum_c acc_c 256^c, exactly, in int64.
+// Only E(i, k) + E(k, i) is consumed, so F is added once per pair: in full
+// to E(k, i) when i's I-block follows k's, halved to both when they share
+// one, and not at all when it precedes.
+constexpr int MB_SAMPLES = 256;  // samples per I-block
+constexpr int MB_STEP = 64;      // list entries per K-step
+constexpr int MB_GROUP = 4;      // samples k per group (4 x 4 digit rows)
+constexpr int MB_KBLK = 16;      // entries per k-block; a sample's list is padded to whole k-blocks
+constexpr int MB_PF = 2;         // K-steps of row gathers in flight; group steps are padded to a multiple
+constexpr int MB_CHUNK = 24;     // K-steps of one group staged in LDS at once (a multiple of MB_PF)
+constexpr int MB_QP = 20;        // LDS pitch (dwords) of one sample quad of B
+constexpr int MB_XCD = 8;        // XCDs; workgroups are dealt to them round-robin
+
+// SNP rows in I-block-major order: block b (samples 256 b .. 256 b + 255)
+// of SNP j at dwords (b m + j) 16 .. + 15, dosage 0/1/2 per 2-bit field,
+// 3 for a missing call (PLINK 01), 0 for samples past n.
+__global__ void k_encode_iblock(const uint8_t* __restrict__ bed, int64_t pitch, int64_t n, int64_t m, int64_t nb,
+                                uint32_t* __restrict__ out) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // dword of the row
+    if (w >= nb * 16) return;
+    for (int64_t j = blockIdx.y; j < m; j += gridDim.y) {
+        uint32_t v = 0;
+        if (w * 4 < pitch) {
+            const uint32_t x = reinterpret_cast<const uint32_t*>(bed + j * pitch)[w];
+            const uint32_t vm = field_mask(n, w);
+            const uint32_t miss = x & ~(x >> 1) & 0x55555555u;
+            v = (dosage_swar(x) | miss | (miss << 1)) & (vm | (vm << 1));
+        }
+        out[((w >> 4) * m + j) * 16 + (w & 15)] = v;
+    }
+}
+
+// The group lists: the entries of samples 4g .. 4g + 3 one after another,
+// each sample's padded with (j = 0, K = 0) to whole k-blocks, the group's to
+// whole K-steps (a multiple of MB_PF). Per K-step: gj = the 64 SNPs, gK =
+// their K, gkl = the sample kl of each of the 4 k-blocks. One block per
+// group.
+__global__ __launch_bounds__(256) void k_group_entries(const int64_t* __restrict__ smp_off,
+                                                       const int32_t* __restrict__ smp_cnt,
+                                                       const int32_t* __restrict__ lj,
+                                                       const uint32_t* __restrict__ lk, int64_t n,
+                                                       const int64_t* __restrict__ gstep, int32_t* __restrict__ gj,
+                                                       uint32_t* __restrict__ gK, uint8_t* __restrict__ gkl) {
+    const int64_t g = blockIdx.x;
+    int64_t c[MB_GROUP + 1], cnt[MB_GROUP];
+    c[0] = 0;
+#pragma unroll
+    for (int kl = 0; kl < MB_GROUP; ++kl) {
+        const int64_t k = g * MB_GROUP + kl;
+        cnt[kl] = k < n ? smp_cnt[k] : 0;
+        c[kl + 1] = c[kl] + (cnt[kl] + MB_KBLK - 1) / MB_KBLK * MB_KBLK;
+    }
+    const int64_t base = gstep[g] * MB_STEP, len = (gstep[g + 1] - gstep[g]) * MB_STEP;
+    for (int64_t e = threadIdx.x; e < len; e += blockDim.x) {
+        int32_t j = 0;
+        uint32_t K = 0;
+        int kl = 0;
+        if (e < c[MB_GROUP]) {
+            while (e >= c[kl + 1]) ++kl;
+            const int64_t r = e - c[kl];
+            if (r < cnt[kl]) {
+                const int64_t p = smp_off[g * MB_GROUP + kl] + r;
+                j = lj[p];
+                K = lk[p];
+            }
+        }
+        gj[base + e] = j;
+        gK[base + e] = K;
+        if (e % MB_KBLK == 0) gkl[(base + e) / MB_KBLK] = (uint8_t)kl;
+    }
+}
+
+// 4 x 4 byte transpose: byte e of o[c] = byte c of d[e].
+__device__ __forceinline__ void transpose4x4(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3, uint32_t (&o)[4]) {
+    const uint32_t ta = __builtin_amdgcn_perm(d1, d0, 0x05010400u);
+    const uint32_t tb = __builtin_amdgcn_perm(d1, d0, 0x07030602u);
+    const uint32_t tc = __builtin_amdgcn_perm(d3, d2, 0x05010400u);
+    const uint32_t td = __builtin_amdgcn_perm(d3, d2, 0x07030602u);
+    o[0] = __builtin_amdgcn_perm(tc, ta, 0x05040100u);
+    o[1] = __builtin_amdgcn_perm(tc, ta, 0x07060302u);
+    o[2] = __builtin_amdgcn_perm(td, tb, 0x05040100u);
+    o[3] = __builtin_amdgcn_perm(td, tb, 0x07060302u);
+}
+
 // Orders one wave's LDS accesses across its lanes (hardware keeps a wave's
 // LDS operations in order; this keeps the compiler from moving them).
 __device__ __forceinline__ void wave_sync() {
@@ -509,192 +383,186 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_miss_cross_h(const uint8_t* __restrict__ bedd, int64_t pitch, int64_t n,
-                                                      int64_t bed_bytes,
-                                                      const int64_t* __restrict__ smp_off,
-                                                      const uint32_t* __restrict__ lg,
-                                                      const uint32_t* __restrict__ lk,
-                                                      const double* __restrict__ sqd, int kw,
-                                                      double* __restrict__ E, int64_t lde) {
-    __shared__ unsigned long long fl[4 * 2 * 8 * 64];  // [wave][lo, hi][sample][lane], 32 KB
-    __shared__ uint2 rings[4 * MC_RING];           // [wave][record], 8 KB
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int h = lane & 1;
-    const int64_t I0 = (int64_t)blockIdx.y * MC_I;
-    const int64_t dw = I0 / 16 + (lane >> 1);  // dword of each SNP row shared by this lane pair
-    const int64_t i_lane = I0 + 16 * (lane >> 1) + 8 * h;
-    const int64_t dwc = dw * 4 < pitch ? dw : 0;  // clamped in-row dword
-    const uint32_t voff = (uint32_t)(dwc * 4);
-    // raw buffer over the re-encoded rows (pitch * m < 2^31 bytes, checked by the host);
-    // 0x00020000: the CDNA buffer-descriptor word 3 for untyped dword access
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)bedd, (short)0, (int)bed_bytes,
-                                                                       0x00020000);
-    unsigned long long* wf = fl + wave * 2 * 8 * 64;  // lo slots; hi slots at + 8 * 64
-    unsigned long long* myf = wf + lane;
-    uint2* ring = rings + wave * MC_RING;
-    // byte selectors of the first transpose stage for this half
-    const uint32_t sel = h ? 0x07030602u : 0x05010400u;
-    const int per_wave = kw / 4;
-    for (int kq = 0; kq < per_wave; ++kq) {
-        const int64_t k = (int64_t)blockIdx.x * kw + wave * per_wave + kq;
-        if (k >= n) break;
-        uint32_t mk[2];
+
+// The 4 B fragments of one K-step (one per sample of the quad, values x 1,
+// 4, 16, 4) from the quad's 16 entry bytes: dosages (a missing call, field
+// 3, reads 0) and, for F, the missing-call indicators.
+__device__ __forceinline__ void b_fragments(const uint4 w4, v4i (&bd)[4], v4i (&bm)[4], bool want_m) {
+    const uint32_t x[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            uint32_t mq = 0;
-#pragma unroll
-            for (int s2 = 0; s2 < 4; ++s2)
-                if (i_lane + 4 * q + s2 > k) mq |= 0x01010101u << (2 * s2);
-            mk[q] = mq;
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t mq = x[q] & (x[q] >> 1) & 0x55555555u;  // 1 at the low bit of each missing field
+        const uint32_t cl = x[q] & ~(mq | (mq << 1));          // missing -> 0
+        bd[0][q] = (int)(cl & 0x03030303u);
+        bd[1][q] = (int)(cl & 0x0c0c0c0cu);
+        bd[2][q] = (int)(cl & 0x30303030u);
+        bd[3][q] = (int)((cl >> 4) & 0x0c0c0c0cu);
+        if (want_m) {
+            bm[0][q] = (int)(mq & 0x01010101u);
+            bm[1][q] = (int)(mq & 0x04040404u);
+            bm[2][q] = (int)(mq & 0x10101010u);
+            bm[3][q] = (int)((mq >> 4) & 0x04040404u);
         }
+    }
+}
+
+// blockIdx.x = b P + p: I-block b (samples 256 b .. + 255) and group range p
+// (groups gp[p] .. gp[p + 1] - 1, about equal in K-steps). With P a multiple
+// of MB_XCD every XCD holds P / MB_XCD consecutive workgroups of each
+// I-block, so the workgroups resident on an XCD gather from the same
+// m x 64 bytes of row segments, which stay in its L2. 256 threads; per group:
+//   stage    per chunk of MB_CHUNK K-steps: the SNPs and k-block samples,
+//            and the digit planes of K (4) and, if F is added here, of K^2
+//            (8), formed from gK (plane c, byte t = digit c of entry t);
+//   gather   thread t loads dword t % 16 (16 samples) of the rows of entries
+//            4 (t / 16) .. + 3 (a wave's load: 64 bytes of 4 rows; MB_PF
+//            steps in flight) and transposes the 4 x 4 bytes, giving per
+//            sample quad a dword of 4 entries x 4 samples, stored in LDS
+//            (B staging, double-buffered, one barrier per step; the dword
+//            groups of a quad are XOR-swizzled so that the stores of a wave
+//            fall in distinct banks);
+//   product  lane l of wave w takes quad Q = 16 w + l % 16 and k-block
+//            l / 16: one 16-byte LDS read gives its 16 entries' bytes, from
+//            which the 4 samples' dosages (and missing-call indicators) are
+//            masked out, exact scales removed at the end; 4 MFMAs for e and
+//            8 for F. The A fragment is plane c of the k-block if the
+//            k-block is sample kl's, else 0, for the lane's row (kl, c);
+//   write    E(k, i) for the group's samples k < n and the I-block's i < n;
+//            E(k, k) = s_k / 2 from sqd.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_miss_mfma(const uint32_t* __restrict__ bedi, int64_t m, int64_t n,
+                                                   int64_t P, const int32_t* __restrict__ gp,
+                                                   const int64_t* __restrict__ gstep, const int32_t* __restrict__ gj,
+                                                   const uint32_t* __restrict__ gK, const uint8_t* __restrict__ gkl,
+                                                   const double* __restrict__ sqd, double* __restrict__ E,
+                                                   int64_t lde) {
+    __shared__ __attribute__((aligned(16))) uint32_t sP[MB_CHUNK * 64];                // K planes, 6 KB
+    __shared__ __attribute__((aligned(16))) uint32_t sV[MB_CHUNK * 128];               // K^2 planes, 12 KB
+    __shared__ __attribute__((aligned(16))) int32_t sJ[MB_CHUNK * MB_STEP];            // 6 KB
+    __shared__ __attribute__((aligned(16))) uint8_t sKL[MB_CHUNK * 4];
+    __shared__ __attribute__((aligned(16))) uint32_t sB[2][(MB_SAMPLES / 4) * MB_QP];  // 10 KB
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t b = blockIdx.x / P, p = blockIdx.x % P;
+    const int64_t I0 = b * MB_SAMPLES;
+    const int kl_out = lane >> 4;                // output rows (kl_out, c) of this lane
+    const int Qf = wave * 16 + (lane & 15);      // sample quad of this lane's B fragments
+    const int kb = lane >> 4;                    // k-block of this lane's fragments
+    const int kl_a = (lane & 15) >> 2, c_a = lane & 3;  // A row (kl, c) of this lane
+    const int rg = tid >> 4, dc = tid & 15;      // gather task: entries 4 rg .. + 3, dword dc
+    // B staging: dword group h (entries 16 h .. + 15) of quad Q at Q QP + 4 (h ^ ((Q >> 4) & 3))
+    uint32_t* const bst0 = sB[0] + 4 * dc * MB_QP + 4 * ((rg >> 2) ^ (dc >> 2)) + (rg & 3);
+    const int brd = Qf * MB_QP + 4 * (kb ^ wave);
+    // rows of I-block b through a buffer resource (m * 64 bytes < 2^31)
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(bedi + b * m * 16), (short)0, (int)(m * 64), 0x00020000);
+    const int64_t g1 = gp[p + 1];
+    for (int64_t g = gp[p]; g < g1; ++g) {
+        const int64_t step0 = gstep[g], nsteps = gstep[g + 1] - step0;
+        const int64_t bk = (g * MB_GROUP) / MB_SAMPLES;
+        const int fm = b > bk ? 2 : b == bk ? 1 : 0;  // F in full, halved, or not at all
+        v4i acc[4], af0[4], af1[4];
 #pragma unroll
-        for (int s2 = 0; s2 < 16; ++s2) myf[s2 * 64] = 0ull;
-        wave_sync();
-        uint32_t acc[8][4];
+        for (int s_ = 0; s_ < 4; ++s_) acc[s_] = af0[s_] = af1[s_] = (v4i){0, 0, 0, 0};
+        for (int64_t c0 = 0; c0 < nsteps; c0 += MB_CHUNK) {
+            const int csteps = (int)(nsteps - c0 < MB_CHUNK ? nsteps - c0 : MB_CHUNK);
+            __syncthreads();  // the previous chunk's readers are done with the staging
+            {
+                const int64_t s0 = step0 + c0;
+                const int4* srcJ = reinterpret_cast<const int4*>(gj + s0 * MB_STEP);
+                const uint4* srcK = reinterpret_cast<const uint4*>(gK + s0 * MB_STEP);
+                for (int x = tid; x < csteps * 16; x += 256) {
+                    reinterpret_cast<int4*>(sJ)[x] = srcJ[x];
+                    const uint4 k4 = srcK[x];
+                    const int st = x >> 4, t4 = x & 15;  // entries 4 t4 .. + 3 of step st
+                    uint32_t o[4];
+                    transpose4x4((k4.x + 0x80808080u) ^ 0x80808080u, (k4.y + 0x80808080u) ^ 0x80808080u,
+                                 (k4.z + 0x80808080u) ^ 0x80808080u, (k4.w + 0x80808080u) ^ 0x80808080u, o);
 #pragma unroll
-        for (int s2 = 0; s2 < 8; ++s2)
+                    for (int c = 0; c < 4; ++c) sP[st * 64 + c * 16 + t4] = o[c];
+                    if (fm != 0) {
+                        const unsigned long long B8 = 0x8080808080808080ull;
+                        const unsigned long long v0 = ((unsigned long long)k4.x * k4.x + B8) ^ B8;
+                        const unsigned long long v1 = ((unsigned long long)k4.y * k4.y + B8) ^ B8;
+                        const unsigned long long v2 = ((unsigned long long)k4.z * k4.z + B8) ^ B8;
+                        const unsigned long long v3 = ((unsigned long long)k4.w * k4.w + B8) ^ B8;
+                        transpose4x4((uint32_t)v0, (uint32_t)v1, (uint32_t)v2, (uint32_t)v3, o);
 #pragma unroll
-            for (int c = 0; c < 4; ++c) acc[s2][c] = 0;
-        const int64_t b = smp_off[k], e = smp_off[k + 1];
-        // Both-missing events leave the dot loop as (bits, group, lane)
-        // records in a wave-private ring; full batches of 64 records are
-        // drained one event per lane, the rest of a record's bits re-queued.
-        // head and tail are wave-uniform; the ring holds at most 191 records
-        // before a drain and 255 during one, so a drain's reads and re-queue
-        // writes never share a slot.
-        uint32_t head = 0, tail = 0;
-        auto push = [&](uint32_t bits, uint32_t meta) {
-            const unsigned long long bal = __ballot(bits != 0);
-            if (bal == 0) return;
-            if (bits != 0) {
-                const uint32_t pos =
-                    tail + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                ring[pos & (MC_RING - 1)] = make_uint2(bits, meta);
-            }
-            tail += (uint32_t)__popcll(bal);
-        };
-        auto drain = [&](uint32_t cnt) {
-            wave_sync();
-            uint32_t bits = 0, meta = 0;
-            if ((uint32_t)lane < cnt) {
-                const uint2 r = ring[(head + lane) & (MC_RING - 1)];
-                bits = r.x;
-                meta = r.y;
-            }
-            head += cnt;
-            if (bits != 0) {
-                const int bit = __ffs(bits) - 1;
-                bits &= bits - 1;
-                const int t = bit >> 3, smp = 4 * (bit & 1) + ((bit >> 1) & 3);
-                const unsigned long long Kt = lk[b + 4 * (int64_t)(meta >> 6) + t];
-                ev_add(&wf[smp * 64 + (meta & 63)], &wf[(8 + smp) * 64 + (meta & 63)],
-                       ((Kt * Kt) >> 10) - (Kt * 6 << 19));
-            }
-            push(bits, meta);
-        };
-        // entries pp .. pp + 7 = groups pp/4, pp/4 + 1 of the regrouped list;
-        // row dword loads through a buffer resource: scalar row offset
-        // j * pitch, loop-invariant lane offset, no per-load address VALU
-        auto jwords = [&](int64_t pp, uint32_t (&jw)[8]) {
+                        for (int c = 0; c < 4; ++c) sV[st * 128 + c * 16 + t4] = o[c];
+                        transpose4x4((uint32_t)(v0 >> 32), (uint32_t)(v1 >> 32), (uint32_t)(v2 >> 32),
+                                     (uint32_t)(v3 >> 32), o);
 #pragma unroll
-            for (int t = 0; t < 8; ++t) jw[t] = lg[(pp / 4 + t / 4) * 8 + (t & 3)];
-        };
-        auto fetch = [&](const uint32_t (&jw)[8], uint32_t (&c)[8]) {
-#pragma unroll
-            for (int t = 0; t < 8; ++t)
-                c[t] = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, jw[t] * (uint32_t)pitch, 0);
-        };
-        auto step = [&](int64_t pp, const uint32_t (&cw)[8]) {
-#pragma unroll
-            for (int g = 0; g < 2; ++g) {
-                const uint32_t c0 = cw[4 * g], c1 = cw[4 * g + 1], c2 = cw[4 * g + 2], c3 = cw[4 * g + 3];
-                const uint32_t* grp = lg + (pp / 4 + g) * 8 + 4;
-                const uint32_t kc[4] = {grp[0], grp[1], grp[2], grp[3]};
-                const uint32_t t0 = __builtin_amdgcn_perm(c1, c0, sel);
-                const uint32_t t1 = __builtin_amdgcn_perm(c3, c2, sel);
-                uint32_t W[2];
-                W[0] = __builtin_amdgcn_perm(t1, t0, 0x05040100u);
-                W[1] = __builtin_amdgcn_perm(t1, t0, 0x07060302u);
-#pragma unroll
-                for (int q = 0; q < 2; ++q)
-#pragma unroll
-                    for (int s2 = 0; s2 < 4; ++s2) {
-                        const uint32_t A = W[q] & (0x03030303u << (2 * s2));
-#pragma unroll
-                        for (int c = 0; c < 4; ++c)
-                            acc[4 * q + s2][c] = __builtin_amdgcn_udot4(A, kc[c], acc[4 * q + s2][c], false);
+                        for (int c = 0; c < 4; ++c) sV[st * 128 + (4 + c) * 16 + t4] = o[c];
                     }
-                const uint32_t ev = (W[0] & (W[0] >> 1) & mk[0]) | ((W[1] & (W[1] >> 1) & mk[1]) << 1);
-                push(ev, ((uint32_t)((pp - b) / 4 + g) << 6) | (uint32_t)lane);
+                }
+                for (int x = tid; x < csteps * 4; x += 256) sKL[x] = gkl[s0 * 4 + x];
             }
-        };
-        // three row buffers: pair n is dotted while the rows of pair n + 2
-        // are in flight and the SNP indices of pair n + 3 load
-        uint32_t r0[8], r1[8], r2[8], jw[8];
-        if (b < e) {
-            jwords(b, jw);
-            fetch(jw, r0);
-        }
-        if (b + 8 < e) {
-            jwords(b + 8, jw);
-            fetch(jw, r1);
-        }
-        if (b + 16 < e) jwords(b + 16, jw);
-        // A long list would overflow the 32-bit accumulators (one entry adds
-        // at most 3 * 64 * 255 to each): every MC_FOLD_ITERS iterations the
-        // partial sums move, exactly, into this lane's own E slots as u64.
-        bool folded = false;
-        auto fold = [&]() {
+            __syncthreads();  // staged
+            // row dwords of MB_PF steps in flight: step st's sit in ring[st % MB_PF]
+            uint32_t ring[MB_PF][4];
+            auto fetch = [&](int st, uint32_t (&cw)[4]) {
+                const int4 jj = *reinterpret_cast<const int4*>(&sJ[st * MB_STEP + 4 * rg]);
+                cw[0] = __builtin_amdgcn_raw_buffer_load_b32(rs, (uint32_t)(jj.x * 64 + dc * 4), 0, 0);
+                cw[1] = __builtin_amdgcn_raw_buffer_load_b32(rs, (uint32_t)(jj.y * 64 + dc * 4), 0, 0);
+                cw[2] = __builtin_amdgcn_raw_buffer_load_b32(rs, (uint32_t)(jj.z * 64 + dc * 4), 0, 0);
+                cw[3] = __builtin_amdgcn_raw_buffer_load_b32(rs, (uint32_t)(jj.w * 64 + dc * 4), 0, 0);
+            };
 #pragma unroll
-            for (int s2 = 0; s2 < 8; ++s2) {
-                const uint64_t t = (uint64_t)acc[s2][0] + ((uint64_t)acc[s2][1] << 8) +
-                                   ((uint64_t)acc[s2][2] << 16) + ((uint64_t)acc[s2][3] << 24);
+            for (int u = 0; u < MB_PF; ++u) fetch(u, ring[u]);
+            for (int st0 = 0; st0 < csteps; st0 += MB_PF) {
 #pragma unroll
-                for (int c = 0; c < 4; ++c) acc[s2][c] = 0;
-                if (i_lane + s2 < n) {
-                    uint64_t* slot = reinterpret_cast<uint64_t*>(E + k * lde + i_lane + s2);
-                    *slot = (folded ? *slot : 0) + t;
+                for (int u = 0; u < MB_PF; ++u) {
+                    const int st = st0 + u;
+                    uint32_t (&cw)[4] = ring[u];
+                    uint32_t o[4];
+                    transpose4x4(cw[0], cw[1], cw[2], cw[3], o);
+                    uint32_t* dst = bst0 + (st & 1) * ((MB_SAMPLES / 4) * MB_QP);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) dst[q * MB_QP] = o[q];
+                    if (st + MB_PF < csteps) fetch(st + MB_PF, cw);
+                    __syncthreads();
+                    const uint4 w4 = *reinterpret_cast<const uint4*>(sB[st & 1] + brd);
+                    const bool on = (int)sKL[st * 4 + kb] == kl_a;
+                    const uint4 pk = *reinterpret_cast<const uint4*>(sP + st * 64 + c_a * 16 + kb * 4);
+                    const v4i ak = on ? (v4i){(int)pk.x, (int)pk.y, (int)pk.z, (int)pk.w} : (v4i){0, 0, 0, 0};
+                    v4i bd[4], bm[4];
+                    b_fragments(w4, bd, bm, fm != 0);
+#pragma unroll
+                    for (int s_ = 0; s_ < 4; ++s_) acc[s_] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ak, bd[s_], acc[s_], 0, 0, 0);
+                    if (fm != 0) {
+                        const uint4 p0 = *reinterpret_cast<const uint4*>(sV + st * 128 + c_a * 16 + kb * 4);
+                        const uint4 p1 = *reinterpret_cast<const uint4*>(sV + st * 128 + (4 + c_a) * 16 + kb * 4);
+                        const v4i a0 = on ? (v4i){(int)p0.x, (int)p0.y, (int)p0.z, (int)p0.w} : (v4i){0, 0, 0, 0};
+                        const v4i a1 = on ? (v4i){(int)p1.x, (int)p1.y, (int)p1.z, (int)p1.w} : (v4i){0, 0, 0, 0};
+#pragma unroll
+                        for (int s_ = 0; s_ < 4; ++s_) {
+                            af0[s_] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, bm[s_], af0[s_], 0, 0, 0);
+                            af1[s_] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, bm[s_], af1[s_], 0, 0, 0);
+                        }
+                    }
                 }
             }
-            folded = true;
-        };
-        int it = 0;
-        for (int64_t p = b; p < e; p += 24) {
-            if (++it == MC_FOLD_ITERS) {
-                fold();
-                it = 0;
-            }
-            if (p + 16 < e) fetch(jw, r2);
-            if (p + 24 < e) jwords(p + 24, jw);
-            step(p, r0);
-            while (tail - head >= 64) drain(64);
-            if (p + 8 >= e) break;
-            if (p + 24 < e) fetch(jw, r0);
-            if (p + 32 < e) jwords(p + 32, jw);
-            step(p + 8, r1);
-            while (tail - head >= 64) drain(64);
-            if (p + 16 >= e) break;
-            if (p + 32 < e) fetch(jw, r1);
-            if (p + 40 < e) jwords(p + 40, jw);
-            step(p + 16, r2);
-            while (tail - head >= 64) drain(64);
         }
-        while (tail != head) drain(tail - head < 64 ? tail - head : 64);
-        wave_sync();
+        // sum_c acc_c 256^c over the 4 digit rows, scale removed
+        const int64_t k_out = g * MB_GROUP + kl_out;
+        if (k_out < n) {
+            double* row = E + k_out * lde + I0;
+            const double fsc = fm == 2 ? 0x1p-58 : 0x1p-59;
 #pragma unroll
-        for (int s2 = 0; s2 < 8; ++s2) {
-            const int64_t i = i_lane + s2;
-            if (i >= n) continue;
-            double v;
-            if (i == k) {
-                v = sqd[k] * 0.5;  // s_k / 2
-            } else {
-                const uint64_t part = folded ? *reinterpret_cast<const uint64_t*>(E + k * lde + i) : 0;
-                const uint64_t T = (part + (uint64_t)acc[s2][0] + ((uint64_t)acc[s2][1] << 8) +
-                                    ((uint64_t)acc[s2][2] << 16) + ((uint64_t)acc[s2][3] << 24)) >> (2 * (s2 & 3));
-                v = (double)T * TWO_M29 + ev_value(myf[s2 * 64], myf[(8 + s2) * 64]);
+            for (int s_ = 0; s_ < 4; ++s_) {
+                const int64_t i_loc = 4 * Qf + s_;
+                if (I0 + i_loc >= n) continue;
+                const int sh = s_ == 0 ? 0 : s_ == 1 ? 2 : s_ == 2 ? 4 : 2;
+                const int64_t ev = ((int64_t)acc[s_][0] + ((int64_t)acc[s_][1] << 8) + ((int64_t)acc[s_][2] << 16) +
+                                    ((int64_t)acc[s_][3] << 24)) >> sh;
+                double v = (double)ev * TWO_M29;
+                if (fm != 0) {
+                    const int64_t flo = ((int64_t)af0[s_][0] + ((int64_t)af0[s_][1] << 8) +
+                                         ((int64_t)af0[s_][2] << 16) + ((int64_t)af0[s_][3] << 24)) >> sh;
+                    const int64_t fhi = ((int64_t)af1[s_][0] + ((int64_t)af1[s_][1] << 8) +
+                                         ((int64_t)af1[s_][2] << 16) + ((int64_t)af1[s_][3] << 24)) >> sh;
+                    v += ((double)fhi * 4294967296.0 + (double)flo) * fsc;
+                }
+                row[i_loc] = (I0 + i_loc == k_out) ? sqd[k_out] * 0.5 : v;  // E(k, k) = s_k / 2
             }
-            E[k * lde + i] = v;
         }
     }
 }
@@ -1005,10 +873,15 @@ struct cb_grm {
     DevBuf<double> E;                 // missing-call correction, n x n
     DevBuf<int64_t> snp_off, smp_off;
     DevBuf<int32_t> snp_list, smp_cnt, lj, cursor;
-    DevBuf<uint32_t> lk, lg;
+    DevBuf<uint32_t> lk;
     DevBuf<unsigned long long> sq;
     DevBuf<double> sqd;
-    DevBuf<uint8_t> bedd;             // re-encoded rows for the missing-call kernel
+    // the missing-call kernel's inputs: rows in I-block-major order, the
+    // I-block segments of the SNP lists, the group lists and A fragments
+    DevBuf<uint32_t> bedi, gK;
+    DevBuf<uint8_t> gkl;
+    DevBuf<int32_t> gj, gp;
+    DevBuf<int64_t> gstep;
 };
 
 namespace {
@@ -1153,30 +1026,61 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
         copy_host(st, u.p, hu.data(), n * 8, hipMemcpyHostToDevice);
 
         grow(E, (size_t)n * n);
-        DevBuf<uint8_t>& bedd = grm->bedd;
-        const int64_t dpitch = round_up(g->pitch, 128);
-        grow(bedd, (size_t)dpitch * m);
-        timed(ctx, "k_encode_d3", [&] {
-            k_encode_d3<<<dim3((unsigned)ceil_div(dpitch / 4, 256), (unsigned)m), 256, 0, st>>>(
-                g->bed.p, bedd.p, g->pitch, dpitch, n, m);
+        // 4b. the correction on INT8 MFMA (k_miss_mfma): rows in I-block-major
+        // order, the group lists and their A fragments
+        const int64_t nb = ceil_div(n, MB_SAMPLES);
+        CB_REQUIRE(m * 64 < ((int64_t)1 << 31), "GRM: too many SNPs for the missing-call kernel's row buffer");
+        DevBuf<uint32_t>& bedi = grm->bedi;
+        grow(bedi, (size_t)nb * m * 16);
+        const unsigned gy = (unsigned)std::min<int64_t>(m, 65535);
+        timed(ctx, "k_encode_iblock", [&] {
+            k_encode_iblock<<<dim3((unsigned)ceil_div(nb * 16, 256), gy), 256, 0, st>>>(g->bed.p, g->pitch, n, m,
+                                                                                      nb, bedi.p);
         });
-        // the regrouped kernel: buffer offsets below 2^31, ring records carry 26-bit group indices
-        if (getenv("CB_MISS_CROSS_V5") || dpitch * m >= ((int64_t)1 << 31) || m >= ((int64_t)1 << 26)) {
-            const int kw = 64;
-            timed(ctx, "k_miss_cross", [&] {
-                k_miss_cross<<<dim3((unsigned)ceil_div(n, kw), (unsigned)ceil_div(n, 1024)), dim3(256), 0, st>>>(
-                    bedd.p, dpitch, n, d_smp_off.p, lj.p, lk.p, sqd.p, kw, E.p, n);
-            });
-        } else {
-            const int kw = 64;
-            DevBuf<uint32_t>& lg = grm->lg;
-            grow(lg, (size_t)nlist * 2);  // nlist / 4 groups of 8 words
-            k_miss_groups<<<dim3((unsigned)ceil_div(nlist / 4, 256)), dim3(256), 0, st>>>(lj.p, lk.p, nlist / 4, lg.p);
-            timed(ctx, "k_miss_cross", [&] {
-                k_miss_cross_h<<<dim3((unsigned)ceil_div(n, kw), (unsigned)ceil_div(n, MC_I)), dim3(256), 0, st>>>(
-                    bedd.p, dpitch, n, dpitch * m, d_smp_off.p, lg.p, lk.p, sqd.p, kw, E.p, n);
-            });
+        // groups of 4 samples, each sample's list padded to whole k-blocks and
+        // the group's to whole rings of K-steps
+        const int64_t ngrp = ceil_div(n, MB_GROUP);
+        std::vector<int64_t> gstep(ngrp + 1, 0);
+        int64_t max_cnt = 0;
+        for (int64_t gi = 0; gi < ngrp; ++gi) {
+            int64_t cnt = 0;
+            for (int64_t k = gi * MB_GROUP; k < std::min(n, (gi + 1) * MB_GROUP); ++k) {
+                cnt += round_up(hcnt[k], MB_KBLK);
+                max_cnt = std::max<int64_t>(max_cnt, hcnt[k]);
+            }
+            gstep[gi + 1] = gstep[gi] + round_up(ceil_div(cnt, MB_STEP), MB_PF);
         }
+        // int32 digit sums: |d| x dosage x mask scale <= 128 x 2 x 16 per entry
+        CB_REQUIRE(max_cnt * 4096 < ((int64_t)1 << 31),
+                   "GRM: a sample with more than 524287 missing calls among the kept SNPs");
+        const int64_t tsteps = gstep[ngrp];
+        DevBuf<int64_t>& dgstep = grm->gstep;
+        grow(dgstep, ngrp + 1);
+        copy_host(st, dgstep.p, gstep.data(), (ngrp + 1) * 8, hipMemcpyHostToDevice);
+        DevBuf<int32_t>& gj = grm->gj;
+        DevBuf<uint32_t>& gK = grm->gK;
+        DevBuf<uint8_t>& gkl = grm->gkl;
+        grow(gj, std::max<int64_t>(1, tsteps * MB_STEP));
+        grow(gK, std::max<int64_t>(1, tsteps * MB_STEP));
+        grow(gkl, std::max<int64_t>(16, tsteps * 4));
+        timed(ctx, "k_group_entries", [&] {
+            k_group_entries<<<(unsigned)ngrp, 256, 0, st>>>(d_smp_off.p, smp_cnt.p, lj.p, lk.p, n, dgstep.p, gj.p,
+                                                           gK.p, gkl.p);
+        });
+        // group ranges of about equal K-steps, P of them per I-block
+        const int64_t P = std::min<int64_t>(2048, round_up(ngrp, MB_XCD));
+        std::vector<int32_t> gp(P + 1);
+        for (int64_t q = 0; q < P; ++q)
+            gp[q] = (int32_t)(std::lower_bound(gstep.begin(), gstep.end(), q * tsteps / P) - gstep.begin());
+        gp[P] = (int32_t)ngrp;
+        DevBuf<int32_t>& dgp = grm->gp;
+        grow(dgp, P + 1);
+        copy_host(st, dgp.p, gp.data(), (P + 1) * 4, hipMemcpyHostToDevice);
+        CB_REQUIRE(nb * P < ((int64_t)1 << 31), "GRM: too many samples for the missing-call kernel's grid");
+        timed(ctx, "k_miss_cross", [&] {
+            k_miss_mfma<<<(unsigned)(nb * P), 256, 0, st>>>(bedi.p, m, n, P, dgp.p, dgstep.p, gj.p, gK.p, gkl.p, sqd.p,
+                                                           E.p, n);
+        });
     }
 
     // 5. Gram + fused correction
