@@ -384,11 +384,10 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The 4 B fragments of one K-step (one per sample of the quad, values x 1,
-// 4, 16, 4) from the quad's 16 entry bytes: dosages (a missing call, field
+// B fragments of one K-step, one per sample of the quad (values x 1, 4, 16,
+// 4), from the quad's 16 entry bytes x: the dosages (a missing call, field
 // 3, reads 0) and, for F, the missing-call indicators.
-__device__ __forceinline__ void b_fragments(const uint4 w4, v4i (&bd)[4], v4i (&bm)[4], bool want_m) {
-    const uint32_t x[4] = {w4.x, w4.y, w4.z, w4.w};
+__device__ __forceinline__ void dosage_fragments(const uint32_t (&x)[4], v4i (&bd)[4]) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const uint32_t mq = x[q] & (x[q] >> 1) & 0x55555555u;  // 1 at the low bit of each missing field
@@ -396,13 +395,17 @@ __device__ __forceinline__ void b_fragments(const uint4 w4, v4i (&bd)[4], v4i (&
         bd[0][q] = (int)(cl & 0x03030303u);
         bd[1][q] = (int)(cl & 0x0c0c0c0cu);
         bd[2][q] = (int)(cl & 0x30303030u);
-        bd[3][q] = (int)((cl >> 4) & 0x0c0c0c0cu);
-        if (want_m) {
-            bm[0][q] = (int)(mq & 0x01010101u);
-            bm[1][q] = (int)(mq & 0x04040404u);
-            bm[2][q] = (int)(mq & 0x10101010u);
-            bm[3][q] = (int)((mq >> 4) & 0x04040404u);
-        }
+        bd[3][q] = (int)((cl & 0xc0c0c0c0u) >> 4);
+    }
+}
+__device__ __forceinline__ void missing_fragments(const uint32_t (&x)[4], v4i (&bm)[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t y = x[q] >> 1;
+        bm[0][q] = (int)(x[q] & y & 0x01010101u);
+        bm[1][q] = (int)(x[q] & y & 0x04040404u);
+        bm[2][q] = (int)(x[q] & y & 0x10101010u);
+        bm[3][q] = (int)((x[q] & y & 0x40404040u) >> 4);
     }
 }
 
@@ -429,7 +432,7 @@ __device__ __forceinline__ void b_fragments(const uint4 w4, v4i (&bd)[4], v4i (&
 //            k-block is sample kl's, else 0, for the lane's row (kl, c);
 //   write    E(k, i) for the group's samples k < n and the I-block's i < n;
 //            E(k, k) = s_k / 2 from sqd.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_miss_mfma(const uint32_t* __restrict__ bedi, int64_t m, int64_t n,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_miss_mfma(const uint32_t* __restrict__ bedi, int64_t m, int64_t n,
                                                    int64_t P, const int32_t* __restrict__ gp,
                                                    const int64_t* __restrict__ gstep, const int32_t* __restrict__ gj,
                                                    const uint32_t* __restrict__ gK, const uint8_t* __restrict__ gkl,
@@ -440,7 +443,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     __shared__ __attribute__((aligned(16))) int32_t sJ[MB_CHUNK * MB_STEP];            // 6 KB
     __shared__ __attribute__((aligned(16))) uint8_t sKL[MB_CHUNK * 4];
     __shared__ __attribute__((aligned(16))) uint32_t sB[2][(MB_SAMPLES / 4) * MB_QP];  // 10 KB
+    __shared__ __attribute__((aligned(16))) uint32_t sZero[4];  // the A fragment of rows off the k-block
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (tid < 4) sZero[tid] = 0u;  // ordered before its readers by the first chunk's barriers
     const int64_t b = blockIdx.x / P, p = blockIdx.x % P;
     const int64_t I0 = b * MB_SAMPLES;
     const int kl_out = lane >> 4;                // output rows (kl_out, c) of this lane
@@ -520,18 +525,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     if (st + MB_PF < csteps) fetch(st + MB_PF, cw);
                     __syncthreads();
                     const uint4 w4 = *reinterpret_cast<const uint4*>(sB[st & 1] + brd);
+                    const uint32_t x[4] = {w4.x, w4.y, w4.z, w4.w};
                     const bool on = (int)sKL[st * 4 + kb] == kl_a;
-                    const uint4 pk = *reinterpret_cast<const uint4*>(sP + st * 64 + c_a * 16 + kb * 4);
-                    const v4i ak = on ? (v4i){(int)pk.x, (int)pk.y, (int)pk.z, (int)pk.w} : (v4i){0, 0, 0, 0};
-                    v4i bd[4], bm[4];
-                    b_fragments(w4, bd, bm, fm != 0);
+                    const v4i ak = *reinterpret_cast<const v4i*>(on ? sP + st * 64 + c_a * 16 + kb * 4 : sZero);
+                    v4i bd[4];
+                    dosage_fragments(x, bd);
 #pragma unroll
                     for (int s_ = 0; s_ < 4; ++s_) acc[s_] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ak, bd[s_], acc[s_], 0, 0, 0);
                     if (fm != 0) {
-                        const uint4 p0 = *reinterpret_cast<const uint4*>(sV + st * 128 + c_a * 16 + kb * 4);
-                        const uint4 p1 = *reinterpret_cast<const uint4*>(sV + st * 128 + (4 + c_a) * 16 + kb * 4);
-                        const v4i a0 = on ? (v4i){(int)p0.x, (int)p0.y, (int)p0.z, (int)p0.w} : (v4i){0, 0, 0, 0};
-                        const v4i a1 = on ? (v4i){(int)p1.x, (int)p1.y, (int)p1.z, (int)p1.w} : (v4i){0, 0, 0, 0};
+                        v4i bm[4];
+                        missing_fragments(x, bm);
+                        const v4i a0 = *reinterpret_cast<const v4i*>(on ? sV + st * 128 + c_a * 16 + kb * 4 : sZero);
+                        const v4i a1 = *reinterpret_cast<const v4i*>(on ? sV + st * 128 + (4 + c_a) * 16 + kb * 4 : sZero);
 #pragma unroll
                         for (int s_ = 0; s_ < 4; ++s_) {
                             af0[s_] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, bm[s_], af0[s_], 0, 0, 0);
