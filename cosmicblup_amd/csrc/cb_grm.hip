@@ -416,7 +416,9 @@ __device__ __forceinline__ void missing_fragments(const uint32_t (&x)[4], v4i (&
 // m x 64 bytes of row segments, which stay in its L2. 256 threads; per group:
 //   stage    per chunk of MB_CHUNK K-steps: the SNPs and k-block samples,
 //            and the digit planes of K (4) and, if F is added here, of K^2
-//            (8), formed from gK (plane c, byte t = digit c of entry t);
+//            (8), formed from gK (plane c, byte t = digit c of entry t); the
+//            lists stream past L2 (non-temporal loads), which keeps the
+//            gathered rows;
 //   gather   thread t loads dword t % 16 (16 samples) of the rows of entries
 //            4 (t / 16) .. + 3 (a wave's load: 64 bytes of 4 rows; MB_PF
 //            steps in flight) and transposes the 4 x 4 bytes, giving per
@@ -472,11 +474,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             __syncthreads();  // the previous chunk's readers are done with the staging
             {
                 const int64_t s0 = step0 + c0;
-                const int4* srcJ = reinterpret_cast<const int4*>(gj + s0 * MB_STEP);
-                const uint4* srcK = reinterpret_cast<const uint4*>(gK + s0 * MB_STEP);
+                const v4i* srcJ = reinterpret_cast<const v4i*>(gj + s0 * MB_STEP);
+                const v4i* srcK = reinterpret_cast<const v4i*>(gK + s0 * MB_STEP);
                 for (int x = tid; x < csteps * 16; x += 256) {
-                    reinterpret_cast<int4*>(sJ)[x] = srcJ[x];
-                    const uint4 k4 = srcK[x];
+                    reinterpret_cast<v4i*>(sJ)[x] = __builtin_nontemporal_load(srcJ + x);
+                    const v4i kv = __builtin_nontemporal_load(srcK + x);
+                    const uint4 k4 = make_uint4((uint32_t)kv[0], (uint32_t)kv[1], (uint32_t)kv[2], (uint32_t)kv[3]);
                     const int st = x >> 4, t4 = x & 15;  // entries 4 t4 .. + 3 of step st
                     uint32_t o[4];
                     transpose4x4((k4.x + 0x80808080u) ^ 0x80808080u, (k4.y + 0x80808080u) ^ 0x80808080u,
@@ -498,7 +501,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                         for (int c = 0; c < 4; ++c) sV[st * 128 + (4 + c) * 16 + t4] = o[c];
                     }
                 }
-                for (int x = tid; x < csteps * 4; x += 256) sKL[x] = gkl[s0 * 4 + x];
+                for (int x = tid; x < csteps * 4; x += 256) sKL[x] = __builtin_nontemporal_load(gkl + s0 * 4 + x);
             }
             __syncthreads();  // staged
             // row dwords of MB_PF steps in flight: step st's sit in ring[st % MB_PF]
@@ -1073,7 +1076,7 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
                                                            gK.p, gkl.p);
         });
         // group ranges of about equal K-steps, P of them per I-block
-        const int64_t P = std::min<int64_t>(2048, round_up(ngrp, MB_XCD));
+        const int64_t P = std::min<int64_t>(4096, round_up(ngrp, MB_XCD));
         std::vector<int32_t> gp(P + 1);
         for (int64_t q = 0; q < P; ++q)
             gp[q] = (int32_t)(std::lower_bound(gstep.begin(), gstep.end(), q * tsteps / P) - gstep.begin());
