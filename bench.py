@@ -24,10 +24,14 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 PEAK_INT8_TOPS = 5000.0  # MI355X dense INT8 MFMA, TOP/s
-# HBM bytes per k_gram launch at n = m = 50,000 with 1% missing, from two
-# rocprofv3 --pmc runs of this script's GRM leg (FETCH_SIZE x 2.0 + WRITE_SIZE
-# x 1.0, the factors a 1 GiB device copy gives): 86.5 GB read + 30.3 GB written
-K_GRAM_TRAFFIC_BYTES = 1.1684e11
+PEAK_HBM_GBS = 8000.0    # MI355X HBM3E, GB/s
+# HBM bytes per launch at n = m = 50,000 with 1% missing, from two rocprofv3
+# --pmc runs of this script's GRM leg (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the
+# factors a 1 GiB device copy gives):
+#   k_gram        85.8 GB read + 30.3 GB written
+#   k_miss_cross  79.7 GB read + 20.0 GB written
+K_GRAM_TRAFFIC_BYTES = 1.161e11
+K_MISS_TRAFFIC_BYTES = 9.97e10
 # headline metric: the GRM part is `value` (TOPS on the n = m = 50,000 GRM);
 # the REML part (GPU mixed vs GPU FP64 plain iterations/s) is under "reml"
 METRIC = "REML iters/sec + GRM TOPS, GPU mixed vs GPU FP64 at \u22641e-6 rel EBV error"
@@ -690,8 +694,22 @@ def main() -> None:
         ms, cnt = ctx.kernel_time(k)
         if cnt:
             kern[k] = ms / cnt
+    c2 = (n, m, args.missing) == (50000, 50000, 0.01)
     gram_ms = kern["k_gram"]
     achieved = ops / (gram_ms * 1e-3) / 1e12
+    gram_roof = {"kernel": "k_gram", "bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TOPS",
+                 "frac": achieved / PEAK_INT8_TOPS, "traffic": K_GRAM_TRAFFIC_BYTES if c2 else None}
+    rooflines = [gram_roof]
+    if "k_miss_cross" in kern:
+        # algorithmic bytes: the packed genotypes read once (n m / 4) and the
+        # n x n FP64 correction E written once
+        miss_bytes = n * m / 4 + 8.0 * n * n
+        gbs = miss_bytes / (kern["k_miss_cross"] * 1e-3) / 1e9
+        rooflines.append({"kernel": "k_miss_cross", "bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS,
+                          "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
+                          "traffic": K_MISS_TRAFFIC_BYTES if c2 else None})
+    # the dominant kernel's roofline is the headline one
+    rooflines.sort(key=lambda r: -kern[r["kernel"]])
     kern_sum = sum(kern.values())
     out = {
         "metric": METRIC,
@@ -709,9 +727,10 @@ def main() -> None:
         "config": {"workload": f"GRM VanRaden n={n} m={m} MAF U(0.01,0.5) HWE, "
                                f"{args.missing:.0%} MCAR missing, G in FP64+FP32",
                    "n": n, "m": m, "missing": args.missing},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TOPS",
-                     "frac": achieved / PEAK_INT8_TOPS,
-                     "traffic": K_GRAM_TRAFFIC_BYTES if (n, m, args.missing) == (50000, 50000, 0.01) else None},
+        "roofline": rooflines[0],
+        "roofline_other_kernels": rooflines[1:],
+        # the whole step against the INT8 MFMA peak
+        "step_frac_int8_peak": ops / dt / 1e12 / PEAK_INT8_TOPS,
         "kernel_ms": kern,
         # per step: the kernels' own time and what lies outside them (host
         # statistics, list building, copies, launch gaps)
