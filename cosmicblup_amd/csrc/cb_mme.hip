@@ -779,30 +779,23 @@ __global__ void k_q32_bulk(const double* __restrict__ Q, float* __restrict__ Q32
     Q32[e] = j < q ? (float)(Q[e] - mu * v[i] * v[j]) : 0.0f;  // padding stays zero
 }
 
-// CG scalar step per column (runs on the device so the host need not sync
-// every iteration). st layout per column c: [rho, pq, rho_new, rr, bb, active, alpha, beta]
+// CG scalar state per column (kept on the device so the host need not sync
+// every iteration). rho_new is the r.z of the current direction; rho the one
+// before it, moved up by the next update of x and r.
 struct ColState {
     double rho, pq, rho_new, rr, bb, alpha, beta;
     int active, restarted;
 };
 
-template <class T>
-__global__ void k_cg_p(T* __restrict__ P, const T* __restrict__ Z, const ColState* __restrict__ st,
-                       int64_t rows, int64_t ld) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = blockIdx.y;
-    if (i >= rows || !st[c].active) return;
-    const int64_t o = (int64_t)c * ld + i;
-    P[o] = (T)((double)Z[o] + st[c].beta * (double)P[o]);
-}
-
-// The iteration's scalar steps read the dot products' chunk partials
-// themselves (summed in chunk order, as k_dots_finish does), so no launch
-// finishes a dot on its own:
+// The scalar steps run inside the vector kernels that use them: thread 0 of
+// every block sums the dot products' chunk partials in chunk order (as
+// k_dots_finish does), so every block derives the same scalars, and block 0
+// of each column writes them back. A field the kernel writes is never one
+// its blocks read for their scalars, so the order of blocks does not matter.
 //   k_col_dots2      the partials of A.B and A.C in one pass over A
-//   k_cg_alpha_parts alpha = rho / p.q (a breakdown freezes the column)
-//   k_cg_beta_parts  rho_new = r.z, rr = r.r, convergence, beta
-//   k_cg_xr_diag     x += alpha p, r -= alpha q, then the Jacobi rows of z
+//   k_cg_xr_diag     alpha = rho_new / p.q (a breakdown freezes the column),
+//                    x += alpha p, r -= alpha q, then the Jacobi rows of z
+//   k_cg_p           rho_new = r.z, rr = r.r, convergence, beta, p = z + beta p
 template <class T, int NTH = 256>
 __global__ __launch_bounds__(NTH) void k_col_dots2(const T* __restrict__ A, const T* __restrict__ B,
                                                    const T* __restrict__ C, int64_t ld, int64_t rows,
@@ -839,44 +832,76 @@ __device__ __forceinline__ double sum_chunks(const double* __restrict__ part, in
     return s;
 }
 
-__global__ void k_cg_alpha_parts(ColState* __restrict__ st, const double* __restrict__ part, int r) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= r) return;
-    ColState s = st[c];
-    s.pq = sum_chunks(part, c);
-    if (s.active && fabs(s.pq) < 1e-300) s.active = 0;  // breakdown: freeze the column
-    s.alpha = s.active ? s.rho / s.pq : 0.0;
-    st[c] = s;
+// rho_new = r.z, rr = r.r, convergence (or NaN) and beta from the partials;
+// active columns: p = z + beta p
+template <class T>
+__global__ __launch_bounds__(256) void k_cg_p(T* __restrict__ P, const T* __restrict__ Z,
+                                              ColState* __restrict__ st, const double* __restrict__ part_rz,
+                                              const double* __restrict__ part_rr, double tol2, int absolute,
+                                              int64_t rows, int64_t ld) {
+    __shared__ double s_beta;
+    __shared__ int s_act;
+    const int c = blockIdx.y;
+    if (threadIdx.x == 0) {
+        const double rho = st[c].rho, bb = st[c].bb;
+        int act = st[c].active;
+        double beta = 0.0;
+        if (act) {
+            const double rho_new = sum_chunks(part_rz, c), rr = sum_chunks(part_rr, c);
+            const double thr = absolute ? tol2 : tol2 * bb;
+            if (!(rr >= thr)) act = 0;  // converged (or NaN)
+            beta = rho != 0.0 ? rho_new / rho : 0.0;
+            if (blockIdx.x == 0) {
+                st[c].rho_new = rho_new;
+                st[c].rr = rr;
+                st[c].beta = beta;
+                st[c].active = act;
+            }
+        }
+        s_beta = beta;
+        s_act = act;
+    }
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows || !s_act) return;
+    const int64_t o = (int64_t)c * ld + i;
+    P[o] = (T)((double)Z[o] + s_beta * (double)P[o]);
 }
 
-__global__ void k_cg_beta_parts(ColState* __restrict__ st, const double* __restrict__ part_rz,
-                                const double* __restrict__ part_rr, int r, double tol2, int absolute) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= r) return;
-    ColState s = st[c];
-    if (!s.active) return;
-    s.rho_new = sum_chunks(part_rz, c);
-    s.rr = sum_chunks(part_rr, c);
-    const double thr = absolute ? tol2 : tol2 * s.bb;
-    if (!(s.rr >= thr)) s.active = 0;  // converged (or NaN)
-    s.beta = s.rho != 0.0 ? s.rho_new / s.rho : 0.0;
-    s.rho = s.rho_new;
-    st[c] = s;
-}
-
+// alpha = rho_new / p.q from the partials (a breakdown freezes the column);
 // active columns: x += alpha p, r -= alpha q; every column: z = r dinv on
 // the rows >= p0 (the block-Jacobi rows < p0 follow in their own kernel)
 template <class TX, class T>
-__global__ void k_cg_xr_diag(TX* __restrict__ X, T* __restrict__ R, const T* __restrict__ P,
-                             const T* __restrict__ Qp, T* __restrict__ Z, const double* __restrict__ dinv,
-                             const ColState* __restrict__ st, int64_t p0, int64_t rows, int64_t ld) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_cg_xr_diag(TX* __restrict__ X, T* __restrict__ R, const T* __restrict__ P,
+                                                    const T* __restrict__ Qp, T* __restrict__ Z,
+                                                    const double* __restrict__ dinv, ColState* __restrict__ st,
+                                                    const double* __restrict__ part, int64_t p0, int64_t rows,
+                                                    int64_t ld) {
+    __shared__ double s_alpha;
+    __shared__ int s_act;
     const int c = blockIdx.y;
+    if (threadIdx.x == 0) {
+        const double rho = st[c].rho_new;
+        int act = st[c].active;
+        const double pq = sum_chunks(part, c);
+        if (act && fabs(pq) < 1e-300) act = 0;  // breakdown: freeze the column
+        const double alpha = act ? rho / pq : 0.0;
+        if (blockIdx.x == 0) {
+            st[c].pq = pq;
+            st[c].alpha = alpha;
+            st[c].active = act;
+            st[c].rho = rho;
+        }
+        s_alpha = alpha;
+        s_act = act;
+    }
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= rows) return;
     const int64_t o = (int64_t)c * ld + i;
     T rv = R[o];
-    if (st[c].active) {
-        const double a = st[c].alpha;
+    if (s_act) {
+        const double a = s_alpha;
         X[o] = (TX)((double)X[o] + a * (double)P[o]);
         rv = (T)((double)rv - a * (double)Qp[o]);
         R[o] = rv;
@@ -973,17 +998,22 @@ __global__ void k_w_apply(const int64_t* __restrict__ xp, const int32_t* __restr
     T[(int64_t)c * ldt + r] = t;
 }
 
-// out = W' v via the transposed design (CSR dim x n)
-__global__ void k_wt_apply(const int64_t* __restrict__ tp, const int32_t* __restrict__ tc,
-                           const double* __restrict__ tv, int64_t dim, const double* __restrict__ V,
-                           int64_t ldv, double* __restrict__ O, int64_t ldo) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// out = W' v via the transposed design (CSR dim x n), a wave per row: the
+// fixed-effect rows hold a record each of a level, the animal rows a few
+template <int LANES>
+__global__ __launch_bounds__(256) void k_wt_apply(const int64_t* __restrict__ tp, const int32_t* __restrict__ tc,
+                                                  const double* __restrict__ tv, int64_t dim,
+                                                  const double* __restrict__ V, int64_t ldv, double* __restrict__ O,
+                                                  int64_t ldo) {
+    const int lane = threadIdx.x % LANES;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LANES;
     const int c = blockIdx.y;
     if (i >= dim) return;
     const double* v = V + (int64_t)c * ldv;
     double s = 0;
-    for (int64_t k = tp[i]; k < tp[i + 1]; ++k) s += tv[k] * v[tc[k]];
-    O[(int64_t)c * ldo + i] = s;
+    for (int64_t k = tp[i] + lane; k < tp[i + 1]; k += LANES) s += tv[k] * v[tc[k]];
+    for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_down(s, off, LANES);
+    if (lane == 0) O[(int64_t)c * ldo + i] = s;
 }
 
 // AI phase vectors: vu[r] = u_hat[aid[r]] / var_u ; ve[r] = e[r] / var_e
@@ -1659,15 +1689,13 @@ void cg_iteration(cb_mme* M, double lambda, TX* X, T* R, T* Z, T* P, T* Qp, ColS
         k_col_dots<T, 1024><<<gd, 1024, 0, st>>>(P, Qp, ld, dim, part);
     else
         k_col_dots<T><<<gd, 256, 0, st>>>(P, Qp, ld, dim, part);
-    k_cg_alpha_parts<<<blocks(r, 64), 64, 0, st>>>(cs, part, r);
-    k_cg_xr_diag<TX, T><<<g2, 256, 0, st>>>(X, R, P, Qp, Z, M->dinv.p, cs, M->p, dim, ld);
+    k_cg_xr_diag<TX, T><<<g2, 256, 0, st>>>(X, R, P, Qp, Z, M->dinv.p, cs, part, M->p, dim, ld);
     apply_precond_rest<T>(M, R, Z, r);
     if (long_dots)
         k_col_dots2<T, 1024><<<gd, 1024, 0, st>>>(R, Z, R, ld, dim, part, part2);
     else
         k_col_dots2<T><<<gd, 256, 0, st>>>(R, Z, R, ld, dim, part, part2);
-    k_cg_beta_parts<<<blocks(r, 64), 64, 0, st>>>(cs, part, part2, r, tol2, absolute ? 1 : 0);
-    k_cg_p<T><<<g2, 256, 0, st>>>(P, Z, cs, dim, ld);
+    k_cg_p<T><<<g2, 256, 0, st>>>(P, Z, cs, part, part2, tol2, absolute ? 1 : 0, dim, ld);
     CB_HIP(hipGetLastError());
 }
 
@@ -1773,7 +1801,7 @@ int64_t pcg_block(cb_mme* M, double lambda, const T* B, TX* X, int r, double tol
     std::vector<ColState> hs(r);
     for (int c = 0; c < r; ++c) {
         hs[c] = ColState{};
-        hs[c].rho = rz[c];
+        hs[c].rho = hs[c].rho_new = rz[c];
         hs[c].bb = bb[c] == 0.0 ? 1.0 : bb[c];  // pcg_solver.h:463: bnorm == 0 -> 1
         hs[c].active = 1;
     }
@@ -1920,7 +1948,7 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
             const bool restart = first || !hs[c].active || !(hs[c].rr <= 100.0 * rr_ref[c]);
             hist[c].push_back(rr[c]);
             rr_ref[c] = rr[c];
-            hs[c].rho = rz[c];
+            hs[c].rho = hs[c].rho_new = rz[c];
             hs[c].rr = rr[c];
             hs[c].active = done[c] ? 0 : 1;
             if (restart && !done[c])
@@ -2425,7 +2453,8 @@ static void reml_eval(cb_mme* M, const cb_reml_options& o, cb_reml_work& w, doub
     // AI matrix, vce.cpp:2596-2645: v_u = Z u / vu, v_e = e / ve; P v = (v - W C^-1 W'v) / ve
     const int64_t ldn = round_up(n, 64);
     k_ai_vectors<<<blocks(n), 256, 0, st>>>(M->aid.p, n, p, w.S.p, w.e.p, 1.0 / vu, 1.0 / ve, w.V.p, ldn);
-    k_wt_apply<<<dim3(blocks(dim), 2), 256, 0, st>>>(M->tp.p, M->tc.p, M->tv.p, dim, w.V.p, ldn, w.B2.p, ld);
+    k_wt_apply<64><<<dim3((unsigned)ceil_div(dim, 4), 2), 256, 0, st>>>(M->tp.p, M->tc.p, M->tv.p, dim, w.V.p, ldn,
+                                                                       w.B2.p, ld);
     CB_HIP(hipGetLastError());
     solve_block(M, lambda, o.solve, w.B2.p, w.S2.p, 2, w.have_s2);
     w.have_s2 = true;
