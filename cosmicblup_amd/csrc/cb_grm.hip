@@ -304,6 +304,9 @@ constexpr int MB_PF = 2;         // K-steps of row gathers in flight; group step
 constexpr int MB_CHUNK = 24;     // K-steps of one group staged in LDS at once (a multiple of MB_PF)
 constexpr int MB_QP = 20;        // LDS pitch (dwords) of one sample quad of B
 constexpr int MB_XCD = 8;        // XCDs; workgroups are dealt to them round-robin
+// chunks between folds of the int32 digit sums into E: 170 x 24 x 64 = 261 120
+// entries of at most 128 x 2 x 16 = 4096 each stay below 2^31
+constexpr int MB_FOLD = 170;
 
 // SNP rows in I-block-major order: block b (samples 256 b .. 256 b + 255)
 // of SNP j at dwords (b m + j) 16 .. + 15, dosage 0/1/2 per 2-bit field,
@@ -433,7 +436,10 @@ __device__ __forceinline__ void missing_fragments(const uint32_t (&x)[4], v4i (&
 //            8 for F. The A fragment is plane c of the k-block if the
 //            k-block is sample kl's, else 0, for the lane's row (kl, c);
 //   write    E(k, i) for the group's samples k < n and the I-block's i < n;
-//            E(k, k) = s_k / 2 from sqd.
+//            E(k, k) = s_k / 2 from sqd. A group longer than MB_FOLD chunks
+//            adds its sums into E every MB_FOLD chunks and starts them again,
+//            so that the int32 digit sums never wrap, whatever the list
+//            lengths.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_miss_mfma(const uint32_t* __restrict__ bedi, int64_t m, int64_t n,
                                                    int64_t P, const int32_t* __restrict__ gp,
                                                    const int64_t* __restrict__ gstep, const int32_t* __restrict__ gj,
@@ -469,6 +475,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         v4i acc[4], af0[4], af1[4];
 #pragma unroll
         for (int s_ = 0; s_ < 4; ++s_) acc[s_] = af0[s_] = af1[s_] = (v4i){0, 0, 0, 0};
+        // E(k, i) from sum_c acc_c 256^c over the 4 digit rows, scale removed:
+        // written, or added to a folded part; the last call also sets E(k, k)
+        auto emit = [&](bool add, bool last) {
+            const int64_t k_out = g * MB_GROUP + kl_out;
+            if (k_out >= n) return;
+            double* row = E + k_out * lde + I0;
+            const double fsc = fm == 2 ? 0x1p-58 : 0x1p-59;
+#pragma unroll
+            for (int s_ = 0; s_ < 4; ++s_) {
+                const int64_t i_loc = 4 * Qf + s_;
+                if (I0 + i_loc >= n) continue;
+                const int sh = s_ == 0 ? 0 : s_ == 1 ? 2 : s_ == 2 ? 4 : 2;
+                const int64_t ev = ((int64_t)acc[s_][0] + ((int64_t)acc[s_][1] << 8) + ((int64_t)acc[s_][2] << 16) +
+                                    ((int64_t)acc[s_][3] << 24)) >> sh;
+                double v = (double)ev * TWO_M29;
+                if (fm != 0) {
+                    const int64_t flo = ((int64_t)af0[s_][0] + ((int64_t)af0[s_][1] << 8) +
+                                         ((int64_t)af0[s_][2] << 16) + ((int64_t)af0[s_][3] << 24)) >> sh;
+                    const int64_t fhi = ((int64_t)af1[s_][0] + ((int64_t)af1[s_][1] << 8) +
+                                         ((int64_t)af1[s_][2] << 16) + ((int64_t)af1[s_][3] << 24)) >> sh;
+                    v += ((double)fhi * 4294967296.0 + (double)flo) * fsc;
+                }
+                if (add) v += row[i_loc];
+                if (last && I0 + i_loc == k_out) v = sqd[k_out] * 0.5;  // E(k, k) = s_k / 2
+                row[i_loc] = v;
+            }
+        };
         for (int64_t c0 = 0; c0 < nsteps; c0 += MB_CHUNK) {
             const int csteps = (int)(nsteps - c0 < MB_CHUNK ? nsteps - c0 : MB_CHUNK);
             __syncthreads();  // the previous chunk's readers are done with the staging
@@ -548,30 +581,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                     }
                 }
             }
-        }
-        // sum_c acc_c 256^c over the 4 digit rows, scale removed
-        const int64_t k_out = g * MB_GROUP + kl_out;
-        if (k_out < n) {
-            double* row = E + k_out * lde + I0;
-            const double fsc = fm == 2 ? 0x1p-58 : 0x1p-59;
+            // a long list: fold the sums into E and start them again
+            const int64_t cdone = c0 / MB_CHUNK + 1;
+            if (cdone % MB_FOLD == 0 && c0 + MB_CHUNK < nsteps) {
+                emit(cdone > MB_FOLD, false);
 #pragma unroll
-            for (int s_ = 0; s_ < 4; ++s_) {
-                const int64_t i_loc = 4 * Qf + s_;
-                if (I0 + i_loc >= n) continue;
-                const int sh = s_ == 0 ? 0 : s_ == 1 ? 2 : s_ == 2 ? 4 : 2;
-                const int64_t ev = ((int64_t)acc[s_][0] + ((int64_t)acc[s_][1] << 8) + ((int64_t)acc[s_][2] << 16) +
-                                    ((int64_t)acc[s_][3] << 24)) >> sh;
-                double v = (double)ev * TWO_M29;
-                if (fm != 0) {
-                    const int64_t flo = ((int64_t)af0[s_][0] + ((int64_t)af0[s_][1] << 8) +
-                                         ((int64_t)af0[s_][2] << 16) + ((int64_t)af0[s_][3] << 24)) >> sh;
-                    const int64_t fhi = ((int64_t)af1[s_][0] + ((int64_t)af1[s_][1] << 8) +
-                                         ((int64_t)af1[s_][2] << 16) + ((int64_t)af1[s_][3] << 24)) >> sh;
-                    v += ((double)fhi * 4294967296.0 + (double)flo) * fsc;
-                }
-                row[i_loc] = (I0 + i_loc == k_out) ? sqd[k_out] * 0.5 : v;  // E(k, k) = s_k / 2
+                for (int s_ = 0; s_ < 4; ++s_) acc[s_] = af0[s_] = af1[s_] = (v4i){0, 0, 0, 0};
             }
         }
+        emit(nsteps > (int64_t)MB_FOLD * MB_CHUNK, true);
     }
 }
 
@@ -1049,18 +1067,12 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
         // the group's to whole rings of K-steps
         const int64_t ngrp = ceil_div(n, MB_GROUP);
         std::vector<int64_t> gstep(ngrp + 1, 0);
-        int64_t max_cnt = 0;
         for (int64_t gi = 0; gi < ngrp; ++gi) {
             int64_t cnt = 0;
-            for (int64_t k = gi * MB_GROUP; k < std::min(n, (gi + 1) * MB_GROUP); ++k) {
+            for (int64_t k = gi * MB_GROUP; k < std::min(n, (gi + 1) * MB_GROUP); ++k)
                 cnt += round_up(hcnt[k], MB_KBLK);
-                max_cnt = std::max<int64_t>(max_cnt, hcnt[k]);
-            }
             gstep[gi + 1] = gstep[gi] + round_up(ceil_div(cnt, MB_STEP), MB_PF);
         }
-        // int32 digit sums: |d| x dosage x mask scale <= 128 x 2 x 16 per entry
-        CB_REQUIRE(max_cnt * 4096 < ((int64_t)1 << 31),
-                   "GRM: a sample with more than 524287 missing calls among the kept SNPs");
         const int64_t tsteps = gstep[ngrp];
         DevBuf<int64_t>& dgstep = grm->gstep;
         grow(dgstep, ngrp + 1);

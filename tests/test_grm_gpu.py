@@ -64,18 +64,16 @@ def test_random_cases(cb, ctx, n, m, miss, seed):
     check_case(cb, x, block_size=512)
 
 
-@pytest.mark.parametrize("kernel", ["regrouped", "plain"])
-def test_long_missing_lists(cb, ctx, monkeypatch, kernel):
-    """Samples 0 and 3 missing at all 200,000 SNPs. Their per-sample lists
-    are long enough that one 32-bit dot accumulator (up to 3 * 64 * 255 per
-    entry) and the 64-bit both-missing sum of the pair (up to 8 * 2^48 per
-    event) would wrap; both missing-call kernels must fold and split them
-    exactly. "plain" forces the kernel used for very large BED images."""
-    if kernel == "plain":
-        monkeypatch.setenv("CB_MISS_CROSS_V5", "1")
-    x, _ = synth.simulate_dosages(64, 200_000, maf=(0.05, 0.5), seed=17)
+def test_long_missing_lists(cb, ctx):
+    """Samples 0 and 3 missing at all 300,000 SNPs, sample 5 at every other
+    one. Their lists outrun one fold of the missing-call kernel's int32 digit
+    sums (261,120 entries of up to 4096 each), so the kernel must add its
+    partial sums into E and start again; the pair (0, 3) shares every
+    missing SNP, the largest both-missing sum."""
+    x, _ = synth.simulate_dosages(64, 300_000, maf=(0.05, 0.5), seed=17)
     x[0, :] = -1
     x[3, :] = -1
+    x[5, ::2] = -1
     check_case(cb, x, block_size=4096)
 
 
