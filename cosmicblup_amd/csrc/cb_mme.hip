@@ -758,10 +758,13 @@ template <class T>
 __global__ void k_stiff_axpy(T* __restrict__ Zu, const double* __restrict__ y, const double* __restrict__ part,
                              double coef, int64_t ld, int64_t q) {
     __shared__ double t;
+    __shared__ double sp[DOT_CHUNKS];
     const int c = blockIdx.y;
+    if (threadIdx.x < DOT_CHUNKS) sp[threadIdx.x] = part[(int64_t)c * DOT_CHUNKS + threadIdx.x];
+    __syncthreads();
     if (threadIdx.x == 0) {
         double s = 0;
-        for (int k = 0; k < DOT_CHUNKS; ++k) s += part[(int64_t)c * DOT_CHUNKS + k];
+        for (int k = 0; k < DOT_CHUNKS; ++k) s += sp[k];
         t = s;
     }
     __syncthreads();
@@ -831,6 +834,18 @@ __device__ __forceinline__ double sum_chunks(const double* __restrict__ part, in
     for (int k = 0; k < DOT_CHUNKS; ++k) s += part[(int64_t)c * DOT_CHUNKS + k];
     return s;
 }
+// the partials of column c into LDS by DOT_CHUNKS threads at once (one round
+// trip instead of a chain of dependent loads); thread 0 then sums them in
+// chunk order, the same sum as sum_chunks
+__device__ __forceinline__ void stage_chunks(const double* __restrict__ part, int c, double* __restrict__ sp,
+                                             int t) {
+    if (t < DOT_CHUNKS) sp[t] = part[(int64_t)c * DOT_CHUNKS + t];
+}
+__device__ __forceinline__ double sum_staged(const double* __restrict__ sp) {
+    double s = 0;
+    for (int k = 0; k < DOT_CHUNKS; ++k) s += sp[k];
+    return s;
+}
 
 // rho_new = r.z, rr = r.r, convergence (or NaN) and beta from the partials;
 // active columns: p = z + beta p
@@ -841,13 +856,18 @@ __global__ __launch_bounds__(256) void k_cg_p(T* __restrict__ P, const T* __rest
                                               int64_t rows, int64_t ld) {
     __shared__ double s_beta;
     __shared__ int s_act;
+    __shared__ double sp[2][DOT_CHUNKS];
     const int c = blockIdx.y;
+    static_assert(2 * DOT_CHUNKS <= 256, "one partial per thread");
+    if (threadIdx.x < DOT_CHUNKS) stage_chunks(part_rz, c, sp[0], threadIdx.x);
+    else stage_chunks(part_rr, c, sp[1], threadIdx.x - DOT_CHUNKS);
+    __syncthreads();
     if (threadIdx.x == 0) {
         const double rho = st[c].rho, bb = st[c].bb;
         int act = st[c].active;
         double beta = 0.0;
         if (act) {
-            const double rho_new = sum_chunks(part_rz, c), rr = sum_chunks(part_rr, c);
+            const double rho_new = sum_staged(sp[0]), rr = sum_staged(sp[1]);
             const double thr = absolute ? tol2 : tol2 * bb;
             if (!(rr >= thr)) act = 0;  // converged (or NaN)
             beta = rho != 0.0 ? rho_new / rho : 0.0;
@@ -879,11 +899,14 @@ __global__ __launch_bounds__(256) void k_cg_xr_diag(TX* __restrict__ X, T* __res
                                                     int64_t ld) {
     __shared__ double s_alpha;
     __shared__ int s_act;
+    __shared__ double sp[DOT_CHUNKS];
     const int c = blockIdx.y;
+    stage_chunks(part, c, sp, threadIdx.x);
+    __syncthreads();
     if (threadIdx.x == 0) {
         const double rho = st[c].rho_new;
         int act = st[c].active;
-        const double pq = sum_chunks(part, c);
+        const double pq = sum_staged(sp);
         if (act && fabs(pq) < 1e-300) act = 0;  // breakdown: freeze the column
         const double alpha = act ? rho / pq : 0.0;
         if (blockIdx.x == 0) {
