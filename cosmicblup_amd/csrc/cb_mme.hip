@@ -87,6 +87,43 @@ __global__ __launch_bounds__(256) void k_csr_spmm(const int64_t* __restrict__ pt
     }
 }
 
+// Y[row, c] = W[row] x_c + alpha Q[row - p] x_{c, p..} for rows in [row0,
+// row1) (the Q term for rows >= p only): W'W's rows and a CSR Q's in one pass
+// over the rows, S lanes per (row, c). Each matrix keeps its own lane count
+// (s1 <= S for W, S for Q) and butterfly, so the two sums and the final
+// y = w + alpha q are the ones the separate kernels form, bit for bit.
+template <class TM, class TV, int S>
+__global__ __launch_bounds__(256) void k_csr2_spmm(const int64_t* __restrict__ ptr1, const int32_t* __restrict__ col1,
+                                                   const TM* __restrict__ val1, int s1,
+                                                   const int64_t* __restrict__ ptr2, const int32_t* __restrict__ col2,
+                                                   const TM* __restrict__ val2, int64_t p, int64_t row0,
+                                                   int64_t row1, const TV* __restrict__ X, int64_t ldx,
+                                                   TV* __restrict__ Y, int64_t ldy, TV alpha) {
+    const int sub = (int)threadIdx.x & (S - 1);
+    const int64_t row = row0 + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / S;
+    const int64_t c = blockIdx.y;
+    TV a1 = 0, a2 = 0;
+    const TV* x = X + c * ldx;
+    if (row < row1) {  // no return before the butterflies: every lane of a group takes part
+        if (sub < s1) {
+            const int64_t e = ptr1[row + 1];
+            for (int64_t k = ptr1[row] + sub; k < e; k += s1) a1 += (TV)val1[k] * x[col1[k]];
+        }
+        if (row >= p) {
+            const int64_t r2 = row - p, e = ptr2[r2 + 1];
+            for (int64_t k = ptr2[r2] + sub; k < e; k += S) a2 += (TV)val2[k] * x[p + col2[k]];
+        }
+    }
+    for (int off = s1 / 2; off > 0; off >>= 1) a1 += __shfl_xor(a1, off, S);
+#pragma unroll
+    for (int off = S / 2; off > 0; off >>= 1) a2 += __shfl_xor(a2, off, S);
+    if (row < row1 && sub == 0) {
+        TV* y = Y + c * ldy + row;
+        const TV w = (TV)1 * a1;
+        *y = row >= p ? w + alpha * a2 : w;
+    }
+}
+
 // lanes per row for a CSR block: the mean row length rounded down to a
 // power of two, 2 when it lies in (1, 2), at most 16 (the 1M-animal A^-1,
 // 6.0 nonzeros per row: 4 lanes 7.1 ms per PBLUP solve, 8 lanes 7.6 ms,
@@ -1527,14 +1564,15 @@ DevBuf<float>& gbuf<float>(cb_mme* M, int which) { return which ? M->yg32 : M->x
 //           with D = tau G^-1 - omega A22^-1 held as one dense block)
 // The rank-1 stiff term of a dense Q's FP32 copy is added by apply_C.
 template <class T>
-void q_apply(cb_mme* M, const T* X, int64_t ldx, T* Y, int64_t ldy, int r, T alpha, const StiffTerm* sk = nullptr) {
+void q_apply(cb_mme* M, const T* X, int64_t ldx, T* Y, int64_t ldy, int r, T alpha, const StiffTerm* sk = nullptr,
+             bool csr_done = false) {
     hipStream_t st = M->ctx->stream;
     const int64_t p = M->p, q = M->q;
     if (M->qkind == 0) {
         dense_apply<T>(M, q, X + p, ldx, Y + p, ldy, r, alpha, sk);
         return;
     }
-    csr_spmm<T, T>(st, M->q_lanes, r, M->qp.p, M->qc.p, qvals<T>(M), 0, q, p, X, ldx, Y, ldy, p, alpha, 1);
+    if (!csr_done) csr_spmm<T, T>(st, M->q_lanes, r, M->qp.p, M->qc.p, qvals<T>(M), 0, q, p, X, ldx, Y, ldy, p, alpha, 1);
     CB_HIP(hipGetLastError());
     if (M->qkind == 2) {
         const int64_t ng = M->ng, ldg = round_up(ng, 64);
@@ -1566,11 +1604,31 @@ void apply_C(cb_mme* M, double lambda, const T* X, T* Y, int r) {
         k_csr_chunks_finish<T><<<dim3(blocks(M->long_rows, 4), (unsigned)r), 256, 0, st>>>(
             M->lr_start.p, M->long_rows, M->n_lchunks, lcp, Y, ld);
     }
-    if (M->dim > M->long_rows)
+    // a CSR Q rides in the pass over W'W's short rows when those hold every
+    // random-effect row and W'W's rows need no more lanes than Q's
+    const bool fused = M->qkind != 0 && M->long_rows <= M->p && M->dim > M->long_rows && M->q_lanes >= 2 &&
+                       M->ww_lanes <= M->q_lanes;
+    if (fused) {
+        const int S = M->q_lanes;
+        const dim3 grid((unsigned)std::max<int64_t>(1, ((M->dim - M->long_rows) * S + 255) / 256), (unsigned)r);
+        timed(ctx, sizeof(T) == 8 ? "k_csr_spmm_f64" : "k_csr_spmm_f32", [&] {
+#define CB_CSR2(S_)                                                                                                  \
+    k_csr2_spmm<T, T, S_><<<grid, 256, 0, st>>>(M->wp.p, M->wc.p, wvals<T>(M), M->ww_lanes, M->qp.p, M->qc.p,        \
+                                                qvals<T>(M), M->p, M->long_rows, M->dim, X, ld, Y, ld, (T)lambda)
+            switch (S) {
+                case 2: CB_CSR2(2); break;
+                case 4: CB_CSR2(4); break;
+                case 8: CB_CSR2(8); break;
+                default: CB_CSR2(16); break;
+            }
+#undef CB_CSR2
+        });
+    } else if (M->dim > M->long_rows) {
         timed(ctx, sizeof(T) == 8 ? "k_csr_spmm_f64" : "k_csr_spmm_f32", [&] {
             csr_spmm<T, T>(st, M->ww_lanes, r, M->wp.p, M->wc.p, wvals<T>(M), M->long_rows, M->dim, 0, X, ld, Y, ld, 0,
                            (T)1, 0);
         });
+    }
     CB_HIP(hipGetLastError());
     // lambda Q part; the FP32 copy of a dense Q holds Q - mu v v', and
     // lambda mu v (v' x_u) is added in the same pass that adds lambda Q x_u
@@ -1581,7 +1639,7 @@ void apply_C(cb_mme* M, double lambda, const T* X, T* Y, int r) {
         const StiffTerm sk{M->stiff_v.p, sp, lambda * M->stiff_mu};
         q_apply<T>(M, X, ld, Y, ld, r, (T)lambda, &sk);
     } else {
-        q_apply<T>(M, X, ld, Y, ld, r, (T)lambda);
+        q_apply<T>(M, X, ld, Y, ld, r, (T)lambda, nullptr, fused);
     }
 }
 
