@@ -25,6 +25,11 @@ sys.path.insert(0, str(ROOT))
 
 PEAK_INT8_TOPS = 5000.0  # MI355X dense INT8 MFMA, TOP/s
 PEAK_HBM_GBS = 8000.0    # MI355X HBM3E, GB/s
+PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 MFMA, TFLOP/s
+# HBM bytes per k_dense_mfma<float> launch at q = 20,000, r = 65 (the REML
+# block product), from the same calibrated FETCH_SIZE / WRITE_SIZE runs:
+# 1.75 GB read + 74 MB written
+K_DENSE_F32_TRAFFIC_BYTES = 1.824e9
 # HBM bytes per launch at n = m = 50,000 with 1% missing, from two rocprofv3
 # --pmc runs of this script's GRM leg (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the
 # factors a 1 GiB device copy gives):
@@ -243,7 +248,13 @@ def reml_leg(ctx, n: int = 20000, m: int = 50000, k: int = 64, h2: float = 0.4, 
                              f"animals ({dt_s:.1f} s) and scaled by (dim / dim_sample)^3 = {scale:.2f} to "
                              f"n = {n}"}
         print(f"[bench] CPU oracle EM iteration at n={n_s}: {dt_s:.1f} s (x{scale:.2f})", flush=True)
-    # per launch of the dominant REML kernel: q^2 FP32 of Q read, 2 q^2 r flops
+    # the dominant kernel of the mixed run: the FP32 block product Q X_u,
+    # 2 q^2 r flops per launch (r = the probes + y)
+    dense_ms = symm_ms / max(symm_n, 1)
+    dense_tf = 2.0 * n * n * (k + 1) / (dense_ms * 1e-3) / 1e12 if symm_n else None
+    roof = {"kernel": "k_dense_mfma_f32", "bound": "mfma", "achieved": dense_tf, "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": dense_tf / PEAK_FP32_TFLOPS if dense_tf else None,
+            "traffic": K_DENSE_F32_TRAFFIC_BYTES if (n, k) == (20000, 64) else None}
     return {"n": n, "m": m, "probes": k, "iters_per_s_mixed": rate_m, "iters_per_s_fp64_plain": rate_p,
             "speedup": rate_m / rate_p, "iters_per_s_fp64_batched": Rb.iterations / tb,
             "iterations_mixed": Rm.iterations, "converged": Rm.converged,
@@ -260,7 +271,7 @@ def reml_leg(ctx, n: int = 20000, m: int = 50000, k: int = 64, h2: float = 0.4, 
                                       "seconds": ta, "iters_per_s": Ra.iterations / ta, "var_u": Ra.var_u,
                                       "var_e": Ra.var_e},
             "qinv": "eigen-bent G^-1 (invert_psd_matrix semantics)", "qinv_setup_s": t_inv,
-            "cpu_baseline": cpu_leg}
+            "roofline": roof, "cpu_baseline": cpu_leg}
 
 
 def c1_leg(ctx, n: int = 2000, m: int = 10000, h2: float = 0.3, ngroups: int = 20, seed: int = 20261203) -> dict:
@@ -572,6 +583,24 @@ def pcg_leg(ctx, n_gen: int = 10, per_gen: int = 100000, n_rec: int = 600000, n_
                                                     for k, v in sols.items()}
         res["time_speedup_mixed_vs_fp64"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
         res["tol_for_1e-6_ebv"] = tol_ebv
+        if tag == "PBLUP":
+            # the dominant kernel of an FP64 iteration, timed with HIP events in
+            # a solve of its own: the pass over W'W's animal rows that also
+            # applies lambda A^-1. Algorithmic bytes per launch: 12 per nonzero
+            # (FP64 value + int32 index) of A^-1 and of the animal rows (3 per
+            # record), two row pointers, x and y once per row
+            opts = mme.solve_options(precision=_lib.CB_PREC_FP64, tol=tol, max_iter=20000)
+            ctx.reset_profile()
+            ctx.set_profiling(True)
+            model.solve_device(lam, dB.data_ptr(), dim, dX.data_ptr(), dim, 1, opts)
+            ctx.set_profiling(False)
+            ms, cnt = ctx.kernel_time("k_csr_spmm_f64")
+            nbytes = 12.0 * (int(A.ptr[-1]) + 3 * n_rec) + 32.0 * n
+            gbs = nbytes / (ms / cnt * 1e-3) / 1e9 if cnt else None
+            res["roofline"] = {"kernel": "k_csr2_spmm (W'W animal rows + lambda A^-1)", "bound": "hbm",
+                               "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                               "frac": gbs / PEAK_HBM_GBS if gbs else None, "traffic": None,
+                               "avg_us": ms / cnt * 1e3 if cnt else None, "bytes_per_launch": nbytes}
         return res
 
     model = mme.MixedModel(fd, rec, y, n, ctx=ctx)
