@@ -652,6 +652,7 @@ def main() -> None:
     ap.add_argument("--no-c1", action="store_true", help="skip the n=2000 dense-exact REML leg")
     ap.add_argument("--no-c5", action="store_true", help="skip the 96-instance batched workload")
     ap.add_argument("--c5-only", action="store_true", help="only run the 96-instance workload")
+    ap.add_argument("--c5-workers", type=int, default=8, help="contexts (streams and host threads) serving C5")
     ap.add_argument("--pcg-only", action="store_true",
                     help="only run the PBLUP PCG leg and print its JSON object")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -688,7 +689,7 @@ def main() -> None:
         return
     if args.c5_only:
         ctx = _lib.Context(0)
-        print(json.dumps(c5_leg(ctx)), flush=True)
+        print(json.dumps(c5_leg(ctx, workers=args.c5_workers)), flush=True)
         return
 
     n, m = args.n, args.m
@@ -780,7 +781,7 @@ def main() -> None:
             out["c1_dense_exact"]["cpu_baseline"] = cpu_reml_c1()
     if not args.no_c5:
         torch.cuda.empty_cache()
-        out["c5_batched"] = c5_leg(ctx)
+        out["c5_batched"] = c5_leg(ctx, workers=args.c5_workers)
     if not args.no_cpu:
         torch.cuda.empty_cache()
         print("[bench] timing the CPU oracle sample", flush=True)
