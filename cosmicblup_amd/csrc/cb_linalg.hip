@@ -5,6 +5,13 @@
 // The symmetric eigensolve is rocSOLVER's divide-and-conquer (dsyevd), the
 // two n^3 products a plain rocBLAS dgemm; the small kernels around them
 // (ridge, floor and column scaling) are here.
+// When every eigenvalue of M + ridge I exceeds 1e-6 and none exceeds 1e6,
+// eps is 1e-6 and the floor changes nothing: the bent inverse is the
+// inverse. A row-sum bound (<= 1e6) and a successful Cholesky of
+// M + (ridge - 1e-6) I certify both, and the inverse then comes from
+// Cholesky (potrf + potri, O(n^3) in BLAS3) instead of the eigensolve, whose
+// tridiagonalisation is bound by HBM. Callers that ask for the extreme
+// eigenvalues take the eigensolve.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -43,14 +50,88 @@ __global__ void k_scale_cols(const double* __restrict__ U, double* __restrict__ 
 }
 
 
+// s[i] = sum_j |A[i, j]| + |shift| [j == i], a wave per row of the symmetric
+// column-major A (row i read as column i)
+__global__ __launch_bounds__(256) void k_abs_row_sums(const double* __restrict__ A, int64_t n, int64_t ld,
+                                                      double shift, double* __restrict__ s) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const double* a = A + i * ld;
+    double t = 0;
+    for (int64_t j = lane; j < n; j += 64) t += fabs(j == i ? a[j] + shift : a[j]);
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off);
+    if (lane == 0) s[i] = t;
+}
+
+// the upper triangle from the lower one (column-major)
+__global__ void k_mirror_lower(double* A, int64_t n, int64_t ld) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // row
+    for (int64_t j = blockIdx.y; j < n; j += gridDim.y)                // columns j > i
+        if (i < j) A[j * ld + i] = A[i * ld + j];
+}
+
+constexpr double FLOOR_MIN = 1e-6;   // eps = max(1e-12 max(l), 1e-6)
+constexpr double FLOOR_ROW_BOUND = 1e6;  // max(l) <= this keeps eps at 1e-6
+
 }  // namespace
 
 namespace cbamd {
+
+// The inverse of M + ridge I by Cholesky when the floor is certainly
+// inactive; false (nothing written) otherwise.
+static bool inverse_if_floor_inactive(cb_context* ctx, const double* dM, int64_t ldm, int64_t n, double ridge,
+                                      double* dOut, int64_t ldo, hipMemcpyKind in_kind, hipMemcpyKind out_kind) {
+    hipStream_t st = ctx->stream;
+    DevBuf<double> U, rs;
+    DevBuf<int> info;
+    U.alloc((size_t)n * n);
+    rs.alloc(n);
+    info.alloc(1);
+    copy_host_2d(st, U.p, n * 8, dM, ldm * 8, n * 8, n, in_kind);
+    // largest eigenvalue <= the largest absolute row sum
+    k_abs_row_sums<<<(unsigned)ceil_div(n, 4), 256, 0, st>>>(U.p, n, n, ridge, rs.p);
+    CB_HIP(hipGetLastError());
+    std::vector<double> hrs(n);
+    copy_host(st, hrs.data(), rs.p, n * 8, hipMemcpyDeviceToHost);
+    CB_HIP(hipStreamSynchronize(st));
+    double bound = 0;
+    for (double v : hrs) bound = std::max(bound, v);
+    if (!(bound <= FLOOR_ROW_BOUND)) return false;
+    // every eigenvalue above 1e-6: M + (ridge - 1e-6) I positive definite
+    k_add_diag<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(U.p, n, n, ridge - FLOOR_MIN);
+    CB_HIP(hipGetLastError());
+    BlasHandle h(st);
+    int hinfo = 0;
+    CB_BLAS(h, rocsolver_dpotrf(h.h, rocblas_fill_lower, (rocblas_int)n, U.p, (rocblas_int)n, info.p),
+            "rocsolver_dpotrf");
+    copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
+    CB_HIP(hipStreamSynchronize(st));
+    if (hinfo != 0) return false;
+    // (M + ridge I)^-1 = L^-T L^-1
+    copy_host_2d(st, U.p, n * 8, dM, ldm * 8, n * 8, n, in_kind);
+    if (ridge != 0.0) k_add_diag<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(U.p, n, n, ridge);
+    CB_HIP(hipGetLastError());
+    CB_BLAS(h, rocsolver_dpotrf(h.h, rocblas_fill_lower, (rocblas_int)n, U.p, (rocblas_int)n, info.p),
+            "rocsolver_dpotrf");
+    CB_BLAS(h, rocsolver_dpotri(h.h, rocblas_fill_lower, (rocblas_int)n, U.p, (rocblas_int)n, info.p),
+            "rocsolver_dpotri");
+    copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
+    CB_HIP(hipStreamSynchronize(st));
+    CB_REQUIRE(hinfo == 0, "cb_invert_psd: Cholesky inverse failed after the positive-definite check");
+    k_mirror_lower<<<dim3((unsigned)ceil_div(n, 256), (unsigned)std::min<int64_t>(n, 65535)), 256, 0, st>>>(U.p, n, n);
+    CB_HIP(hipGetLastError());
+    copy_host_2d(st, dOut, ldo * 8, U.p, n * 8, n * 8, n, out_kind);
+    CB_HIP(hipStreamSynchronize(st));
+    return true;
+}
 
 void invert_psd(cb_context* ctx, const double* dM, int64_t ldm, int64_t n, double ridge, double* dOut,
                 int64_t ldo, hipMemcpyKind in_kind, hipMemcpyKind out_kind, double* min_eig, double* max_eig) {
     CB_REQUIRE(n > 0 && n < (int64_t)1 << 31 && ldm >= n && ldo >= n, "cb_invert_psd: bad dimensions");
     set_device(ctx);  // also points tls_stream at this context's stream (drains, releases)
+    if (!min_eig && !max_eig && inverse_if_floor_inactive(ctx, dM, ldm, n, ridge, dOut, ldo, in_kind, out_kind))
+        return;
     hipStream_t st = ctx->stream;
     DevBuf<double> U, V, l, e;
     DevBuf<int> info;

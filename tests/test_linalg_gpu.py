@@ -41,3 +41,25 @@ def test_device_variant_in_place(cb, ctx):
     ev = np.linalg.eigvalsh(M)
     assert abs(lo - ev[0]) < 1e-12 * ev[-1] and abs(hi - ev[-1]) < 1e-12 * ev[-1]
     np.testing.assert_allclose(d.cpu().numpy(), og.invert_psd_matrix(M), rtol=0, atol=1e-10 * np.abs(np.linalg.inv(M)).max())
+
+
+@pytest.mark.parametrize("lam_min", [2e-6, 5e-7])
+def test_floor_inactive_cholesky_path_and_active_floor(cb, ctx, lam_min):
+    """A symmetric matrix with prescribed eigenvalues: its smallest just above
+    the 1e-6 floor (the library inverts by Cholesky, the floor being
+    inactive) or just below it (the eigensolve path floors it). Both match
+    the oracle's eigen-bent inverse."""
+    n = 400
+    rng = np.random.default_rng(5)
+    Qm, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.linspace(0.05, 3.0, n)
+    lam[0] = lam_min
+    M = (Qm * lam) @ Qm.T
+    M = 0.5 * (M + M.T)
+    got = linalg.invert_psd_matrix(M)
+    ref = og.invert_psd_matrix(M)
+    assert np.abs(got - ref).max() <= 1e-8 * np.abs(ref).max()
+    if lam_min > 1e-6:  # the Cholesky inverse is mirrored, so exactly symmetric
+        assert np.array_equal(got, got.T)
+    top = np.linalg.eigvalsh(got).max()
+    assert abs(top - 1.0 / max(lam_min, 1e-6)) < 1e-6 * top
