@@ -10,12 +10,19 @@
 // inverse. A row-sum bound (<= 1e6) and a successful Cholesky of
 // M + (ridge - 1e-6) I certify both, and the inverse then comes from
 // Cholesky (potrf + potri, O(n^3) in BLAS3) instead of the eigensolve, whose
-// tridiagonalisation is bound by HBM. Callers that ask for the extreme
-// eigenvalues take the eigensolve.
+// tridiagonalisation is bound by HBM. A VanRaden G has one eigenvalue below
+// the floor (its centring direction): that eigenpair (l1, v) comes from
+// inverse iteration on a Cholesky factor of M + ridge I + 1e-3 I, a Cholesky
+// of M + (ridge - 1e-6) I + v v' certifies that every other eigenvalue
+// exceeds 1e-6, and M + ridge I + (1e-6 - l1) v v' - the matrix with that one
+// eigenvalue floored - is inverted by Cholesky. Anything else (more than one
+// eigenvalue below the floor, no convergence, a failed certificate) takes
+// the eigensolve, as do callers that ask for the extreme eigenvalues.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "cb_blas.h"
@@ -71,17 +78,131 @@ __global__ void k_mirror_lower(double* A, int64_t n, int64_t ld) {
         if (i < j) A[j * ld + i] = A[i * ld + j];
 }
 
+// A[i, j] += s v_i v_j (column-major)
+__global__ void k_rank1(double* A, int64_t n, int64_t ld, double s, const double* __restrict__ v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int64_t j = blockIdx.y; j < n; j += gridDim.y) A[j * ld + i] += s * v[i] * v[j];
+}
+
 constexpr double FLOOR_MIN = 1e-6;   // eps = max(1e-12 max(l), 1e-6)
+constexpr double FLOOR_SHIFT = 1e-3;  // shift of the factor the inverse iteration solves with
+constexpr int FLOOR_MAX_IT = 60;
 constexpr double FLOOR_ROW_BOUND = 1e6;  // max(l) <= this keeps eps at 1e-6
 
 }  // namespace
 
 namespace cbamd {
 
-// The inverse of M + ridge I by Cholesky when the floor is certainly
-// inactive; false (nothing written) otherwise.
-static bool inverse_if_floor_inactive(cb_context* ctx, const double* dM, int64_t ldm, int64_t n, double ridge,
-                                      double* dOut, int64_t ldo, hipMemcpyKind in_kind, hipMemcpyKind out_kind) {
+// U := the Cholesky inverse of the positive-definite matrix in U (lower
+// factor, then mirrored), copied to out
+static void cholesky_inverse_out(cb_context* ctx, BlasHandle& h, DevBuf<double>& U, DevBuf<int>& info, int64_t n,
+                                 double* dOut, int64_t ldo, hipMemcpyKind out_kind) {
+    hipStream_t st = ctx->stream;
+    int hinfo = 0;
+    CB_BLAS(h, rocsolver_dpotrf(h.h, rocblas_fill_lower, (rocblas_int)n, U.p, (rocblas_int)n, info.p),
+            "rocsolver_dpotrf");
+    CB_BLAS(h, rocsolver_dpotri(h.h, rocblas_fill_lower, (rocblas_int)n, U.p, (rocblas_int)n, info.p),
+            "rocsolver_dpotri");
+    copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
+    CB_HIP(hipStreamSynchronize(st));
+    CB_REQUIRE(hinfo == 0, "cb_invert_psd: Cholesky inverse failed after the positive-definite check");
+    k_mirror_lower<<<dim3((unsigned)ceil_div(n, 256), (unsigned)std::min<int64_t>(n, 65535)), 256, 0, st>>>(U.p, n, n);
+    CB_HIP(hipGetLastError());
+    copy_host_2d(st, dOut, ldo * 8, U.p, n * 8, n * 8, n, out_kind);
+    CB_HIP(hipStreamSynchronize(st));
+}
+
+// The one eigenpair below the floor and the bent inverse by Cholesky (see
+// the file header); false (nothing written) when that does not apply. U
+// holds M + ridge I on entry (it is overwritten).
+static bool inverse_with_one_floored(cb_context* ctx, BlasHandle& h, DevBuf<double>& U, DevBuf<int>& info,
+                                     int64_t n, double bound, double* dOut, int64_t ldo, hipMemcpyKind out_kind) {
+    hipStream_t st = ctx->stream;
+    DevBuf<double> B, dv, dw;
+    B.alloc((size_t)n * n);
+    dv.alloc(n);
+    dw.alloc(n);
+    CB_HIP(hipMemcpyAsync(B.p, U.p, (size_t)n * n * 8, hipMemcpyDeviceToDevice, st));  // M + ridge I, kept
+    k_add_diag<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(U.p, n, n, FLOOR_SHIFT);
+    CB_HIP(hipGetLastError());
+    int hinfo = 0;
+    CB_BLAS(h, rocsolver_dpotrf(h.h, rocblas_fill_lower, (rocblas_int)n, U.p, (rocblas_int)n, info.p),
+            "rocsolver_dpotrf");
+    copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
+    CB_HIP(hipStreamSynchronize(st));
+    if (hinfo != 0) return false;
+    // inverse iteration from the centring direction, lightly perturbed
+    std::vector<double> v(n), w(n), y(n);
+    double nv = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        v[i] = 1.0 + 1e-3 * std::sin(0.7 * (double)i);
+        nv += v[i] * v[i];
+    }
+    for (double& x : v) x /= std::sqrt(nv);
+    const double one = 1.0, zero = 0.0;
+    CB_ROCBLAS(rocblas_set_pointer_mode(h.h, rocblas_pointer_mode_host));
+    double l1 = 0, res = 0;
+    // the direction converges by (l1 + shift) / (l2 + shift) per step until
+    // its change reaches the rounding level; two more steps polish it
+    int polish = -1;
+    for (int it = 0; it < FLOOR_MAX_IT && polish != 0; ++it) {
+        copy_host(st, dw.p, v.data(), n * 8, hipMemcpyHostToDevice);
+        CB_BLAS(h, rocsolver_dpotrs(h.h, rocblas_fill_lower, (rocblas_int)n, 1, U.p, (rocblas_int)n, dw.p,
+                                    (rocblas_int)n), "rocsolver_dpotrs");
+        copy_host(st, w.data(), dw.p, n * 8, hipMemcpyDeviceToHost);
+        CB_HIP(hipStreamSynchronize(st));
+        double nw = 0, dot = 0;
+        for (int64_t i = 0; i < n; ++i) nw += w[i] * w[i];
+        nw = std::sqrt(nw);
+        if (!(nw > 0)) return false;
+        for (int64_t i = 0; i < n; ++i) {
+            w[i] /= nw;
+            dot += w[i] * v[i];
+        }
+        // sign-aligned change of the direction
+        double dchg = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            const double d = w[i] - (dot < 0 ? -v[i] : v[i]);
+            dchg = std::max(dchg, std::fabs(d));
+        }
+        v.swap(w);
+        if (polish > 0) --polish;
+        else if (polish < 0 && it >= 3 && dchg < 1e-13) polish = 2;
+    }
+    if (polish != 0) return false;  // no convergence: the eigensolve decides
+    // Rayleigh quotient and residual against M + ridge I
+    copy_host(st, dv.p, v.data(), n * 8, hipMemcpyHostToDevice);
+    CB_BLAS(h, rocblas_dgemv(h.h, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n, &one, B.p, (rocblas_int)n,
+                             dv.p, 1, &zero, dw.p, 1), "rocblas_dgemv");
+    copy_host(st, y.data(), dw.p, n * 8, hipMemcpyDeviceToHost);
+    CB_HIP(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < n; ++i) l1 += v[i] * y[i];
+    for (int64_t i = 0; i < n; ++i) res = std::max(res, std::fabs(y[i] - l1 * v[i]));
+    if (!(l1 < FLOOR_MIN) || !(res <= 1e-12 * bound)) return false;
+    // every other eigenvalue above the floor: M + ridge I - 1e-6 I + v v' positive definite
+    CB_HIP(hipMemcpyAsync(U.p, B.p, (size_t)n * n * 8, hipMemcpyDeviceToDevice, st));
+    k_add_diag<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(U.p, n, n, -FLOOR_MIN);
+    const dim3 g2((unsigned)ceil_div(n, 256), (unsigned)std::min<int64_t>(n, 65535));
+    k_rank1<<<g2, 256, 0, st>>>(U.p, n, n, 1.0, dv.p);
+    CB_HIP(hipGetLastError());
+    CB_BLAS(h, rocsolver_dpotrf(h.h, rocblas_fill_lower, (rocblas_int)n, U.p, (rocblas_int)n, info.p),
+            "rocsolver_dpotrf");
+    copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
+    CB_HIP(hipStreamSynchronize(st));
+    if (hinfo != 0) return false;
+    // the bent matrix: that eigenvalue moved to the floor
+    CB_HIP(hipMemcpyAsync(U.p, B.p, (size_t)n * n * 8, hipMemcpyDeviceToDevice, st));
+    k_rank1<<<g2, 256, 0, st>>>(U.p, n, n, FLOOR_MIN - l1, dv.p);
+    CB_HIP(hipGetLastError());
+    cholesky_inverse_out(ctx, h, U, info, n, dOut, ldo, out_kind);
+    return true;
+}
+
+// The bent inverse without the eigensolve, when the floor is inactive or
+// acts on one eigenvalue only; false (nothing written) otherwise.
+static bool inverse_by_cholesky(cb_context* ctx, const double* dM, int64_t ldm, int64_t n, double ridge,
+                                double* dOut, int64_t ldo, hipMemcpyKind in_kind, hipMemcpyKind out_kind) {
     hipStream_t st = ctx->stream;
     DevBuf<double> U, rs;
     DevBuf<int> info;
@@ -107,22 +228,13 @@ static bool inverse_if_floor_inactive(cb_context* ctx, const double* dM, int64_t
             "rocsolver_dpotrf");
     copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
     CB_HIP(hipStreamSynchronize(st));
-    if (hinfo != 0) return false;
-    // (M + ridge I)^-1 = L^-T L^-1
+    // U := M + ridge I again
     copy_host_2d(st, U.p, n * 8, dM, ldm * 8, n * 8, n, in_kind);
     if (ridge != 0.0) k_add_diag<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(U.p, n, n, ridge);
     CB_HIP(hipGetLastError());
-    CB_BLAS(h, rocsolver_dpotrf(h.h, rocblas_fill_lower, (rocblas_int)n, U.p, (rocblas_int)n, info.p),
-            "rocsolver_dpotrf");
-    CB_BLAS(h, rocsolver_dpotri(h.h, rocblas_fill_lower, (rocblas_int)n, U.p, (rocblas_int)n, info.p),
-            "rocsolver_dpotri");
-    copy_host(st, &hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost);
-    CB_HIP(hipStreamSynchronize(st));
-    CB_REQUIRE(hinfo == 0, "cb_invert_psd: Cholesky inverse failed after the positive-definite check");
-    k_mirror_lower<<<dim3((unsigned)ceil_div(n, 256), (unsigned)std::min<int64_t>(n, 65535)), 256, 0, st>>>(U.p, n, n);
-    CB_HIP(hipGetLastError());
-    copy_host_2d(st, dOut, ldo * 8, U.p, n * 8, n * 8, n, out_kind);
-    CB_HIP(hipStreamSynchronize(st));
+    if (hinfo != 0) return inverse_with_one_floored(ctx, h, U, info, n, bound, dOut, ldo, out_kind);
+    // no eigenvalue below the floor: (M + ridge I)^-1 = L^-T L^-1
+    cholesky_inverse_out(ctx, h, U, info, n, dOut, ldo, out_kind);
     return true;
 }
 
@@ -130,7 +242,7 @@ void invert_psd(cb_context* ctx, const double* dM, int64_t ldm, int64_t n, doubl
                 int64_t ldo, hipMemcpyKind in_kind, hipMemcpyKind out_kind, double* min_eig, double* max_eig) {
     CB_REQUIRE(n > 0 && n < (int64_t)1 << 31 && ldm >= n && ldo >= n, "cb_invert_psd: bad dimensions");
     set_device(ctx);  // also points tls_stream at this context's stream (drains, releases)
-    if (!min_eig && !max_eig && inverse_if_floor_inactive(ctx, dM, ldm, n, ridge, dOut, ldo, in_kind, out_kind))
+    if (!min_eig && !max_eig && inverse_by_cholesky(ctx, dM, ldm, n, ridge, dOut, ldo, in_kind, out_kind))
         return;
     hipStream_t st = ctx->stream;
     DevBuf<double> U, V, l, e;

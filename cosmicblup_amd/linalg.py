@@ -22,10 +22,15 @@ def invert_psd_matrix(M: np.ndarray, ridge: float = 0.0, ctx: _lib.Context | Non
 
 
 def invert_psd_device(dM: int, ldm: int, n: int, dout: int, ldo: int, ridge: float = 0.0,
-                      ctx: _lib.Context | None = None) -> tuple[float, float]:
-    """Same on device memory (may alias). Returns the smallest and largest
-    eigenvalue of M + ridge I before the floor."""
+                      ctx: _lib.Context | None = None, eigenvalues: bool = False):
+    """Same on device memory (may alias). With eigenvalues=True returns the
+    smallest and largest eigenvalue of M + ridge I before the floor (the
+    library then takes the full eigensolve); otherwise (None, None), and the
+    library inverts by Cholesky where the floor allows it."""
     ctx = ctx or _lib.default_context()
+    if not eigenvalues:
+        _lib.check(_lib.lib().cb_invert_psd_device(ctx.handle, dM, ldm, n, float(ridge), dout, ldo, None, None))
+        return None, None
     lo, hi = C.c_double(), C.c_double()
     _lib.check(_lib.lib().cb_invert_psd_device(ctx.handle, dM, ldm, n, float(ridge), dout, ldo,
                                                C.byref(lo), C.byref(hi)))

@@ -343,7 +343,12 @@ int cb_reml_copy_probes(cb_mme* mme, const cb_reml_options* opts, double* out, i
  * eps = max(1e-12 max(l), 1e-6), l <- max(l, eps), out = U diag(1/l) U^T
  * (G^-1 at relationship_builder.cpp:471). Host arrays (M, out: leading
  * dimensions ldm, ldo). min_eig / max_eig (may be NULL) receive the extreme
- * eigenvalues of M + ridge I before the floor. */
+ * eigenvalues of M + ridge I before the floor. With both NULL the library
+ * avoids the eigensolve where the floor allows: no eigenvalue below 1e-6
+ * (inverse by Cholesky) or exactly one (that eigenpair by inverse iteration,
+ * floored by a rank-1 update, then Cholesky), each certified by a Cholesky
+ * factorisation, with the row sums bounding max(l) by 1e6; otherwise, and
+ * whenever an eigenvalue is asked for, the symmetric eigensolve. */
 int cb_invert_psd(cb_context* ctx, const double* M, int64_t ldm, int64_t n, double ridge, double* out,
                   int64_t ldo, double* min_eig, double* max_eig);
 /* Same with M and out in device memory of ctx's device (they may alias). */

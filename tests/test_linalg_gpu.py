@@ -37,29 +37,31 @@ def test_device_variant_in_place(cb, ctx):
     M = B @ B.T / 150
     d = torch.from_numpy(M.copy()).cuda()
     torch.cuda.synchronize()
-    lo, hi = linalg.invert_psd_device(d.data_ptr(), 128, 128, d.data_ptr(), 128)
+    lo, hi = linalg.invert_psd_device(d.data_ptr(), 128, 128, d.data_ptr(), 128, eigenvalues=True)
     ev = np.linalg.eigvalsh(M)
     assert abs(lo - ev[0]) < 1e-12 * ev[-1] and abs(hi - ev[-1]) < 1e-12 * ev[-1]
     np.testing.assert_allclose(d.cpu().numpy(), og.invert_psd_matrix(M), rtol=0, atol=1e-10 * np.abs(np.linalg.inv(M)).max())
 
 
-@pytest.mark.parametrize("lam_min", [2e-6, 5e-7])
-def test_floor_inactive_cholesky_path_and_active_floor(cb, ctx, lam_min):
-    """A symmetric matrix with prescribed eigenvalues: its smallest just above
-    the 1e-6 floor (the library inverts by Cholesky, the floor being
-    inactive) or just below it (the eigensolve path floors it). Both match
-    the oracle's eigen-bent inverse."""
+@pytest.mark.parametrize("small", [(2e-6,), (5e-7,), (-3e-9,), (5e-7, 3e-7)])
+def test_bent_inverse_paths_match_oracle(cb, ctx, small):
+    """A symmetric matrix with prescribed eigenvalues around the 1e-6 floor:
+    all above it (the floor is inactive: Cholesky inverse), one below it (one
+    eigenpair by inverse iteration, floored by a rank-1 update, Cholesky
+    inverse), two below it (the eigensolve). Each matches the oracle's
+    eigen-bent inverse, and the floored directions sit at 1 / 1e-6."""
     n = 400
     rng = np.random.default_rng(5)
     Qm, _ = np.linalg.qr(rng.standard_normal((n, n)))
     lam = np.linspace(0.05, 3.0, n)
-    lam[0] = lam_min
+    lam[: len(small)] = small
     M = (Qm * lam) @ Qm.T
     M = 0.5 * (M + M.T)
     got = linalg.invert_psd_matrix(M)
     ref = og.invert_psd_matrix(M)
     assert np.abs(got - ref).max() <= 1e-8 * np.abs(ref).max()
-    if lam_min > 1e-6:  # the Cholesky inverse is mirrored, so exactly symmetric
+    if len(small) == 1:  # the Cholesky inverses are mirrored, so exactly symmetric
         assert np.array_equal(got, got.T)
-    top = np.linalg.eigvalsh(got).max()
-    assert abs(top - 1.0 / max(lam_min, 1e-6)) < 1e-6 * top
+    ev = np.sort(np.linalg.eigvalsh(got))[::-1]
+    want = np.sort(1.0 / np.maximum(small, 1e-6))[::-1]
+    np.testing.assert_allclose(ev[: len(small)], want, rtol=1e-6)
