@@ -50,6 +50,7 @@ namespace {
 constexpr int DOT_CHUNKS = 128;   // row chunks of the two-stage column dot
 constexpr double REPLACE_DROP2 = 1e-6;  // squared residual drop that triggers a replacement
 constexpr int64_t LONG_CHUNK = 4096;    // nonzeros per block of a long W'W row
+constexpr int64_t MULTI_FIN_MAX = 16;   // long rows of several chunks a dot chunk may finish
 constexpr int MF_ROWS = 128;      // rows per k_dense_mfma tile (4 waves x 32)
 constexpr int MF_KC = 32;         // k per chunk
 constexpr int MF_KCP = MF_KC + 4; // LDS pitch of a staged X column
@@ -57,6 +58,84 @@ constexpr int MF_MAXNT = 5;       // 16-column subtiles per launch (80 columns)
 
 // ---------------------------------------------------------------------------
 // Generic device kernels
+
+// The few long rows of W'W (fixed effects; the mean row touches every
+// unknown) in two deterministic stages: block per (chunk of <= LONG_CHUNK
+// nonzeros, column) -> partial, then a wave per (row, column) summing its
+// chunks in order. FP64 accumulation. Thread t adds nonzeros t, t + 256, ...
+// of the chunk in that order; four terms' loads are issued before their
+// gathers and their adds. When row1 is given, a chunk that is the whole of
+// its row also writes the row's result y_c[row] (the value the finishing
+// sum of that one partial gives, 0 + part), so only rows of several chunks
+// are left to finish.
+template <class TM, class TV>
+__device__ __forceinline__ void long_chunk_part(int64_t ch, int64_t nchunks, const int64_t* __restrict__ cbeg,
+                                                const int64_t* __restrict__ cend, const int32_t* __restrict__ col,
+                                                const TM* __restrict__ val, const TV* __restrict__ x,
+                                                double* __restrict__ part_c, const int32_t* __restrict__ row1,
+                                                TV* __restrict__ y_c) {
+    __shared__ double red[4];
+    double acc = 0;
+    const int64_t e = cend[ch];
+    int64_t k = cbeg[ch] + threadIdx.x;
+    for (; k + 3 * 256 < e; k += 4 * 256) {
+        const int32_t c0 = col[k], c1 = col[k + 256], c2 = col[k + 512], c3 = col[k + 768];
+        const TM v0 = val[k], v1 = val[k + 256], v2 = val[k + 512], v3 = val[k + 768];
+        const TV x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+        acc += (double)v0 * (double)x0;
+        acc += (double)v1 * (double)x1;
+        acc += (double)v2 * (double)x2;
+        acc += (double)v3 * (double)x3;
+    }
+    for (; k < e; k += 256) acc += (double)val[k] * (double)x[col[k]];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double v = (red[0] + red[1]) + (red[2] + red[3]);
+        part_c[ch] = v;
+        if (row1 && row1[ch] >= 0) y_c[row1[ch]] = (TV)(0.0 + v);
+    }
+}
+
+// One lane's share of a CSR row sum: val[k] x[col[k]] for k = b, b + s, ...
+// < e, added in that order. The first two terms' indices and values are
+// loaded together, then their two gathers, so that a row of up to 2 s
+// nonzeros waits for one round trip of each kind instead of one per term;
+// the loads of terms past the end read entry 0 (and x[0]) and are not
+// added (the arrays and x hold at least one entry).
+template <class TM, class TV>
+struct CsrLane {
+    int64_t k0, k1;
+    bool t0, t1;
+    int32_t c0, c1;
+    TM v0, v1;
+    TV x0, x1;
+    __device__ __forceinline__ void indices(const int32_t* __restrict__ col, const TM* __restrict__ val,
+                                            int64_t b, int64_t e, int s, bool lane_on) {
+        t0 = lane_on && b < e;
+        t1 = lane_on && b + s < e;
+        k0 = t0 ? b : 0;
+        k1 = t1 ? b + s : 0;
+        c0 = col[k0];
+        c1 = col[k1];
+        v0 = val[k0];
+        v1 = val[k1];
+    }
+    __device__ __forceinline__ void gather(const TV* __restrict__ x) {
+        x0 = x[t0 ? c0 : 0];
+        x1 = x[t1 ? c1 : 0];
+    }
+    // the two terms, then the rest of the row one term at a time
+    __device__ __forceinline__ TV sum(const int32_t* __restrict__ col, const TM* __restrict__ val,
+                                      const TV* __restrict__ x, int64_t b, int64_t e, int s, TV acc) const {
+        if (t0) acc += (TV)v0 * x0;
+        if (t1) acc += (TV)v1 * x1;
+        if (t1)
+            for (int64_t k = b + 2 * s; k < e; k += s) acc += (TV)val[k] * x[col[k]];
+        return acc;
+    }
+};
 
 // Y[row, c] = alpha * sum_k val(row,k) X[col(k), c] (+ Y[row, c] when
 // accumulating) for rows in [row0, row1): S lanes per (row, c), lane j
@@ -76,8 +155,11 @@ __global__ __launch_bounds__(256) void k_csr_spmm(const int64_t* __restrict__ pt
     TV acc = 0;
     if (row < row1) {  // no return before the butterfly: every lane of a group takes part
         const TV* x = X + c * ldx + col_off;
-        const int64_t e = ptr[row + 1];
-        for (int64_t k = ptr[row] + sub; k < e; k += S) acc += (TV)val[k] * x[col[k]];
+        const int64_t b = ptr[row] + sub, e = ptr[row + 1];
+        CsrLane<TM, TV> L;
+        L.indices(col, val, b, e, S, true);
+        L.gather(x);
+        acc = L.sum(col, val, x, b, e, S, acc);
     }
 #pragma unroll
     for (int off = S / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, S);
@@ -91,28 +173,45 @@ __global__ __launch_bounds__(256) void k_csr_spmm(const int64_t* __restrict__ pt
 // row1) (the Q term for rows >= p only): W'W's rows and a CSR Q's in one pass
 // over the rows, S lanes per (row, c). Each matrix keeps its own lane count
 // (s1 <= S for W, S for Q) and butterfly, so the two sums and the final
-// y = w + alpha q are the ones the separate kernels form, bit for bit.
+// y = w + alpha q are the ones the separate kernels form, bit for bit. Both
+// matrices' first terms are loaded before either's gathers.
+//
+// The first nlc workgroups instead form the chunk partials of W'W's long rows
+// [0, row0) (k_csr_spmm_chunks' work, same sums), so that those rows' gathers
+// run beside the short rows'; with lc_row1 the rows of one chunk are written
+// here, the others are finished later from the partials.
 template <class TM, class TV, int S>
 __global__ __launch_bounds__(256) void k_csr2_spmm(const int64_t* __restrict__ ptr1, const int32_t* __restrict__ col1,
                                                    const TM* __restrict__ val1, int s1,
                                                    const int64_t* __restrict__ ptr2, const int32_t* __restrict__ col2,
                                                    const TM* __restrict__ val2, int64_t p, int64_t row0,
                                                    int64_t row1, const TV* __restrict__ X, int64_t ldx,
-                                                   TV* __restrict__ Y, int64_t ldy, TV alpha) {
-    const int sub = (int)threadIdx.x & (S - 1);
-    const int64_t row = row0 + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / S;
+                                                   TV* __restrict__ Y, int64_t ldy, TV alpha, int64_t nlc,
+                                                   const int64_t* __restrict__ lc_beg,
+                                                   const int64_t* __restrict__ lc_end, double* __restrict__ lpart,
+                                                   const int32_t* __restrict__ lc_row1) {
     const int64_t c = blockIdx.y;
-    TV a1 = 0, a2 = 0;
     const TV* x = X + c * ldx;
+    if ((int64_t)blockIdx.x < nlc) {
+        long_chunk_part<TM, TV>(blockIdx.x, nlc, lc_beg, lc_end, col1, val1, x, lpart + c * nlc, lc_row1,
+                                Y + c * ldy);
+        return;
+    }
+    const int sub = (int)threadIdx.x & (S - 1);
+    const int64_t row = row0 + (((int64_t)blockIdx.x - nlc) * blockDim.x + threadIdx.x) / S;
+    TV a1 = 0, a2 = 0;
     if (row < row1) {  // no return before the butterflies: every lane of a group takes part
-        if (sub < s1) {
-            const int64_t e = ptr1[row + 1];
-            for (int64_t k = ptr1[row] + sub; k < e; k += s1) a1 += (TV)val1[k] * x[col1[k]];
-        }
-        if (row >= p) {
-            const int64_t r2 = row - p, e = ptr2[r2 + 1];
-            for (int64_t k = ptr2[r2] + sub; k < e; k += S) a2 += (TV)val2[k] * x[p + col2[k]];
-        }
+        const bool has2 = row >= p;
+        const int64_t r2 = has2 ? row - p : 0;
+        const int64_t b1 = ptr1[row] + sub, e1 = ptr1[row + 1];
+        const int64_t b2 = ptr2[r2] + sub, e2 = has2 ? ptr2[r2 + 1] : 0;
+        CsrLane<TM, TV> L1, L2;
+        L1.indices(col1, val1, b1, e1, s1, sub < s1);
+        L2.indices(col2, val2, b2, e2, S, has2);
+        L1.gather(x);
+        L2.gather(x + p);
+        a1 = L1.sum(col1, val1, x, b1, e1, s1, a1);
+        a2 = L2.sum(col2, val2, x + p, b2, e2, S, a2);
     }
     for (int off = s1 / 2; off > 0; off >>= 1) a1 += __shfl_xor(a1, off, S);
 #pragma unroll
@@ -155,26 +254,15 @@ void csr_spmm(hipStream_t st, int S, int r, const int64_t* ptr, const int32_t* c
     CB_HIP(hipGetLastError());
 }
 
-// The few long rows of W'W (fixed effects; the mean row touches every
-// unknown) in two deterministic stages: block per (chunk of <= LONG_CHUNK
-// nonzeros, column) -> partial, then thread per (row, column) summing its
-// chunks in order. FP64 accumulation.
 template <class TM, class TV>
 __global__ __launch_bounds__(256) void k_csr_spmm_chunks(const int64_t* __restrict__ cbeg,
                                                          const int64_t* __restrict__ cend,
                                                          int64_t nchunks, const int32_t* __restrict__ col,
                                                          const TM* __restrict__ val, const TV* __restrict__ X,
                                                          int64_t ldx, double* __restrict__ part) {
-    __shared__ double red[4];
-    const int64_t ch = blockIdx.x;
     const int64_t c = blockIdx.y;
-    const TV* x = X + c * ldx;
-    double acc = 0;
-    for (int64_t k = cbeg[ch] + threadIdx.x; k < cend[ch]; k += 256) acc += (double)val[k] * (double)x[col[k]];
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) part[c * nchunks + ch] = (red[0] + red[1]) + (red[2] + red[3]);
+    long_chunk_part<TM, TV>(blockIdx.x, nchunks, cbeg, cend, col, val, X + c * ldx, part + c * nchunks, nullptr,
+                            nullptr);
 }
 
 // a wave per (row, column): lane l sums the row's chunks l, l + 64, ... in
@@ -734,6 +822,46 @@ __global__ __launch_bounds__(NTH) void k_col_dots(const T* __restrict__ A, const
     if (threadIdx.x == 0) part[(int64_t)c * DOT_CHUNKS + blockIdx.x] = red[0];
 }
 
+// k_col_dots with the long rows of B = C P that span several chunks finished
+// first: each such row (listed in multi) is summed from its chunk partials (a
+// wave per row, the order of k_csr_chunks_finish) by the dot chunk whose row
+// range holds it and written to B before that chunk's dot reads it. Rows of
+// one chunk were written by the chunk pass.
+template <class T, int NTH = 256>
+__global__ __launch_bounds__(NTH) void k_col_dots_fin(const T* __restrict__ A, T* __restrict__ B, int64_t ld,
+                                                      int64_t rows, double* __restrict__ part,
+                                                      const int64_t* __restrict__ rstart, int64_t long_rows,
+                                                      int64_t nchunks, const double* __restrict__ lpart,
+                                                      const int64_t* __restrict__ multi, int64_t n_multi) {
+    __shared__ double red[NTH];
+    const int c = blockIdx.y;
+    const int64_t per = (rows + DOT_CHUNKS - 1) / DOT_CHUNKS;
+    const int64_t r0 = (int64_t)blockIdx.x * per, r1 = min(rows, r0 + per);
+    if (r0 < min(r1, long_rows)) {  // the same for the whole workgroup
+        const int lane = threadIdx.x & 63;
+        for (int64_t j = threadIdx.x >> 6; j < n_multi; j += NTH / 64) {
+            const int64_t row = multi[j];
+            if (row < r0 || row >= r1) continue;
+            double s = 0;
+            for (int64_t k = rstart[row] + lane; k < rstart[row + 1]; k += 64) s += lpart[c * nchunks + k];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+            if (lane == 0) B[(int64_t)c * ld + row] = (T)s;
+        }
+        __syncthreads();
+    }
+    double s = 0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += NTH)
+        s += (double)A[(int64_t)c * ld + i] * (double)B[(int64_t)c * ld + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = NTH / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)c * DOT_CHUNKS + blockIdx.x] = red[0];
+}
+
 // vectors at least this long take 1024 threads per dot chunk
 constexpr int64_t DOT_LONG_ROWS = int64_t(1) << 18;
 
@@ -1228,6 +1356,9 @@ struct cb_mme {
     // chunks of at most LONG_CHUNK nonzeros of the long rows, in row order
     int64_t n_lchunks = 0;
     DevBuf<int64_t> lc_beg, lc_end, lr_start;
+    DevBuf<int32_t> lc_row1;   // per chunk: its row when that row is one chunk, else -1
+    DevBuf<int64_t> lr_multi;  // the long rows of more than one chunk
+    int64_t n_multi = 0;
     DevBuf<double> lc_part;
     std::vector<double> wtw_diag;  // diagonal of W'W
     std::vector<double> xtx;       // p x p block (constant in lambda)
@@ -1589,32 +1720,43 @@ void q_apply(cb_mme* M, const T* X, int64_t ldx, T* Y, int64_t ldy, int r, T alp
     }
 }
 
-// Y = C(lambda) X for an r-column block, arithmetic in T.
+// Y = C(lambda) X for an r-column block, arithmetic in T. With lcp_out the
+// long rows of W'W that span several chunks may be left unfinished when the
+// pass over the short rows carries the chunk partials (its condition puts
+// every long row among the fixed effects, where no Q term lands) and there
+// are at most MULTI_FIN_MAX of them: *lcp_out then points at the partials,
+// which the caller must finish (k_col_dots_fin) before Y's rows [0,
+// long_rows) are read; otherwise *lcp_out is null and Y is complete.
 template <class T>
-void apply_C(cb_mme* M, double lambda, const T* X, T* Y, int r) {
+void apply_C(cb_mme* M, double lambda, const T* X, T* Y, int r, double** lcp_out = nullptr) {
     cb_context* ctx = M->ctx;
     hipStream_t st = ctx->stream;
     const int64_t ld = M->ldv;
-    // W'W part (overwrites Y)
-    if (M->long_rows > 0) {
-        double* lcp = scratch_or<double>(M, SK_LCPART, M->lc_part, (size_t)std::max<int64_t>(1, M->n_lchunks) * r);
-        if (M->n_lchunks > 0)
-            k_csr_spmm_chunks<T, T><<<dim3((unsigned)M->n_lchunks, (unsigned)r), 256, 0, st>>>(
-            M->lc_beg.p, M->lc_end.p, M->n_lchunks, M->wc.p, wvals<T>(M), X, ld, lcp);
-        k_csr_chunks_finish<T><<<dim3(blocks(M->long_rows, 4), (unsigned)r), 256, 0, st>>>(
-            M->lr_start.p, M->long_rows, M->n_lchunks, lcp, Y, ld);
-    }
     // a CSR Q rides in the pass over W'W's short rows when those hold every
     // random-effect row and W'W's rows need no more lanes than Q's
     const bool fused = M->qkind != 0 && M->long_rows <= M->p && M->dim > M->long_rows && M->q_lanes >= 2 &&
                        M->ww_lanes <= M->q_lanes;
+    const bool defer = lcp_out != nullptr && fused && M->long_rows > 0 && M->n_multi <= MULTI_FIN_MAX;
+    if (lcp_out) *lcp_out = nullptr;
+    double* lcp = nullptr;
+    // W'W part (overwrites Y); the long rows' chunk partials ride in the
+    // fused pass when there is one
+    if (M->long_rows > 0) {
+        lcp = scratch_or<double>(M, SK_LCPART, M->lc_part, (size_t)std::max<int64_t>(1, M->n_lchunks) * r);
+        if (M->n_lchunks > 0 && !fused)
+            k_csr_spmm_chunks<T, T><<<dim3((unsigned)M->n_lchunks, (unsigned)r), 256, 0, st>>>(
+            M->lc_beg.p, M->lc_end.p, M->n_lchunks, M->wc.p, wvals<T>(M), X, ld, lcp);
+    }
     if (fused) {
         const int S = M->q_lanes;
-        const dim3 grid((unsigned)std::max<int64_t>(1, ((M->dim - M->long_rows) * S + 255) / 256), (unsigned)r);
+        const int64_t nlc = M->long_rows > 0 ? M->n_lchunks : 0;
+        const dim3 grid((unsigned)(nlc + std::max<int64_t>(1, ((M->dim - M->long_rows) * S + 255) / 256)),
+                        (unsigned)r);
         timed(ctx, sizeof(T) == 8 ? "k_csr_spmm_f64" : "k_csr_spmm_f32", [&] {
 #define CB_CSR2(S_)                                                                                                  \
     k_csr2_spmm<T, T, S_><<<grid, 256, 0, st>>>(M->wp.p, M->wc.p, wvals<T>(M), M->ww_lanes, M->qp.p, M->qc.p,        \
-                                                qvals<T>(M), M->p, M->long_rows, M->dim, X, ld, Y, ld, (T)lambda)
+                                                qvals<T>(M), M->p, M->long_rows, M->dim, X, ld, Y, ld, (T)lambda, nlc, \
+                                                M->lc_beg.p, M->lc_end.p, lcp, M->lc_row1.p)
             switch (S) {
                 case 2: CB_CSR2(2); break;
                 case 4: CB_CSR2(4); break;
@@ -1630,6 +1772,15 @@ void apply_C(cb_mme* M, double lambda, const T* X, T* Y, int r) {
         });
     }
     CB_HIP(hipGetLastError());
+    if (M->long_rows > 0) {
+        if (defer) {
+            *lcp_out = lcp;
+        } else {
+            k_csr_chunks_finish<T><<<dim3(blocks(M->long_rows, 4), (unsigned)r), 256, 0, st>>>(
+                M->lr_start.p, M->long_rows, M->n_lchunks, lcp, Y, ld);
+            CB_HIP(hipGetLastError());
+        }
+    }
     // lambda Q part; the FP32 copy of a dense Q holds Q - mu v v', and
     // lambda mu v (v' x_u) is added in the same pass that adds lambda Q x_u
     if (M->qkind == 0 && M->stiff && sizeof(T) == 4) {
@@ -1764,12 +1915,21 @@ void cg_iteration(cb_mme* M, double lambda, TX* X, T* R, T* Z, T* P, T* Qp, ColS
     const int64_t ld = M->ldv, dim = M->dim;
     const dim3 g2(blocks(dim), (unsigned)r);
     const dim3 gd(DOT_CHUNKS, (unsigned)r);
-    apply_C<T>(M, lambda, P, Qp, r);
+    double* lcp = nullptr;
+    apply_C<T>(M, lambda, P, Qp, r, &lcp);
     const bool long_dots = dim >= DOT_LONG_ROWS;
-    if (long_dots)
+    if (lcp) {  // Qp's long rows are finished by the dot
+        if (long_dots)
+            k_col_dots_fin<T, 1024><<<gd, 1024, 0, st>>>(P, Qp, ld, dim, part, M->lr_start.p, M->long_rows,
+                                                         M->n_lchunks, lcp, M->lr_multi.p, M->n_multi);
+        else
+            k_col_dots_fin<T><<<gd, 256, 0, st>>>(P, Qp, ld, dim, part, M->lr_start.p, M->long_rows, M->n_lchunks,
+                                                  lcp, M->lr_multi.p, M->n_multi);
+    } else if (long_dots) {
         k_col_dots<T, 1024><<<gd, 1024, 0, st>>>(P, Qp, ld, dim, part);
-    else
+    } else {
         k_col_dots<T><<<gd, 256, 0, st>>>(P, Qp, ld, dim, part);
+    }
     k_cg_xr_diag<TX, T><<<g2, 256, 0, st>>>(X, R, P, Qp, Z, M->dinv.p, cs, part, M->p, dim, ld);
     apply_precond_rest<T>(M, R, Z, r);
     if (long_dots)
@@ -2212,6 +2372,13 @@ static void assemble(cb_mme* M, const int64_t* x_ptr, const int32_t* x_col, cons
         lrs[r + 1] = (int64_t)lcb.size();
     }
     M->n_lchunks = (int64_t)lcb.size();
+    std::vector<int32_t> lrow1(lcb.size(), -1);
+    std::vector<int64_t> lmulti;
+    for (int64_t r = 0; r < long_rows; ++r) {
+        if (lrs[r + 1] - lrs[r] == 1) lrow1[lrs[r]] = (int32_t)r;
+        else lmulti.push_back(r);
+    }
+    M->n_multi = (int64_t)lmulti.size();
     // RHS = W'y (mme_builder.cpp:321-328), y'y
     M->rhs.assign(dim, 0.0);
     for (int64_t i = 0; i < dim; ++i)
@@ -2232,6 +2399,7 @@ static void assemble(cb_mme* M, const int64_t* x_ptr, const int32_t* x_col, cons
     up(M->tp, tp); up(M->tc, tc); up(M->tv, tv);
     up(M->wp, wp); up(M->wc, wc); up(M->wv64, wv);
     up(M->lc_beg, lcb); up(M->lc_end, lce); up(M->lr_start, lrs);
+    up(M->lc_row1, lrow1); up(M->lr_multi, lmulti);
     std::vector<float> wv32(wv.begin(), wv.end());
     up(M->wv32, wv32);
     CB_HIP(hipStreamSynchronize(st));
