@@ -48,6 +48,7 @@ using namespace cbamd;
 namespace {
 
 constexpr int DOT_CHUNKS = 128;   // row chunks of the two-stage column dot
+constexpr int CG_RPT = 4;         // rows per thread of the CG vector kernels (k_cg_xr_diag, k_cg_p)
 constexpr double REPLACE_DROP2 = 1e-6;  // squared residual drop that triggers a replacement
 constexpr int64_t LONG_CHUNK = 4096;    // nonzeros per block of a long W'W row
 constexpr int64_t MULTI_FIN_MAX = 16;   // long rows of several chunks a dot chunk may finish
@@ -1023,6 +1024,17 @@ __global__ __launch_bounds__(256) void k_cg_p(T* __restrict__ P, const T* __rest
     __shared__ int s_act;
     __shared__ double sp[2][DOT_CHUNKS];
     const int c = blockIdx.y;
+    // CG_RPT rows per thread (rows base + j 256 + t); their loads go out
+    // before the scalar step, which they do not need
+    const int64_t base = (int64_t)blockIdx.x * (256 * CG_RPT) + threadIdx.x;
+    T zv[CG_RPT], pv[CG_RPT];
+#pragma unroll
+    for (int j = 0; j < CG_RPT; ++j) {
+        const int64_t i = base + j * 256;
+        const int64_t o = (int64_t)c * ld + (i < rows ? i : 0);
+        zv[j] = Z[o];
+        pv[j] = P[o];
+    }
     static_assert(2 * DOT_CHUNKS <= 256, "one partial per thread");
     if (threadIdx.x < DOT_CHUNKS) stage_chunks(part_rz, c, sp[0], threadIdx.x);
     else stage_chunks(part_rr, c, sp[1], threadIdx.x - DOT_CHUNKS);
@@ -1047,25 +1059,49 @@ __global__ __launch_bounds__(256) void k_cg_p(T* __restrict__ P, const T* __rest
         s_act = act;
     }
     __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= rows || !s_act) return;
-    const int64_t o = (int64_t)c * ld + i;
-    P[o] = (T)((double)Z[o] + s_beta * (double)P[o]);
+    if (!s_act) return;
+#pragma unroll
+    for (int j = 0; j < CG_RPT; ++j) {
+        const int64_t i = base + j * 256;
+        if (i < rows) P[(int64_t)c * ld + i] = (T)((double)zv[j] + s_beta * (double)pv[j]);
+    }
 }
 
 // alpha = rho_new / p.q from the partials (a breakdown freezes the column);
 // active columns: x += alpha p, r -= alpha q; every column: z = r dinv on
-// the rows >= p0 (the block-Jacobi rows < p0 follow in their own kernel)
+// the rows >= p0 (the block-Jacobi rows < p0 follow in their own kernel).
+// A workgroup covers 256 CG_RPT rows. With px, each workgroup also writes its
+// sums of r.z over its rows >= p0 and of r.r over all its rows (each
+// thread's rows in order, then a fixed tree over the workgroup) to px[c * 2 nx +
+// blockIdx.x] and px[c * 2 nx + nx + blockIdx.x] (nx = gridDim.x), from the
+// values stored in R and Z.
 template <class TX, class T>
 __global__ __launch_bounds__(256) void k_cg_xr_diag(TX* __restrict__ X, T* __restrict__ R, const T* __restrict__ P,
                                                     const T* __restrict__ Qp, T* __restrict__ Z,
                                                     const double* __restrict__ dinv, ColState* __restrict__ st,
                                                     const double* __restrict__ part, int64_t p0, int64_t rows,
-                                                    int64_t ld) {
+                                                    int64_t ld, double* __restrict__ px) {
     __shared__ double s_alpha;
     __shared__ int s_act;
     __shared__ double sp[DOT_CHUNKS];
+    __shared__ double red[2][256];
     const int c = blockIdx.y;
+    // CG_RPT rows per thread (rows base + j 256 + t); their loads go out
+    // before the scalar step, which they do not need
+    const int64_t base = (int64_t)blockIdx.x * (256 * CG_RPT) + threadIdx.x;
+    TX xv[CG_RPT];
+    T rv0[CG_RPT], pv[CG_RPT], qv[CG_RPT];
+    double dv[CG_RPT];
+#pragma unroll
+    for (int j = 0; j < CG_RPT; ++j) {
+        const int64_t i = base + j * 256, ii = i < rows ? i : 0;
+        const int64_t o = (int64_t)c * ld + ii;
+        xv[j] = X[o];
+        rv0[j] = R[o];
+        pv[j] = P[o];
+        qv[j] = Qp[o];
+        dv[j] = dinv[ii];
+    }
     stage_chunks(part, c, sp, threadIdx.x);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1084,17 +1120,105 @@ __global__ __launch_bounds__(256) void k_cg_xr_diag(TX* __restrict__ X, T* __res
         s_act = act;
     }
     __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= rows) return;
-    const int64_t o = (int64_t)c * ld + i;
-    T rv = R[o];
-    if (s_act) {
-        const double a = s_alpha;
-        X[o] = (TX)((double)X[o] + a * (double)P[o]);
-        rv = (T)((double)rv - a * (double)Qp[o]);
-        R[o] = rv;
+    double rz = 0, rr = 0;
+#pragma unroll
+    for (int j = 0; j < CG_RPT; ++j) {
+        const int64_t i = base + j * 256;
+        if (i >= rows) break;
+        const int64_t o = (int64_t)c * ld + i;
+        T rv = rv0[j];
+        if (s_act) {
+            const double a = s_alpha;
+            X[o] = (TX)((double)xv[j] + a * (double)pv[j]);
+            rv = (T)((double)rv - a * (double)qv[j]);
+            R[o] = rv;
+        }
+        rr += (double)rv * (double)rv;
+        if (i >= p0) {
+            const T zv = (T)((double)rv * dv[j]);
+            Z[o] = zv;
+            rz += (double)rv * (double)zv;
+        }
     }
-    if (i >= p0) Z[o] = (T)((double)rv * dinv[i]);
+    if (!px) return;  // the same for every workgroup
+    red[0][threadIdx.x] = rz;
+    red[1][threadIdx.x] = rr;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        px[(int64_t)c * 2 * gridDim.x + blockIdx.x] = red[0][0];
+        px[(int64_t)c * 2 * gridDim.x + gridDim.x + blockIdx.x] = red[1][0];
+    }
+}
+
+// z[0:p, c] = Binv r[0:p, c] for the block rows and the column's r.z and r.r
+// chunk partials for k_cg_p, in DOT_CHUNKS workgroups of NTH threads: a
+// wave per block row (four running sums per lane over the row), and
+// workgroup b also adds the b-th of DOT_CHUNKS contiguous runs of
+// k_cg_xr_diag's nx workgroup partials (px: r.z of rows >= p, then r.r).
+// part_rz[c][b] = that run's r.z partials + r_i z_i of the workgroup's block
+// rows, part_rr[c][b] = the run's r.r partials; fixed order throughout.
+template <class T, int NTH>
+__global__ __launch_bounds__(NTH) void k_precond_block_dots(const T* __restrict__ R, T* __restrict__ Z,
+                                                            const double* __restrict__ Binv, int64_t p, int64_t ld,
+                                                            const double* __restrict__ px, int64_t nx,
+                                                            double* __restrict__ part_rz,
+                                                            double* __restrict__ part_rr) {
+    constexpr int W = NTH / 64;
+    __shared__ double red[2][NTH];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = blockIdx.y;
+    const T* rc = R + (int64_t)c * ld;
+    double a = 0, u = 0;
+    for (int64_t i = (int64_t)blockIdx.x * W + wave; i < p; i += (int64_t)DOT_CHUNKS * W) {
+        const double* b = Binv + i * p;
+        double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        int64_t k = lane;
+        for (; k + 192 < p; k += 256) {
+            const double b0 = b[k], b1 = b[k + 64], b2 = b[k + 128], b3 = b[k + 192];
+            const double r0 = (double)rc[k], r1 = (double)rc[k + 64], r2 = (double)rc[k + 128],
+                         r3 = (double)rc[k + 192];
+            s0 += b0 * r0;
+            s1 += b1 * r1;
+            s2 += b2 * r2;
+            s3 += b3 * r3;
+        }
+        for (; k < p; k += 64) s0 += b[k] * (double)rc[k];
+        double s = (s0 + s1) + (s2 + s3);
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+        if (lane == 0) {
+            const T z = (T)s;
+            Z[(int64_t)c * ld + i] = z;
+            a += (double)rc[i] * (double)z;
+        }
+    }
+    const int64_t per = (nx + DOT_CHUNKS - 1) / DOT_CHUNKS;
+    const int64_t k0 = (int64_t)blockIdx.x * per, k1 = min(nx, k0 + per);
+    const double* pc = px + (int64_t)c * 2 * nx;
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += NTH) {
+        a += pc[k];
+        u += pc[nx + k];
+    }
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = u;
+    __syncthreads();
+    for (int w = NTH / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part_rz[(int64_t)c * DOT_CHUNKS + blockIdx.x] = red[0][0];
+        part_rr[(int64_t)c * DOT_CHUNKS + blockIdx.x] = red[1][0];
+    }
 }
 
 // R = B - A   (column-major, same ld)
@@ -1906,6 +2030,16 @@ void apply_precond(cb_mme* M, const T* R, T* Z, int r) {
     apply_precond_rest<T>(M, R, Z, r);
 }
 
+// The iteration's r.z and r.r are formed inside the x/r update and the
+// block preconditioner when there is a block (p > 0) and no stiff term
+// (which changes z after the block); part2 then holds the DOT_CHUNKS r.r
+// chunk partials per column followed by the x/r update's two sets of
+// workgroup partials per column.
+inline bool fused_dots(const cb_mme* M) { return M->p > 0 && !M->stiff; }
+inline size_t part2_count(const cb_mme* M, int r) {
+    return ((size_t)DOT_CHUNKS + (fused_dots(M) ? 2 * (size_t)ceil_div(M->dim, 256 * CG_RPT) : 0)) * r;
+}
+
 // One CG iteration on an r-column block after P holds the search
 // directions: Qp = C P, alpha, x and r, z = M^-1 r, beta, convergence and P.
 template <class TX, class T>
@@ -1913,7 +2047,7 @@ void cg_iteration(cb_mme* M, double lambda, TX* X, T* R, T* Z, T* P, T* Qp, ColS
                   double* part2, int r, double tol2, bool absolute) {
     hipStream_t st = M->ctx->stream;
     const int64_t ld = M->ldv, dim = M->dim;
-    const dim3 g2(blocks(dim), (unsigned)r);
+    const dim3 g2(blocks(dim, 256 * CG_RPT), (unsigned)r);  // the CG vector kernels' grid
     const dim3 gd(DOT_CHUNKS, (unsigned)r);
     double* lcp = nullptr;
     apply_C<T>(M, lambda, P, Qp, r, &lcp);
@@ -1930,12 +2064,22 @@ void cg_iteration(cb_mme* M, double lambda, TX* X, T* R, T* Z, T* P, T* Qp, ColS
     } else {
         k_col_dots<T><<<gd, 256, 0, st>>>(P, Qp, ld, dim, part);
     }
-    k_cg_xr_diag<TX, T><<<g2, 256, 0, st>>>(X, R, P, Qp, Z, M->dinv.p, cs, part, M->p, dim, ld);
-    apply_precond_rest<T>(M, R, Z, r);
-    if (long_dots)
-        k_col_dots2<T, 1024><<<gd, 1024, 0, st>>>(R, Z, R, ld, dim, part, part2);
-    else
-        k_col_dots2<T><<<gd, 256, 0, st>>>(R, Z, R, ld, dim, part, part2);
+    if (fused_dots(M)) {
+        // r.z and r.r ride in the x/r update (workgroup partials after the
+        // DOT_CHUNKS r.r chunk partials in part2) and the block
+        // preconditioner, which folds them into the chunk partials k_cg_p reads
+        const int64_t nx = g2.x;
+        double* px = part2 + (size_t)DOT_CHUNKS * r;
+        k_cg_xr_diag<TX, T><<<g2, 256, 0, st>>>(X, R, P, Qp, Z, M->dinv.p, cs, part, M->p, dim, ld, px);
+        k_precond_block_dots<T, 1024><<<gd, 1024, 0, st>>>(R, Z, M->Binv.p, M->p, ld, px, nx, part, part2);
+    } else {
+        k_cg_xr_diag<TX, T><<<g2, 256, 0, st>>>(X, R, P, Qp, Z, M->dinv.p, cs, part, M->p, dim, ld, nullptr);
+        apply_precond_rest<T>(M, R, Z, r);
+        if (long_dots)
+            k_col_dots2<T, 1024><<<gd, 1024, 0, st>>>(R, Z, R, ld, dim, part, part2);
+        else
+            k_col_dots2<T><<<gd, 256, 0, st>>>(R, Z, R, ld, dim, part, part2);
+    }
     k_cg_p<T><<<g2, 256, 0, st>>>(P, Z, cs, part, part2, tol2, absolute ? 1 : 0, dim, ld);
     CB_HIP(hipGetLastError());
 }
@@ -2061,7 +2205,7 @@ int64_t pcg_block(cb_mme* M, double lambda, const T* B, TX* X, int r, double tol
     int64_t it = 0;
     const int check_every = 8;
     bool any_active = std::any_of(hs.begin(), hs.end(), [](const ColState& s) { return s.active; });
-    const WsPtr<double> part2 = ws<double>(M, WS_PART2, (size_t)DOT_CHUNKS * r);
+    const WsPtr<double> part2 = ws<double>(M, WS_PART2, part2_count(M, r));
     auto iteration = [&] {
         cg_iteration<TX, T>(M, lambda, X, R.p, Z.p, P.p, Qp.p, cs.p, part.p, part2.p, r, tol2, absolute);
     };
@@ -2204,7 +2348,7 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
     int64_t it = 0;
     const int check_every = 4;
     int64_t replacements = 0;
-    const WsPtr<double> part2 = ws<double>(M, WS_PART2, (size_t)DOT_CHUNKS * r);
+    const WsPtr<double> part2 = ws<double>(M, WS_PART2, part2_count(M, r));
     // the recursive test in the iteration only pauses a column until the
     // next replacement
     auto iteration = [&] {
@@ -3174,7 +3318,7 @@ static size_t noalloc_need(const cb_mme* M, int64_t r, int precision) {
     const size_t vec = ld * R;
     size_t need = 2 * (vec * 8 + a);  // the right-hand-side and solution blocks
     need += (precision == CB_PREC_FP64 ? 4 * (vec * 8 + a) : 4 * (vec * 4 + a) + 2 * (vec * 8 + a));
-    need += 2 * ((size_t)DOT_CHUNKS * R * 8 + a) + 2 * (R * 8 + a) + R * sizeof(ColState) + a;
+    need += ((size_t)DOT_CHUNKS * R * 8 + a) + (part2_count(M, (int)r) * 8 + a) + 2 * (R * 8 + a) + R * sizeof(ColState) + a;
     if (M->long_rows > 0) need += (size_t)std::max<int64_t>(1, M->n_lchunks) * R * 8 + a;
     if (M->qkind == 0) {
         if (M->stiff) need += (size_t)DOT_CHUNKS * R * 8 + a;
