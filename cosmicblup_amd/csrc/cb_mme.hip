@@ -49,6 +49,8 @@ namespace {
 
 constexpr int DOT_CHUNKS = 128;   // row chunks of the two-stage column dot
 constexpr int CG_RPT = 4;         // rows per thread of the CG vector kernels (k_cg_xr_diag, k_cg_p)
+constexpr int PB_LDS = 4096;      // block rows whose r k_precond_block_dots stages in LDS
+constexpr int PB_U = 16;          // Binv entries a lane of k_precond_block_dots loads at once
 constexpr double REPLACE_DROP2 = 1e-6;  // squared residual drop that triggers a replacement
 constexpr int64_t LONG_CHUNK = 4096;    // nonzeros per block of a long W'W row
 constexpr int64_t MULTI_FIN_MAX = 16;   // long rows of several chunks a dot chunk may finish
@@ -1159,7 +1161,8 @@ __global__ __launch_bounds__(256) void k_cg_xr_diag(TX* __restrict__ X, T* __res
 
 // z[0:p, c] = Binv r[0:p, c] for the block rows and the column's r.z and r.r
 // chunk partials for k_cg_p, in DOT_CHUNKS workgroups of NTH threads: a
-// wave per block row (four running sums per lane over the row), and
+// wave per block row (r[0:p] staged in LDS when p <= PB_LDS; a lane loads
+// PB_U of its row's entries at once, into four running sums), and
 // workgroup b also adds the b-th of DOT_CHUNKS contiguous runs of
 // k_cg_xr_diag's nx workgroup partials (px: r.z of rows >= p, then r.r).
 // part_rz[c][b] = that run's r.z partials + r_i z_i of the workgroup's block
@@ -1172,25 +1175,35 @@ __global__ __launch_bounds__(NTH) void k_precond_block_dots(const T* __restrict_
                                                             double* __restrict__ part_rr) {
     constexpr int W = NTH / 64;
     __shared__ double red[2][NTH];
+    __shared__ double rs[PB_LDS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c = blockIdx.y;
     const T* rc = R + (int64_t)c * ld;
+    const bool staged = p <= PB_LDS && (int64_t)blockIdx.x * W < p;  // the same for the whole workgroup
+    if (staged) {
+        for (int64_t k = threadIdx.x; k < p; k += NTH) rs[k] = (double)rc[k];
+        __syncthreads();
+    }
     double a = 0, u = 0;
     for (int64_t i = (int64_t)blockIdx.x * W + wave; i < p; i += (int64_t)DOT_CHUNKS * W) {
         const double* b = Binv + i * p;
-        double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-        int64_t k = lane;
-        for (; k + 192 < p; k += 256) {
-            const double b0 = b[k], b1 = b[k + 64], b2 = b[k + 128], b3 = b[k + 192];
-            const double r0 = (double)rc[k], r1 = (double)rc[k + 64], r2 = (double)rc[k + 128],
-                         r3 = (double)rc[k + 192];
-            s0 += b0 * r0;
-            s1 += b1 * r1;
-            s2 += b2 * r2;
-            s3 += b3 * r3;
+        double sum[4] = {0, 0, 0, 0};
+        for (int64_t k0 = lane; k0 < p; k0 += 64 * PB_U) {
+            double bv[PB_U], rv[PB_U];
+#pragma unroll
+            for (int j = 0; j < PB_U; ++j) {
+                const int64_t k = k0 + 64 * j;
+                bv[j] = k < p ? b[k] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < PB_U; ++j) {
+                const int64_t k = k0 + 64 * j;
+                rv[j] = k < p ? (staged ? rs[k] : (double)rc[k]) : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < PB_U; ++j) sum[j & 3] += bv[j] * rv[j];
         }
-        for (; k < p; k += 64) s0 += b[k] * (double)rc[k];
-        double s = (s0 + s1) + (s2 + s3);
+        double s = (sum[0] + sum[1]) + (sum[2] + sum[3]);
         for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
         if (lane == 0) {
             const T z = (T)s;
