@@ -282,30 +282,39 @@ __global__ __launch_bounds__(256) void k_sample_sq(const int64_t* __restrict__ s
 //   e(k, i) = sum_{j in miss(k)} mu_j X_ij     (X = 0 where i is missing)
 //   F(k, i) = sum_{j in miss(k) and miss(i)} mu_j^2.
 // With mu_j = K_j 2^-29 (K_j < 2^30 an integer), e is the exact integer
-// sum_j K_j X_ij and F the exact integer sum_j K_j^2 (units 2^-58). Written
-// in signed base-256 digits d_c in [-128, 127] (K = sum_c d_c 256^c: the
-// bytes of (K + 0x80808080) ^ 0x80808080; K^2 likewise with 8 digits), both
-// over a group of four samples k are int8 matrix products: A (16 rows = 4
-// samples x 4 digits, columns = the group's list entries, row (kl, c)
-// holding d_c of the entries of sample kl and 0 elsewhere) times B (64
-// entries x 256 samples i, from the entries' SNP rows: the dosage for e, the
-// missing-call indicator for F, with digits 0-3 and 4-7 of K^2 in two A
-// tiles), 64 entries per v_mfma_i32_16x16x64_i8. Row (kl, c) of the 16x16
-// output is element c of lane 16 kl + i's accumulator, so the digits
-// combine in the lane: sum_c acc_c 256^c, exactly, in int64.
-// Only E(i, k) + E(k, i) is consumed, so F is added once per pair: in full
-// to E(k, i) when i's I-block follows k's, halved to both when they share
-// one, and not at all when it precedes.
+// sum_j K_j X_ij and F the exact integer sum_j K_j^2 (units 2^-58).
+// The kernel forms, over a group of four samples k, the int8 matrix product
+// e'(k, i) = sum_{j in miss(k)} K_j c_ij, where c_ij is the 2-bit code of the
+// I-block rows: the dosage, or 3 for a missing call. So e' = e + 3 F1 with
+// F1(k, i) = sum_{j in miss(k) and miss(i)} K_j, which is symmetric in
+// (k, i). Only E(k, i) + E(i, k) is consumed, and it needs
+//   e(k, i) + e(i, k) + F = e'(k, i) + e'(i, k) + D,
+//   D(k, i) = sum_{j in miss(k) and miss(i)} (K_j^2 - 3 2^30 K_j)   (units 2^-58),
+// so D, again symmetric, is added once per pair: in full to E(k, i) when i's
+// I-block follows k's, halved to both when they share one, and not at all
+// when it precedes. D is the int8 product of the digits of K^2 - 3 2^30 K
+// with the missing-call indicators, formed only in the workgroups that add
+// it.
+// K is written in signed base-256 digits d_c in [-128, 127] (K = sum_c d_c
+// 256^c: the bytes of (K + 0x80808080) ^ 0x80808080), K^2 - 3 2^30 K (a
+// negative 64-bit integer) likewise with 8 digits. A has 16 rows = 4 samples
+// x 4 digits; row (kl, c) holds d_c of the entries of sample kl and 0
+// elsewhere. Each K-step of 64 list entries holds 16 entries of each sample
+// of the group (k-block kl = sample kl), so a lane's A row is non-zero only in
+// its own k-block, fixed for the whole kernel. B is 64 entries x 256 samples
+// i from the entries' SNP rows; 64 entries per v_mfma_i32_16x16x64_i8. Row
+// (kl, c) of the 16x16 output is element c of lane 16 kl + i's accumulator,
+// so the digits combine in the lane: sum_c acc_c 256^c, exactly, in int64.
 constexpr int MB_SAMPLES = 256;  // samples per I-block
 constexpr int MB_STEP = 64;      // list entries per K-step
 constexpr int MB_GROUP = 4;      // samples k per group (4 x 4 digit rows)
 constexpr int MB_KBLK = 16;      // entries per k-block; a sample's list is padded to whole k-blocks
 constexpr int MB_PF = 2;         // K-steps of row gathers in flight; group steps are padded to a multiple
 constexpr int MB_CHUNK = 24;     // K-steps of one group staged in LDS at once (a multiple of MB_PF)
-constexpr int MB_QP = 20;        // LDS pitch (dwords) of one sample quad of B
+constexpr int MB_RP = 80;        // LDS pitch (bytes) of one gathered row: 64 bytes + 16 (bank spread)
 constexpr int MB_XCD = 8;        // XCDs; workgroups are dealt to them round-robin
 // chunks between folds of the int32 digit sums into E: 170 x 24 x 64 = 261 120
-// entries of at most 128 x 2 x 16 = 4096 each stay below 2^31
+// entries of at most 128 x 3 x 16 = 6144 each stay below 2^31
 constexpr int MB_FOLD = 170;
 
 // SNP rows in I-block-major order: block b (samples 256 b .. 256 b + 255)
@@ -327,43 +336,31 @@ __global__ void k_encode_iblock(const uint8_t* __restrict__ bed, int64_t pitch, 
     }
 }
 
-// The group lists: the entries of samples 4g .. 4g + 3 one after another,
-// each sample's padded with (j = 0, K = 0) to whole k-blocks, the group's to
-// whole K-steps (a multiple of MB_PF). Per K-step: gj = the 64 SNPs, gK =
-// their K, gkl = the sample kl of each of the 4 k-blocks. One block per
-// group.
+// The group lists: K-step t of group g holds entries 16 t .. 16 t + 15 of
+// samples 4g .. 4g + 3 in its k-blocks 0 .. 3, pads (j = 0, K = 0) past the
+// end of a sample's list; the group has as many K-steps as its longest
+// list needs, rounded up to a multiple of MB_PF. gj = the 64 SNPs of each
+// K-step, gK = their K. One block per group.
 __global__ __launch_bounds__(256) void k_group_entries(const int64_t* __restrict__ smp_off,
                                                        const int32_t* __restrict__ smp_cnt,
                                                        const int32_t* __restrict__ lj,
                                                        const uint32_t* __restrict__ lk, int64_t n,
                                                        const int64_t* __restrict__ gstep, int32_t* __restrict__ gj,
-                                                       uint32_t* __restrict__ gK, uint8_t* __restrict__ gkl) {
+                                                       uint32_t* __restrict__ gK) {
     const int64_t g = blockIdx.x;
-    int64_t c[MB_GROUP + 1], cnt[MB_GROUP];
-    c[0] = 0;
-#pragma unroll
-    for (int kl = 0; kl < MB_GROUP; ++kl) {
-        const int64_t k = g * MB_GROUP + kl;
-        cnt[kl] = k < n ? smp_cnt[k] : 0;
-        c[kl + 1] = c[kl] + (cnt[kl] + MB_KBLK - 1) / MB_KBLK * MB_KBLK;
-    }
     const int64_t base = gstep[g] * MB_STEP, len = (gstep[g + 1] - gstep[g]) * MB_STEP;
     for (int64_t e = threadIdx.x; e < len; e += blockDim.x) {
+        const int kl = (int)((e >> 4) & 3);
+        const int64_t r = (e >> 6) * MB_KBLK + (e & 15);  // entry r of sample kl's list
+        const int64_t k = g * MB_GROUP + kl;
         int32_t j = 0;
         uint32_t K = 0;
-        int kl = 0;
-        if (e < c[MB_GROUP]) {
-            while (e >= c[kl + 1]) ++kl;
-            const int64_t r = e - c[kl];
-            if (r < cnt[kl]) {
-                const int64_t p = smp_off[g * MB_GROUP + kl] + r;
-                j = lj[p];
-                K = lk[p];
-            }
+        if (k < n && r < smp_cnt[k]) {
+            j = lj[smp_off[k] + r];
+            K = lk[smp_off[k] + r];
         }
         gj[base + e] = j;
         gK[base + e] = K;
-        if (e % MB_KBLK == 0) gkl[(base + e) / MB_KBLK] = (uint8_t)kl;
     }
 }
 
@@ -387,28 +384,36 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Transposed 8-byte LDS read (ds_read_b64_tr_b8): within each group of 16
+// lanes, lane i receives byte (i & 7) of the 8-byte rows addressed by lanes
+// 2t + (i >> 3), t = 0 .. 7, as its byte t.
+__device__ __forceinline__ uint2 lds_read_tr8(const void* p) {
+    typedef int v2i_t __attribute__((ext_vector_type(2)));
+    const v2i_t v = __builtin_amdgcn_ds_read_tr8_b64_v2i32(
+        (__attribute__((address_space(3))) v2i_t*)(const_cast<void*>(p)));
+    return make_uint2((uint32_t)v.x, (uint32_t)v.y);
+}
+
 // B fragments of one K-step, one per sample of the quad (values x 1, 4, 16,
-// 4), from the quad's 16 entry bytes x: the dosages (a missing call, field
-// 3, reads 0) and, for F, the missing-call indicators.
-__device__ __forceinline__ void dosage_fragments(const uint32_t (&x)[4], v4i (&bd)[4]) {
+// 4), from the quad's 16 entry bytes x: the 2-bit codes (dosage, 3 for a
+// missing call) and, where D is added, the missing-call indicators.
+__device__ __forceinline__ void code_fragments(const uint32_t (&x)[4], v4i (&bd)[4]) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const uint32_t mq = x[q] & (x[q] >> 1) & 0x55555555u;  // 1 at the low bit of each missing field
-        const uint32_t cl = x[q] & ~(mq | (mq << 1));          // missing -> 0
-        bd[0][q] = (int)(cl & 0x03030303u);
-        bd[1][q] = (int)(cl & 0x0c0c0c0cu);
-        bd[2][q] = (int)(cl & 0x30303030u);
-        bd[3][q] = (int)((cl & 0xc0c0c0c0u) >> 4);
+        bd[0][q] = (int)(x[q] & 0x03030303u);
+        bd[1][q] = (int)(x[q] & 0x0c0c0c0cu);
+        bd[2][q] = (int)(x[q] & 0x30303030u);
+        bd[3][q] = (int)((x[q] >> 4) & 0x0c0c0c0cu);
     }
 }
 __device__ __forceinline__ void missing_fragments(const uint32_t (&x)[4], v4i (&bm)[4]) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const uint32_t y = x[q] >> 1;
-        bm[0][q] = (int)(x[q] & y & 0x01010101u);
-        bm[1][q] = (int)(x[q] & y & 0x04040404u);
-        bm[2][q] = (int)(x[q] & y & 0x10101010u);
-        bm[3][q] = (int)((x[q] & y & 0x40404040u) >> 4);
+        const uint32_t y = x[q] & (x[q] >> 1);
+        bm[0][q] = (int)(y & 0x01010101u);
+        bm[1][q] = (int)(y & 0x04040404u);
+        bm[2][q] = (int)(y & 0x10101010u);
+        bm[3][q] = (int)((y >> 4) & 0x04040404u);
     }
 }
 
@@ -417,24 +422,22 @@ __device__ __forceinline__ void missing_fragments(const uint32_t (&x)[4], v4i (&
 // of MB_XCD every XCD holds P / MB_XCD consecutive workgroups of each
 // I-block, so the workgroups resident on an XCD gather from the same
 // m x 64 bytes of row segments, which stay in its L2. 256 threads; per group:
-//   stage    per chunk of MB_CHUNK K-steps: the SNPs and k-block samples,
-//            and the digit planes of K (4) and, if F is added here, of K^2
-//            (8), formed from gK (plane c, byte t = digit c of entry t); the
-//            lists stream past L2 (non-temporal loads), which keeps the
-//            gathered rows;
-//   gather   thread t loads dword t % 16 (16 samples) of the rows of entries
-//            4 (t / 16) .. + 3 (a wave's load: 64 bytes of 4 rows; MB_PF
-//            steps in flight) and transposes the 4 x 4 bytes, giving per
-//            sample quad a dword of 4 entries x 4 samples, stored in LDS
-//            (B staging, double-buffered, one barrier per step; the dword
-//            groups of a quad are XOR-swizzled so that the stores of a wave
-//            fall in distinct banks);
+//   stage    per chunk of MB_CHUNK K-steps: the SNPs, and the digit planes
+//            of K (4) and, if D is added here, of K^2 - 3 2^30 K (8), formed
+//            from gK (plane c, byte t = digit c of entry t); the lists stream
+//            past L2 (non-temporal loads), which keeps the gathered rows;
+//   gather   wave w loads the 64-byte rows of entries 16 w .. 16 w + 15, a
+//            quarter row (16 bytes) per lane (MB_PF steps in flight), and
+//            stores them untransposed in LDS (double-buffered, one barrier
+//            per step; row pitch MB_RP bytes spreads the banks);
 //   product  lane l of wave w takes quad Q = 16 w + l % 16 and k-block
-//            l / 16: one 16-byte LDS read gives its 16 entries' bytes, from
-//            which the 4 samples' dosages (and missing-call indicators) are
-//            masked out, exact scales removed at the end; 4 MFMAs for e and
-//            8 for F. The A fragment is plane c of the k-block if the
-//            k-block is sample kl's, else 0, for the lane's row (kl, c);
+//            kb = l / 16: two transposed reads (ds_read_b64_tr_b8) give its
+//            16 entries' bytes (lane pairs 2t, 2t + 1 of a lane group address
+//            the two 8-byte halves of the wave's quads in entry row
+//            16 kb + 8h + t), from which the 4 samples' codes (and
+//            missing-call indicators) are masked out, exact scales removed
+//            at the end; 4 MFMAs for e' and 8 for D. The A fragment of row
+//            (kl, c) is plane c of k-block kb when kb = kl, else a zero block;
 //   write    E(k, i) for the group's samples k < n and the I-block's i < n;
 //            E(k, k) = s_k / 2 from sqd. A group longer than MB_FOLD chunks
 //            adds its sums into E every MB_FOLD chunks and starts them again,
@@ -443,15 +446,13 @@ __device__ __forceinline__ void missing_fragments(const uint32_t (&x)[4], v4i (&
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_miss_mfma(const uint32_t* __restrict__ bedi, int64_t m, int64_t n,
                                                    int64_t P, const int32_t* __restrict__ gp,
                                                    const int64_t* __restrict__ gstep, const int32_t* __restrict__ gj,
-                                                   const uint32_t* __restrict__ gK, const uint8_t* __restrict__ gkl,
-                                                   const double* __restrict__ sqd, double* __restrict__ E,
-                                                   int64_t lde) {
+                                                   const uint32_t* __restrict__ gK, const double* __restrict__ sqd,
+                                                   double* __restrict__ E, int64_t lde) {
     __shared__ __attribute__((aligned(16))) uint32_t sP[MB_CHUNK * 64];                // K planes, 6 KB
-    __shared__ __attribute__((aligned(16))) uint32_t sV[MB_CHUNK * 128];               // K^2 planes, 12 KB
+    __shared__ __attribute__((aligned(16))) uint32_t sV[MB_CHUNK * 128];               // D planes, 12 KB
     __shared__ __attribute__((aligned(16))) int32_t sJ[MB_CHUNK * MB_STEP];            // 6 KB
-    __shared__ __attribute__((aligned(16))) uint8_t sKL[MB_CHUNK * 4];
-    __shared__ __attribute__((aligned(16))) uint32_t sB[2][(MB_SAMPLES / 4) * MB_QP];  // 10 KB
-    __shared__ __attribute__((aligned(16))) uint32_t sZero[4];  // the A fragment of rows off the k-block
+    __shared__ __attribute__((aligned(16))) uint8_t sR[2][MB_STEP * MB_RP];            // rows, 10 KB
+    __shared__ __attribute__((aligned(16))) uint32_t sZero[4];  // the A fragment of rows off their k-block
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     if (tid < 4) sZero[tid] = 0u;  // ordered before its readers by the first chunk's barriers
     const int64_t b = blockIdx.x / P, p = blockIdx.x % P;
@@ -460,10 +461,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int Qf = wave * 16 + (lane & 15);      // sample quad of this lane's B fragments
     const int kb = lane >> 4;                    // k-block of this lane's fragments
     const int kl_a = (lane & 15) >> 2, c_a = lane & 3;  // A row (kl, c) of this lane
-    const int rg = tid >> 4, dc = tid & 15;      // gather task: entries 4 rg .. + 3, dword dc
-    // B staging: dword group h (entries 16 h .. + 15) of quad Q at Q QP + 4 (h ^ ((Q >> 4) & 3))
-    uint32_t* const bst0 = sB[0] + 4 * dc * MB_QP + 4 * ((rg >> 2) ^ (dc >> 2)) + (rg & 3);
-    const int brd = Qf * MB_QP + 4 * (kb ^ wave);
+    const bool on = kb == kl_a;                  // A row (kl, c) is non-zero only in k-block kl
+    const uint32_t* const pa = on ? sP + c_a * 16 + kb * 4 : sZero;
+    const uint32_t* const pv0 = on ? sV + c_a * 16 + kb * 4 : sZero;
+    const uint32_t* const pv1 = on ? sV + (4 + c_a) * 16 + kb * 4 : sZero;
+    const int sa = on ? 64 : 0, sv = on ? 128 : 0;  // dwords per step
+    const int er = 16 * wave + (lane >> 2), qr = lane & 3;  // gather: entry er, quarter qr
+    const int wst = er * MB_RP + 16 * qr;
+    const int rtr = (16 * kb + ((lane & 15) >> 1)) * MB_RP + 16 * wave + 8 * (lane & 1);
     // rows of I-block b through a buffer resource (m * 64 bytes < 2^31)
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(bedi + b * m * 16), (short)0, (int)(m * 64), 0x00020000);
@@ -471,7 +476,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int64_t g = gp[p]; g < g1; ++g) {
         const int64_t step0 = gstep[g], nsteps = gstep[g + 1] - step0;
         const int64_t bk = (g * MB_GROUP) / MB_SAMPLES;
-        const int fm = b > bk ? 2 : b == bk ? 1 : 0;  // F in full, halved, or not at all
+        const int fm = b > bk ? 2 : b == bk ? 1 : 0;  // D in full, halved, or not at all
         v4i acc[4], af0[4], af1[4];
 #pragma unroll
         for (int s_ = 0; s_ < 4; ++s_) acc[s_] = af0[s_] = af1[s_] = (v4i){0, 0, 0, 0};
@@ -520,11 +525,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
                     for (int c = 0; c < 4; ++c) sP[st * 64 + c * 16 + t4] = o[c];
                     if (fm != 0) {
+                        // K^2 - 3 2^30 K (mod 2^64, a negative integer), in signed digits
                         const unsigned long long B8 = 0x8080808080808080ull;
-                        const unsigned long long v0 = ((unsigned long long)k4.x * k4.x + B8) ^ B8;
-                        const unsigned long long v1 = ((unsigned long long)k4.y * k4.y + B8) ^ B8;
-                        const unsigned long long v2 = ((unsigned long long)k4.z * k4.z + B8) ^ B8;
-                        const unsigned long long v3 = ((unsigned long long)k4.w * k4.w + B8) ^ B8;
+                        auto dval = [&](uint32_t K) {
+                            const unsigned long long k = K;
+                            return (k * k - 3ull * (k << 30) + B8) ^ B8;
+                        };
+                        const unsigned long long v0 = dval(k4.x), v1 = dval(k4.y), v2 = dval(k4.z), v3 = dval(k4.w);
                         transpose4x4((uint32_t)v0, (uint32_t)v1, (uint32_t)v2, (uint32_t)v3, o);
 #pragma unroll
                         for (int c = 0; c < 4; ++c) sV[st * 128 + c * 16 + t4] = o[c];
@@ -534,45 +541,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                         for (int c = 0; c < 4; ++c) sV[st * 128 + (4 + c) * 16 + t4] = o[c];
                     }
                 }
-                for (int x = tid; x < csteps * 4; x += 256) sKL[x] = __builtin_nontemporal_load(gkl + s0 * 4 + x);
             }
             __syncthreads();  // staged
-            // row dwords of MB_PF steps in flight: step st's sit in ring[st % MB_PF]
-            uint32_t ring[MB_PF][4];
-            auto fetch = [&](int st, uint32_t (&cw)[4]) {
-                const int4 jj = *reinterpret_cast<const int4*>(&sJ[st * MB_STEP + 4 * rg]);
-                cw[0] = __builtin_amdgcn_raw_buffer_load_b32(rs, (uint32_t)(jj.x * 64 + dc * 4), 0, 0);
-                cw[1] = __builtin_amdgcn_raw_buffer_load_b32(rs, (uint32_t)(jj.y * 64 + dc * 4), 0, 0);
-                cw[2] = __builtin_amdgcn_raw_buffer_load_b32(rs, (uint32_t)(jj.z * 64 + dc * 4), 0, 0);
-                cw[3] = __builtin_amdgcn_raw_buffer_load_b32(rs, (uint32_t)(jj.w * 64 + dc * 4), 0, 0);
+            // Quarter rows of MB_PF steps in flight: step st's in ring[st % MB_PF].
+            // Every step issues the gather of step st + MB_PF (a step past the
+            // chunk reloads its last one, unused), so the wait for a step's
+            // rows never covers the gathers issued after them.
+            uint4 ring[MB_PF];
+            auto fetch = [&](int st) {
+                const int sc = st < csteps ? st : csteps - 1;
+                const int32_t j = sJ[sc * MB_STEP + er];
+                const v4i v = __builtin_amdgcn_raw_buffer_load_b128(rs, (uint32_t)(j * 64 + 16 * qr), 0, 0);
+                return make_uint4((uint32_t)v[0], (uint32_t)v[1], (uint32_t)v[2], (uint32_t)v[3]);
             };
 #pragma unroll
-            for (int u = 0; u < MB_PF; ++u) fetch(u, ring[u]);
+            for (int u = 0; u < MB_PF; ++u) {
+                ring[u] = fetch(u);
+                __builtin_amdgcn_sched_barrier(0);  // issued in ring order
+            }
             for (int st0 = 0; st0 < csteps; st0 += MB_PF) {
 #pragma unroll
                 for (int u = 0; u < MB_PF; ++u) {
-                    const int st = st0 + u;
-                    uint32_t (&cw)[4] = ring[u];
-                    uint32_t o[4];
-                    transpose4x4(cw[0], cw[1], cw[2], cw[3], o);
-                    uint32_t* dst = bst0 + (st & 1) * ((MB_SAMPLES / 4) * MB_QP);
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) dst[q * MB_QP] = o[q];
-                    if (st + MB_PF < csteps) fetch(st + MB_PF, cw);
+                    const int st = st0 + u;  // csteps is a multiple of MB_PF
+                    uint8_t* const rb = sR[st & 1];
+                    *reinterpret_cast<uint4*>(rb + wst) = ring[u];
+                    ring[u] = fetch(st + MB_PF);
                     __syncthreads();
-                    const uint4 w4 = *reinterpret_cast<const uint4*>(sB[st & 1] + brd);
-                    const uint32_t x[4] = {w4.x, w4.y, w4.z, w4.w};
-                    const bool on = (int)sKL[st * 4 + kb] == kl_a;
-                    const v4i ak = *reinterpret_cast<const v4i*>(on ? sP + st * 64 + c_a * 16 + kb * 4 : sZero);
+                    const uint2 lo = lds_read_tr8(rb + rtr);
+                    const uint2 hi = lds_read_tr8(rb + rtr + 8 * MB_RP);
+                    const uint32_t x[4] = {lo.x, lo.y, hi.x, hi.y};
+                    const v4i ak = *reinterpret_cast<const v4i*>(pa + st * sa);
                     v4i bd[4];
-                    dosage_fragments(x, bd);
+                    code_fragments(x, bd);
 #pragma unroll
                     for (int s_ = 0; s_ < 4; ++s_) acc[s_] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ak, bd[s_], acc[s_], 0, 0, 0);
                     if (fm != 0) {
                         v4i bm[4];
                         missing_fragments(x, bm);
-                        const v4i a0 = *reinterpret_cast<const v4i*>(on ? sV + st * 128 + c_a * 16 + kb * 4 : sZero);
-                        const v4i a1 = *reinterpret_cast<const v4i*>(on ? sV + st * 128 + (4 + c_a) * 16 + kb * 4 : sZero);
+                        const v4i a0 = *reinterpret_cast<const v4i*>(pv0 + st * sv);
+                        const v4i a1 = *reinterpret_cast<const v4i*>(pv1 + st * sv);
 #pragma unroll
                         for (int s_ = 0; s_ < 4; ++s_) {
                             af0[s_] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, bm[s_], af0[s_], 0, 0, 0);
@@ -905,7 +912,6 @@ struct cb_grm {
     // the missing-call kernel's inputs: rows in I-block-major order, the
     // I-block segments of the SNP lists, the group lists and A fragments
     DevBuf<uint32_t> bedi, gK;
-    DevBuf<uint8_t> gkl;
     DevBuf<int32_t> gj, gp;
     DevBuf<int64_t> gstep;
 };
@@ -1063,15 +1069,16 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
             k_encode_iblock<<<dim3((unsigned)ceil_div(nb * 16, 256), gy), 256, 0, st>>>(g->bed.p, g->pitch, n, m,
                                                                                       nb, bedi.p);
         });
-        // groups of 4 samples, each sample's list padded to whole k-blocks and
-        // the group's to whole rings of K-steps
+        // groups of 4 samples; a group has as many K-steps as its longest list
+        // needs (16 entries of each sample per K-step), rounded up to a
+        // multiple of MB_PF
         const int64_t ngrp = ceil_div(n, MB_GROUP);
         std::vector<int64_t> gstep(ngrp + 1, 0);
         for (int64_t gi = 0; gi < ngrp; ++gi) {
-            int64_t cnt = 0;
+            int64_t longest = 0;
             for (int64_t k = gi * MB_GROUP; k < std::min(n, (gi + 1) * MB_GROUP); ++k)
-                cnt += round_up(hcnt[k], MB_KBLK);
-            gstep[gi + 1] = gstep[gi] + round_up(ceil_div(cnt, MB_STEP), MB_PF);
+                longest = std::max<int64_t>(longest, hcnt[k]);
+            gstep[gi + 1] = gstep[gi] + round_up(ceil_div(longest, MB_KBLK), MB_PF);
         }
         const int64_t tsteps = gstep[ngrp];
         DevBuf<int64_t>& dgstep = grm->gstep;
@@ -1079,13 +1086,11 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
         copy_host(st, dgstep.p, gstep.data(), (ngrp + 1) * 8, hipMemcpyHostToDevice);
         DevBuf<int32_t>& gj = grm->gj;
         DevBuf<uint32_t>& gK = grm->gK;
-        DevBuf<uint8_t>& gkl = grm->gkl;
         grow(gj, std::max<int64_t>(1, tsteps * MB_STEP));
         grow(gK, std::max<int64_t>(1, tsteps * MB_STEP));
-        grow(gkl, std::max<int64_t>(16, tsteps * 4));
         timed(ctx, "k_group_entries", [&] {
             k_group_entries<<<(unsigned)ngrp, 256, 0, st>>>(d_smp_off.p, smp_cnt.p, lj.p, lk.p, n, dgstep.p, gj.p,
-                                                           gK.p, gkl.p);
+                                                           gK.p);
         });
         // group ranges of about equal K-steps, P of them per I-block
         const int64_t P = std::min<int64_t>(4096, round_up(ngrp, MB_XCD));
@@ -1098,7 +1103,7 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
         copy_host(st, dgp.p, gp.data(), (P + 1) * 4, hipMemcpyHostToDevice);
         CB_REQUIRE(nb * P < ((int64_t)1 << 31), "GRM: too many samples for the missing-call kernel's grid");
         timed(ctx, "k_miss_cross", [&] {
-            k_miss_mfma<<<(unsigned)(nb * P), 256, 0, st>>>(bedi.p, m, n, P, dgp.p, dgstep.p, gj.p, gK.p, gkl.p, sqd.p,
+            k_miss_mfma<<<(unsigned)(nb * P), 256, 0, st>>>(bedi.p, m, n, P, dgp.p, dgstep.p, gj.p, gK.p, sqd.p,
                                                            E.p, n);
         });
     }
