@@ -311,10 +311,17 @@ constexpr int MB_GROUP = 4;      // samples k per group (4 x 4 digit rows)
 constexpr int MB_KBLK = 16;      // entries per k-block; a sample's list is padded to whole k-blocks
 constexpr int MB_PF = 2;         // K-steps of row gathers in flight; group steps are padded to a multiple
 constexpr int MB_CHUNK = 24;     // K-steps of one group staged in LDS at once (a multiple of MB_PF)
-constexpr int MB_RP = 80;        // LDS pitch (bytes) of one gathered row: 64 bytes + 16 (bank spread)
+// LDS layout of the missing-call kernel, chosen so that its reads and
+// writes hit distinct banks: a digit plane of one K-step takes 20 dwords
+// (16 used), a K-step's 4 K planes 96 dwords and its 8 D planes 160; a
+// gathered row takes 64 bytes, its 16-byte quarters swizzled by row
+constexpr int MB_PLN = 20;
+constexpr int MB_SPK = 96;
+constexpr int MB_SPD = 160;
 constexpr int MB_XCD = 8;        // XCDs; workgroups are dealt to them round-robin
-// chunks between folds of the int32 digit sums into E: 170 x 24 x 64 = 261 120
-// entries of at most 128 x 3 x 16 = 6144 each stay below 2^31
+// chunks between folds of the int32 digit sums into E: an accumulator row
+// (kl, c) sees 16 entries per K-step, 170 x 24 x 16 = 65 280 between folds,
+// each adding at most 128 x 64 = 8192, so the sums stay below 2^31
 constexpr int MB_FOLD = 170;
 
 // SNP rows in I-block-major order: block b (samples 256 b .. 256 b + 255)
@@ -394,9 +401,10 @@ __device__ __forceinline__ uint2 lds_read_tr8(const void* p) {
     return make_uint2((uint32_t)v.x, (uint32_t)v.y);
 }
 
-// B fragments of one K-step, one per sample of the quad (values x 1, 4, 16,
-// 4), from the quad's 16 entry bytes x: the 2-bit codes (dosage, 3 for a
-// missing call) and, where D is added, the missing-call indicators.
+// B fragments of one K-step, one per sample of the quad, from the quad's 16
+// entry bytes x: the 2-bit codes (dosage, 3 for a missing call; scaled by 1,
+// 4, 16, 4) and, where D is added, the missing-call indicators (scaled by 1,
+// 4, 16, 64).
 __device__ __forceinline__ void code_fragments(const uint32_t (&x)[4], v4i (&bd)[4]) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -413,7 +421,7 @@ __device__ __forceinline__ void missing_fragments(const uint32_t (&x)[4], v4i (&
         bm[0][q] = (int)(y & 0x01010101u);
         bm[1][q] = (int)(y & 0x04040404u);
         bm[2][q] = (int)(y & 0x10101010u);
-        bm[3][q] = (int)((y >> 4) & 0x04040404u);
+        bm[3][q] = (int)(y & 0x40404040u);  // an indicator at bit 6 (64) is still a positive int8
     }
 }
 
@@ -429,7 +437,7 @@ __device__ __forceinline__ void missing_fragments(const uint32_t (&x)[4], v4i (&
 //   gather   wave w loads the 64-byte rows of entries 16 w .. 16 w + 15, a
 //            quarter row (16 bytes) per lane (MB_PF steps in flight), and
 //            stores them untransposed in LDS (double-buffered, one barrier
-//            per step; row pitch MB_RP bytes spreads the banks);
+//            per step; the quarter q of row r sits at q ^ ((r >> 1) & 3));
 //   product  lane l of wave w takes quad Q = 16 w + l % 16 and k-block
 //            kb = l / 16: two transposed reads (ds_read_b64_tr_b8) give its
 //            16 entries' bytes (lane pairs 2t, 2t + 1 of a lane group address
@@ -448,13 +456,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                    const int64_t* __restrict__ gstep, const int32_t* __restrict__ gj,
                                                    const uint32_t* __restrict__ gK, const double* __restrict__ sqd,
                                                    double* __restrict__ E, int64_t lde) {
-    __shared__ __attribute__((aligned(16))) uint32_t sP[MB_CHUNK * 64];                // K planes, 6 KB
-    __shared__ __attribute__((aligned(16))) uint32_t sV[MB_CHUNK * 128];               // D planes, 12 KB
+    __shared__ __attribute__((aligned(128))) uint32_t sP[MB_CHUNK * MB_SPK];           // K planes, 9 KB
+    __shared__ __attribute__((aligned(128))) uint32_t sV[MB_CHUNK * MB_SPD];           // D planes, 15 KB
     __shared__ __attribute__((aligned(16))) int32_t sJ[MB_CHUNK * MB_STEP];            // 6 KB
-    __shared__ __attribute__((aligned(16))) uint8_t sR[2][MB_STEP * MB_RP];            // rows, 10 KB
-    __shared__ __attribute__((aligned(16))) uint32_t sZero[4];  // the A fragment of rows off their k-block
+    __shared__ __attribute__((aligned(128))) uint8_t sR[2][MB_STEP * 64];              // rows, 8 KB
+    // the A fragments of rows off their k-block: zero blocks at dwords 0, 8,
+    // 16 and 24, each in banks that no on-block read of the same pass uses
+    __shared__ __attribute__((aligned(128))) uint32_t sZero[32];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    if (tid < 4) sZero[tid] = 0u;  // ordered before its readers by the first chunk's barriers
+    if (tid < 32) sZero[tid] = 0u;  // ordered before its readers by the first chunk's barriers
     const int64_t b = blockIdx.x / P, p = blockIdx.x % P;
     const int64_t I0 = b * MB_SAMPLES;
     const int kl_out = lane >> 4;                // output rows (kl_out, c) of this lane
@@ -462,13 +472,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int kb = lane >> 4;                    // k-block of this lane's fragments
     const int kl_a = (lane & 15) >> 2, c_a = lane & 3;  // A row (kl, c) of this lane
     const bool on = kb == kl_a;                  // A row (kl, c) is non-zero only in k-block kl
-    const uint32_t* const pa = on ? sP + c_a * 16 + kb * 4 : sZero;
-    const uint32_t* const pv0 = on ? sV + c_a * 16 + kb * 4 : sZero;
-    const uint32_t* const pv1 = on ? sV + (4 + c_a) * 16 + kb * 4 : sZero;
-    const int sa = on ? 64 : 0, sv = on ? 128 : 0;  // dwords per step
+    const uint32_t* const pa = on ? sP + c_a * MB_PLN + kb * 4 : sZero + (kb < 2 ? 16 : 24);
+    const uint32_t* const pv0 = on ? sV + c_a * MB_PLN + kb * 4 : sZero + (kb < 2 ? 16 : 24);
+    const uint32_t* const pv1 = on ? sV + (4 + c_a) * MB_PLN + kb * 4 : sZero + (kb < 2 ? 0 : 8);
+    const int sa = on ? MB_SPK : 0, sv = on ? MB_SPD : 0;  // dwords per step
     const int er = 16 * wave + (lane >> 2), qr = lane & 3;  // gather: entry er, quarter qr
-    const int wst = er * MB_RP + 16 * qr;
-    const int rtr = (16 * kb + ((lane & 15) >> 1)) * MB_RP + 16 * wave + 8 * (lane & 1);
+    const int wst = er * 64 + 16 * (qr ^ ((er >> 1) & 3));
+    // rows 16 kb + 8 h + (lane & 15) / 2 have (r >> 1) & 3 = (lane >> 2) & 3 for both h
+    const int rtr = (16 * kb + ((lane & 15) >> 1)) * 64 + 16 * (wave ^ ((lane >> 2) & 3)) + 8 * (lane & 1);
     // rows of I-block b through a buffer resource (m * 64 bytes < 2^31)
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(bedi + b * m * 16), (short)0, (int)(m * 64), 0x00020000);
@@ -491,15 +502,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             for (int s_ = 0; s_ < 4; ++s_) {
                 const int64_t i_loc = 4 * Qf + s_;
                 if (I0 + i_loc >= n) continue;
-                const int sh = s_ == 0 ? 0 : s_ == 1 ? 2 : s_ == 2 ? 4 : 2;
+                const int sh = s_ == 0 ? 0 : s_ == 1 ? 2 : s_ == 2 ? 4 : 2;  // the code scales
                 const int64_t ev = ((int64_t)acc[s_][0] + ((int64_t)acc[s_][1] << 8) + ((int64_t)acc[s_][2] << 16) +
                                     ((int64_t)acc[s_][3] << 24)) >> sh;
                 double v = (double)ev * TWO_M29;
                 if (fm != 0) {
+                    const int shd = 2 * s_;  // the indicator scales
                     const int64_t flo = ((int64_t)af0[s_][0] + ((int64_t)af0[s_][1] << 8) +
-                                         ((int64_t)af0[s_][2] << 16) + ((int64_t)af0[s_][3] << 24)) >> sh;
+                                         ((int64_t)af0[s_][2] << 16) + ((int64_t)af0[s_][3] << 24)) >> shd;
                     const int64_t fhi = ((int64_t)af1[s_][0] + ((int64_t)af1[s_][1] << 8) +
-                                         ((int64_t)af1[s_][2] << 16) + ((int64_t)af1[s_][3] << 24)) >> sh;
+                                         ((int64_t)af1[s_][2] << 16) + ((int64_t)af1[s_][3] << 24)) >> shd;
                     v += ((double)fhi * 4294967296.0 + (double)flo) * fsc;
                 }
                 if (add) v += row[i_loc];
@@ -523,7 +535,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                     transpose4x4((k4.x + 0x80808080u) ^ 0x80808080u, (k4.y + 0x80808080u) ^ 0x80808080u,
                                  (k4.z + 0x80808080u) ^ 0x80808080u, (k4.w + 0x80808080u) ^ 0x80808080u, o);
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) sP[st * 64 + c * 16 + t4] = o[c];
+                    for (int c = 0; c < 4; ++c) sP[st * MB_SPK + c * MB_PLN + t4] = o[c];
                     if (fm != 0) {
                         // K^2 - 3 2^30 K (mod 2^64, a negative integer), in signed digits
                         const unsigned long long B8 = 0x8080808080808080ull;
@@ -534,11 +546,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                         const unsigned long long v0 = dval(k4.x), v1 = dval(k4.y), v2 = dval(k4.z), v3 = dval(k4.w);
                         transpose4x4((uint32_t)v0, (uint32_t)v1, (uint32_t)v2, (uint32_t)v3, o);
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) sV[st * 128 + c * 16 + t4] = o[c];
+                        for (int c = 0; c < 4; ++c) sV[st * MB_SPD + c * MB_PLN + t4] = o[c];
                         transpose4x4((uint32_t)(v0 >> 32), (uint32_t)(v1 >> 32), (uint32_t)(v2 >> 32),
                                      (uint32_t)(v3 >> 32), o);
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) sV[st * 128 + (4 + c) * 16 + t4] = o[c];
+                        for (int c = 0; c < 4; ++c) sV[st * MB_SPD + (4 + c) * MB_PLN + t4] = o[c];
                     }
                 }
             }
@@ -568,7 +580,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                     ring[u] = fetch(st + MB_PF);
                     __syncthreads();
                     const uint2 lo = lds_read_tr8(rb + rtr);
-                    const uint2 hi = lds_read_tr8(rb + rtr + 8 * MB_RP);
+                    const uint2 hi = lds_read_tr8(rb + rtr + 8 * 64);
                     const uint32_t x[4] = {lo.x, lo.y, hi.x, hi.y};
                     const v4i ak = *reinterpret_cast<const v4i*>(pa + st * sa);
                     v4i bd[4];
