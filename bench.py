@@ -7,9 +7,10 @@ is one full GrmBuilder::compute: SNP statistics, packing, integer Gram on
 INT8 MFMA, FP64 corrections, FP64 + FP32 G written.
 
 Metric: GRM TOPS = m * n * (n + 1) algorithmic integer ops (unique pairs,
-MAC = 2) / step time. The roofline object is for the dominant kernel
-(k_gram) timed with HIP events on its launch stream; cpu_baseline is the
-oracle's OpenMP FP64 GRM on a bounded sub-block, timed on the host cores.
+MAC = 2) / step time. The roofline object is for the dominant kernel of the
+step (timed with HIP events on its launch stream); cpu_baseline is the
+reference's GRM arithmetic (centred Z, FP64 Z Z^T through numpy's BLAS) on a
+bounded sample, timed on the host cores.
 """
 from __future__ import annotations
 
@@ -33,10 +34,10 @@ K_DENSE_F32_TRAFFIC_BYTES = 1.824e9
 # HBM bytes per launch at n = m = 50,000 with 1% missing, from two rocprofv3
 # --pmc runs of this script's GRM leg (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the
 # factors a 1 GiB device copy gives):
-#   k_gram        85.8 GB read + 30.3 GB written
-#   k_miss_cross  79.7 GB read + 20.0 GB written
-K_GRAM_TRAFFIC_BYTES = 1.161e11
-K_MISS_TRAFFIC_BYTES = 9.97e10
+#   k_gram        87.7 GB read + 30.3 GB written
+#   k_miss_cross  75.4 GB read + 20.1 GB written
+K_GRAM_TRAFFIC_BYTES = 1.180e11
+K_MISS_TRAFFIC_BYTES = 9.55e10
 # headline metric: the GRM part is `value` (TOPS on the n = m = 50,000 GRM);
 # the REML part (GPU mixed vs GPU FP64 plain iterations/s) is under "reml"
 METRIC = "REML iters/sec + GRM TOPS, GPU mixed vs GPU FP64 at \u22641e-6 rel EBV error"
