@@ -6,9 +6,10 @@
 // VCE. Covariate projection: A = the last n - c columns of the full Q of a
 // QR of X (rocSOLVER geqrf + orgqr), G_r = A' G A and Y_r = A' Y (rocBLAS),
 // G_r = U diag(d) U' (dsyevd), rows y_i of U' Y_r. The EM then needs only
-// (d_i, y_i): k_mv_em computes each row's contribution to S_g and S_e
-// (t x t, t <= 4) in registers and reduces them over fixed row chunks; the
-// chunks are summed in order on the host, so every run gives the same bits.
+// (d_i, y_i): k_mv_em_run runs every EM iteration in one launch of one
+// workgroup, each row's contribution to S_g and S_e (t x t, t <= 4) formed
+// in registers and summed in a fixed order, so every run gives the same bits
+// and no iteration waits for the host.
 //
 // Solve. The reference assembles the Kronecker MME [Ve^-1 (x) X'X,
 // Ve^-1 (x) X'; ., Ve^-1 (x) I + Vg^-1 (x) G^-1] (unknowns trait-major) and
@@ -34,7 +35,6 @@ using namespace cbamd;
 
 namespace {
 
-constexpr int MV_CHUNKS = 256;
 constexpr int MV_MAXT = 4;
 constexpr double MV_SOLVE_TOL = 1e-12;  // relative residual of each transformed-trait solve
 
@@ -75,93 +75,127 @@ __device__ void inv_small(const double (&a)[T][T], double (&inv)[T][T]) {
         for (int j = 0; j < T; ++j) inv[i][j] = m[i][T + j];
 }
 
-// One EM pass (MvEigenVce::updateEM): per row i of the rotated data,
+// One row's EM contribution (MvEigenVce::updateEM): for row i of the
+// rotated data,
 //   V_i = d_i Vg + Ve,  W_g = d_i Vg V_i^-1,  u = W_g y_i,
 //   S_g += u u' / d_i + Vg - W_g Vg   (Vg alone when d_i <= 1e-12),
 //   W_e = Ve V_i^-1,  e = W_e y_i,  S_e += e e' + Ve - W_e Ve.
-// part[chunk][0 .. T*T) = S_g partial, [T*T .. 2 T*T) = S_e partial.
 template <int T>
-__global__ __launch_bounds__(256) void k_mv_em(const double* __restrict__ d, const double* __restrict__ Y,
-                                               int64_t ns, int64_t ldy, const double* __restrict__ vgve,
-                                               double* __restrict__ part) {
-    __shared__ double red[256];
-    double Vg[T][T], Ve[T][T];
+__device__ __forceinline__ void em_row(double di, const double (&y)[T], const double (&Vg)[T][T],
+                                       const double (&Ve)[T][T], double (&sg)[T][T], double (&se)[T][T]) {
+    double V[T][T], P[T][T];
+#pragma unroll
+    for (int a = 0; a < T; ++a)
+#pragma unroll
+        for (int b = 0; b < T; ++b) V[a][b] = Vg[a][b] * di + Ve[a][b];
+    inv_small<T>(V, P);
+    double Wg[T][T], We[T][T], u[T], e[T];
 #pragma unroll
     for (int a = 0; a < T; ++a)
 #pragma unroll
         for (int b = 0; b < T; ++b) {
-            Vg[a][b] = vgve[a * T + b];
-            Ve[a][b] = vgve[T * T + a * T + b];
+            double sgv = 0, sev = 0;
+#pragma unroll
+            for (int k = 0; k < T; ++k) {
+                sgv += Vg[a][k] * di * P[k][b];
+                sev += Ve[a][k] * P[k][b];
+            }
+            Wg[a][b] = sgv;
+            We[a][b] = sev;
         }
-    double sg[T][T], se[T][T];
+#pragma unroll
+    for (int a = 0; a < T; ++a) {
+        double su = 0, sv = 0;
+#pragma unroll
+        for (int k = 0; k < T; ++k) {
+            su += Wg[a][k] * y[k];
+            sv += We[a][k] * y[k];
+        }
+        u[a] = su;
+        e[a] = sv;
+    }
+    const bool idf = di > 1e-12;
 #pragma unroll
     for (int a = 0; a < T; ++a)
 #pragma unroll
-        for (int b = 0; b < T; ++b) sg[a][b] = se[a][b] = 0.0;
-    const int64_t per = (ns + MV_CHUNKS - 1) / MV_CHUNKS;
-    const int64_t r0 = (int64_t)blockIdx.x * per, r1 = min(ns, r0 + per);
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) {
-        const double di = d[i];
-        double V[T][T], P[T][T], y[T];
-#pragma unroll
-        for (int a = 0; a < T; ++a) {
-            y[a] = Y[(int64_t)a * ldy + i];
-#pragma unroll
-            for (int b = 0; b < T; ++b) V[a][b] = Vg[a][b] * di + Ve[a][b];
-        }
-        inv_small<T>(V, P);
-        double Wg[T][T], We[T][T], u[T], e[T];
-#pragma unroll
-        for (int a = 0; a < T; ++a)
-#pragma unroll
-            for (int b = 0; b < T; ++b) {
-                double sgv = 0, sev = 0;
-#pragma unroll
-                for (int k = 0; k < T; ++k) {
-                    sgv += Vg[a][k] * di * P[k][b];
-                    sev += Ve[a][k] * P[k][b];
-                }
-                Wg[a][b] = sgv;
-                We[a][b] = sev;
-            }
-#pragma unroll
-        for (int a = 0; a < T; ++a) {
-            double su = 0, sv = 0;
+        for (int b = 0; b < T; ++b) {
+            double wvg = 0, wve = 0;
 #pragma unroll
             for (int k = 0; k < T; ++k) {
-                su += Wg[a][k] * y[k];
-                sv += We[a][k] * y[k];
+                wvg += Wg[a][k] * Vg[k][b];
+                wve += We[a][k] * Ve[k][b];
             }
-            u[a] = su;
-            e[a] = sv;
+            sg[a][b] += idf ? (u[a] * u[b] / di + Vg[a][b] - wvg) : Vg[a][b];
+            se[a][b] += e[a] * e[b] + Ve[a][b] - wve;
         }
-        const bool idf = di > 1e-12;
+}
+
+// The whole EM run on the device, one workgroup: each iteration sums the
+// rows' contributions (thread t takes rows t, t + 256, ... in order; then a
+// butterfly within each wave and the four waves' sums in order, so every
+// run gives the same bits), sets Vg = S_g / ns and Ve = S_e / ns, and stops
+// when both relative Frobenius changes are below tol (eigen_vce.cpp:240-246)
+// or after max_iter iterations. vgve holds Vg then Ve (row-major t x t): the
+// start on entry, the estimates on exit; res = (iterations, converged).
+template <int T>
+__global__ __launch_bounds__(256) void k_mv_em_run(const double* __restrict__ d, const double* __restrict__ Y,
+                                                   int64_t ns, int64_t ldy, double* __restrict__ vgve,
+                                                   int64_t max_iter, double tol, int64_t* __restrict__ res) {
+    constexpr int NQ = 2 * T * T;
+    __shared__ double red[4][NQ];
+    __shared__ double cur[NQ];  // Vg, then Ve
+    __shared__ int state[2];    // converged, stop
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid < NQ) cur[tid] = vgve[tid];
+    __syncthreads();
+    int64_t it = 0;  // counted by thread 0
+    for (;;) {
+        double Vg[T][T], Ve[T][T], sg[T][T], se[T][T];
 #pragma unroll
         for (int a = 0; a < T; ++a)
 #pragma unroll
             for (int b = 0; b < T; ++b) {
-                double wvg = 0, wve = 0;
-#pragma unroll
-                for (int k = 0; k < T; ++k) {
-                    wvg += Wg[a][k] * Vg[k][b];
-                    wve += We[a][k] * Ve[k][b];
-                }
-                sg[a][b] += idf ? (u[a] * u[b] / di + Vg[a][b] - wvg) : Vg[a][b];
-                se[a][b] += e[a] * e[b] + Ve[a][b] - wve;
+                Vg[a][b] = cur[a * T + b];
+                Ve[a][b] = cur[T * T + a * T + b];
+                sg[a][b] = se[a][b] = 0.0;
             }
-    }
-    // block reduction, entry by entry, fixed order
+        for (int64_t i = tid; i < ns; i += 256) {
+            double y[T];
 #pragma unroll
-    for (int q = 0; q < 2 * T * T; ++q) {
-        const int a = (q % (T * T)) / T, b = q % T;
-        red[threadIdx.x] = q < T * T ? sg[a][b] : se[a][b];
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
+            for (int a = 0; a < T; ++a) y[a] = Y[(int64_t)a * ldy + i];
+            em_row<T>(d[i], y, Vg, Ve, sg, se);
         }
-        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * 2 * T * T + q] = red[0];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double v = q < T * T ? sg[q / T][q % T] : se[(q - T * T) / T][(q - T * T) % T];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            if (lane == 0) red[wave][q] = v;
+        }
         __syncthreads();
+        if (tid == 0) {
+            double S[NQ];
+            for (int q = 0; q < NQ; ++q) S[q] = (((red[0][q] + red[1][q]) + red[2][q]) + red[3][q]) / (double)ns;
+            double dg = 0, ng = 0, de = 0, ne = 0;
+            for (int q = 0; q < T * T; ++q) {
+                dg += (S[q] - cur[q]) * (S[q] - cur[q]);
+                ng += cur[q] * cur[q];
+                de += (S[T * T + q] - cur[T * T + q]) * (S[T * T + q] - cur[T * T + q]);
+                ne += cur[T * T + q] * cur[T * T + q];
+            }
+            const bool conv = sqrt(dg) / fmax(sqrt(ng), 1e-12) < tol && sqrt(de) / fmax(sqrt(ne), 1e-12) < tol;
+            for (int q = 0; q < NQ; ++q) cur[q] = S[q];
+            ++it;
+            state[0] = conv ? 1 : 0;
+            state[1] = conv || it >= max_iter ? 1 : 0;
+        }
+        __syncthreads();
+        if (state[1]) break;  // the same for every thread
+    }
+    if (tid < NQ) vgve[tid] = cur[tid];
+    if (tid == 0) {
+        res[0] = it;
+        res[1] = state[0];
     }
 }
 
@@ -261,9 +295,9 @@ Canonical canonical(const std::vector<double>& Vg, const std::vector<double>& Ve
 }
 
 template <int T>
-void launch_em(const double* d, const double* Y, int64_t ns, int64_t ldy, const double* vgve, double* part,
-               hipStream_t st) {
-    k_mv_em<T><<<MV_CHUNKS, 256, 0, st>>>(d, Y, ns, ldy, vgve, part);
+void launch_em(const double* d, const double* Y, int64_t ns, int64_t ldy, double* vgve, int64_t max_iter, double tol,
+               int64_t* res, hipStream_t st) {
+    k_mv_em_run<T><<<1, 256, 0, st>>>(d, Y, ns, ldy, vgve, max_iter, tol, res);
 }
 
 }  // namespace
@@ -358,54 +392,31 @@ int cb_mv_vce(cb_context* ctx, int64_t n, int64_t t, int64_t c, const double* Y,
                 for (int64_t i = 0; i < n; ++i) sum += (Y[(size_t)a * n + i] - mean[a]) * (Y[(size_t)b * n + i] - mean[b]);
                 vg[a * T + b] = ve[a * T + b] = 0.5 * sum / (double)(ns - 1);
             }
-        DevBuf<double> vgve, part;
+        DevBuf<double> vgve;
+        DevBuf<int64_t> res;
         vgve.alloc(2 * T * T);
-        part.alloc((size_t)MV_CHUNKS * 2 * T * T);
-        std::vector<double> hp((size_t)MV_CHUNKS * 2 * T * T), sg((size_t)T * T), se((size_t)T * T);
+        res.alloc(2);
         int64_t it = 0;
         int conv = 0;
-        auto fro = [&](const std::vector<double>& a, const std::vector<double>* b) {
-            double s = 0;
-            for (size_t k = 0; k < a.size(); ++k) {
-                const double dlt = b ? a[k] - (*b)[k] : a[k];
-                s += dlt * dlt;
-            }
-            return std::sqrt(s);
-        };
-        while (it < opts->max_iter) {
+        if (opts->max_iter > 0) {
             std::vector<double> both(vg);
             both.insert(both.end(), ve.begin(), ve.end());
             copy_host(st, vgve.p, both.data(), both.size() * 8, hipMemcpyHostToDevice);
             switch (T) {
-                case 1: launch_em<1>(dd.p, Uy.p, ns, ns, vgve.p, part.p, st); break;
-                case 2: launch_em<2>(dd.p, Uy.p, ns, ns, vgve.p, part.p, st); break;
-                case 3: launch_em<3>(dd.p, Uy.p, ns, ns, vgve.p, part.p, st); break;
-                default: launch_em<4>(dd.p, Uy.p, ns, ns, vgve.p, part.p, st); break;
+                case 1: launch_em<1>(dd.p, Uy.p, ns, ns, vgve.p, opts->max_iter, opts->tol, res.p, st); break;
+                case 2: launch_em<2>(dd.p, Uy.p, ns, ns, vgve.p, opts->max_iter, opts->tol, res.p, st); break;
+                case 3: launch_em<3>(dd.p, Uy.p, ns, ns, vgve.p, opts->max_iter, opts->tol, res.p, st); break;
+                default: launch_em<4>(dd.p, Uy.p, ns, ns, vgve.p, opts->max_iter, opts->tol, res.p, st); break;
             }
             CB_HIP(hipGetLastError());
-            copy_host(st, hp.data(), part.p, hp.size() * 8, hipMemcpyDeviceToHost);
+            int64_t hres[2];
+            copy_host(st, both.data(), vgve.p, both.size() * 8, hipMemcpyDeviceToHost);
+            copy_host(st, hres, res.p, sizeof hres, hipMemcpyDeviceToHost);
             CB_HIP(hipStreamSynchronize(st));
-            std::fill(sg.begin(), sg.end(), 0.0);
-            std::fill(se.begin(), se.end(), 0.0);
-            for (int k = 0; k < MV_CHUNKS; ++k)
-                for (int q = 0; q < T * T; ++q) {
-                    sg[q] += hp[(size_t)k * 2 * T * T + q];
-                    se[q] += hp[(size_t)k * 2 * T * T + T * T + q];
-                }
-            for (int q = 0; q < T * T; ++q) {
-                sg[q] /= (double)ns;
-                se[q] /= (double)ns;
-            }
-            // relative Frobenius changes (eigen_vce.cpp:240-246)
-            const double dg = fro(sg, &vg) / std::max(fro(vg, nullptr), 1e-12);
-            const double de = fro(se, &ve) / std::max(fro(ve, nullptr), 1e-12);
-            vg = sg;
-            ve = se;
-            ++it;
-            if (dg < opts->tol && de < opts->tol) {
-                conv = 1;
-                break;
-            }
+            std::copy(both.begin(), both.begin() + T * T, vg.begin());
+            std::copy(both.begin() + T * T, both.end(), ve.begin());
+            it = hres[0];
+            conv = (int)hres[1];
         }
         std::copy(vg.begin(), vg.end(), Vg_out);
         std::copy(ve.begin(), ve.end(), Ve_out);

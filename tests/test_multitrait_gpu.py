@@ -36,6 +36,19 @@ def test_mv_vce_matches_oracle(cb, ctx, t):
     np.testing.assert_allclose(v.getVe(), Ve, rtol=1e-5, atol=1e-7)
 
 
+def test_mv_vce_iteration_cap_matches_oracle(cb, ctx):
+    """The EM loop runs on the device: stopped by max_iter it reports the cap,
+    not converged, with the estimates of that many EM steps."""
+    Y, X, G = mv_problem(t=3)
+    v = multitrait.MvEigenVce(multitrait.MvEigenVce.Options(max_iter=7, tol=1e-12))
+    v.prepare(Y, X, G)
+    v.runNullModel()
+    Vg, Ve, it, conv = orl.mv_eigen_vce(Y, X, G, max_iter=7, tol=1e-12)
+    assert (v.iterations, v.converged) == (it, conv) == (7, False)
+    np.testing.assert_allclose(v.getVg(), Vg, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(v.getVe(), Ve, rtol=1e-9, atol=1e-12)
+
+
 def test_mv_solve_matches_oracle(cb, ctx):
     Y, X, G = mv_problem(n=300)
     Gi = np.linalg.inv(G)
