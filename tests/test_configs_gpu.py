@@ -117,3 +117,50 @@ def test_c1_dense_exact_reml_n2000_matches_oracle(cb, ctx):
     assert abs(R.var_u - o["vu"]) / o["vu"] < 1e-6 and abs(R.var_e - o["ve"]) / o["ve"] < 1e-6
     u, ur = R.solution[M.p:], o["sol"][M.p:]
     assert np.abs(u - ur).max() / np.abs(ur).max() < 1e-6
+
+
+@pytest.mark.timeout(900)
+def test_c3_pblup_1m_animals_solution_satisfies_independent_mme(cb, ctx):
+    """Sparse-MME PBLUP at its real size: a 1,000,000-animal pedigree (10
+    generations of 100,000, parents uniform from the previous generation,
+    5% unknown), A^-1 by Henderson's rules with inbreeding, 600,000 records
+    in 2000 contemporary groups, h2 = 0.25. The device solutions (FP64, and
+    the mixed FP32 recursion with FP64 residual replacement, both to relative
+    residual 1e-12) are put into the MME assembled independently on the host
+    with scipy (C = W'W + lambda [0 0; 0 A^-1], b = W'y, W = [X Z]): the
+    relative residual ||C x - b|| / ||b|| there must be below 1e-11, and the
+    two precisions' EBVs must agree to 1e-6 relative."""
+    import scipy.sparse as sp
+
+    from cosmicblup_amd.pedigree import invertA_henderson
+
+    seed, h2, n_rec, n_groups = 20263203, 0.25, 600_000, 2000
+    dam, sire = synth.simulate_pedigree(10, 100_000, unknown=0.05, seed=seed)
+    q = dam.size
+    A = invertA_henderson(dam, sire)
+    rec, y, grp = synth.simulate_pedigree_phenotype(dam, sire, A.dii, n_rec, h2=h2, n_groups=n_groups,
+                                                    seed=seed + 1)
+    fd = design.build_fixed_design([[f"cg{int(c):04d}"] for c in grp])
+    lam = (1.0 - h2) / h2
+    model = mme.MixedModel(fd, rec, y, q, ctx=ctx)
+    model.set_qinv_csr(A.ptr, A.col, A.val)
+    b_dev = model.rhs()
+
+    X = sp.csr_matrix((fd.val, fd.col, fd.ptr), shape=(n_rec, fd.p))
+    Z = sp.csr_matrix((np.ones(n_rec), np.asarray(rec, dtype=np.int64), np.arange(n_rec + 1)), shape=(n_rec, q))
+    W = sp.hstack([X, Z]).tocsr()
+    Ai = sp.csr_matrix((A.val, A.col, A.ptr), shape=(q, q))
+    Cm = (W.T @ W + sp.block_diag([sp.csr_matrix((fd.p, fd.p)), lam * Ai])).tocsr()
+    b = W.T @ np.asarray(y, dtype=np.float64)
+    np.testing.assert_allclose(b_dev, b, rtol=1e-12, atol=1e-12 * np.abs(b).max())
+
+    sols = {}
+    for name, prec in (("fp64", _lib.CB_PREC_FP64), ("mixed", _lib.CB_PREC_MIXED)):
+        x, it, rel = model.solve(lam, b, mme.solve_options(precision=prec, tol=1e-12, max_iter=20000))
+        res = np.linalg.norm(Cm @ x - b) / np.linalg.norm(b)
+        print(f"[c3] {name}: {it} iterations, device rel {rel:.2e}, scipy rel {res:.2e}", flush=True)
+        assert res < 1e-11, (name, res)
+        sols[name] = x
+    p = fd.p
+    scale = np.abs(sols["fp64"][p:]).max()
+    assert np.abs(sols["mixed"][p:] - sols["fp64"][p:]).max() / scale < 1e-6
