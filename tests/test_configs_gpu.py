@@ -119,48 +119,103 @@ def test_c1_dense_exact_reml_n2000_matches_oracle(cb, ctx):
     assert np.abs(u - ur).max() / np.abs(ur).max() < 1e-6
 
 
-@pytest.mark.timeout(900)
-def test_c3_pblup_1m_animals_solution_satisfies_independent_mme(cb, ctx):
-    """Sparse-MME PBLUP at its real size: a 1,000,000-animal pedigree (10
-    generations of 100,000, parents uniform from the previous generation,
-    5% unknown), A^-1 by Henderson's rules with inbreeding, 600,000 records
-    in 2000 contemporary groups, h2 = 0.25. The device solutions (FP64, and
-    the mixed FP32 recursion with FP64 residual replacement, both to relative
-    residual 1e-12) are put into the MME assembled independently on the host
-    with scipy (C = W'W + lambda [0 0; 0 A^-1], b = W'y, W = [X Z]): the
-    relative residual ||C x - b|| / ||b|| there must be below 1e-11, and the
-    two precisions' EBVs must agree to 1e-6 relative."""
+C3 = dict(seed=20263203, h2=0.25, n_rec=600_000, n_groups=2000, n_geno=20_000, m_geno=40_000)
+
+
+@pytest.fixture(scope="module")
+def c3_problem(cb):
+    """The 1,000,000-animal pedigree MME (10 generations of 100,000, parents
+    uniform from the previous generation, 5% unknown; A^-1 by Henderson's
+    rules with inbreeding; 600,000 records in 2000 contemporary groups; h2 =
+    0.25) and its W'W + lambda [0 0; 0 A^-1] and W'y assembled independently
+    on the host with scipy (W = [X Z])."""
     import scipy.sparse as sp
 
     from cosmicblup_amd.pedigree import invertA_henderson
 
-    seed, h2, n_rec, n_groups = 20263203, 0.25, 600_000, 2000
+    seed, h2, n_rec = C3["seed"], C3["h2"], C3["n_rec"]
     dam, sire = synth.simulate_pedigree(10, 100_000, unknown=0.05, seed=seed)
     q = dam.size
     A = invertA_henderson(dam, sire)
-    rec, y, grp = synth.simulate_pedigree_phenotype(dam, sire, A.dii, n_rec, h2=h2, n_groups=n_groups,
+    rec, y, grp = synth.simulate_pedigree_phenotype(dam, sire, A.dii, n_rec, h2=h2, n_groups=C3["n_groups"],
                                                     seed=seed + 1)
     fd = design.build_fixed_design([[f"cg{int(c):04d}"] for c in grp])
     lam = (1.0 - h2) / h2
-    model = mme.MixedModel(fd, rec, y, q, ctx=ctx)
-    model.set_qinv_csr(A.ptr, A.col, A.val)
-    b_dev = model.rhs()
-
     X = sp.csr_matrix((fd.val, fd.col, fd.ptr), shape=(n_rec, fd.p))
     Z = sp.csr_matrix((np.ones(n_rec), np.asarray(rec, dtype=np.int64), np.arange(n_rec + 1)), shape=(n_rec, q))
     W = sp.hstack([X, Z]).tocsr()
     Ai = sp.csr_matrix((A.val, A.col, A.ptr), shape=(q, q))
     Cm = (W.T @ W + sp.block_diag([sp.csr_matrix((fd.p, fd.p)), lam * Ai])).tocsr()
     b = W.T @ np.asarray(y, dtype=np.float64)
-    np.testing.assert_allclose(b_dev, b, rtol=1e-12, atol=1e-12 * np.abs(b).max())
+    return dict(dam=dam, sire=sire, A=A, rec=rec, y=y, fd=fd, lam=lam, q=q, Cm=Cm, b=b)
 
+
+def _c3_solves(model, lam, b, C_apply, p):
     sols = {}
     for name, prec in (("fp64", _lib.CB_PREC_FP64), ("mixed", _lib.CB_PREC_MIXED)):
         x, it, rel = model.solve(lam, b, mme.solve_options(precision=prec, tol=1e-12, max_iter=20000))
-        res = np.linalg.norm(Cm @ x - b) / np.linalg.norm(b)
-        print(f"[c3] {name}: {it} iterations, device rel {rel:.2e}, scipy rel {res:.2e}", flush=True)
+        res = np.linalg.norm(C_apply(x) - b) / np.linalg.norm(b)
+        print(f"[c3] {name}: {it} iterations, device rel {rel:.2e}, host rel {res:.2e}", flush=True)
         assert res < 1e-11, (name, res)
         sols[name] = x
-    p = fd.p
     scale = np.abs(sols["fp64"][p:]).max()
     assert np.abs(sols["mixed"][p:] - sols["fp64"][p:]).max() / scale < 1e-6
+
+
+@pytest.mark.timeout(900)
+def test_c3_pblup_1m_animals_solution_satisfies_independent_mme(cb, ctx, c3_problem):
+    """PBLUP at C3's size: the device solutions (FP64, and the mixed FP32
+    recursion with FP64 residual replacement, both to relative residual
+    1e-12) have relative residual ||C x - b|| / ||b|| below 1e-11 in the MME
+    assembled on the host, and the two precisions' EBVs agree to 1e-6."""
+    P = c3_problem
+    model = mme.MixedModel(P["fd"], P["rec"], P["y"], P["q"], ctx=ctx)
+    A = P["A"]
+    model.set_qinv_csr(A.ptr, A.col, A.val)
+    b = P["b"]
+    b_dev = model.rhs()
+    np.testing.assert_allclose(b_dev, b, rtol=1e-12, atol=1e-12 * np.abs(b).max())
+    _c3_solves(model, P["lam"], b, lambda x: P["Cm"] @ x, P["fd"].p)
+
+
+@pytest.mark.timeout(900)
+def test_c3_ssgblup_20k_genotyped_solution_satisfies_independent_mme(cb, ctx, c3_problem):
+    """ssGBLUP at C3's size: the last 20,000 animals genotyped at 40,000 SNPs
+    (GRM on the device), H^-1 = A^-1 + [0 0; 0 D] with D = G^-1 - A22^-1 on
+    the genotyped rows (tuneG and blending on the device). The device
+    solutions with the split operator (CSR A^-1 plus the gathered dense D)
+    satisfy C = W'W + lambda H^-1, applied on the host as the scipy MME plus
+    lambda P' D P x with D copied from the device, to 1e-11."""
+    import torch
+
+    from cosmicblup_amd import grm as cgrm
+    from cosmicblup_amd import plink, ssgblup
+
+    P = c3_problem
+    q, p, lam = P["q"], P["fd"].p, P["lam"]
+    ng, mg = C3["n_geno"], C3["m_geno"]
+    gmap = np.arange(q - ng, q, dtype=np.int32)
+    rows, pitch = synth.bed_on_device(ng, mg, maf=(0.01, 0.5), missing_rate=0.0, seed=C3["seed"] + 2)
+    reader = plink.PlinkReader.from_device_rows(ctx, rows.data_ptr(), pitch, ng, mg)
+    gb = cgrm.GrmBuilder(cgrm.Options(), ctx=ctx)
+    gb.compute(reader, outputs=_lib.CB_GRM_OUT_F64)
+    G = torch.empty((ng, ng), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    gb.copyGrmToDevice(G.data_ptr(), ng)
+    del gb, reader, rows
+    D, _, _ = ssgblup.build_hinv_blocks(P["dam"], P["sire"], gmap, G, ctx=ctx)
+    del G
+    model = mme.MixedModel(P["fd"], P["rec"], P["y"], q, ctx=ctx)
+    A = P["A"]
+    model.set_qinv_split(A.ptr, A.col, A.val, gmap, D.data_ptr(), ng)
+    Dh = D.cpu().numpy()  # row-major ng x ng
+    del D
+    torch.cuda.empty_cache()
+    g_rows = p + gmap
+
+    def C_apply(x):
+        y = P["Cm"] @ x
+        y[g_rows] += lam * (Dh @ x[g_rows])
+        return y
+
+    _c3_solves(model, lam, P["b"], C_apply, p)
