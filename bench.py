@@ -633,6 +633,47 @@ def pcg_leg(ctx, n_gen: int = 10, per_gen: int = 100000, n_rec: int = 600000, n_
     return out
 
 
+def dense_vs_vendor(ctx, q: int = 20000, r: int = 65, reps: int = 20) -> dict:
+    """The REML block product Q X (q x q by q x r) by the library's kernels
+    (k_dense_mfma FP32 and FP64, k_dense_h on FP16 halves, each with its
+    split-K reduction; synthetic operands, the library's split, HIP events on
+    its stream) beside
+    torch.matmul of the same shape, which runs the vendor GEMM (rocBLAS /
+    hipBLASLt): FP32, FP64, and FP16 (one FP16 copy of Q, not the two halves
+    the library's FP16 path reads)."""
+    import ctypes as C
+
+    import torch
+
+    from cosmicblup_amd import _lib
+
+    flops = 2.0 * q * q * r
+    out = {"q": q, "r": r, "library": {}, "vendor": {}}
+    for name, elem in (("fp32", 4), ("fp64", 8), ("fp16_halves", 2)):
+        tb, ks = C.c_int(), C.c_int()
+        _lib.check(_lib.lib().cb_tune_dense_lookup(elem, q, r, C.byref(tb), C.byref(ks)))
+        ms = C.c_double()
+        _lib.check(_lib.lib().cb_tune_dense_time(ctx.handle, elem, q, r, tb.value, ks.value, reps, C.byref(ms)))
+        out["library"][name] = {"ms": ms.value, "tflops": flops / ms.value / 1e9}
+    for name, dt in (("fp32", torch.float32), ("fp64", torch.float64), ("fp16", torch.float16)):
+        A = torch.randn(q, q, device="cuda", dtype=torch.float32).to(dt)
+        X = torch.randn(q, r, device="cuda", dtype=torch.float32).to(dt)
+        for _ in range(3):
+            Y = A @ X
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            Y = A @ X
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out["vendor"][name] = {"ms": ms, "tflops": flops / ms / 1e9}
+        del A, X, Y
+        torch.cuda.empty_cache()
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -658,6 +699,9 @@ def main() -> None:
                     help="only run the PBLUP PCG leg and print its JSON object")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
+    ap.add_argument("--dense-vs-vendor", action="store_true",
+                    help="only time the REML block product (q = 20,000, 65 columns) by the library's kernels "
+                         "and by torch.matmul (the vendor GEMM) in FP32, FP64 and FP16")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: there is no multi-GPU path; one GPU's share "
@@ -680,6 +724,9 @@ def main() -> None:
     import cosmicblup_amd as cb
     from cosmicblup_amd import _lib, synth
 
+    if args.dense_vs_vendor:
+        print(json.dumps(dense_vs_vendor(_lib.Context(0))), flush=True)
+        return
     if args.reml_only:
         ctx = _lib.Context(0)
         print(json.dumps(reml_leg(ctx, n=args.reml_n, m=args.reml_m, plain_iters=args.plain_iters)), flush=True)
