@@ -633,6 +633,32 @@ def pcg_leg(ctx, n_gen: int = 10, per_gen: int = 100000, n_rec: int = 600000, n_
     return out
 
 
+def int8_vendor(n: int, m: int, reps: int = 5) -> dict:
+    """The vendor INT8 GEMM (torch._int_mm: rocBLAS / hipBLASLt, int8 in,
+    int32 out) on the GRM's shape, X (n x m) times X' (m x n), for a
+    reference point beside k_gram. It forms all n^2 entries (2 n^2 m ops);
+    k_gram forms the lower triangle (m n (n + 1) ops) from 2-bit codes and
+    writes G in FP64 and FP32 in its epilogue."""
+    import torch
+
+    X = torch.randint(0, 3, (n, m), dtype=torch.int8, device="cuda")
+    for _ in range(2):
+        Y = torch._int_mm(X, X.t())  # X' as the column-major view of X
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        Y = torch._int_mm(X, X.t())
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del X, Y
+    torch.cuda.empty_cache()
+    ops = 2.0 * n * n * m
+    return {"n": n, "m": m, "vendor_int8_gemm": {"ms": ms, "tops": ops / ms / 1e9, "ops": ops,
+                                                 "frac_int8_peak": ops / ms / 1e9 / PEAK_INT8_TOPS}}
+
+
 def dense_vs_vendor(ctx, q: int = 20000, r: int = 65, reps: int = 20) -> dict:
     """The REML block product Q X (q x q by q x r) by the library's kernels
     (k_dense_mfma FP32 and FP64, k_dense_h on FP16 halves, each with its
@@ -699,6 +725,8 @@ def main() -> None:
                     help="only run the PBLUP PCG leg and print its JSON object")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
+    ap.add_argument("--int8-vs-vendor", action="store_true",
+                    help="only time the vendor INT8 GEMM (torch._int_mm) on the GRM's n x m by m x n shape")
     ap.add_argument("--dense-vs-vendor", action="store_true",
                     help="only time the REML block product (q = 20,000, 65 columns) by the library's kernels "
                          "and by torch.matmul (the vendor GEMM) in FP32, FP64 and FP16")
@@ -724,6 +752,9 @@ def main() -> None:
     import cosmicblup_amd as cb
     from cosmicblup_amd import _lib, synth
 
+    if args.int8_vs_vendor:
+        print(json.dumps(int8_vendor(args.n, args.m)), flush=True)
+        return
     if args.dense_vs_vendor:
         print(json.dumps(dense_vs_vendor(_lib.Context(0))), flush=True)
         return
