@@ -248,30 +248,35 @@ __global__ __launch_bounds__(256) void k_miss_sample_lists(
     }
 }
 
-// s_i = sum over sample i's list of K^2 >> 6 (mu^2 in 2^-52 units), one
-// wave per sample. A long list overflows 64 bits, so the low and high words
-// of the terms are summed apart (exact for lists below 2^31 entries).
+// s_i = sum over sample i's list of K^2 >> 6 (mu^2 in 2^-52 units) and
+// k_i = sum of K, one wave per sample. A long list overflows 64 bits, so the
+// low and high words of the K^2 terms are summed apart (exact for lists
+// below 2^31 entries).
 __global__ __launch_bounds__(256) void k_sample_sq(const int64_t* __restrict__ smp_off,
                                                    const uint32_t* __restrict__ lk, int64_t n,
                                                    unsigned long long* __restrict__ sq_lo,
-                                                   unsigned long long* __restrict__ sq_hi) {
+                                                   unsigned long long* __restrict__ sq_hi,
+                                                   unsigned long long* __restrict__ ksum) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t i = (int64_t)blockIdx.x * 4 + wave;
     if (i >= n) return;
-    unsigned long long lo = 0, hi = 0;
+    unsigned long long lo = 0, hi = 0, ks = 0;
     for (int64_t p = smp_off[i] + lane; p < smp_off[i + 1]; p += 64) {
         const unsigned long long K = lk[p];
         const unsigned long long k2 = (K * K) >> 6;
         lo += k2 & 0xffffffffull;
         hi += k2 >> 32;
+        ks += K;
     }
     for (int off = 32; off > 0; off >>= 1) {
         lo += __shfl_down(lo, off);
         hi += __shfl_down(hi, off);
+        ks += __shfl_down(ks, off);
     }
     if (lane == 0) {
         sq_lo[i] = lo;
         sq_hi[i] = hi;
+        ksum[i] = ks;
     }
 }
 
@@ -319,10 +324,13 @@ constexpr int MB_PLN = 20;
 constexpr int MB_SPK = 96;
 constexpr int MB_SPD = 160;
 constexpr int MB_XCD = 8;        // XCDs; workgroups are dealt to them round-robin
-// chunks between folds of the int32 digit sums into E: an accumulator row
-// (kl, c) sees 16 entries per K-step, 170 x 24 x 16 = 65 280 between folds,
-// each adding at most 128 x 64 = 8192, so the sums stay below 2^31
-constexpr int MB_FOLD = 170;
+// chunks between folds of the digit sums into E: an accumulator row (kl, c)
+// sees 16 entries per K-step, N = 85 x 24 x 16 = 32 640 between folds, each
+// adding at most 128 x 128, so the int32 digit sums stay below 2^30; and the
+// values they encode, at most N 2^30 x 128 < 2^52 for the codes and
+// N 2^31 x 64 < 2^52 for either half of D, are integers that the FP64
+// recombination of their digits forms exactly
+constexpr int MB_FOLD = 85;
 
 // SNP rows in I-block-major order: block b (samples 256 b .. 256 b + 255)
 // of SNP j at dwords (b m + j) 16 .. + 15, dosage 0/1/2 per 2-bit field,
@@ -402,16 +410,18 @@ __device__ __forceinline__ uint2 lds_read_tr8(const void* p) {
 }
 
 // B fragments of one K-step, one per sample of the quad, from the quad's 16
-// entry bytes x: the 2-bit codes (dosage, 3 for a missing call; scaled by 1,
-// 4, 16, 4) and, where D is added, the missing-call indicators (scaled by 1,
-// 4, 16, 64).
+// entry bytes x: the 2-bit codes c (dosage, 3 for a missing call) as c, 4c
+// and 16c for samples 0-2 and, for sample 3, its field in place with the top
+// bit flipped, which as a signed byte is 64 (c - 2) (one bitwise operation;
+// the -128 per entry is put back from the sum of the entries' K); where D is
+// added, the missing-call indicators (scaled by 1, 4, 16, 64).
 __device__ __forceinline__ void code_fragments(const uint32_t (&x)[4], v4i (&bd)[4]) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         bd[0][q] = (int)(x[q] & 0x03030303u);
         bd[1][q] = (int)(x[q] & 0x0c0c0c0cu);
         bd[2][q] = (int)(x[q] & 0x30303030u);
-        bd[3][q] = (int)((x[q] >> 4) & 0x0c0c0c0cu);
+        bd[3][q] = (int)((x[q] ^ 0x80808080u) & 0xc0c0c0c0u);
     }
 }
 __device__ __forceinline__ void missing_fragments(const uint32_t (&x)[4], v4i (&bm)[4]) {
@@ -455,6 +465,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                    int64_t P, const int32_t* __restrict__ gp,
                                                    const int64_t* __restrict__ gstep, const int32_t* __restrict__ gj,
                                                    const uint32_t* __restrict__ gK, const double* __restrict__ sqd,
+                                                   const unsigned long long* __restrict__ sks,
                                                    double* __restrict__ E, int64_t lde) {
     __shared__ __attribute__((aligned(128))) uint32_t sP[MB_CHUNK * MB_SPK];           // K planes, 9 KB
     __shared__ __attribute__((aligned(128))) uint32_t sV[MB_CHUNK * MB_SPD];           // D planes, 15 KB
@@ -491,34 +502,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         v4i acc[4], af0[4], af1[4];
 #pragma unroll
         for (int s_ = 0; s_ < 4; ++s_) acc[s_] = af0[s_] = af1[s_] = (v4i){0, 0, 0, 0};
-        // E(k, i) from sum_c acc_c 256^c over the 4 digit rows, scale removed:
-        // written, or added to a folded part; the last call also sets E(k, k)
+        // sum_c a_c 256^c of the 4 digit rows in FP64: every partial sum is
+        // an integer multiple of 2^8 or 2^16 below 2^53 in magnitude (the fold
+        // bound), so the value is exact
+        auto digits = [](const v4i& a) {
+            double t = fma((double)a[3], 0x1p24, (double)a[2] * 0x1p16);
+            t = fma((double)a[1], 0x1p8, t);
+            return t + (double)a[0];
+        };
+        // E(k, i) from the digit sums, scales removed: written, or added to a
+        // folded part; the last call also sets E(k, k) and puts back sample 3's
+        // code offset, 2 K for every entry of sample k's list (2 sks[k]).
         auto emit = [&](bool add, bool last) {
             const int64_t k_out = g * MB_GROUP + kl_out;
             if (k_out >= n) return;
             double* row = E + k_out * lde + I0;
             const double fsc = fm == 2 ? 0x1p-58 : 0x1p-59;
+            const double off3 = last ? 2.0 * (double)sks[k_out] : 0.0;  // exact below 2^22 entries
 #pragma unroll
             for (int s_ = 0; s_ < 4; ++s_) {
                 const int64_t i_loc = 4 * Qf + s_;
                 if (I0 + i_loc >= n) continue;
-                const int sh = s_ == 0 ? 0 : s_ == 1 ? 2 : s_ == 2 ? 4 : 2;  // the code scales
-                const int64_t ev = ((int64_t)acc[s_][0] + ((int64_t)acc[s_][1] << 8) + ((int64_t)acc[s_][2] << 16) +
-                                    ((int64_t)acc[s_][3] << 24)) >> sh;
-                double v = (double)ev * TWO_M29;
+                // the code scales 1, 4, 16, and 64 (c - 2) for sample 3
+                const double ev = s_ == 3 ? digits(acc[3]) * 0x1p-6 + off3
+                                          : digits(acc[s_]) * (s_ == 0 ? 1.0 : s_ == 1 ? 0.25 : 0.0625);
+                double v = ev * TWO_M29;
                 if (fm != 0) {
-                    const int shd = 2 * s_;  // the indicator scales
-                    const int64_t flo = ((int64_t)af0[s_][0] + ((int64_t)af0[s_][1] << 8) +
-                                         ((int64_t)af0[s_][2] << 16) + ((int64_t)af0[s_][3] << 24)) >> shd;
-                    const int64_t fhi = ((int64_t)af1[s_][0] + ((int64_t)af1[s_][1] << 8) +
-                                         ((int64_t)af1[s_][2] << 16) + ((int64_t)af1[s_][3] << 24)) >> shd;
-                    v += ((double)fhi * 4294967296.0 + (double)flo) * fsc;
+                    const double isc = s_ == 0 ? 1.0 : s_ == 1 ? 0.25 : s_ == 2 ? 0.0625 : 0.015625;  // indicators
+                    const double flo = digits(af0[s_]) * isc, fhi = digits(af1[s_]) * isc;
+                    v += (fhi * 4294967296.0 + flo) * fsc;
                 }
                 if (add) v += row[i_loc];
                 if (last && I0 + i_loc == k_out) v = sqd[k_out] * 0.5;  // E(k, k) = s_k / 2
                 row[i_loc] = v;
             }
         };
+
         for (int64_t c0 = 0; c0 < nsteps; c0 += MB_CHUNK) {
             const int csteps = (int)(nsteps - c0 < MB_CHUNK ? nsteps - c0 : MB_CHUNK);
             __syncthreads();  // the previous chunk's readers are done with the staging
@@ -1035,11 +1054,11 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
         DevBuf<int32_t>& lj = grm->lj;
         DevBuf<int32_t>& cursor = grm->cursor;
         DevBuf<uint32_t>& lk = grm->lk;
-        DevBuf<unsigned long long>& sq = grm->sq;  // per sample: low words, then high words
+        DevBuf<unsigned long long>& sq = grm->sq;  // per sample: K^2 low words, high words, then the sums of K
         grow(lj, nlist);
         grow(lk, nlist);
         grow(cursor, n);
-        grow(sq, 2 * n);
+        grow(sq, 3 * n);
         CB_HIP(hipMemsetAsync(lj.p, 0, nlist * 4, st));
         CB_HIP(hipMemsetAsync(lk.p, 0, nlist * 4, st));
         CB_HIP(hipMemsetAsync(cursor.p, 0, n * 4, st));
@@ -1049,7 +1068,8 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
                 lk.p);
         });
         timed(ctx, "k_sample_sq", [&] {
-            k_sample_sq<<<dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st>>>(d_smp_off.p, lk.p, n, sq.p, sq.p + n);
+            k_sample_sq<<<dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st>>>(d_smp_off.p, lk.p, n, sq.p, sq.p + n,
+                                                                            sq.p + 2 * n);
         });
         // u_i += sum_{j miss i} mu_j^2
         std::vector<unsigned long long> hsq(2 * n);
@@ -1116,7 +1136,7 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
         CB_REQUIRE(nb * P < ((int64_t)1 << 31), "GRM: too many samples for the missing-call kernel's grid");
         timed(ctx, "k_miss_cross", [&] {
             k_miss_mfma<<<(unsigned)(nb * P), 256, 0, st>>>(bedi.p, m, n, P, dgp.p, dgstep.p, gj.p, gK.p, sqd.p,
-                                                           E.p, n);
+                                                           sq.p + 2 * n, E.p, n);
         });
     }
 

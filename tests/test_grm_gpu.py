@@ -66,10 +66,11 @@ def test_random_cases(cb, ctx, n, m, miss, seed):
 
 def test_long_missing_lists(cb, ctx):
     """Samples 0 and 3 missing at all 300,000 SNPs, sample 5 at every other
-    one. Their lists outrun one fold of the missing-call kernel's int32 digit
-    sums (261,120 entries of up to 4096 each), so the kernel must add its
-    partial sums into E and start again; the pair (0, 3) shares every
-    missing SNP, the largest both-missing sum."""
+    one. Their lists outrun one fold of the missing-call kernel's digit sums
+    (32,640 entries of a sample), so the kernel must add its partial sums
+    into E and start again, each fold with its own sum of K for sample 3's
+    code offset; the pair (0, 3) shares every missing SNP, the largest
+    both-missing sum."""
     x, _ = synth.simulate_dosages(64, 300_000, maf=(0.05, 0.5), seed=17)
     x[0, :] = -1
     x[3, :] = -1
