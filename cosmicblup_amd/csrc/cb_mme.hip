@@ -956,6 +956,7 @@ __global__ void k_q32_bulk(const double* __restrict__ Q, float* __restrict__ Q32
 struct ColState {
     double rho, pq, rho_new, rr, bb, alpha, beta;
     int active, restarted;
+    int xr_act;  // x and r moved by alpha in this iteration (k_cg_xr_block, read by k_cg_p3)
 };
 
 // The scalar steps run inside the vector kernels that use them: thread 0 of
@@ -1231,6 +1232,224 @@ __global__ __launch_bounds__(NTH) void k_precond_block_dots(const T* __restrict_
     if (threadIdx.x == 0) {
         part_rz[(int64_t)c * DOT_CHUNKS + blockIdx.x] = red[0][0];
         part_rr[(int64_t)c * DOT_CHUNKS + blockIdx.x] = red[1][0];
+    }
+}
+
+// The CG step after the p.q dot when there is a block-Jacobi block of p0 <=
+// PB_LDS rows and no stiff term: k_cg_xr_diag and k_precond_block_dots in
+// one launch. Grid (nbw + nx, r), 256 threads, dynamic LDS of p0 doubles.
+// Every workgroup forms alpha from the dot's chunk partials as
+// k_cg_xr_diag does.
+//  - workgroups nbw.. (CG_RPT rows per thread, loads before the scalar
+//    step): x += alpha p on every row; on the rows >= p0 also r -= alpha q,
+//    z = r dinv, and the partials of r.z and r.r;
+//  - workgroups 0..nbw-1: the block rows' new r (r - alpha q) staged in LDS,
+//    then z = Binv r for the rows i = 4 b + wave, + 4 nbw, ..., summed as
+//    k_precond_block_dots sums them, and those rows' r.z and r.r. The block
+//    rows' r stays unwritten here (every block workgroup reads the old one);
+//    k_cg_p3 writes it, with the same alpha.
+// px[c][2 ntot]: r.z partials of the ntot = gridDim.x workgroups, then r.r.
+template <class TX, class T>
+__global__ __launch_bounds__(256) void k_cg_xr_block(TX* __restrict__ X, T* __restrict__ R, const T* __restrict__ P,
+                                                     const T* __restrict__ Qp, T* __restrict__ Z,
+                                                     const double* __restrict__ dinv, const double* __restrict__ Binv,
+                                                     ColState* __restrict__ st, const double* __restrict__ part,
+                                                     int64_t p0, int64_t rows, int64_t ld, int64_t nbw,
+                                                     double* __restrict__ px) {
+    extern __shared__ double rs[];  // p0 doubles
+    __shared__ double red[2][256];
+    __shared__ double sp[DOT_CHUNKS];
+    __shared__ double s_alpha;
+    __shared__ int s_act;
+    const int c = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const bool blk = (int64_t)blockIdx.x < nbw;  // the same for the whole workgroup
+    const int64_t ntot = gridDim.x;
+    const int64_t base = ((int64_t)blockIdx.x - nbw) * (256 * CG_RPT) + tid;
+    TX xv[CG_RPT];
+    T rv0[CG_RPT], pv[CG_RPT], qv[CG_RPT];
+    double dv[CG_RPT];
+    if (!blk) {
+#pragma unroll
+        for (int j = 0; j < CG_RPT; ++j) {
+            const int64_t i = base + j * 256, ii = i < rows ? i : 0;
+            const int64_t o = (int64_t)c * ld + ii;
+            xv[j] = X[o];
+            pv[j] = P[o];
+            rv0[j] = R[o];
+            qv[j] = Qp[o];
+            dv[j] = dinv[ii];
+        }
+    }
+    stage_chunks(part, c, sp, tid);
+    __syncthreads();
+    if (tid == 0) {
+        const double rho = st[c].rho_new;
+        int act = st[c].active;
+        const double pq = sum_staged(sp);
+        if (act && fabs(pq) < 1e-300) act = 0;  // breakdown: freeze the column
+        const double alpha = act ? rho / pq : 0.0;
+        if (blockIdx.x == 0) {
+            st[c].pq = pq;
+            st[c].alpha = alpha;
+            st[c].active = act;
+            st[c].rho = rho;
+            st[c].xr_act = act;
+        }
+        s_alpha = alpha;
+        s_act = act;
+    }
+    __syncthreads();
+    const double a = s_alpha;
+    double rz = 0, rr = 0;
+    if (!blk) {
+#pragma unroll
+        for (int j = 0; j < CG_RPT; ++j) {
+            const int64_t i = base + j * 256;
+            if (i >= rows) break;
+            const int64_t o = (int64_t)c * ld + i;
+            if (s_act) X[o] = (TX)((double)xv[j] + a * (double)pv[j]);
+            if (i >= p0) {
+                T rv = rv0[j];
+                if (s_act) {
+                    rv = (T)((double)rv - a * (double)qv[j]);
+                    R[o] = rv;
+                }
+                rr += (double)rv * (double)rv;
+                const T zv = (T)((double)rv * dv[j]);
+                Z[o] = zv;
+                rz += (double)rv * (double)zv;
+            }
+        }
+    } else {
+        const T* rc = R + (int64_t)c * ld;
+        const T* qc = Qp + (int64_t)c * ld;
+        for (int64_t k = tid; k < p0; k += 256) {
+            const double r0 = (double)rc[k];
+            rs[k] = s_act ? (double)(T)(r0 - a * (double)qc[k]) : r0;
+        }
+        __syncthreads();
+        for (int64_t i = (int64_t)blockIdx.x * 4 + wave; i < p0; i += nbw * 4) {
+            const double* b = Binv + i * p0;
+            double sum[4] = {0, 0, 0, 0};
+            for (int64_t k0 = lane; k0 < p0; k0 += 64 * PB_U) {
+                double bv[PB_U], rv[PB_U];
+#pragma unroll
+                for (int j = 0; j < PB_U; ++j) {
+                    const int64_t k = k0 + 64 * j;
+                    bv[j] = k < p0 ? b[k] : 0.0;
+                }
+#pragma unroll
+                for (int j = 0; j < PB_U; ++j) {
+                    const int64_t k = k0 + 64 * j;
+                    rv[j] = k < p0 ? rs[k] : 0.0;
+                }
+#pragma unroll
+                for (int j = 0; j < PB_U; ++j) sum[j & 3] += bv[j] * rv[j];
+            }
+            double t = (sum[0] + sum[1]) + (sum[2] + sum[3]);
+            for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off);
+            if (lane == 0) {
+                const T z = (T)t;
+                Z[(int64_t)c * ld + i] = z;
+                rz += rs[i] * (double)z;
+                rr += rs[i] * rs[i];
+            }
+        }
+    }
+    red[0][tid] = rz;
+    red[1][tid] = rr;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            red[0][tid] += red[0][tid + w];
+            red[1][tid] += red[1][tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        px[(int64_t)c * 2 * ntot + blockIdx.x] = red[0][0];
+        px[(int64_t)c * 2 * ntot + ntot + blockIdx.x] = red[1][0];
+    }
+}
+
+// After k_cg_xr_block: rho_new = r.z and rr = r.r from its ntot partials
+// (each thread its strided run in order, then a fixed tree), convergence and
+// beta as k_cg_p; the block rows' r moved by that kernel's alpha (r - alpha
+// q, the value it used); active columns: p = z + beta p.
+template <class T>
+__global__ __launch_bounds__(256) void k_cg_p3(T* __restrict__ P, const T* __restrict__ Z, T* __restrict__ R,
+                                               const T* __restrict__ Qp, ColState* __restrict__ st,
+                                               const double* __restrict__ px, int64_t ntot, double tol2,
+                                               int absolute, int64_t p0, int64_t rows, int64_t ld) {
+    __shared__ double red[2][256];
+    __shared__ double s_beta, s_alpha;
+    __shared__ int s_act, s_xr;
+    const int c = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * (256 * CG_RPT) + tid;
+    const bool has_block = (int64_t)blockIdx.x * (256 * CG_RPT) < p0;  // the same for the whole workgroup
+    T zv[CG_RPT], pv[CG_RPT], rb[CG_RPT], qb[CG_RPT];
+#pragma unroll
+    for (int j = 0; j < CG_RPT; ++j) {
+        const int64_t i = base + j * 256;
+        const int64_t o = (int64_t)c * ld + (i < rows ? i : 0);
+        zv[j] = Z[o];
+        pv[j] = P[o];
+        if (has_block) {
+            rb[j] = R[o];
+            qb[j] = Qp[o];
+        }
+    }
+    double sa = 0, sb = 0;
+    for (int64_t k = tid; k < ntot; k += 256) {
+        sa += px[(int64_t)c * 2 * ntot + k];
+        sb += px[(int64_t)c * 2 * ntot + ntot + k];
+    }
+    red[0][tid] = sa;
+    red[1][tid] = sb;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            red[0][tid] += red[0][tid + w];
+            red[1][tid] += red[1][tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double rho = st[c].rho, bb = st[c].bb;
+        int act = st[c].active;
+        double beta = 0.0;
+        if (act) {
+            const double rho_new = red[0][0], rr = red[1][0];
+            const double thr = absolute ? tol2 : tol2 * bb;
+            if (!(rr >= thr)) act = 0;  // converged (or NaN)
+            beta = rho != 0.0 ? rho_new / rho : 0.0;
+            if (blockIdx.x == 0) {
+                st[c].rho_new = rho_new;
+                st[c].rr = rr;
+                st[c].beta = beta;
+                st[c].active = act;
+            }
+        }
+        s_beta = beta;
+        s_act = act;
+        s_alpha = st[c].alpha;  // this kernel writes neither alpha nor xr_act
+        s_xr = st[c].xr_act;
+    }
+    __syncthreads();
+    if (has_block && s_xr) {
+#pragma unroll
+        for (int j = 0; j < CG_RPT; ++j) {
+            const int64_t i = base + j * 256;
+            if (i < p0) R[(int64_t)c * ld + i] = (T)((double)rb[j] - s_alpha * (double)qb[j]);
+        }
+    }
+    if (!s_act) return;
+#pragma unroll
+    for (int j = 0; j < CG_RPT; ++j) {
+        const int64_t i = base + j * 256;
+        if (i < rows) P[(int64_t)c * ld + i] = (T)((double)zv[j] + s_beta * (double)pv[j]);
     }
 }
 
@@ -1857,6 +2076,13 @@ void q_apply(cb_mme* M, const T* X, int64_t ldx, T* Y, int64_t ldy, int r, T alp
     }
 }
 
+// a CSR Q rides in the pass over W'W's short rows when those hold every
+// random-effect row and W'W's rows need no more lanes than Q's
+inline bool csr_fused(const cb_mme* M) {
+    return M->qkind != 0 && M->long_rows <= M->p && M->dim > M->long_rows && M->q_lanes >= 2 &&
+           M->ww_lanes <= M->q_lanes;
+}
+
 // Y = C(lambda) X for an r-column block, arithmetic in T. With lcp_out the
 // long rows of W'W that span several chunks may be left unfinished when the
 // pass over the short rows carries the chunk partials (its condition puts
@@ -1869,10 +2095,7 @@ void apply_C(cb_mme* M, double lambda, const T* X, T* Y, int r, double** lcp_out
     cb_context* ctx = M->ctx;
     hipStream_t st = ctx->stream;
     const int64_t ld = M->ldv;
-    // a CSR Q rides in the pass over W'W's short rows when those hold every
-    // random-effect row and W'W's rows need no more lanes than Q's
-    const bool fused = M->qkind != 0 && M->long_rows <= M->p && M->dim > M->long_rows && M->q_lanes >= 2 &&
-                       M->ww_lanes <= M->q_lanes;
+    const bool fused = csr_fused(M);
     const bool defer = lcp_out != nullptr && fused && M->long_rows > 0 && M->n_multi <= MULTI_FIN_MAX;
     if (lcp_out) *lcp_out = nullptr;
     double* lcp = nullptr;
@@ -2049,8 +2272,15 @@ void apply_precond(cb_mme* M, const T* R, T* Z, int r) {
 // chunk partials per column followed by the x/r update's two sets of
 // workgroup partials per column.
 inline bool fused_dots(const cb_mme* M) { return M->p > 0 && !M->stiff; }
+
+// The x/r update and the block preconditioner in one launch (k_cg_xr_block,
+// then k_cg_p3): one block row per wave, a block of at most PB_LDS rows
+inline bool xr_block_path(const cb_mme* M) { return fused_dots(M) && M->p <= PB_LDS; }
+inline int64_t xr_block_wgs(const cb_mme* M) { return ceil_div(M->p, 4); }
+// doubles per column of the r.z / r.r partials (the workspace part2)
 inline size_t part2_count(const cb_mme* M, int r) {
-    return ((size_t)DOT_CHUNKS + (fused_dots(M) ? 2 * (size_t)ceil_div(M->dim, 256 * CG_RPT) : 0)) * r;
+    return ((size_t)DOT_CHUNKS +
+            (fused_dots(M) ? 2 * ((size_t)ceil_div(M->dim, 256 * CG_RPT) + (size_t)ceil_div(M->p, 4)) : 0)) * r;
 }
 
 // One CG iteration on an r-column block after P holds the search
@@ -2076,6 +2306,16 @@ void cg_iteration(cb_mme* M, double lambda, TX* X, T* R, T* Z, T* P, T* Qp, ColS
         k_col_dots<T, 1024><<<gd, 1024, 0, st>>>(P, Qp, ld, dim, part);
     } else {
         k_col_dots<T><<<gd, 256, 0, st>>>(P, Qp, ld, dim, part);
+    }
+    if (xr_block_path(M)) {
+        // alpha, x, r and z of the Jacobi and block rows with the r.z / r.r
+        // partials in one launch, then beta, p and the block rows' r
+        const int64_t nbw = xr_block_wgs(M), ntot = nbw + g2.x;
+        k_cg_xr_block<TX, T><<<dim3((unsigned)ntot, (unsigned)r), 256, (size_t)M->p * sizeof(double), st>>>(
+            X, R, P, Qp, Z, M->dinv.p, M->Binv.p, cs, part, M->p, dim, ld, nbw, part2);
+        k_cg_p3<T><<<g2, 256, 0, st>>>(P, Z, R, Qp, cs, part2, ntot, tol2, absolute ? 1 : 0, M->p, dim, ld);
+        CB_HIP(hipGetLastError());
+        return;
     }
     if (fused_dots(M)) {
         // r.z and r.r ride in the x/r update (workgroup partials after the

@@ -307,6 +307,41 @@ def test_pblup_native_ainv_with_inbreeding_mixed_vs_dense(cb, ctx):
     np.testing.assert_allclose([R.var_u, R.var_e], [o["vu"], o["ve"]], rtol=VC_TOL)
 
 
+def test_pblup_mean_row_of_several_chunks(cb, ctx):
+    """PBLUP with 8000 records: the mean row of W'W has more than one chunk
+    of 4096 nonzeros, so the CG iteration finishes it from its chunk partials
+    inside the dot and update kernels, and the 40 group rows are one chunk
+    each. FP64 and mixed solutions against a sparse direct solve of the same
+    MME assembled on the host with scipy."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+
+    from cosmicblup_amd.pedigree import invertA_henderson
+
+    dam, sire = synth.simulate_pedigree(3, 4000, unknown=0.05, seed=31)
+    q = dam.size
+    A = invertA_henderson(dam, sire)
+    rng = np.random.default_rng(32)
+    rec = np.sort(rng.choice(q, 8000, replace=False)).astype(np.int32)
+    y = rng.standard_normal(rec.size) + 4.0
+    fd = design.build_fixed_design([[f"g{int(v):02d}"] for v in rng.integers(0, 40, rec.size)])
+    lam = 2.5
+    X = sp.csr_matrix((fd.val, fd.col, fd.ptr), shape=(rec.size, fd.p))
+    Zi = sp.csr_matrix((np.ones(rec.size), rec.astype(np.int64), np.arange(rec.size + 1)), shape=(rec.size, q))
+    W = sp.hstack([X, Zi]).tocsr()
+    Ai = sp.csr_matrix((A.val, A.col, A.ptr), shape=(q, q))
+    Cm = (W.T @ W + sp.block_diag([sp.csr_matrix((fd.p, fd.p)), lam * Ai])).tocsc()
+    b = W.T @ y
+    ref = spl.spsolve(Cm, b)
+    M = mme.MixedModel(fd, rec, y, q, ctx=ctx)
+    M.set_qinv_csr(A.ptr, A.col, A.val)
+    for prec in (_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED):
+        x, it, rel = M.solve(lam, b, mme.solve_options(precision=prec, tol=1e-12, max_iter=5000))
+        assert rel < 1e-12, (prec, it, rel)
+        assert np.linalg.norm(Cm @ x - b) / np.linalg.norm(b) < 1e-11
+        assert ebv_err(x[M.p:], ref[M.p:]) < EBV_TOL
+
+
 def test_pblup_csr_ainv(cb, ctx):
     """PBLUP with Henderson A^-1 as CSR (SparseMatrixAdapter path)."""
     rng = np.random.default_rng(2)
