@@ -33,6 +33,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <initializer_list>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -1763,6 +1765,17 @@ struct cb_mme {
     // one solve to the next (grown, never shrunk): repeated solves allocate
     // and free nothing, and so never wait on the device for a free
     DevBuf<uint8_t> ws[16];
+    // pinned host page, allocated with the model, for the solvers' small
+    // exchanges of device state (the column states and dots of each check):
+    // a kernel stores them there (or loads them from there), so a check
+    // waits on the stream once and uses no copy engine. Lower half: device
+    // to host; upper half: host to device (h2d_pending until the stream has
+    // been waited on since).
+    void* hpage = nullptr;
+    bool h2d_pending = false;
+    ~cb_mme() {
+        if (hpage) (void)hipHostFree(hpage);
+    }
 };
 
 namespace {
@@ -1809,6 +1822,89 @@ WsPtr<T> ws(cb_mme* M, WsSlot slot, size_t count) {
     static_assert(WS_COUNT <= 16, "work buffer slots");
     uint8_t* p = scratch_or<uint8_t>(M, (int)slot, M->ws[slot], count * sizeof(T));
     return WsPtr<T>{reinterpret_cast<T*>(p), count};
+}
+
+constexpr size_t HPAGE_HALF = size_t(32) << 10;  // bytes of each half of a model's pinned page
+
+// Host polls the stream until its queued work is done (no sleep, so the
+// solver's checks see the device finish at once).
+void wait_idle(cb_mme* M) {
+    hipStream_t st = M->ctx->stream;
+    for (;;) {
+        const hipError_t e = hipStreamQuery(st);
+        if (e == hipSuccess) break;
+        if (e != hipErrorNotReady) CB_HIP(e);
+        for (int k = 0; k < 64; ++k) __builtin_ia32_pause();
+    }
+    M->h2d_pending = false;
+}
+
+// h[...] = the words of up to three device regions, one after the other
+__global__ void k_to_host(const uint32_t* __restrict__ a, int64_t na, const uint32_t* __restrict__ b, int64_t nb,
+                          const uint32_t* __restrict__ c, int64_t nc, uint32_t* __restrict__ h) {
+    const int64_t n = na + nb + nc;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+        h[i] = i < na ? a[i] : i < na + nb ? b[i - na] : c[i - na - nb];
+}
+__global__ void k_from_host(const uint32_t* __restrict__ h, int64_t n, uint32_t* __restrict__ d) {
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) d[i] = h[i];
+}
+
+struct HostPart {
+    void* host;
+    const void* dev;
+    size_t bytes;
+};
+// Reads up to three small device regions (whole words) behind the stream's
+// queued work: one kernel stores them into the pinned page and one wait
+// follows; without the page, or past its half, through copy_host.
+void read_small(cb_mme* M, std::initializer_list<HostPart> parts) {
+    hipStream_t st = M->ctx->stream;
+    size_t tot = 0;
+    bool words = parts.size() <= 3;
+    for (const HostPart& q : parts) {
+        tot += q.bytes;
+        words = words && q.bytes % 4 == 0;
+    }
+    if (!M->hpage || tot > HPAGE_HALF || !words) {
+        for (const HostPart& q : parts) copy_host(st, q.host, q.dev, q.bytes, hipMemcpyDeviceToHost);
+        CB_HIP(hipStreamSynchronize(st));
+        M->h2d_pending = false;
+        return;
+    }
+    const uint32_t* src[3] = {nullptr, nullptr, nullptr};
+    int64_t nw[3] = {0, 0, 0};
+    int k = 0;
+    for (const HostPart& q : parts) {
+        src[k] = static_cast<const uint32_t*>(q.dev);
+        nw[k] = (int64_t)(q.bytes / 4);
+        ++k;
+    }
+    k_to_host<<<1, 256, 0, st>>>(src[0], nw[0], src[1], nw[1], src[2], nw[2], static_cast<uint32_t*>(M->hpage));
+    CB_HIP(hipGetLastError());
+    wait_idle(M);
+    size_t off = 0;
+    for (const HostPart& q : parts) {
+        std::memcpy(q.host, static_cast<const char*>(M->hpage) + off, q.bytes);
+        off += q.bytes;
+    }
+}
+// Writes a small host block (whole words) to the device in stream order:
+// staged in the page's upper half and loaded by a kernel, with no wait
+// (a second write first waits for the stream, which has then read the first).
+void write_small(cb_mme* M, void* dev, const void* host, size_t bytes) {
+    hipStream_t st = M->ctx->stream;
+    if (!M->hpage || bytes > HPAGE_HALF || bytes % 4 != 0) {
+        copy_host(st, dev, host, bytes, hipMemcpyHostToDevice);
+        return;
+    }
+    if (M->h2d_pending) wait_idle(M);
+    char* up = static_cast<char*>(M->hpage) + HPAGE_HALF;
+    std::memcpy(up, host, bytes);
+    k_from_host<<<1, 256, 0, st>>>(reinterpret_cast<const uint32_t*>(up), (int64_t)(bytes / 4),
+                                   static_cast<uint32_t*>(dev));
+    CB_HIP(hipGetLastError());
+    M->h2d_pending = true;
 }
 
 template <class T>
@@ -2408,7 +2504,7 @@ int64_t pcg_block(cb_mme* M, double lambda, const T* B, TX* X, int r, double tol
     const size_t nv = (size_t)ld * r;
     build_precond(M, lambda, block_precond);
     const WsPtr<T> R = ws<T>(M, WS_R, nv), Z = ws<T>(M, WS_Z, nv), P = ws<T>(M, WS_P, nv), Qp = ws<T>(M, WS_QP, nv);
-    const WsPtr<double> part = ws<double>(M, WS_PART, (size_t)DOT_CHUNKS * r), d1 = ws<double>(M, WS_D1, r),
+    const WsPtr<double> part = ws<double>(M, WS_PART, (size_t)DOT_CHUNKS * r), d1 = ws<double>(M, WS_D1, 2 * (size_t)r),
                         d2 = ws<double>(M, WS_D2, r);
     const WsPtr<ColState> cs = ws<ColState>(M, WS_CS, r);
     CB_HIP(hipMemsetAsync(R.p, 0, R.bytes(), st));
@@ -2432,10 +2528,11 @@ int64_t pcg_block(cb_mme* M, double lambda, const T* B, TX* X, int r, double tol
     CB_HIP(hipMemcpyAsync(P.p, Z.p, nv * sizeof(T), hipMemcpyDeviceToDevice, st));
     col_dots<T>(M, R.p, Z.p, r, part.p, d1.p);
     col_dots<T>(M, B, B, r, part.p, d2.p);
-    std::vector<double> rz(r), bb(r);
-    copy_host(st, rz.data(), d1.p, r * 8, hipMemcpyDeviceToHost);
-    copy_host(st, bb.data(), d2.p, r * 8, hipMemcpyDeviceToHost);
-    CB_HIP(hipStreamSynchronize(st));
+    col_dots<T>(M, R.p, R.p, r, part.p, d1.p + r);  // the initial residual check
+    std::vector<double> rz(r), bb(r), rr0(r);
+    read_small(M, {{rz.data(), d1.p, (size_t)r * 8},
+                   {bb.data(), d2.p, (size_t)r * 8},
+                   {rr0.data(), d1.p + r, (size_t)r * 8}});
     std::vector<ColState> hs(r);
     for (int c = 0; c < r; ++c) {
         hs[c] = ColState{};
@@ -2443,18 +2540,14 @@ int64_t pcg_block(cb_mme* M, double lambda, const T* B, TX* X, int r, double tol
         hs[c].bb = bb[c] == 0.0 ? 1.0 : bb[c];  // pcg_solver.h:463: bnorm == 0 -> 1
         hs[c].active = 1;
     }
-    // initial residual check (a zero RHS column is solved by x = 0)
-    col_dots<T>(M, R.p, R.p, r, part.p, d1.p);
-    std::vector<double> rr0(r);
-    copy_host(st, rr0.data(), d1.p, r * 8, hipMemcpyDeviceToHost);
-    CB_HIP(hipStreamSynchronize(st));
+    // (a zero RHS column is solved by x = 0)
     const double tol2 = tol * tol;
     for (int c = 0; c < r; ++c) {
         const double thr = absolute ? tol2 : tol2 * hs[c].bb;
         hs[c].rr = rr0[c];
         if (!(rr0[c] >= thr)) hs[c].active = 0;
     }
-    copy_host(st, cs.p, hs.data(), r * sizeof(ColState), hipMemcpyHostToDevice);
+    write_small(M, cs.p, hs.data(), r * sizeof(ColState));
     int64_t it = 0;
     const int check_every = 8;
     bool any_active = std::any_of(hs.begin(), hs.end(), [](const ColState& s) { return s.active; });
@@ -2481,13 +2574,11 @@ int64_t pcg_block(cb_mme* M, double lambda, const T* B, TX* X, int r, double tol
             ++it;
         }
         if (it % check_every == 0 || it == max_iter) {
-            copy_host(st, hs.data(), cs.p, r * sizeof(ColState), hipMemcpyDeviceToHost);
-            CB_HIP(hipStreamSynchronize(st));
+            read_small(M, {{hs.data(), cs.p, r * sizeof(ColState)}});
             any_active = std::any_of(hs.begin(), hs.end(), [](const ColState& s) { return s.active; });
         }
     }
-    copy_host(st, hs.data(), cs.p, r * sizeof(ColState), hipMemcpyDeviceToHost);
-    CB_HIP(hipStreamSynchronize(st));
+    read_small(M, {{hs.data(), cs.p, r * sizeof(ColState)}});
     if (rel_out) {
         rel_out->resize(r);
         for (int c = 0; c < r; ++c)
@@ -2535,8 +2626,7 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
     std::vector<std::vector<double>> hist(r);  // true residual at each replacement
     std::vector<ColState> hs(r);
     col_dots<double>(M, B64, B64, r, part.p, d1.p);
-    copy_host(st, bb.data(), d1.p, r * 8, hipMemcpyDeviceToHost);
-    CB_HIP(hipStreamSynchronize(st));
+    read_small(M, {{bb.data(), d1.p, (size_t)r * 8}});
     for (int c = 0; c < r; ++c) bb[c] = bb[c] == 0.0 ? 1.0 : bb[c];
     std::vector<int> done(r, 0);  // true-residual convergence per column
     // (re)start from the true residual: R64 = B - C x, R = (float)R64,
@@ -2552,10 +2642,11 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
         apply_precond<float>(M, R.p, Z.p, r);
         col_dots<double>(M, R64.p, R64.p, r, part.p, d1.p);
         col_dots<float>(M, R.p, Z.p, r, part.p, d2.p);
-        copy_host(st, rr.data(), d1.p, r * 8, hipMemcpyDeviceToHost);
-        copy_host(st, rz.data(), d2.p, r * 8, hipMemcpyDeviceToHost);
-        if (!first) copy_host(st, hs.data(), cs.p, r * sizeof(ColState), hipMemcpyDeviceToHost);
-        CB_HIP(hipStreamSynchronize(st));
+        if (first) read_small(M, {{rr.data(), d1.p, (size_t)r * 8}, {rz.data(), d2.p, (size_t)r * 8}});
+        else
+            read_small(M, {{rr.data(), d1.p, (size_t)r * 8},
+                           {rz.data(), d2.p, (size_t)r * 8},
+                           {hs.data(), cs.p, r * sizeof(ColState)}});
         bool all = true;
         for (int c = 0; c < r; ++c) {
             if (!std::isfinite(rr[c]))
@@ -2594,7 +2685,7 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
                                       hipMemcpyDeviceToDevice, st));
             all = all && done[c];
         }
-        copy_host(st, cs.p, hs.data(), r * sizeof(ColState), hipMemcpyHostToDevice);
+        write_small(M, cs.p, hs.data(), r * sizeof(ColState));
         return all;
     };
     bool all_done = replace(true);
@@ -2625,8 +2716,7 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
             ++it;
         }
         if (it % check_every == 0 || it == o.max_iter) {
-            copy_host(st, hs.data(), cs.p, r * sizeof(ColState), hipMemcpyDeviceToHost);
-            CB_HIP(hipStreamSynchronize(st));
+            read_small(M, {{hs.data(), cs.p, r * sizeof(ColState)}});
             bool need = false;
             for (int c = 0; c < r; ++c) {
                 if (done[c]) continue;
@@ -3212,6 +3302,12 @@ int cb_mme_create(cb_context* ctx, int64_t n, int64_t p, int64_t q, const int64_
             delete M;
             throw;
         }
+        // the pinned page of the solvers' state exchanges (without it they
+        // copy through the staging buffer)
+        if (hipHostMalloc(&M->hpage, 2 * HPAGE_HALF, hipHostMallocCoherent) != hipSuccess) {
+            M->hpage = nullptr;
+            (void)hipGetLastError();
+        }
         *out = M;
     });
 }
@@ -3571,7 +3667,7 @@ static size_t noalloc_need(const cb_mme* M, int64_t r, int precision) {
     const size_t vec = ld * R;
     size_t need = 2 * (vec * 8 + a);  // the right-hand-side and solution blocks
     need += (precision == CB_PREC_FP64 ? 4 * (vec * 8 + a) : 4 * (vec * 4 + a) + 2 * (vec * 8 + a));
-    need += ((size_t)DOT_CHUNKS * R * 8 + a) + (part2_count(M, (int)r) * 8 + a) + 2 * (R * 8 + a) + R * sizeof(ColState) + a;
+    need += ((size_t)DOT_CHUNKS * R * 8 + a) + (part2_count(M, (int)r) * 8 + a) + (3 * R * 8 + 2 * a) + R * sizeof(ColState) + a;
     if (M->long_rows > 0) need += (size_t)std::max<int64_t>(1, M->n_lchunks) * R * 8 + a;
     if (M->qkind == 0) {
         if (M->stiff) need += (size_t)DOT_CHUNKS * R * 8 + a;
