@@ -721,6 +721,8 @@ def main() -> None:
     ap.add_argument("--no-c5", action="store_true", help="skip the 96-instance batched workload")
     ap.add_argument("--c5-only", action="store_true", help="only run the 96-instance workload")
     ap.add_argument("--c5-workers", type=int, default=8, help="contexts (streams and host threads) serving C5")
+    ap.add_argument("--graphs", action="store_true",
+                    help="replay the PCG iteration blocks as captured HIP graphs (cb_context_set_graphs)")
     ap.add_argument("--pcg-only", action="store_true",
                     help="only run the PBLUP PCG leg and print its JSON object")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -760,10 +762,12 @@ def main() -> None:
         return
     if args.reml_only:
         ctx = _lib.Context(0)
+        ctx.set_graphs(args.graphs)
         print(json.dumps(reml_leg(ctx, n=args.reml_n, m=args.reml_m, plain_iters=args.plain_iters)), flush=True)
         return
     if args.pcg_only:
         ctx = _lib.Context(0)
+        ctx.set_graphs(args.graphs)
         print(json.dumps(pcg_leg(ctx)), flush=True)
         return
     if args.c5_only:
@@ -776,6 +780,7 @@ def main() -> None:
     rows, pitch = synth.bed_on_device(n, m, maf=(0.01, 0.5), missing_rate=args.missing, seed=20262203)
     torch.cuda.synchronize()
     ctx = _lib.Context(0)
+    ctx.set_graphs(args.graphs)
     reader = cb.PlinkReader.from_device_rows(ctx, rows.data_ptr(), pitch, n, m)
     del rows
     torch.cuda.empty_cache()
