@@ -1237,29 +1237,33 @@ __global__ __launch_bounds__(NTH) void k_precond_block_dots(const T* __restrict_
     }
 }
 
-// The CG step after the p.q dot when there is a block-Jacobi block of p0 <=
-// PB_LDS rows and no stiff term: k_cg_xr_diag and k_precond_block_dots in
-// one launch. Grid (nbw + nx, r), 256 threads, dynamic LDS of p0 doubles.
-// Every workgroup forms alpha from the dot's chunk partials as
-// k_cg_xr_diag does.
+// The CG step after the p.q dot when the block-Jacobi block (if any) has p0
+// <= PB_LDS rows: k_cg_xr_diag, the preconditioner's block and stiff parts
+// and the r.z / r.r dots in one launch. Grid (nbw + nx, r), 256 threads,
+// dynamic LDS of p0 doubles. Every workgroup forms alpha from the dot's
+// chunk partials as k_cg_xr_diag does.
 //  - workgroups nbw.. (CG_RPT rows per thread, loads before the scalar
 //    step): x += alpha p on every row; on the rows >= p0 also r -= alpha q,
-//    z = r dinv, and the partials of r.z and r.r;
+//    z = r dinv, and the partials of r.z and r.r; with STIFF also of v.z
+//    and r.y over those rows (v the stiff direction, y = D'^-1 v);
 //  - workgroups 0..nbw-1: the block rows' new r (r - alpha q) staged in LDS,
 //    then z = Binv r for the rows i = 4 b + wave, + 4 nbw, ..., summed as
 //    k_precond_block_dots sums them, and those rows' r.z and r.r. The block
 //    rows' r stays unwritten here (every block workgroup reads the old one);
 //    k_cg_p3 writes it, with the same alpha.
-// px[c][2 ntot]: r.z partials of the ntot = gridDim.x workgroups, then r.r.
-template <class TX, class T>
+// px[c][NS ntot] (NS = 2, or 4 with STIFF): r.z partials of the ntot =
+// gridDim.x workgroups, then r.r, then v.z and r.y.
+template <class TX, class T, bool STIFF>
 __global__ __launch_bounds__(256) void k_cg_xr_block(TX* __restrict__ X, T* __restrict__ R, const T* __restrict__ P,
                                                      const T* __restrict__ Qp, T* __restrict__ Z,
                                                      const double* __restrict__ dinv, const double* __restrict__ Binv,
+                                                     const double* __restrict__ sv, const double* __restrict__ sy,
                                                      ColState* __restrict__ st, const double* __restrict__ part,
                                                      int64_t p0, int64_t rows, int64_t ld, int64_t nbw,
                                                      double* __restrict__ px) {
+    constexpr int NS = STIFF ? 4 : 2;
     extern __shared__ double rs[];  // p0 doubles
-    __shared__ double red[2][256];
+    __shared__ double red[NS][256];
     __shared__ double sp[DOT_CHUNKS];
     __shared__ double s_alpha;
     __shared__ int s_act;
@@ -1270,7 +1274,7 @@ __global__ __launch_bounds__(256) void k_cg_xr_block(TX* __restrict__ X, T* __re
     const int64_t base = ((int64_t)blockIdx.x - nbw) * (256 * CG_RPT) + tid;
     TX xv[CG_RPT];
     T rv0[CG_RPT], pv[CG_RPT], qv[CG_RPT];
-    double dv[CG_RPT];
+    double dv[CG_RPT], vv[CG_RPT], yv[CG_RPT];
     if (!blk) {
 #pragma unroll
         for (int j = 0; j < CG_RPT; ++j) {
@@ -1281,6 +1285,11 @@ __global__ __launch_bounds__(256) void k_cg_xr_block(TX* __restrict__ X, T* __re
             rv0[j] = R[o];
             qv[j] = Qp[o];
             dv[j] = dinv[ii];
+            if (STIFF) {
+                const int64_t u = ii >= p0 ? ii - p0 : 0;
+                vv[j] = sv[u];
+                yv[j] = sy[u];
+            }
         }
     }
     stage_chunks(part, c, sp, tid);
@@ -1303,7 +1312,9 @@ __global__ __launch_bounds__(256) void k_cg_xr_block(TX* __restrict__ X, T* __re
     }
     __syncthreads();
     const double a = s_alpha;
-    double rz = 0, rr = 0;
+    double acc[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) acc[k] = 0;  // r.z, r.r (, v.z, r.y)
     if (!blk) {
 #pragma unroll
         for (int j = 0; j < CG_RPT; ++j) {
@@ -1317,10 +1328,14 @@ __global__ __launch_bounds__(256) void k_cg_xr_block(TX* __restrict__ X, T* __re
                     rv = (T)((double)rv - a * (double)qv[j]);
                     R[o] = rv;
                 }
-                rr += (double)rv * (double)rv;
+                acc[1] += (double)rv * (double)rv;
                 const T zv = (T)((double)rv * dv[j]);
                 Z[o] = zv;
-                rz += (double)rv * (double)zv;
+                acc[0] += (double)rv * (double)zv;
+                if (STIFF) {
+                    acc[NS - 2] += (double)zv * vv[j];
+                    acc[NS - 1] += (double)rv * yv[j];
+                }
             }
         }
     } else {
@@ -1354,67 +1369,76 @@ __global__ __launch_bounds__(256) void k_cg_xr_block(TX* __restrict__ X, T* __re
             if (lane == 0) {
                 const T z = (T)t;
                 Z[(int64_t)c * ld + i] = z;
-                rz += rs[i] * (double)z;
-                rr += rs[i] * rs[i];
+                acc[0] += rs[i] * (double)z;
+                acc[1] += rs[i] * rs[i];
             }
         }
     }
-    red[0][tid] = rz;
-    red[1][tid] = rr;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) red[k][tid] = acc[k];
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (tid < w) {
-            red[0][tid] += red[0][tid + w];
-            red[1][tid] += red[1][tid + w];
+#pragma unroll
+            for (int k = 0; k < NS; ++k) red[k][tid] += red[k][tid + w];
         }
         __syncthreads();
     }
     if (tid == 0) {
-        px[(int64_t)c * 2 * ntot + blockIdx.x] = red[0][0];
-        px[(int64_t)c * 2 * ntot + ntot + blockIdx.x] = red[1][0];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) px[((int64_t)c * NS + k) * ntot + blockIdx.x] = red[k][0];
     }
 }
 
 // After k_cg_xr_block: rho_new = r.z and rr = r.r from its ntot partials
 // (each thread its strided run in order, then a fixed tree), convergence and
 // beta as k_cg_p; the block rows' r moved by that kernel's alpha (r - alpha
-// q, the value it used); active columns: p = z + beta p.
-template <class T>
-__global__ __launch_bounds__(256) void k_cg_p3(T* __restrict__ P, const T* __restrict__ Z, T* __restrict__ R,
-                                               const T* __restrict__ Qp, ColState* __restrict__ st,
+// q, the value it used); active columns: p = z + beta p. With STIFF the
+// preconditioner's rank-1 part applies here: z_u -= gamma (v.z_u) y on the
+// rows >= p0 (written back to Z), and r.z is r.z_u - gamma (v.z_u) (r.y)
+// plus the block rows' r.z.
+template <class T, bool STIFF>
+__global__ __launch_bounds__(256) void k_cg_p3(T* __restrict__ P, T* __restrict__ Z, T* __restrict__ R,
+                                               const T* __restrict__ Qp, const double* __restrict__ sy,
+                                               double gamma, ColState* __restrict__ st,
                                                const double* __restrict__ px, int64_t ntot, double tol2,
                                                int absolute, int64_t p0, int64_t rows, int64_t ld) {
-    __shared__ double red[2][256];
-    __shared__ double s_beta, s_alpha;
+    constexpr int NS = STIFF ? 4 : 2;
+    __shared__ double red[NS][256];
+    __shared__ double s_beta, s_alpha, s_t;
     __shared__ int s_act, s_xr;
     const int c = blockIdx.y;
     const int tid = threadIdx.x;
     const int64_t base = (int64_t)blockIdx.x * (256 * CG_RPT) + tid;
     const bool has_block = (int64_t)blockIdx.x * (256 * CG_RPT) < p0;  // the same for the whole workgroup
     T zv[CG_RPT], pv[CG_RPT], rb[CG_RPT], qb[CG_RPT];
+    double yv[CG_RPT];
 #pragma unroll
     for (int j = 0; j < CG_RPT; ++j) {
-        const int64_t i = base + j * 256;
-        const int64_t o = (int64_t)c * ld + (i < rows ? i : 0);
+        const int64_t i = base + j * 256, ii = i < rows ? i : 0;
+        const int64_t o = (int64_t)c * ld + ii;
         zv[j] = Z[o];
         pv[j] = P[o];
+        if (STIFF) yv[j] = sy[ii >= p0 ? ii - p0 : 0];
         if (has_block) {
             rb[j] = R[o];
             qb[j] = Qp[o];
         }
     }
-    double sa = 0, sb = 0;
+    double acc[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) acc[k] = 0;
     for (int64_t k = tid; k < ntot; k += 256) {
-        sa += px[(int64_t)c * 2 * ntot + k];
-        sb += px[(int64_t)c * 2 * ntot + ntot + k];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) acc[q] += px[((int64_t)c * NS + q) * ntot + k];
     }
-    red[0][tid] = sa;
-    red[1][tid] = sb;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) red[k][tid] = acc[k];
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (tid < w) {
-            red[0][tid] += red[0][tid + w];
-            red[1][tid] += red[1][tid + w];
+#pragma unroll
+            for (int k = 0; k < NS; ++k) red[k][tid] += red[k][tid + w];
         }
         __syncthreads();
     }
@@ -1422,8 +1446,10 @@ __global__ __launch_bounds__(256) void k_cg_p3(T* __restrict__ P, const T* __res
         const double rho = st[c].rho, bb = st[c].bb;
         int act = st[c].active;
         double beta = 0.0;
+        const double t = STIFF ? red[NS - 2][0] : 0.0;  // v.z_u
         if (act) {
-            const double rho_new = red[0][0], rr = red[1][0];
+            const double rho_new = STIFF ? red[0][0] - gamma * t * red[NS - 1][0] : red[0][0];
+            const double rr = red[1][0];
             const double thr = absolute ? tol2 : tol2 * bb;
             if (!(rr >= thr)) act = 0;  // converged (or NaN)
             beta = rho != 0.0 ? rho_new / rho : 0.0;
@@ -1436,6 +1462,7 @@ __global__ __launch_bounds__(256) void k_cg_p3(T* __restrict__ P, const T* __res
         }
         s_beta = beta;
         s_act = act;
+        s_t = t;
         s_alpha = st[c].alpha;  // this kernel writes neither alpha nor xr_act
         s_xr = st[c].xr_act;
     }
@@ -1451,7 +1478,13 @@ __global__ __launch_bounds__(256) void k_cg_p3(T* __restrict__ P, const T* __res
 #pragma unroll
     for (int j = 0; j < CG_RPT; ++j) {
         const int64_t i = base + j * 256;
-        if (i < rows) P[(int64_t)c * ld + i] = (T)((double)zv[j] + s_beta * (double)pv[j]);
+        if (i >= rows) continue;
+        T z = zv[j];
+        if (STIFF && i >= p0) {
+            z = (T)((double)z - gamma * s_t * yv[j]);
+            Z[(int64_t)c * ld + i] = z;
+        }
+        P[(int64_t)c * ld + i] = (T)((double)z + s_beta * (double)pv[j]);
     }
 }
 
@@ -2371,12 +2404,11 @@ inline bool fused_dots(const cb_mme* M) { return M->p > 0 && !M->stiff; }
 
 // The x/r update and the block preconditioner in one launch (k_cg_xr_block,
 // then k_cg_p3): one block row per wave, a block of at most PB_LDS rows
-inline bool xr_block_path(const cb_mme* M) { return fused_dots(M) && M->p <= PB_LDS; }
+inline bool xr_block_path(const cb_mme* M) { return M->p <= PB_LDS; }
 inline int64_t xr_block_wgs(const cb_mme* M) { return ceil_div(M->p, 4); }
 // doubles per column of the r.z / r.r partials (the workspace part2)
 inline size_t part2_count(const cb_mme* M, int r) {
-    return ((size_t)DOT_CHUNKS +
-            (fused_dots(M) ? 2 * ((size_t)ceil_div(M->dim, 256 * CG_RPT) + (size_t)ceil_div(M->p, 4)) : 0)) * r;
+    return ((size_t)DOT_CHUNKS + 4 * ((size_t)ceil_div(M->dim, 256 * CG_RPT) + (size_t)ceil_div(M->p, 4))) * r;
 }
 
 // One CG iteration on an r-column block after P holds the search
@@ -2404,12 +2436,23 @@ void cg_iteration(cb_mme* M, double lambda, TX* X, T* R, T* Z, T* P, T* Qp, ColS
         k_col_dots<T><<<gd, 256, 0, st>>>(P, Qp, ld, dim, part);
     }
     if (xr_block_path(M)) {
-        // alpha, x, r and z of the Jacobi and block rows with the r.z / r.r
-        // partials in one launch, then beta, p and the block rows' r
+        // alpha, x, r and z of the Jacobi and block rows with the dot
+        // partials in one launch, then the stiff part, beta, p and the block
+        // rows' r
         const int64_t nbw = xr_block_wgs(M), ntot = nbw + g2.x;
-        k_cg_xr_block<TX, T><<<dim3((unsigned)ntot, (unsigned)r), 256, (size_t)M->p * sizeof(double), st>>>(
-            X, R, P, Qp, Z, M->dinv.p, M->Binv.p, cs, part, M->p, dim, ld, nbw, part2);
-        k_cg_p3<T><<<g2, 256, 0, st>>>(P, Z, R, Qp, cs, part2, ntot, tol2, absolute ? 1 : 0, M->p, dim, ld);
+        const dim3 g3((unsigned)ntot, (unsigned)r);
+        const size_t lds = (size_t)M->p * sizeof(double);
+        if (M->stiff) {
+            k_cg_xr_block<TX, T, true><<<g3, 256, lds, st>>>(X, R, P, Qp, Z, M->dinv.p, M->Binv.p, M->stiff_v.p,
+                                                             M->stiff_y.p, cs, part, M->p, dim, ld, nbw, part2);
+            k_cg_p3<T, true><<<g2, 256, 0, st>>>(P, Z, R, Qp, M->stiff_y.p, M->stiff_gamma, cs, part2, ntot, tol2,
+                                                 absolute ? 1 : 0, M->p, dim, ld);
+        } else {
+            k_cg_xr_block<TX, T, false><<<g3, 256, lds, st>>>(X, R, P, Qp, Z, M->dinv.p, M->Binv.p, nullptr,
+                                                              nullptr, cs, part, M->p, dim, ld, nbw, part2);
+            k_cg_p3<T, false><<<g2, 256, 0, st>>>(P, Z, R, Qp, nullptr, 0.0, cs, part2, ntot, tol2,
+                                                  absolute ? 1 : 0, M->p, dim, ld);
+        }
         CB_HIP(hipGetLastError());
         return;
     }
