@@ -36,7 +36,6 @@ constexpr int TILE = 256;        // samples per Gram tile side
 constexpr int STAGE_SNPS = 256;  // SNPs per LDS stage (16 packed words)
 constexpr int LDS_PITCH = 20;    // words per LDS row (16 + 4 pad: conflict-free b128 reads)
 constexpr double TWO_M29 = 1.0 / 536870912.0;  // 2^-29
-constexpr double TWO_M52 = 1.0 / 4503599627370496.0;  // 2^-52
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
@@ -638,10 +637,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 __device__ __forceinline__ v4i decode16(uint32_t w) {
     return (v4i){(int)(w & 0x03030303u), (int)((w >> 2) & 0x03030303u),
                  (int)((w >> 4) & 0x03030303u), (int)((w >> 6) & 0x03030303u)};
-}
-__device__ __forceinline__ uint32_t word_of(const u32x4 (&v)[2], int kk) {
-    const u32x4 x = v[kk >> 2];
-    return (kk & 3) == 0 ? x.x : (kk & 3) == 1 ? x.y : (kk & 3) == 2 ? x.z : x.w;
 }
 
 // INT8-MFMA Gram over the lower triangle of 256x256 tiles.

@@ -1000,14 +1000,9 @@ __global__ __launch_bounds__(NTH) void k_col_dots2(const T* __restrict__ A, cons
     }
 }
 
-__device__ __forceinline__ double sum_chunks(const double* __restrict__ part, int c) {
-    double s = 0;
-    for (int k = 0; k < DOT_CHUNKS; ++k) s += part[(int64_t)c * DOT_CHUNKS + k];
-    return s;
-}
 // the partials of column c into LDS by DOT_CHUNKS threads at once (one round
 // trip instead of a chain of dependent loads); thread 0 then sums them in
-// chunk order, the same sum as sum_chunks
+// chunk order
 __device__ __forceinline__ void stage_chunks(const double* __restrict__ part, int c, double* __restrict__ sp,
                                              int t) {
     if (t < DOT_CHUNKS) sp[t] = part[(int64_t)c * DOT_CHUNKS + t];
@@ -1499,26 +1494,6 @@ __global__ void k_sub(const TB* __restrict__ B, const T* __restrict__ A, T* __re
     R[o] = (T)((double)B[o] - (double)A[o]);
 }
 
-// out32 = (float)(in64 * s_c)
-__global__ void k_scale_to_f32(const double* __restrict__ in, float* __restrict__ out,
-                               const double* __restrict__ scale, int64_t rows, int64_t ld) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = blockIdx.y;
-    if (i >= rows) return;
-    const int64_t o = (int64_t)c * ld + i;
-    out[o] = (float)(in[o] * scale[c]);
-}
-
-// x64 += (double)d32 * s_c
-__global__ void k_axpy_from_f32(double* __restrict__ x, const float* __restrict__ d,
-                                const double* __restrict__ s, int64_t rows, int64_t ld) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = blockIdx.y;
-    if (i >= rows) return;
-    const int64_t o = (int64_t)c * ld + i;
-    x[o] += (double)d[o] * s[c];
-}
-
 template <class TD, class TS>
 __global__ void k_convert(const TS* __restrict__ s, TD* __restrict__ d, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1603,12 +1578,6 @@ __global__ void k_ai_vectors(const int32_t* __restrict__ aid, int64_t n, int64_t
     if (r >= n) return;
     V[r] = aid[r] >= 0 ? sol[p + aid[r]] * inv_vu : 0.0;
     V[ldv + r] = e[r] * inv_ve;
-}
-
-__global__ void k_axpby(const double* __restrict__ a, double alpha, const double* __restrict__ b,
-                        double beta, double* __restrict__ out, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = alpha * a[i] + beta * b[i];
 }
 
 // The REML score's data terms from one block of MME solutions, in one pass
@@ -2734,7 +2703,6 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
     bool all_done = replace(true);
     int64_t it = 0;
     const int check_every = 4;
-    int64_t replacements = 0;
     const WsPtr<double> part2 = ws<double>(M, WS_PART2, part2_count(M, r));
     // the recursive test in the iteration only pauses a column until the
     // next replacement
@@ -2772,7 +2740,6 @@ int64_t pcg_reliable(cb_mme* M, double lambda, const double* B64, double* X, int
             }
             if (need) {
                 all_done = replace(false);
-                ++replacements;
             }
         }
     }
@@ -2794,7 +2761,7 @@ static SolveStats solve_block(cb_mme* M, double lambda, const cb_solve_options& 
                               double* X, int r, bool warm = false) {
     cb_context* ctx = M->ctx;
     hipStream_t st = ctx->stream;
-    const int64_t ld = M->ldv, dim = M->dim;
+    const int64_t ld = M->ldv;
     const size_t nv = (size_t)ld * r;
     const bool absolute = o.stop_rule == CB_STOP_ABSOLUTE;
     const bool block = o.precond == CB_PRECOND_BLOCK_JACOBI;
