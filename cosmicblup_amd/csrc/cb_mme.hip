@@ -185,7 +185,7 @@ __global__ __launch_bounds__(256) void k_csr_spmm(const int64_t* __restrict__ pt
 // [0, row0) (k_csr_spmm_chunks' work, same sums), so that those rows' gathers
 // run beside the short rows'; with lc_row1 the rows of one chunk are written
 // here, the others are finished later from the partials.
-template <class TM, class TV, int S>
+template <class TM, class TV, int S, bool HAS_Q = true>
 __global__ __launch_bounds__(256) void k_csr2_spmm(const int64_t* __restrict__ ptr1, const int32_t* __restrict__ col1,
                                                    const TM* __restrict__ val1, int s1,
                                                    const int64_t* __restrict__ ptr2, const int32_t* __restrict__ col2,
@@ -206,25 +206,33 @@ __global__ __launch_bounds__(256) void k_csr2_spmm(const int64_t* __restrict__ p
     const int64_t row = row0 + (((int64_t)blockIdx.x - nlc) * blockDim.x + threadIdx.x) / S;
     TV a1 = 0, a2 = 0;
     if (row < row1) {  // no return before the butterflies: every lane of a group takes part
-        const bool has2 = row >= p;
-        const int64_t r2 = has2 ? row - p : 0;
         const int64_t b1 = ptr1[row] + sub, e1 = ptr1[row + 1];
-        const int64_t b2 = ptr2[r2] + sub, e2 = has2 ? ptr2[r2 + 1] : 0;
-        CsrLane<TM, TV> L1, L2;
+        CsrLane<TM, TV> L1;
         L1.indices(col1, val1, b1, e1, s1, sub < s1);
-        L2.indices(col2, val2, b2, e2, S, has2);
-        L1.gather(x);
-        L2.gather(x + p);
-        a1 = L1.sum(col1, val1, x, b1, e1, s1, a1);
-        a2 = L2.sum(col2, val2, x + p, b2, e2, S, a2);
+        if (HAS_Q) {
+            const bool has2 = row >= p;
+            const int64_t r2 = has2 ? row - p : 0;
+            const int64_t b2 = ptr2[r2] + sub, e2 = has2 ? ptr2[r2 + 1] : 0;
+            CsrLane<TM, TV> L2;
+            L2.indices(col2, val2, b2, e2, S, has2);
+            L1.gather(x);
+            L2.gather(x + p);
+            a1 = L1.sum(col1, val1, x, b1, e1, s1, a1);
+            a2 = L2.sum(col2, val2, x + p, b2, e2, S, a2);
+        } else {
+            L1.gather(x);
+            a1 = L1.sum(col1, val1, x, b1, e1, s1, a1);
+        }
     }
     for (int off = s1 / 2; off > 0; off >>= 1) a1 += __shfl_xor(a1, off, S);
+    if (HAS_Q) {
 #pragma unroll
-    for (int off = S / 2; off > 0; off >>= 1) a2 += __shfl_xor(a2, off, S);
+        for (int off = S / 2; off > 0; off >>= 1) a2 += __shfl_xor(a2, off, S);
+    }
     if (row < row1 && sub == 0) {
         TV* y = Y + c * ldy + row;
         const TV w = (TV)1 * a1;
-        *y = row >= p ? w + alpha * a2 : w;
+        *y = HAS_Q && row >= p ? w + alpha * a2 : w;
     }
 }
 
@@ -2194,16 +2202,41 @@ void apply_C(cb_mme* M, double lambda, const T* X, T* Y, int r, double** lcp_out
     hipStream_t st = ctx->stream;
     const int64_t ld = M->ldv;
     const bool fused = csr_fused(M);
-    const bool defer = lcp_out != nullptr && fused && M->long_rows > 0 && M->n_multi <= MULTI_FIN_MAX;
+    // a dense Q: W'W's short rows and its long rows' chunk partials in one
+    // pass of their own (the long rows are fixed-effect rows, where Q adds
+    // nothing)
+    const bool wonly = M->qkind == 0 && M->long_rows > 0 && M->long_rows <= M->p && M->dim > M->long_rows;
+    const bool defer =
+        lcp_out != nullptr && (fused || wonly) && M->long_rows > 0 && M->n_multi <= MULTI_FIN_MAX;
     if (lcp_out) *lcp_out = nullptr;
     double* lcp = nullptr;
     // W'W part (overwrites Y); the long rows' chunk partials ride in the
     // fused pass when there is one
     if (M->long_rows > 0) {
         lcp = scratch_or<double>(M, SK_LCPART, M->lc_part, (size_t)std::max<int64_t>(1, M->n_lchunks) * r);
-        if (M->n_lchunks > 0 && !fused)
+        if (M->n_lchunks > 0 && !fused && !wonly)
             k_csr_spmm_chunks<T, T><<<dim3((unsigned)M->n_lchunks, (unsigned)r), 256, 0, st>>>(
             M->lc_beg.p, M->lc_end.p, M->n_lchunks, M->wc.p, wvals<T>(M), X, ld, lcp);
+    }
+    if (wonly) {
+        const int S = M->ww_lanes;
+        const int64_t nlc = M->n_lchunks;
+        const dim3 grid((unsigned)(nlc + std::max<int64_t>(1, ((M->dim - M->long_rows) * S + 255) / 256)),
+                        (unsigned)r);
+        timed(ctx, sizeof(T) == 8 ? "k_csr_spmm_f64" : "k_csr_spmm_f32", [&] {
+#define CB_CSRW(S_)                                                                                                  \
+    k_csr2_spmm<T, T, S_, false><<<grid, 256, 0, st>>>(M->wp.p, M->wc.p, wvals<T>(M), S_, nullptr, nullptr, nullptr,  \
+                                                       M->dim, M->long_rows, M->dim, X, ld, Y, ld, (T)0, nlc,         \
+                                                       M->lc_beg.p, M->lc_end.p, lcp, M->lc_row1.p)
+            switch (S) {
+                case 1: CB_CSRW(1); break;
+                case 2: CB_CSRW(2); break;
+                case 4: CB_CSRW(4); break;
+                case 8: CB_CSRW(8); break;
+                default: CB_CSRW(16); break;
+            }
+#undef CB_CSRW
+        });
     }
     if (fused) {
         const int S = M->q_lanes;
@@ -2223,7 +2256,7 @@ void apply_C(cb_mme* M, double lambda, const T* X, T* Y, int r, double** lcp_out
             }
 #undef CB_CSR2
         });
-    } else if (M->dim > M->long_rows) {
+    } else if (!wonly && M->dim > M->long_rows) {
         timed(ctx, sizeof(T) == 8 ? "k_csr_spmm_f64" : "k_csr_spmm_f32", [&] {
             csr_spmm<T, T>(st, M->ww_lanes, r, M->wp.p, M->wc.p, wvals<T>(M), M->long_rows, M->dim, 0, X, ld, Y, ld, 0,
                            (T)1, 0);

@@ -342,6 +342,27 @@ def test_pblup_mean_row_of_several_chunks(cb, ctx):
         assert ebv_err(x[M.p:], ref[M.p:]) < EBV_TOL
 
 
+def test_dense_q_mean_row_of_several_chunks(cb, ctx):
+    """Dense Q with 4600 records: the mean row of W'W (4611 nonzeros) has two
+    chunks of 4096, formed in the same pass as the short rows and finished
+    inside the p.q dot; FP64 and mixed solutions against a dense solve."""
+    rng = np.random.default_rng(44)
+    n = 4600
+    B = rng.standard_normal((n, 40))
+    Q = B @ B.T / 40.0 + np.eye(n)
+    y = rng.standard_normal(n) + 2.0
+    fd = design.build_fixed_design([[f"g{int(v)}"] for v in rng.integers(0, 10, n)])
+    aid = np.arange(n, dtype=np.int32)
+    M = mme.MixedModel(fd, aid, y, n)
+    M.set_qinv_dense(Q)
+    C, rhs, _ = orl.mme_dense(design.dense_x(fd), aid, y, Q, 1.7)
+    ref = np.linalg.solve(C, rhs)
+    for prec in (_lib.CB_PREC_FP64, _lib.CB_PREC_MIXED):
+        X, _, _ = M.solve(1.7, rhs, mme.solve_options(precision=prec, tol=1e-11))
+        assert ebv_err(X[M.p:], ref[M.p:]) < EBV_TOL
+        assert np.abs(X[:M.p] - ref[:M.p]).max() / np.abs(ref[:M.p]).max() < 1e-6
+
+
 def test_pblup_csr_ainv(cb, ctx):
     """PBLUP with Henderson A^-1 as CSR (SparseMatrixAdapter path)."""
     rng = np.random.default_rng(2)
