@@ -646,7 +646,9 @@ __device__ __forceinline__ v4i decode16(uint32_t w) {
 // product is the Gram regardless of how the hardware orders k.
 // MODE 0: FP64/FP32 G with fused correction including the missing-call
 // term E; MODE 2: the same without missing calls; MODE 1: raw int32 Gram.
-template <int MODE>
+// W2 (n even, so every row of G and E starts 16-byte aligned): the FP64/FP32
+// epilogue moves two adjacent entries per lane and access.
+template <int MODE, bool W2 = false>
 __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, int64_t n,
                                               int64_t n_pad, int64_t n_stages,
                                               const double* __restrict__ u, double Cc,
@@ -796,7 +798,7 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
     double* uJ = uI + TILE;
     static_assert((4 * 32 * 33 + 2 * 32 * 33 + 2 * TILE) * 8 <= LDS_WORDS * 4, "epilogue LDS");
     const bool diag = ti == tj;
-    const double Sdiv = S > 0.0 ? S : 1.0;
+    const double Sinv = 1.0 / (S > 0.0 ? S : 1.0);
     if (MODE != 1) {
         const int64_t ri = ti * TILE + tid, rj = tj * TILE + tid;
         uI[tid] = ri < n ? u[ri] : 0.0;
@@ -832,6 +834,87 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
                 }
             }
             wave_sync();
+        }
+        return;
+    }
+    if constexpr (W2) {
+        // Lane (q, p) = (lane / 16, lane % 16) holds the pair of entries
+        // 2p, 2p+1 of a 32-wide run and takes four runs (4k + q) per access:
+        // the E loads and the G64 stores move 16 bytes per lane, the G32
+        // stores 8, half the accesses of the one-entry path below. The values
+        // are formed exactly as there. With n even a pair is either wholly
+        // inside the matrix or wholly outside, and n - 2 is a valid clamp.
+        const int p = lane & 15, q = lane >> 4;
+        auto load_e2 = [&](int ab, double2 (&ev)[8], double2 (&et)[8]) {
+            const int64_t I0 = ti * TILE + wr * 128 + (ab >> 2) * 32;
+            const int64_t J0 = tj * TILE + wc * 128 + (ab & 3) * 32;
+            const int64_t jc = J0 + 2 * p < n ? J0 + 2 * p : n - 2, ic = I0 + 2 * p < n ? I0 + 2 * p : n - 2;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int64_t i = I0 + 4 * k + q < n ? I0 + 4 * k + q : n - 1;
+                ev[k] = *reinterpret_cast<const double2*>(E + i * lde + jc);
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int64_t j = J0 + 4 * k + q < n ? J0 + 4 * k + q : n - 1;
+                et[k] = *reinterpret_cast<const double2*>(E + j * lde + ic);
+            }
+        };
+        auto process2 = [&](int ab, const double2 (&ev)[8], const double2 (&et)[8]) {
+            const int a = ab >> 2, b = ab & 3;
+            block_to_slab(ab);
+            const int64_t I0 = ti * TILE + wr * 128 + a * 32;
+            const int64_t J0 = tj * TILE + wc * 128 + b * 32;
+            if (HAS_E) {  // stage E(i,j), rows 4k + q, columns 2p, 2p+1
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    slabD[(4 * k + q) * 33 + 2 * p] = ev[k].x;
+                    slabD[(4 * k + q) * 33 + 2 * p + 1] = ev[k].y;
+                }
+                wave_sync();
+            }
+            // rows i, i+1 = I0 + 2p, +1, columns j = J0 + 4k + q: form g, write the mirror
+            const int64_t i = I0 + 2 * p;
+            const double ui0 = uI[wr * 128 + a * 32 + 2 * p], ui1 = uI[wr * 128 + a * 32 + 2 * p + 1];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int col = 4 * k + q;
+                const int64_t j = J0 + col;
+                const double uj = uJ[wc * 128 + b * 32 + col];
+                const double e0 = HAS_E ? slabD[(2 * p) * 33 + col] + et[k].x : 0.0;
+                const double e1 = HAS_E ? slabD[(2 * p + 1) * 33 + col] + et[k].y : 0.0;
+                const double v0 = (((double)slabX[(2 * p) * 33 + col] - (ui0 + uj)) + Cc) + e0;
+                const double v1 = (((double)slabX[(2 * p + 1) * 33 + col] - (ui1 + uj)) + Cc) + e1;
+                const double g0 = v0 * Sinv, g1 = v1 * Sinv;
+                slabD[(2 * p) * 33 + col] = g0;
+                slabD[(2 * p + 1) * 33 + col] = g1;
+                if (!diag && i < n && j < n) {
+                    if (G64) *reinterpret_cast<double2*>(G64 + j * ld + i) = make_double2(g0, g1);
+                    if (G32) *reinterpret_cast<float2*>(G32 + j * ld + i) = make_float2((float)g0, (float)g1);
+                }
+            }
+            wave_sync();
+            // own positions: rows I0 + 4k + q, columns J0 + 2p, +1
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int row = 4 * k + q;
+                const int64_t ii = I0 + row, j = J0 + 2 * p;
+                if (ii < n && j < n) {
+                    const double g0 = slabD[row * 33 + 2 * p], g1 = slabD[row * 33 + 2 * p + 1];
+                    if (G64) *reinterpret_cast<double2*>(G64 + ii * ld + j) = make_double2(g0, g1);
+                    if (G32) *reinterpret_cast<float2*>(G32 + ii * ld + j) = make_float2((float)g0, (float)g1);
+                }
+            }
+            wave_sync();
+        };
+        double2 evA[8], etA[8], evB[8], etB[8];
+        if (HAS_E) load_e2(0, evA, etA);
+#pragma unroll 1
+        for (int ab = 0; ab < 16; ab += 2) {
+            if (HAS_E) load_e2(ab + 1, evB, etB);
+            process2(ab, evA, etA);
+            if (HAS_E && ab + 2 < 16) load_e2(ab + 2, evA, etA);
+            process2(ab + 1, evB, etB);
         }
         return;
     }
@@ -875,7 +958,7 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
             if (i < n && j < n) {
                 const double e = HAS_E ? slabD[r * 33 + col] + et[rr] : 0.0;
                 const double v = (((double)slabX[r * 33 + col] - (ui + uJ[wc * 128 + b * 32 + col])) + Cc) + e;
-                const double g = v / Sdiv;
+                const double g = v * Sinv;
                 slabD[r * 33 + col] = g;
                 if (!diag) {
                     if (G64) G64[j * ld + i] = g;
@@ -1143,10 +1226,19 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
     if (outputs & CB_GRM_OUT_F32) grm->G32.alloc((size_t)n * n); else grm->G32.release();
     if (outputs & CB_GRM_OUT_INT) grm->Gi.alloc((size_t)n * n); else grm->Gi.release();
     if (outputs & (CB_GRM_OUT_F64 | CB_GRM_OUT_F32)) {
+        // G and E are n x n with leading dimension n: for even n every row
+        // starts 16-byte aligned (device allocations are 256-byte aligned)
+        const bool w2 = n % 2 == 0;
         timed(ctx, "k_gram", [&] {
-            if (has_e)
+            if (has_e && w2)
+                k_gram<0, true><<<dim3((unsigned)ntiles), dim3(256), 0, st>>>(
+                    X2.p, n, n_pad, n_stages, u.p, Cc, total, E.p, n, grm->G64.p, grm->G32.p, nullptr, n);
+            else if (has_e)
                 k_gram<0><<<dim3((unsigned)ntiles), dim3(256), 0, st>>>(
                     X2.p, n, n_pad, n_stages, u.p, Cc, total, E.p, n, grm->G64.p, grm->G32.p, nullptr, n);
+            else if (w2)
+                k_gram<2, true><<<dim3((unsigned)ntiles), dim3(256), 0, st>>>(
+                    X2.p, n, n_pad, n_stages, u.p, Cc, total, nullptr, n, grm->G64.p, grm->G32.p, nullptr, n);
             else
                 k_gram<2><<<dim3((unsigned)ntiles), dim3(256), 0, st>>>(
                     X2.p, n, n_pad, n_stages, u.p, Cc, total, nullptr, n, grm->G64.p, grm->G32.p, nullptr, n);
