@@ -58,6 +58,11 @@ def test_toy_fixture(cb, ctx):
     (517, 1029, 0.01, 3),    # ragged, missing calls
     (1000, 3000, 0.05, 4),   # heavier missingness, several tiles
     (5, 17, 0.1, 5),         # tiny
+    # the Gram epilogue moves 2 entries per lane when n is even, else 1:
+    # ragged even n with and without missing calls (517 and 5 take the
+    # 1-entry path)
+    (770, 1500, 0.02, 6),
+    (302, 500, 0.0, 7),
 ])
 def test_random_cases(cb, ctx, n, m, miss, seed):
     x, _ = synth.simulate_dosages(n, m, maf=(0.0, 0.5), missing_rate=miss, seed=seed)
