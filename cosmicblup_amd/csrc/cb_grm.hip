@@ -40,6 +40,8 @@ constexpr double TWO_M29 = 1.0 / 536870912.0;  // 2^-29
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 // 2-bit-field mask (low bit of each field) for the samples of word w that are < n.
 __device__ __forceinline__ uint32_t field_mask(int64_t n, int64_t w) {
@@ -790,7 +792,9 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
     // one block ahead, so their latency hides under the previous block's
     // stores; u of the tile's rows and columns is staged in LDS once. The
     // slabs are wave-private, so a wave orders its own LDS traffic and
-    // never waits for the other waves.
+    // never waits for the other waves. E is read once and G written once:
+    // both go non-temporal, so that they do not evict the sample panels the
+    // other tiles of the XCD are still reading from L2.
     __syncthreads();
     double* slabD = reinterpret_cast<double*>(lds) + wave * (32 * 33);
     int32_t* slabX = reinterpret_cast<int32_t*>(reinterpret_cast<double*>(lds) + 4 * 32 * 33) + wave * (32 * 33);
@@ -845,22 +849,22 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
         // are formed exactly as there. With n even a pair is either wholly
         // inside the matrix or wholly outside, and n - 2 is a valid clamp.
         const int p = lane & 15, q = lane >> 4;
-        auto load_e2 = [&](int ab, double2 (&ev)[8], double2 (&et)[8]) {
+        auto load_e2 = [&](int ab, f64x2 (&ev)[8], f64x2 (&et)[8]) {
             const int64_t I0 = ti * TILE + wr * 128 + (ab >> 2) * 32;
             const int64_t J0 = tj * TILE + wc * 128 + (ab & 3) * 32;
             const int64_t jc = J0 + 2 * p < n ? J0 + 2 * p : n - 2, ic = I0 + 2 * p < n ? I0 + 2 * p : n - 2;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 const int64_t i = I0 + 4 * k + q < n ? I0 + 4 * k + q : n - 1;
-                ev[k] = *reinterpret_cast<const double2*>(E + i * lde + jc);
+                ev[k] = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(E + i * lde + jc));
             }
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 const int64_t j = J0 + 4 * k + q < n ? J0 + 4 * k + q : n - 1;
-                et[k] = *reinterpret_cast<const double2*>(E + j * lde + ic);
+                et[k] = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(E + j * lde + ic));
             }
         };
-        auto process2 = [&](int ab, const double2 (&ev)[8], const double2 (&et)[8]) {
+        auto process2 = [&](int ab, const f64x2 (&ev)[8], const f64x2 (&et)[8]) {
             const int a = ab >> 2, b = ab & 3;
             block_to_slab(ab);
             const int64_t I0 = ti * TILE + wr * 128 + a * 32;
@@ -889,8 +893,8 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
                 slabD[(2 * p) * 33 + col] = g0;
                 slabD[(2 * p + 1) * 33 + col] = g1;
                 if (!diag && i < n && j < n) {
-                    if (G64) *reinterpret_cast<double2*>(G64 + j * ld + i) = make_double2(g0, g1);
-                    if (G32) *reinterpret_cast<float2*>(G32 + j * ld + i) = make_float2((float)g0, (float)g1);
+                    if (G64) __builtin_nontemporal_store((f64x2){g0, g1}, reinterpret_cast<f64x2*>(G64 + j * ld + i));
+                    if (G32) __builtin_nontemporal_store((f32x2){(float)g0, (float)g1}, reinterpret_cast<f32x2*>(G32 + j * ld + i));
                 }
             }
             wave_sync();
@@ -901,13 +905,13 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
                 const int64_t ii = I0 + row, j = J0 + 2 * p;
                 if (ii < n && j < n) {
                     const double g0 = slabD[row * 33 + 2 * p], g1 = slabD[row * 33 + 2 * p + 1];
-                    if (G64) *reinterpret_cast<double2*>(G64 + ii * ld + j) = make_double2(g0, g1);
-                    if (G32) *reinterpret_cast<float2*>(G32 + ii * ld + j) = make_float2((float)g0, (float)g1);
+                    if (G64) __builtin_nontemporal_store((f64x2){g0, g1}, reinterpret_cast<f64x2*>(G64 + ii * ld + j));
+                    if (G32) __builtin_nontemporal_store((f32x2){(float)g0, (float)g1}, reinterpret_cast<f32x2*>(G32 + ii * ld + j));
                 }
             }
             wave_sync();
         };
-        double2 evA[8], etA[8], evB[8], etB[8];
+        f64x2 evA[8], etA[8], evB[8], etB[8];
         if (HAS_E) load_e2(0, evA, etA);
 #pragma unroll 1
         for (int ab = 0; ab < 16; ab += 2) {
@@ -930,12 +934,12 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
 #pragma unroll
         for (int rr = 0; rr < 16; ++rr) {
             const int64_t i = I0 + h * 16 + rr < n ? I0 + h * 16 + rr : n - 1;
-            ev[rr] = E[i * lde + jr];
+            ev[rr] = __builtin_nontemporal_load(E + i * lde + jr);
         }
 #pragma unroll
         for (int rr = 0; rr < 16; ++rr) {
             const int64_t j = J0 + h * 16 + rr < n ? J0 + h * 16 + rr : n - 1;
-            et[rr] = E[j * lde + ir];
+            et[rr] = __builtin_nontemporal_load(E + j * lde + ir);
         }
     };
     auto process = [&](int ab, const double (&ev)[16], const double (&et)[16]) {
@@ -961,8 +965,8 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
                 const double g = v * Sinv;
                 slabD[r * 33 + col] = g;
                 if (!diag) {
-                    if (G64) G64[j * ld + i] = g;
-                    if (G32) G32[j * ld + i] = (float)g;
+                    if (G64) __builtin_nontemporal_store(g, G64 + j * ld + i);
+                    if (G32) __builtin_nontemporal_store((float)g, G32 + j * ld + i);
                 }
             }
         }
@@ -974,8 +978,8 @@ __global__ __launch_bounds__(256) void k_gram(const uint32_t* __restrict__ X2, i
             const int64_t ii = I0 + row, j = J0 + r;
             if (ii < n && j < n) {
                 const double g = slabD[row * 33 + r];
-                if (G64) G64[ii * ld + j] = g;
-                if (G32) G32[ii * ld + j] = (float)g;
+                if (G64) __builtin_nontemporal_store(g, G64 + ii * ld + j);
+                if (G32) __builtin_nontemporal_store((float)g, G32 + ii * ld + j);
             }
         }
         wave_sync();
