@@ -803,7 +803,7 @@ def main() -> None:
     ctx.set_profiling(False)
     ops = float(m) * n * (n + 1)
     kern = {}
-    for k in ("k_snp_stats", "k_pack", "k_row_sums", "k_miss_snp_lists", "k_miss_sample_lists", "k_sample_sq",
+    for k in ("k_snp_stats", "k_pack", "k_row_sums", "k_miss_count", "k_miss_scan", "k_miss_emit", "k_sample_sq",
               "k_encode_iblock", "k_group_entries", "k_miss_cross", "k_gram"):
         ms, cnt = ctx.kernel_time(k)
         if cnt:

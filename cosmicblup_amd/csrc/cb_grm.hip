@@ -18,7 +18,8 @@
 //   k_snp_stats   per-SNP counts + filter (grm_builder.cpp:175-263), HBM-bound
 //   k_pack        SNP-major BED -> sample-major 2-bit dosage tiles, HBM-bound
 //   k_row_sums    exact integer sum_j K_j X_ij per sample
-//   k_miss_*      missing-call lists; E with F folded in (dot4 + rare-path LDS)
+//   k_miss_count, k_miss_scan, k_miss_emit  per-sample missing-call lists,
+//                 no atomics; k_miss_mfma  E with F folded in (INT8 MFMA)
 //   k_gram        INT8 MFMA Gram over lower-triangle 256x256 tiles with the
 //                 FP64 centring/scaling fused into the epilogue, MFMA-bound
 #include <algorithm>
@@ -191,61 +192,101 @@ __global__ __launch_bounds__(256) void k_row_sums(const uint32_t* __restrict__ X
 }
 
 // ---------------------------------------------------------------------------
-// Missing-call lists. Per SNP (valid only): the sorted list S_j of samples
-// missing at j (wave-ordered compaction) and per-sample counts.
-__global__ __launch_bounds__(256) void k_miss_snp_lists(
-    const uint8_t* __restrict__ bed, int64_t pitch, int64_t n, int64_t m,
-    const uint8_t* __restrict__ valid, const int64_t* __restrict__ snp_off,
-    int32_t* __restrict__ snp_list, int32_t* __restrict__ sample_cnt) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t j = (int64_t)blockIdx.x * 4 + wave;
-    if (j >= m || !valid[j]) return;
-    const uint32_t* row = reinterpret_cast<const uint32_t*>(bed + j * pitch);
-    const int64_t nwords = (n + 15) / 16;
-    int64_t base = snp_off[j];
-    for (int64_t w0 = 0; w0 < nwords; w0 += 64) {
-        const int64_t w = w0 + lane;
-        uint32_t mb = 0;
-        if (w < nwords) {
-            const uint32_t x = row[w];
-            mb = ~(x >> 1) & x & field_mask(n, w);  // code 01
-        }
-        const uint32_t c = __popc(mb);
-        uint32_t incl = c;
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(incl, off);
-            if (lane >= off) incl += y;
-        }
-        const uint32_t total = __shfl(incl, 63);
-        int64_t pos = base + incl - c;
-        while (mb) {
-            const int f = __ffs(mb) - 1;  // even bit position
-            mb &= mb - 1;
-            const int64_t i = w * 16 + (f >> 1);
-            snp_list[pos++] = (int32_t)i;
-            atomicAdd(&sample_cnt[i], 1);
-        }
-        base += total;
-    }
+// Missing-call lists (per sample: the SNP j and K_j of every missing call at
+// a kept SNP), built without atomics. The SNPs are cut into chunks of
+// MC_ROWS; thread (word column w, chunk c) reads word w of each BED row of
+// the chunk, the 2-bit codes of samples 16 w .. 16 w + 15, and keeps its 16
+// per-sample counters in LDS (private to the thread). Pass 1 writes the
+// counts of each (chunk, sample); a scan over the chunks turns them into
+// offsets; pass 2 repeats the reads and writes each entry at its sample's
+// offset. A sample's list comes out in SNP order. Pads (to a multiple of 8)
+// keep j = 0 and K = 0.
+constexpr int MC_ROWS = 512;
+
+// the missing-call (code 01) bit of each 2-bit field of word w of SNP j,
+// zero for a dropped SNP and for samples >= n
+__device__ __forceinline__ uint32_t missing_bits(const uint8_t* __restrict__ bed, int64_t pitch,
+                                                 const uint8_t* __restrict__ valid, int64_t j, int64_t w,
+                                                 uint32_t fmask) {
+    const uint32_t x = valid[j] ? reinterpret_cast<const uint32_t*>(bed + j * pitch)[w] : 0u;
+    return ~(x >> 1) & x & fmask;
 }
 
-// Per-sample lists (entry = SNP index and K_j), filled from the SNP lists.
-// Entry order inside a sample list is arbitrary: every consumer sums exact
-// integers. Pads (to a multiple of 8) keep j = 0 and K = 0.
-__global__ __launch_bounds__(256) void k_miss_sample_lists(
-    const int64_t* __restrict__ snp_off, const int32_t* __restrict__ snp_list, int64_t m,
-    const uint8_t* __restrict__ valid, const uint32_t* __restrict__ kfix,
-    const int64_t* __restrict__ smp_off, int32_t* __restrict__ cursor,
-    int32_t* __restrict__ lj, uint32_t* __restrict__ lk) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t j = (int64_t)blockIdx.x * 4 + wave;
-    if (j >= m || !valid[j]) return;
-    const uint32_t K = kfix[j];
-    for (int64_t p = snp_off[j] + lane; p < snp_off[j + 1]; p += 64) {
-        const int32_t i = snp_list[p];
-        const int64_t pos = smp_off[i] + atomicAdd(&cursor[i], 1);
-        lj[pos] = (int32_t)j;
-        lk[pos] = K;
+// Pass 1: cnt[c n + i] = missing calls of sample i at the kept SNPs of chunk c.
+__global__ __launch_bounds__(256) void k_miss_count(const uint8_t* __restrict__ bed, int64_t pitch, int64_t n,
+                                                    int64_t m, const uint8_t* __restrict__ valid,
+                                                    int32_t* __restrict__ cnt) {
+    __shared__ int32_t lc[256 * 17];  // 16 counters per thread (pitch 17)
+    const int tid = threadIdx.x;
+    const int64_t nw = (n + 15) / 16, w = (int64_t)blockIdx.x * 256 + tid, c = blockIdx.y;
+    if (w >= nw) return;  // the counters are private: no barrier follows
+    int32_t* my = lc + tid * 17;
+#pragma unroll
+    for (int f = 0; f < 16; ++f) my[f] = 0;
+    const uint32_t fm = field_mask(n, w);
+    const int64_t j0 = c * MC_ROWS, j1 = min(m, j0 + MC_ROWS);
+    for (int64_t jb = j0; jb < j1; jb += 8) {
+        uint32_t mb[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) mb[t] = jb + t < j1 ? missing_bits(bed, pitch, valid, jb + t, w, fm) : 0u;
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+            for (uint32_t b = mb[t]; b; b &= b - 1) ++my[(__ffs(b) - 1) >> 1];
+    }
+#pragma unroll
+    for (int f = 0; f < 16; ++f)
+        if (w * 16 + f < n) cnt[c * n + w * 16 + f] = my[f];
+}
+
+// Per sample (one thread): the counts of the nc chunks become exclusive
+// offsets, in place, and the total goes to smp_cnt.
+__global__ __launch_bounds__(256) void k_miss_scan(int32_t* __restrict__ cnt, int64_t n, int64_t nc,
+                                                   int32_t* __restrict__ smp_cnt) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    int32_t run = 0;
+    for (int64_t c = 0; c < nc; ++c) {
+        const int32_t t = cnt[c * n + i];
+        cnt[c * n + i] = run;
+        run += t;
+    }
+    smp_cnt[i] = run;
+}
+
+// Pass 2: the entries (j, K_j) of chunk c at smp_off[i] + cnt[c n + i] + ...
+__global__ __launch_bounds__(256) void k_miss_emit(const uint8_t* __restrict__ bed, int64_t pitch, int64_t n,
+                                                   int64_t m, const uint8_t* __restrict__ valid,
+                                                   const uint32_t* __restrict__ kfix,
+                                                   const int64_t* __restrict__ smp_off,
+                                                   const int32_t* __restrict__ cnt, int32_t* __restrict__ lj,
+                                                   uint32_t* __restrict__ lk) {
+    __shared__ int64_t lp[256 * 17];  // 16 write positions per thread (pitch 17)
+    const int tid = threadIdx.x;
+    const int64_t nw = (n + 15) / 16, w = (int64_t)blockIdx.x * 256 + tid, c = blockIdx.y;
+    if (w >= nw) return;  // the positions are private: no barrier follows
+    int64_t* my = lp + tid * 17;
+#pragma unroll
+    for (int f = 0; f < 16; ++f) {
+        const int64_t i = w * 16 + f;
+        my[f] = i < n ? smp_off[i] + cnt[c * n + i] : 0;
+    }
+    const uint32_t fm = field_mask(n, w);
+    const int64_t j0 = c * MC_ROWS, j1 = min(m, j0 + MC_ROWS);
+    for (int64_t jb = j0; jb < j1; jb += 8) {
+        uint32_t mb[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) mb[t] = jb + t < j1 ? missing_bits(bed, pitch, valid, jb + t, w, fm) : 0u;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            if (!mb[t]) continue;
+            const int32_t j = (int32_t)(jb + t);
+            const uint32_t K = kfix[j];
+            for (uint32_t b = mb[t]; b; b &= b - 1) {
+                const int64_t pos = my[(__ffs(b) - 1) >> 1]++;
+                lj[pos] = j;
+                lk[pos] = K;
+            }
+        }
     }
 }
 
@@ -1017,8 +1058,8 @@ struct cb_grm {
     DevBuf<uint32_t> X2;              // sample-major packed dosages
     DevBuf<double> u;                 // per-sample centring terms
     DevBuf<double> E;                 // missing-call correction, n x n
-    DevBuf<int64_t> snp_off, smp_off;
-    DevBuf<int32_t> snp_list, smp_cnt, lj, cursor;
+    DevBuf<int64_t> smp_off;
+    DevBuf<int32_t> mcnt, smp_cnt, lj;
     DevBuf<uint32_t> lk;
     DevBuf<unsigned long long> sq;
     DevBuf<double> sqd;
@@ -1110,19 +1151,18 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
     const bool has_e = nmiss_total > 0 && (outputs & (CB_GRM_OUT_F64 | CB_GRM_OUT_F32));
     DevBuf<double>& E = grm->E;
     if (has_e) {
-        std::vector<int64_t> snp_off(m + 1, 0);
-        for (int64_t j = 0; j < m; ++j) snp_off[j + 1] = snp_off[j] + (hvalid[j] ? n - hnobs[j] : 0);
-        DevBuf<int64_t>& d_snp_off = grm->snp_off;
-        grow(d_snp_off, m + 1);
-        copy_host(st, d_snp_off.p, snp_off.data(), (m + 1) * 8, hipMemcpyHostToDevice);
-        DevBuf<int32_t>& snp_list = grm->snp_list;
+        const int64_t nw = ceil_div(n, 16), nc = ceil_div(m, MC_ROWS);
+        CB_REQUIRE(nc < 65536, "GRM: too many SNPs for the missing-call list kernels' grid");
+        DevBuf<int32_t>& mcnt = grm->mcnt;
         DevBuf<int32_t>& smp_cnt = grm->smp_cnt;
-        grow(snp_list, std::max<int64_t>(1, nmiss_total));
+        grow(mcnt, nc * n);
         grow(smp_cnt, n);
-        CB_HIP(hipMemsetAsync(smp_cnt.p, 0, n * 4, st));
-        timed(ctx, "k_miss_snp_lists", [&] {
-            k_miss_snp_lists<<<dim3((unsigned)ceil_div(m, 4)), dim3(256), 0, st>>>(
-                g->bed.p, g->pitch, n, m, grm->valid.p, d_snp_off.p, snp_list.p, smp_cnt.p);
+        const dim3 lgrid((unsigned)ceil_div(nw, 256), (unsigned)nc);
+        timed(ctx, "k_miss_count", [&] {
+            k_miss_count<<<lgrid, 256, 0, st>>>(g->bed.p, g->pitch, n, m, grm->valid.p, mcnt.p);
+        });
+        timed(ctx, "k_miss_scan", [&] {
+            k_miss_scan<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(mcnt.p, n, nc, smp_cnt.p);
         });
         std::vector<int32_t> hcnt(n);
         copy_host(st, hcnt.data(), smp_cnt.p, n * 4, hipMemcpyDeviceToHost);
@@ -1134,20 +1174,16 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
         copy_host(st, d_smp_off.p, smp_off.data(), (n + 1) * 8, hipMemcpyHostToDevice);
         const int64_t nlist = std::max<int64_t>(8, smp_off[n]);
         DevBuf<int32_t>& lj = grm->lj;
-        DevBuf<int32_t>& cursor = grm->cursor;
         DevBuf<uint32_t>& lk = grm->lk;
         DevBuf<unsigned long long>& sq = grm->sq;  // per sample: K^2 low words, high words, then the sums of K
         grow(lj, nlist);
         grow(lk, nlist);
-        grow(cursor, n);
         grow(sq, 3 * n);
         CB_HIP(hipMemsetAsync(lj.p, 0, nlist * 4, st));
         CB_HIP(hipMemsetAsync(lk.p, 0, nlist * 4, st));
-        CB_HIP(hipMemsetAsync(cursor.p, 0, n * 4, st));
-        timed(ctx, "k_miss_sample_lists", [&] {
-            k_miss_sample_lists<<<dim3((unsigned)ceil_div(m, 4)), dim3(256), 0, st>>>(
-                d_snp_off.p, snp_list.p, m, grm->valid.p, grm->kfix.p, d_smp_off.p, cursor.p, lj.p,
-                lk.p);
+        timed(ctx, "k_miss_emit", [&] {
+            k_miss_emit<<<lgrid, 256, 0, st>>>(g->bed.p, g->pitch, n, m, grm->valid.p, grm->kfix.p, d_smp_off.p,
+                                               mcnt.p, lj.p, lk.p);
         });
         timed(ctx, "k_sample_sq", [&] {
             k_sample_sq<<<dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st>>>(d_smp_off.p, lk.p, n, sq.p, sq.p + n,
