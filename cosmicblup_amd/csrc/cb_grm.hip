@@ -201,7 +201,7 @@ __global__ __launch_bounds__(256) void k_row_sums(const uint32_t* __restrict__ X
 // offsets; pass 2 repeats the reads and writes each entry at its sample's
 // offset. A sample's list comes out in SNP order. Pads (to a multiple of 8)
 // keep j = 0 and K = 0.
-constexpr int MC_ROWS = 512;
+constexpr int MC_ROWS = 128;
 
 // the missing-call (code 01) bit of each 2-bit field of word w of SNP j,
 // zero for a dropped SNP and for samples >= n
