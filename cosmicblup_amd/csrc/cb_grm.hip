@@ -258,8 +258,8 @@ __global__ __launch_bounds__(256) void k_miss_emit(const uint8_t* __restrict__ b
                                                    int64_t m, const uint8_t* __restrict__ valid,
                                                    const uint32_t* __restrict__ kfix,
                                                    const int64_t* __restrict__ smp_off,
-                                                   const int32_t* __restrict__ cnt, int32_t* __restrict__ lj,
-                                                   uint32_t* __restrict__ lk) {
+                                                   const int32_t* __restrict__ cnt,
+                                                   unsigned long long* __restrict__ ljk) {
     __shared__ int64_t lp[256 * 17];  // 16 write positions per thread (pitch 17)
     const int tid = threadIdx.x;
     const int64_t nw = (n + 15) / 16, w = (int64_t)blockIdx.x * 256 + tid, c = blockIdx.y;
@@ -283,8 +283,7 @@ __global__ __launch_bounds__(256) void k_miss_emit(const uint8_t* __restrict__ b
             const uint32_t K = kfix[j];
             for (uint32_t b = mb[t]; b; b &= b - 1) {
                 const int64_t pos = my[(__ffs(b) - 1) >> 1]++;
-                lj[pos] = j;
-                lk[pos] = K;
+                ljk[pos] = ((unsigned long long)K << 32) | (uint32_t)j;  // one 8-byte write per entry
             }
         }
     }
@@ -295,7 +294,7 @@ __global__ __launch_bounds__(256) void k_miss_emit(const uint8_t* __restrict__ b
 // low and high words of the K^2 terms are summed apart (exact for lists
 // below 2^31 entries).
 __global__ __launch_bounds__(256) void k_sample_sq(const int64_t* __restrict__ smp_off,
-                                                   const uint32_t* __restrict__ lk, int64_t n,
+                                                   const unsigned long long* __restrict__ ljk, int64_t n,
                                                    unsigned long long* __restrict__ sq_lo,
                                                    unsigned long long* __restrict__ sq_hi,
                                                    unsigned long long* __restrict__ ksum) {
@@ -304,7 +303,7 @@ __global__ __launch_bounds__(256) void k_sample_sq(const int64_t* __restrict__ s
     if (i >= n) return;
     unsigned long long lo = 0, hi = 0, ks = 0;
     for (int64_t p = smp_off[i] + lane; p < smp_off[i + 1]; p += 64) {
-        const unsigned long long K = lk[p];
+        const unsigned long long K = ljk[p] >> 32;
         const unsigned long long k2 = (K * K) >> 6;
         lo += k2 & 0xffffffffull;
         hi += k2 >> 32;
@@ -400,8 +399,7 @@ __global__ void k_encode_iblock(const uint8_t* __restrict__ bed, int64_t pitch, 
 // K-step, gK = their K. One block per group.
 __global__ __launch_bounds__(256) void k_group_entries(const int64_t* __restrict__ smp_off,
                                                        const int32_t* __restrict__ smp_cnt,
-                                                       const int32_t* __restrict__ lj,
-                                                       const uint32_t* __restrict__ lk, int64_t n,
+                                                       const unsigned long long* __restrict__ ljk, int64_t n,
                                                        const int64_t* __restrict__ gstep, int32_t* __restrict__ gj,
                                                        uint32_t* __restrict__ gK) {
     const int64_t g = blockIdx.x;
@@ -413,8 +411,9 @@ __global__ __launch_bounds__(256) void k_group_entries(const int64_t* __restrict
         int32_t j = 0;
         uint32_t K = 0;
         if (k < n && r < smp_cnt[k]) {
-            j = lj[smp_off[k] + r];
-            K = lk[smp_off[k] + r];
+            const unsigned long long v = ljk[smp_off[k] + r];
+            j = (int32_t)(uint32_t)v;
+            K = (uint32_t)(v >> 32);
         }
         gj[base + e] = j;
         gK[base + e] = K;
@@ -1059,8 +1058,8 @@ struct cb_grm {
     DevBuf<double> u;                 // per-sample centring terms
     DevBuf<double> E;                 // missing-call correction, n x n
     DevBuf<int64_t> smp_off;
-    DevBuf<int32_t> mcnt, smp_cnt, lj;
-    DevBuf<uint32_t> lk;
+    DevBuf<int32_t> mcnt, smp_cnt;
+    DevBuf<unsigned long long> ljk;  // list entries: K_j << 32 | j
     DevBuf<unsigned long long> sq;
     DevBuf<double> sqd;
     // the missing-call kernel's inputs: rows in I-block-major order, the
@@ -1173,20 +1172,17 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
         grow(d_smp_off, n + 1);
         copy_host(st, d_smp_off.p, smp_off.data(), (n + 1) * 8, hipMemcpyHostToDevice);
         const int64_t nlist = std::max<int64_t>(8, smp_off[n]);
-        DevBuf<int32_t>& lj = grm->lj;
-        DevBuf<uint32_t>& lk = grm->lk;
+        DevBuf<unsigned long long>& ljk = grm->ljk;
         DevBuf<unsigned long long>& sq = grm->sq;  // per sample: K^2 low words, high words, then the sums of K
-        grow(lj, nlist);
-        grow(lk, nlist);
+        grow(ljk, nlist);
         grow(sq, 3 * n);
-        CB_HIP(hipMemsetAsync(lj.p, 0, nlist * 4, st));
-        CB_HIP(hipMemsetAsync(lk.p, 0, nlist * 4, st));
+        CB_HIP(hipMemsetAsync(ljk.p, 0, nlist * 8, st));
         timed(ctx, "k_miss_emit", [&] {
             k_miss_emit<<<lgrid, 256, 0, st>>>(g->bed.p, g->pitch, n, m, grm->valid.p, grm->kfix.p, d_smp_off.p,
-                                               mcnt.p, lj.p, lk.p);
+                                               mcnt.p, ljk.p);
         });
         timed(ctx, "k_sample_sq", [&] {
-            k_sample_sq<<<dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st>>>(d_smp_off.p, lk.p, n, sq.p, sq.p + n,
+            k_sample_sq<<<dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st>>>(d_smp_off.p, ljk.p, n, sq.p, sq.p + n,
                                                                             sq.p + 2 * n);
         });
         // u_i += sum_{j miss i} mu_j^2
@@ -1239,7 +1235,7 @@ static void compute_grm(cb_grm* grm, const cb_genotypes* g, int outputs) {
         grow(gj, std::max<int64_t>(1, tsteps * MB_STEP));
         grow(gK, std::max<int64_t>(1, tsteps * MB_STEP));
         timed(ctx, "k_group_entries", [&] {
-            k_group_entries<<<(unsigned)ngrp, 256, 0, st>>>(d_smp_off.p, smp_cnt.p, lj.p, lk.p, n, dgstep.p, gj.p,
+            k_group_entries<<<(unsigned)ngrp, 256, 0, st>>>(d_smp_off.p, smp_cnt.p, ljk.p, n, dgstep.p, gj.p,
                                                            gK.p);
         });
         // group ranges of about equal K-steps, P of them per I-block
