@@ -34,10 +34,10 @@ K_DENSE_F32_TRAFFIC_BYTES = 1.824e9
 # HBM bytes per launch at n = m = 50,000 with 1% missing, from two rocprofv3
 # --pmc runs of this script's GRM leg (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the
 # factors a 1 GiB device copy gives):
-#   k_gram        87.7 GB read + 30.0 GB written
-#   k_miss_cross  76.5 GB read + 20.3 GB written
-K_GRAM_TRAFFIC_BYTES = 1.177e11
-K_MISS_TRAFFIC_BYTES = 9.68e10
+#   k_gram        86.7 GB read + 30.0 GB written
+#   k_miss_cross  76.7 GB read + 20.3 GB written
+K_GRAM_TRAFFIC_BYTES = 1.167e11
+K_MISS_TRAFFIC_BYTES = 9.70e10
 # headline metric: the GRM part is `value` (TOPS on the n = m = 50,000 GRM);
 # the REML part (GPU mixed vs GPU FP64 plain iterations/s) is under "reml"
 METRIC = "REML iters/sec + GRM TOPS, GPU mixed vs GPU FP64 at \u22641e-6 rel EBV error"
